@@ -1,0 +1,209 @@
+"""MI355X-native Super-Grid (HDGrid) filtered-ANN search: Python host-side mirror of the C ABI.
+
+The compute path is ``lib/libhdgrid.so`` (HIP kernels for gfx950 + C++ host code).  This module only
+binds it with ctypes and mirrors the reference's operator interface for the path
+(``index_base_t::build`` / ``index_base_t::query``, src/core/interfaces.cuh:27-46): device tensors in,
+caller-allocated device tensors out.  Importing it fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libhdgrid.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"hdgrid_amd: {LIB_PATH} is missing -- build it first (python -c 'import __graft_entry__ as g; g.build()')"
+    )
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+_u64 = ctypes.c_uint64
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+_cp = ctypes.c_char_p
+
+HDG_OK = 0
+STATUS_NAMES = {0: "HDG_OK", 1: "HDG_EINVAL", 2: "HDG_ECONFIG", 3: "HDG_EIO", 4: "HDG_EHIP",
+                5: "HDG_ENOMEM", 6: "HDG_ESTATE", 7: "HDG_EUNSUPPORTED"}
+
+
+def _sig(name, res, args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_lib_last_error = _sig("hdg_last_error", _cp, [])
+_create = _sig("hdg_create", ctypes.c_int, [_cp, _cp, _cp, ctypes.c_int, ctypes.POINTER(_vp)])
+_create_text = _sig("hdg_create_from_text", ctypes.c_int, [_cp, _cp, _cp, ctypes.c_int, ctypes.POINTER(_vp)])
+_build = _sig("hdg_build", ctypes.c_int, [_vp, _vp, _u64, _u64, _vp, _u64, _vp])
+_query = _sig("hdg_query", ctypes.c_int, [_vp, _vp, _u64, _u64, _vp, _u64, _vp, _vp, _vp])
+_destroy = _sig("hdg_destroy", ctypes.c_int, [_vp])
+_set_param = _sig("hdg_set_param", ctypes.c_int, [_vp, _cp, _cp])
+_get_param = _sig("hdg_get_param", ctypes.c_int, [_vp, _cp, _u64p])
+_save = _sig("hdg_save", ctypes.c_int, [_vp, _cp])
+_load = _sig("hdg_load", ctypes.c_int, [_vp, _cp, _vp])
+_stats = _sig("hdg_last_query_stats", ctypes.c_int, [_vp, _u64, _u64p, _u64p, _u64p, _u64p, _u64p, _u64p])
+_chosen = _sig("hdg_last_chosen_subs", ctypes.c_int, [_vp, _u64, ctypes.POINTER(ctypes.c_uint32)])
+_stage_ms = _sig("hdg_last_stage_ms", ctypes.c_int, [_vp, _cp, _u64, ctypes.POINTER(ctypes.c_float), _u64p])
+_split = _sig("hdg_split_task", ctypes.c_int, [ctypes.POINTER(ctypes.c_double), _u64, _u64, _u64, _u64, _u64, _u64,
+                                              _u64p, _u64p])
+_find_max_factor = _sig("hdg_find_max_factor", ctypes.c_int, [_u64, _u64])
+_validate = _sig("hdg_validate_config_text", ctypes.c_int, [_cp, _cp, _cp, _u64])
+
+
+class HDGridError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def _check(st: int):
+    if st != HDG_OK:
+        raise HDGridError(st, (_lib_last_error() or b"").decode())
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def find_max_factor(a: int, b: int) -> int:
+    """findMaxFactor (math_utils.h:40-60)."""
+    return int(_find_max_factor(a, b))
+
+
+def split_task(coeff, n_data, n_queries, topk, mem_bound, available, data_width=1):
+    """Memory-bound batch predictor (search-engine-auto-batch.cu:430-489) -> (query_batch, data_batch)."""
+    arr = (ctypes.c_double * 4)(*[float(c) for c in coeff])
+    qb, db = _u64(), _u64()
+    _check(_split(arr, n_data, n_queries, topk, mem_bound, available, data_width, ctypes.byref(qb), ctypes.byref(db)))
+    return qb.value, db.value
+
+
+def validate_config_text(conf_text: str, filter_text: str) -> str:
+    """Parse HDGrid.conf + filter.conf text without touching a device; returns the parsed values as JSON."""
+    buf = ctypes.create_string_buffer(8192)
+    _check(_validate(conf_text.encode(), filter_text.encode(), buf, len(buf)))
+    return buf.value.decode()
+
+
+def _ptr(t) -> int:
+    return t.data_ptr()
+
+
+def _stream_ptr(stream) -> Optional[int]:
+    if stream is None:
+        return None
+    return int(stream.cuda_stream) if hasattr(stream, "cuda_stream") else int(stream)
+
+
+@dataclass
+class QueryStats:
+    cand: "object"
+    real: "object"
+    refined: "object"
+    scored: "object"
+    adc: "object"
+    kprime: "object"
+
+
+class SuperGridIndex:
+    """The ``super-grid`` index (super_grid::index, Super-Grid-IVF_PQ.cuh:82-110) on one HIP device.
+
+    ``build(dataset, labels)`` and ``query(queries, query_labels)`` take torch tensors resident on the
+    index's device (float32, row-major), like the reference's device_matrix_views.
+    """
+
+    def __init__(self, conf_path: Optional[str] = None, filter_conf_path: Optional[str] = None, *,
+                 conf_text: Optional[str] = None, filter_text: Optional[str] = None, device: int = 0,
+                 algorithm: str = "super-grid"):
+        h = _vp()
+        if conf_text is not None:
+            _check(_create_text(algorithm.encode(), conf_text.encode(), (filter_text or "").encode(), device,
+                                ctypes.byref(h)))
+        else:
+            _check(_create(algorithm.encode(), conf_path.encode(),
+                           filter_conf_path.encode() if filter_conf_path else None, device, ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- configuration -----------------------------------------------------------------------
+    def set_param(self, key: str, value) -> None:
+        _check(_set_param(self._h, key.encode(), str(value).encode()))
+
+    def get_param(self, key: str) -> int:
+        v = _u64()
+        _check(_get_param(self._h, key.encode(), ctypes.byref(v)))
+        return v.value
+
+    # -- operator API ------------------------------------------------------------------------
+    def build(self, dataset, labels, stream=None) -> None:
+        import torch
+        assert dataset.is_cuda and labels.is_cuda, "dataset and labels must be device tensors"
+        assert dataset.dtype == torch.float32 and labels.dtype == torch.float32
+        dataset = dataset.contiguous()
+        labels = labels.contiguous()
+        _check(_build(self._h, _ptr(dataset), dataset.shape[0], dataset.shape[1], _ptr(labels),
+                      labels.shape[1] if labels.dim() == 2 else 1, _stream_ptr(stream)))
+
+    def query(self, queries, query_labels, out_dist=None, out_ids=None, stream=None):
+        import torch
+        assert queries.is_cuda and query_labels.is_cuda, "queries and labels must be device tensors"
+        queries = queries.contiguous()
+        query_labels = query_labels.contiguous()
+        nq = queries.shape[0]
+        k = self.get_param("TOPK")
+        if out_dist is None:
+            out_dist = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
+        if out_ids is None:
+            out_ids = torch.empty((nq, k), dtype=torch.int64, device=queries.device)
+        qcols = query_labels.shape[1] if query_labels.dim() == 2 else 1
+        _check(_query(self._h, _ptr(queries), nq, queries.shape[1], _ptr(query_labels), qcols, _ptr(out_dist),
+                      _ptr(out_ids), _stream_ptr(stream)))
+        return out_dist, out_ids
+
+    # -- state / counters ----------------------------------------------------------------------
+    def save(self, directory: str) -> None:
+        _check(_save(self._h, directory.encode()))
+
+    def load(self, directory: str, stream=None) -> None:
+        _check(_load(self._h, directory.encode(), _stream_ptr(stream)))
+
+    def last_stats(self, nq: int) -> QueryStats:
+        import numpy as np
+        arrs = [np.zeros(nq, dtype=np.uint64) for _ in range(6)]
+        ptrs = [a.ctypes.data_as(_u64p) for a in arrs]
+        _check(_stats(self._h, nq, *ptrs))
+        return QueryStats(*arrs)
+
+    def last_chosen_subs(self, nq: int):
+        import numpy as np
+        sl = self.get_param("SUB_LISTS")
+        out = np.zeros((nq, sl), dtype=np.uint32)
+        _check(_chosen(self._h, nq, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return out
+
+    def last_stage_ms(self):
+        names = ctypes.create_string_buffer(1024)
+        ms = (ctypes.c_float * 32)()
+        n = _u64()
+        _check(_stage_ms(self._h, names, len(names), ms, ctypes.byref(n)))
+        keys = names.value.decode().split(",") if n.value else []
+        return {k: float(ms[i]) for i, k in enumerate(keys)}

@@ -1,0 +1,282 @@
+// C ABI (include/hdgrid.h) over hdg::Index.  Every entry point converts exceptions into status codes
+// and a thread-local message; nothing here exits the process.
+#include "../../include/hdgrid.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "hdg_index.hpp"
+
+struct hdg_index {
+  hdg::Index* impl;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+hdg_status guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return HDG_OK;
+  } catch (const hdg::Error& e) {
+    g_last_error = e.what();
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return HDG_ENOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return HDG_EINVAL;
+  }
+}
+
+// index-factory.cuh:32,84 accept only "super-grid"; the README spells it "Super-Grid" -- accept both.
+void check_algorithm(const char* algorithm) {
+  if (!algorithm) hdg::fail(HDG_ECONFIG, "null algorithm name");
+  std::string a(algorithm);
+  std::transform(a.begin(), a.end(), a.begin(), [](unsigned char c) { return (char)std::tolower(c); });
+  if (a != "super-grid") hdg::fail(HDG_ECONFIG, "Unknown algorithm: " + std::string(algorithm));
+}
+
+void check_device(int device) {
+  int n = 0;
+  HDG_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) hdg::fail(HDG_EINVAL, "device ordinal out of range");
+}
+
+// split_task restated (search-engine-auto-batch.cu:430-489).  The reference's bisection never ends
+// when no query batch of >= 125 fits; this one reports that instead.
+void split_task(const double* coeff, uint64_t n_data, uint64_t n_queries, uint64_t topk, uint64_t mem_bound,
+                uint64_t available, uint64_t data_width, uint64_t& qb, uint64_t& db) {
+  const uint64_t upper = std::min(available / 5, mem_bound);
+  uint64_t r_d = n_data, l_d = topk, l_q = 1, r_q;
+  if (n_data < topk) hdg::fail(HDG_EINVAL, "n_data < topk");
+  while (true) {
+    const uint64_t mid_d = (r_d + l_d + 1) >> 1;
+    r_q = n_queries;
+    l_q = 1;
+    while (l_q < r_q) {
+      const uint64_t mid_q = (l_q + r_q + 1) >> 1;
+      const double v = coeff[0] * (double)mid_d + coeff[1] * (double)mid_q +
+                       coeff[2] * (double)mid_d * (double)mid_q + coeff[3] + 1;
+      const uint64_t value = v <= 0 ? 0 : (v >= 1.8446744073709552e19 ? ~0ull : (uint64_t)v);
+      if (value > upper) r_q = mid_q - 1; else l_q = mid_q;
+    }
+    if (l_q < 125) {
+      if (mid_d == 0 || mid_d - 1 < l_d) hdg::fail(HDG_EINVAL, "no data batch admits a query batch of 125");
+      r_d = mid_d - 1;
+    } else {
+      l_d = mid_d;
+    }
+    if (l_d == r_d && l_q >= 125) break;
+  }
+  qb = 125;
+  db = (uint64_t)hdg::find_max_factor(l_d, n_data);
+  qb = (uint64_t)hdg::find_max_factor(qb, n_queries);
+  if (db * 4 <= n_data) db *= data_width;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hdg_last_error(void) { return g_last_error.c_str(); }
+
+hdg_status hdg_create(const char* algorithm, const char* conf_path, const char* filter_conf_path, int device,
+                      hdg_index** out) {
+  return guarded([&] {
+    if (!out || !conf_path) hdg::fail(HDG_EINVAL, "null argument");
+    *out = nullptr;
+    check_algorithm(algorithm);
+    check_device(device);
+    hdg::Config cfg;
+    hdg::parse_config_file(conf_path, cfg);
+    std::string fpath = filter_conf_path && *filter_conf_path ? std::string(filter_conf_path)
+                                                               : cfg.path + "filter.conf";
+    hdg::FilterConfig fc;
+    hdg::parse_filter_file(fpath, fc);
+    *out = new hdg_index{new hdg::Index(cfg, fc, device)};
+  });
+}
+
+hdg_status hdg_create_from_text(const char* algorithm, const char* conf_text, const char* filter_text, int device,
+                                hdg_index** out) {
+  return guarded([&] {
+    if (!out || !conf_text || !filter_text) hdg::fail(HDG_EINVAL, "null argument");
+    *out = nullptr;
+    check_algorithm(algorithm);
+    check_device(device);
+    hdg::Config cfg;
+    hdg::parse_config_text(conf_text, cfg);
+    hdg::FilterConfig fc;
+    hdg::parse_filter_text(filter_text, fc);
+    *out = new hdg_index{new hdg::Index(cfg, fc, device)};
+  });
+}
+
+hdg_status hdg_validate_config_text(const char* conf_text, const char* filter_text, char* out, uint64_t cap) {
+  return guarded([&] {
+    if (!conf_text || !filter_text || !out || cap == 0) hdg::fail(HDG_EINVAL, "null argument");
+    hdg::Config cfg;
+    hdg::parse_config_text(conf_text, cfg);
+    hdg::FilterConfig fc;
+    hdg::parse_filter_text(filter_text, fc);
+    std::string js = "{";
+    static const char* keys[] = {"DATA_WIDTH", "INDEX_WIDTH", "TOPK", "BREAK_DOWN", "ENABLE_MULTI_GPU",
+                                 "MEM_BOUND", "IS_CALC_MEM_PREDICTOR_COEFF", "LOWEST_QUERY_BATCH", "FILTER_DIM",
+                                 "PQ_DIM", "N_CLUSTERS", "EXPS0", "CLUSTERS", "SUB_CLUSTERS", "MAX_GRIDS", "N_LIST",
+                                 "SUB_LISTS", "N_SPLIT", "LABEL_MASK", "IS_SPLIT", "SEED", "KMEANS_ITERS",
+                                 "KMEANS_TRAIN", "PQ_TRAIN", "BALANCE_ITERS", "PARITY_BATCH", "QUERY_CHUNK"};
+    for (const char* k : keys) {
+      uint64_t v = 0;
+      hdg::get_config_value(cfg, k, v);
+      js += "\"" + std::string(k) + "\": " + std::to_string(v) + ", ";
+    }
+    std::string path;
+    for (char c : cfg.path) {
+      if (c == '"' || c == '\\') path += '\\';
+      path += c;
+    }
+    js += "\"PATH\": \"" + path + "\", \"l\": " + std::to_string(fc.l) + ", \"filter_type\": [";
+    for (int i = 0; i < fc.l; ++i) js += (i ? ", " : "") + std::to_string(fc.filter_type[i]);
+    js += "], \"shift_val\": [";
+    for (size_t i = 0; i < fc.shift_val.size(); ++i) {
+      char b[64];
+      std::snprintf(b, sizeof(b), "%.9g", (double)fc.shift_val[i]);
+      js += (i ? ", " : "") + std::string(b);
+    }
+    js += "], \"div_value\": [";
+    for (int i = 0; i < fc.l; ++i) js += (i ? ", " : "") + std::to_string(fc.div_value[i]);
+    js += "], \"interval_map\": [";
+    for (int i = 0; i < fc.l; ++i) {
+      js += i ? ", [" : "[";
+      for (size_t j = 0; j < fc.interval_map[i].size(); ++j)
+        js += (j ? ", " : "") + std::to_string(fc.interval_map[i][j]);
+      js += "]";
+    }
+    js += "]}";
+    if (js.size() + 1 > cap) hdg::fail(HDG_EINVAL, "output buffer too small");
+    std::memcpy(out, js.c_str(), js.size() + 1);
+  });
+}
+
+hdg_status hdg_build(hdg_index* index, const float* dataset, uint64_t n, uint64_t dim, const float* labels,
+                     uint64_t label_cols, void* stream) {
+  return guarded([&] {
+    if (!index) hdg::fail(HDG_EINVAL, "null index");
+    index->impl->build(dataset, n, dim, labels, label_cols, (hipStream_t)stream);
+  });
+}
+
+hdg_status hdg_query(hdg_index* index, const float* queries, uint64_t nq, uint64_t dim, const float* query_labels,
+                     uint64_t qlabel_cols, float* out_dist, uint64_t* out_ids, void* stream) {
+  return guarded([&] {
+    if (!index) hdg::fail(HDG_EINVAL, "null index");
+    index->impl->query(queries, nq, dim, query_labels, qlabel_cols, out_dist, out_ids, (hipStream_t)stream);
+  });
+}
+
+hdg_status hdg_destroy(hdg_index* index) {
+  return guarded([&] {
+    if (!index) return;
+    delete index->impl;
+    delete index;
+  });
+}
+
+hdg_status hdg_set_param(hdg_index* index, const char* key, const char* value) {
+  return guarded([&] {
+    if (!index || !key || !value) hdg::fail(HDG_EINVAL, "null argument");
+    if (!hdg::set_config_value(index->impl->config(), key, value))
+      hdg::fail(HDG_ECONFIG, std::string("unknown HDGrid.conf key: ") + key);
+  });
+}
+
+hdg_status hdg_get_param(const hdg_index* index, const char* key, uint64_t* value) {
+  return guarded([&] {
+    if (!index || !key || !value) hdg::fail(HDG_EINVAL, "null argument");
+    if (!hdg::get_config_value(index->impl->config(), key, *value))
+      hdg::fail(HDG_ECONFIG, std::string("unknown integer HDGrid.conf key: ") + key);
+  });
+}
+
+hdg_status hdg_save(const hdg_index* index, const char* dir) {
+  return guarded([&] {
+    if (!index || !dir) hdg::fail(HDG_EINVAL, "null argument");
+    index->impl->save(dir);
+  });
+}
+
+hdg_status hdg_load(hdg_index* index, const char* dir, void* stream) {
+  return guarded([&] {
+    if (!index || !dir) hdg::fail(HDG_EINVAL, "null argument");
+    index->impl->load(dir, (hipStream_t)stream);
+  });
+}
+
+hdg_status hdg_last_query_stats(const hdg_index* index, uint64_t nq, uint64_t* cand, uint64_t* real,
+                                uint64_t* refined, uint64_t* scored, uint64_t* adc, uint64_t* kprime) {
+  return guarded([&] {
+    if (!index) hdg::fail(HDG_EINVAL, "null index");
+    const hdg::QueryStats& st = index->impl->stats();
+    if (nq != st.nq) hdg::fail(HDG_EINVAL, "nq differs from the last query call");
+    auto cp = [&](uint64_t* dst, const std::vector<uint64_t>& src) {
+      if (dst) std::copy(src.begin(), src.end(), dst);
+    };
+    cp(cand, st.cand);
+    cp(real, st.real);
+    cp(refined, st.refined);
+    cp(scored, st.scored);
+    cp(adc, st.adc);
+    cp(kprime, st.kprime);
+  });
+}
+
+hdg_status hdg_last_chosen_subs(const hdg_index* index, uint64_t nq, uint32_t* out) {
+  return guarded([&] {
+    if (!index || !out) hdg::fail(HDG_EINVAL, "null argument");
+    const auto& v = index->impl->stats().chosen_subs;
+    const uint64_t sl = index->impl->config().sub_lists;
+    if (nq * sl != v.size()) hdg::fail(HDG_EINVAL, "nq differs from the last query chunk");
+    std::copy(v.begin(), v.end(), out);
+  });
+}
+
+hdg_status hdg_last_stage_ms(const hdg_index* index, char* names, uint64_t cap, float* ms, uint64_t* n) {
+  return guarded([&] {
+    if (!index || !n) hdg::fail(HDG_EINVAL, "null argument");
+    std::string joined;
+    const auto& st = index->impl->stage_ms;
+    for (size_t i = 0; i < st.size(); ++i) {
+      if (i) joined += ",";
+      joined += st[i].first;
+      if (ms) ms[i] = st[i].second;
+    }
+    *n = st.size();
+    if (names && cap) {
+      std::strncpy(names, joined.c_str(), cap - 1);
+      names[cap - 1] = 0;
+    }
+  });
+}
+
+hdg_status hdg_split_task(const double coeff[4], uint64_t n_data, uint64_t n_queries, uint64_t topk,
+                          uint64_t mem_bound, uint64_t available, uint64_t data_width, uint64_t* query_batch,
+                          uint64_t* data_batch) {
+  return guarded([&] {
+    if (!coeff || !query_batch || !data_batch) hdg::fail(HDG_EINVAL, "null argument");
+    split_task(coeff, n_data, n_queries, topk, mem_bound, available, data_width, *query_batch, *data_batch);
+  });
+}
+
+int hdg_find_max_factor(uint64_t a, uint64_t b) { return hdg::find_max_factor(a, b); }
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// HDGrid.conf and filter.conf parsers with the reference's grammar.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace hdg {
+
+// Keys of the reference's HDGrid.conf (config_utils.h:97-108, product-quantization.cuh:32-36,
+// Super-Grid-IVF_PQ.cuh:46-55).  Missing keys keep the defaults below (the README sample values);
+// the reference leaves them uninitialised.  Build-side keys that the reference does not have are
+// listed after the reference keys; the reference parser logs and skips them.
+struct Config {
+  // common_config
+  uint64_t data_width = 1;
+  uint64_t index_width = 0;
+  uint64_t topk = 100;
+  uint64_t break_down = 1;
+  uint64_t enable_multi_gpu = 0;
+  uint64_t mem_bound = 5000000000ull;
+  uint64_t is_calc_mem_predictor_coeff = 0;
+  uint64_t lowest_query_batch = 125;
+  uint64_t filter_dim = 1;
+  std::string path;
+  // pq::index_params
+  uint64_t pq_dim = 4;
+  uint64_t n_clusters = 256;
+  uint64_t exps0 = 30;
+  // super_grid::index_params
+  uint64_t clusters = 512;
+  uint64_t sub_clusters = 10;
+  uint64_t max_grids = 30000000;
+  uint64_t n_list = 64;
+  uint64_t sub_lists = 128;
+  uint64_t n_split = 1;
+  uint64_t label_mask = 0;
+  uint64_t is_split = 0;
+  // build-side keys of this implementation
+  uint64_t seed = 0x4844ull;           // SEED: k-means / sampling seed
+  uint64_t kmeans_iters = 16;          // KMEANS_ITERS: Lloyd iterations (fine + PQ)
+  uint64_t kmeans_train = 262144;      // KMEANS_TRAIN: max training rows for the fine k-means
+  uint64_t pq_train = 65536;           // PQ_TRAIN: max training rows per PQ subspace
+  uint64_t balance_iters = 25;         // BALANCE_ITERS: iterations of the sub-center grouping
+  uint64_t parity_batch = 0;           // PARITY_BATCH: beta group size (0 = findMaxFactor(125, nq))
+  uint64_t query_chunk = 16384;        // QUERY_CHUNK: queries processed per device pass
+};
+
+// Parse with config_utils.h:33-88 semantics: `KEY = value` per line, key case-insensitive and
+// trimmed; integer values stored as u64; other unquoted numbers written as f32 bits into the low
+// 4 bytes of the field; quoted values only for PATH.  Unknown keys are logged and skipped.
+// Throws hdg::Error(kConfigError) for an unreadable file or a value the reference would crash on.
+void parse_config_file(const std::string& path, Config& cfg);
+void parse_config_text(const std::string& text, Config& cfg);
+// Set one key (same value grammar); returns false for an unknown key.
+bool set_config_value(Config& cfg, const std::string& key, const std::string& value);
+// Read an integer key (case-insensitive); returns false for PATH or an unknown key.
+bool get_config_value(const Config& cfg, const std::string& key, uint64_t& value);
+
+// filter.conf (config_utils.h:112-172).
+struct FilterConfig {
+  int l = 0;
+  std::vector<int> filter_type;               // [l]
+  std::vector<float> shift_val;               // [2l]
+  std::vector<int> div_value;                 // [l]
+  std::vector<std::vector<int>> interval_map; // [l]
+  // all interval maps concatenated in filter order, as f32 (preprocessing.cu:145-150)
+  std::vector<float> flat_map() const;
+};
+void parse_filter_file(const std::string& path, FilterConfig& fc);
+void parse_filter_text(const std::string& text, FilterConfig& fc);
+
+}  // namespace hdg

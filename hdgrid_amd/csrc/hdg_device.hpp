@@ -1,0 +1,120 @@
+// Device building blocks shared by the query kernels: 256-thread workgroup scans, LDS bitonic sort,
+// radix selection over unique 64-bit keys, and the exact FP32 L2 in the reference's order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hdg {
+namespace dev {
+
+constexpr int kWG = 256;              // threads per workgroup for the per-query kernels
+constexpr int kSortCap = 4096;        // keys sorted in LDS (32 KiB)
+constexpr uint64_t kKeyPad = ~0ull;
+
+// Unique, totally ordered key for (non-negative f32 distance, 32-bit tie index).  For x >= 0 (and
+// +inf / NaN, which sort last) the IEEE bit pattern is monotone.
+__device__ __forceinline__ uint64_t make_key(float v, uint32_t tie) {
+  return ((uint64_t)__float_as_uint(v) << 32) | tie;
+}
+__device__ __forceinline__ float key_value(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+__device__ __forceinline__ uint32_t key_tie(uint64_t k) { return (uint32_t)k; }
+
+// Exclusive prefix sum over the 256 threads of a workgroup; `total` receives the sum.
+// `tmp` must hold kWG/64 + 1 words of LDS.
+__device__ __forceinline__ uint32_t wg_exclusive_scan(uint32_t v, uint32_t* tmp, uint32_t& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) tmp[wave] = x;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kWG / 64; ++w) {
+    uint32_t t = tmp[w];
+    if (w < wave) base += t;
+    tot += t;
+  }
+  __syncthreads();
+  total = tot;
+  return base + x - v;
+}
+
+// Bitonic sort of keys[0..n) in LDS ascending (n <= kSortCap); pads with kKeyPad up to a power of 2.
+__device__ inline void wg_sort_lds(uint64_t* keys, uint32_t n) {
+  uint32_t np = 1;
+  while (np < n) np <<= 1;
+  for (uint32_t i = n + threadIdx.x; i < np; i += kWG) keys[i] = kKeyPad;
+  __syncthreads();
+  for (uint32_t k = 2; k <= np; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t t = threadIdx.x; t < np / 2; t += kWG) {
+        uint32_t i = 2 * t - (t & (j - 1));
+        uint32_t p = i + j;
+        bool up = (i & k) == 0;
+        uint64_t a = keys[i], b = keys[p];
+        if ((a > b) == up) {
+          keys[i] = b;
+          keys[p] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Radix selection: writes the k smallest of the unique keys g[0..n) (global memory) to out[0..k),
+// unordered.  Requires k < n.  `hist` is 2048 words of LDS, `sh` 8 words of LDS.
+__device__ inline void wg_radix_select(const uint64_t* __restrict__ g, uint32_t n, uint32_t k,
+                                       uint64_t* __restrict__ out, uint32_t* hist, uint64_t* sh) {
+  uint64_t prefix = 0, mask = 0;
+  uint32_t need = k;  // rank (1-based) of the k-th key inside the current prefix bucket
+  const int shifts[6] = {53, 42, 31, 20, 9, 0};
+  const int widths[6] = {11, 11, 11, 11, 11, 9};
+  for (int pass = 0; pass < 6; ++pass) {
+    const int sft = shifts[pass];
+    const uint64_t dmask = (1ull << widths[pass]) - 1;
+    for (int i = threadIdx.x; i < 2048; i += kWG) hist[i] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += kWG) {
+      uint64_t key = g[i];
+      if ((key & mask) == prefix) atomicAdd(&hist[(key >> sft) & dmask], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t cum = 0, b = 0;
+      for (b = 0; b <= dmask; ++b) {
+        if (cum + hist[b] >= need) break;
+        cum += hist[b];
+      }
+      sh[0] = b;
+      sh[1] = cum;
+    }
+    __syncthreads();
+    const uint64_t b = sh[0];
+    need -= (uint32_t)sh[1];
+    prefix |= b << sft;
+    mask |= dmask << sft;
+    __syncthreads();
+  }
+  // prefix is now the k-th smallest key; keys are unique, so exactly k keys are <= prefix
+  if (threadIdx.x == 0) sh[2] = 0;
+  __syncthreads();
+  uint32_t* slot = reinterpret_cast<uint32_t*>(&sh[2]);
+  for (uint32_t i = threadIdx.x; i < n; i += kWG) {
+    uint64_t key = g[i];
+    if (key <= prefix) {
+      uint32_t s = atomicAdd(slot, 1u);
+      if (s < k) out[s] = key;  // exactly k qualify for unique keys; the guard bounds the write anyway
+    }
+  }
+  __syncthreads();
+}
+
+}  // namespace dev
+}  // namespace hdg

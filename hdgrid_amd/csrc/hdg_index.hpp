@@ -1,0 +1,106 @@
+// Super-Grid index: device-resident state, build and query entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hdg_common.hpp"
+#include "hdg_config.hpp"
+#include "hdg_grid.hpp"
+
+namespace hdg {
+
+// Per-query counters of the last query call (host copies), used for algorithmic-byte accounting.
+struct QueryStats {
+  std::vector<uint64_t> cand;      // C_q: Morton-range candidates
+  std::vector<uint64_t> real;      // in-box count of the chosen sub-clusters
+  std::vector<uint64_t> refined;   // R_q: vectors re-ranked exactly
+  std::vector<uint64_t> scored;    // S_q: centers + sub-centers scored exactly
+  std::vector<uint64_t> adc;       // candidates scanned with PQ ADC (0 when PQ keeps all)
+  std::vector<uint64_t> kprime;    // PQ keep width used
+  std::vector<uint32_t> chosen_subs;  // [nq][sub_lists] (UINT32_MAX pad), last chunk only
+  uint64_t nq = 0;
+};
+
+// Device layout.  Every per-point array is in z-pool order: sub-clusters in cluster-major order
+// (Super-Grid-IVF_PQ.cu:486-504), points inside a sub-cluster ascending by (Morton code, id)
+// (:604-617).  Reading a Morton range therefore reads contiguous memory for codes, labels,
+// cells and vectors.
+struct IndexState {
+  GridShape grid;
+  uint64_t N = 0, d = 0, C = 0, CS = 0, M = 0, K = 0, pq_len = 0;
+  int l = 0;
+  int max_subs_per_cluster = 0;
+  uint32_t max_seg_len = 0;
+  // host copies kept for save / inspection
+  std::vector<uint32_t> h_sub_off;   // [C+1] sub-cluster range of cluster c
+  std::vector<uint64_t> h_seg_off;   // [CS+1] z-pool range of sub-cluster s
+  std::vector<uint32_t> h_zid;       // [N]
+  std::vector<uint64_t> h_zcode;     // [N]
+  // device arrays
+  DevBuf vec;         // f32 [N][d]
+  DevBuf lab;         // f32 [N][l] (original label order)
+  DevBuf cells;       // u32 [n_valid][N] cell coordinate of each valid sorted dim
+  DevBuf zcode;       // u64 [N]
+  DevBuf zid;         // u32 [N] original row id
+  DevBuf codes;       // u8  [N][M]
+  DevBuf pq_centers;  // f32 [M][K][pq_len]
+  DevBuf centers;     // f32 [C][d]
+  DevBuf sub_centers; // f32 [CS][d] cluster-major
+  DevBuf sub_off;     // u32 [C+1]
+  DevBuf seg_off;     // u32 [CS+1]
+  DevBuf cluster_of_sub;  // u32 [CS]
+  bool built = false;
+};
+
+class Index {
+ public:
+  Index(const Config& cfg, const FilterConfig& fc, int device);
+  ~Index();
+
+  // Super-Grid-IVF_PQ.cu:420-646 (index::build).  dataset [n][d], labels [n][label_cols] are
+  // device pointers; label_cols must equal filter.conf's l.
+  void build(const float* dataset, uint64_t n, uint64_t d, const float* labels, uint64_t label_cols,
+             hipStream_t stream);
+  // Super-Grid-IVF_PQ.cu:985-1248 (index::query) on nq queries; reference batches of
+  // `parity_batch` queries share beta.  Outputs are device pointers [nq][topk].
+  void query(const float* queries, uint64_t nq, uint64_t d, const float* qlabels, uint64_t qcols,
+             float* out_dist, uint64_t* out_ids, hipStream_t stream);
+
+  void save(const std::string& dir) const;
+  void load(const std::string& dir, hipStream_t stream);
+
+  const Config& config() const { return cfg_; }
+  Config& config() { return cfg_; }
+  const FilterConfig& filter() const { return fc_; }
+  const IndexState& state() const { return st_; }
+  const QueryStats& stats() const { return stats_; }
+  int device() const { return device_; }
+
+  // timing of the last query call per stage (ms, host-side event timing), for profiling
+  std::vector<std::pair<std::string, float>> stage_ms;
+
+ private:
+  void query_chunk(const float* queries, uint64_t q0, uint64_t nq, const float* qlabels,
+                   uint64_t qcols, uint64_t group, float* out_dist, uint64_t* out_ids,
+                   hipStream_t stream);
+  Config cfg_;
+  FilterConfig fc_;
+  int device_;
+  IndexState st_;
+  QueryStats stats_;
+  // device copies of the filter configuration
+  DevBuf f_type_, f_shift_, f_map_, f_div_;
+  uint64_t f_map_len_ = 0;
+  // query workspace (grown on demand, reused across calls)
+  struct Work;
+  Work* w_;
+};
+
+// findMaxFactor (math_utils.h:40-60).
+int find_max_factor(uint64_t a, uint64_t b);
+
+}  // namespace hdg

@@ -1,0 +1,124 @@
+/*
+ * hdgrid.h -- C ABI of the MI355X-native Super-Grid (HDGrid) filtered-ANN search library.
+ *
+ * The entry points replace the reference's C++ operator API for the Super-Grid path
+ * (reference tree: wangzheng456123/HDGrid, paths relative to its root):
+ *   index_base_t::build / index_base_t::query      src/core/interfaces.cuh:27-46
+ *   super_grid::index::build                       src/index/2-level_index/Super-Grid-IVF_PQ.cu:420-646
+ *   super_grid::index::query                       src/index/2-level_index/Super-Grid-IVF_PQ.cu:985-1248
+ *   index_factory::create / config_factory::create src/index/index-factory.cuh:12-38,40-89
+ *   HDGrid.conf and filter.conf parsing            src/utils/config_utils.h:12-194
+ *
+ * Conventions.
+ *   - Array arguments of hdg_build / hdg_query are DEVICE pointers (HIP), row-major, on the index's
+ *     device, exactly like the reference's raft::device_matrix_view inputs and caller-allocated
+ *     outputs (interfaces.cuh:6-25).
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *   - Every call returns an hdg_status; on failure hdg_last_error() describes the error (thread
+ *     local).  The library never calls exit().
+ *   - Result ids are row indices of the dataset given to hdg_build; results are sorted by ascending
+ *     distance.  Missing results: id 2^32 (l == 1) or UINT64_MAX with distance 1e6 (EXPS0 > 1), or
+ *     id UINT64_MAX with distance FLT_MAX (EXPS0 <= 1) -- the reference's pad values.
+ */
+#ifndef HDGRID_H
+#define HDGRID_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hdg_index hdg_index;
+typedef int hdg_status;
+
+#define HDG_OK 0
+#define HDG_EINVAL 1       /* invalid argument or input the reference cannot represent */
+#define HDG_ECONFIG 2      /* HDGrid.conf / filter.conf error, unknown algorithm */
+#define HDG_EIO 3          /* file I/O */
+#define HDG_EHIP 4         /* HIP runtime error */
+#define HDG_ENOMEM 5       /* device allocation failed */
+#define HDG_ESTATE 6       /* call order (query before build, ...) */
+#define HDG_EUNSUPPORTED 7 /* outside the supported envelope (d > 512, clusters > 4096, ...) */
+
+/* Thread-local description of the last failure of a call on this thread. */
+const char* hdg_last_error(void);
+
+/* Creates an index for `algorithm` ("super-grid", case-insensitive) from an HDGrid.conf and a
+ * filter.conf on HIP device `device`.  filter_conf_path may be NULL: then <PATH>filter.conf is used,
+ * as the engine does (search-engine-auto-batch.cu:134-141).
+ * Replaces index_factory::create + config_factory::create (index-factory.cuh:12-38,40-89) and the
+ * super_grid::index constructor (Super-Grid-IVF_PQ.cuh:84-87). */
+hdg_status hdg_create(const char* algorithm, const char* conf_path, const char* filter_conf_path,
+                      int device, hdg_index** out);
+
+/* Same as hdg_create with the two configuration files given as text.  No reference counterpart:
+ * for hosts that hold configuration in memory. */
+hdg_status hdg_create_from_text(const char* algorithm, const char* conf_text, const char* filter_text,
+                                int device, hdg_index** out);
+
+/* Parses an HDGrid.conf text and a filter.conf text without touching any device and writes the parsed
+ * values as one JSON object into out (cap bytes).  No reference counterpart: configuration checking
+ * for hosts and tools (the reference parses inside its constructors, config_utils.h:33-172). */
+hdg_status hdg_validate_config_text(const char* conf_text, const char* filter_text, char* out, uint64_t cap);
+
+/* Builds (or rebuilds) the index from dataset [n][dim] f32 and labels [n][label_cols] f32 (device
+ * pointers; label_cols must equal filter.conf's l).  The index copies what it needs; the inputs may
+ * be freed afterwards.
+ * Replaces super_grid::index::build(build_input_t const&) (Super-Grid-IVF_PQ.cu:420-646). */
+hdg_status hdg_build(hdg_index* index, const float* dataset, uint64_t n, uint64_t dim,
+                     const float* labels, uint64_t label_cols, void* stream);
+
+/* Filtered top-k search of queries [nq][dim] f32 with query_labels [nq][qlabel_cols] f32
+ * (qlabel_cols = l * FILTER_DIM).  Writes out_dist [nq][topk] f32 and out_ids [nq][topk] u64 (device
+ * pointers).  Queries are processed in reference batches of PARITY_BATCH (default
+ * findMaxFactor(125, nq), search-engine-auto-batch.cu:484-486) that share beta.
+ * Replaces super_grid::index::query(query_input_t const&, query_output_t&)
+ * (Super-Grid-IVF_PQ.cu:985-1248). */
+hdg_status hdg_query(hdg_index* index, const float* queries, uint64_t nq, uint64_t dim,
+                     const float* query_labels, uint64_t qlabel_cols, float* out_dist,
+                     uint64_t* out_ids, void* stream);
+
+/* Releases the index and its device memory (the reference frees MEM_INDEX, mmr.cuh:73-80). */
+hdg_status hdg_destroy(hdg_index* index);
+
+/* Applies one HDGrid.conf `key = value` (same grammar as the file, config_utils.h:33-88). */
+hdg_status hdg_set_param(hdg_index* index, const char* key, const char* value);
+/* Reads an integer HDGrid.conf key. */
+hdg_status hdg_get_param(const hdg_index* index, const char* key, uint64_t* value);
+
+/* Writes the index state to directory `dir` (.npy arrays + meta.json) / reads it back.  No reference
+ * counterpart (the reference rebuilds the index per data batch): checkpoint/resume and the shared
+ * state that the CPU oracle is run on. */
+hdg_status hdg_save(const hdg_index* index, const char* dir);
+hdg_status hdg_load(hdg_index* index, const char* dir, void* stream);
+
+/* Per-query counters of the last hdg_query (host arrays of nq entries; any may be NULL):
+ * cand = Morton-range candidates C_q (Super-Grid-IVF_PQ.cu:1184), real = in-box count of the chosen
+ * sub-clusters (:1199-1210), refined = vectors re-ranked exactly, scored = centers + sub-centers
+ * scored exactly, adc = candidates scanned with PQ, kprime = PQ keep width. */
+hdg_status hdg_last_query_stats(const hdg_index* index, uint64_t nq, uint64_t* cand, uint64_t* real,
+                                uint64_t* refined, uint64_t* scored, uint64_t* adc, uint64_t* kprime);
+
+/* Chosen sub-clusters of the last hdg_query's last chunk, [nq][SUB_LISTS] u32 host array in
+ * z-pool sub-cluster numbering, UINT32_MAX padded (Super-Grid-IVF_PQ.cu:1046-1056 output). */
+hdg_status hdg_last_chosen_subs(const hdg_index* index, uint64_t nq, uint32_t* out);
+
+/* Stage timings (ms) of the last hdg_query chunk: names joined by ',' into `names` (cap bytes),
+ * values into ms[0..*n). */
+hdg_status hdg_last_stage_ms(const hdg_index* index, char* names, uint64_t cap, float* ms, uint64_t* n);
+
+/* Memory-bound batch predictor: split_task (search-engine-auto-batch.cu:430-489) with the
+ * reference's model c0*d + c1*q + c2*d*q + c3 <= min(available/5, mem_bound). */
+hdg_status hdg_split_task(const double coeff[4], uint64_t n_data, uint64_t n_queries, uint64_t topk,
+                          uint64_t mem_bound, uint64_t available, uint64_t data_width,
+                          uint64_t* query_batch, uint64_t* data_batch);
+
+/* findMaxFactor (math_utils.h:40-60): largest factor of b below a (b itself when b <= a). */
+int hdg_find_max_factor(uint64_t a, uint64_t b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HDGRID_H */

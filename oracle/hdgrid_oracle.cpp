@@ -1,0 +1,943 @@
+// HDGrid Super-Grid oracle: a CPU restatement of the reference's filtered-ANN query path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP implementation in hdgrid_amd/.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The product never
+// links, loads or calls it.
+//
+// Every function cites the reference file:line it restates (paths relative to the reference tree).
+// Floating-point operations follow the reference's order and rounding:
+//   * exact L2 (refine.cu:36-75): per thread t < T=min(d,512) an FMA chain over dims t, t+T, ...
+//     (nvcc contracts `sum += diff*diff` into fma), then thread 0 adds the T partials in order;
+//   * PQ LUT (product-quantization.cu:4-35): `ans += (c-q)*(c-q)` contracted to an fma chain over pq_len;
+//   * ADC (product-quantization.cu:37-77): plain f32 adds over m = 0..M-1 starting from 0;
+//   * filter recovery (preprocessing.cu:3-30): CUDA's max(float,double) is fmax in double, the result is
+//     rounded to f32 on store;
+//   * grid cells (Super-Grid-IVF_PQ.cu:24-26): f32 subtract/multiply, saturating f32->u64, u64 divide.
+// The file is compiled with -ffp-contract=off so that every fma here is an explicit fmaf().
+//
+// Selection tie-breaks (RAFT select_k order is not pinned by any reference test):
+//   coarse / sub-cluster / PQ selection: ascending (distance, column of the reference's candidate matrix);
+//   final re-rank: ascending (exact distance, id).
+//
+// Parity status per function is listed in the header block of each section.
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <queue>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int kMaxDims = 8;                       // zorder.cuh:13
+constexpr uint64_t kIdPad = 4294967296ull;        // sqrt((double)UINT64_MAX): mat_operators.cuh:185
+constexpr uint64_t kU64Max = ~0ull;               // compaction pad: filter.cu:171,227
+constexpr float kExactPad = 1e6f;                 // refine.cu:53
+
+thread_local std::string g_err;
+
+// f32 -> u64 conversion as the GPU performs it (cvt.rzi.u64.f32 saturates; NaN -> 0).
+inline uint64_t f2u_sat(float x) {
+  if (!(x > 0.0f)) return 0;                       // negative, zero and NaN
+  if (x >= 18446744073709551616.0f) return kU64Max;
+  return (uint64_t)x;
+}
+
+// f32 -> u64 conversion as GCC emits it for x86-64 host code (search-engine host code, e.g.
+// Super-Grid-IVF_PQ.cu:1222):  x < 2^63 -> cvttss2si(x);  else cvttss2si(x - 2^63) ^ 2^63.
+// cvttss2si returns 0x8000000000000000 for NaN / out-of-range inputs.
+inline uint64_t f2u_host(float x) {
+  auto cvt = [](float v) -> int64_t {
+    if (std::isnan(v) || v >= 9223372036854775808.0f || v < -9223372036854775808.0f)
+      return INT64_MIN;
+    return (int64_t)v;
+  };
+  if (x < 9223372036854775808.0f) return (uint64_t)cvt(x);
+  return (uint64_t)cvt(x - 9223372036854775808.0f) ^ 0x8000000000000000ull;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Exact FP32 L2 in the reference's order: refine.cu:36-75 (compute_l2_distances_kernel_shared).
+// Pinned by: closed form (d <= 512 is a sequential sum of rounded squares), tests compare with
+// an FP64 restatement within 4 ulp-scaled bounds and with hand-computed small cases.
+inline float exact_l2(const float* q, const float* x, uint64_t d) {
+  const uint64_t T = std::min<uint64_t>(d, 512);
+  float total = 0.0f;
+  for (uint64_t t = 0; t < T; ++t) {
+    float sum = 0.0f;
+    for (uint64_t i = t; i < d; i += T) {
+      float diff = q[i] - x[i];
+      sum = fmaf(diff, diff, sum);
+    }
+    total += sum;
+  }
+  return total;
+}
+
+struct Key {
+  float v;
+  uint64_t col;
+};
+inline bool key_less(const Key& a, const Key& b) {
+  if (a.v < b.v) return true;
+  if (b.v < a.v) return false;
+  return a.col < b.col;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+// =============================================================================================
+// A1: filter recovery -- preprocessing.cu:3-30 (denormalize_labels_kernel), :206-230.
+// qlabels is the flat [nq][qcols] array; types 0..2 read element t*l+i and type 3 reads elements
+// 2(t*l+i) and 2(t*l+i)+1 of that flat array, exactly as the kernel indexes it.
+// interval map: all map filters share one flattened map (preprocessing.cu:145-150), 1-based labels.
+// Returns 0 on success, 1 if a map label falls outside the map or a div value is 0.
+// Pinned by: test_filter.cuh pass pattern (predicate side) and closed-form checks in tests.
+int orc_recover_filters(const float* qlabels, uint64_t nq, int l, const int* ftype,
+                        const float* shift_val, const float* imap, uint64_t imap_len,
+                        const int* div_value, float* out) {
+  for (uint64_t t = 0; t < nq; ++t) {
+    for (int i = 0; i < l; ++i) {
+      uint64_t idx = t * (uint64_t)l + (uint64_t)i;
+      switch (ftype[i]) {
+        case 0: {
+          double lo = (double)qlabels[idx] - std::fmax((double)shift_val[2 * i], 1e-7);
+          double hi = (double)qlabels[idx] + std::fmax((double)shift_val[2 * i + 1], 1e-7);
+          out[idx * 2] = (float)lo;
+          out[idx * 2 + 1] = (float)hi;
+          break;
+        }
+        case 1: {
+          int val = (int)(qlabels[idx] - 1.0f);
+          if (val < 0 || (uint64_t)(2 * (int64_t)val + 1) >= imap_len) {
+            g_err = "map filter label outside interval_map";
+            return 1;
+          }
+          out[idx * 2] = imap[2 * val];
+          out[idx * 2 + 1] = imap[2 * val + 1];
+          break;
+        }
+        case 2: {
+          int val = (int)qlabels[idx];
+          if (div_value[i] == 0) {
+            g_err = "div filter with div_value 0";
+            return 1;
+          }
+          out[idx * 2] = (float)(val / div_value[i]);
+          out[idx * 2 + 1] = FLT_MAX;
+          break;
+        }
+        case 3:
+          out[2 * idx] = qlabels[2 * idx];
+          out[2 * idx + 1] = qlabels[2 * idx + 1];
+          break;
+        default:
+          break;  // unknown type: the kernel leaves the range untouched
+      }
+    }
+  }
+  return 0;
+}
+
+// =============================================================================================
+// B1: grid shape -- Super-Grid-IVF_PQ.cu:305-412 (calc_grid_shape) + preprocessing.cu:175-204.
+// Outputs are in ascending-span ("sorted") dim order: dims[i], span[i], left[i], shuffle[i] = original
+// dim of sorted position i.  morton[0]=dims, [1]=max_bits, [2]=valid_map.
+// The span is ceilf(max) - floorf(min) evaluated in f32 (the float overloads of ceil/floor).
+// Returns 1 for inputs the reference cannot handle (negative labels -> UB u64 cast; zero span -> /0).
+// Pinned by: closed form + hand-derived shapes for the configs (e.g. [4,1369,5477] at max_grids 3e7).
+int orc_grid_shape(const float* labels, uint64_t n, int l, uint64_t max_grids, uint64_t label_mask_cfg,
+                   int* dims, uint64_t* span, uint64_t* left, int* shuffle, int* morton,
+                   uint64_t* tot_grids) {
+  if (l < 1 || l > kMaxDims) {
+    g_err = "label dims must be in [1, 8]";
+    return 1;
+  }
+  std::vector<float> gmin(l, FLT_MAX), gmax(l, -FLT_MAX);
+  for (uint64_t p = 0; p < n; ++p)
+    for (int i = 0; i < l; ++i) {
+      float v = labels[p * l + i];
+      gmin[i] = std::min(gmin[i], v);
+      gmax[i] = std::max(gmax[i], v);
+    }
+  uint8_t label_mask = (uint8_t)~0u;
+  uint8_t mask = (uint8_t)label_mask_cfg;
+  label_mask &= (uint8_t)~mask;
+  int cnt = 0;
+  for (int i = 0; i < l; ++i)
+    if ((label_mask >> i) & 1) cnt++;
+  std::vector<std::pair<uint64_t, int>> span_pairs(l);
+  for (int i = 0; i < l; ++i) {
+    if (gmin[i] < 0.0f) {
+      g_err = "negative label values are not supported";
+      return 1;
+    }
+    float s = std::ceil(gmax[i]) - std::floor(gmin[i]);
+    span_pairs[i] = std::make_pair(f2u_host(s), i);
+  }
+  int k = 1;
+  int main_grids = (int)(k * std::floor(std::sqrt((double)max_grids)));
+  std::sort(span_pairs.begin(), span_pairs.end());
+  if (l == 1) main_grids = (int)std::min<uint64_t>(max_grids, span_pairs[0].first);
+  int remain = 1, remain_cnt = cnt - 1, last_valid = l - 1;
+  if (l > 1) remain = (int)(max_grids / (uint64_t)main_grids);
+  uint8_t morton_mask = (uint8_t)~0u;
+  for (int i = 0; i < l; ++i) {
+    int id = span_pairs[i].second;
+    uint64_t len = span_pairs[i].first;
+    if ((label_mask >> id) & 1) {
+      if (remain_cnt == 0) {
+        last_valid = i;
+        continue;
+      }
+      if (len == 0) {
+        g_err = "label column with zero span";
+        return 1;
+      }
+      uint64_t root = (uint64_t)std::floor(std::pow((double)main_grids, 1.0 / (double)(float)(cnt - 1)));
+      int dims_len = (int)std::min<uint64_t>((uint64_t)remain, std::min<uint64_t>(len, root));
+      if (remain_cnt == 1) dims_len = (int)std::min<uint64_t>((uint64_t)remain, len);
+      remain_cnt--;
+      dims[i] = dims_len;
+      if (dims_len == 0) {
+        g_err = "grid dim of size 0";
+        return 1;
+      }
+      remain /= dims_len;
+    } else {
+      morton_mask &= (uint8_t)~(1 << i);
+      dims[i] = 1;
+    }
+  }
+  if (span_pairs[last_valid].first == 0 || main_grids <= 0) {
+    g_err = "label column with zero span";
+    return 1;
+  }
+  dims[last_valid] = main_grids;
+  uint64_t tg = 1;
+  for (int i = 0; i < l; ++i) tg *= (uint64_t)dims[i];
+  *tot_grids = tg;
+  for (int i = 0; i < l; ++i) {
+    int id = span_pairs[i].second;
+    span[i] = span_pairs[i].first;
+    left[i] = f2u_host(std::floor(gmin[id]));
+    shuffle[i] = id;
+  }
+  int maxd = *std::max_element(dims, dims + l);
+  morton[0] = l;
+  morton[1] = (int)std::ceil(std::log2((double)(maxd + 1)));
+  uint8_t final_mask = (uint8_t)((1ull << l) - 1);
+  morton[2] = (int)(uint8_t)(morton_mask & final_mask);
+  return 0;
+}
+
+// B2 cell of one value: Super-Grid-IVF_PQ.cu:24-26 (get_grid_id_device).  Returns the int the device
+// function returns (u64 result truncated to int).
+static inline int grid_id(float x, uint64_t l, uint64_t len, int M) {
+  float t = (x - (float)l) * (float)M;
+  return (int)(uint32_t)(f2u_sat(t) / len);
+}
+
+// B2: per-point cell coordinates in sorted dim order, clamped to dims-1, and flat row-major id
+// (Super-Grid-IVF_PQ.cu:28-52).  coords [n][l].
+void orc_grid_cells(const float* labels, uint64_t n, int l, const int* dims, const uint64_t* span,
+                    const uint64_t* left, const int* shuffle, int* coords, uint64_t* flat) {
+  for (uint64_t p = 0; p < n; ++p) {
+    int* c = coords + p * l;
+    for (int i = 0; i < l; ++i) {
+      int d = shuffle[i];
+      int g = grid_id(labels[p * l + d], left[i], span[i], dims[i]);
+      c[i] = std::min(g, dims[i] - 1);
+    }
+    if (flat) {
+      // flatten_index (Super-Grid-IVF_PQ.cu:3-13) computes in int
+      int idx = 0, stride = 1;
+      for (int i = l - 1; i >= 0; --i) {
+        idx += c[i] * stride;
+        stride *= dims[i];
+      }
+      flat[p] = (uint64_t)(int64_t)idx;
+    }
+  }
+}
+
+// A2: query ranges -> integer grid boxes, sorted dim order, pairs (lo,hi):
+// Super-Grid-IVF_PQ.cu:930-961 (get_filter_grid_ranges_kernel).
+void orc_filter_grid_ranges(const float* ranges, uint64_t nq, int l, const int* dims,
+                            const uint64_t* span, const uint64_t* left, const int* shuffle, int* out) {
+  for (uint64_t q = 0; q < nq; ++q) {
+    const float* box = ranges + q * 2 * l;
+    int* o = out + q * 2 * l;
+    for (int i = 0; i < l; ++i) {
+      int d = shuffle[i];
+      float lo = (float)std::max<uint64_t>(f2u_sat(box[2 * d]), left[i]);
+      float hi = (float)std::min<uint64_t>(f2u_sat(box[2 * d + 1]), left[i] + span[i]);
+      int gl = grid_id(lo, left[i], span[i], dims[i]);
+      int gr = grid_id(hi, left[i], span[i], dims[i]);
+      o[2 * i + 1] = std::min(gr, dims[i] - 1);
+      o[2 * i] = std::min(o[2 * i + 1], gl);
+    }
+  }
+}
+
+// B5/A7: Morton encode -- zorder.cu:3-15.  MSB-first bit planes, dims in sorted order, masked dims
+// skipped.  Pinned by: bit-interleave closed form and decode round trip (zorder.cu:17-34) in tests.
+uint64_t orc_morton(const int* coords, int ndims, int max_bits, int valid_map) {
+  uint64_t code = 0;
+  for (int b = 0; b < max_bits; ++b)
+    for (int i = 0; i < ndims; ++i) {
+      if (!((valid_map >> i) & 1)) continue;
+      int bit = (coords[i] >> (max_bits - 1 - b)) & 1;
+      code = (code << 1) | (uint64_t)bit;
+    }
+  return code;
+}
+
+void orc_morton_batch(const int* coords, uint64_t n, int ndims, int max_bits, int valid_map,
+                      uint64_t* out) {
+  for (uint64_t p = 0; p < n; ++p) out[p] = orc_morton(coords + p * ndims, ndims, max_bits, valid_map);
+}
+
+// =============================================================================================
+// B3/B4/A3: multi-dim prefix sums, compressed rows, corner queries.  HOPS.cu:3-74,
+// Super-Grid-IVF_PQ.cu:79-303 (write_grid, compress_HOPS_data, filter_zero_tail_kernel,
+// compute_diff_nd) and :648-790 (query_compressed_sum_device, get_prefix_sum_device,
+// get_prefix_sum_kernel, get_cluster_sum_kernel).  Used on small grids; tests pin it against a
+// brute-force box count (the closed form it must equal).
+struct OrcHops {
+  std::vector<int> dims;
+  int diff_dim = 0;
+  struct Node {
+    int64_t sums;  // offset into pool, -1 == nullptr
+    int pre;
+    int len;
+  };
+  std::vector<Node> nodes;
+  std::vector<int> pos, pre;  // pool of int2 (pos, prefix)
+};
+
+static int flatten(const int* c, const int* dims, int nd) {
+  int idx = 0, stride = 1;
+  for (int i = nd - 1; i >= 0; --i) {
+    idx += c[i] * stride;
+    stride *= dims[i];
+  }
+  return idx;
+}
+
+static void unpack_except(int flat_idx, const int* shape, int nd, int dim, int* c) {
+  for (int i = nd - 1; i >= 0; --i) {
+    int is_dim = (i == dim);
+    c[i] = ((1 - is_dim) * flat_idx) % shape[i];
+    flat_idx /= (1 - is_dim) * shape[i] + is_dim;
+  }
+}
+
+void* orc_hops_build(const int* coords, uint64_t n, int l, const int* dims) {
+  OrcHops* h = new OrcHops();
+  h->dims.assign(dims, dims + l);
+  uint64_t tot = 1;
+  for (int i = 0; i < l; ++i) tot *= (uint64_t)dims[i];
+  std::vector<int64_t> cnt(tot, 0);
+  for (uint64_t p = 0; p < n; ++p) cnt[flatten(coords + p * l, dims, l)]++;
+  // inclusive prefix along every axis (HOPS.cu:70-74)
+  for (int axis = 0; axis < l; ++axis) {
+    uint64_t inner = 1;
+    for (int i = axis + 1; i < l; ++i) inner *= (uint64_t)dims[i];
+    uint64_t axis_size = (uint64_t)dims[axis];
+    uint64_t outer = tot / (inner * axis_size);
+    for (uint64_t a = 0; a < outer; ++a)
+      for (uint64_t c = 0; c < inner; ++c)
+        for (uint64_t b = 1; b < axis_size; ++b) {
+          uint64_t cur = (a * axis_size + b) * inner + c;
+          cnt[cur] += cnt[cur - inner];
+        }
+  }
+  int diff_dim = l - 1;
+  for (int i = l - 1; i >= 0; --i)
+    if (dims[i] > 1) {
+      diff_dim = i;
+      break;
+    }
+  h->diff_dim = diff_dim;
+  int max_len = dims[diff_dim];
+  uint64_t rows = tot / (uint64_t)max_len;
+  // row mask: filter_zero_tail_kernel (Super-Grid-IVF_PQ.cu:161-203)
+  std::vector<int> mask(rows, 1);
+  if (l > 1) {
+    for (uint64_t r = 0; r < rows; ++r) {
+      int c[kMaxDims];
+      unpack_except((int)r, dims, l, diff_dim, c);
+      int last_dim = diff_dim - 1;
+      while (last_dim >= 0 && dims[last_dim] <= 1) last_dim--;
+      if (last_dim < 0 || c[last_dim] == 0) {
+        mask[r] = 1;
+        continue;
+      }
+      int dv = 0;
+      for (int i = 0; i < max_len; ++i) {
+        c[diff_dim] = i;
+        int i1 = flatten(c, dims, l);
+        c[last_dim]--;
+        int i2 = flatten(c, dims, l);
+        c[last_dim]++;
+        if (cnt[i1] != cnt[i2]) {
+          dv = 1;
+          break;
+        }
+      }
+      mask[r] = dv;
+    }
+  }
+  h->nodes.resize(rows);
+  uint64_t last = 0;
+  for (uint64_t r = 0; r < rows; ++r) {
+    h->nodes[r].pre = (int)last;
+    h->nodes[r].sums = -1;
+    h->nodes[r].len = 0;
+    if (mask[r]) {
+      last = r;
+      int c[kMaxDims];
+      unpack_except((int)r, dims, l, diff_dim, c);
+      int64_t start = (int64_t)h->pos.size();
+      int len = 0;
+      for (int t = 0; t < max_len; ++t) {
+        c[diff_dim] = t;
+        int64_t cur = cnt[flatten(c, dims, l)];
+        bool keep;
+        if (t == 0) {
+          keep = true;
+        } else {
+          c[diff_dim] = t - 1;
+          keep = cur - cnt[flatten(c, dims, l)] != 0;
+          c[diff_dim] = t;
+        }
+        if (keep) {
+          h->pos.push_back(t);
+          h->pre.push_back((int)cur);
+          len++;
+        }
+      }
+      h->nodes[r].sums = start;
+      h->nodes[r].len = len;
+    }
+  }
+  return h;
+}
+
+void orc_hops_free(void* h) { delete (OrcHops*)h; }
+
+static uint64_t hops_point(const OrcHops* h, const int* coords) {
+  int l = (int)h->dims.size();
+  uint64_t fr = (uint64_t)(int64_t)flatten(coords, h->dims.data(), l);
+  fr /= (uint64_t)h->dims[h->diff_dim];
+  int key = coords[h->diff_dim];
+  OrcHops::Node node = h->nodes[fr];
+  if (node.sums < 0) node = h->nodes[node.pre];
+  uint64_t res = 0;
+  int lo = 0, hi = node.len - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    if (h->pos[node.sums + mid] <= key) {
+      res = (uint64_t)(int64_t)h->pre[node.sums + mid];
+      lo = mid + 1;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return res;
+}
+
+// get_prefix_sum_device + the tid-0 sum of get_prefix_sum_kernel (Super-Grid-IVF_PQ.cu:693-765).
+// box: [2l] sorted-order pairs.  h == nullptr (empty sub-cluster) -> 0.
+uint64_t orc_hops_count(const void* hv, const int* box, int valid_map) {
+  const OrcHops* h = (const OrcHops*)hv;
+  if (!h) return 0;
+  int real_dim = (int)h->dims.size();
+  int d = 0;
+  for (int i = 0; i < real_dim; ++i)
+    if ((valid_map >> i) & 1) d++;
+  uint64_t sum = 0;
+  for (int mask = 0; mask < (1 << d); ++mask) {
+    int coords[kMaxDims] = {0};
+    int sign = 1;
+    int vm = valid_map, cur = 0;
+    bool zero = false;
+    for (int i = 0; i < d; ++i) {
+      if (vm) {
+        while (!((vm >> cur) & 1)) cur++;
+        vm &= ~(1 << cur);
+      }
+      if ((mask >> i) & 1) {
+        coords[cur] = box[2 * cur];
+        sign *= -1;
+        if (coords[cur] > 0) {
+          coords[cur]--;
+        } else {
+          zero = true;
+          break;
+        }
+      } else {
+        coords[cur] = box[2 * cur + 1];
+      }
+    }
+    int part = 0;
+    if (!zero) {
+      int s = (int)hops_point(h, coords);
+      part = (int)(uint64_t)(int64_t)(sign * s);
+    }
+    sum += (uint64_t)(int64_t)part;
+  }
+  return sum;
+}
+
+// Brute-force box count: #points whose cell coords lie in the box on every valid dim.  This is the
+// closed form the corner queries must equal (pins B3/B4/A3).
+uint64_t orc_box_count(const int* coords, uint64_t n, int l, const int* box, int valid_map) {
+  uint64_t c = 0;
+  for (uint64_t p = 0; p < n; ++p) {
+    bool in = true;
+    for (int i = 0; i < l && in; ++i) {
+      if (!((valid_map >> i) & 1)) continue;
+      int v = coords[p * l + i];
+      if (v < box[2 * i] || v > box[2 * i + 1]) in = false;
+    }
+    c += in;
+  }
+  return c;
+}
+
+// =============================================================================================
+// A11: PQ LUT and ADC -- product-quantization.cu:4-35,37-77.
+// centers [M][K][pq_len]; q [d]; lut [M][K].  Pinned by the golden vectors of test_lut_build.cuh:41-60
+// and test_compute_pq_dis.cuh:47-97 plus ground_truth.cuh:60-163.
+void orc_pq_lut(const float* centers, const float* q, uint64_t d, uint64_t M, uint64_t K,
+                uint64_t pq_len, float* lut) {
+  for (uint64_t m = 0; m < M; ++m)
+    for (uint64_t c = 0; c < K; ++c) {
+      float ans = 0.0f;
+      for (uint64_t t = 0; t < pq_len; ++t) {
+        if (m * pq_len + t >= d) break;
+        float diff = centers[m * pq_len * K + c * pq_len + t] - q[m * pq_len + t];
+        ans = fmaf(diff, diff, ans);
+      }
+      lut[m * K + c] = ans;
+    }
+}
+
+// codes column-major [M][N] (reference layout, product-quantization.cu:70)
+float orc_adc(const uint8_t* codes, uint64_t n_total, uint64_t id, const float* lut, uint64_t M,
+              uint64_t K) {
+  float sum = 0.0f;
+  for (uint64_t m = 0; m < M; ++m) sum += lut[m * K + codes[m * n_total + id]];
+  return sum;
+}
+
+// Exact L2 exported for tests.
+float orc_exact_l2(const float* q, const float* x, uint64_t d) { return exact_l2(q, x, d); }
+
+// =============================================================================================
+// Whole query path: Super-Grid-IVF_PQ.cu:985-1248 (index::query) for one index state, run in
+// reference batches of `batch` queries (search-engine-auto-batch.cu:250; batch = findMaxFactor(125,nq)).
+struct OrcState {
+  // sizes
+  uint64_t N, d, C, CS, M, K, pq_len;
+  int l;
+  // config
+  uint64_t n_list, sub_lists, exps0, topk;
+  // grid
+  const int* dims;
+  const uint64_t* span;
+  const uint64_t* left;
+  const int* shuffle;
+  int max_bits, valid_map;
+  // filter config
+  const int* ftype;
+  const float* shift_val;
+  const float* imap;
+  uint64_t imap_len;
+  const int* div_value;
+  // data (original order)
+  const float* data;    // [N][d]
+  const float* labels;  // [N][l]
+  // index
+  const float* centers;      // [C][d]
+  const float* sub_centers;  // [CS][d]
+  const uint64_t* sub_off;   // [C] first sub-cluster of cluster c
+  const uint64_t* sub_cnt;   // [C]
+  const uint64_t* seg_off;   // [CS] z-pool offset of sub-cluster s
+  const uint64_t* seg_len;   // [CS]
+  const uint64_t* zcodes;    // [N] z-pool
+  const uint64_t* zids;      // [N] z-pool
+  const float* pq_centers;   // [M][K][pq_len]
+  const uint8_t* codes;      // [M][N] by original id
+};
+
+// Per-query trace for tests: sizes and checks of every intermediate stage.
+struct OrcTrace {
+  int64_t* n_clusters_nz;   // [nq]
+  int64_t* n_subs_nz;       // [nq]
+  int64_t* cand_count;      // [nq] C_q
+  int64_t* real_count;      // [nq] in-box count of chosen sub-clusters
+  int64_t* n_survivors;     // [nq] refine input entries with id < N
+  int64_t* k_prime;         // [nq] PQ keep width used for this query
+  uint64_t* chosen_subs;    // [nq][sub_lists] chosen sub-cluster ids, UINT64_MAX pad (may be null)
+  uint64_t* sub_counts;     // [nq][CS] in-box counts (may be null)
+};
+
+static void point_cells(const OrcState& S, std::vector<int>& cells) {
+  cells.resize(S.N * S.l);
+  orc_grid_cells(S.labels, S.N, S.l, S.dims, S.span, S.left, S.shuffle, cells.data(), nullptr);
+}
+
+// Top-k of keys by key_less (stable unique keys); returns sorted.
+static void topk_sorted(std::vector<Key>& items, uint64_t k) {
+  k = std::min<uint64_t>(k, items.size());
+  std::partial_sort(items.begin(), items.begin() + k, items.end(), key_less);
+  items.resize(k);
+}
+
+int orc_query(const OrcState* Sp, const float* queries, const float* qlabels, uint64_t nq,
+              uint64_t qcols, uint64_t batch, float* out_dist, uint64_t* out_ids, OrcTrace* trace) {
+  const OrcState& S = *Sp;
+  const int l = S.l;
+  const uint64_t k = S.topk;
+  if (qcols < (uint64_t)l) {
+    g_err = "query label columns < l";
+    return 1;
+  }
+  for (int i = 0; i < l; ++i)
+    if (S.ftype[i] == 3 && qcols < 2 * (uint64_t)l) {
+      g_err = "explicit-range filter needs 2*l query label columns";
+      return 1;
+    }
+  std::vector<int> cells;
+  point_cells(S, cells);
+  // z-pool cells: cells of point zids[p]
+  std::vector<float> ranges(nq * 2 * l);
+  std::vector<int> boxes(nq * 2 * l);
+  if (batch == 0) batch = nq;
+  for (uint64_t b0 = 0; b0 < nq; b0 += batch) {
+    const uint64_t b1 = std::min(nq, b0 + batch);
+    const uint64_t nb = b1 - b0;
+    // ---- A1/A2 on this batch's [nb][qcols] view: the kernel indexes the view's flat array ----
+    if (orc_recover_filters(qlabels + b0 * qcols, nb, l, S.ftype, S.shift_val, S.imap, S.imap_len,
+                            S.div_value, &ranges[b0 * 2 * l]))
+      return 1;
+    orc_filter_grid_ranges(&ranges[b0 * 2 * l], nb, l, S.dims, S.span, S.left, S.shuffle,
+                           &boxes[b0 * 2 * l]);
+    // ---- A3: in-box counts per (query, sub-cluster), cluster sums (int accumulation) ----
+    std::vector<uint64_t> counts(nb * S.CS);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t qi = 0; qi < (int64_t)nb; ++qi) {
+      const int* box = &boxes[(b0 + qi) * 2 * l];
+      for (uint64_t s = 0; s < S.CS; ++s) {
+        uint64_t c = 0;
+        for (uint64_t p = S.seg_off[s]; p < S.seg_off[s] + S.seg_len[s]; ++p) {
+          const int* pc = &cells[S.zids[p] * l];
+          bool in = true;
+          for (int i = 0; i < l && in; ++i) {
+            if (!((S.valid_map >> i) & 1)) continue;
+            if (pc[i] < box[2 * i] || pc[i] > box[2 * i + 1]) in = false;
+          }
+          c += in;
+        }
+        counts[qi * S.CS + s] = c;
+      }
+    }
+    // ---- A4: non-zero clusters (filter.cu:242-260) ----
+    std::vector<std::vector<uint64_t>> valid_clusters(nb);
+    uint64_t max_counts = 0;
+    for (uint64_t qi = 0; qi < nb; ++qi) {
+      for (uint64_t c = 0; c < S.C; ++c) {
+        int sum = 0;  // get_cluster_sum_kernel accumulates in int (:781-789)
+        for (uint64_t i = 0; i < S.sub_cnt[c]; ++i) sum += (int)counts[qi * S.CS + S.sub_off[c] + i];
+        if ((uint64_t)(int64_t)sum != 0) valid_clusters[qi].push_back(c);
+      }
+      max_counts = std::max<uint64_t>(max_counts, valid_clusters[qi].size());
+      if (trace && trace->n_clusters_nz) trace->n_clusters_nz[b0 + qi] = (int64_t)valid_clusters[qi].size();
+    }
+    const uint64_t kc = std::min<uint64_t>(S.n_list, max_counts);
+    // ---- A5: exact distance to valid clusters, sorted top kc (refine.cu:78-119, is_sort=true) ----
+    std::vector<std::vector<uint64_t>> chosen_clusters(nb);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t qi = 0; qi < (int64_t)nb; ++qi) {
+      const float* q = queries + (b0 + qi) * S.d;
+      std::vector<Key> items;
+      const auto& vc = valid_clusters[qi];
+      for (uint64_t j = 0; j < S.C; ++j) {
+        float dist = j < vc.size() ? exact_l2(q, S.centers + vc[j] * S.d, S.d) : kExactPad;
+        items.push_back({dist, j});
+      }
+      topk_sorted(items, kc);
+      for (auto& it : items) chosen_clusters[qi].push_back(it.col < vc.size() ? vc[it.col] : kU64Max);
+    }
+    // ---- A6: select_sub_clusters (Super-Grid-IVF_PQ.cu:849-901) ----
+    std::vector<std::vector<uint64_t>> table(nb), valid_subs(nb);
+    uint64_t table_len = 0, max_sub = 0;
+    for (uint64_t qi = 0; qi < nb; ++qi) {
+      for (uint64_t c : chosen_clusters[qi]) {
+        if (c >= S.C) continue;  // pad cluster: start 2^32, length 0
+        for (uint64_t i = 0; i < S.sub_cnt[c]; ++i) table[qi].push_back(S.sub_off[c] + i);
+      }
+      table_len = std::max<uint64_t>(table_len, table[qi].size());
+      for (uint64_t t = 0; t < table[qi].size(); ++t) {
+        uint64_t sid = table[qi][t];
+        uint64_t cval = sid < S.CS ? counts[qi * S.CS + sid] : 0;
+        if (cval != 0) valid_subs[qi].push_back(sid);
+      }
+      max_sub = std::max<uint64_t>(max_sub, valid_subs[qi].size());
+      if (trace && trace->n_subs_nz) trace->n_subs_nz[b0 + qi] = (int64_t)valid_subs[qi].size();
+    }
+    const uint64_t ks = std::min<uint64_t>(S.sub_lists, max_sub);
+    std::vector<std::vector<uint64_t>> chosen_subs(nb);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t qi = 0; qi < (int64_t)nb; ++qi) {
+      const float* q = queries + (b0 + qi) * S.d;
+      std::vector<Key> items;
+      const auto& vs = valid_subs[qi];
+      for (uint64_t j = 0; j < table_len; ++j) {
+        float dist = (j < vs.size() && vs[j] < S.CS) ? exact_l2(q, S.sub_centers + vs[j] * S.d, S.d)
+                                                      : kExactPad;
+        items.push_back({dist, j});
+      }
+      topk_sorted(items, ks);
+      for (auto& it : items) chosen_subs[qi].push_back(it.col < vs.size() ? vs[it.col] : kU64Max);
+    }
+    // ---- A7/A8/A9: Morton ranges and candidate rows (zorder.cu:656-685, mat_operators.cuh:692-794) --
+    std::vector<std::vector<uint64_t>> cands(nb);
+    std::vector<uint64_t> real(nb, 0);
+    uint64_t W = 0;
+    for (uint64_t qi = 0; qi < nb; ++qi) {
+      const int* box = &boxes[(b0 + qi) * 2 * l];
+      int lo_c[kMaxDims], hi_c[kMaxDims];
+      for (int i = 0; i < l; ++i) {
+        lo_c[i] = box[2 * i];
+        hi_c[i] = box[2 * i + 1];
+      }
+      uint64_t zlo = orc_morton(lo_c, l, S.max_bits, S.valid_map);
+      uint64_t zhi = orc_morton(hi_c, l, S.max_bits, S.valid_map);
+      for (uint64_t s : chosen_subs[qi]) {
+        if (s >= S.CS) continue;  // start UINT64_MAX, count 0
+        const uint64_t* zc = S.zcodes + S.seg_off[s];
+        uint64_t n = S.seg_len[s];
+        uint64_t lower = (uint64_t)(std::lower_bound(zc, zc + n, zlo) - zc);
+        uint64_t upper = (uint64_t)(std::lower_bound(zc, zc + n, zhi + 1) - zc);
+        for (uint64_t p = lower; p < upper; ++p) cands[qi].push_back(S.zids[S.seg_off[s] + p]);
+        real[qi] += counts[qi * S.CS + s];
+      }
+      W = std::max<uint64_t>(W, cands[qi].size());
+      if (trace && trace->cand_count) trace->cand_count[b0 + qi] = (int64_t)cands[qi].size();
+      if (trace && trace->real_count) trace->real_count[b0 + qi] = (int64_t)real[qi];
+      if (trace && trace->chosen_subs) {
+        for (uint64_t j = 0; j < S.sub_lists; ++j)
+          trace->chosen_subs[(b0 + qi) * S.sub_lists + j] =
+              j < chosen_subs[qi].size() ? chosen_subs[qi][j] : kU64Max;
+      }
+      if (trace && trace->sub_counts)
+        for (uint64_t s = 0; s < S.CS; ++s) trace->sub_counts[(b0 + qi) * S.CS + s] = counts[qi * S.CS + s];
+    }
+    // ---- A10: beta and n_candies (Super-Grid-IVF_PQ.cu:1198-1222), l > 1 only ----
+    const uint64_t refine_ratio = S.exps0;
+    uint64_t n_candies = 0;
+    if (refine_ratio > 1 && l > 1) {
+      uint64_t real_c = 0, coarse_c = 0;
+      for (uint64_t qi = 0; qi < nb; ++qi) {
+        real_c += real[qi];
+        coarse_c += cands[qi].size();
+      }
+      float beta = (float)coarse_c / (float)real_c;
+      n_candies = std::min<uint64_t>(f2u_host((float)(refine_ratio * k) * beta), W);
+    }
+    // ---- A11..A13 per query ----
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t qi = 0; qi < (int64_t)nb; ++qi) {
+      const uint64_t qg = b0 + qi;
+      const float* q = queries + qg * S.d;
+      const float* rg = &ranges[qg * 2 * l];
+      const auto& cand = cands[qi];
+      const uint64_t Cq = cand.size();
+      std::vector<float> lut(S.M * S.K);
+      orc_pq_lut(S.pq_centers, q, S.d, S.M, S.K, S.pq_len, lut.data());
+      auto pq_of = [&](uint64_t col) -> float {
+        if (col >= Cq) return FLT_MAX;  // candidate pad 2^32 >= N
+        return orc_adc(S.codes, S.N, cand[col], lut.data(), S.M, S.K);
+      };
+      auto passes = [&](uint64_t id) -> bool {  // filter.cu:98-132
+        if (id >= S.N) return false;
+        for (int i = 0; i < l; ++i) {
+          float lab = S.labels[id * l + i];
+          if (lab < rg[2 * i] || lab > rg[2 * i + 1]) return false;
+        }
+        return true;
+      };
+      float* od = out_dist + qg * k;
+      uint64_t* oi = out_ids + qg * k;
+      if (refine_ratio <= 1) {
+        // query_prefilter: exact label filter over the W candidate columns, then PQ top-k
+        std::vector<Key> items;
+        std::vector<uint64_t> surv;
+        for (uint64_t c = 0; c < Cq; ++c)
+          if (passes(cand[c])) surv.push_back(cand[c]);
+        for (uint64_t j = 0; j < surv.size(); ++j)
+          items.push_back({orc_adc(S.codes, S.N, surv[j], lut.data(), S.M, S.K), j});
+        topk_sorted(items, k);
+        for (uint64_t j = 0; j < k; ++j) {
+          if (j < items.size()) {
+            od[j] = items[j].v;
+            oi[j] = surv[items[j].col];
+          } else {
+            od[j] = FLT_MAX;
+            oi[j] = kU64Max;
+          }
+        }
+        if (trace && trace->n_survivors) trace->n_survivors[qg] = (int64_t)surv.size();
+        if (trace && trace->k_prime) trace->k_prime[qg] = (int64_t)k;
+        continue;
+      }
+      // refine input list (ids; pads 2^32 or UINT64_MAX)
+      std::vector<uint64_t> L;
+      uint64_t kp;
+      if (l == 1) {
+        kp = refine_ratio * k;
+        if (W >= kp) {
+          std::vector<Key> items;
+          for (uint64_t c = 0; c < W; ++c) items.push_back({pq_of(c), c});
+          topk_sorted(items, kp);
+          for (auto& it : items) L.push_back(it.col < Cq ? cand[it.col] : kIdPad);
+        } else {
+          // stride mismatch (product-quantization.cu:254-264): every column passes through
+          for (uint64_t c = 0; c < kp; ++c) L.push_back(c < Cq ? cand[c] : (c < W ? kIdPad : kU64Max));
+        }
+      } else {
+        kp = n_candies;
+        std::vector<Key> items;
+        for (uint64_t c = 0; c < W; ++c) items.push_back({pq_of(c), c});
+        topk_sorted(items, kp);
+        // prefilter (prefiltering.cu:14-34): order-preserving compaction, UINT64_MAX pad
+        for (auto& it : items) {
+          uint64_t id = it.col < Cq ? cand[it.col] : kIdPad;
+          if (passes(id)) L.push_back(id);
+        }
+        while (L.size() < kp) L.push_back(kU64Max);
+      }
+      if (trace && trace->k_prime) trace->k_prime[qg] = (int64_t)kp;
+      // ---- A13: exact re-rank (refine.cu:78-119), tie-break (dist, id) ----
+      struct R {
+        float v;
+        uint64_t id;
+      };
+      std::vector<R> rr;
+      int64_t nsurv = 0;
+      for (uint64_t id : L) {
+        float dist = id < S.N ? exact_l2(q, S.data + id * S.d, S.d) : kExactPad;
+        nsurv += id < S.N;
+        rr.push_back({dist, id});
+      }
+      if (trace && trace->n_survivors) trace->n_survivors[qg] = nsurv;
+      uint64_t kk = std::min<uint64_t>(k, rr.size());
+      std::partial_sort(rr.begin(), rr.begin() + kk, rr.end(), [](const R& a, const R& b) {
+        if (a.v < b.v) return true;
+        if (b.v < a.v) return false;
+        return a.id < b.id;
+      });
+      for (uint64_t j = 0; j < k; ++j) {
+        if (j < kk) {
+          od[j] = rr[j].v;
+          oi[j] = rr[j].id;
+        } else {
+          od[j] = kExactPad;
+          oi[j] = kU64Max;
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+// =============================================================================================
+// FP64 brute-force filtered ground truth.  Restates ground_truth.cuh:166-233 with the filter.conf
+// predicate (filter.cu:98-132 on the A1 ranges) instead of its hard-coded 3-day window, FP64
+// accumulation, and ascending (distance, id) order.  Missing neighbours: id UINT64_MAX, dist +inf.
+// Pinned by: direct FP64 closed form; tests cross-check against numpy on small inputs.
+int orc_bruteforce(const float* data, const float* labels, uint64_t N, uint64_t d, int l,
+                   const float* queries, const float* ranges, uint64_t nq, uint64_t k, int threads,
+                   double* out_dist, uint64_t* out_ids) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t qi = 0; qi < (int64_t)nq; ++qi) {
+    const float* q = queries + qi * d;
+    const float* rg = ranges + qi * 2 * l;
+    std::priority_queue<std::pair<double, uint64_t>> heap;
+    for (uint64_t p = 0; p < N; ++p) {
+      bool ok = true;
+      for (int i = 0; i < l && ok; ++i) {
+        float lab = labels[p * l + i];
+        if (lab < rg[2 * i] || lab > rg[2 * i + 1]) ok = false;
+      }
+      if (!ok) continue;
+      double s = 0.0;
+      const float* x = data + p * d;
+      for (uint64_t t = 0; t < d; ++t) {
+        double df = (double)x[t] - (double)q[t];
+        s += df * df;
+      }
+      if (heap.size() < k) {
+        heap.push({s, p});
+      } else if (std::make_pair(s, p) < heap.top()) {
+        heap.pop();
+        heap.push({s, p});
+      }
+    }
+    uint64_t n = heap.size();
+    for (int64_t j = (int64_t)k - 1; j >= 0; --j) {
+      if ((uint64_t)j < n) {
+        out_dist[qi * k + j] = heap.top().first;
+        out_ids[qi * k + j] = heap.top().second;
+        heap.pop();
+      } else {
+        out_dist[qi * k + j] = INFINITY;
+        out_ids[qi * k + j] = kU64Max;
+      }
+    }
+  }
+  return 0;
+}
+
+// Largest factor of b that is < a, or b when b <= a: math_utils.h:40-60 (findMaxFactor).
+int orc_find_max_factor(uint64_t a, uint64_t b) {
+  if (b <= a) return (int)b;
+  int mf = 1;
+  for (int i = 1; i <= std::sqrt((double)b); ++i) {
+    if (b % (uint64_t)i == 0) {
+      if ((uint64_t)i < a) mf = std::max(mf, i);
+      int pf = (int)(b / (uint64_t)i);
+      if ((uint64_t)pf < a) mf = std::max(mf, pf);
+    }
+  }
+  return mf;
+}
+
+int orc_omp_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"
