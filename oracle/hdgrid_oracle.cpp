@@ -545,6 +545,25 @@ float orc_adc(const uint8_t* codes, uint64_t n_total, uint64_t id, const float* 
   return sum;
 }
 
+// Exact label predicate over candidate labels: label_matrix_by_filter_kernel (filter.cu:72-95) /
+// label_matrix_by_filter_indices_kernel (:98-132): pass iff lo <= label <= hi on every dim (f32).
+// ranges [nq][2l]; labels [nq][ncand][l]; mask [nq][ncand].
+void orc_filter_mask(const float* ranges, const float* labels, uint64_t nq, uint64_t ncand, int l, int* mask) {
+  for (uint64_t q = 0; q < nq; ++q)
+    for (uint64_t c = 0; c < ncand; ++c) {
+      int ans = 1;
+      for (int i = 0; i < l; ++i) {
+        float lo = ranges[q * 2 * l + 2 * i], hi = ranges[q * 2 * l + 2 * i + 1];
+        float v = labels[(q * ncand + c) * l + i];
+        if (v < lo || v > hi) {
+          ans = 0;
+          break;
+        }
+      }
+      mask[q * ncand + c] = ans;
+    }
+}
+
 // Exact L2 exported for tests.
 float orc_exact_l2(const float* q, const float* x, uint64_t d) { return exact_l2(q, x, d); }
 

@@ -151,6 +151,16 @@ def adc(codes_col_major, n_total, ids, lut):
                     dtype=np.float32)
 
 
+def filter_mask(ranges, cand_labels):
+    """ranges [nq][2l], cand_labels [nq][ncand][l] -> int mask [nq][ncand]."""
+    ranges = _c(ranges, np.float32)
+    cand_labels = _c(cand_labels, np.float32)
+    nq, ncand, l = cand_labels.shape
+    out = np.zeros((nq, ncand), np.int32)
+    _lib.orc_filter_mask(_p(ranges), _p(cand_labels), _u64(nq), _u64(ncand), ctypes.c_int(l), _p(out))
+    return out
+
+
 def exact_l2(q, x):
     q, x = _c(q, np.float32), _c(x, np.float32)
     _lib.orc_exact_l2.restype = ctypes.c_float
