@@ -72,7 +72,8 @@ __global__ void prep_kernel(const float* __restrict__ qlabels, uint64_t qcols, u
                             uint64_t imap_len, const int* __restrict__ divv, float* __restrict__ ranges,
                             int* __restrict__ boxes, uint32_t* __restrict__ box_lo,
                             uint32_t* __restrict__ box_w, uint64_t* __restrict__ zlo,
-                            uint64_t* __restrict__ zhi, uint32_t* __restrict__ err) {
+                            uint64_t* __restrict__ zhi, uint64_t* __restrict__ zlo_cnt,
+                            uint32_t* __restrict__ err) {
   uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (qi >= qn) return;
   const uint64_t qg = q0 + qi;
@@ -141,21 +142,34 @@ __global__ void prep_kernel(const float* __restrict__ qlabels, uint64_t qcols, u
   }
   zlo[qi] = morton_code(lo_c, l, D.max_bits, D.valid_map);
   zhi[qi] = morton_code(hi_c, l, D.max_bits, D.valid_map);
+  // Morton range used by the count: a lower corner that wrapped negative (a truncated u64 cell id)
+  // still admits every cell >= 0, so its code is taken at 0.
+  for (int i = 0; i < l; ++i) lo_c[i] = lo_c[i] < 0 ? 0 : lo_c[i];
+  zlo_cnt[qi] = morton_code(lo_c, l, D.max_bits, D.valid_map);
 }
 
 // ------------------------------------------------------------------------------------------------
 // A3: counts[q][s] = #points of sub-cluster s whose cells lie in q's box on every valid dim.  This is
 // the integer the reference obtains by 2^l-corner inclusion-exclusion over compressed prefix sums
-// (Super-Grid-IVF_PQ.cu:648-790); here it is counted directly by streaming the packed cells.
-// Block = 256 lanes = 256 queries; blockIdx.y = chunk of kCountSegChunk segments.  Cells are read
-// with wave-uniform (scalar) loads; each lane tests its own query box.
+// (Super-Grid-IVF_PQ.cu:648-790).  Here it is counted directly: a point inside the box has a Morton
+// code in [z(lo), z(hi)] (the bit-interleave is monotone in every coordinate), and a sub-cluster's
+// points are sorted by Morton code, so only that code range of each segment is box-tested.
+// Block = 256 lanes = 256 queries; blockIdx.y = chunk of kCountSegChunk segments.  Each segment is
+// staged in LDS in pieces of kCountPiece points (codes + cells); every lane binary-searches its own
+// code range in LDS and tests the points inside it.
+constexpr int kCountPiece = 1024;
+
 template <int NV>
 __global__ __launch_bounds__(256) void count_kernel(const uint32_t* __restrict__ cells, uint64_t N,
+                                                    const uint64_t* __restrict__ zcode,
                                                     const uint32_t* __restrict__ seg_off, uint64_t CS,
                                                     const uint32_t* __restrict__ box_lo,
-                                                    const uint32_t* __restrict__ box_w, uint64_t qn,
+                                                    const uint32_t* __restrict__ box_w,
+                                                    const uint64_t* __restrict__ zlo,
+                                                    const uint64_t* __restrict__ zhi, uint64_t qn,
                                                     uint32_t* __restrict__ counts) {
-  __shared__ uint32_t tile[256][kCountSegTile + 1];
+  __shared__ uint64_t zs[kCountPiece];
+  __shared__ uint32_t cs[NV][kCountPiece];
   const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   const bool live = q < qn;
   uint32_t lo[NV], w[NV];
@@ -164,25 +178,40 @@ __global__ __launch_bounds__(256) void count_kernel(const uint32_t* __restrict__
     lo[j] = live ? box_lo[(uint64_t)j * qn + q] : 1u;
     w[j] = live ? box_w[(uint64_t)j * qn + q] : 0u;
   }
+  const uint64_t za = live ? zlo[q] : ~0ull, zb = live ? zhi[q] : 0ull;
   const uint64_t s_begin = (uint64_t)blockIdx.y * kCountSegChunk;
   const uint64_t s_end = s_begin + kCountSegChunk < CS ? s_begin + kCountSegChunk : CS;
-  for (uint64_t sb = s_begin; sb < s_end; sb += kCountSegTile) {
-    const uint64_t se = sb + kCountSegTile < s_end ? sb + kCountSegTile : s_end;
-    for (uint64_t s = sb; s < se; ++s) {
-      const uint32_t p0 = seg_off[s], p1 = seg_off[s + 1];
-      uint32_t cnt = 0;
-      for (uint32_t p = p0; p < p1; ++p) {
+  for (uint64_t s = s_begin; s < s_end; ++s) {
+    const uint32_t p0 = seg_off[s], p1 = seg_off[s + 1];
+    uint32_t cnt = 0;
+    for (uint32_t base = p0; base < p1; base += kCountPiece) {
+      const uint32_t m = p1 - base < (uint32_t)kCountPiece ? p1 - base : (uint32_t)kCountPiece;
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < m; i += 256) {
+        zs[i] = zcode[base + i];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) cs[j][i] = cells[(uint64_t)j * N + base + i];
+      }
+      __syncthreads();
+      if (za > zs[m - 1] || zb < zs[0]) continue;
+      uint32_t a = 0, b = m;  // first code >= za
+      while (a < b) {
+        uint32_t mid = (a + b) >> 1;
+        if (zs[mid] < za) a = mid + 1; else b = mid;
+      }
+      uint32_t c = a, e = m;  // first code > zb
+      while (c < e) {
+        uint32_t mid = (c + e) >> 1;
+        if (zs[mid] <= zb) c = mid + 1; else e = mid;
+      }
+      for (uint32_t i = a; i < c; ++i) {
         bool in = true;
 #pragma unroll
-        for (int j = 0; j < NV; ++j) in = in && (cells[(uint64_t)j * N + p] - lo[j] <= w[j]);
+        for (int j = 0; j < NV; ++j) in = in && (cs[j][i] - lo[j] <= w[j]);
         cnt += in ? 1u : 0u;
       }
-      tile[threadIdx.x][s - sb] = cnt;
     }
-    if (live) {
-      uint32_t* row = counts + q * CS + sb;
-      for (uint64_t s = sb; s < se; ++s) row[s - sb] = tile[threadIdx.x][s - sb];
-    }
+    if (live) counts[q * CS + s] = cnt;
   }
 }
 
@@ -196,10 +225,27 @@ __device__ float exact_tile(const float* __restrict__ X, uint64_t d, const uint3
   float acc = 0.0f;
   for (uint64_t t0 = 0; t0 < d; t0 += 32) {
     const uint32_t w = (uint32_t)(d - t0 < 32 ? d - t0 : 32);
-    for (uint32_t e = threadIdx.x; e < 256u * 32u; e += kWG) {
-      const uint32_t r = e >> 5, c = e & 31;
-      const uint32_t row = rows[r];
-      stage[r * 33 + c] = (row != kNone && c < w) ? X[(uint64_t)row * d + t0 + c] : 0.0f;
+    if ((d & 3) == 0) {
+      // 8 lanes per row: each wave load instruction reads 8 whole 128-byte row chunks
+#pragma unroll
+      for (int it = 0; it < 8; ++it) {
+        const uint32_t e = threadIdx.x + it * kWG;
+        const uint32_t r = e >> 3, c4 = (e & 7) * 4;
+        const uint32_t row = rows[r];
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (row != kNone && c4 < w) v = *reinterpret_cast<const float4*>(X + (uint64_t)row * d + t0 + c4);
+        float* st = stage + r * 33 + c4;
+        st[0] = v.x;
+        st[1] = v.y;
+        st[2] = v.z;
+        st[3] = v.w;
+      }
+    } else {
+      for (uint32_t e = threadIdx.x; e < 256u * 32u; e += kWG) {
+        const uint32_t r = e >> 5, c = e & 31;
+        const uint32_t row = rows[r];
+        stage[r * 33 + c] = (row != kNone && c < w) ? X[(uint64_t)row * d + t0 + c] : 0.0f;
+      }
     }
     __syncthreads();
     if (my != kNone) {
@@ -850,7 +896,7 @@ __global__ __launch_bounds__(256) void final_kernel(
 // ================================================================================================
 // Host orchestration
 struct Index::Work {
-  DevBuf ranges, boxes, box_lo, box_w, zlo, zhi, err;
+  DevBuf ranges, boxes, box_lo, box_w, zlo, zhi, zlo_cnt, err;
   DevBuf counts, list_c, nnz_c, chosen_c, n_chosen_c, list_s, nnz_s, chosen_s, n_chosen_s;
   DevBuf keys, gstat, rstart, rpre, cand, real;
   DevBuf mode, kprime, lsize, asize, chunks, pad32, padmax, nL;
@@ -954,6 +1000,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.box_w.reserve((uint64_t)std::max(1, g.n_valid) * qn * 4);
   w.zlo.reserve(qn * 8);
   w.zhi.reserve(qn * 8);
+  w.zlo_cnt.reserve(qn * 8);
   w.err.reserve(4);
   w.counts.reserve(qn * CS * 4);
   w.list_c.reserve(qn * C * 4);
@@ -995,7 +1042,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   hipLaunchKernelGGL(prep_kernel, dim3((unsigned)cdiv(qn, 128)), dim3(128), 0, s, qlabels, qcols, q0, qn, G, D,
                      f_type_.as<int>(), f_shift_.as<float>(), f_map_.as<float>(), f_map_len_, f_div_.as<int>(),
                      w.ranges.as<float>(), w.boxes.as<int>(), w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(),
-                     w.zlo.as<uint64_t>(), w.zhi.as<uint64_t>(), w.err.as<uint32_t>());
+                     w.zlo.as<uint64_t>(), w.zhi.as<uint64_t>(), w.zlo_cnt.as<uint64_t>(), w.err.as<uint32_t>());
   HDG_HIP(hipGetLastError());
   mark();
   {
@@ -1005,8 +1052,9 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     switch (g.n_valid) {
 #define HDG_COUNT_CASE(NV)                                                                              \
   case NV:                                                                                              \
-    hipLaunchKernelGGL(count_kernel<NV>, grid, dim3(256), 0, s, cells, S.N, so, CS, w.box_lo.as<uint32_t>(), \
-                       w.box_w.as<uint32_t>(), qn, w.counts.as<uint32_t>());                           \
+    hipLaunchKernelGGL(count_kernel<NV>, grid, dim3(256), 0, s, cells, S.N, S.zcode.as<uint64_t>(), so, CS,     \
+                       w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), w.zlo_cnt.as<uint64_t>(),             \
+                       w.zhi.as<uint64_t>(), qn, w.counts.as<uint32_t>());                                    \
     break;
       HDG_COUNT_CASE(1)
       HDG_COUNT_CASE(2)
@@ -1027,6 +1075,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      S.sub_off.as<uint32_t>(), (uint32_t)C, G, q0, w.list_c.as<uint32_t>(), w.nnz_c.as<uint32_t>(),
                      w.gstat.as<unsigned long long>());
   HDG_HIP(hipGetLastError());
+  mark();
   hipLaunchKernelGGL(coarse_select_kernel, dim3((unsigned)qn), dim3(256), sort_lds, s, queries, d,
                      S.centers.as<float>(), (uint32_t)C, n_list, G, q0, w.list_c.as<uint32_t>(),
                      w.nnz_c.as<uint32_t>(), w.gstat.as<unsigned long long>(), w.keys.as<uint64_t>(),
@@ -1037,6 +1086,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      S.sub_off.as<uint32_t>(), n_list, G, q0, w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>(),
                      cap, w.list_s.as<uint32_t>(), w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>());
   HDG_HIP(hipGetLastError());
+  mark();
   hipLaunchKernelGGL(sub_select_kernel, dim3((unsigned)qn), dim3(256), sort_lds, s, queries, d,
                      S.sub_centers.as<float>(), sub_lists, G, q0, cap, w.list_s.as<uint32_t>(),
                      w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>(), w.keys.as<uint64_t>(),
@@ -1049,6 +1099,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      w.zhi.as<uint64_t>(), w.rstart.as<uint32_t>(), w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(),
                      w.real.as<uint64_t>(), w.gstat.as<unsigned long long>());
   HDG_HIP(hipGetLastError());
+  mark();
   hipLaunchKernelGGL(plan_kernel, dim3((unsigned)cdiv(qn, 128)), dim3(128), 0, s, qn, q0, G, l, cfg_.exps0,
                      topk, w.cand.as<uint64_t>(), w.gstat.as<unsigned long long>(), w.mode.as<uint32_t>(),
                      w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(),
@@ -1074,6 +1125,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(), w.rpre.as<uint32_t>(), S.lab.as<float>(),
                      w.ranges.as<float>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>());
   HDG_HIP(hipGetLastError());
+  mark();
   if (totChunks > 0) {
     hipLaunchKernelGGL(adc_scan_kernel, dim3((unsigned)totChunks), dim3(256), kLutPhaseBytes, s, qn, q0, queries,
                        d, S.M, S.K, S.pq_len, S.pq_centers.as<float>(), S.codes.as<uint8_t>(), sub_lists,
@@ -1118,8 +1170,11 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
       stats_.adc[q0 + i] = mode[i] == 1 ? cand[i] : 0;
       stats_.scored[q0 + i] = (uint64_t)nc[i] + ns[i];
     }
-    static const char* names[] = {"prep", "count", "coarse", "sub", "ranges+plan+sync", "fill+adc", "final"};
-    for (int i = 0; i + 1 < evn && i < 7; ++i) {
+    static const char* names[] = {"prep_kernel",   "count_kernel",      "cluster_nnz_kernel", "coarse_select_kernel",
+                                  "sub_nnz_kernel", "sub_select_kernel", "ranges_kernel",      "plan+sync",
+                                  "keep_all_kernel", "adc_kernels",      "final_kernel"};
+    const int nnames = (int)(sizeof(names) / sizeof(names[0]));
+    for (int i = 0; i + 1 < evn && i < nnames; ++i) {
       float ms = 0;
       HDG_HIP(hipEventElapsedTime(&ms, w.ev[i], w.ev[i + 1]));
       stage_ms.push_back({names[i], ms});
