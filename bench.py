@@ -114,7 +114,7 @@ def main():
     # algorithmic bytes per launch of each kernel (one launch covers all nq queries)
     n_valid = 3
     alg = {
-        "count_kernel": float(n_valid * 4 * args.n),  # every packed cell read once per launch
+        "count_kernel": float((8 + n_valid * 4) * args.n),  # Morton code + cells of every point, once
         "final_kernel": float(st.refined.sum() * 4 * d_ + args.nq * 100 * 12),
         "keep_all_kernel": float(st.cand.sum() * 4 * l_),
         "adc_kernels": float(st.adc.sum() * M),
@@ -129,7 +129,7 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 6) if achieved else None, "traffic": None,
                 "avg_ms": round(dom_ms, 4), "alg_bytes_per_launch": dom_bytes}
     pipe_bytes = (st.adc.sum() * M + st.refined.sum() * 4 * d_ + st.cand.sum() * 4 * l_ + st.scored.sum() * 4 * d_
-                  + args.nq * (4 * d_ + 12 * 100) + n_valid * 4 * args.n)
+                  + args.nq * (4 * d_ + 12 * 100) + (8 + n_valid * 4) * args.n)
     log(f"dominant kernel {dom}: {dom_ms:.3f} ms/launch; pipeline algorithmic bytes/step {pipe_bytes / 1e9:.3f} GB "
         f"-> {pipe_bytes / (ms_per_step * 1e-3) / 1e9:.1f} GB/s")
     log(f"per-query means: C_q={st.cand.mean():.1f} real={st.real.mean():.1f} refined={st.refined.mean():.1f} "
