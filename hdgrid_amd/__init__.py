@@ -56,6 +56,34 @@ _split = _sig("hdg_split_task", ctypes.c_int, [ctypes.POINTER(ctypes.c_double), 
                                               _u64p, _u64p])
 _find_max_factor = _sig("hdg_find_max_factor", ctypes.c_int, [_u64, _u64])
 _validate = _sig("hdg_validate_config_text", ctypes.c_int, [_cp, _cp, _cp, _u64])
+_fp = ctypes.POINTER(ctypes.c_float)
+_budget_fit = _sig("hdg_budget_fit", ctypes.c_int,
+                   [_vp, _vp, _u64, _u64, _vp, _u64, _u64p, _u64, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                    ctypes.c_double, ctypes.c_float, ctypes.c_int, ctypes.c_float, _u64, _fp, _vp])
+_budget_save = _sig("hdg_budget_save", ctypes.c_int, [_vp, _cp])
+_budget_load = _sig("hdg_budget_load", ctypes.c_int, [_vp, _cp, _vp])
+_budget_grads = _sig("hdg_budget_debug_grads", ctypes.c_int, [_fp, _fp, _fp, ctypes.c_uint32, _fp, _fp])
+_budget_nparams = _sig("hdg_budget_param_count", ctypes.c_int, [])
+_coarse_scores = _sig("hdg_debug_coarse_scores", ctypes.c_int, [_vp, _vp, _u64, _fp, _vp])
+
+BUDGET_FEATURES = 48
+
+
+def budget_param_count() -> int:
+    return int(_budget_nparams())
+
+
+def budget_debug_grads(params, X, Y):
+    """One forward/backward of the HIP training kernel on host arrays -> (grads [P], loss)."""
+    import numpy as np
+    params = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.ascontiguousarray(Y, np.float32)
+    g = np.zeros(budget_param_count(), np.float32)
+    loss = np.zeros(1, np.float32)
+    _check(_budget_grads(params.ctypes.data_as(_fp), X.ctypes.data_as(_fp), Y.ctypes.data_as(_fp), X.shape[0],
+                         g.ctypes.data_as(_fp), loss.ctypes.data_as(_fp)))
+    return g, float(loss[0])
 
 
 class HDGridError(RuntimeError):
@@ -198,6 +226,36 @@ class SuperGridIndex:
         sl = self.get_param("SUB_LISTS")
         out = np.zeros((nq, sl), dtype=np.uint32)
         _check(_chosen(self._h, nq, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return out
+
+    # -- learned budget (PRECISION 1, BUDGET_MODEL 1) -----------------------------------------
+    def budget_fit(self, queries, query_labels, gt_ids, levels=10, t0=4.0, s0=8.0, ratio=2 ** 0.5, target=0.95,
+                   epochs=30, lr=2e-3, seed=0x4844, stream=None):
+        """Label training queries with their minimal budget level and train the MLP; returns the report."""
+        import numpy as np
+        queries = queries.contiguous()
+        query_labels = query_labels.contiguous()
+        gt = np.ascontiguousarray(gt_ids, dtype=np.uint64)
+        rep = (ctypes.c_float * 8)()
+        qcols = query_labels.shape[1] if query_labels.dim() == 2 else 1
+        _check(_budget_fit(self._h, _ptr(queries), queries.shape[0], queries.shape[1], _ptr(query_labels), qcols,
+                           gt.ctypes.data_as(_u64p), gt.shape[1], levels, t0, s0, ratio, target, epochs, lr, seed,
+                           rep, _stream_ptr(stream)))
+        keys = ["final_loss", "margin", "holdout_recall", "holdout_level", "mean_label", "fixed_recall", "n_train",
+                "n_holdout"]
+        return {k: float(rep[i]) for i, k in enumerate(keys)}
+
+    def budget_save(self, path: str) -> None:
+        _check(_budget_save(self._h, path.encode()))
+
+    def budget_load(self, path: str, stream=None) -> None:
+        _check(_budget_load(self._h, path.encode(), _stream_ptr(stream)))
+
+    def debug_coarse_scores(self, queries, stream=None):
+        import numpy as np
+        queries = queries.contiguous()
+        out = np.zeros((queries.shape[0], self.get_param("CLUSTERS")), np.float32)
+        _check(_coarse_scores(self._h, _ptr(queries), queries.shape[0], out.ctypes.data_as(_fp), _stream_ptr(stream)))
         return out
 
     def last_stage_ms(self):

@@ -1,0 +1,541 @@
+// Learned per-query probe budget for PRECISION 1 (BUDGET_MODEL = 1).
+//
+// Budget ladder: level i probes T[i] clusters and keeps S[i] sub-clusters (both grow geometrically).
+// A query's label is the smallest level whose results reach the target recall@k against the ground
+// truth supplied by the caller (the traces come from running this pipeline at every level).
+// Model: 48 features -> 64 ReLU -> 64 ReLU -> 1 (predicted level).  Features: log1p of the filter width of
+// each label dim (8), log1p of the grid-cell extent of each valid dim (8), the 32 smallest coarse
+// distances from the BF16 coarse scores (32); standardised with training mean / std.
+// Training: BF16 operands (inputs, weights and activations rounded to bf16), FP32 accumulation, FP32
+// master weights, Adam, MSE loss.  Inference adds a margin calibrated on a held-out split; a non-finite
+// prediction falls back to the fixed budget (the highest level).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+#include <random>
+
+#include "hdg_budget.hpp"
+#include "hdg_fast.hpp"
+#include "hdg_index.hpp"
+
+namespace hdg {
+
+namespace {
+
+constexpr int kF = kBudgetFeatures;  // 48
+constexpr int kH = 64;
+constexpr int kP1 = kF * kH, kP2 = kH * kH;
+constexpr int kOffW1 = 0, kOffB1 = kP1, kOffW2 = kP1 + kH, kOffB2 = kP1 + kH + kP2, kOffW3 = kOffB2 + kH,
+              kOffB3 = kOffW3 + kH;
+constexpr int kParams = kOffB3 + 1;  // 7361
+
+__device__ __forceinline__ float bf16r(float v) {
+  uint32_t u = __float_as_uint(v);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return __uint_as_float(u & 0xFFFF0000u);
+}
+
+// Features of one query (unstandardised).
+__device__ void query_features(uint64_t qi, int l, const float* ranges, const uint32_t* box_w, int n_valid,
+                               uint64_t qn, const float* feat_dist, float* f) {
+  for (int i = 0; i < 8; ++i) {
+    float v = 0.0f;
+    if (i < l) {
+      float w = ranges[qi * 2 * l + 2 * i + 1] - ranges[qi * 2 * l + 2 * i];
+      w = w > 0.0f ? (w < 1e12f ? w : 1e12f) : 0.0f;
+      v = log1pf(w);
+    }
+    f[i] = v;
+  }
+  for (int j = 0; j < 8; ++j) f[8 + j] = j < n_valid ? log1pf((float)box_w[(uint64_t)j * qn + qi] + 1.0f) : 0.0f;
+  for (int j = 0; j < 32; ++j) f[16 + j] = feat_dist[qi * kFeatDists + j];
+}
+
+__global__ void features_kernel(uint64_t qn, int l, const float* __restrict__ ranges,
+                                const uint32_t* __restrict__ box_w, int n_valid, const float* __restrict__ feat_dist,
+                                float* __restrict__ out) {
+  const uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi >= qn) return;
+  float f[kF];
+  query_features(qi, l, ranges, box_w, n_valid, qn, feat_dist, f);
+  for (int i = 0; i < kF; ++i) out[qi * kF + i] = f[i];
+}
+
+// Forward of one sample with bf16-rounded operands; stores pre-activations when z1/z2 are given.
+__device__ float mlp_forward(const float* w, const float* x, float* z1, float* z2) {
+  float h1[kH], h2[kH];
+  for (int j = 0; j < kH; ++j) {
+    float a = w[kOffB1 + j];
+    for (int i = 0; i < kF; ++i) a += bf16r(x[i]) * w[kOffW1 + i * kH + j];
+    if (z1) z1[j] = a;
+    h1[j] = bf16r(a > 0.0f ? a : 0.0f);
+  }
+  for (int j = 0; j < kH; ++j) {
+    float a = w[kOffB2 + j];
+    for (int i = 0; i < kH; ++i) a += h1[i] * w[kOffW2 + i * kH + j];
+    if (z2) z2[j] = a;
+    h2[j] = bf16r(a > 0.0f ? a : 0.0f);
+  }
+  float y = w[kOffB3];
+  for (int i = 0; i < kH; ++i) y += h2[i] * w[kOffW3 + i];
+  return y;
+}
+
+// Inference: features -> standardise -> MLP -> level -> (clusters, sub-clusters); trims n_chosen_c.
+__global__ __launch_bounds__(256) void infer_kernel(uint64_t qn, int l, const float* __restrict__ ranges,
+                                                    const uint32_t* __restrict__ box_w, int n_valid,
+                                                    const float* __restrict__ feat_dist,
+                                                    const float* __restrict__ wq, const float* __restrict__ mean,
+                                                    const float* __restrict__ inv_std, float margin, int levels,
+                                                    const uint32_t* __restrict__ ladT,
+                                                    const uint32_t* __restrict__ ladS,
+                                                    uint32_t* __restrict__ n_chosen_c, uint32_t* __restrict__ bT,
+                                                    uint32_t* __restrict__ bS) {
+  __shared__ float w[kParams];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = wq[i];
+  __syncthreads();
+  const uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi >= qn) return;
+  float f[kF];
+  query_features(qi, l, ranges, box_w, n_valid, qn, feat_dist, f);
+  for (int i = 0; i < kF; ++i) f[i] = (f[i] - mean[i]) * inv_std[i];
+  const float y = mlp_forward(w, f, nullptr, nullptr);
+  int lev = levels - 1;  // fallback: the fixed budget
+  if (isfinite(y)) {
+    float t = ceilf(y + margin);
+    lev = t < 0.0f ? 0 : (t > (float)(levels - 1) ? levels - 1 : (int)t);
+  }
+  const uint32_t T = ladT[lev];
+  bT[qi] = T;
+  bS[qi] = ladS[lev];
+  if (n_chosen_c[qi] > T) n_chosen_c[qi] = T;
+}
+
+// Training forward + backward over a mini-batch; gradients accumulated into g (f32, atomics per block).
+__global__ __launch_bounds__(128) void train_step_kernel(const float* __restrict__ X, const float* __restrict__ Y,
+                                                         const uint32_t* __restrict__ idx, uint32_t bsz,
+                                                         const float* __restrict__ wq, float* __restrict__ g,
+                                                         float* __restrict__ loss) {
+  __shared__ float w[kParams];
+  __shared__ float gs[kParams];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) {
+    w[i] = wq[i];
+    gs[i] = 0.0f;
+  }
+  __syncthreads();
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < bsz) {
+    const uint32_t s = idx[b];
+    float x[kF];
+    for (int i = 0; i < kF; ++i) x[i] = bf16r(X[(uint64_t)s * kF + i]);
+    float z1[kH], z2[kH];
+    const float y = mlp_forward(w, x, z1, z2);
+    const float err = y - Y[s];
+    atomicAdd(loss, err * err / (float)bsz);
+    const float gy = 2.0f * err / (float)bsz;
+    float h1[kH], h2[kH], g2[kH];
+    for (int j = 0; j < kH; ++j) {
+      h1[j] = bf16r(z1[j] > 0.0f ? z1[j] : 0.0f);
+      h2[j] = bf16r(z2[j] > 0.0f ? z2[j] : 0.0f);
+    }
+    atomicAdd(&gs[kOffB3], gy);
+    for (int j = 0; j < kH; ++j) {
+      atomicAdd(&gs[kOffW3 + j], gy * h2[j]);
+      g2[j] = z2[j] > 0.0f ? bf16r(gy * w[kOffW3 + j]) : 0.0f;
+    }
+    for (int i = 0; i < kH; ++i) {
+      float back = 0.0f;
+      for (int j = 0; j < kH; ++j) {
+        atomicAdd(&gs[kOffW2 + i * kH + j], h1[i] * g2[j]);
+        back += w[kOffW2 + i * kH + j] * g2[j];
+      }
+      const float g1 = z1[i] > 0.0f ? bf16r(back) : 0.0f;
+      atomicAdd(&gs[kOffB1 + i], g1);
+      for (int f = 0; f < kF; ++f) atomicAdd(&gs[kOffW1 + f * kH + i], x[f] * g1);
+    }
+    for (int j = 0; j < kH; ++j) atomicAdd(&gs[kOffB2 + j], g2[j]);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x)
+    if (gs[i] != 0.0f) atomicAdd(&g[i], gs[i]);
+}
+
+// Adam on the f32 master weights; writes the bf16-rounded working copy.
+__global__ void adam_kernel(float* __restrict__ master, float* __restrict__ m, float* __restrict__ v,
+                            const float* __restrict__ g, float* __restrict__ wq, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kParams) return;
+  const float gi = g[i];
+  m[i] = b1 * m[i] + (1.0f - b1) * gi;
+  v[i] = b2 * v[i] + (1.0f - b2) * gi * gi;
+  master[i] -= lr * (m[i] / c1) / (sqrtf(v[i] / c2) + eps);
+  wq[i] = bf16r(master[i]);
+}
+
+__global__ void predict_kernel(const float* __restrict__ X, uint64_t n, const float* __restrict__ wq,
+                               float* __restrict__ out) {
+  __shared__ float w[kParams];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = wq[i];
+  __syncthreads();
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  float x[kF];
+  for (int i = 0; i < kF; ++i) x[i] = bf16r(X[s * kF + i]);
+  out[s] = mlp_forward(w, x, nullptr, nullptr);
+}
+
+}  // namespace
+
+struct Index::BudgetModel {
+  bool ready = false;
+  std::vector<uint32_t> ladT, ladS;
+  std::vector<float> master, mean, inv_std;
+  float margin = 0.0f;
+  DevBuf d_w, d_mean, d_inv, d_ladT, d_ladS;
+  // feature capture during a fit
+  float* sink = nullptr;
+  uint64_t sink_rows = 0;
+  BudgetReport report;
+};
+
+bool Index::budget_model_ready() const { return budget_ && budget_->ready; }
+
+void Index::run_budget_model(uint64_t qn, int l, const float* ranges, const uint32_t* box_lo, const uint32_t* box_w,
+                             int n_valid, const float* feat_dist, uint32_t* n_chosen_c, uint32_t* budget_T,
+                             uint32_t* budget_S, hipStream_t s) {
+  (void)box_lo;
+  BudgetModel& B = *budget_;
+  hipLaunchKernelGGL(infer_kernel, dim3((unsigned)((qn + 255) / 256)), dim3(256), 0, s, qn, l, ranges, box_w,
+                     n_valid, feat_dist, B.d_w.as<float>(), B.d_mean.as<float>(), B.d_inv.as<float>(), B.margin,
+                     (int)B.ladT.size(), B.d_ladT.as<uint32_t>(), B.d_ladS.as<uint32_t>(), n_chosen_c, budget_T,
+                     budget_S);
+  HDG_HIP(hipGetLastError());
+}
+
+void Index::capture_features(uint64_t q0, uint64_t qn, int l, const float* ranges, const uint32_t* box_w,
+                             int n_valid, const float* feat_dist, hipStream_t s) {
+  if (!budget_ || !budget_->sink) return;
+  if (q0 + qn > budget_->sink_rows) fail(kBadState, "feature capture beyond the training set");
+  hipLaunchKernelGGL(features_kernel, dim3((unsigned)((qn + 255) / 256)), dim3(256), 0, s, qn, l, ranges, box_w,
+                     n_valid, feat_dist, budget_->sink + q0 * kF);
+  HDG_HIP(hipGetLastError());
+}
+
+static std::vector<float> recall_per_query(const std::vector<uint64_t>& res, const uint64_t* gt, uint64_t n,
+                                           uint64_t k) {
+  std::vector<float> r(n);
+  std::vector<uint64_t> a(k), b(k);
+  for (uint64_t q = 0; q < n; ++q) {
+    uint64_t nb = 0;
+    for (uint64_t j = 0; j < k; ++j) {
+      uint64_t x = gt[q * k + j];
+      if (x < (1ull << 63)) b[nb++] = x;
+    }
+    std::sort(b.begin(), b.begin() + nb);
+    nb = (uint64_t)(std::unique(b.begin(), b.begin() + nb) - b.begin());
+    for (uint64_t j = 0; j < k; ++j) a[j] = res[q * k + j];
+    std::sort(a.begin(), a.end());
+    uint64_t na = (uint64_t)(std::unique(a.begin(), a.end()) - a.begin());
+    uint64_t hit = 0, i = 0, j = 0;
+    while (i < na && j < nb) {
+      if (a[i] == b[j]) {
+        ++hit;
+        ++i;
+        ++j;
+      } else if (a[i] < b[j]) {
+        ++i;
+      } else {
+        ++j;
+      }
+    }
+    r[q] = nb ? (float)hit / (float)nb : 1.0f;
+  }
+  return r;
+}
+
+void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float* qlabels, uint64_t qcols,
+                       const uint64_t* gt_host, uint64_t k, const BudgetFitParams& P, hipStream_t s) {
+  HDG_HIP(hipSetDevice(device_));
+  if (!st_.built) fail(kBadState, "budget fit before build");
+  if (k != cfg_.topk) fail(kInvalidArgument, "ground truth width differs from TOPK");
+  if (P.levels < 2 || P.levels > 32) fail(kInvalidArgument, "levels must be in [2, 32]");
+  if (!budget_) budget_ = new BudgetModel();
+  BudgetModel& B = *budget_;
+  B.ready = false;
+  // ladder
+  const uint32_t n_list = (uint32_t)std::min<uint64_t>(cfg_.n_list, st_.C);
+  const uint32_t sub_lists = (uint32_t)cfg_.sub_lists;
+  B.ladT.resize(P.levels);
+  B.ladS.resize(P.levels);
+  for (int i = 0; i < P.levels; ++i) {
+    const double f = std::pow(P.ratio, i);
+    B.ladT[i] = (uint32_t)std::min<double>(n_list, std::max(1.0, std::round(P.t0 * f)));
+    B.ladS[i] = (uint32_t)std::min<double>(sub_lists, std::max(1.0, std::round(P.s0 * f)));
+  }
+  // traces: recall of every training query at every level, features from one extra pass
+  const Config saved = cfg_;
+  cfg_.precision = 1;
+  cfg_.budget_model = 0;
+  DevBuf d_dist, d_ids, d_feat;
+  d_dist.reserve(n * k * 4);
+  d_ids.reserve(n * k * 8);
+  d_feat.reserve(n * kF * 4);
+  std::vector<uint64_t> ids(n * k);
+  std::vector<std::vector<float>> rec(P.levels);
+  try {
+    for (int i = 0; i < P.levels; ++i) {
+      cfg_.probe_clusters = B.ladT[i];
+      cfg_.sub_lists = B.ladS[i];
+      if (i == P.levels - 1) {
+        B.sink = d_feat.as<float>();
+        B.sink_rows = n;
+      }
+      query(queries, n, d, qlabels, qcols, d_dist.as<float>(), d_ids.as<uint64_t>(), s);
+      B.sink = nullptr;
+      HDG_HIP(hipMemcpy(ids.data(), d_ids.data(), n * k * 8, hipMemcpyDeviceToHost));
+      rec[i] = recall_per_query(ids, gt_host, n, k);
+    }
+  } catch (...) {
+    cfg_ = saved;
+    B.sink = nullptr;
+    throw;
+  }
+  cfg_ = saved;
+  // labels: smallest level reaching the target recall (the top level when none does)
+  std::vector<float> y(n);
+  for (uint64_t q = 0; q < n; ++q) {
+    int lev = P.levels - 1;
+    for (int i = 0; i < P.levels; ++i)
+      if (rec[i][q] >= P.target) {
+        lev = i;
+        break;
+      }
+    y[q] = (float)lev;
+  }
+  std::vector<float> X(n * kF);
+  HDG_HIP(hipMemcpy(X.data(), d_feat.data(), n * kF * 4, hipMemcpyDeviceToHost));
+  // split: the last holdout share of a seeded permutation calibrates the margin
+  std::vector<uint32_t> perm(n);
+  std::iota(perm.begin(), perm.end(), 0u);
+  std::mt19937_64 rng(P.seed);
+  std::shuffle(perm.begin(), perm.end(), rng);
+  const uint64_t n_hold = std::max<uint64_t>(1, (uint64_t)(n * P.holdout));
+  const uint64_t n_train = n - n_hold;
+  B.mean.assign(kF, 0.0f);
+  B.inv_std.assign(kF, 1.0f);
+  for (int f = 0; f < kF; ++f) {
+    double s1 = 0, s2 = 0;
+    for (uint64_t t = 0; t < n_train; ++t) {
+      double v = X[(uint64_t)perm[t] * kF + f];
+      s1 += v;
+      s2 += v * v;
+    }
+    double mu = s1 / n_train, var = s2 / n_train - mu * mu;
+    B.mean[f] = (float)mu;
+    B.inv_std[f] = var > 1e-12 ? (float)(1.0 / std::sqrt(var)) : 1.0f;
+  }
+  for (uint64_t q = 0; q < n; ++q)
+    for (int f = 0; f < kF; ++f) X[q * kF + f] = (X[q * kF + f] - B.mean[f]) * B.inv_std[f];
+  // init (He) and train on the GPU
+  B.master.assign(kParams, 0.0f);
+  {
+    std::normal_distribution<float> N01(0.0f, 1.0f);
+    for (int i = 0; i < kP1; ++i) B.master[kOffW1 + i] = N01(rng) * std::sqrt(2.0f / kF);
+    for (int i = 0; i < kP2; ++i) B.master[kOffW2 + i] = N01(rng) * std::sqrt(2.0f / kH);
+    for (int i = 0; i < kH; ++i) B.master[kOffW3 + i] = N01(rng) * std::sqrt(1.0f / kH);
+    float ymean = 0;
+    for (uint64_t t = 0; t < n_train; ++t) ymean += y[perm[t]];
+    B.master[kOffB3] = ymean / (float)n_train;
+  }
+  DevBuf dX, dY, dIdx, dMaster, dM, dV, dG, dWq, dLoss;
+  upload(dX, X, s);
+  upload(dY, y, s);
+  upload(dMaster, B.master, s);
+  std::vector<float> zeros(kParams, 0.0f), wq(kParams);
+  upload(dM, zeros, s);
+  upload(dV, zeros, s);
+  dG.reserve(kParams * 4);
+  for (int i = 0; i < kParams; ++i) {
+    uint32_t u;
+    std::memcpy(&u, &B.master[i], 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    std::memcpy(&wq[i], &u, 4);
+  }
+  upload(dWq, wq, s);
+  dLoss.reserve(4);
+  dIdx.reserve(n_train * 4);
+  std::vector<uint32_t> order(perm.begin(), perm.begin() + n_train);
+  const uint32_t bsz = (uint32_t)std::min<uint64_t>(P.batch, n_train);
+  int step = 0;
+  float last_loss = 0.0f;
+  for (int ep = 0; ep < P.epochs; ++ep) {
+    std::shuffle(order.begin(), order.end(), rng);
+    upload(dIdx, order, s);
+    float ep_loss = 0.0f;
+    int nb = 0;
+    for (uint64_t b0 = 0; b0 + bsz <= n_train; b0 += bsz) {
+      HDG_HIP(hipMemsetAsync(dG.data(), 0, kParams * 4, s));
+      HDG_HIP(hipMemsetAsync(dLoss.data(), 0, 4, s));
+      hipLaunchKernelGGL(train_step_kernel, dim3((bsz + 127) / 128), dim3(128), 0, s, dX.as<float>(), dY.as<float>(),
+                         dIdx.as<uint32_t>() + b0, bsz, dWq.as<float>(), dG.as<float>(), dLoss.as<float>());
+      ++step;
+      const float c1 = 1.0f - std::pow(0.9f, (float)step), c2 = 1.0f - std::pow(0.999f, (float)step);
+      hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, dMaster.as<float>(),
+                         dM.as<float>(), dV.as<float>(), dG.as<float>(), dWq.as<float>(), P.lr, 0.9f, 0.999f, 1e-8f,
+                         c1, c2);
+      HDG_HIP(hipGetLastError());
+      float lv = 0.0f;
+      HDG_HIP(hipMemcpyAsync(&lv, dLoss.data(), 4, hipMemcpyDeviceToHost, s));
+      HDG_HIP(hipStreamSynchronize(s));
+      ep_loss += lv;
+      ++nb;
+    }
+    last_loss = nb ? ep_loss / nb : 0.0f;
+  }
+  HDG_HIP(hipMemcpy(B.master.data(), dMaster.data(), kParams * 4, hipMemcpyDeviceToHost));
+  HDG_HIP(hipMemcpy(wq.data(), dWq.data(), kParams * 4, hipMemcpyDeviceToHost));
+  // predictions for all samples; calibrate the margin on the holdout
+  DevBuf dPred;
+  dPred.reserve(n * 4);
+  hipLaunchKernelGGL(predict_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, dX.as<float>(), n,
+                     dWq.as<float>(), dPred.as<float>());
+  HDG_HIP(hipGetLastError());
+  std::vector<float> pred(n);
+  HDG_HIP(hipMemcpy(pred.data(), dPred.data(), n * 4, hipMemcpyDeviceToHost));
+  auto level_of = [&](float yhat, float margin) {
+    if (!std::isfinite(yhat)) return P.levels - 1;
+    float t = std::ceil(yhat + margin);
+    return t < 0 ? 0 : (t > P.levels - 1 ? P.levels - 1 : (int)t);
+  };
+  float chosen = P.max_margin;
+  double hold_rec = 0.0, hold_lev = 0.0;
+  for (float m = 0.0f; m <= P.max_margin + 1e-6f; m += 0.125f) {
+    double rsum = 0, lsum = 0;
+    for (uint64_t t = n_train; t < n; ++t) {
+      const uint32_t q = perm[t];
+      const int lev = level_of(pred[q], m);
+      rsum += rec[lev][q];
+      lsum += lev;
+    }
+    hold_rec = rsum / n_hold;
+    hold_lev = lsum / n_hold;
+    if (hold_rec >= P.target) {
+      chosen = m;
+      break;
+    }
+  }
+  B.margin = chosen;
+  double mean_label = 0, fixed_rec = 0;
+  for (uint64_t q = 0; q < n; ++q) {
+    mean_label += y[q];
+    fixed_rec += rec[P.levels - 1][q];
+  }
+  B.report = BudgetReport{last_loss, B.margin, (float)hold_rec, (float)hold_lev, (float)(mean_label / n),
+                          (float)(fixed_rec / n), (uint64_t)n_train, (uint64_t)n_hold};
+  upload(B.d_w, wq, s);
+  upload(B.d_mean, B.mean, s);
+  upload(B.d_inv, B.inv_std, s);
+  upload(B.d_ladT, B.ladT, s);
+  upload(B.d_ladS, B.ladS, s);
+  B.ready = true;
+}
+
+const BudgetReport& Index::budget_report() const {
+  if (!budget_) fail(kBadState, "no budget model");
+  return budget_->report;
+}
+
+void Index::budget_save(const std::string& path) const {
+  if (!budget_model_ready()) fail(kBadState, "no budget model to save");
+  const BudgetModel& B = *budget_;
+  std::ofstream f(path, std::ios::binary);
+  if (!f) fail(kIoError, "cannot write " + path);
+  const uint32_t magic = 0x48444742u, levels = (uint32_t)B.ladT.size(), params = kParams;
+  f.write((const char*)&magic, 4);
+  f.write((const char*)&levels, 4);
+  f.write((const char*)&params, 4);
+  f.write((const char*)&B.margin, 4);
+  f.write((const char*)B.ladT.data(), levels * 4);
+  f.write((const char*)B.ladS.data(), levels * 4);
+  f.write((const char*)B.mean.data(), kF * 4);
+  f.write((const char*)B.inv_std.data(), kF * 4);
+  f.write((const char*)B.master.data(), kParams * 4);
+}
+
+void Index::budget_load(const std::string& path, hipStream_t s) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) fail(kIoError, "cannot read " + path);
+  uint32_t magic = 0, levels = 0, params = 0;
+  float margin = 0;
+  f.read((char*)&magic, 4);
+  f.read((char*)&levels, 4);
+  f.read((char*)&params, 4);
+  f.read((char*)&margin, 4);
+  if (magic != 0x48444742u || params != (uint32_t)kParams || levels < 2 || levels > 32)
+    fail(kIoError, "not a budget model file: " + path);
+  if (!budget_) budget_ = new BudgetModel();
+  BudgetModel& B = *budget_;
+  B.ladT.resize(levels);
+  B.ladS.resize(levels);
+  B.mean.resize(kF);
+  B.inv_std.resize(kF);
+  B.master.resize(kParams);
+  f.read((char*)B.ladT.data(), levels * 4);
+  f.read((char*)B.ladS.data(), levels * 4);
+  f.read((char*)B.mean.data(), kF * 4);
+  f.read((char*)B.inv_std.data(), kF * 4);
+  f.read((char*)B.master.data(), kParams * 4);
+  if (!f) fail(kIoError, "short budget model file: " + path);
+  B.margin = margin;
+  std::vector<float> wq(kParams);
+  for (int i = 0; i < kParams; ++i) {
+    uint32_t u;
+    std::memcpy(&u, &B.master[i], 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    std::memcpy(&wq[i], &u, 4);
+  }
+  upload(B.d_w, wq, s);
+  upload(B.d_mean, B.mean, s);
+  upload(B.d_inv, B.inv_std, s);
+  upload(B.d_ladT, B.ladT, s);
+  upload(B.d_ladS, B.ladS, s);
+  B.ready = true;
+}
+
+void Index::budget_free() {
+  delete budget_;
+  budget_ = nullptr;
+}
+
+// One forward/backward of the training kernel on a given batch (gradient check against a CPU oracle).
+void budget_debug_grads(const float* params, const float* X, const float* Y, uint32_t n, float* grads, float* loss) {
+  DevBuf dX, dY, dIdx, dW, dG, dL;
+  std::vector<float> x(X, X + (size_t)n * kF), y(Y, Y + n), w(params, params + kParams);
+  std::vector<uint32_t> idx(n);
+  std::iota(idx.begin(), idx.end(), 0u);
+  upload(dX, x, nullptr);
+  upload(dY, y, nullptr);
+  upload(dIdx, idx, nullptr);
+  upload(dW, w, nullptr);
+  dG.reserve(kParams * 4);
+  dL.reserve(4);
+  HDG_HIP(hipMemset(dG.data(), 0, kParams * 4));
+  HDG_HIP(hipMemset(dL.data(), 0, 4));
+  hipLaunchKernelGGL(train_step_kernel, dim3((n + 127) / 128), dim3(128), 0, nullptr, dX.as<float>(), dY.as<float>(),
+                     dIdx.as<uint32_t>(), n, dW.as<float>(), dG.as<float>(), dL.as<float>());
+  HDG_HIP(hipGetLastError());
+  HDG_HIP(hipDeviceSynchronize());
+  HDG_HIP(hipMemcpy(grads, dG.data(), kParams * 4, hipMemcpyDeviceToHost));
+  HDG_HIP(hipMemcpy(loss, dL.data(), 4, hipMemcpyDeviceToHost));
+}
+
+int budget_param_count() { return kParams; }
+
+}  // namespace hdg

@@ -277,6 +277,31 @@ std::vector<uint32_t> sample_rows(uint64_t n, uint64_t want, uint64_t seed) {
 
 }  // namespace
 
+void prepare_mixed_centers(IndexState& st, const std::vector<float>& centers, hipStream_t s) {
+  const uint64_t C = st.C, d = st.d;
+  st.Dp = (d + 31) / 32 * 32;
+  std::vector<uint16_t> b16(C * st.Dp, 0);
+  std::vector<float> norm(C, 0.0f);
+  for (uint64_t c = 0; c < C; ++c) {
+    double acc = 0.0;
+    for (uint64_t t = 0; t < d; ++t) {
+      uint32_t u;
+      float v = centers[c * d + t];
+      std::memcpy(&u, &v, 4);
+      u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+      const uint16_t h = (uint16_t)(u >> 16);
+      b16[c * st.Dp + t] = h;
+      uint32_t back = (uint32_t)h << 16;
+      float r;
+      std::memcpy(&r, &back, 4);
+      acc += (double)r * r;
+    }
+    norm[c] = (float)acc;
+  }
+  upload(st.centers_bf16, b16, s);
+  upload(st.center_norm, norm, s);
+}
+
 void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* labels, uint64_t label_cols,
                   hipStream_t s) {
   HDG_HIP(hipSetDevice(device_));
@@ -444,6 +469,7 @@ void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* lab
   upload(st_.zid, st_.h_zid, s);
   upload(st_.pq_centers, pq_centers, s);
   upload(st_.centers, coarse, s);
+  prepare_mixed_centers(st_, coarse, s);
   upload(st_.sub_centers, sub_centers, s);
   upload(st_.sub_off, st_.h_sub_off, s);
   {

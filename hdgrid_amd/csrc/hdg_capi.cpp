@@ -279,4 +279,61 @@ hdg_status hdg_split_task(const double coeff[4], uint64_t n_data, uint64_t n_que
 
 int hdg_find_max_factor(uint64_t a, uint64_t b) { return hdg::find_max_factor(a, b); }
 
+hdg_status hdg_budget_fit(hdg_index* index, const float* queries, uint64_t n, uint64_t dim, const float* query_labels,
+                          uint64_t qlabel_cols, const uint64_t* gt_ids, uint64_t k, int levels, double t0, double s0,
+                          double ratio, float target, int epochs, float lr, uint64_t seed, float* report,
+                          void* stream) {
+  return guarded([&] {
+    if (!index || !queries || !query_labels || !gt_ids) hdg::fail(HDG_EINVAL, "null argument");
+    if (n < 20) hdg::fail(HDG_EINVAL, "budget fit needs at least 20 training queries");
+    hdg::BudgetFitParams p;
+    p.levels = levels;
+    p.t0 = t0;
+    p.s0 = s0;
+    p.ratio = ratio;
+    p.target = target;
+    p.epochs = epochs;
+    p.lr = lr;
+    p.seed = seed;
+    index->impl->budget_fit(queries, n, dim, query_labels, qlabel_cols, gt_ids, k, p, (hipStream_t)stream);
+    if (report) {
+      const hdg::BudgetReport& r = index->impl->budget_report();
+      const float vals[8] = {r.final_loss, r.margin, r.holdout_recall, r.holdout_level, r.mean_label,
+                             r.fixed_recall, (float)r.n_train, (float)r.n_holdout};
+      std::memcpy(report, vals, sizeof(vals));
+    }
+  });
+}
+
+hdg_status hdg_budget_save(const hdg_index* index, const char* path) {
+  return guarded([&] {
+    if (!index || !path) hdg::fail(HDG_EINVAL, "null argument");
+    index->impl->budget_save(path);
+  });
+}
+
+hdg_status hdg_budget_load(hdg_index* index, const char* path, void* stream) {
+  return guarded([&] {
+    if (!index || !path) hdg::fail(HDG_EINVAL, "null argument");
+    index->impl->budget_load(path, (hipStream_t)stream);
+  });
+}
+
+hdg_status hdg_budget_debug_grads(const float* params, const float* X, const float* Y, uint32_t n, float* grads,
+                                  float* loss) {
+  return guarded([&] {
+    if (!params || !X || !Y || !grads || !loss || n == 0) hdg::fail(HDG_EINVAL, "null argument");
+    hdg::budget_debug_grads(params, X, Y, n, grads, loss);
+  });
+}
+
+int hdg_budget_param_count(void) { return hdg::budget_param_count(); }
+
+hdg_status hdg_debug_coarse_scores(hdg_index* index, const float* queries, uint64_t nq, float* out, void* stream) {
+  return guarded([&] {
+    if (!index || !queries || !out) hdg::fail(HDG_EINVAL, "null argument");
+    index->impl->debug_coarse_scores(queries, nq, out, (hipStream_t)stream);
+  });
+}
+
 }  // extern "C"

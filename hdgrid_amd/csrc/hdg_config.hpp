@@ -45,6 +45,9 @@ struct Config {
   uint64_t balance_iters = 25;         // BALANCE_ITERS: iterations of the sub-center grouping
   uint64_t parity_batch = 0;           // PARITY_BATCH: beta group size (0 = findMaxFactor(125, nq))
   uint64_t query_chunk = 16384;        // QUERY_CHUNK: queries processed per device pass
+  uint64_t precision = 0;              // PRECISION: 0 reference-exact FP32, 1 BF16 coarse + FP16 LUT
+  uint64_t probe_clusters = 0;         // PROBE_CLUSTERS: clusters probed per query in PRECISION 1 (0 = N_LIST)
+  uint64_t budget_model = 0;           // BUDGET_MODEL: 1 = per-query budgets from the loaded MLP
 };
 
 // Parse with config_utils.h:33-88 semantics: `KEY = value` per line, key case-insensitive and

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "hdg_budget.hpp"
 #include "hdg_common.hpp"
 #include "hdg_config.hpp"
 #include "hdg_grid.hpp"
@@ -53,8 +54,15 @@ struct IndexState {
   DevBuf sub_off;     // u32 [C+1]
   DevBuf seg_off;     // u32 [CS+1]
   DevBuf cluster_of_sub;  // u32 [CS]
+  // mixed-precision coarse scoring (PRECISION 1)
+  uint64_t Dp = 0;        // d rounded up to a multiple of 32
+  DevBuf centers_bf16;    // bf16 [C][Dp], zero padded
+  DevBuf center_norm;     // f32 [C]: |c|^2 of the bf16-rounded center
   bool built = false;
 };
+
+// Fills centers_bf16 / center_norm / Dp from host centers [C][d].
+void prepare_mixed_centers(IndexState& st, const std::vector<float>& centers, hipStream_t s);
 
 class Index {
  public:
@@ -83,7 +91,26 @@ class Index {
   // timing of the last query call per stage (ms, host-side event timing), for profiling
   std::vector<std::pair<std::string, float>> stage_ms;
 
+  // learned per-query budget (hdg_budget.hip)
+  bool budget_model_ready() const;
+  void run_budget_model(uint64_t qn, int l, const float* ranges, const uint32_t* box_lo, const uint32_t* box_w,
+                        int n_valid, const float* feat_dist, uint32_t* n_chosen_c, uint32_t* budget_T,
+                        uint32_t* budget_S, hipStream_t s);
+  void capture_features(uint64_t q0, uint64_t qn, int l, const float* ranges, const uint32_t* box_w, int n_valid,
+                        const float* feat_dist, hipStream_t s);
+  // Labels every training query with its minimal budget level (traces of this pipeline against the
+  // caller's ground truth ids, host [n][k]), then trains the MLP on the GPU.
+  void budget_fit(const float* queries, uint64_t n, uint64_t d, const float* qlabels, uint64_t qcols,
+                  const uint64_t* gt_host, uint64_t k, const BudgetFitParams& p, hipStream_t s);
+  const BudgetReport& budget_report() const;
+  void budget_save(const std::string& path) const;
+  void budget_load(const std::string& path, hipStream_t s);
+  void budget_free();
+  void debug_coarse_scores(const float* queries, uint64_t nq, float* out_host, hipStream_t s);
+  struct BudgetModel;
+
  private:
+  BudgetModel* budget_ = nullptr;
   void query_chunk(const float* queries, uint64_t q0, uint64_t nq, const float* qlabels,
                    uint64_t qcols, uint64_t group, float* out_dist, uint64_t* out_ids,
                    hipStream_t stream);

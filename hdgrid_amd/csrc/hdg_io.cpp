@@ -212,7 +212,11 @@ void Index::load(const std::string& dir, hipStream_t s) {
       for (uint32_t p = S.h_sub_off[c]; p < S.h_sub_off[c + 1]; ++p) cos[p] = (uint32_t)c;
     upload(S.cluster_of_sub, cos, s);
   }
-  upload(S.centers, read_npy<float>(dir + "/centers.npy", sh), s);
+  {
+    std::vector<float> centers = read_npy<float>(dir + "/centers.npy", sh);
+    upload(S.centers, centers, s);
+    prepare_mixed_centers(S, centers, s);
+  }
   upload(S.sub_centers, read_npy<float>(dir + "/sub_centers.npy", sh), s);
   upload(S.pq_centers, read_npy<float>(dir + "/pq_centers.npy", sh), s);
   upload(S.codes, read_npy<uint8_t>(dir + "/codes.npy", sh), s);

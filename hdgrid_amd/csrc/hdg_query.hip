@@ -19,7 +19,10 @@
 #include <chrono>
 #include <cstring>
 
+#include <hip/hip_fp16.h>
+
 #include "hdg_device.hpp"
+#include "hdg_fast.hpp"
 #include "hdg_index.hpp"
 
 namespace hdg {
@@ -422,12 +425,14 @@ __global__ __launch_bounds__(256) void sub_nnz_kernel(const uint32_t* __restrict
 
 // A6 part 2: exact distances to the non-zero sub-centers, top ks = min(sub_lists, group max), with
 // the group's table width supplying the 1e6 pad columns (Super-Grid-IVF_PQ.cu:885-896).  Output is
-// sorted ascending (the reference's order here is RAFT-defined).
+// sorted ascending (the reference's order here is RAFT-defined).  With a per-query budget array
+// (PRECISION 1 with a budget model) the query keeps its own min(budget, sub_lists) nearest and no pads.
 __global__ __launch_bounds__(256) void sub_select_kernel(
     const float* __restrict__ queries, uint64_t d, const float* __restrict__ sub_centers,
     uint32_t sub_lists, uint64_t G, uint64_t q0, uint32_t cap, const uint32_t* __restrict__ list_s,
     const uint32_t* __restrict__ nnz_s, const unsigned long long* __restrict__ gstat,
-    uint64_t* __restrict__ keys, uint32_t* __restrict__ chosen_s, uint32_t* __restrict__ n_chosen_s) {
+    const uint32_t* __restrict__ sub_budget, uint64_t* __restrict__ keys, uint32_t* __restrict__ chosen_s,
+    uint32_t* __restrict__ n_chosen_s) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t* sorted = reinterpret_cast<uint64_t*>(smem);
   float* stage = reinterpret_cast<float*>(smem);
@@ -451,8 +456,13 @@ __global__ __launch_bounds__(256) void sub_select_kernel(
   }
   __syncthreads();
   const uint64_t g = (q0 + qi) / G;
-  const uint32_t ks = (uint32_t)min<uint64_t>(sub_lists, gstat[g * 8 + 1]);
-  const uint64_t table_len = gstat[g * 8 + 2];
+  uint32_t ks = (uint32_t)min<uint64_t>(sub_lists, gstat[g * 8 + 1]);
+  uint64_t table_len = gstat[g * 8 + 2];
+  if (sub_budget) {
+    const uint32_t b = sub_budget[qi];
+    ks = b == 0 || b > sub_lists ? sub_lists : b;
+    table_len = n;
+  }
   const uint32_t m = select_sorted(gk, n, ks, sorted, hist, sh);
   __syncthreads();
   const uint32_t keep = kept_with_pads(sorted, m, ks, table_len - n);
@@ -682,6 +692,13 @@ __global__ __launch_bounds__(256) void keep_all_kernel(
 constexpr uint32_t kLutPhaseBytes = 48 * 1024;
 constexpr int kAdcPer = kAdcChunk / kWG;
 
+// LUT entry types: f32 (reference-exact) or f16 (PRECISION 1); sums always accumulate in f32.
+__device__ __forceinline__ float lutf(float v) { return v; }
+__device__ __forceinline__ float lutf(__half v) { return __half2float(v); }
+__device__ __forceinline__ void lut_store(float* p, float v) { *p = v; }
+__device__ __forceinline__ void lut_store(__half* p, float v) { *p = __float2half(v); }
+
+template <typename LT>
 __global__ __launch_bounds__(256) void adc_scan_kernel(
     uint64_t qn, uint64_t q0, const float* __restrict__ queries, uint64_t d, uint64_t M, uint64_t K,
     uint64_t pq_len, const float* __restrict__ pq_centers, const uint8_t* __restrict__ codes,
@@ -690,7 +707,7 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
     const uint32_t* __restrict__ L, const uint64_t* __restrict__ loff, const uint32_t* __restrict__ nL,
     const uint64_t* __restrict__ coff, const uint64_t* __restrict__ aoff, uint64_t* __restrict__ akeys) {
   extern __shared__ __align__(16) unsigned char smem[];
-  float* lut = reinterpret_cast<float*>(smem);  // [mp][K]
+  LT* lut = reinterpret_cast<LT*>(smem);  // [mp][K]
   // which query: last qi with coff[qi] <= blockIdx.x
   const uint64_t blk = blockIdx.x;
   uint64_t lo = 0, hi = qn;
@@ -718,7 +735,7 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
     pos[i] = 0;
     if (col < c1) pos[i] = from_list ? list[col] : col_to_pos(rs, rp, nr, (uint32_t)col);
   }
-  uint64_t mp = kLutPhaseBytes / (K * sizeof(float));
+  uint64_t mp = kLutPhaseBytes / (K * sizeof(LT));
   if (mp > 4) mp &= ~3ull;
   const bool words = (M % 4) == 0 && (mp % 4) == 0;
   for (uint64_t m0 = 0; m0 < M; m0 += mp) {
@@ -731,7 +748,7 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
         float diff = pq_centers[m * pq_len * K + c * pq_len + t] - q[m * pq_len + t];
         ans = fmaf(diff, diff, ans);
       }
-      lut[e] = ans;
+      lut_store(&lut[e], ans);
     }
     __syncthreads();
 #pragma unroll
@@ -743,14 +760,14 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
       if (words) {
         for (uint64_t m = m0; m < m1; m += 4) {
           const uint32_t wv = *reinterpret_cast<const uint32_t*>(cd + m);
-          const float* lr = lut + (m - m0) * K;
-          sum += lr[wv & 0xFF];
-          sum += lr[K + ((wv >> 8) & 0xFF)];
-          sum += lr[2 * K + ((wv >> 16) & 0xFF)];
-          sum += lr[3 * K + (wv >> 24)];
+          const LT* lr = lut + (m - m0) * K;
+          sum += lutf(lr[wv & 0xFF]);
+          sum += lutf(lr[K + ((wv >> 8) & 0xFF)]);
+          sum += lutf(lr[2 * K + ((wv >> 16) & 0xFF)]);
+          sum += lutf(lr[3 * K + (wv >> 24)]);
         }
       } else {
-        for (uint64_t m = m0; m < m1; ++m) sum += lut[(m - m0) * K + cd[m]];
+        for (uint64_t m = m0; m < m1; ++m) sum += lutf(lut[(m - m0) * K + cd[m]]);
       }
       part[i] = sum;
     }
@@ -901,6 +918,7 @@ struct Index::Work {
   DevBuf keys, gstat, rstart, rpre, cand, real;
   DevBuf mode, kprime, lsize, asize, chunks, pad32, padmax, nL;
   DevBuf L, Lkeys, akeys;
+  DevBuf S, feat, budget_T, budget_S;  // PRECISION 1: coarse scores, model features, per-query budgets
   uint64_t* pinned = nullptr;
   hipEvent_t ev[16];
   bool ev_init = false;
@@ -935,6 +953,7 @@ Index::Index(const Config& cfg, const FilterConfig& fc, int device) : cfg_(cfg),
 }
 
 Index::~Index() {
+  budget_free();
   if (w_) {
     (void)hipSetDevice(device_);
     if (w_->pinned) (void)hipHostFree(w_->pinned);
@@ -942,6 +961,18 @@ Index::~Index() {
       for (auto& e : w_->ev) (void)hipEventDestroy(e);
     delete w_;
   }
+}
+
+void Index::debug_coarse_scores(const float* queries, uint64_t nq, float* out_host, hipStream_t s) {
+  HDG_HIP(hipSetDevice(device_));
+  if (!st_.built) fail(kBadState, "query before build");
+  DevBuf S;
+  S.reserve(nq * st_.C * 4);
+  launch_coarse_gemm(queries, 0, nq, st_.d, st_.Dp, st_.centers_bf16.as<uint16_t>(), st_.center_norm.as<float>(),
+                     (uint32_t)st_.C, S.as<float>(), s);
+  HDG_HIP(hipGetLastError());
+  HDG_HIP(hipMemcpyAsync(out_host, S.data(), nq * st_.C * 4, hipMemcpyDeviceToHost, s));
+  HDG_HIP(hipStreamSynchronize(s));
 }
 
 void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* qlabels, uint64_t qcols,
@@ -1045,6 +1076,41 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      w.zlo.as<uint64_t>(), w.zhi.as<uint64_t>(), w.zlo_cnt.as<uint64_t>(), w.err.as<uint32_t>());
   HDG_HIP(hipGetLastError());
   mark();
+  const bool mixed = cfg_.precision == 1;
+  const uint32_t* budget_T = nullptr;
+  const uint32_t* budget_S = nullptr;
+  if (mixed && cfg_.budget_model == 1) {
+    if (!budget_model_ready()) fail(kBadState, "BUDGET_MODEL = 1 but no budget model is loaded");
+    w.budget_T.reserve(qn * 4);
+    w.budget_S.reserve(qn * 4);
+    budget_T = w.budget_T.as<uint32_t>();
+    budget_S = w.budget_S.as<uint32_t>();
+  }
+  if (mixed) {
+    // BF16 MFMA coarse scores -> per-query top-T clusters -> counts of their sub-clusters only
+    w.S.reserve(qn * C * 4);
+    w.feat.reserve(qn * kFeatDists * 4);
+    launch_coarse_gemm(queries, q0, qn, d, S.Dp, S.centers_bf16.as<uint16_t>(), S.center_norm.as<float>(),
+                       (uint32_t)C, w.S.as<float>(), s);
+    HDG_HIP(hipGetLastError());
+    mark();
+    const uint32_t probe = budget_T ? n_list : (uint32_t)(cfg_.probe_clusters ? cfg_.probe_clusters : n_list);
+    launch_coarse_topk(queries, q0, qn, d, w.S.as<float>(), (uint32_t)C, n_list, probe, nullptr,
+                       w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>(), w.feat.as<float>(), s);
+    HDG_HIP(hipGetLastError());
+    capture_features(q0, qn, l, w.ranges.as<float>(), w.box_w.as<uint32_t>(), g.n_valid, w.feat.as<float>(), s);
+    // learned budgets: features -> MLP -> (clusters, sub-clusters) per query; trims n_chosen_c
+    if (budget_T) run_budget_model(qn, l, w.ranges.as<float>(), w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(),
+                                   g.n_valid, w.feat.as<float>(), w.n_chosen_c.as<uint32_t>(),
+                                   w.budget_T.as<uint32_t>(), w.budget_S.as<uint32_t>(), s);
+    mark();
+    launch_budget_count(g.n_valid, S.cells.as<uint32_t>(), S.N, S.zcode.as<uint64_t>(), S.seg_off.as<uint32_t>(),
+                        S.sub_off.as<uint32_t>(), CS, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(),
+                        w.zlo_cnt.as<uint64_t>(), w.zhi.as<uint64_t>(), qn, n_list, w.chosen_c.as<uint32_t>(),
+                        w.n_chosen_c.as<uint32_t>(), w.counts.as<uint32_t>(), s);
+    HDG_HIP(hipGetLastError());
+    mark();
+  } else {
   {
     dim3 grid((unsigned)cdiv(qn, 256), (unsigned)cdiv(CS, kCountSegChunk));
     const uint32_t* cells = S.cells.as<uint32_t>();
@@ -1082,6 +1148,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>());
   HDG_HIP(hipGetLastError());
   mark();
+  }
   hipLaunchKernelGGL(sub_nnz_kernel, dim3((unsigned)qn), dim3(256), 0, s, w.counts.as<uint32_t>(), CS,
                      S.sub_off.as<uint32_t>(), n_list, G, q0, w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>(),
                      cap, w.list_s.as<uint32_t>(), w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>());
@@ -1089,7 +1156,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   mark();
   hipLaunchKernelGGL(sub_select_kernel, dim3((unsigned)qn), dim3(256), sort_lds, s, queries, d,
                      S.sub_centers.as<float>(), sub_lists, G, q0, cap, w.list_s.as<uint32_t>(),
-                     w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>(), w.keys.as<uint64_t>(),
+                     w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>(), budget_S, w.keys.as<uint64_t>(),
                      w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>());
   HDG_HIP(hipGetLastError());
   mark();
@@ -1127,7 +1194,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   HDG_HIP(hipGetLastError());
   mark();
   if (totChunks > 0) {
-    hipLaunchKernelGGL(adc_scan_kernel, dim3((unsigned)totChunks), dim3(256), kLutPhaseBytes, s, qn, q0, queries,
+    auto kern = mixed ? adc_scan_kernel<__half> : adc_scan_kernel<float>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)totChunks), dim3(256), kLutPhaseBytes, s, qn, q0, queries,
                        d, S.M, S.K, S.pq_len, S.pq_centers.as<float>(), S.codes.as<uint8_t>(), sub_lists,
                        w.mode.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),
                        w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.L.as<uint32_t>(), w.lsize.as<uint64_t>(),
@@ -1157,7 +1225,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     HDG_HIP(hipMemcpyAsync(kp.data(), w.kprime.data(), qn * 8, hipMemcpyDeviceToHost, s));
     HDG_HIP(hipMemcpyAsync(nl.data(), w.nL.data(), qn * 4, hipMemcpyDeviceToHost, s));
     HDG_HIP(hipMemcpyAsync(mode.data(), w.mode.data(), qn * 4, hipMemcpyDeviceToHost, s));
-    HDG_HIP(hipMemcpyAsync(nc.data(), w.nnz_c.data(), qn * 4, hipMemcpyDeviceToHost, s));
+    HDG_HIP(hipMemcpyAsync(nc.data(), mixed ? w.n_chosen_c.data() : w.nnz_c.data(), qn * 4, hipMemcpyDeviceToHost,
+                           s));
     HDG_HIP(hipMemcpyAsync(ns.data(), w.nnz_s.data(), qn * 4, hipMemcpyDeviceToHost, s));
     stats_.chosen_subs.resize(qn * sub_lists);
     HDG_HIP(hipMemcpyAsync(stats_.chosen_subs.data(), w.chosen_s.data(), qn * sub_lists * 4, hipMemcpyDeviceToHost, s));
@@ -1170,10 +1239,16 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
       stats_.adc[q0 + i] = mode[i] == 1 ? cand[i] : 0;
       stats_.scored[q0 + i] = (uint64_t)nc[i] + ns[i];
     }
-    static const char* names[] = {"prep_kernel",   "count_kernel",      "cluster_nnz_kernel", "coarse_select_kernel",
-                                  "sub_nnz_kernel", "sub_select_kernel", "ranges_kernel",      "plan+sync",
-                                  "keep_all_kernel", "adc_kernels",      "final_kernel"};
-    const int nnames = (int)(sizeof(names) / sizeof(names[0]));
+    static const char* plain_names[] = {"prep_kernel",    "count_kernel",      "cluster_nnz_kernel",
+                                        "coarse_select_kernel", "sub_nnz_kernel", "sub_select_kernel",
+                                        "ranges_kernel",  "plan+sync",         "keep_all_kernel",
+                                        "adc_kernels",    "final_kernel"};
+    static const char* mixed_names[] = {"prep_kernel",    "coarse_gemm_kernel", "coarse_topk_kernel",
+                                        "budget_count_kernel", "sub_nnz_kernel", "sub_select_kernel",
+                                        "ranges_kernel",  "plan+sync",          "keep_all_kernel",
+                                        "adc_kernels",    "final_kernel"};
+    const char* const* names = mixed ? mixed_names : plain_names;
+    const int nnames = 11;
     for (int i = 0; i + 1 < evn && i < nnames; ++i) {
       float ms = 0;
       HDG_HIP(hipEventElapsedTime(&ms, w.ev[i], w.ev[i + 1]));

@@ -108,6 +108,31 @@ hdg_status hdg_last_chosen_subs(const hdg_index* index, uint64_t nq, uint32_t* o
  * values into ms[0..*n). */
 hdg_status hdg_last_stage_ms(const hdg_index* index, char* names, uint64_t cap, float* ms, uint64_t* n);
 
+/* Learned per-query probe budget (used when PRECISION = 1 and BUDGET_MODEL = 1).  No reference
+ * counterpart: the reference probes a fixed n_list / sub_lists for every query.
+ * hdg_budget_fit labels each of the n training queries (device arrays, like hdg_query) with the
+ * smallest budget level whose top-k reaches `target` recall against gt_ids (host [n][TOPK] u64, invalid
+ * ids >= 2^63), running this pipeline at every level, then trains the 48-64-64-1 MLP on the device.
+ * Ladder: level i probes round(t0*ratio^i) clusters and keeps round(s0*ratio^i) sub-clusters (capped by
+ * N_LIST / SUB_LISTS).  report receives 8 floats: final loss, margin, holdout recall, holdout mean level,
+ * mean label, recall of the top level, n_train, n_holdout. */
+hdg_status hdg_budget_fit(hdg_index* index, const float* queries, uint64_t n, uint64_t dim,
+                          const float* query_labels, uint64_t qlabel_cols, const uint64_t* gt_ids, uint64_t k,
+                          int levels, double t0, double s0, double ratio, float target, int epochs, float lr,
+                          uint64_t seed, float* report, void* stream);
+hdg_status hdg_budget_save(const hdg_index* index, const char* path);
+hdg_status hdg_budget_load(hdg_index* index, const char* path, void* stream);
+/* One forward/backward pass of the training kernel on n samples (X [n][48], Y [n], params [P], all host);
+ * writes the parameter gradients (grads [P]) and the batch loss.  For checking the HIP backward against a
+ * CPU implementation.  P = hdg_budget_param_count(). */
+hdg_status hdg_budget_debug_grads(const float* params, const float* X, const float* Y, uint32_t n, float* grads,
+                                  float* loss);
+int hdg_budget_param_count(void);
+
+/* Coarse scores of the PRECISION 1 path for nq queries (device array): out [nq][CLUSTERS] host f32 with
+ * out[q][c] = |c|^2 - 2 <q, c> computed by the BF16 MFMA kernel.  For checking that kernel. */
+hdg_status hdg_debug_coarse_scores(hdg_index* index, const float* queries, uint64_t nq, float* out, void* stream);
+
 /* Memory-bound batch predictor: split_task (search-engine-auto-batch.cu:430-489) with the
  * reference's model c0*d + c1*q + c2*d*q + c3 <= min(available/5, mem_bound). */
 hdg_status hdg_split_task(const double coeff[4], uint64_t n_data, uint64_t n_queries, uint64_t topk,

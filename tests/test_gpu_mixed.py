@@ -1,0 +1,153 @@
+"""GPU tests of the production patterns: BF16 MFMA coarse scoring, the mixed-precision path, and the learned
+per-query budget (HIP forward/backward checked against the CPU implementation in oracle/budget_mlp.py)."""
+import tempfile
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("no GPU", allow_module_level=True)
+
+import hdgrid_amd as H  # noqa: E402
+from hdgrid_amd import datasets  # noqa: E402
+from oracle import budget_mlp as BM  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+DEV = torch.device("cuda:0")
+FAST = dict(KMEANS_ITERS=6, BALANCE_ITERS=8, KMEANS_TRAIN=65536, PQ_TRAIN=16384)
+
+
+@pytest.fixture(scope="module")
+def mixed_index():
+    w = datasets.mixed_1m(n=60000, nq=2400, d=96, components=48, clusters=64, sub_clusters=8, n_list=64,
+                          sub_lists=64)
+    idx = H.SuperGridIndex(conf_text=w.conf_text(**FAST), filter_text=w.filter_text)
+    print(f"[mixed] build n={w.data.shape[0]} d={w.data.shape[1]}", flush=True)
+    idx.build(torch.from_numpy(w.data).to(DEV), torch.from_numpy(w.labels).to(DEV))
+    tmp = tempfile.TemporaryDirectory()
+    idx.save(tmp.name)
+    state = O.load_state(tmp.name)
+    ranges = O.recover_filters(w.qlabels, 3, **w.filter_arrays)
+    _, gt = O.bruteforce(w.data, w.labels, w.queries, ranges, 100)
+    return dict(w=w, idx=idx, state=state, gt=gt, tmp=tmp, q=torch.from_numpy(w.queries).to(DEV),
+                ql=torch.from_numpy(w.qlabels).to(DEV))
+
+
+def _bf16(x):
+    return BM.bf16(x)
+
+
+def test_coarse_scores_mfma_layout(mixed_index):
+    """BF16 MFMA scores equal |c|^2 - 2 q.c of bf16-rounded operands up to f32 accumulation order."""
+    m = mixed_index
+    q = m["w"].queries[:300]
+    got = m["idx"].debug_coarse_scores(torch.from_numpy(q).to(DEV))
+    cb = _bf16(m["state"]["centers"]).astype(np.float64)
+    qb = _bf16(q).astype(np.float64)
+    ref = (cb * cb).sum(1)[None, :] - 2.0 * qb @ cb.T
+    scale = np.abs(qb).sum(1)[:, None] * np.abs(cb).max() * 4
+    assert np.all(np.abs(got - ref) <= 1e-5 * scale + 1e-5), float(np.abs(got - ref).max())
+    # ranking agrees with exact FP32 distance except near-ties
+    exact = ((q[:, None, :].astype(np.float64) - m["state"]["centers"][None, :, :]) ** 2).sum(-1)
+    agree = (np.argmin(got, 1) == np.argmin(exact, 1)).mean()
+    assert agree > 0.9, agree
+
+
+def test_mixed_full_probe_matches_plain(mixed_index):
+    m = mixed_index
+    idx = m["idx"]
+    idx.set_param("PRECISION", 0)
+    d0, i0 = idx.query(m["q"], m["ql"])
+    idx.set_param("PRECISION", 1)
+    idx.set_param("PROBE_CLUSTERS", 64)
+    d1, i1 = idx.query(m["q"], m["ql"])
+    st = idx.last_stats(m["q"].shape[0])
+    idx.set_param("PRECISION", 0)
+    assert (st.adc == 0).all()  # keep-all path: FP16 LUT not involved, so results must be identical
+    np.testing.assert_array_equal(i0.cpu().numpy(), i1.cpu().numpy())
+    np.testing.assert_array_equal(d0.cpu().numpy(), d1.cpu().numpy())
+
+
+def test_mixed_probe_budget_tradeoff(mixed_index):
+    m = mixed_index
+    idx = m["idx"]
+    idx.set_param("PRECISION", 1)
+    recs, cand = [], []
+    for T in [4, 16, 64]:
+        idx.set_param("PROBE_CLUSTERS", T)
+        _, ids = idx.query(m["q"], m["ql"])
+        recs.append(datasets.recall(m["gt"], ids.cpu().numpy().astype(np.uint64), 100))
+        cand.append(idx.last_stats(m["q"].shape[0]).cand.mean())
+    idx.set_param("PRECISION", 0)
+    print("[mixed] recall by probe", recs, "cand", cand)
+    assert recs[2] >= recs[0] - 0.01 and cand[2] >= cand[0]
+    assert recs[2] > 0.8
+
+
+def test_mixed_adc_fp16_lut_close_to_fp32():
+    """ADC path with the FP16 LUT: ids overlap the FP32-LUT ids heavily (same candidates, PQ order may differ)."""
+    w = datasets.plain_small(n=30000, nq=200, d=64, components=32, clusters=64, sub_clusters=8, shift=50000)
+    idx = H.SuperGridIndex(conf_text=w.conf_text(**FAST), filter_text=w.filter_text)
+    idx.build(torch.from_numpy(w.data).to(DEV), torch.from_numpy(w.labels).to(DEV))
+    q, ql = torch.from_numpy(w.queries).to(DEV), torch.from_numpy(w.qlabels).to(DEV)
+    idx.set_param("EXPS0", 2)
+    idx.set_param("PRECISION", 1)
+    idx.set_param("PROBE_CLUSTERS", 64)
+    _, i1 = idx.query(q, ql)
+    assert (idx.last_stats(200).adc > 0).any()
+    idx.set_param("PRECISION", 0)
+    _, i0 = idx.query(q, ql)
+    a, b = i0.cpu().numpy(), i1.cpu().numpy()
+    overlap = np.mean([len(set(a[r]) & set(b[r])) / 100 for r in range(a.shape[0])])
+    assert overlap > 0.97, overlap
+
+
+def test_budget_backward_matches_cpu_oracle():
+    rng = np.random.default_rng(11)
+    n = 96
+    p = (rng.standard_normal(BM.N_PARAMS) * 0.2).astype(np.float32)
+    X = rng.standard_normal((n, BM.F)).astype(np.float32)
+    Y = rng.uniform(0, 8, n).astype(np.float32)
+    g_gpu, loss_gpu = H.budget_debug_grads(p, X, Y)
+    g_cpu, loss_cpu = BM.grads(p, X, Y)
+    assert abs(loss_gpu - loss_cpu) <= 1e-4 * max(1.0, loss_cpu)
+    scale = np.abs(g_cpu).max()
+    np.testing.assert_allclose(g_gpu, g_cpu, rtol=2e-3, atol=2e-5 * scale)
+    # both ReLU masks are active on this batch, so a dropped mask would change the gradients
+    _, z1, z2, _, _, _ = BM.forward(p, X)
+    assert (z1 <= 0).any() and (z1 > 0).any() and (z2 <= 0).any() and (z2 > 0).any()
+
+
+def test_budget_fit_gain(mixed_index):
+    """Train on 2000 queries, evaluate on 400 held-out: the learned budget keeps recall near the target
+    while probing fewer clusters than the fixed top budget."""
+    m = mixed_index
+    idx = m["idx"]
+    idx.set_param("PRECISION", 1)
+    ntr = 2000
+    rep = idx.budget_fit(m["q"][:ntr], m["ql"][:ntr], m["gt"][:ntr], levels=8, t0=2, s0=4, epochs=40, target=0.95)
+    print("[budget] report", rep)
+    qe, qle, gte = m["q"][ntr:], m["ql"][ntr:], m["gt"][ntr:]
+    ne = qe.shape[0]
+    # fixed budget = top ladder level (64 clusters, 64 sub-clusters)
+    idx.set_param("BUDGET_MODEL", 0)
+    idx.set_param("PROBE_CLUSTERS", 64)
+    _, i_fix = idx.query(qe, qle)
+    st_fix = idx.last_stats(ne)
+    r_fix = datasets.recall(gte, i_fix.cpu().numpy().astype(np.uint64), 100)
+    idx.set_param("BUDGET_MODEL", 1)
+    _, i_l = idx.query(qe, qle)
+    st_l = idx.last_stats(ne)
+    r_l = datasets.recall(gte, i_l.cpu().numpy().astype(np.uint64), 100)
+    idx.set_param("BUDGET_MODEL", 0)
+    idx.set_param("PRECISION", 0)
+    print(f"[budget] fixed recall {r_fix:.4f} scored {st_fix.scored.mean():.1f} cand {st_fix.cand.mean():.1f}; "
+          f"learned recall {r_l:.4f} scored {st_l.scored.mean():.1f} cand {st_l.cand.mean():.1f}")
+    assert rep["final_loss"] < 10.0
+    assert r_l >= min(r_fix, 0.95) - 0.04
+    assert st_l.scored.mean() < st_fix.scored.mean()
+    with tempfile.TemporaryDirectory() as td:
+        idx.budget_save(td + "/m.bin")
+        idx.budget_load(td + "/m.bin")
