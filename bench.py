@@ -3,11 +3,19 @@
 Workload (the configuration the headline metric is quoted on): 1M base vectors, d=384 f32, L2-normalised
 Gaussian mixture (512 components); three label dims -- timestamp U[0, 31,536,000) with shift [604800, 0],
 length U[0, 2000] with shift [30, 30], category 1..5 with interval_map [1,2,1,2,3,4,3,5,4,5]; 10k queries,
-topk 100, n_list 512, sub_lists 128 (raised until recall@100 >= 0.95 against the FP64 filtered brute force),
-clusters 512 x sub_clusters 10, PQ_DIM 96 (4 dims per subvector), 256 codewords.  Synthetic data, seeded.
+topk 100, n_list 512, clusters 512 x sub_clusters 10, PQ_DIM 96 (4 dims per subvector), 256 codewords.
+Synthetic data, seeded.
 
-A step is one hdg_query over all 10k queries with the index and the queries resident in HBM.  The CPU
-baseline is the oracle's FP64 filtered brute force (a C++ restatement of the reference's CPU search,
+A step is one hdg_query over all 10k queries with the index and the queries resident in HBM.  Three paths
+are measured, each at the smallest sub-cluster budget reaching recall@100 >= 0.95 against the FP64 filtered
+brute force (on a 2,000-query sample):
+  plain   -- PRECISION 0, the reference-exact FP32 path (sub_lists doubled from 128 until the target);
+  mixed   -- PRECISION 1: BF16 MFMA cluster and sub-cluster scoring, FP16 LUT, FP32 re-rank, fixed budget;
+  learned -- PRECISION 1 + BUDGET_MODEL 1: per-query sub-cluster budget from the MLP trained on traces of
+             this pipeline over --train seeded training queries (labels = minimal budget level per query).
+`value` is the learned+mixed QPS; the plain and mixed QPS are reported in `config`.
+
+The CPU baseline is the oracle's FP64 filtered brute force (a C++ restatement of the reference's CPU search,
 ground_truth.cuh:166-233, with filter.conf predicates) on a bounded query sample, all host threads.
 """
 import argparse
@@ -30,13 +38,15 @@ def log(*a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--train", type=int, default=200_000, help="budget-model training queries")
     ap.add_argument("--gt-sample", type=int, default=2000, help="queries scored against the FP64 brute force")
     ap.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline")
     ap.add_argument("--target-recall", type=float, default=0.95)
+    ap.add_argument("--paths", default="plain,mixed,learned")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
@@ -66,61 +76,100 @@ def main():
     log(f"index built in {build_s:.1f}s")
     del data_d, lab_d
 
-    # FP64 filtered ground truth on a query sample (the recall oracle)
     ranges = O.recover_filters(w.qlabels, 3, **w.filter_arrays)
     gs = min(args.gt_sample, args.nq)
     t0 = time.time()
     _, gt_ids = O.bruteforce(w.data, w.labels, w.queries[:gs], ranges[:gs], 100)
     log(f"ground truth for {gs} queries in {time.time() - t0:.1f}s")
 
-    def run_once():
+    def recall_now():
         d, i = idx.query(q_d, ql_d)
-        return d, i
-
-    # smallest sub_lists budget reaching the target recall
-    budget = idx.get_param("SUB_LISTS")
-    rec = 0.0
-    while True:
-        idx.set_param("SUB_LISTS", budget)
-        d, i = run_once()
         torch.cuda.synchronize()
-        rec = datasets.recall(gt_ids, i[:gs].cpu().numpy().astype(np.uint64), 100)
-        log(f"sub_lists={budget}: recall@100={rec:.4f} on {gs} queries")
-        if rec >= args.target_recall or budget >= 2048:
-            break
-        budget *= 2
+        return datasets.recall(gt_ids, i[:gs].cpu().numpy().astype(np.uint64), 100)
 
-    for _ in range(args.warmup):
-        run_once()
-    torch.cuda.synchronize()
-    stage_sum = {}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run_once()
-        for k, v in idx.last_stage_ms().items():
-            stage_sum[k] = stage_sum.get(k, 0.0) + v
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed * 1e3 / args.steps
-    qps = args.nq * args.steps / elapsed
-    stage_avg = {k: v / args.steps for k, v in stage_sum.items()}
-    log("stage ms (HIP events, average over timed steps): " +
-        ", ".join(f"{k}={v:.3f}" for k, v in sorted(stage_avg.items(), key=lambda kv: -kv[1])))
+    def timed(label):
+        for _ in range(args.warmup):
+            idx.query(q_d, ql_d)
+        torch.cuda.synchronize()
+        stage_sum = {}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            idx.query(q_d, ql_d)
+            for k, v in idx.last_stage_ms().items():
+                stage_sum[k] = stage_sum.get(k, 0.0) + v
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        stage = {k: v / args.steps for k, v in stage_sum.items()}
+        log(f"{label}: {el * 1e3 / args.steps:.3f} ms/step, stages: " +
+            ", ".join(f"{k}={v:.3f}" for k, v in sorted(stage.items(), key=lambda kv: -kv[1])))
+        return args.nq * args.steps / el, el * 1e3 / args.steps, stage, idx.last_stats(args.nq)
 
-    st = idx.last_stats(args.nq)
+    def sweep(start, factor, cap=4096):
+        b = start
+        while True:
+            idx.set_param("SUB_LISTS", int(b))
+            r = recall_now()
+            log(f"  sub_lists={int(b)}: recall@100={r:.4f}")
+            if r >= args.target_recall or b >= cap:
+                return int(b), r
+            b = b * factor
+
+    paths = args.paths.split(",")
+    res = {}
+    # ---- plain (reference-exact) ----
+    if "plain" in paths:
+        idx.set_param("PRECISION", 0)
+        b, r = sweep(128, 2)
+        qps, ms, stage, st = timed("plain")
+        res["plain"] = dict(qps=qps, ms=ms, recall=r, sub_lists=b, stage=stage, st=st)
+    # ---- mixed, fixed budget ----
+    if "mixed" in paths or "learned" in paths:
+        idx.set_param("PRECISION", 1)
+        idx.set_param("PROBE_CLUSTERS", 0)
+        b, r = sweep(64, 2 ** 0.5)
+        qps, ms, stage, st = timed("mixed")
+        res["mixed"] = dict(qps=qps, ms=ms, recall=r, sub_lists=b, stage=stage, st=st)
+    # ---- learned budget + mixed ----
+    if "learned" in paths:
+        t0 = time.time()
+        tq, tql = w.sample_queries(args.train, 0x7E57)
+        tr = O.recover_filters(tql, 3, **w.filter_arrays)
+        _, tgt = O.bruteforce(w.data, w.labels, tq, tr, 100)
+        log(f"training set: {args.train} queries, ground truth in {time.time() - t0:.1f}s")
+        cap = max(res["mixed"]["sub_lists"] * 2, 256)
+        idx.set_param("SUB_LISTS", cap)
+        t0 = time.time()
+        rep = idx.budget_fit(torch.from_numpy(tq).to(dev), torch.from_numpy(tql).to(dev), tgt, levels=12, t0=0,
+                             s0=8, ratio=2 ** 0.5, target=args.target_recall, epochs=30)
+        log(f"budget model trained in {time.time() - t0:.1f}s: {rep}")
+        idx.set_param("BUDGET_MODEL", 1)
+        r = recall_now()
+        log(f"  learned budget: recall@100={r:.4f}")
+        qps, ms, stage, st = timed("learned+mixed")
+        res["learned"] = dict(qps=qps, ms=ms, recall=r, sub_lists=cap, stage=stage, st=st, report=rep)
+        idx.set_param("BUDGET_MODEL", 0)
+
+    head = "learned" if "learned" in res else ("mixed" if "mixed" in res else "plain")
+    H_ = res[head]
+    st = H_["st"]
+    stage_avg = H_["stage"]
     d_ = w.queries.shape[1]
     l_ = 3
     M = idx.get_param("PQ_DIM")
-    # algorithmic bytes per launch of each kernel (one launch covers all nq queries)
     n_valid = 3
-    alg = {
-        "count_kernel": float((8 + n_valid * 4) * args.n),  # Morton code + cells of every point, once
+    N = args.n
+    C, CS = 512, 5120
+    alg = {  # algorithmic HBM bytes per launch (one launch covers all nq queries)
+        "count_kernel": float((8 + n_valid * 4) * N),
         "final_kernel": float(st.refined.sum() * 4 * d_ + args.nq * 100 * 12),
         "keep_all_kernel": float(st.cand.sum() * 4 * l_),
         "adc_kernels": float(st.adc.sum() * M),
+        "sub_gemm+select": float(CS * 2 * ((d_ + 31) // 32 * 32) + args.nq * (4 * d_) + args.nq * CS * 4 * 2),
+        "coarse_gemm+select+budget": float(C * 2 * ((d_ + 31) // 32 * 32) + args.nq * 4 * d_ + args.nq * C * 4 * 2),
         "coarse_select_kernel": float(st.scored.sum() * 4 * d_),
+        "sub_select_kernel": float(st.scored.sum() * 4 * d_),
     }
-    dom = max((k for k in stage_avg if k.endswith("kernel") or k == "adc_kernels"), key=lambda k: stage_avg[k])
+    dom = max(stage_avg, key=lambda k: stage_avg[k] if k != "plan+sync" else -1)
     dom_ms = stage_avg[dom]
     dom_bytes = alg.get(dom)
     achieved = (dom_bytes / (dom_ms * 1e-3) / 1e9) if (dom_bytes and dom_ms > 0) else None
@@ -128,14 +177,9 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6) if achieved else None, "traffic": None,
                 "avg_ms": round(dom_ms, 4), "alg_bytes_per_launch": dom_bytes}
-    pipe_bytes = (st.adc.sum() * M + st.refined.sum() * 4 * d_ + st.cand.sum() * 4 * l_ + st.scored.sum() * 4 * d_
-                  + args.nq * (4 * d_ + 12 * 100) + (8 + n_valid * 4) * args.n)
-    log(f"dominant kernel {dom}: {dom_ms:.3f} ms/launch; pipeline algorithmic bytes/step {pipe_bytes / 1e9:.3f} GB "
-        f"-> {pipe_bytes / (ms_per_step * 1e-3) / 1e9:.1f} GB/s")
-    log(f"per-query means: C_q={st.cand.mean():.1f} real={st.real.mean():.1f} refined={st.refined.mean():.1f} "
-        f"scored={st.scored.mean():.1f} adc={st.adc.mean():.1f}")
+    log(f"reported path {head}: dominant stage {dom} {dom_ms:.3f} ms; C_q={st.cand.mean():.1f} "
+        f"real={st.real.mean():.1f} refined={st.refined.mean():.1f} scored={st.scored.mean():.1f}")
 
-    # CPU baseline: FP64 filtered brute force on a bounded sample, all host threads
     cs = min(args.cpu_sample, args.nq)
     threads = O.omp_threads()
     t0 = time.perf_counter()
@@ -144,17 +188,33 @@ def main():
     cpu = {"value": round(cs / cpu_s, 2), "unit": "queries/s", "cores": threads, "kind": "port",
            "sample": f"{cs} of the {args.nq} queries, FP64 filtered brute force over all {args.n} base vectors"}
 
+    def summary(k):
+        v = res[k]
+        return {"qps": round(v["qps"], 1), "ms_per_step": round(v["ms"], 4), "recall@100": round(v["recall"], 4),
+                "sub_lists": v["sub_lists"]}
+
+    config = {"workload": f"{w.name}: {args.n} x {d_} f32 L2-normalised 512-component mixture, 3 label dims "
+                          f"(ts shift [604800,0], len shift [30,30], cat map), {args.nq} queries",
+              "path": {"learned": "learned budget + BF16/FP16 mixed + FP32 re-rank",
+                       "mixed": "BF16/FP16 mixed + FP32 re-rank, fixed budget",
+                       "plain": "plain fp32 (reference-exact)"}[head],
+              "topk": 100, "n_list": 512, "clusters": C, "sub_clusters": 10, "pq_dim": M,
+              "recall@100": round(H_["recall"], 4), "recall_sample": gs, "build_s": round(build_s, 1)}
+    for k in res:
+        config[k] = summary(k)
+    if "plain" in res:
+        for k in res:
+            if k != "plain":
+                config[f"{k}_vs_plain"] = round(res[k]["qps"] / res["plain"]["qps"], 3)
+    if "learned" in res:
+        config["budget_model"] = {k: round(v, 4) for k, v in res["learned"]["report"].items()}
+        config["budget_model"]["train_queries"] = args.train
+
     out = {
-        "metric": METRIC, "value": round(qps, 2), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{w.name}: {args.n} x {d_} f32 L2-normalised 512-component mixture, 3 label dims "
-                               f"(ts shift [604800,0], len shift [30,30], cat map), {args.nq} queries",
-                   "path": "plain fp32 (reference-exact)", "topk": 100, "n_list": idx.get_param("N_LIST"),
-                   "sub_lists": budget, "clusters": 512, "sub_clusters": 10, "pq_dim": M,
-                   "recall@100": round(rec, 4), "recall_sample": gs, "build_s": round(build_s, 1)},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
+        "metric": METRIC, "value": round(H_["qps"], 2), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(H_["ms"], 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16/f16 scoring, f32 re-rank" if head != "plain" else "f32",
+        "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -15,6 +15,14 @@ from typing import Optional
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libhdgrid.so")
 
+# torch ships its own HIP runtime and loads it by an unversioned name; if libhdgrid.so (which needs
+# libamdhip64.so.7) were loaded first, torch would add a second HIP/HSA runtime to the process and one of the
+# two would see no device.  Loading torch first makes the library bind to torch's runtime.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - the library also works without torch
+    pass
+
 if not os.path.exists(LIB_PATH):
     raise ImportError(
         f"hdgrid_amd: {LIB_PATH} is missing -- build it first (python -c 'import __graft_entry__ as g; g.build()')"
@@ -229,7 +237,7 @@ class SuperGridIndex:
         return out
 
     # -- learned budget (PRECISION 1, BUDGET_MODEL 1) -----------------------------------------
-    def budget_fit(self, queries, query_labels, gt_ids, levels=10, t0=4.0, s0=8.0, ratio=2 ** 0.5, target=0.95,
+    def budget_fit(self, queries, query_labels, gt_ids, levels=10, t0=0.0, s0=8.0, ratio=2 ** 0.5, target=0.95,
                    epochs=30, lr=2e-3, seed=0x4844, stream=None):
         """Label training queries with their minimal budget level and train the MLP; returns the report."""
         import numpy as np

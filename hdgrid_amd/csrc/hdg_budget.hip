@@ -274,7 +274,8 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
   B.ladS.resize(P.levels);
   for (int i = 0; i < P.levels; ++i) {
     const double f = std::pow(P.ratio, i);
-    B.ladT[i] = (uint32_t)std::min<double>(n_list, std::max(1.0, std::round(P.t0 * f)));
+    // t0 <= 0: every level probes all non-zero clusters (full counts) and only the sub-cluster budget grows
+    B.ladT[i] = P.t0 > 0 ? (uint32_t)std::min<double>(n_list, std::max(1.0, std::round(P.t0 * f))) : n_list;
     B.ladS[i] = (uint32_t)std::min<double>(sub_lists, std::max(1.0, std::round(P.s0 * f)));
   }
   // traces: recall of every training query at every level, features from one extra pass
@@ -289,7 +290,7 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
   std::vector<std::vector<float>> rec(P.levels);
   try {
     for (int i = 0; i < P.levels; ++i) {
-      cfg_.probe_clusters = B.ladT[i];
+      cfg_.probe_clusters = P.t0 > 0 ? B.ladT[i] : 0;
       cfg_.sub_lists = B.ladS[i];
       if (i == P.levels - 1) {
         B.sink = d_feat.as<float>();

@@ -277,20 +277,20 @@ std::vector<uint32_t> sample_rows(uint64_t n, uint64_t want, uint64_t seed) {
 
 }  // namespace
 
-void prepare_mixed_centers(IndexState& st, const std::vector<float>& centers, hipStream_t s) {
-  const uint64_t C = st.C, d = st.d;
-  st.Dp = (d + 31) / 32 * 32;
-  std::vector<uint16_t> b16(C * st.Dp, 0);
-  std::vector<float> norm(C, 0.0f);
-  for (uint64_t c = 0; c < C; ++c) {
+// rows [n][d] f32 -> bf16 [n][Dp] (round to nearest even, zero padded) and |row_bf16|^2 in f32.
+static void to_bf16_rows(const std::vector<float>& rows, uint64_t n, uint64_t d, uint64_t Dp, DevBuf& b16_out,
+                         DevBuf& norm_out, hipStream_t s) {
+  std::vector<uint16_t> b16(n * Dp, 0);
+  std::vector<float> norm(n, 0.0f);
+  for (uint64_t c = 0; c < n; ++c) {
     double acc = 0.0;
     for (uint64_t t = 0; t < d; ++t) {
       uint32_t u;
-      float v = centers[c * d + t];
+      float v = rows[c * d + t];
       std::memcpy(&u, &v, 4);
-      u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+      u += 0x7FFFu + ((u >> 16) & 1u);
       const uint16_t h = (uint16_t)(u >> 16);
-      b16[c * st.Dp + t] = h;
+      b16[c * Dp + t] = h;
       uint32_t back = (uint32_t)h << 16;
       float r;
       std::memcpy(&r, &back, 4);
@@ -298,8 +298,15 @@ void prepare_mixed_centers(IndexState& st, const std::vector<float>& centers, hi
     }
     norm[c] = (float)acc;
   }
-  upload(st.centers_bf16, b16, s);
-  upload(st.center_norm, norm, s);
+  upload(b16_out, b16, s);
+  upload(norm_out, norm, s);
+}
+
+void prepare_mixed_centers(IndexState& st, const std::vector<float>& centers, const std::vector<float>& sub_centers,
+                           hipStream_t s) {
+  st.Dp = (st.d + 31) / 32 * 32;
+  to_bf16_rows(centers, st.C, st.d, st.Dp, st.centers_bf16, st.center_norm, s);
+  to_bf16_rows(sub_centers, st.CS, st.d, st.Dp, st.sub_centers_bf16, st.sub_center_norm, s);
 }
 
 void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* labels, uint64_t label_cols,
@@ -469,7 +476,7 @@ void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* lab
   upload(st_.zid, st_.h_zid, s);
   upload(st_.pq_centers, pq_centers, s);
   upload(st_.centers, coarse, s);
-  prepare_mixed_centers(st_, coarse, s);
+  prepare_mixed_centers(st_, coarse, sub_centers, s);
   upload(st_.sub_centers, sub_centers, s);
   upload(st_.sub_off, st_.h_sub_off, s);
   {

@@ -189,7 +189,81 @@ __global__ __launch_bounds__(256) void budget_count_kernel(const uint32_t* __res
   }
 }
 
+// Top-k of a query's listed items by their row of scores (ascending, ties to the lower list position):
+// the mixed-precision replacement of the exact-FP32 refine + select_k used for cluster (A5) and
+// sub-cluster (A6) selection.  k = budget[q] when given (0 = k_default), clamped to k_default.
+// Optionally writes the 32 smallest listed scores (+ |q|^2) as budget-model features.
+__global__ __launch_bounds__(256) void scored_select_kernel(const float* __restrict__ queries, uint64_t q0,
+                                                            uint64_t d, const uint32_t* __restrict__ list,
+                                                            uint32_t cap, const uint32_t* __restrict__ n_list_q,
+                                                            const float* __restrict__ S, uint64_t ld,
+                                                            uint32_t k_default, const uint32_t* __restrict__ budget,
+                                                            uint32_t out_ld, uint32_t* __restrict__ chosen,
+                                                            uint32_t* __restrict__ n_chosen,
+                                                            float* __restrict__ feat_dist,
+                                                            uint64_t* __restrict__ scratch) {
+  __shared__ uint64_t keys[dev::kSortCap];
+  __shared__ uint32_t hist[2048];
+  __shared__ uint64_t sh[8];
+  __shared__ float red[kWG];
+  const uint64_t qi = blockIdx.x;
+  const uint32_t n = n_list_q[qi];
+  const uint32_t* lst = list + qi * cap;
+  const float* row = S + qi * ld;
+  uint32_t k = k_default;
+  if (budget) {
+    const uint32_t b = budget[qi];
+    if (b != 0 && b < k) k = b;
+  }
+  uint32_t m = n < k ? n : k;
+  if (n <= (uint32_t)dev::kSortCap) {
+    for (uint32_t j = threadIdx.x; j < n; j += kWG) keys[j] = ((uint64_t)ord(row[lst[j]]) << 32) | j;
+    __syncthreads();
+    dev::wg_sort_lds(keys, n);
+  } else {
+    uint64_t* g = scratch + qi * cap;
+    for (uint32_t j = threadIdx.x; j < n; j += kWG) g[j] = ((uint64_t)ord(row[lst[j]]) << 32) | j;
+    __syncthreads();
+    if (m > (uint32_t)dev::kSortCap) m = dev::kSortCap;
+    dev::wg_radix_select(g, n, m, keys, hist, sh);
+    dev::wg_sort_lds(keys, m);
+  }
+  for (uint32_t j = threadIdx.x; j < m; j += kWG) chosen[qi * out_ld + j] = lst[(uint32_t)keys[j]];
+  if (threadIdx.x == 0) n_chosen[qi] = m;
+  if (feat_dist) {
+    const float* q = queries + (q0 + qi) * d;
+    float part = 0.0f;
+    for (uint64_t t = threadIdx.x; t < d; t += kWG) part += q[t] * q[t];
+    red[threadIdx.x] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float qq = 0.0f;
+      for (int i = 0; i < kWG; ++i) qq += red[i];
+      red[0] = qq;
+    }
+    __syncthreads();
+    const uint32_t avail = n < 32 ? n : 32;
+    for (uint32_t j = threadIdx.x; j < (uint32_t)kFeatDists; j += kWG) {
+      float v = 0.0f;
+      if (avail > 0) {
+        const uint32_t o = (uint32_t)(keys[j < avail ? j : avail - 1] >> 32);
+        const uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+        v = __uint_as_float(u) + red[0];
+      }
+      feat_dist[qi * kFeatDists + j] = v;
+    }
+  }
+}
+
 }  // namespace
+
+void launch_scored_select(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, const uint32_t* list,
+                          uint32_t cap, const uint32_t* n_list_q, const float* S, uint64_t ld, uint32_t k_default,
+                          const uint32_t* budget, uint32_t out_ld, uint32_t* chosen, uint32_t* n_chosen,
+                          float* feat_dist, uint64_t* scratch, hipStream_t s) {
+  hipLaunchKernelGGL(scored_select_kernel, dim3((unsigned)qn), dim3(256), 0, s, queries, q0, d, list, cap, n_list_q, S,
+                     ld, k_default, budget, out_ld, chosen, n_chosen, feat_dist, scratch);
+}
 
 void launch_coarse_gemm(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, uint64_t Dp,
                         const uint16_t* cb16, const float* cnorm, uint32_t C, float* S, hipStream_t s) {

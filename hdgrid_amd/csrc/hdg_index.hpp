@@ -58,11 +58,14 @@ struct IndexState {
   uint64_t Dp = 0;        // d rounded up to a multiple of 32
   DevBuf centers_bf16;    // bf16 [C][Dp], zero padded
   DevBuf center_norm;     // f32 [C]: |c|^2 of the bf16-rounded center
+  DevBuf sub_centers_bf16;  // bf16 [CS][Dp]
+  DevBuf sub_center_norm;   // f32 [CS]
   bool built = false;
 };
 
-// Fills centers_bf16 / center_norm / Dp from host centers [C][d].
-void prepare_mixed_centers(IndexState& st, const std::vector<float>& centers, hipStream_t s);
+// Fills the bf16 center copies, their norms and Dp from host centers [C][d] and sub-centers [CS][d].
+void prepare_mixed_centers(IndexState& st, const std::vector<float>& centers, const std::vector<float>& sub_centers,
+                           hipStream_t s);
 
 class Index {
  public:

@@ -214,10 +214,11 @@ void Index::load(const std::string& dir, hipStream_t s) {
   }
   {
     std::vector<float> centers = read_npy<float>(dir + "/centers.npy", sh);
+    std::vector<float> subc = read_npy<float>(dir + "/sub_centers.npy", sh);
     upload(S.centers, centers, s);
-    prepare_mixed_centers(S, centers, s);
+    upload(S.sub_centers, subc, s);
+    prepare_mixed_centers(S, centers, subc, s);
   }
-  upload(S.sub_centers, read_npy<float>(dir + "/sub_centers.npy", sh), s);
   upload(S.pq_centers, read_npy<float>(dir + "/pq_centers.npy", sh), s);
   upload(S.codes, read_npy<uint8_t>(dir + "/codes.npy", sh), s);
   upload(S.cells, read_npy<uint32_t>(dir + "/cells.npy", sh), s);

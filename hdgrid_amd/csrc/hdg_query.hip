@@ -1077,8 +1077,11 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   HDG_HIP(hipGetLastError());
   mark();
   const bool mixed = cfg_.precision == 1;
-  const uint32_t* budget_T = nullptr;
-  const uint32_t* budget_S = nullptr;
+  // PRECISION 1 with PROBE_CLUSTERS > 0: BF16 coarse top-T, counts of those clusters only (lazy);
+  // PRECISION 1 with PROBE_CLUSTERS = 0: exact full counts, BF16 MFMA scores for cluster and sub-cluster choice.
+  const bool lazy = mixed && cfg_.probe_clusters > 0;
+  uint32_t* budget_T = nullptr;
+  uint32_t* budget_S = nullptr;
   if (mixed && cfg_.budget_model == 1) {
     if (!budget_model_ready()) fail(kBadState, "BUDGET_MODEL = 1 but no budget model is loaded");
     w.budget_T.reserve(qn * 4);
@@ -1087,31 +1090,10 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     budget_S = w.budget_S.as<uint32_t>();
   }
   if (mixed) {
-    // BF16 MFMA coarse scores -> per-query top-T clusters -> counts of their sub-clusters only
-    w.S.reserve(qn * C * 4);
+    w.S.reserve(qn * std::max<uint64_t>(C, CS) * 4);
     w.feat.reserve(qn * kFeatDists * 4);
-    launch_coarse_gemm(queries, q0, qn, d, S.Dp, S.centers_bf16.as<uint16_t>(), S.center_norm.as<float>(),
-                       (uint32_t)C, w.S.as<float>(), s);
-    HDG_HIP(hipGetLastError());
-    mark();
-    const uint32_t probe = budget_T ? n_list : (uint32_t)(cfg_.probe_clusters ? cfg_.probe_clusters : n_list);
-    launch_coarse_topk(queries, q0, qn, d, w.S.as<float>(), (uint32_t)C, n_list, probe, nullptr,
-                       w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>(), w.feat.as<float>(), s);
-    HDG_HIP(hipGetLastError());
-    capture_features(q0, qn, l, w.ranges.as<float>(), w.box_w.as<uint32_t>(), g.n_valid, w.feat.as<float>(), s);
-    // learned budgets: features -> MLP -> (clusters, sub-clusters) per query; trims n_chosen_c
-    if (budget_T) run_budget_model(qn, l, w.ranges.as<float>(), w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(),
-                                   g.n_valid, w.feat.as<float>(), w.n_chosen_c.as<uint32_t>(),
-                                   w.budget_T.as<uint32_t>(), w.budget_S.as<uint32_t>(), s);
-    mark();
-    launch_budget_count(g.n_valid, S.cells.as<uint32_t>(), S.N, S.zcode.as<uint64_t>(), S.seg_off.as<uint32_t>(),
-                        S.sub_off.as<uint32_t>(), CS, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(),
-                        w.zlo_cnt.as<uint64_t>(), w.zhi.as<uint64_t>(), qn, n_list, w.chosen_c.as<uint32_t>(),
-                        w.n_chosen_c.as<uint32_t>(), w.counts.as<uint32_t>(), s);
-    HDG_HIP(hipGetLastError());
-    mark();
-  } else {
-  {
+  }
+  auto launch_full_count = [&]() {
     dim3 grid((unsigned)cdiv(qn, 256), (unsigned)cdiv(CS, kCountSegChunk));
     const uint32_t* cells = S.cells.as<uint32_t>();
     const uint32_t* so = S.seg_off.as<uint32_t>();
@@ -1135,29 +1117,70 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
         fail(kBadState, "no valid label dims");
     }
     HDG_HIP(hipGetLastError());
-  }
-  mark();
-  hipLaunchKernelGGL(cluster_nnz_kernel, dim3((unsigned)qn), dim3(256), 0, s, w.counts.as<uint32_t>(), CS,
-                     S.sub_off.as<uint32_t>(), (uint32_t)C, G, q0, w.list_c.as<uint32_t>(), w.nnz_c.as<uint32_t>(),
-                     w.gstat.as<unsigned long long>());
-  HDG_HIP(hipGetLastError());
-  mark();
-  hipLaunchKernelGGL(coarse_select_kernel, dim3((unsigned)qn), dim3(256), sort_lds, s, queries, d,
-                     S.centers.as<float>(), (uint32_t)C, n_list, G, q0, w.list_c.as<uint32_t>(),
-                     w.nnz_c.as<uint32_t>(), w.gstat.as<unsigned long long>(), w.keys.as<uint64_t>(),
-                     w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>());
-  HDG_HIP(hipGetLastError());
-  mark();
+  };
+  auto budgets = [&]() {
+    capture_features(q0, qn, l, w.ranges.as<float>(), w.box_w.as<uint32_t>(), g.n_valid, w.feat.as<float>(), s);
+    if (budget_T)
+      run_budget_model(qn, l, w.ranges.as<float>(), w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), g.n_valid,
+                       w.feat.as<float>(), w.n_chosen_c.as<uint32_t>(), budget_T, budget_S, s);
+  };
+  if (lazy) {
+    launch_coarse_gemm(queries, q0, qn, d, S.Dp, S.centers_bf16.as<uint16_t>(), S.center_norm.as<float>(),
+                       (uint32_t)C, w.S.as<float>(), s);
+    HDG_HIP(hipGetLastError());
+    mark();
+    launch_coarse_topk(queries, q0, qn, d, w.S.as<float>(), (uint32_t)C, n_list, (uint32_t)cfg_.probe_clusters,
+                       nullptr, w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>(), w.feat.as<float>(), s);
+    HDG_HIP(hipGetLastError());
+    budgets();
+    mark();
+    launch_budget_count(g.n_valid, S.cells.as<uint32_t>(), S.N, S.zcode.as<uint64_t>(), S.seg_off.as<uint32_t>(),
+                        S.sub_off.as<uint32_t>(), CS, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(),
+                        w.zlo_cnt.as<uint64_t>(), w.zhi.as<uint64_t>(), qn, n_list, w.chosen_c.as<uint32_t>(),
+                        w.n_chosen_c.as<uint32_t>(), w.counts.as<uint32_t>(), s);
+    HDG_HIP(hipGetLastError());
+    mark();
+  } else {
+    launch_full_count();
+    mark();
+    hipLaunchKernelGGL(cluster_nnz_kernel, dim3((unsigned)qn), dim3(256), 0, s, w.counts.as<uint32_t>(), CS,
+                       S.sub_off.as<uint32_t>(), (uint32_t)C, G, q0, w.list_c.as<uint32_t>(), w.nnz_c.as<uint32_t>(),
+                       w.gstat.as<unsigned long long>());
+    HDG_HIP(hipGetLastError());
+    mark();
+    if (!mixed) {
+      hipLaunchKernelGGL(coarse_select_kernel, dim3((unsigned)qn), dim3(256), sort_lds, s, queries, d,
+                         S.centers.as<float>(), (uint32_t)C, n_list, G, q0, w.list_c.as<uint32_t>(),
+                         w.nnz_c.as<uint32_t>(), w.gstat.as<unsigned long long>(), w.keys.as<uint64_t>(),
+                         w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>());
+    } else {
+      launch_coarse_gemm(queries, q0, qn, d, S.Dp, S.centers_bf16.as<uint16_t>(), S.center_norm.as<float>(),
+                         (uint32_t)C, w.S.as<float>(), s);
+      launch_scored_select(queries, q0, qn, d, w.list_c.as<uint32_t>(), (uint32_t)C, w.nnz_c.as<uint32_t>(),
+                           w.S.as<float>(), C, n_list, nullptr, n_list, w.chosen_c.as<uint32_t>(),
+                           w.n_chosen_c.as<uint32_t>(), w.feat.as<float>(), w.keys.as<uint64_t>(), s);
+      budgets();
+    }
+    HDG_HIP(hipGetLastError());
+    mark();
   }
   hipLaunchKernelGGL(sub_nnz_kernel, dim3((unsigned)qn), dim3(256), 0, s, w.counts.as<uint32_t>(), CS,
                      S.sub_off.as<uint32_t>(), n_list, G, q0, w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>(),
                      cap, w.list_s.as<uint32_t>(), w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>());
   HDG_HIP(hipGetLastError());
   mark();
-  hipLaunchKernelGGL(sub_select_kernel, dim3((unsigned)qn), dim3(256), sort_lds, s, queries, d,
-                     S.sub_centers.as<float>(), sub_lists, G, q0, cap, w.list_s.as<uint32_t>(),
-                     w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>(), budget_S, w.keys.as<uint64_t>(),
-                     w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>());
+  if (mixed && !lazy) {
+    launch_coarse_gemm(queries, q0, qn, d, S.Dp, S.sub_centers_bf16.as<uint16_t>(), S.sub_center_norm.as<float>(),
+                       (uint32_t)CS, w.S.as<float>(), s);
+    launch_scored_select(queries, q0, qn, d, w.list_s.as<uint32_t>(), cap, w.nnz_s.as<uint32_t>(), w.S.as<float>(),
+                         CS, sub_lists, budget_S, sub_lists, w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(),
+                         nullptr, w.keys.as<uint64_t>(), s);
+  } else {
+    hipLaunchKernelGGL(sub_select_kernel, dim3((unsigned)qn), dim3(256), sort_lds, s, queries, d,
+                       S.sub_centers.as<float>(), sub_lists, G, q0, cap, w.list_s.as<uint32_t>(),
+                       w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>(), budget_S, w.keys.as<uint64_t>(),
+                       w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>());
+  }
   HDG_HIP(hipGetLastError());
   mark();
   hipLaunchKernelGGL(ranges_kernel, dim3((unsigned)qn), dim3(256), 0, s, S.zcode.as<uint64_t>(),
@@ -1225,7 +1248,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     HDG_HIP(hipMemcpyAsync(kp.data(), w.kprime.data(), qn * 8, hipMemcpyDeviceToHost, s));
     HDG_HIP(hipMemcpyAsync(nl.data(), w.nL.data(), qn * 4, hipMemcpyDeviceToHost, s));
     HDG_HIP(hipMemcpyAsync(mode.data(), w.mode.data(), qn * 4, hipMemcpyDeviceToHost, s));
-    HDG_HIP(hipMemcpyAsync(nc.data(), mixed ? w.n_chosen_c.data() : w.nnz_c.data(), qn * 4, hipMemcpyDeviceToHost,
+    HDG_HIP(hipMemcpyAsync(nc.data(), lazy ? w.n_chosen_c.data() : w.nnz_c.data(), qn * 4, hipMemcpyDeviceToHost,
                            s));
     HDG_HIP(hipMemcpyAsync(ns.data(), w.nnz_s.data(), qn * 4, hipMemcpyDeviceToHost, s));
     stats_.chosen_subs.resize(qn * sub_lists);
@@ -1243,11 +1266,15 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                                         "coarse_select_kernel", "sub_nnz_kernel", "sub_select_kernel",
                                         "ranges_kernel",  "plan+sync",         "keep_all_kernel",
                                         "adc_kernels",    "final_kernel"};
-    static const char* mixed_names[] = {"prep_kernel",    "coarse_gemm_kernel", "coarse_topk_kernel",
-                                        "budget_count_kernel", "sub_nnz_kernel", "sub_select_kernel",
-                                        "ranges_kernel",  "plan+sync",          "keep_all_kernel",
+    static const char* lazy_names[] = {"prep_kernel",    "coarse_gemm_kernel", "coarse_topk+budget",
+                                       "budget_count_kernel", "sub_nnz_kernel", "sub_select_kernel",
+                                       "ranges_kernel",  "plan+sync",          "keep_all_kernel",
+                                       "adc_kernels",    "final_kernel"};
+    static const char* mixed_names[] = {"prep_kernel",    "count_kernel",        "cluster_nnz_kernel",
+                                        "coarse_gemm+select+budget", "sub_nnz_kernel", "sub_gemm+select",
+                                        "ranges_kernel",  "plan+sync",           "keep_all_kernel",
                                         "adc_kernels",    "final_kernel"};
-    const char* const* names = mixed ? mixed_names : plain_names;
+    const char* const* names = lazy ? lazy_names : (mixed ? mixed_names : plain_names);
     const int nnames = 11;
     for (int i = 0; i + 1 < evn && i < nnames; ++i) {
       float ms = 0;

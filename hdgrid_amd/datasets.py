@@ -90,6 +90,11 @@ class Workload:
     qlabels: np.ndarray
     filter_arrays: Dict[str, list] = field(default_factory=dict)  # ftype, shift_val, imap, div_value
     notes: List[str] = field(default_factory=list)
+    sampler: Optional[object] = None  # sampler(n, seed) -> (queries, qlabels) from the same distribution
+
+    def sample_queries(self, n: int, seed: int):
+        """Further queries from the workload's distribution (e.g. training queries for the budget model)."""
+        return self.sampler(n, seed)
 
     def conf_text(self, **overrides) -> str:
         c = dict(self.conf)
@@ -149,7 +154,12 @@ def mixed_1m(n: int = 1_000_000, nq: int = 10_000, d: int = 384, components: int
             "filter2:\n\ttype = 1\n\tinterval_map = [1, 2, 1, 2, 3, 4, 3, 5, 4, 5]\n")
     fa = dict(ftype=[0, 0, 1], shift_val=[604800.0, 0.0, 30.0, 30.0, 0.0, 0.0],
               imap=[1.0, 2.0, 1.0, 2.0, 3.0, 4.0, 3.0, 5.0, 4.0, 5.0], div_value=[0, 0, 0])
-    return Workload("mixed-1m-d384-l3", conf, filt, data, labels, queries, qlabels, fa)
+
+    def sampler(m, s):
+        r = np.random.default_rng(s)
+        return gaussian_mixture(r, m, d, means, 0.3, normalize=True), labs(r, m)
+
+    return Workload("mixed-1m-d384-l3", conf, filt, data, labels, queries, qlabels, fa, sampler=sampler)
 
 
 def explicit_ranges(n: int = 200_000, nq: int = 2000, d: int = 128, components: int = 1024, seed: int = 0x4D6003,
