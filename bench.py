@@ -160,14 +160,11 @@ def main():
     N = args.n
     C, CS = 512, 5120
     alg = {  # algorithmic HBM bytes per launch (one launch covers all nq queries)
-        "count_kernel": float((8 + n_valid * 4) * N),
-        "final_kernel": float(st.refined.sum() * 4 * d_ + args.nq * 100 * 12),
+        "final_dist+pick": float(st.refined.sum() * 4 * d_ + args.nq * 100 * 12),
         "keep_all_kernel": float(st.cand.sum() * 4 * l_),
         "adc_kernels": float(st.adc.sum() * M),
         "sub_gemm+select": float(CS * 2 * ((d_ + 31) // 32 * 32) + args.nq * (4 * d_) + args.nq * CS * 4 * 2),
         "coarse_gemm+select+budget": float(C * 2 * ((d_ + 31) // 32 * 32) + args.nq * 4 * d_ + args.nq * C * 4 * 2),
-        "coarse_select_kernel": float(st.scored.sum() * 4 * d_),
-        "sub_select_kernel": float(st.scored.sum() * 4 * d_),
     }
     dom = max(stage_avg, key=lambda k: stage_avg[k] if k != "plan+sync" else -1)
     dom_ms = stage_avg[dom]

@@ -416,7 +416,9 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
   };
   float chosen = P.max_margin;
   double hold_rec = 0.0, hold_lev = 0.0;
-  for (float m = 0.0f; m <= P.max_margin + 1e-6f; m += 0.125f) {
+  // smallest margin (negative ones included) at which the holdout's mean recall reaches the target: the
+  // per-query labels each meet the target on their own, so the mean can afford to undershoot some
+  for (float m = -P.max_margin; m <= P.max_margin + 1e-6f; m += 0.125f) {
     double rsum = 0, lsum = 0;
     for (uint64_t t = n_train; t < n; ++t) {
       const uint32_t q = perm[t];

@@ -277,6 +277,54 @@ std::vector<uint32_t> sample_rows(uint64_t n, uint64_t want, uint64_t seed) {
 
 }  // namespace
 
+void build_bucket_grid(IndexState& st, const std::vector<uint32_t>& h_cells, hipStream_t s) {
+  const GridShape& g = st.grid;
+  const int nv = g.n_valid;
+  const uint64_t N = st.N;
+  st.Nb = 0;
+  if (nv == 0 || N == 0) return;
+  // about four points per bucket; the share is handed out from the dim with the fewest cells up, so a
+  // dim with few cells keeps one bucket per cell and leaves the rest to the wide dims
+  std::vector<int> order(nv);
+  std::iota(order.begin(), order.end(), 0);
+  for (int j = 0; j < nv; ++j) st.Mv[j] = (uint32_t)g.dims[g.valid_dims[j]];
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return st.Mv[a] < st.Mv[b]; });
+  double remain = std::max(1.0, (double)N / 4.0);
+  for (int r = 0; r < nv; ++r) {
+    const int j = order[r];
+    const double share = std::pow(remain, 1.0 / (double)(nv - r));
+    st.nb[j] = (uint32_t)std::max(1.0, std::min<double>(st.Mv[j], std::floor(share + 0.5)));
+    remain = std::max(1.0, remain / st.nb[j]);
+  }
+  st.bstride[nv - 1] = 1;
+  for (int j = nv - 2; j >= 0; --j) st.bstride[j] = st.bstride[j + 1] * st.nb[j + 1];
+  st.Nb = st.bstride[0] * st.nb[0];
+  if (st.Nb >= (1ull << 32)) fail(kUnsupported, "bucket grid too large");
+  // stable counting sort of z-pool positions by bucket
+  std::vector<uint32_t> start(st.Nb + 1, 0), bid(N);
+  for (uint64_t p = 0; p < N; ++p) {
+    uint64_t b = 0;
+    for (int j = 0; j < nv; ++j)
+      b += ((uint64_t)h_cells[(uint64_t)j * N + p] * st.nb[j] / st.Mv[j]) * st.bstride[j];
+    bid[p] = (uint32_t)b;
+    start[b + 1]++;
+  }
+  for (uint64_t b = 0; b < st.Nb; ++b) start[b + 1] += start[b];
+  std::vector<uint32_t> cur(start.begin(), start.end() - 1), ecell((uint64_t)nv * N), esub(N), epos(N);
+  uint64_t sub = 0;
+  for (uint64_t p = 0; p < N; ++p) {
+    while (st.h_seg_off[sub + 1] <= p) ++sub;
+    const uint32_t e = cur[bid[p]]++;
+    for (int j = 0; j < nv; ++j) ecell[(uint64_t)j * N + e] = h_cells[(uint64_t)j * N + p];
+    esub[e] = (uint32_t)sub;
+    epos[e] = (uint32_t)p;
+  }
+  upload(st.bstart, start, s);
+  upload(st.epos, epos, s);
+  upload(st.ecell, ecell, s);
+  upload(st.esub, esub, s);
+}
+
 // rows [n][d] f32 -> bf16 [n][Dp] (round to nearest even, zero padded) and |row_bf16|^2 in f32.
 static void to_bf16_rows(const std::vector<float>& rows, uint64_t n, uint64_t d, uint64_t Dp, DevBuf& b16_out,
                          DevBuf& norm_out, hipStream_t s) {
@@ -471,6 +519,8 @@ void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* lab
     for (int i = 0; i < l; ++i) h_zlab[p * l + i] = h_lab[(uint64_t)id * l + i];
   }
   upload(st_.cells, h_cells, s);
+  build_bucket_grid(st_, h_cells, s);
+  st_.labels_integral = std::all_of(h_lab.begin(), h_lab.end(), [](float v) { return v == std::floor(v); });
   upload(st_.lab, h_zlab, s);
   upload(st_.zcode, st_.h_zcode, s);
   upload(st_.zid, st_.h_zid, s);

@@ -226,27 +226,15 @@ hdg_status hdg_last_query_stats(const hdg_index* index, uint64_t nq, uint64_t* c
                                 uint64_t* refined, uint64_t* scored, uint64_t* adc, uint64_t* kprime) {
   return guarded([&] {
     if (!index) hdg::fail(HDG_EINVAL, "null index");
-    const hdg::QueryStats& st = index->impl->stats();
-    if (nq != st.nq) hdg::fail(HDG_EINVAL, "nq differs from the last query call");
-    auto cp = [&](uint64_t* dst, const std::vector<uint64_t>& src) {
-      if (dst) std::copy(src.begin(), src.end(), dst);
-    };
-    cp(cand, st.cand);
-    cp(real, st.real);
-    cp(refined, st.refined);
-    cp(scored, st.scored);
-    cp(adc, st.adc);
-    cp(kprime, st.kprime);
+    uint64_t* const out[6] = {cand, real, refined, scored, adc, kprime};
+    index->impl->last_stats(nq, out);
   });
 }
 
 hdg_status hdg_last_chosen_subs(const hdg_index* index, uint64_t nq, uint32_t* out) {
   return guarded([&] {
     if (!index || !out) hdg::fail(HDG_EINVAL, "null argument");
-    const auto& v = index->impl->stats().chosen_subs;
-    const uint64_t sl = index->impl->config().sub_lists;
-    if (nq * sl != v.size()) hdg::fail(HDG_EINVAL, "nq differs from the last query chunk");
-    std::copy(v.begin(), v.end(), out);
+    index->impl->last_chosen_subs(nq, out);
   });
 }
 
@@ -254,7 +242,7 @@ hdg_status hdg_last_stage_ms(const hdg_index* index, char* names, uint64_t cap, 
   return guarded([&] {
     if (!index || !n) hdg::fail(HDG_EINVAL, "null argument");
     std::string joined;
-    const auto& st = index->impl->stage_ms;
+    const auto st = index->impl->stage_times();
     for (size_t i = 0; i < st.size(); ++i) {
       if (i) joined += ",";
       joined += st[i].first;

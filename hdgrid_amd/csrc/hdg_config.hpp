@@ -48,6 +48,8 @@ struct Config {
   uint64_t precision = 0;              // PRECISION: 0 reference-exact FP32, 1 BF16 coarse + FP16 LUT
   uint64_t probe_clusters = 0;         // PROBE_CLUSTERS: clusters probed per query in PRECISION 1 (0 = N_LIST)
   uint64_t budget_model = 0;           // BUDGET_MODEL: 1 = per-query budgets from the loaded MLP
+  uint64_t sparse_count = 1;           // SPARSE_COUNT: 1 = bucket-grid walk for selective boxes, 0 = dense count
+  uint64_t sparse_max_tests = 131072;  // SPARSE_MAX_TESTS: entries a query may test on the walk before the dense count
 };
 
 // Parse with config_utils.h:33-88 semantics: `KEY = value` per line, key case-insensitive and

@@ -14,18 +14,6 @@
 
 namespace hdg {
 
-// Per-query counters of the last query call (host copies), used for algorithmic-byte accounting.
-struct QueryStats {
-  std::vector<uint64_t> cand;      // C_q: Morton-range candidates
-  std::vector<uint64_t> real;      // in-box count of the chosen sub-clusters
-  std::vector<uint64_t> refined;   // R_q: vectors re-ranked exactly
-  std::vector<uint64_t> scored;    // S_q: centers + sub-centers scored exactly
-  std::vector<uint64_t> adc;       // candidates scanned with PQ ADC (0 when PQ keeps all)
-  std::vector<uint64_t> kprime;    // PQ keep width used
-  std::vector<uint32_t> chosen_subs;  // [nq][sub_lists] (UINT32_MAX pad), last chunk only
-  uint64_t nq = 0;
-};
-
 // Device layout.  Every per-point array is in z-pool order: sub-clusters in cluster-major order
 // (Super-Grid-IVF_PQ.cu:486-504), points inside a sub-cluster ascending by (Morton code, id)
 // (:604-617).  Reading a Morton range therefore reads contiguous memory for codes, labels,
@@ -60,8 +48,25 @@ struct IndexState {
   DevBuf center_norm;     // f32 [C]: |c|^2 of the bf16-rounded center
   DevBuf sub_centers_bf16;  // bf16 [CS][Dp]
   DevBuf sub_center_norm;   // f32 [CS]
+  // bucket grid over the valid dims for enumerating a query's in-box points: bucket coordinate
+  // b_j = cell_j * nb[j] / Mv[j]; buckets row-major with the last valid dim fastest; entries sorted
+  // by (bucket, z-pool position).
+  uint32_t Mv[kMaxDims] = {0};      // cells per valid dim
+  uint32_t nb[kMaxDims] = {0};      // buckets per valid dim
+  uint64_t bstride[kMaxDims] = {0};
+  uint64_t Nb = 0;
+  DevBuf bstart;      // u32 [Nb+1]
+  DevBuf ecell;       // u32 [n_valid][N] cells of the entries
+  DevBuf esub;        // u32 [N] sub-cluster (table position) of the entries
+  DevBuf epos;        // u32 [N] z-pool position of the entries
+  // every label value is an integer: then a point passing the exact label predicate lies in the query's
+  // cell box, so the in-box points of the chosen sub-clusters are a superset of the keep-all survivors
+  bool labels_integral = false;
   bool built = false;
 };
+
+// Bucket grid from the z-pool-ordered cells ([n_valid][N]) and sub-cluster offsets; sets the bucket fields.
+void build_bucket_grid(IndexState& st, const std::vector<uint32_t>& h_cells, hipStream_t s);
 
 // Fills the bf16 center copies, their norms and Dp from host centers [C][d] and sub-centers [CS][d].
 void prepare_mixed_centers(IndexState& st, const std::vector<float>& centers, const std::vector<float>& sub_centers,
@@ -88,11 +93,16 @@ class Index {
   Config& config() { return cfg_; }
   const FilterConfig& filter() const { return fc_; }
   const IndexState& state() const { return st_; }
-  const QueryStats& stats() const { return stats_; }
   int device() const { return device_; }
 
-  // timing of the last query call per stage (ms, host-side event timing), for profiling
-  std::vector<std::pair<std::string, float>> stage_ms;
+  // Per-query counters of the last query call, columns (C_q Morton-range candidates, in-box count of
+  // the chosen sub-clusters, vectors re-ranked exactly, centers + sub-centers scored, candidates scanned
+  // with PQ ADC, PQ keep width); out[c] may be null.  Synchronises with the device.
+  void last_stats(uint64_t nq, uint64_t* const out[6]) const;
+  // Chosen sub-cluster ids of the last query call, [nq][sub_lists], UINT32_MAX past each query's end.
+  void last_chosen_subs(uint64_t nq, uint32_t* out) const;
+  // Per-stage device time (ms, HIP events) of the last query chunk, for profiling.
+  std::vector<std::pair<std::string, float>> stage_times() const;
 
   // learned per-query budget (hdg_budget.hip)
   bool budget_model_ready() const;
@@ -121,7 +131,6 @@ class Index {
   FilterConfig fc_;
   int device_;
   IndexState st_;
-  QueryStats stats_;
   // device copies of the filter configuration
   DevBuf f_type_, f_shift_, f_map_, f_div_;
   uint64_t f_map_len_ = 0;

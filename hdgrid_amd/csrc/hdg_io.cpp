@@ -3,6 +3,8 @@
 // resumed search and the CPU checker both read.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -221,9 +223,18 @@ void Index::load(const std::string& dir, hipStream_t s) {
   }
   upload(S.pq_centers, read_npy<float>(dir + "/pq_centers.npy", sh), s);
   upload(S.codes, read_npy<uint8_t>(dir + "/codes.npy", sh), s);
-  upload(S.cells, read_npy<uint32_t>(dir + "/cells.npy", sh), s);
+  {
+    std::vector<uint32_t> cells = read_npy<uint32_t>(dir + "/cells.npy", sh);
+    if (cells.size() != (uint64_t)g.n_valid * S.N) fail(kIoError, "inconsistent cells array in " + dir);
+    upload(S.cells, cells, s);
+    build_bucket_grid(S, cells, s);
+  }
   upload(S.vec, read_npy<float>(dir + "/vec.npy", sh), s);
-  upload(S.lab, read_npy<float>(dir + "/lab.npy", sh), s);
+  {
+    std::vector<float> lab = read_npy<float>(dir + "/lab.npy", sh);
+    S.labels_integral = std::all_of(lab.begin(), lab.end(), [](float v) { return v == std::floor(v); });
+    upload(S.lab, lab, s);
+  }
   S.built = true;
   st_ = std::move(S);
 }

@@ -1,14 +1,17 @@
 // Super-Grid query path on the GPU: Super-Grid-IVF_PQ.cu:985-1248 (index::query) and the kernels it
 // reaches, redesigned for gfx950.  Stage map (reference row -> kernel here):
 //   A1 filter recovery + A2 grid boxes + A7 Morton corners  -> prep_kernel
-//   A3 in-box counts per (query, sub-cluster)               -> count_kernel (streams packed cells)
-//   A4/A5 non-zero clusters + exact coarse top-n_list       -> cluster_nnz_kernel, coarse_select_kernel
-//   A6 non-zero sub-clusters + exact top-sub_lists          -> sub_nnz_kernel, sub_select_kernel
+//   A3/A4 in-box counts, non-zero sub-clusters and clusters -> sparse_count_kernel (bucket-grid walk);
+//                                                              dense_count_kernel + dense_emit_kernel for
+//                                                              the boxes too wide for the walk
+//   A5 exact coarse top-n_list                              -> pair_dist_kernel + coarse_pick_kernel
+//   A6 non-zero sub-clusters of the chosen clusters, exact
+//      top-sub_lists                                        -> sub_expand_kernel, pair_dist_kernel + sub_pick_kernel
 //   A8/A9 Morton ranges, candidate counts, beta sums        -> ranges_kernel
 //   A10 beta / n_candies, buffer plan                       -> plan_kernel + scan
 //   A11 PQ ADC scan + top-k' (only where C_q > k')          -> adc_scan_kernel, adc_select_kernel
 //   A12 label post-filter (l > 1)                           -> inside keep_all / adc_select
-//   A13 exact FP32 re-rank + final top-k                    -> final_kernel
+//   A13 exact FP32 re-rank + final top-k                    -> pair_dist_kernel + final_pick_kernel
 // Arithmetic that decides integer results (cells, Morton codes, counts) and the FP32 distances follow
 // the reference's operation order; the file is compiled with -ffp-contract=off and every FMA that the
 // reference's compiler contracts is an explicit fmaf().
@@ -17,6 +20,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 
 #include <hip/hip_fp16.h>
@@ -35,8 +39,7 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr float kExactPad = 1e6f;               // refine.cu:53
 constexpr uint64_t kIdPad = 4294967296ull;      // mat_operators.cuh:185
 constexpr uint64_t kU64Max = ~0ull;             // filter.cu:171
-constexpr int kCountSegTile = 32;               // segments per LDS flush in count_kernel
-constexpr int kCountSegChunk = 64;              // segments per count_kernel block
+constexpr int kCountSegChunk = 64;              // segments per dense_count_kernel block
 constexpr uint32_t kAdcChunk = 8192;            // candidates per adc_scan block
 
 struct Dims {
@@ -65,6 +68,17 @@ Dims make_dims(const GridShape& g) {
 }
 
 __host__ __device__ inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Every query of the chunk goes to the dense count (no bucket grid, or too many sub-clusters for LDS).
+__global__ void iota_kernel(uint64_t qn, uint32_t* __restrict__ dense_list, uint32_t* __restrict__ n_dense,
+                            uint32_t* __restrict__ n_ib) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < qn) {
+    dense_list[i] = (uint32_t)i;
+    n_ib[i] = 0xFFFFFFFFu;
+  }
+  if (i == 0) *n_dense = (uint32_t)qn;
+}
 
 // ------------------------------------------------------------------------------------------------
 // A1 + A2 + A7.  One thread per query.  Query qg belongs to group g = qg / G; the recovery kernel of
@@ -152,29 +166,261 @@ __global__ void prep_kernel(const float* __restrict__ qlabels, uint64_t qcols, u
 }
 
 // ------------------------------------------------------------------------------------------------
-// A3: counts[q][s] = #points of sub-cluster s whose cells lie in q's box on every valid dim.  This is
-// the integer the reference obtains by 2^l-corner inclusion-exclusion over compressed prefix sums
-// (Super-Grid-IVF_PQ.cu:648-790).  Here it is counted directly: a point inside the box has a Morton
-// code in [z(lo), z(hi)] (the bit-interleave is monotone in every coordinate), and a sub-cluster's
-// points are sorted by Morton code, so only that code range of each segment is box-tested.
-// Block = 256 lanes = 256 queries; blockIdx.y = chunk of kCountSegChunk segments.  Each segment is
+// A3/A4 (sparse).  The integer the reference obtains per (query, sub-cluster) by 2^l-corner
+// inclusion-exclusion over compressed prefix sums (Super-Grid-IVF_PQ.cu:648-790) is the number of the
+// sub-cluster's points whose cells lie in the query's box.  A selective box holds few points, so they
+// are enumerated through the bucket grid: the box covers a block of buckets whose rows along the last
+// valid dim are contiguous entry ranges; every entry in them is box-tested on its cells and counted
+// into a per-wave LDS histogram over sub-clusters.  The non-zero sub-clusters are then emitted in
+// ascending table order with their counts, and the non-zero clusters (filter.cu:242-260 over
+// get_cluster_sum_kernel's sums) fall out as the runs of cluster_of_sub.  A query whose box spans more
+// than kMaxCombos bucket rows or max_tests entries is handed to the dense count instead.
+// One wave per workgroup, persistent over queries; LDS = hist[CS] + bit set[CS/32].
+struct Buckets {
+  uint32_t M[kMaxDims], nb[kMaxDims];
+  uint64_t stride[kMaxDims];
+};
+
+constexpr uint64_t kMaxCombos = 4096;
+constexpr uint32_t kInboxCap = 4096;  // in-box points listed per walked query
+constexpr uint32_t kSparseMaxCS = 8192;  // LDS histogram: (CS + CS/32) words per wave
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+template <int NV>
+__global__ __launch_bounds__(64) void sparse_count_kernel(
+    Buckets B, const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ ecell,
+    const uint32_t* __restrict__ esub, uint64_t N, const uint32_t* __restrict__ cluster_of_sub, uint32_t CS,
+    uint32_t C, const uint32_t* __restrict__ box_lo, const uint32_t* __restrict__ box_w, uint64_t qn, uint64_t G,
+    uint64_t q0, uint32_t max_tests, const uint32_t* __restrict__ epos, uint32_t ibcap,
+    uint32_t* __restrict__ ib_pos, uint32_t* __restrict__ ib_sub, uint32_t* __restrict__ n_ib,
+    uint32_t* __restrict__ sub_list, uint32_t* __restrict__ sub_cnt,
+    uint32_t* __restrict__ ns_out, uint32_t* __restrict__ list_c, uint32_t* __restrict__ cl_beg,
+    uint32_t* __restrict__ nnz_c, uint32_t* __restrict__ dense_list, uint32_t* __restrict__ n_dense,
+    unsigned long long* __restrict__ gstat) {
+  extern __shared__ __align__(16) uint32_t sm[];
+  uint32_t* hist = sm;       // [CS]
+  uint32_t* bits = sm + CS;  // [nw]
+  __shared__ uint32_t pre_sh[64], e0_sh[64];
+  const uint32_t nw = (CS + 31) >> 5;
+  const int lane = threadIdx.x;
+  for (uint32_t i = lane; i < CS; i += 64) hist[i] = 0;
+  for (uint32_t i = lane; i < nw; i += 64) bits[i] = 0;
+  __syncthreads();
+  for (uint64_t qi = blockIdx.x; qi < qn; qi += gridDim.x) {
+    uint32_t lo[NV], w[NV], blo[NV], bhi[NV];
+    bool empty = false;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      lo[j] = box_lo[(uint64_t)j * qn + qi];
+      w[j] = box_w[(uint64_t)j * qn + qi];
+      // the box test (cell - lo) mod 2^32 <= w admits exactly the cells in [max(lo, 0), lo + w]
+      const int lw = (int)lo[j], h = (int)(lo[j] + w[j]);
+      const int a = lw < 0 ? 0 : lw;
+      if (h < a) empty = true;
+      blo[j] = h < a ? 0u : (uint32_t)((uint64_t)a * B.nb[j] / B.M[j]);
+      bhi[j] = h < a ? 0u : (uint32_t)((uint64_t)h * B.nb[j] / B.M[j]);
+    }
+    uint64_t combos = 1;
+#pragma unroll
+    for (int j = 0; j + 1 < NV; ++j) combos *= (uint64_t)(bhi[j] - blo[j] + 1);
+    bool to_dense = !empty && combos > kMaxCombos;
+    uint32_t ibn = 0;  // in-box points listed so far (in walk order)
+    if (!empty && !to_dense) {
+      uint64_t tests = 0;
+      for (uint64_t c0 = 0; c0 < combos; c0 += 64) {
+        const uint64_t c = c0 + (uint64_t)lane;
+        uint32_t e0 = 0, len = 0;
+        if (c < combos) {
+          uint64_t rem = c, base = 0;
+#pragma unroll
+          for (int j = NV - 2; j >= 0; --j) {
+            const uint64_t wj = bhi[j] - blo[j] + 1;
+            base += (uint64_t)(blo[j] + (uint32_t)(rem % wj)) * B.stride[j];
+            rem /= wj;
+          }
+          e0 = bstart[base + blo[NV - 1]];
+          len = bstart[base + bhi[NV - 1] + 1] - e0;
+        }
+        const uint32_t incl = wave_incl_scan(len, lane);
+        const uint32_t total = __shfl(incl, 63, 64);
+        pre_sh[lane] = incl - len;
+        e0_sh[lane] = e0;
+        __syncthreads();
+        tests += total;
+        if (tests > max_tests) {
+          to_dense = true;
+          break;
+        }
+        // four entries per lane per pass, so their loads are in flight together
+        for (uint32_t t0 = 0; t0 < total; t0 += 256) {
+          uint32_t ev[4], cv[4][NV], sv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t t = t0 + (uint32_t)lane + 64u * u;
+            uint32_t a = 0, b = 64;  // last k with pre_sh[k] <= t; that lane's range holds t
+            while (b - a > 1) {
+              const uint32_t m = (a + b) >> 1;
+              if (pre_sh[m] <= t) a = m; else b = m;
+            }
+            ev[u] = t < total ? e0_sh[a] + (t - pre_sh[a]) : kNone;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const bool ok = ev[u] != kNone;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) cv[u][j] = ok ? ecell[(uint64_t)j * N + ev[u]] : lo[j] - 1u;
+            sv[u] = ok ? esub[ev[u]] : 0u;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            bool in = ev[u] != kNone;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) in = in && (cv[u][j] - lo[j] <= w[j]);
+            const uint64_t mask = __ballot(in);
+            const uint32_t at = ibn + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            ibn += (uint32_t)__popcll(mask);
+            if (in) {
+              const uint32_t s2 = sv[u];
+              if (atomicAdd(&hist[s2], 1u) == 0u) atomicOr(&bits[s2 >> 5], 1u << (s2 & 31));
+              if (at < ibcap) {
+                ib_pos[qi * ibcap + at] = epos[ev[u]];
+                ib_sub[qi * ibcap + at] = s2;
+              }
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+    if (to_dense) {
+      for (uint32_t wi = lane; wi < nw; wi += 64) {
+        uint32_t word = bits[wi];
+        while (word) {
+          const uint32_t b = __ffs(word) - 1;
+          word &= word - 1;
+          hist[wi * 32 + b] = 0;
+        }
+        bits[wi] = 0;
+      }
+      if (lane == 0) {
+        dense_list[atomicAdd(n_dense, 1u)] = (uint32_t)qi;
+        n_ib[qi] = kNone;
+      }
+      __syncthreads();
+      continue;
+    }
+    // emit the non-zero sub-clusters ascending, and the cluster runs among them
+    uint32_t* sl = sub_list + qi * CS;
+    uint32_t* sc = sub_cnt + qi * CS;
+    uint32_t* lc = list_c + qi * C;
+    uint32_t* cb = cl_beg + qi * (C + 1);
+    uint32_t base = 0, ncl = 0;
+    int carry = -1;  // last emitted sub-cluster
+    for (uint32_t w0 = 0; w0 < nw; w0 += 64) {
+      const uint32_t wi = w0 + (uint32_t)lane;
+      const uint32_t word = wi < nw ? bits[wi] : 0u;
+      const uint32_t pc = __popc(word);
+      const uint32_t incl = wave_incl_scan(pc, lane);
+      const uint32_t tot = __shfl(incl, 63, 64);
+      int mine = word ? (int)(wi * 32 + 31 - __clz(word)) : -1;
+      int mx = mine;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(mx, o, 64);
+        if (lane >= o) mx = mx > y ? mx : y;
+      }
+      int prev = __shfl_up(mx, 1, 64);
+      if (lane == 0) prev = -1;
+      prev = prev > carry ? prev : carry;
+      const int prev_cl = prev >= 0 ? (int)cluster_of_sub[prev] : -1;
+      uint32_t starts = 0;
+      {
+        uint32_t wv = word;
+        int pcl = prev_cl;
+        while (wv) {
+          const uint32_t b = __ffs(wv) - 1;
+          wv &= wv - 1;
+          const int cl = (int)cluster_of_sub[wi * 32 + b];
+          starts += cl != pcl;
+          pcl = cl;
+        }
+      }
+      const uint32_t sincl = wave_incl_scan(starts, lane);
+      const uint32_t stot = __shfl(sincl, 63, 64);
+      uint32_t pos = base + incl - pc, cpos = ncl + sincl - starts;
+      {
+        uint32_t wv = word;
+        int pcl = prev_cl;
+        while (wv) {
+          const uint32_t b = __ffs(wv) - 1;
+          wv &= wv - 1;
+          const uint32_t s = wi * 32 + b;
+          const int cl = (int)cluster_of_sub[s];
+          if (cl != pcl) {
+            lc[cpos] = (uint32_t)cl;
+            cb[cpos] = pos;
+            ++cpos;
+          }
+          pcl = cl;
+          sl[pos] = s;
+          sc[pos] = hist[s];
+          hist[s] = 0;
+          ++pos;
+        }
+      }
+      if (wi < nw) bits[wi] = 0;
+      base += tot;
+      ncl += stot;
+      const int lastmx = __shfl(mx, 63, 64);
+      carry = lastmx > carry ? lastmx : carry;
+    }
+    if (lane == 0) {
+      ns_out[qi] = base;
+      nnz_c[qi] = ncl;
+      cb[ncl] = base;
+      n_ib[qi] = ibn <= ibcap ? ibn : kNone;
+      atomicMax(&gstat[((q0 + qi) / G) * 8 + 0], (unsigned long long)ncl);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// A3 (dense), for the queries the sparse pass handed over: a point inside the box has a Morton code
+// in [z(lo), z(hi)] (the bit interleave is monotone in every coordinate) and a sub-cluster's points
+// are sorted by Morton code, so only that code range of each segment is box-tested.  Block = 256
+// lanes = 256 handed-over queries; blockIdx.y = chunk of kCountSegChunk segments.  Each segment is
 // staged in LDS in pieces of kCountPiece points (codes + cells); every lane binary-searches its own
-// code range in LDS and tests the points inside it.
+// code range in LDS and tests the points inside it.  counts[slot][s], slot = position in dense_list.
 constexpr int kCountPiece = 1024;
 
 template <int NV>
-__global__ __launch_bounds__(256) void count_kernel(const uint32_t* __restrict__ cells, uint64_t N,
-                                                    const uint64_t* __restrict__ zcode,
-                                                    const uint32_t* __restrict__ seg_off, uint64_t CS,
-                                                    const uint32_t* __restrict__ box_lo,
-                                                    const uint32_t* __restrict__ box_w,
-                                                    const uint64_t* __restrict__ zlo,
-                                                    const uint64_t* __restrict__ zhi, uint64_t qn,
-                                                    uint32_t* __restrict__ counts) {
+__global__ __launch_bounds__(256) void dense_count_kernel(const uint32_t* __restrict__ cells, uint64_t N,
+                                                          const uint64_t* __restrict__ zcode,
+                                                          const uint32_t* __restrict__ seg_off, uint64_t CS,
+                                                          const uint32_t* __restrict__ box_lo,
+                                                          const uint32_t* __restrict__ box_w,
+                                                          const uint64_t* __restrict__ zlo,
+                                                          const uint64_t* __restrict__ zhi, uint64_t qn,
+                                                          const uint32_t* __restrict__ dense_list,
+                                                          const uint32_t* __restrict__ n_dense,
+                                                          uint32_t* __restrict__ counts) {
   __shared__ uint64_t zs[kCountPiece];
   __shared__ uint32_t cs[NV][kCountPiece];
-  const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  const bool live = q < qn;
+  const uint32_t nd = *n_dense;
+  const uint64_t slot = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if ((uint64_t)blockIdx.x * 256 >= nd) return;  // uniform per block
+  const bool live = slot < nd;
+  const uint64_t q = live ? dense_list[slot] : 0;
   uint32_t lo[NV], w[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
@@ -214,247 +460,361 @@ __global__ __launch_bounds__(256) void count_kernel(const uint32_t* __restrict__
         cnt += in ? 1u : 0u;
       }
     }
-    if (live) counts[q * CS + s] = cnt;
+    if (live) counts[slot * CS + s] = cnt;
+  }
+}
+
+// Non-zero sub-clusters (ascending, with counts) and non-zero clusters of a densely counted query.
+// Phase 1 overwrites each count with the number of non-zero sub-clusters before it, which phase 2
+// reads at the cluster boundaries.  One workgroup per handed-over query.
+__global__ __launch_bounds__(256) void dense_emit_kernel(uint32_t* __restrict__ counts, uint32_t CS,
+                                                         const uint32_t* __restrict__ sub_off, uint32_t C,
+                                                         uint64_t G, uint64_t q0,
+                                                         const uint32_t* __restrict__ dense_list,
+                                                         const uint32_t* __restrict__ n_dense,
+                                                         uint32_t* __restrict__ sub_list, uint32_t* __restrict__ sub_cnt,
+                                                         uint32_t* __restrict__ ns_out, uint32_t* __restrict__ list_c,
+                                                         uint32_t* __restrict__ cl_beg, uint32_t* __restrict__ nnz_c,
+                                                         unsigned long long* __restrict__ gstat) {
+  __shared__ uint32_t tmp[8];
+  const uint64_t slot = blockIdx.x;
+  if (slot >= *n_dense) return;
+  const uint64_t q = dense_list[slot];
+  uint32_t* row = counts + slot * CS;
+  uint32_t base = 0;
+  for (uint32_t b = 0; b < CS; b += kWG) {
+    const uint32_t s = b + threadIdx.x;
+    const uint32_t c = s < CS ? row[s] : 0u;
+    uint32_t total;
+    const uint32_t pos = dev::wg_exclusive_scan(c != 0 ? 1u : 0u, tmp, total);
+    if (c != 0) {
+      sub_list[q * CS + base + pos] = s;
+      sub_cnt[q * CS + base + pos] = c;
+    }
+    if (s < CS) row[s] = base + pos;
+    base += total;
+  }
+  __syncthreads();
+  const uint32_t ns = base;
+  uint32_t nc = 0;
+  for (uint32_t b = 0; b < C; b += kWG) {
+    const uint32_t c = b + threadIdx.x;
+    uint32_t beg = 0, end = 0;
+    if (c < C) {
+      const uint32_t s0 = sub_off[c], s1 = sub_off[c + 1];
+      beg = s0 < CS ? row[s0] : ns;
+      end = s1 < CS ? row[s1] : ns;
+    }
+    const uint32_t flag = end > beg ? 1u : 0u;
+    uint32_t total;
+    const uint32_t pos = dev::wg_exclusive_scan(flag, tmp, total);
+    if (flag) {
+      list_c[q * C + nc + pos] = c;
+      cl_beg[q * (C + 1) + nc + pos] = beg;
+    }
+    nc += total;
+  }
+  if (threadIdx.x == 0) {
+    ns_out[q] = ns;
+    nnz_c[q] = nc;
+    cl_beg[q * (C + 1) + nc] = ns;
+    atomicMax(&gstat[((q0 + q) / G) * 8 + 0], (unsigned long long)nc);
   }
 }
 
 // ------------------------------------------------------------------------------------------------
-// Exact FP32 L2 of the workgroup's query against up to 256 rows (one per thread), the reference's
-// order for d <= 512 (refine.cu:36-75): a sequential f32 sum over dims of rounded squares.
-// rows[] in LDS holds row indices (kNone = no row).  stage: 256 x 33 floats of LDS.
-__device__ float exact_tile(const float* __restrict__ X, uint64_t d, const uint32_t* rows,
-                            const float* qv, float* stage) {
-  const uint32_t my = rows[threadIdx.x];
-  float acc = 0.0f;
-  for (uint64_t t0 = 0; t0 < d; t0 += 32) {
-    const uint32_t w = (uint32_t)(d - t0 < 32 ? d - t0 : 32);
-    if ((d & 3) == 0) {
-      // 8 lanes per row: each wave load instruction reads 8 whole 128-byte row chunks
-#pragma unroll
-      for (int it = 0; it < 8; ++it) {
-        const uint32_t e = threadIdx.x + it * kWG;
-        const uint32_t r = e >> 3, c4 = (e & 7) * 4;
-        const uint32_t row = rows[r];
-        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (row != kNone && c4 < w) v = *reinterpret_cast<const float4*>(X + (uint64_t)row * d + t0 + c4);
-        float* st = stage + r * 33 + c4;
-        st[0] = v.x;
-        st[1] = v.y;
-        st[2] = v.z;
-        st[3] = v.w;
-      }
-    } else {
-      for (uint32_t e = threadIdx.x; e < 256u * 32u; e += kWG) {
-        const uint32_t r = e >> 5, c = e & 31;
-        const uint32_t row = rows[r];
-        stage[r * 33 + c] = (row != kNone && c < w) ? X[(uint64_t)row * d + t0 + c] : 0.0f;
-      }
-    }
-    __syncthreads();
-    if (my != kNone) {
-      for (uint32_t c = 0; c < w; ++c) {
-        float diff = qv[t0 + c] - stage[threadIdx.x * 33 + c];
-        float sq = diff * diff;
-        acc = acc + sq;
-      }
-    }
-    __syncthreads();
+// Exact FP32 L2 of listed rows, the reference's order for d <= 512 (refine.cu:36-75): a sequential f32
+// sum over dims of rounded squares, one lane per row.  Work = (query, 64 consecutive list entries);
+// woff[q] = chunks before query q (chunk_scan_kernel), woff[qn] = total.  One wave per workgroup; each
+// workgroup takes a contiguous run of chunks, so the query vector is staged once per query.  Rows pass
+// through LDS 64 dims at a time with 16 lanes per row (each wave load reads four whole 256-byte row
+// pieces); the next stage's loads are in flight while the current stage is summed from LDS.
+// KIND 0: rows = list[q * lstride + j], key tie = j.  KIND 1: rows = list[loff[q] + j] (z-pool
+// positions), key tie = zid[row].
+constexpr int kDistStage = 64;
+constexpr int kStageLd = 68;  // LDS row stride in floats (16-byte aligned rows)
+
+__device__ __forceinline__ uint64_t owner_of(const uint64_t* __restrict__ woff, uint64_t qn, uint64_t k) {
+  uint64_t a = 0, b = qn;  // last q with woff[q] <= k; that query owns chunk k
+  while (b - a > 1) {
+    const uint64_t m = (a + b) >> 1;
+    if (woff[m] <= k) a = m; else b = m;
   }
+  return a;
+}
+
+__device__ __forceinline__ float sq_chain4(float acc, float4 qq, float4 x) {
+  float diff = qq.x - x.x;
+  float sq = diff * diff;
+  acc = acc + sq;
+  diff = qq.y - x.y;
+  sq = diff * diff;
+  acc = acc + sq;
+  diff = qq.z - x.z;
+  sq = diff * diff;
+  acc = acc + sq;
+  diff = qq.w - x.w;
+  sq = diff * diff;
+  acc = acc + sq;
   return acc;
 }
 
-// Sort keys[0..n) (global scratch) ascending into `sorted` (LDS when n <= kSortCap) or, for larger
-// n, select the k smallest and sort those.  Returns how many sorted keys are available (min(n,k)).
-__device__ uint32_t select_sorted(uint64_t* __restrict__ g, uint32_t n, uint32_t k, uint64_t* lds,
+template <int KIND>
+__global__ __launch_bounds__(64) void pair_dist_kernel(const float* __restrict__ queries, uint64_t q0, uint64_t d,
+                                                       const float* __restrict__ X, const uint64_t* __restrict__ woff,
+                                                       uint64_t qn, const uint32_t* __restrict__ list,
+                                                       uint64_t lstride, const uint64_t* __restrict__ loff,
+                                                       const uint32_t* __restrict__ nrows,
+                                                       const uint32_t* __restrict__ zid, uint64_t* __restrict__ keys) {
+  __shared__ __align__(16) float qv[512];
+  __shared__ __align__(16) float stage[64 * kStageLd];
+  __shared__ uint32_t rows[64];
+  const int lane = threadIdx.x;
+  const uint64_t total = woff[qn];
+  const uint64_t per = (total + gridDim.x - 1) / gridDim.x;
+  uint64_t k = (uint64_t)blockIdx.x * per;
+  const uint64_t kend = k + per < total ? k + per : total;
+  if (k >= kend) return;
+  uint64_t q = owner_of(woff, qn, k);
+  uint64_t cur = ~0ull;
+  const bool vec4 = (d & 3) == 0;
+  for (; k < kend; ++k) {
+    while (woff[q + 1] <= k) ++q;
+    const uint64_t chunk = k - woff[q];
+    const uint64_t lb = KIND == 1 ? loff[q] : q * lstride;
+    const uint32_t n = nrows[q];
+    if (q != cur) {
+      for (uint64_t t = lane; t < d; t += 64) qv[t] = queries[(q0 + q) * d + t];
+      cur = q;
+    }
+    const uint32_t j = (uint32_t)chunk * 64 + lane;
+    const uint32_t myrow = j < n ? list[lb + j] : kNone;
+    rows[lane] = myrow;
+    __syncthreads();
+    float acc = 0.0f;
+    if (vec4) {
+      float4 pre[16];
+      auto load_stage = [&](uint64_t t0) {
+#pragma unroll
+        for (int it = 0; it < 16; ++it) {
+          const uint32_t e = lane + it * 64;
+          const uint32_t r = e >> 4, c4 = (e & 15) * 4;
+          const uint32_t row = rows[r];
+          pre[it] = (row != kNone && t0 + c4 < d) ? *reinterpret_cast<const float4*>(X + (uint64_t)row * d + t0 + c4)
+                                                  : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+      };
+      load_stage(0);
+      for (uint64_t t0 = 0; t0 < d; t0 += kDistStage) {
+#pragma unroll
+        for (int it = 0; it < 16; ++it) {
+          const uint32_t e = lane + it * 64;
+          *reinterpret_cast<float4*>(stage + (e >> 4) * kStageLd + (e & 15) * 4) = pre[it];
+        }
+        __syncthreads();
+        if (t0 + kDistStage < d) load_stage(t0 + kDistStage);
+        if (myrow != kNone) {
+          const float4* s4 = reinterpret_cast<const float4*>(stage + lane * kStageLd);
+          const float4* q4 = reinterpret_cast<const float4*>(qv + t0);
+          const uint32_t n4 = (uint32_t)((d - t0 < (uint64_t)kDistStage ? d - t0 : (uint64_t)kDistStage) / 4);
+          if (n4 == 16) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) acc = sq_chain4(acc, q4[c], s4[c]);
+          } else {
+            for (uint32_t c = 0; c < n4; ++c) acc = sq_chain4(acc, q4[c], s4[c]);
+          }
+        }
+        __syncthreads();
+      }
+    } else {
+      for (uint64_t t0 = 0; t0 < d; t0 += kDistStage) {
+        const uint32_t wdt = (uint32_t)(d - t0 < (uint64_t)kDistStage ? d - t0 : (uint64_t)kDistStage);
+        for (uint32_t e = lane; e < 64u * kDistStage; e += 64) {
+          const uint32_t r = e / kDistStage, c = e % kDistStage;
+          const uint32_t row = rows[r];
+          stage[r * kStageLd + c] = (row != kNone && c < wdt) ? X[(uint64_t)row * d + t0 + c] : 0.0f;
+        }
+        __syncthreads();
+        if (myrow != kNone) {
+          const float* st = stage + lane * kStageLd;
+          for (uint32_t c = 0; c < wdt; ++c) {
+            const float diff = qv[t0 + c] - st[c];
+            const float sq = diff * diff;
+            acc = acc + sq;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (myrow != kNone) keys[lb + j] = dev::make_key(acc, KIND == 1 ? zid[myrow] : j);
+  }
+}
+
+// woff[q] = sum over q' < q of ceil(n[q'] / 64) (0 for queries whose mode equals skip_mode), woff[qn] =
+// total.  One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void chunk_scan_kernel(const uint32_t* __restrict__ n, const uint32_t* __restrict__ mode,
+                                                          uint32_t skip_mode, uint64_t qn, uint64_t* __restrict__ woff) {
+  __shared__ unsigned long long wsum[16];
+  __shared__ unsigned long long carry_sh;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry_sh = 0;
+  __syncthreads();
+  for (uint64_t b = 0; b < qn; b += 1024) {
+    const uint64_t i = b + threadIdx.x;
+    unsigned long long x = 0;
+    if (i < qn && !(mode && mode[i] == skip_mode)) x = (n[i] + 63u) / 64u;
+    unsigned long long v = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned long long y = __shfl_up(v, o, 64);
+      if (lane >= o) v += y;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    unsigned long long before = carry_sh, tot = 0;
+    for (int w2 = 0; w2 < 16; ++w2) {
+      if (w2 < wave) before += wsum[w2];
+      tot += wsum[w2];
+    }
+    if (i < qn) woff[i] = before + v - x;
+    __syncthreads();
+    if (threadIdx.x == 0) carry_sh += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) woff[qn] = carry_sh;
+}
+
+// Sort keys[0..n) (global scratch) ascending into `lds` (capacity `cap` keys, a power of two) when n <= cap,
+// or select the k smallest and sort those (k <= cap).  Returns how many sorted keys are available (min(n,k)).
+__device__ uint32_t select_sorted(uint64_t* __restrict__ g, uint32_t n, uint32_t k, uint64_t* lds, uint32_t cap,
                                   uint32_t* hist, uint64_t* sh) {
   uint32_t m = n < k ? n : k;
-  if (n <= (uint32_t)dev::kSortCap) {
+  if (n <= cap) {
     for (uint32_t i = threadIdx.x; i < n; i += kWG) lds[i] = g[i];
     __syncthreads();
     dev::wg_sort_lds(lds, n);
     return m;
   }
-  if (m > (uint32_t)dev::kSortCap) m = dev::kSortCap;  // callers never ask for more
+  if (m > cap) m = cap;  // callers size cap >= k
   dev::wg_radix_select(g, n, m, lds, hist, sh);
   dev::wg_sort_lds(lds, m);
   return m;
 }
 
-// ------------------------------------------------------------------------------------------------
-// A4: non-zero clusters of each query, ascending (filter.cu:242-260 over get_cluster_sum_kernel's
-// int sums, Super-Grid-IVF_PQ.cu:767-790).  One workgroup per query.
-__global__ __launch_bounds__(256) void cluster_nnz_kernel(const uint32_t* __restrict__ counts, uint64_t CS,
-                                                          const uint32_t* __restrict__ sub_off, uint32_t C,
-                                                          uint64_t G, uint64_t q0,
-                                                          uint32_t* __restrict__ list_c,
-                                                          uint32_t* __restrict__ nnz_c,
-                                                          unsigned long long* __restrict__ gstat) {
-  __shared__ uint32_t tmp[8];
-  const uint64_t qi = blockIdx.x;
-  const uint32_t per = (C + kWG - 1) / kWG;
-  const uint32_t c0 = threadIdx.x * per, c1 = min(C, c0 + per);
-  const uint32_t* row = counts + qi * CS;
-  uint32_t mine = 0;
-  for (uint32_t c = c0; c < c1; ++c) {
-    int sum = 0;
-    for (uint32_t s = sub_off[c]; s < sub_off[c + 1]; ++s) sum += (int)row[s];
-    mine += sum != 0;
-  }
-  uint32_t total;
-  uint32_t pos = dev::wg_exclusive_scan(mine, tmp, total);
-  for (uint32_t c = c0; c < c1; ++c) {
-    int sum = 0;
-    for (uint32_t s = sub_off[c]; s < sub_off[c + 1]; ++s) sum += (int)row[s];
-    if (sum != 0) list_c[qi * C + pos++] = c;
-  }
-  if (threadIdx.x == 0) {
-    nnz_c[qi] = total;
-    atomicMax(&gstat[((q0 + qi) / G) * 8 + 0], (unsigned long long)total);
-  }
+// LDS sort capacity for a top-k: at least 1024 keys and at least k, a power of two.
+inline uint32_t sort_cap(uint64_t k) {
+  uint32_t c = 1024;
+  while (c < k) c <<= 1;
+  return c;
 }
 
-// Number of real items kept by a reference top-k over `real` sorted items followed by `pads` pad
-// columns of value 1e6 (pad columns sort after every real column of equal value).
+// Number of leading sorted keys whose value is <= 1e6 (the keys ascend by value).
+__device__ __forceinline__ uint32_t count_le_pad(const uint64_t* sorted, uint32_t m) {
+  uint32_t a = 0, b = m;
+  while (a < b) {
+    const uint32_t mid = (a + b) >> 1;
+    if (dev::key_value(sorted[mid]) <= kExactPad) a = mid + 1; else b = mid;
+  }
+  return a;
+}
+
 __device__ __forceinline__ uint32_t kept_with_pads(const uint64_t* sorted, uint32_t m, uint32_t k,
                                                    uint64_t pads) {
-  // count real items with value <= 1e6 among the sorted prefix
-  uint32_t n_le = 0;
-  while (n_le < m && dev::key_value(sorted[n_le]) <= kExactPad) ++n_le;
+  const uint32_t n_le = count_le_pad(sorted, m);
   uint32_t keep = k < n_le ? k : n_le;
   if (k > n_le + pads) keep = n_le + (uint32_t)min<uint64_t>(k - n_le - pads, m - n_le);
   return keep;
 }
 
-// A5: exact distances to the non-zero clusters, sorted top kc = min(n_list, group max nnz)
-// (Super-Grid-IVF_PQ.cu:1027-1042, refine.cu:78-119 with is_sort = true).
-__global__ __launch_bounds__(256) void coarse_select_kernel(
-    const float* __restrict__ queries, uint64_t d, const float* __restrict__ centers, uint32_t C,
-    uint32_t n_list, uint64_t G, uint64_t q0, const uint32_t* __restrict__ list_c,
-    const uint32_t* __restrict__ nnz_c, const unsigned long long* __restrict__ gstat,
-    uint64_t* __restrict__ keys, uint32_t* __restrict__ chosen_c, uint32_t* __restrict__ n_chosen_c) {
+// A5 select: top kc = min(n_list, group max nnz) of the exact cluster distances, sorted
+// (Super-Grid-IVF_PQ.cu:1027-1042, refine.cu:78-119 with is_sort = true).  chosen_c holds positions in
+// the query's non-zero cluster list.
+__global__ __launch_bounds__(256) void coarse_pick_kernel(uint32_t C, uint32_t n_list, uint64_t G, uint64_t q0,
+                                                          uint32_t scap,
+                                                          const uint32_t* __restrict__ nnz_c,
+                                                          const unsigned long long* __restrict__ gstat,
+                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ chosen_c,
+                                                          uint32_t* __restrict__ n_chosen_c) {
   extern __shared__ __align__(16) unsigned char smem[];
-  uint64_t* sorted = reinterpret_cast<uint64_t*>(smem);               // kSortCap keys
-  float* stage = reinterpret_cast<float*>(smem);                      // aliases `sorted` (phase 1)
-  __shared__ float qv[512];
-  __shared__ uint32_t rows[256];
+  uint64_t* sorted = reinterpret_cast<uint64_t*>(smem);
   __shared__ uint32_t hist[2048];
   __shared__ uint64_t sh[8];
   const uint64_t qi = blockIdx.x;
-  const float* q = queries + (q0 + qi) * d;
-  for (uint32_t t = threadIdx.x; t < d; t += kWG) qv[t] = q[t];
   const uint32_t n = nnz_c[qi];
-  uint64_t* gk = keys + qi * C;
-  __syncthreads();
-  for (uint32_t b = 0; b < n; b += kWG) {
-    const uint32_t j = b + threadIdx.x;
-    rows[threadIdx.x] = j < n ? list_c[qi * C + j] : kNone;
-    __syncthreads();
-    float dist = exact_tile(centers, d, rows, qv, stage);
-    if (j < n) gk[j] = dev::make_key(dist, j);
-    __syncthreads();
-  }
-  __syncthreads();
   const uint64_t gmax = gstat[((q0 + qi) / G) * 8 + 0];
   const uint32_t kc = (uint32_t)min<uint64_t>(n_list, gmax);
-  const uint32_t m = select_sorted(gk, n, kc, sorted, hist, sh);
+  const uint32_t m = select_sorted(keys + qi * C, n, kc, sorted, scap, hist, sh);
   __syncthreads();
   const uint32_t keep = kept_with_pads(sorted, m, kc, (uint64_t)C - n);
-  for (uint32_t j = threadIdx.x; j < keep; j += kWG)
-    chosen_c[qi * n_list + j] = list_c[qi * C + dev::key_tie(sorted[j])];
+  for (uint32_t j = threadIdx.x; j < keep; j += kWG) chosen_c[qi * n_list + j] = dev::key_tie(sorted[j]);
   if (threadIdx.x == 0) n_chosen_c[qi] = keep;
 }
 
-// ------------------------------------------------------------------------------------------------
-// A6 part 1: expand the chosen clusters (in their sorted order) into sub-cluster table positions
-// (build_offset_table, mat_operators.cuh:692-794) and keep the non-zero ones in table order
-// (collect_segments_data + filter_none_zero_data, Super-Grid-IVF_PQ.cu:866-883).
-__global__ __launch_bounds__(256) void sub_nnz_kernel(const uint32_t* __restrict__ counts, uint64_t CS,
-                                                      const uint32_t* __restrict__ sub_off,
-                                                      uint32_t n_list, uint64_t G, uint64_t q0,
-                                                      const uint32_t* __restrict__ chosen_c,
-                                                      const uint32_t* __restrict__ n_chosen_c,
-                                                      uint32_t cap, uint32_t* __restrict__ list_s,
-                                                      uint32_t* __restrict__ nnz_s,
-                                                      unsigned long long* __restrict__ gstat) {
-  __shared__ uint32_t pre[1025];
+// A6 part 1: the chosen clusters (in their sorted order) expanded into their non-zero sub-clusters in
+// table order (build_offset_table, mat_operators.cuh:692-794; collect_segments_data +
+// filter_none_zero_data, Super-Grid-IVF_PQ.cu:866-883), with their in-box counts.  The group maxima
+// of the non-zero count and of the table width (all sub-clusters of the chosen clusters) feed A6's k
+// and pad columns.
+__global__ __launch_bounds__(256) void sub_expand_kernel(uint32_t CS, uint32_t C, const uint32_t* __restrict__ sub_off,
+                                                         uint32_t n_list, uint64_t G, uint64_t q0,
+                                                         const uint32_t* __restrict__ list_c,
+                                                         const uint32_t* __restrict__ cl_beg,
+                                                         const uint32_t* __restrict__ sub_list,
+                                                         const uint32_t* __restrict__ sub_cnt,
+                                                         const uint32_t* __restrict__ chosen_c,
+                                                         const uint32_t* __restrict__ n_chosen_c, uint32_t cap,
+                                                         uint32_t* __restrict__ list_s, uint32_t* __restrict__ list_scnt,
+                                                         uint32_t* __restrict__ nnz_s,
+                                                         unsigned long long* __restrict__ gstat) {
   __shared__ uint32_t tmp[8];
-  __shared__ uint32_t base_sh;
   const uint64_t qi = blockIdx.x;
   const uint32_t nc = n_chosen_c[qi];
-  // prefix of sub-cluster counts of the chosen clusters (nc <= n_list <= 1024)
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0;
-    for (uint32_t j = 0; j < nc; ++j) {
-      pre[j] = acc;
-      uint32_t c = chosen_c[qi * n_list + j];
-      acc += sub_off[c + 1] - sub_off[c];
+  uint32_t base = 0, row_len = 0;
+  for (uint32_t b = 0; b < nc; b += kWG) {
+    const uint32_t i = b + threadIdx.x;
+    uint32_t beg = 0, len = 0, tl = 0;
+    if (i < nc) {
+      const uint32_t j = chosen_c[qi * n_list + i];
+      const uint32_t c = list_c[qi * C + j];
+      beg = cl_beg[qi * (C + 1) + j];
+      len = cl_beg[qi * (C + 1) + j + 1] - beg;
+      tl = sub_off[c + 1] - sub_off[c];
     }
-    pre[nc] = acc;
-    base_sh = 0;
-  }
-  __syncthreads();
-  const uint32_t row_len = pre[nc];
-  const uint32_t* row = counts + qi * CS;
-  for (uint32_t b = 0; b < row_len; b += kWG) {
-    const uint32_t t = b + threadIdx.x;
-    uint32_t sid = kNone, flag = 0;
-    if (t < row_len) {
-      uint32_t lo = 0, hi = nc;  // last j with pre[j] <= t
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (pre[mid] <= t) lo = mid; else hi = mid;
-      }
-      uint32_t c = chosen_c[qi * n_list + lo];
-      sid = sub_off[c] + (t - pre[lo]);
-      flag = row[sid] != 0;
+    uint32_t total, ttot;
+    const uint32_t pos = dev::wg_exclusive_scan(len, tmp, total);
+    (void)dev::wg_exclusive_scan(tl, tmp, ttot);
+    for (uint32_t e = 0; e < len; ++e) {
+      list_s[qi * cap + base + pos + e] = sub_list[qi * CS + beg + e];
+      list_scnt[qi * cap + base + pos + e] = sub_cnt[qi * CS + beg + e];
     }
-    uint32_t total;
-    uint32_t pos = dev::wg_exclusive_scan(flag, tmp, total);
-    if (flag) list_s[qi * cap + base_sh + pos] = sid;
-    __syncthreads();
-    if (threadIdx.x == 0) base_sh += total;
-    __syncthreads();
+    base += total;
+    row_len += ttot;
   }
   if (threadIdx.x == 0) {
-    nnz_s[qi] = base_sh;
+    nnz_s[qi] = base;
     const uint64_t g = (q0 + qi) / G;
-    atomicMax(&gstat[g * 8 + 1], (unsigned long long)base_sh);
+    atomicMax(&gstat[g * 8 + 1], (unsigned long long)base);
     atomicMax(&gstat[g * 8 + 2], (unsigned long long)row_len);
   }
 }
 
-// A6 part 2: exact distances to the non-zero sub-centers, top ks = min(sub_lists, group max), with
-// the group's table width supplying the 1e6 pad columns (Super-Grid-IVF_PQ.cu:885-896).  Output is
-// sorted ascending (the reference's order here is RAFT-defined).  With a per-query budget array
-// (PRECISION 1 with a budget model) the query keeps its own min(budget, sub_lists) nearest and no pads.
-__global__ __launch_bounds__(256) void sub_select_kernel(
-    const float* __restrict__ queries, uint64_t d, const float* __restrict__ sub_centers,
-    uint32_t sub_lists, uint64_t G, uint64_t q0, uint32_t cap, const uint32_t* __restrict__ list_s,
-    const uint32_t* __restrict__ nnz_s, const unsigned long long* __restrict__ gstat,
-    const uint32_t* __restrict__ sub_budget, uint64_t* __restrict__ keys, uint32_t* __restrict__ chosen_s,
-    uint32_t* __restrict__ n_chosen_s) {
+// A6 part 2 select: top ks = min(sub_lists, group max) of the exact sub-center distances, with the
+// group's table width supplying the 1e6 pad columns (Super-Grid-IVF_PQ.cu:885-896).  Output sorted
+// ascending (the reference's order here is RAFT-defined) as positions in list_s.  With a per-query
+// budget array the query keeps its own min(budget, sub_lists) nearest and no pads.
+__global__ __launch_bounds__(256) void sub_pick_kernel(uint32_t sub_lists, uint64_t G, uint64_t q0, uint32_t cap,
+                                                       uint32_t scap,
+                                                       const uint32_t* __restrict__ nnz_s,
+                                                       const unsigned long long* __restrict__ gstat,
+                                                       const uint32_t* __restrict__ sub_budget,
+                                                       uint64_t* __restrict__ keys, uint32_t* __restrict__ chosen_s,
+                                                       uint32_t* __restrict__ n_chosen_s) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t* sorted = reinterpret_cast<uint64_t*>(smem);
-  float* stage = reinterpret_cast<float*>(smem);
-  __shared__ float qv[512];
-  __shared__ uint32_t rows[256];
   __shared__ uint32_t hist[2048];
   __shared__ uint64_t sh[8];
   const uint64_t qi = blockIdx.x;
-  const float* q = queries + (q0 + qi) * d;
-  for (uint32_t t = threadIdx.x; t < d; t += kWG) qv[t] = q[t];
   const uint32_t n = nnz_s[qi];
-  uint64_t* gk = keys + qi * cap;
-  __syncthreads();
-  for (uint32_t b = 0; b < n; b += kWG) {
-    const uint32_t j = b + threadIdx.x;
-    rows[threadIdx.x] = j < n ? list_s[qi * cap + j] : kNone;
-    __syncthreads();
-    float dist = exact_tile(sub_centers, d, rows, qv, stage);
-    if (j < n) gk[j] = dev::make_key(dist, j);
-    __syncthreads();
-  }
-  __syncthreads();
   const uint64_t g = (q0 + qi) / G;
   uint32_t ks = (uint32_t)min<uint64_t>(sub_lists, gstat[g * 8 + 1]);
   uint64_t table_len = gstat[g * 8 + 2];
@@ -463,25 +823,25 @@ __global__ __launch_bounds__(256) void sub_select_kernel(
     ks = b == 0 || b > sub_lists ? sub_lists : b;
     table_len = n;
   }
-  const uint32_t m = select_sorted(gk, n, ks, sorted, hist, sh);
+  const uint32_t m = select_sorted(keys + qi * cap, n, ks, sorted, scap, hist, sh);
   __syncthreads();
   const uint32_t keep = kept_with_pads(sorted, m, ks, table_len - n);
-  for (uint32_t j = threadIdx.x; j < keep; j += kWG)
-    chosen_s[qi * sub_lists + j] = list_s[qi * cap + dev::key_tie(sorted[j])];
+  for (uint32_t j = threadIdx.x; j < keep; j += kWG) chosen_s[qi * sub_lists + j] = dev::key_tie(sorted[j]);
   if (threadIdx.x == 0) n_chosen_s[qi] = keep;
 }
 
 // ------------------------------------------------------------------------------------------------
 // A8/A9: per chosen sub-cluster, the Morton range [lower_bound(zlo), lower_bound(zhi+1)) inside its
 // z-pool segment (zorder.cu:656-685).  One workgroup per query, one thread per chosen sub-cluster.
-// Also sums C_q (candidates) and the in-box count of the chosen sub-clusters (A10 inputs).
+// Also sums C_q (candidates) and the in-box count of the chosen sub-clusters (A10 inputs), and
+// records the chosen sub-cluster ids (UINT32_MAX past the end).
 __global__ __launch_bounds__(256) void ranges_kernel(
-    const uint64_t* __restrict__ zcode, const uint32_t* __restrict__ seg_off,
-    const uint32_t* __restrict__ counts, uint64_t CS, uint32_t sub_lists, uint64_t G, uint64_t q0,
-    const uint32_t* __restrict__ chosen_s, const uint32_t* __restrict__ n_chosen_s,
-    const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zhi, uint32_t* __restrict__ rstart,
-    uint32_t* __restrict__ rpre, uint64_t* __restrict__ cand, uint64_t* __restrict__ real,
-    unsigned long long* __restrict__ gstat) {
+    const uint64_t* __restrict__ zcode, const uint32_t* __restrict__ seg_off, uint32_t cap,
+    const uint32_t* __restrict__ list_s, const uint32_t* __restrict__ list_scnt, uint32_t sub_lists, uint64_t G,
+    uint64_t q0, const uint32_t* __restrict__ chosen_s, const uint32_t* __restrict__ n_chosen_s,
+    const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zhi, uint32_t* __restrict__ chosen_sid,
+    uint32_t* __restrict__ rstart, uint32_t* __restrict__ rpre, uint64_t* __restrict__ cand,
+    uint64_t* __restrict__ real, unsigned long long* __restrict__ gstat) {
   __shared__ uint32_t tmp[8];
   __shared__ uint32_t base_sh;
   __shared__ unsigned long long real_sh;
@@ -495,29 +855,33 @@ __global__ __launch_bounds__(256) void ranges_kernel(
   __syncthreads();
   for (uint32_t j0 = 0; j0 < sub_lists; j0 += kWG) {
     const uint32_t j = j0 + threadIdx.x;
-    uint32_t cnt = 0, start = 0;
+    uint32_t cnt = 0, start = 0, sid = kNone;
     if (j < n) {
-      const uint32_t s = chosen_s[qi * sub_lists + j];
-      const uint32_t p0 = seg_off[s], p1 = seg_off[s + 1];
-      uint32_t lo = p0, hi = p1;
-      while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (zcode[mid] < a) lo = mid + 1; else hi = mid;
+      const uint32_t idx = chosen_s[qi * sub_lists + j];
+      sid = list_s[qi * cap + idx];
+      const uint32_t p0 = seg_off[sid], p1 = seg_off[sid + 1];
+      // lower_bound(zlo) and lower_bound(zhi + 1) side by side, so their loads overlap
+      uint32_t lo = p0, hi = p1, lo2 = p0, hi2 = p1;
+      while (lo < hi || lo2 < hi2) {
+        const uint32_t m1 = (lo + hi) >> 1, m2 = (lo2 + hi2) >> 1;
+        const uint64_t z1 = lo < hi ? zcode[m1] : 0, z2 = lo2 < hi2 ? zcode[m2] : 0;
+        if (lo < hi) {
+          if (z1 < a) lo = m1 + 1; else hi = m1;
+        }
+        if (lo2 < hi2) {
+          if (z2 < b) lo2 = m2 + 1; else hi2 = m2;
+        }
       }
       start = lo;
-      uint32_t lo2 = lo, hi2 = p1;
-      while (lo2 < hi2) {
-        uint32_t mid = (lo2 + hi2) >> 1;
-        if (zcode[mid] < b) lo2 = mid + 1; else hi2 = mid;
-      }
-      cnt = lo2 - lo;
-      atomicAdd(&real_sh, (unsigned long long)counts[qi * CS + s]);
+      cnt = lo2 > lo ? lo2 - lo : 0u;  // an empty code range (zlo > zhi) holds no candidate
+      atomicAdd(&real_sh, (unsigned long long)list_scnt[qi * cap + idx]);
     }
     uint32_t total;
     uint32_t pos = dev::wg_exclusive_scan(cnt, tmp, total);
     if (j < sub_lists) {
       rstart[qi * sub_lists + j] = start;
       rpre[qi * (sub_lists + 1) + j] = base_sh + pos;
+      chosen_sid[qi * sub_lists + j] = sid;
     }
     __syncthreads();
     if (threadIdx.x == 0) base_sh += total;
@@ -594,33 +958,49 @@ __global__ void plan_kernel(uint64_t qn, uint64_t q0, uint64_t G, int l, uint64_
   padmax[qi] = pmax;  // l > 1: set after the post-filter (n_candies - survivors)
 }
 
-// Exclusive scan of n u64 values in place (single workgroup), total to out[n].
-__global__ __launch_bounds__(256) void scan_kernel(uint64_t* __restrict__ v, uint64_t n) {
-  __shared__ unsigned long long carry_sh;
-  __shared__ unsigned long long part[256];
-  if (threadIdx.x == 0) carry_sh = 0;
+// Exclusive scans of three u64 arrays in place (one workgroup of 1024 threads), totals to v[n].
+__global__ __launch_bounds__(1024) void scan3_kernel(uint64_t* __restrict__ v0, uint64_t* __restrict__ v1,
+                                                     uint64_t* __restrict__ v2, uint64_t n) {
+  __shared__ unsigned long long wsum[3][16];
+  __shared__ unsigned long long carry[3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t* const vs[3] = {v0, v1, v2};
+  if (threadIdx.x < 3) carry[threadIdx.x] = 0;
   __syncthreads();
-  for (uint64_t b = 0; b < n; b += 256) {
-    uint64_t i = b + threadIdx.x;
-    unsigned long long x = i < n ? v[i] : 0;
-    part[threadIdx.x] = x;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-      unsigned long long y = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
-      __syncthreads();
-      part[threadIdx.x] += y;
-      __syncthreads();
+  for (uint64_t b = 0; b < n; b += 1024) {
+    const uint64_t i = b + threadIdx.x;
+    unsigned long long x[3], incl[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      x[a] = i < n ? vs[a][i] : 0ull;
+      unsigned long long y = x[a];
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long t = __shfl_up(y, o, 64);
+        if (lane >= o) y += t;
+      }
+      incl[a] = y;
+      if (lane == 63) wsum[a][wave] = y;
     }
-    unsigned long long incl = part[threadIdx.x];
-    if (i < n) v[i] = carry_sh + incl - x;
     __syncthreads();
-    if (threadIdx.x == 255) carry_sh += incl;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      unsigned long long before = carry[a];
+      for (int w2 = 0; w2 < wave; ++w2) before += wsum[a][w2];
+      if (i < n) vs[a][i] = before + incl[a] - x[a];
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      unsigned long long tot = 0;
+      for (int w2 = 0; w2 < 16; ++w2) tot += wsum[threadIdx.x][w2];
+      carry[threadIdx.x] += tot;
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) v[n] = carry_sh;
+  if (threadIdx.x < 3) vs[threadIdx.x][n] = carry[threadIdx.x];
 }
 
-// Position in the z-pool of candidate column `col` of query row (rstart/rpre of that query).
+// Position in the z-pool of candidate column `col` of a query row (rstart/rpre of that query, in LDS).
 __device__ __forceinline__ uint32_t col_to_pos(const uint32_t* rs, const uint32_t* rp, uint32_t n,
                                                uint32_t col) {
   uint32_t lo = 0, hi = n;  // last j with rp[j] <= col
@@ -641,21 +1021,35 @@ __device__ __forceinline__ bool label_pass(const float* __restrict__ lab, int l,
 }
 
 // Mode 0 / 2: copy every candidate (l == 1) or every candidate passing the exact label predicate
-// (l > 1 and the query_prefilter path) into the refine list.  One workgroup per query.
+// (l > 1 and the query_prefilter path) into the refine list, in candidate-column order
+// (filter.cu:146-164).  One workgroup per query; the query's range table is staged in LDS.
+__device__ __forceinline__ bool inbox_keeps(uint64_t qi, int l, int integral, const uint32_t* __restrict__ mode,
+                                            const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo,
+                                            const uint64_t* __restrict__ zlo_cnt) {
+  return mode[qi] == 0 && n_ib[qi] != kNone && zlo[qi] == zlo_cnt[qi] && (l == 1 || integral);
+}
+
 __global__ __launch_bounds__(256) void keep_all_kernel(
-    uint64_t qn, int l, uint32_t sub_lists, const uint32_t* __restrict__ mode,
+    uint64_t qn, int l, int integral, uint32_t sub_lists, const uint32_t* __restrict__ mode,
+    const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zlo_cnt,
     const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ rstart,
     const uint32_t* __restrict__ rpre, const float* __restrict__ lab, const float* __restrict__ ranges,
     const uint64_t* __restrict__ loff, uint32_t* __restrict__ L, uint32_t* __restrict__ nL) {
   __shared__ uint32_t tmp[8];
+  extern __shared__ uint32_t rsp[];  // rs[sub_lists], rp[sub_lists + 1]
+  uint32_t* rs = rsp;
+  uint32_t* rp = rsp + sub_lists;
   const uint64_t qi = blockIdx.x;
-  if (mode[qi] == 1) return;
+  if (mode[qi] == 1 || inbox_keeps(qi, l, integral, mode, n_ib, zlo, zlo_cnt)) return;
   const bool filt = (l > 1) || mode[qi] == 2;
   const uint32_t nr = n_chosen_s[qi];
-  const uint32_t* rs = rstart + qi * sub_lists;
-  const uint32_t* rp = rpre + qi * (sub_lists + 1);
-  const uint32_t Cq = rp[nr];
+  const uint32_t Cq = rpre[qi * (sub_lists + 1) + nr];
   uint32_t* out = L + loff[qi];
+  for (uint32_t j = threadIdx.x; j <= nr; j += kWG) {
+    rp[j] = rpre[qi * (sub_lists + 1) + j];
+    if (j < nr) rs[j] = rstart[qi * sub_lists + j];
+  }
+  __syncthreads();
   if (!filt) {
     for (uint32_t j = 0; j < nr; ++j) {
       const uint32_t p0 = rs[j], n = rp[j + 1] - rp[j];
@@ -664,16 +1058,85 @@ __global__ __launch_bounds__(256) void keep_all_kernel(
     if (threadIdx.x == 0) nL[qi] = Cq;
     return;
   }
-  // order-preserving compaction in candidate-column order (filter.cu:146-164)
   float rg[2 * kMaxDims];
   for (int i = 0; i < 2 * l; ++i) rg[i] = ranges[qi * 2 * l + i];
+  // each thread owns kKeepPer consecutive columns, so all their label loads are in flight together and
+  // one scan per kWG * kKeepPer columns keeps the column order
+  constexpr int kKeepPer = 8;
   uint32_t base = 0;
-  for (uint32_t b = 0; b < Cq; b += kWG) {
-    const uint32_t col = b + threadIdx.x;
+  for (uint32_t b = 0; b < Cq; b += kWG * kKeepPer) {
+    const uint32_t c0 = b + threadIdx.x * kKeepPer;
+    uint32_t pos[kKeepPer];
+    uint32_t flags = 0, cnt = 0;
+    if (c0 < Cq) {
+      uint32_t lo = 0, hi = nr;  // last range j with rp[j] <= c0
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (rp[mid] <= c0) lo = mid; else hi = mid;
+      }
+      uint32_t j = lo;
+#pragma unroll
+      for (int i = 0; i < kKeepPer; ++i) {
+        const uint32_t col = c0 + i;
+        while (j + 1 < nr && rp[j + 1] <= col) ++j;
+        pos[i] = rs[j] + (col - rp[j]);
+      }
+#pragma unroll
+      for (int i = 0; i < kKeepPer; ++i) {
+        if (c0 + i < Cq && label_pass(lab, l, pos[i], rg)) {
+          flags |= 1u << i;
+          ++cnt;
+        }
+      }
+    }
+    uint32_t total;
+    uint32_t at = base + dev::wg_exclusive_scan(cnt, tmp, total);
+#pragma unroll
+    for (int i = 0; i < kKeepPer; ++i)
+      if (flags & (1u << i)) out[at++] = pos[i];
+    base += total;
+  }
+  if (threadIdx.x == 0) nL[qi] = base;
+}
+
+// Keep-all refine list from the query's in-box points (mode 0, a walked query whose Morton lower corner
+// did not wrap): every label survivor among the Morton-range candidates lies in the cell box (labels are
+// integers, or l == 1 where the range is the box), so the survivors are exactly the in-box points of the
+// chosen sub-clusters that pass the label predicate (none is filtered when l == 1).  The list order differs
+// from the candidate-column order, which the exact (distance, id) top-k does not depend on.  One
+// workgroup per query; the chosen sub-clusters are a bit set in LDS.
+__global__ __launch_bounds__(256) void inbox_keep_kernel(
+    int l, int integral, uint32_t CS, uint32_t sub_lists, const uint32_t* __restrict__ mode,
+    const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zlo_cnt,
+    const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ chosen_sid, uint32_t ibcap,
+    const uint32_t* __restrict__ ib_pos, const uint32_t* __restrict__ ib_sub, const float* __restrict__ lab,
+    const float* __restrict__ ranges, const uint64_t* __restrict__ loff, uint32_t* __restrict__ L,
+    uint32_t* __restrict__ nL) {
+  __shared__ uint32_t tmp[8];
+  __shared__ uint32_t bits[kSparseMaxCS / 32];
+  const uint64_t qi = blockIdx.x;
+  if (!inbox_keeps(qi, l, integral, mode, n_ib, zlo, zlo_cnt)) return;
+  const uint32_t nw = (CS + 31) / 32;
+  for (uint32_t i = threadIdx.x; i < nw; i += kWG) bits[i] = 0;
+  __syncthreads();
+  const uint32_t nc = n_chosen_s[qi];
+  for (uint32_t j = threadIdx.x; j < nc; j += kWG) {
+    const uint32_t sid = chosen_sid[qi * sub_lists + j];
+    atomicOr(&bits[sid >> 5], 1u << (sid & 31));
+  }
+  __syncthreads();
+  float rg[2 * kMaxDims];
+  for (int i = 0; i < 2 * l; ++i) rg[i] = ranges[qi * 2 * l + i];
+  const uint32_t n = n_ib[qi];
+  uint32_t* out = L + loff[qi];
+  uint32_t base = 0;
+  for (uint32_t b = 0; b < n; b += kWG) {
+    const uint32_t i = b + threadIdx.x;
     uint32_t pos = 0, flag = 0;
-    if (col < Cq) {
-      pos = col_to_pos(rs, rp, nr, col);
-      flag = label_pass(lab, l, pos, rg) ? 1u : 0u;
+    if (i < n) {
+      const uint32_t sid = ib_sub[qi * ibcap + i];
+      pos = ib_pos[qi * ibcap + i];
+      flag = ((bits[sid >> 5] >> (sid & 31)) & 1u) && (l == 1 || label_pass(lab, l, pos, rg)) ? 1u : 0u;
     }
     uint32_t total;
     const uint32_t at = dev::wg_exclusive_scan(flag, tmp, total);
@@ -824,47 +1287,26 @@ __global__ __launch_bounds__(256) void adc_select_kernel(
   if (threadIdx.x == 0) nL[qi] = l == 1 ? kp : cnt_sh;
 }
 
-// A13: exact FP32 distances of the refine list, top-k by (distance, id) with the reference's pad
-// columns (refine.cu:78-119), or for the query_prefilter path the top-k by (pq distance, column).
-__global__ __launch_bounds__(256) void final_kernel(
-    uint64_t qn, uint64_t q0, const float* __restrict__ queries, uint64_t d, const float* __restrict__ vec,
-    const uint32_t* __restrict__ zid, int l, uint64_t topk, const uint32_t* __restrict__ mode,
+// A13 select: top-k by (exact distance, id) of the refine list with the reference's pad columns
+// (refine.cu:78-119), or for the query_prefilter path the top-k by (pq distance, column).
+__global__ __launch_bounds__(256) void final_pick_kernel(
+    uint64_t q0, uint32_t scap, const uint32_t* __restrict__ zid, int l, uint64_t topk, const uint32_t* __restrict__ mode,
     const uint64_t* __restrict__ kprime, const uint64_t* __restrict__ loff, const uint32_t* __restrict__ L,
     const uint32_t* __restrict__ nL, const uint64_t* __restrict__ pad32, const uint64_t* __restrict__ padmax,
     uint64_t* __restrict__ keys, uint64_t* __restrict__ akeys, const uint64_t* __restrict__ aoff,
     float* __restrict__ out_dist, uint64_t* __restrict__ out_ids) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t* sorted = reinterpret_cast<uint64_t*>(smem);
-  float* stage = reinterpret_cast<float*>(smem);
-  __shared__ float qv[512];
-  __shared__ uint32_t rows[256];
   __shared__ uint32_t hist[2048];
   __shared__ uint64_t sh[8];
   const uint64_t qi = blockIdx.x;
-  const float* q = queries + (q0 + qi) * d;
-  for (uint32_t t = threadIdx.x; t < d; t += kWG) qv[t] = q[t];
   const uint32_t n = nL[qi];
   const uint32_t* list = L + loff[qi];
-  uint64_t* gk = keys + loff[qi];
+  uint64_t* gk = mode[qi] == 2 ? akeys + aoff[qi] : keys + loff[qi];
   float* od = out_dist + (q0 + qi) * topk;
   uint64_t* oi = out_ids + (q0 + qi) * topk;
-  __syncthreads();
-  if (mode[qi] == 2) {
-    // query_prefilter path: adc_scan_kernel left the PQ keys of the filtered list in akeys
-    gk = akeys + aoff[qi];
-  } else {
-    for (uint32_t b = 0; b < n; b += kWG) {
-      const uint32_t j = b + threadIdx.x;
-      rows[threadIdx.x] = j < n ? list[j] : kNone;
-      __syncthreads();
-      float dist = exact_tile(vec, d, rows, qv, stage);
-      if (j < n) gk[j] = dev::make_key(dist, zid[list[j]]);
-      __syncthreads();
-    }
-  }
-  __syncthreads();
   const uint32_t k = (uint32_t)topk;
-  const uint32_t m = select_sorted(gk, n, k, sorted, hist, sh);
+  const uint32_t m = select_sorted(gk, n, k, sorted, scap, hist, sh);
   __syncthreads();
   if (mode[qi] == 2) {
     for (uint32_t j = threadIdx.x; j < k; j += kWG) {
@@ -878,34 +1320,54 @@ __global__ __launch_bounds__(256) void final_kernel(
     }
     return;
   }
-  // pads of the refine list: (1e6, 2^32) x p32 then (1e6, UINT64_MAX) x pmax
+  // pads of the refine list: (1e6, 2^32) x p32 then (1e6, UINT64_MAX) x pmax, placed after the real
+  // items of value <= 1e6
   uint64_t p32 = pad32[qi], pmax = padmax[qi];
   if (l > 1) pmax = kprime[qi] > n ? kprime[qi] - n : 0;
-  if (threadIdx.x == 0) {
-    uint32_t n_le = 0;
-    while (n_le < m && dev::key_value(sorted[n_le]) <= kExactPad) ++n_le;
-    uint32_t j = 0, r = 0;
-    for (; j < k && r < n_le; ++j, ++r) {
-      od[j] = dev::key_value(sorted[r]);
-      oi[j] = dev::key_tie(sorted[r]);
+  const uint32_t n_le = count_le_pad(sorted, m);
+  for (uint32_t j = threadIdx.x; j < k; j += kWG) {
+    float dv;
+    uint64_t iv;
+    if (j < n_le) {
+      dv = dev::key_value(sorted[j]);
+      iv = dev::key_tie(sorted[j]);
+    } else if ((uint64_t)(j - n_le) < p32) {
+      dv = kExactPad;
+      iv = kIdPad;
+    } else if ((uint64_t)(j - n_le) < p32 + pmax) {
+      dv = kExactPad;
+      iv = kU64Max;
+    } else {
+      const uint64_t r = (uint64_t)j - p32 - pmax;  // real items beyond 1e6
+      if (r < m) {
+        dv = dev::key_value(sorted[r]);
+        iv = dev::key_tie(sorted[r]);
+      } else {
+        dv = kExactPad;
+        iv = kU64Max;
+      }
     }
-    for (; j < k && p32 > 0; ++j, --p32) {
-      od[j] = kExactPad;
-      oi[j] = kIdPad;
-    }
-    for (; j < k && pmax > 0; ++j, --pmax) {
-      od[j] = kExactPad;
-      oi[j] = kU64Max;
-    }
-    for (; j < k && r < m; ++j, ++r) {
-      od[j] = dev::key_value(sorted[r]);
-      oi[j] = dev::key_tie(sorted[r]);
-    }
-    for (; j < k; ++j) {
-      od[j] = kExactPad;
-      oi[j] = kU64Max;
-    }
+    od[j] = dv;
+    oi[j] = iv;
   }
+}
+
+// Per-query counters of the chunk into the call-wide table st[q][6] = (C_q, in-box count of the chosen
+// sub-clusters, refined, scored, ADC-scanned, k').
+__global__ void stats_kernel(uint64_t qn, uint64_t q0, const uint64_t* __restrict__ cand,
+                             const uint64_t* __restrict__ real, const uint32_t* __restrict__ nL,
+                             const uint32_t* __restrict__ nnz_c, const uint32_t* __restrict__ nnz_s,
+                             const uint32_t* __restrict__ mode, const uint64_t* __restrict__ kprime,
+                             uint64_t* __restrict__ st) {
+  const uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi >= qn) return;
+  uint64_t* o = st + (q0 + qi) * 6;
+  o[0] = cand[qi];
+  o[1] = real[qi];
+  o[2] = nL[qi];
+  o[3] = (uint64_t)nnz_c[qi] + nnz_s[qi];
+  o[4] = mode[qi] == 1 ? cand[qi] : 0;
+  o[5] = kprime[qi];
 }
 
 }  // namespace
@@ -914,14 +1376,21 @@ __global__ __launch_bounds__(256) void final_kernel(
 // Host orchestration
 struct Index::Work {
   DevBuf ranges, boxes, box_lo, box_w, zlo, zhi, zlo_cnt, err;
-  DevBuf counts, list_c, nnz_c, chosen_c, n_chosen_c, list_s, nnz_s, chosen_s, n_chosen_s;
-  DevBuf keys, gstat, rstart, rpre, cand, real;
+  DevBuf dense_list, n_dense, counts;                        // A3 dense hand-over
+  DevBuf ib_pos, ib_sub, n_ib;                               // in-box points listed by the walk
+  DevBuf sub_list, sub_cnt, ns, list_c, cl_beg, nnz_c;       // non-zero sub-clusters / clusters
+  DevBuf chosen_c, n_chosen_c, list_s, list_scnt, nnz_s, chosen_s, n_chosen_s, chosen_sid;
+  DevBuf keys, woff, gstat, rstart, rpre, cand, real;
   DevBuf mode, kprime, lsize, asize, chunks, pad32, padmax, nL;
   DevBuf L, Lkeys, akeys;
-  DevBuf S, feat, budget_T, budget_S;  // PRECISION 1: coarse scores, model features, per-query budgets
+  DevBuf S, feat, budget_T, budget_S, qb16;  // PRECISION 1: scores, model features, budgets, bf16 queries
+  DevBuf st, st_chosen;                // call-wide counters [nq][6] and chosen sub-cluster ids [nq][sub_lists]
+  uint64_t st_nq = 0, st_sub_lists = 0;
   uint64_t* pinned = nullptr;
   hipEvent_t ev[16];
   bool ev_init = false;
+  int ev_used = 0;
+  const char* const* ev_names = nullptr;
 };
 
 int find_max_factor(uint64_t a, uint64_t b) {
@@ -963,12 +1432,48 @@ Index::~Index() {
   }
 }
 
+void Index::last_stats(uint64_t nq, uint64_t* const out[6]) const {
+  Work& w = *w_;
+  if (nq != w.st_nq) fail(kInvalidArgument, "nq differs from the last query call");
+  if (nq == 0) return;
+  HDG_HIP(hipSetDevice(device_));
+  std::vector<uint64_t> t(nq * 6);
+  HDG_HIP(hipMemcpy(t.data(), w.st.data(), nq * 6 * 8, hipMemcpyDeviceToHost));
+  for (int c = 0; c < 6; ++c)
+    if (out[c])
+      for (uint64_t q = 0; q < nq; ++q) out[c][q] = t[q * 6 + c];
+}
+
+void Index::last_chosen_subs(uint64_t nq, uint32_t* out) const {
+  Work& w = *w_;
+  if (nq != w.st_nq) fail(kInvalidArgument, "nq differs from the last query call");
+  if (nq == 0) return;
+  HDG_HIP(hipSetDevice(device_));
+  HDG_HIP(hipMemcpy(out, w.st_chosen.data(), nq * w.st_sub_lists * 4, hipMemcpyDeviceToHost));
+}
+
+std::vector<std::pair<std::string, float>> Index::stage_times() const {
+  Work& w = *w_;
+  std::vector<std::pair<std::string, float>> r;
+  if (w.ev_used < 2) return r;
+  HDG_HIP(hipSetDevice(device_));
+  HDG_HIP(hipEventSynchronize(w.ev[w.ev_used - 1]));
+  for (int i = 0; i + 1 < w.ev_used; ++i) {
+    float ms = 0;
+    HDG_HIP(hipEventElapsedTime(&ms, w.ev[i], w.ev[i + 1]));
+    r.push_back({w.ev_names[i], ms});
+  }
+  return r;
+}
+
 void Index::debug_coarse_scores(const float* queries, uint64_t nq, float* out_host, hipStream_t s) {
   HDG_HIP(hipSetDevice(device_));
   if (!st_.built) fail(kBadState, "query before build");
-  DevBuf S;
+  DevBuf S, qb;
   S.reserve(nq * st_.C * 4);
-  launch_coarse_gemm(queries, 0, nq, st_.d, st_.Dp, st_.centers_bf16.as<uint16_t>(), st_.center_norm.as<float>(),
+  qb.reserve(nq * st_.Dp * 2);
+  launch_to_bf16(queries, 0, nq, st_.d, st_.Dp, qb.as<uint16_t>(), s);
+  launch_coarse_gemm(qb.as<uint16_t>(), nq, st_.Dp, st_.centers_bf16.as<uint16_t>(), st_.center_norm.as<float>(),
                      (uint32_t)st_.C, S.as<float>(), s);
   HDG_HIP(hipGetLastError());
   HDG_HIP(hipMemcpyAsync(out_host, S.data(), nq * st_.C * 4, hipMemcpyDeviceToHost, s));
@@ -989,15 +1494,12 @@ void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* ql
       fail(kInvalidArgument, "explicit-range filter needs 2*l query label columns");
   uint64_t G = cfg_.parity_batch ? cfg_.parity_batch : (uint64_t)find_max_factor(125, nq);
   uint64_t chunk = std::max<uint64_t>(G, (cfg_.query_chunk / G) * G);
-  stats_ = QueryStats();
-  stats_.nq = nq;
-  stats_.cand.resize(nq);
-  stats_.real.resize(nq);
-  stats_.refined.resize(nq);
-  stats_.scored.resize(nq);
-  stats_.adc.resize(nq);
-  stats_.kprime.resize(nq);
-  stage_ms.clear();
+  Work& w = *w_;
+  w.st.reserve(nq * 6 * 8);
+  w.st_chosen.reserve(nq * cfg_.sub_lists * 4);
+  w.st_nq = nq;
+  w.st_sub_lists = cfg_.sub_lists;
+  w.ev_used = 0;
   for (uint64_t q0 = 0; q0 < nq; q0 += chunk) {
     uint64_t qn = std::min(chunk, nq - q0);
     query_chunk(queries, q0, qn, qlabels, qcols, G, out_dist, out_ids, s);
@@ -1015,34 +1517,48 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   const uint32_t sub_lists = (uint32_t)cfg_.sub_lists;
   const uint64_t topk = cfg_.topk;
   if (d > 512) fail(kUnsupported, "vector dimension above 512");
-  if (C > (uint64_t)dev::kSortCap) fail(kUnsupported, "clusters above 8192");
+  if (C > (uint64_t)dev::kSortCap) fail(kUnsupported, "clusters above 4096");
   if (cfg_.n_list > 1024) fail(kUnsupported, "n_list above 1024");
-  if (topk > (uint64_t)dev::kSortCap || topk == 0) fail(kUnsupported, "topk must be in [1, 8192]");
-  if (sub_lists == 0 || sub_lists > (uint32_t)dev::kSortCap) fail(kUnsupported, "sub_lists must be in [1, 8192]");
+  if (topk > (uint64_t)dev::kSortCap || topk == 0) fail(kUnsupported, "topk must be in [1, 4096]");
+  if (sub_lists == 0 || sub_lists > (uint32_t)dev::kSortCap) fail(kUnsupported, "sub_lists must be in [1, 4096]");
+  if (g.n_valid < 1 || g.n_valid > 8) fail(kBadState, "no valid label dims");
   const uint32_t cap = (uint32_t)std::min<uint64_t>(CS, (uint64_t)n_list * S.max_subs_per_cluster);
   const uint64_t ng = cdiv(q0 + qn, G) - q0 / G;  // groups touched by this chunk
   const uint64_t g_base = q0 / G;
   const Dims D = make_dims(g);
+  const bool sparse = cfg_.sparse_count != 0 && S.Nb > 0 && CS <= kSparseMaxCS;
 
   // workspace
   w.ranges.reserve(qn * 2 * l * 4);
   w.boxes.reserve(qn * 2 * l * 4);
-  w.box_lo.reserve((uint64_t)std::max(1, g.n_valid) * qn * 4);
-  w.box_w.reserve((uint64_t)std::max(1, g.n_valid) * qn * 4);
+  w.box_lo.reserve((uint64_t)g.n_valid * qn * 4);
+  w.box_w.reserve((uint64_t)g.n_valid * qn * 4);
   w.zlo.reserve(qn * 8);
   w.zhi.reserve(qn * 8);
   w.zlo_cnt.reserve(qn * 8);
   w.err.reserve(4);
+  w.dense_list.reserve(qn * 4);
+  w.n_dense.reserve(4);
   w.counts.reserve(qn * CS * 4);
+  w.ib_pos.reserve(qn * kInboxCap * 4);
+  w.ib_sub.reserve(qn * kInboxCap * 4);
+  w.n_ib.reserve(qn * 4);
+  w.sub_list.reserve(qn * CS * 4);
+  w.sub_cnt.reserve(qn * CS * 4);
+  w.ns.reserve(qn * 4);
   w.list_c.reserve(qn * C * 4);
+  w.cl_beg.reserve(qn * (C + 1) * 4);
   w.nnz_c.reserve(qn * 4);
   w.chosen_c.reserve(qn * n_list * 4);
   w.n_chosen_c.reserve(qn * 4);
   w.list_s.reserve(qn * cap * 4);
+  w.list_scnt.reserve(qn * cap * 4);
   w.nnz_s.reserve(qn * 4);
   w.chosen_s.reserve(qn * sub_lists * 4);
+  w.chosen_sid.reserve(qn * sub_lists * 4);
   w.n_chosen_s.reserve(qn * 4);
   w.keys.reserve(qn * std::max<uint64_t>(C, cap) * 8);
+  w.woff.reserve((qn + 1) * 8);
   w.gstat.reserve((ng + g_base + 1) * 8 * 8);
   w.rstart.reserve(qn * sub_lists * 4);
   w.rpre.reserve(qn * (sub_lists + 1) * 4);
@@ -1059,10 +1575,11 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   unsigned long long* gstat = w.gstat.as<unsigned long long>() + g_base * 8;
   HDG_HIP(hipMemsetAsync(gstat, 0, ng * 8 * 8, s));
   HDG_HIP(hipMemsetAsync(w.err.data(), 0, 4, s));
-  HDG_HIP(hipMemsetAsync(w.chosen_s.data(), 0xFF, qn * sub_lists * 4, s));
+  HDG_HIP(hipMemsetAsync(w.n_dense.data(), 0, 4, s));
 
-  // dynamic LDS of the per-query kernels: the sort buffer, aliased by the 256 x 33 distance stage
-  const size_t sort_lds = std::max<size_t>(dev::kSortCap * sizeof(uint64_t), 256 * 33 * sizeof(float));
+  // dynamic LDS of the per-query select kernels: the sort buffer
+  const uint32_t scap_c = sort_cap(n_list), scap_s = sort_cap(sub_lists), scap_f = sort_cap(topk);
+  const unsigned dist_grid = 4096;
   if (!w.ev_init) {
     for (auto& e : w.ev) HDG_HIP(hipEventCreate(&e));
     w.ev_init = true;
@@ -1077,9 +1594,6 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   HDG_HIP(hipGetLastError());
   mark();
   const bool mixed = cfg_.precision == 1;
-  // PRECISION 1 with PROBE_CLUSTERS > 0: BF16 coarse top-T, counts of those clusters only (lazy);
-  // PRECISION 1 with PROBE_CLUSTERS = 0: exact full counts, BF16 MFMA scores for cluster and sub-cluster choice.
-  const bool lazy = mixed && cfg_.probe_clusters > 0;
   uint32_t* budget_T = nullptr;
   uint32_t* budget_S = nullptr;
   if (mixed && cfg_.budget_model == 1) {
@@ -1092,102 +1606,126 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   if (mixed) {
     w.S.reserve(qn * std::max<uint64_t>(C, CS) * 4);
     w.feat.reserve(qn * kFeatDists * 4);
+    w.qb16.reserve(qn * S.Dp * 2);
+    launch_to_bf16(queries, q0, qn, d, S.Dp, w.qb16.as<uint16_t>(), s);
+    HDG_HIP(hipGetLastError());
   }
-  auto launch_full_count = [&]() {
-    dim3 grid((unsigned)cdiv(qn, 256), (unsigned)cdiv(CS, kCountSegChunk));
+
+  // ---- A3/A4: non-zero sub-clusters and clusters of every query ----
+  {
+    Buckets B;
+    for (int j = 0; j < kMaxDims; ++j) {
+      B.M[j] = S.Mv[j];
+      B.nb[j] = S.nb[j];
+      B.stride[j] = S.bstride[j];
+    }
+    const uint32_t nw = (uint32_t)((CS + 31) / 32);
+    const size_t lds = (CS + nw) * sizeof(uint32_t);
+    const uint32_t max_tests = (uint32_t)std::min<uint64_t>(cfg_.sparse_max_tests, 0xFFFFFFF0ull);
+    const unsigned grid = (unsigned)std::min<uint64_t>(qn, 2048);
     const uint32_t* cells = S.cells.as<uint32_t>();
-    const uint32_t* so = S.seg_off.as<uint32_t>();
+    dim3 dgrid((unsigned)cdiv(qn, 256), (unsigned)cdiv(CS, kCountSegChunk));
     switch (g.n_valid) {
-#define HDG_COUNT_CASE(NV)                                                                              \
-  case NV:                                                                                              \
-    hipLaunchKernelGGL(count_kernel<NV>, grid, dim3(256), 0, s, cells, S.N, S.zcode.as<uint64_t>(), so, CS,     \
-                       w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), w.zlo_cnt.as<uint64_t>(),             \
-                       w.zhi.as<uint64_t>(), qn, w.counts.as<uint32_t>());                                    \
+#define HDG_A3_CASE(NV)                                                                                            \
+  case NV:                                                                                                         \
+    if (sparse) {                                                                                                  \
+      hipLaunchKernelGGL(sparse_count_kernel<NV>, dim3(grid), dim3(64), lds, s, B, S.bstart.as<uint32_t>(),         \
+                         S.ecell.as<uint32_t>(), S.esub.as<uint32_t>(), S.N, S.cluster_of_sub.as<uint32_t>(),      \
+                         (uint32_t)CS, (uint32_t)C, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), qn, G, q0,    \
+                         max_tests, S.epos.as<uint32_t>(), kInboxCap, w.ib_pos.as<uint32_t>(),           \
+                         w.ib_sub.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.sub_list.as<uint32_t>(), w.sub_cnt.as<uint32_t>(), w.ns.as<uint32_t>(),                 \
+                         w.list_c.as<uint32_t>(), w.cl_beg.as<uint32_t>(), w.nnz_c.as<uint32_t>(),                 \
+                         w.dense_list.as<uint32_t>(), w.n_dense.as<uint32_t>(), w.gstat.as<unsigned long long>()); \
+    } else {                                                                                                       \
+      hipLaunchKernelGGL(iota_kernel, dim3((unsigned)cdiv(qn, 256)), dim3(256), 0, s, qn,                          \
+                         w.dense_list.as<uint32_t>(), w.n_dense.as<uint32_t>(), w.n_ib.as<uint32_t>());           \
+    }                                                                                                              \
+    hipLaunchKernelGGL(dense_count_kernel<NV>, dgrid, dim3(256), 0, s, cells, S.N, S.zcode.as<uint64_t>(),          \
+                       S.seg_off.as<uint32_t>(), CS, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(),              \
+                       w.zlo_cnt.as<uint64_t>(), w.zhi.as<uint64_t>(), qn, w.dense_list.as<uint32_t>(),            \
+                       w.n_dense.as<uint32_t>(), w.counts.as<uint32_t>());                                         \
     break;
-      HDG_COUNT_CASE(1)
-      HDG_COUNT_CASE(2)
-      HDG_COUNT_CASE(3)
-      HDG_COUNT_CASE(4)
-      HDG_COUNT_CASE(5)
-      HDG_COUNT_CASE(6)
-      HDG_COUNT_CASE(7)
-      HDG_COUNT_CASE(8)
-#undef HDG_COUNT_CASE
+      HDG_A3_CASE(1)
+      HDG_A3_CASE(2)
+      HDG_A3_CASE(3)
+      HDG_A3_CASE(4)
+      HDG_A3_CASE(5)
+      HDG_A3_CASE(6)
+      HDG_A3_CASE(7)
+      HDG_A3_CASE(8)
+#undef HDG_A3_CASE
       default:
         fail(kBadState, "no valid label dims");
     }
     HDG_HIP(hipGetLastError());
-  };
+    hipLaunchKernelGGL(dense_emit_kernel, dim3((unsigned)qn), dim3(256), 0, s, w.counts.as<uint32_t>(), (uint32_t)CS,
+                       S.sub_off.as<uint32_t>(), (uint32_t)C, G, q0, w.dense_list.as<uint32_t>(),
+                       w.n_dense.as<uint32_t>(), w.sub_list.as<uint32_t>(), w.sub_cnt.as<uint32_t>(),
+                       w.ns.as<uint32_t>(), w.list_c.as<uint32_t>(), w.cl_beg.as<uint32_t>(), w.nnz_c.as<uint32_t>(),
+                       w.gstat.as<unsigned long long>());
+    HDG_HIP(hipGetLastError());
+  }
+  mark();
   auto budgets = [&]() {
     capture_features(q0, qn, l, w.ranges.as<float>(), w.box_w.as<uint32_t>(), g.n_valid, w.feat.as<float>(), s);
     if (budget_T)
       run_budget_model(qn, l, w.ranges.as<float>(), w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), g.n_valid,
                        w.feat.as<float>(), w.n_chosen_c.as<uint32_t>(), budget_T, budget_S, s);
   };
-  if (lazy) {
-    launch_coarse_gemm(queries, q0, qn, d, S.Dp, S.centers_bf16.as<uint16_t>(), S.center_norm.as<float>(),
-                       (uint32_t)C, w.S.as<float>(), s);
-    HDG_HIP(hipGetLastError());
-    mark();
-    launch_coarse_topk(queries, q0, qn, d, w.S.as<float>(), (uint32_t)C, n_list, (uint32_t)cfg_.probe_clusters,
-                       nullptr, w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>(), w.feat.as<float>(), s);
-    HDG_HIP(hipGetLastError());
-    budgets();
-    mark();
-    launch_budget_count(g.n_valid, S.cells.as<uint32_t>(), S.N, S.zcode.as<uint64_t>(), S.seg_off.as<uint32_t>(),
-                        S.sub_off.as<uint32_t>(), CS, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(),
-                        w.zlo_cnt.as<uint64_t>(), w.zhi.as<uint64_t>(), qn, n_list, w.chosen_c.as<uint32_t>(),
-                        w.n_chosen_c.as<uint32_t>(), w.counts.as<uint32_t>(), s);
-    HDG_HIP(hipGetLastError());
-    mark();
+  // ---- A5: cluster choice ----
+  if (!mixed) {
+    hipLaunchKernelGGL(chunk_scan_kernel, dim3(1), dim3(1024), 0, s, w.nnz_c.as<uint32_t>(), nullptr, 0u, qn,
+                       w.woff.as<uint64_t>());
+    hipLaunchKernelGGL(pair_dist_kernel<0>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.centers.as<float>(),
+                       w.woff.as<uint64_t>(), qn, w.list_c.as<uint32_t>(), C, nullptr, w.nnz_c.as<uint32_t>(),
+                       nullptr, w.keys.as<uint64_t>());
+    hipLaunchKernelGGL(coarse_pick_kernel, dim3((unsigned)qn), dim3(256), scap_c * 8, s, (uint32_t)C, n_list, G, q0,
+                       scap_c,
+                       w.nnz_c.as<uint32_t>(), w.gstat.as<unsigned long long>(), w.keys.as<uint64_t>(),
+                       w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>());
   } else {
-    launch_full_count();
-    mark();
-    hipLaunchKernelGGL(cluster_nnz_kernel, dim3((unsigned)qn), dim3(256), 0, s, w.counts.as<uint32_t>(), CS,
-                       S.sub_off.as<uint32_t>(), (uint32_t)C, G, q0, w.list_c.as<uint32_t>(), w.nnz_c.as<uint32_t>(),
-                       w.gstat.as<unsigned long long>());
-    HDG_HIP(hipGetLastError());
-    mark();
-    if (!mixed) {
-      hipLaunchKernelGGL(coarse_select_kernel, dim3((unsigned)qn), dim3(256), sort_lds, s, queries, d,
-                         S.centers.as<float>(), (uint32_t)C, n_list, G, q0, w.list_c.as<uint32_t>(),
-                         w.nnz_c.as<uint32_t>(), w.gstat.as<unsigned long long>(), w.keys.as<uint64_t>(),
-                         w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>());
-    } else {
-      launch_coarse_gemm(queries, q0, qn, d, S.Dp, S.centers_bf16.as<uint16_t>(), S.center_norm.as<float>(),
-                         (uint32_t)C, w.S.as<float>(), s);
-      launch_scored_select(queries, q0, qn, d, w.list_c.as<uint32_t>(), (uint32_t)C, w.nnz_c.as<uint32_t>(),
-                           w.S.as<float>(), C, n_list, nullptr, n_list, w.chosen_c.as<uint32_t>(),
-                           w.n_chosen_c.as<uint32_t>(), w.feat.as<float>(), w.keys.as<uint64_t>(), s);
-      budgets();
-    }
-    HDG_HIP(hipGetLastError());
-    mark();
+    const uint32_t kc = cfg_.probe_clusters > 0 ? std::min<uint32_t>(n_list, (uint32_t)cfg_.probe_clusters) : n_list;
+    launch_coarse_gemm(w.qb16.as<uint16_t>(), qn, S.Dp, S.centers_bf16.as<uint16_t>(), S.center_norm.as<float>(),
+                       (uint32_t)C, w.S.as<float>(), s);
+    launch_scored_select(queries, q0, qn, d, w.list_c.as<uint32_t>(), (uint32_t)C, w.nnz_c.as<uint32_t>(),
+                         w.S.as<float>(), C, kc, nullptr, n_list, w.chosen_c.as<uint32_t>(),
+                         w.n_chosen_c.as<uint32_t>(), w.feat.as<float>(), w.keys.as<uint64_t>(), s);
+    budgets();
   }
-  hipLaunchKernelGGL(sub_nnz_kernel, dim3((unsigned)qn), dim3(256), 0, s, w.counts.as<uint32_t>(), CS,
-                     S.sub_off.as<uint32_t>(), n_list, G, q0, w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>(),
-                     cap, w.list_s.as<uint32_t>(), w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>());
   HDG_HIP(hipGetLastError());
   mark();
-  if (mixed && !lazy) {
-    launch_coarse_gemm(queries, q0, qn, d, S.Dp, S.sub_centers_bf16.as<uint16_t>(), S.sub_center_norm.as<float>(),
-                       (uint32_t)CS, w.S.as<float>(), s);
+  // ---- A6: sub-cluster choice ----
+  hipLaunchKernelGGL(sub_expand_kernel, dim3((unsigned)qn), dim3(256), 0, s, (uint32_t)CS, (uint32_t)C,
+                     S.sub_off.as<uint32_t>(), n_list, G, q0, w.list_c.as<uint32_t>(), w.cl_beg.as<uint32_t>(),
+                     w.sub_list.as<uint32_t>(), w.sub_cnt.as<uint32_t>(), w.chosen_c.as<uint32_t>(),
+                     w.n_chosen_c.as<uint32_t>(), cap, w.list_s.as<uint32_t>(), w.list_scnt.as<uint32_t>(),
+                     w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>());
+  HDG_HIP(hipGetLastError());
+  mark();
+  if (!mixed) {
+    hipLaunchKernelGGL(chunk_scan_kernel, dim3(1), dim3(1024), 0, s, w.nnz_s.as<uint32_t>(), nullptr, 0u, qn,
+                       w.woff.as<uint64_t>());
+    hipLaunchKernelGGL(pair_dist_kernel<0>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d,
+                       S.sub_centers.as<float>(), w.woff.as<uint64_t>(), qn, w.list_s.as<uint32_t>(), (uint64_t)cap,
+                       nullptr, w.nnz_s.as<uint32_t>(), nullptr, w.keys.as<uint64_t>());
+    hipLaunchKernelGGL(sub_pick_kernel, dim3((unsigned)qn), dim3(256), scap_s * 8, s, sub_lists, G, q0, cap, scap_s,
+                       w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>(), budget_S, w.keys.as<uint64_t>(),
+                       w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>());
+  } else {
+    launch_coarse_gemm(w.qb16.as<uint16_t>(), qn, S.Dp, S.sub_centers_bf16.as<uint16_t>(),
+                       S.sub_center_norm.as<float>(), (uint32_t)CS, w.S.as<float>(), s);
     launch_scored_select(queries, q0, qn, d, w.list_s.as<uint32_t>(), cap, w.nnz_s.as<uint32_t>(), w.S.as<float>(),
                          CS, sub_lists, budget_S, sub_lists, w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(),
                          nullptr, w.keys.as<uint64_t>(), s);
-  } else {
-    hipLaunchKernelGGL(sub_select_kernel, dim3((unsigned)qn), dim3(256), sort_lds, s, queries, d,
-                       S.sub_centers.as<float>(), sub_lists, G, q0, cap, w.list_s.as<uint32_t>(),
-                       w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>(), budget_S, w.keys.as<uint64_t>(),
-                       w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>());
   }
   HDG_HIP(hipGetLastError());
   mark();
   hipLaunchKernelGGL(ranges_kernel, dim3((unsigned)qn), dim3(256), 0, s, S.zcode.as<uint64_t>(),
-                     S.seg_off.as<uint32_t>(), w.counts.as<uint32_t>(), CS, sub_lists, G, q0,
-                     w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.zlo.as<uint64_t>(),
-                     w.zhi.as<uint64_t>(), w.rstart.as<uint32_t>(), w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(),
-                     w.real.as<uint64_t>(), w.gstat.as<unsigned long long>());
+                     S.seg_off.as<uint32_t>(), cap, w.list_s.as<uint32_t>(), w.list_scnt.as<uint32_t>(), sub_lists, G,
+                     q0, w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.zlo.as<uint64_t>(),
+                     w.zhi.as<uint64_t>(), w.chosen_sid.as<uint32_t>(), w.rstart.as<uint32_t>(),
+                     w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.real.as<uint64_t>(),
+                     w.gstat.as<unsigned long long>());
   HDG_HIP(hipGetLastError());
   mark();
   hipLaunchKernelGGL(plan_kernel, dim3((unsigned)cdiv(qn, 128)), dim3(128), 0, s, qn, q0, G, l, cfg_.exps0,
@@ -1195,9 +1733,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(),
                      w.chunks.as<uint64_t>(), w.pad32.as<uint64_t>(), w.padmax.as<uint64_t>());
   HDG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(256), 0, s, w.lsize.as<uint64_t>(), qn);
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(256), 0, s, w.asize.as<uint64_t>(), qn);
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(256), 0, s, w.chunks.as<uint64_t>(), qn);
+  hipLaunchKernelGGL(scan3_kernel, dim3(1), dim3(1024), 0, s, w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(),
+                     w.chunks.as<uint64_t>(), qn);
   HDG_HIP(hipGetLastError());
   // the one host synchronisation of a chunk: sizes of the data-dependent buffers
   HDG_HIP(hipMemcpyAsync(&w.pinned[0], w.lsize.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
@@ -1211,9 +1748,18 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.L.reserve(std::max<uint64_t>(totL, 1) * 4);
   w.Lkeys.reserve(std::max<uint64_t>(totL, 1) * 8);
   w.akeys.reserve(std::max<uint64_t>(totA, 1) * 8);
-  hipLaunchKernelGGL(keep_all_kernel, dim3((unsigned)qn), dim3(256), 0, s, qn, l, sub_lists, w.mode.as<uint32_t>(),
-                     w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(), w.rpre.as<uint32_t>(), S.lab.as<float>(),
-                     w.ranges.as<float>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>());
+  const int integral = S.labels_integral ? 1 : 0;
+  if (sparse)
+    hipLaunchKernelGGL(inbox_keep_kernel, dim3((unsigned)qn), dim3(256), 0, s, l, integral, (uint32_t)CS, sub_lists,
+                       w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(), w.zlo_cnt.as<uint64_t>(),
+                       w.n_chosen_s.as<uint32_t>(), w.chosen_sid.as<uint32_t>(), kInboxCap, w.ib_pos.as<uint32_t>(),
+                       w.ib_sub.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
+                       w.L.as<uint32_t>(), w.nL.as<uint32_t>());
+  hipLaunchKernelGGL(keep_all_kernel, dim3((unsigned)qn), dim3(256), (2 * sub_lists + 1) * 4, s, qn, l, integral,
+                     sub_lists, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
+                     w.zlo_cnt.as<uint64_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),
+                     w.rpre.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
+                     w.L.as<uint32_t>(), w.nL.as<uint32_t>());
   HDG_HIP(hipGetLastError());
   mark();
   if (totChunks > 0) {
@@ -1232,56 +1778,33 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     HDG_HIP(hipGetLastError());
   }
   mark();
-  hipLaunchKernelGGL(final_kernel, dim3((unsigned)qn), dim3(256), sort_lds, s, qn, q0, queries, d,
-                     S.vec.as<float>(), S.zid.as<uint32_t>(), l, topk, w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(),
-                     w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), w.pad32.as<uint64_t>(),
-                     w.padmax.as<uint64_t>(), w.Lkeys.as<uint64_t>(), w.akeys.as<uint64_t>(), w.asize.as<uint64_t>(),
-                     out_dist, out_ids);
+  // ---- A13: exact re-rank of the refine lists, final top-k ----
+  hipLaunchKernelGGL(chunk_scan_kernel, dim3(1), dim3(1024), 0, s, w.nL.as<uint32_t>(), w.mode.as<uint32_t>(), 2u, qn,
+                     w.woff.as<uint64_t>());
+  hipLaunchKernelGGL(pair_dist_kernel<1>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.vec.as<float>(),
+                     w.woff.as<uint64_t>(), qn, w.L.as<uint32_t>(), (uint64_t)0, w.lsize.as<uint64_t>(),
+                     w.nL.as<uint32_t>(), S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>());
+  hipLaunchKernelGGL(final_pick_kernel, dim3((unsigned)qn), dim3(256), scap_f * 8, s, q0, scap_f, S.zid.as<uint32_t>(), l,
+                     topk,
+                     w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(),
+                     w.nL.as<uint32_t>(), w.pad32.as<uint64_t>(), w.padmax.as<uint64_t>(), w.Lkeys.as<uint64_t>(),
+                     w.akeys.as<uint64_t>(), w.asize.as<uint64_t>(), out_dist, out_ids);
   HDG_HIP(hipGetLastError());
   mark();
-  // per-query statistics (host copies)
-  {
-    std::vector<uint64_t> cand(qn), real(qn), kp(qn);
-    std::vector<uint32_t> nl(qn), mode(qn), nc(qn), ns(qn);
-    HDG_HIP(hipMemcpyAsync(cand.data(), w.cand.data(), qn * 8, hipMemcpyDeviceToHost, s));
-    HDG_HIP(hipMemcpyAsync(real.data(), w.real.data(), qn * 8, hipMemcpyDeviceToHost, s));
-    HDG_HIP(hipMemcpyAsync(kp.data(), w.kprime.data(), qn * 8, hipMemcpyDeviceToHost, s));
-    HDG_HIP(hipMemcpyAsync(nl.data(), w.nL.data(), qn * 4, hipMemcpyDeviceToHost, s));
-    HDG_HIP(hipMemcpyAsync(mode.data(), w.mode.data(), qn * 4, hipMemcpyDeviceToHost, s));
-    HDG_HIP(hipMemcpyAsync(nc.data(), lazy ? w.n_chosen_c.data() : w.nnz_c.data(), qn * 4, hipMemcpyDeviceToHost,
-                           s));
-    HDG_HIP(hipMemcpyAsync(ns.data(), w.nnz_s.data(), qn * 4, hipMemcpyDeviceToHost, s));
-    stats_.chosen_subs.resize(qn * sub_lists);
-    HDG_HIP(hipMemcpyAsync(stats_.chosen_subs.data(), w.chosen_s.data(), qn * sub_lists * 4, hipMemcpyDeviceToHost, s));
-    HDG_HIP(hipStreamSynchronize(s));
-    for (uint64_t i = 0; i < qn; ++i) {
-      stats_.cand[q0 + i] = cand[i];
-      stats_.real[q0 + i] = real[i];
-      stats_.kprime[q0 + i] = kp[i];
-      stats_.refined[q0 + i] = nl[i];
-      stats_.adc[q0 + i] = mode[i] == 1 ? cand[i] : 0;
-      stats_.scored[q0 + i] = (uint64_t)nc[i] + ns[i];
-    }
-    static const char* plain_names[] = {"prep_kernel",    "count_kernel",      "cluster_nnz_kernel",
-                                        "coarse_select_kernel", "sub_nnz_kernel", "sub_select_kernel",
-                                        "ranges_kernel",  "plan+sync",         "keep_all_kernel",
-                                        "adc_kernels",    "final_kernel"};
-    static const char* lazy_names[] = {"prep_kernel",    "coarse_gemm_kernel", "coarse_topk+budget",
-                                       "budget_count_kernel", "sub_nnz_kernel", "sub_select_kernel",
-                                       "ranges_kernel",  "plan+sync",          "keep_all_kernel",
-                                       "adc_kernels",    "final_kernel"};
-    static const char* mixed_names[] = {"prep_kernel",    "count_kernel",        "cluster_nnz_kernel",
-                                        "coarse_gemm+select+budget", "sub_nnz_kernel", "sub_gemm+select",
-                                        "ranges_kernel",  "plan+sync",           "keep_all_kernel",
-                                        "adc_kernels",    "final_kernel"};
-    const char* const* names = lazy ? lazy_names : (mixed ? mixed_names : plain_names);
-    const int nnames = 11;
-    for (int i = 0; i + 1 < evn && i < nnames; ++i) {
-      float ms = 0;
-      HDG_HIP(hipEventElapsedTime(&ms, w.ev[i], w.ev[i + 1]));
-      stage_ms.push_back({names[i], ms});
-    }
-  }
+  hipLaunchKernelGGL(stats_kernel, dim3((unsigned)cdiv(qn, 256)), dim3(256), 0, s, qn, q0, w.cand.as<uint64_t>(),
+                     w.real.as<uint64_t>(), w.nL.as<uint32_t>(), w.nnz_c.as<uint32_t>(), w.nnz_s.as<uint32_t>(),
+                     w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.st.as<uint64_t>());
+  HDG_HIP(hipGetLastError());
+  HDG_HIP(hipMemcpyAsync(w.st_chosen.as<uint32_t>() + q0 * sub_lists, w.chosen_sid.data(), qn * sub_lists * 4,
+                         hipMemcpyDeviceToDevice, s));
+  static const char* plain_names[] = {"prep_kernel",     "count_kernels",   "coarse_dist+pick", "sub_expand_kernel",
+                                      "sub_dist+pick",   "ranges_kernel",   "plan+sync",        "keep_all_kernel",
+                                      "adc_kernels",     "final_dist+pick"};
+  static const char* mixed_names[] = {"prep_kernel",     "count_kernels",   "coarse_gemm+select+budget",
+                                      "sub_expand_kernel", "sub_gemm+select", "ranges_kernel", "plan+sync",
+                                      "keep_all_kernel", "adc_kernels",     "final_dist+pick"};
+  w.ev_names = mixed ? mixed_names : plain_names;
+  w.ev_used = evn;
 }
 
 }  // namespace hdg

@@ -55,17 +55,24 @@ def test_coarse_scores_mfma_layout(mixed_index):
     assert agree > 0.9, agree
 
 
-def test_mixed_full_probe_matches_plain(mixed_index):
+def test_mixed_without_pruning_matches_plain(mixed_index):
+    """With every non-zero cluster and sub-cluster kept, BF16 scoring changes only the order in which they
+    are visited; in keep-all mode the candidate set, beta and the exact re-rank are those of the plain
+    path, so results are bit-identical."""
     m = mixed_index
     idx = m["idx"]
+    idx.set_param("SUB_LISTS", 4096)
     idx.set_param("PRECISION", 0)
     d0, i0 = idx.query(m["q"], m["ql"])
+    st0 = idx.last_stats(m["q"].shape[0])
     idx.set_param("PRECISION", 1)
-    idx.set_param("PROBE_CLUSTERS", 64)
+    idx.set_param("PROBE_CLUSTERS", 0)
     d1, i1 = idx.query(m["q"], m["ql"])
     st = idx.last_stats(m["q"].shape[0])
     idx.set_param("PRECISION", 0)
-    assert (st.adc == 0).all()  # keep-all path: FP16 LUT not involved, so results must be identical
+    idx.set_param("SUB_LISTS", 64)
+    assert (st.adc == 0).all() and (st0.adc == 0).all()  # keep-all: no PQ ordering involved
+    np.testing.assert_array_equal(st0.cand, st.cand)
     np.testing.assert_array_equal(i0.cpu().numpy(), i1.cpu().numpy())
     np.testing.assert_array_equal(d0.cpu().numpy(), d1.cpu().numpy())
 
@@ -93,8 +100,9 @@ def test_mixed_adc_fp16_lut_close_to_fp32():
     idx.build(torch.from_numpy(w.data).to(DEV), torch.from_numpy(w.labels).to(DEV))
     q, ql = torch.from_numpy(w.queries).to(DEV), torch.from_numpy(w.qlabels).to(DEV)
     idx.set_param("EXPS0", 2)
+    idx.set_param("SUB_LISTS", 512)  # every non-zero sub-cluster on both paths: same candidates
     idx.set_param("PRECISION", 1)
-    idx.set_param("PROBE_CLUSTERS", 64)
+    idx.set_param("PROBE_CLUSTERS", 0)
     _, i1 = idx.query(q, ql)
     assert (idx.last_stats(200).adc > 0).any()
     idx.set_param("PRECISION", 0)

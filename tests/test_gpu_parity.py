@@ -121,6 +121,17 @@ def l3():
 
 
 @pytest.fixture(scope="module")
+def l3_frac():
+    """Non-integer labels on the length dim: a label survivor can sit outside the cell box, so the
+    keep-all list must come from the Morton-range candidates."""
+    w = datasets.mixed_1m(n=40000, nq=300, d=64, components=32, clusters=64, sub_clusters=6, n_list=64,
+                          sub_lists=64)
+    w.labels[:, 1] += np.float32(0.375)
+    w.qlabels[:, 1] += np.float32(0.625)
+    return Built(w)
+
+
+@pytest.fixture(scope="module")
 def l2_explicit():
     w = datasets.explicit_ranges(n=40000, nq=200, d=64, components=64, clusters=64, sub_clusters=6)
     return Built(w)
@@ -165,6 +176,29 @@ def test_l3_small_batch(l3):
 def test_l2_explicit_adc(l2_explicit):
     res = l2_explicit.run(EXPS0=2)
     assert_parity(res, "explicit ranges exps0=2")
+
+
+def test_l3_dense_count(l3):
+    """Every query through the dense Morton-range count instead of the bucket-grid walk."""
+    assert_parity(l3.run(EXPS0=30, SPARSE_COUNT=0), "l3 dense count")
+    l3.idx.set_param("SPARSE_COUNT", 1)
+
+
+def test_l2_explicit_mixed_count_paths(l2_explicit):
+    """Uneven selectivity with a small walk budget: wide boxes go to the dense count, narrow ones stay on
+    the walk, in the same call."""
+    res = l2_explicit.run(EXPS0=2, SPARSE_MAX_TESTS=3000)
+    assert_parity(res, "explicit ranges, split count paths")
+    l2_explicit.idx.set_param("SPARSE_MAX_TESTS", 131072)
+
+
+def test_l1_dense_count(l1_wide):
+    assert_parity(l1_wide.run(EXPS0=2, SPARSE_COUNT=0), "l1 wide dense count")
+    l1_wide.idx.set_param("SPARSE_COUNT", 1)
+
+
+def test_l3_fractional_labels(l3_frac):
+    assert_parity(l3_frac.run(EXPS0=30), "l3 fractional labels")
 
 
 def test_recall_vs_bruteforce(l3):
