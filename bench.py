@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline")
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--paths", default="plain,mixed,learned")
+    ap.add_argument("--save-budget", default=None, help="write the trained budget model here")
+    ap.add_argument("--load-budget", default=None, help="use this budget model instead of training one")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
@@ -131,17 +133,24 @@ def main():
         res["mixed"] = dict(qps=qps, ms=ms, recall=r, sub_lists=b, stage=stage, st=st)
     # ---- learned budget + mixed ----
     if "learned" in paths:
-        t0 = time.time()
-        tq, tql = w.sample_queries(args.train, 0x7E57)
-        tr = O.recover_filters(tql, 3, **w.filter_arrays)
-        _, tgt = O.bruteforce(w.data, w.labels, tq, tr, 100)
-        log(f"training set: {args.train} queries, ground truth in {time.time() - t0:.1f}s")
         cap = max(res["mixed"]["sub_lists"] * 2, 256)
         idx.set_param("SUB_LISTS", cap)
-        t0 = time.time()
-        rep = idx.budget_fit(torch.from_numpy(tq).to(dev), torch.from_numpy(tql).to(dev), tgt, levels=12, t0=0,
-                             s0=8, ratio=2 ** 0.5, target=args.target_recall, epochs=30)
-        log(f"budget model trained in {time.time() - t0:.1f}s: {rep}")
+        if args.load_budget:
+            idx.budget_load(args.load_budget)
+            rep = {}
+            log(f"budget model loaded from {args.load_budget}")
+        else:
+            t0 = time.time()
+            tq, tql = w.sample_queries(args.train, 0x7E57)
+            tr = O.recover_filters(tql, 3, **w.filter_arrays)
+            _, tgt = O.bruteforce(w.data, w.labels, tq, tr, 100)
+            log(f"training set: {args.train} queries, ground truth in {time.time() - t0:.1f}s")
+            t0 = time.time()
+            rep = idx.budget_fit(torch.from_numpy(tq).to(dev), torch.from_numpy(tql).to(dev), tgt, levels=12,
+                                 t0=0, s0=8, ratio=2 ** 0.5, target=args.target_recall, epochs=30)
+            log(f"budget model trained in {time.time() - t0:.1f}s: {rep}")
+            if args.save_budget:
+                idx.budget_save(args.save_budget)
         idx.set_param("BUDGET_MODEL", 1)
         r = recall_now()
         log(f"  learned budget: recall@100={r:.4f}")
@@ -203,7 +212,7 @@ def main():
         for k in res:
             if k != "plain":
                 config[f"{k}_vs_plain"] = round(res[k]["qps"] / res["plain"]["qps"], 3)
-    if "learned" in res:
+    if "learned" in res and res["learned"]["report"]:
         config["budget_model"] = {k: round(v, 4) for k, v in res["learned"]["report"].items()}
         config["budget_model"]["train_queries"] = args.train
 

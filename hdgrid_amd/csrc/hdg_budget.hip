@@ -86,6 +86,11 @@ __device__ float mlp_forward(const float* w, const float* x, float* z1, float* z
 }
 
 // Inference: features -> standardise -> MLP -> level -> (clusters, sub-clusters); trims n_chosen_c.
+// 16 lanes per query: each lane owns 4 hidden units of each layer (every unit's sum runs over its inputs
+// in order, as in mlp_forward), the output sum is taken by the group's first lane.  A workgroup of 256
+// threads serves 16 queries and keeps the bf16 weights in LDS.
+constexpr int kInferQ = 16;
+
 __global__ __launch_bounds__(256) void infer_kernel(uint64_t qn, int l, const float* __restrict__ ranges,
                                                     const uint32_t* __restrict__ box_w, int n_valid,
                                                     const float* __restrict__ feat_dist,
@@ -96,14 +101,42 @@ __global__ __launch_bounds__(256) void infer_kernel(uint64_t qn, int l, const fl
                                                     uint32_t* __restrict__ n_chosen_c, uint32_t* __restrict__ bT,
                                                     uint32_t* __restrict__ bS) {
   __shared__ float w[kParams];
+  __shared__ float xs[kInferQ][kF];
+  __shared__ float h1s[kInferQ][kH];
+  __shared__ float h2s[kInferQ][kH];
   for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = wq[i];
+  const int g = threadIdx.x & 15, qg = threadIdx.x >> 4;
+  const uint64_t qi = (uint64_t)blockIdx.x * kInferQ + qg;
+  const bool live = qi < qn;
+  if (live && g == 0) {
+    float f[kF];
+    query_features(qi, l, ranges, box_w, n_valid, qn, feat_dist, f);
+    for (int i = 0; i < kF; ++i) xs[qg][i] = bf16r((f[i] - mean[i]) * inv_std[i]);
+  }
   __syncthreads();
-  const uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (qi >= qn) return;
-  float f[kF];
-  query_features(qi, l, ranges, box_w, n_valid, qn, feat_dist, f);
-  for (int i = 0; i < kF; ++i) f[i] = (f[i] - mean[i]) * inv_std[i];
-  const float y = mlp_forward(w, f, nullptr, nullptr);
+  if (live) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = g * 4 + u;
+      float a = w[kOffB1 + j];
+      for (int i = 0; i < kF; ++i) a += xs[qg][i] * w[kOffW1 + i * kH + j];
+      h1s[qg][j] = bf16r(a > 0.0f ? a : 0.0f);
+    }
+  }
+  __syncthreads();
+  if (live) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = g * 4 + u;
+      float a = w[kOffB2 + j];
+      for (int i = 0; i < kH; ++i) a += h1s[qg][i] * w[kOffW2 + i * kH + j];
+      h2s[qg][j] = bf16r(a > 0.0f ? a : 0.0f);
+    }
+  }
+  __syncthreads();
+  if (!live || g != 0) return;
+  float y = w[kOffB3];
+  for (int i = 0; i < kH; ++i) y += h2s[qg][i] * w[kOffW3 + i];
   int lev = levels - 1;  // fallback: the fixed budget
   if (isfinite(y)) {
     float t = ceilf(y + margin);
@@ -210,7 +243,7 @@ void Index::run_budget_model(uint64_t qn, int l, const float* ranges, const uint
                              uint32_t* budget_S, hipStream_t s) {
   (void)box_lo;
   BudgetModel& B = *budget_;
-  hipLaunchKernelGGL(infer_kernel, dim3((unsigned)((qn + 255) / 256)), dim3(256), 0, s, qn, l, ranges, box_w,
+  hipLaunchKernelGGL(infer_kernel, dim3((unsigned)((qn + kInferQ - 1) / kInferQ)), dim3(256), 0, s, qn, l, ranges, box_w,
                      n_valid, feat_dist, B.d_w.as<float>(), B.d_mean.as<float>(), B.d_inv.as<float>(), B.margin,
                      (int)B.ladT.size(), B.d_ladT.as<uint32_t>(), B.d_ladS.as<uint32_t>(), n_chosen_c, budget_T,
                      budget_S);

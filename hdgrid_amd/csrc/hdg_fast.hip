@@ -50,8 +50,10 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
                                                           const uint16_t* __restrict__ cb16,
                                                           const float* __restrict__ cnorm, uint32_t C,
                                                           float* __restrict__ S) {
-  __shared__ __align__(16) uint16_t As[kGemmTile * kGemmLd];
-  __shared__ __align__(16) uint16_t Bs[kGemmTile * kGemmLd];
+  // operand tiles, reused by the epilogue as 4 waves x 16 rows x 65 floats
+  __shared__ __align__(16) uint16_t smem[2 * kGemmTile * kGemmLd];
+  uint16_t* As = smem;
+  uint16_t* Bs = smem + kGemmTile * kGemmLd;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = wave >> 1, wn = wave & 1;
   const uint64_t qbase = (uint64_t)blockIdx.x * kGemmTile;
@@ -95,19 +97,28 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
       for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     __syncthreads();
   }
+  // epilogue: each wave passes its 64 x 64 block through LDS 16 rows at a time, so that every store
+  // writes 256 contiguous bytes of a row of S
+  float* ep = reinterpret_cast<float*>(smem) + wave * 16 * 65;
+  float cn[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const uint64_t c = cbase + wn * 64 + j * 16 + r16;
-    if (c >= C) continue;
-    const float cn = cnorm[c];
+    cn[j] = c < C ? cnorm[c] : 0.0f;
+  }
+  const uint64_t cw = cbase + wn * 64 + lane;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < 4; ++i) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const uint64_t q = qbase + wm * 64 + i * 16 + kg * 4 + t;
-        if (q < qn) S[q * C + c] = cn - 2.0f * acc[i][j][t];
-      }
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) ep[(kg * 4 + t) * 65 + j * 16 + r16] = cn[j] - 2.0f * acc[i][j][t];
+    __syncthreads();
+    for (int r = 0; r < 16; ++r) {
+      const uint64_t q = qbase + wm * 64 + i * 16 + r;
+      if (q < qn && cw < C) S[q * C + cw] = ep[r * 65 + lane];
     }
+    __syncthreads();
   }
 }
 

@@ -261,7 +261,7 @@ __global__ __launch_bounds__(64) void sparse_count_kernel(
         }
         // four entries per lane per pass, so their loads are in flight together
         for (uint32_t t0 = 0; t0 < total; t0 += 256) {
-          uint32_t ev[4], cv[4][NV], sv[4];
+          uint32_t ev[4], cv[4][NV], sv[4], pv[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const uint32_t t = t0 + (uint32_t)lane + 64u * u;
@@ -278,6 +278,7 @@ __global__ __launch_bounds__(64) void sparse_count_kernel(
 #pragma unroll
             for (int j = 0; j < NV; ++j) cv[u][j] = ok ? ecell[(uint64_t)j * N + ev[u]] : lo[j] - 1u;
             sv[u] = ok ? esub[ev[u]] : 0u;
+            pv[u] = ok ? epos[ev[u]] : 0u;
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
@@ -291,7 +292,7 @@ __global__ __launch_bounds__(64) void sparse_count_kernel(
               const uint32_t s2 = sv[u];
               if (atomicAdd(&hist[s2], 1u) == 0u) atomicOr(&bits[s2 >> 5], 1u << (s2 & 31));
               if (at < ibcap) {
-                ib_pos[qi * ibcap + at] = epos[ev[u]];
+                ib_pos[qi * ibcap + at] = pv[u];
                 ib_sub[qi * ibcap + at] = s2;
               }
             }
@@ -318,70 +319,48 @@ __global__ __launch_bounds__(64) void sparse_count_kernel(
       __syncthreads();
       continue;
     }
-    // emit the non-zero sub-clusters ascending, and the cluster runs among them
+    // emit the non-zero sub-clusters ascending with their counts, then the cluster runs among them
     uint32_t* sl = sub_list + qi * CS;
     uint32_t* sc = sub_cnt + qi * CS;
     uint32_t* lc = list_c + qi * C;
     uint32_t* cb = cl_beg + qi * (C + 1);
-    uint32_t base = 0, ncl = 0;
-    int carry = -1;  // last emitted sub-cluster
+    uint32_t base = 0;
     for (uint32_t w0 = 0; w0 < nw; w0 += 64) {
       const uint32_t wi = w0 + (uint32_t)lane;
-      const uint32_t word = wi < nw ? bits[wi] : 0u;
+      uint32_t word = wi < nw ? bits[wi] : 0u;
       const uint32_t pc = __popc(word);
       const uint32_t incl = wave_incl_scan(pc, lane);
       const uint32_t tot = __shfl(incl, 63, 64);
-      int mine = word ? (int)(wi * 32 + 31 - __clz(word)) : -1;
-      int mx = mine;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(mx, o, 64);
-        if (lane >= o) mx = mx > y ? mx : y;
-      }
-      int prev = __shfl_up(mx, 1, 64);
-      if (lane == 0) prev = -1;
-      prev = prev > carry ? prev : carry;
-      const int prev_cl = prev >= 0 ? (int)cluster_of_sub[prev] : -1;
-      uint32_t starts = 0;
-      {
-        uint32_t wv = word;
-        int pcl = prev_cl;
-        while (wv) {
-          const uint32_t b = __ffs(wv) - 1;
-          wv &= wv - 1;
-          const int cl = (int)cluster_of_sub[wi * 32 + b];
-          starts += cl != pcl;
-          pcl = cl;
-        }
-      }
-      const uint32_t sincl = wave_incl_scan(starts, lane);
-      const uint32_t stot = __shfl(sincl, 63, 64);
-      uint32_t pos = base + incl - pc, cpos = ncl + sincl - starts;
-      {
-        uint32_t wv = word;
-        int pcl = prev_cl;
-        while (wv) {
-          const uint32_t b = __ffs(wv) - 1;
-          wv &= wv - 1;
-          const uint32_t s = wi * 32 + b;
-          const int cl = (int)cluster_of_sub[s];
-          if (cl != pcl) {
-            lc[cpos] = (uint32_t)cl;
-            cb[cpos] = pos;
-            ++cpos;
-          }
-          pcl = cl;
-          sl[pos] = s;
-          sc[pos] = hist[s];
-          hist[s] = 0;
-          ++pos;
-        }
+      uint32_t pos = base + incl - pc;
+      while (word) {
+        const uint32_t b = __ffs(word) - 1;
+        word &= word - 1;
+        const uint32_t s2 = wi * 32 + b;
+        sl[pos] = s2;
+        sc[pos] = hist[s2];
+        hist[s2] = 0;
+        ++pos;
       }
       if (wi < nw) bits[wi] = 0;
       base += tot;
-      ncl += stot;
-      const int lastmx = __shfl(mx, 63, 64);
-      carry = lastmx > carry ? lastmx : carry;
+    }
+    __syncthreads();  // the wave reads its sub-cluster list back
+    uint32_t ncl = 0;
+    int prev_cl = -1;
+    for (uint32_t i0 = 0; i0 < base; i0 += 64) {
+      const uint32_t i = i0 + (uint32_t)lane;
+      const int cl = i < base ? (int)cluster_of_sub[sl[i]] : -2;
+      int pcl = __shfl_up(cl, 1, 64);
+      if (lane == 0) pcl = prev_cl;
+      const bool start = i < base && cl != pcl;
+      const uint64_t mask = __ballot(start);
+      const uint32_t at = ncl + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+      if (start) {
+        lc[at] = (uint32_t)cl;
+        cb[at] = i;
+      }
+      ncl += (uint32_t)__popcll(mask);
+      prev_cl = __shfl(cl, 63, 64);
     }
     if (lane == 0) {
       ns_out[qi] = base;
@@ -525,7 +504,7 @@ __global__ __launch_bounds__(256) void dense_emit_kernel(uint32_t* __restrict__ 
 // ------------------------------------------------------------------------------------------------
 // Exact FP32 L2 of listed rows, the reference's order for d <= 512 (refine.cu:36-75): a sequential f32
 // sum over dims of rounded squares, one lane per row.  Work = (query, 64 consecutive list entries);
-// woff[q] = chunks before query q (chunk_scan_kernel), woff[qn] = total.  One wave per workgroup; each
+// woff[q] = chunks before query q (scan_local_kernel), woff[qn] = total.  One wave per workgroup; each
 // workgroup takes a contiguous run of chunks, so the query vector is staged once per query.  Rows pass
 // through LDS 64 dims at a time with 16 lanes per row (each wave load reads four whole 256-byte row
 // pieces); the next stage's loads are in flight while the current stage is summed from LDS.
@@ -650,38 +629,81 @@ __global__ __launch_bounds__(64) void pair_dist_kernel(const float* __restrict__
   }
 }
 
-// woff[q] = sum over q' < q of ceil(n[q'] / 64) (0 for queries whose mode equals skip_mode), woff[qn] =
-// total.  One workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void chunk_scan_kernel(const uint32_t* __restrict__ n, const uint32_t* __restrict__ mode,
-                                                          uint32_t skip_mode, uint64_t qn, uint64_t* __restrict__ woff) {
-  __shared__ unsigned long long wsum[16];
-  __shared__ unsigned long long carry_sh;
+// Exclusive scans in two passes over blocks of 1024 entries: scan_local_kernel scans every block and
+// records its total, scan_fix_kernel adds the totals of the blocks before it (there are at most
+// kScanBlocks blocks: QUERY_CHUNK <= 65536).
+constexpr int kScanBlocks = 64;
+
+// Block-wide exclusive scan of NA values per thread (NA independent scans share the barriers).
+template <int NA>
+__device__ __forceinline__ void block_excl_scan(unsigned long long (&v)[NA], unsigned long long (*wsum)[16],
+                                                unsigned long long (&total)[NA]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) carry_sh = 0;
-  __syncthreads();
-  for (uint64_t b = 0; b < qn; b += 1024) {
-    const uint64_t i = b + threadIdx.x;
-    unsigned long long x = 0;
-    if (i < qn && !(mode && mode[i] == skip_mode)) x = (n[i] + 63u) / 64u;
-    unsigned long long v = x;
+  unsigned long long x[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    x[a] = v[a];
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-      const unsigned long long y = __shfl_up(v, o, 64);
-      if (lane >= o) v += y;
+      const unsigned long long y = __shfl_up(x[a], o, 64);
+      if (lane >= o) x[a] += y;
     }
-    if (lane == 63) wsum[wave] = v;
-    __syncthreads();
-    unsigned long long before = carry_sh, tot = 0;
-    for (int w2 = 0; w2 < 16; ++w2) {
-      if (w2 < wave) before += wsum[w2];
-      tot += wsum[w2];
-    }
-    if (i < qn) woff[i] = before + v - x;
-    __syncthreads();
-    if (threadIdx.x == 0) carry_sh += tot;
-    __syncthreads();
+    if (lane == 63) wsum[a][wave] = x[a];
   }
-  if (threadIdx.x == 0) woff[qn] = carry_sh;
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    unsigned long long before = 0, tot = 0;
+    for (int w2 = 0; w2 < 16; ++w2) {
+      const unsigned long long t = wsum[a][w2];
+      if (w2 < wave) before += t;
+      tot += t;
+    }
+    total[a] = tot;
+    v[a] = before + x[a] - v[a];
+  }
+}
+
+// In-place local scans of up to three u64 arrays (null = absent); block totals to bsum[a][block].
+// With `n32` given, array 0 is first filled from it as ceil(n32[i] / 64), 0 where mode[i] == skip_mode.
+__global__ __launch_bounds__(1024) void scan_local_kernel(uint64_t* __restrict__ v0, uint64_t* __restrict__ v1,
+                                                          uint64_t* __restrict__ v2, uint64_t n,
+                                                          const uint32_t* __restrict__ n32,
+                                                          const uint32_t* __restrict__ mode, uint32_t skip_mode,
+                                                          unsigned long long* __restrict__ bsum) {
+  __shared__ unsigned long long wsum[3][16];
+  uint64_t* const vs[3] = {v0, v1, v2};
+  const uint64_t i = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
+  unsigned long long x[3], tot[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) x[a] = (vs[a] && i < n) ? vs[a][i] : 0ull;
+  if (n32) x[0] = (i < n && !(mode && mode[i] == skip_mode)) ? (n32[i] + 63u) / 64u : 0ull;
+  block_excl_scan<3>(x, wsum, tot);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (vs[a] && i < n) vs[a][i] = x[a];
+    if (threadIdx.x == 0) bsum[a * kScanBlocks + blockIdx.x] = tot[a];
+  }
+}
+
+__global__ __launch_bounds__(1024) void scan_fix_kernel(uint64_t* __restrict__ v0, uint64_t* __restrict__ v1,
+                                                        uint64_t* __restrict__ v2, uint64_t n,
+                                                        const unsigned long long* __restrict__ bsum) {
+  uint64_t* const vs[3] = {v0, v1, v2};
+  const uint64_t i = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
+  const uint32_t nb = (uint32_t)((n + 1023) / 1024);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (!vs[a]) continue;
+    unsigned long long before = 0, all = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+      const unsigned long long t = bsum[a * kScanBlocks + b];
+      if (b < blockIdx.x) before += t;
+      all += t;
+    }
+    if (i < n) vs[a][i] += before;
+    if (blockIdx.x == 0 && threadIdx.x == 0) vs[a][n] = all;
+  }
 }
 
 // Sort keys[0..n) (global scratch) ascending into `lds` (capacity `cap` keys, a power of two) when n <= cap,
@@ -841,12 +863,36 @@ __global__ __launch_bounds__(256) void ranges_kernel(
     uint64_t q0, const uint32_t* __restrict__ chosen_s, const uint32_t* __restrict__ n_chosen_s,
     const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zhi, uint32_t* __restrict__ chosen_sid,
     uint32_t* __restrict__ rstart, uint32_t* __restrict__ rpre, uint64_t* __restrict__ cand,
-    uint64_t* __restrict__ real, unsigned long long* __restrict__ gstat) {
+    uint64_t* __restrict__ real, unsigned long long* __restrict__ gstat, const uint32_t* __restrict__ n_ib,
+    int inbox_mode) {
   __shared__ uint32_t tmp[8];
   __shared__ uint32_t base_sh;
   __shared__ unsigned long long real_sh;
   const uint64_t qi = blockIdx.x;
   const uint32_t n = n_chosen_s[qi];
+  if (inbox_mode && n_ib[qi] != kNone) {
+    // refined from its in-box list (mode 3): only the chosen ids and the in-box total are needed
+    unsigned long long r = 0;
+    for (uint32_t j = threadIdx.x; j < sub_lists; j += kWG) {
+      uint32_t sid = kNone;
+      if (j < n) {
+        const uint32_t idx = chosen_s[qi * sub_lists + j];
+        sid = list_s[qi * cap + idx];
+        r += list_scnt[qi * cap + idx];
+      }
+      chosen_sid[qi * sub_lists + j] = sid;
+    }
+    if (threadIdx.x == 0) real_sh = 0;
+    __syncthreads();
+    atomicAdd(&real_sh, r);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      cand[qi] = 0;
+      real[qi] = real_sh;
+      rpre[qi * (sub_lists + 1)] = 0;
+    }
+    return;
+  }
   const uint64_t a = zlo[qi], b = zhi[qi] + 1;
   if (threadIdx.x == 0) {
     base_sh = 0;
@@ -916,9 +962,21 @@ __global__ void plan_kernel(uint64_t qn, uint64_t q0, uint64_t G, int l, uint64_
                             uint32_t* __restrict__ mode, uint64_t* __restrict__ kprime,
                             uint64_t* __restrict__ lsize, uint64_t* __restrict__ asize,
                             uint64_t* __restrict__ chunks, uint64_t* __restrict__ pad32,
-                            uint64_t* __restrict__ padmax) {
+                            uint64_t* __restrict__ padmax, const uint32_t* __restrict__ n_ib, int inbox_mode) {
   uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (qi >= qn) return;
+  if (inbox_mode && n_ib[qi] != kNone) {
+    // mode 3 (mixed precision, walked query): exact re-rank of the chosen sub-clusters' in-box points that
+    // pass the label predicate
+    mode[qi] = 3;
+    kprime[qi] = 0;
+    lsize[qi] = n_ib[qi];
+    asize[qi] = 0;
+    chunks[qi] = 0;
+    pad32[qi] = 0;
+    padmax[qi] = 0;
+    return;
+  }
   const uint64_t g = (q0 + qi) / G;
   const uint64_t Cq = cand[qi];
   const uint64_t W = gstat[g * 8 + 5];
@@ -958,48 +1016,6 @@ __global__ void plan_kernel(uint64_t qn, uint64_t q0, uint64_t G, int l, uint64_
   padmax[qi] = pmax;  // l > 1: set after the post-filter (n_candies - survivors)
 }
 
-// Exclusive scans of three u64 arrays in place (one workgroup of 1024 threads), totals to v[n].
-__global__ __launch_bounds__(1024) void scan3_kernel(uint64_t* __restrict__ v0, uint64_t* __restrict__ v1,
-                                                     uint64_t* __restrict__ v2, uint64_t n) {
-  __shared__ unsigned long long wsum[3][16];
-  __shared__ unsigned long long carry[3];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint64_t* const vs[3] = {v0, v1, v2};
-  if (threadIdx.x < 3) carry[threadIdx.x] = 0;
-  __syncthreads();
-  for (uint64_t b = 0; b < n; b += 1024) {
-    const uint64_t i = b + threadIdx.x;
-    unsigned long long x[3], incl[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      x[a] = i < n ? vs[a][i] : 0ull;
-      unsigned long long y = x[a];
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const unsigned long long t = __shfl_up(y, o, 64);
-        if (lane >= o) y += t;
-      }
-      incl[a] = y;
-      if (lane == 63) wsum[a][wave] = y;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      unsigned long long before = carry[a];
-      for (int w2 = 0; w2 < wave; ++w2) before += wsum[a][w2];
-      if (i < n) vs[a][i] = before + incl[a] - x[a];
-    }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-      unsigned long long tot = 0;
-      for (int w2 = 0; w2 < 16; ++w2) tot += wsum[threadIdx.x][w2];
-      carry[threadIdx.x] += tot;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x < 3) vs[threadIdx.x][n] = carry[threadIdx.x];
-}
-
 // Position in the z-pool of candidate column `col` of a query row (rstart/rpre of that query, in LDS).
 __device__ __forceinline__ uint32_t col_to_pos(const uint32_t* rs, const uint32_t* rp, uint32_t n,
                                                uint32_t col) {
@@ -1026,7 +1042,7 @@ __device__ __forceinline__ bool label_pass(const float* __restrict__ lab, int l,
 __device__ __forceinline__ bool inbox_keeps(uint64_t qi, int l, int integral, const uint32_t* __restrict__ mode,
                                             const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo,
                                             const uint64_t* __restrict__ zlo_cnt) {
-  return mode[qi] == 0 && n_ib[qi] != kNone && zlo[qi] == zlo_cnt[qi] && (l == 1 || integral);
+  return mode[qi] == 3 || (mode[qi] == 0 && n_ib[qi] != kNone && zlo[qi] == zlo_cnt[qi] && (l == 1 || integral));
 }
 
 __global__ __launch_bounds__(256) void keep_all_kernel(
@@ -1128,6 +1144,7 @@ __global__ __launch_bounds__(256) void inbox_keep_kernel(
   float rg[2 * kMaxDims];
   for (int i = 0; i < 2 * l; ++i) rg[i] = ranges[qi * 2 * l + i];
   const uint32_t n = n_ib[qi];
+  const uint32_t md = mode[qi];
   uint32_t* out = L + loff[qi];
   uint32_t base = 0;
   for (uint32_t b = 0; b < n; b += kWG) {
@@ -1136,7 +1153,8 @@ __global__ __launch_bounds__(256) void inbox_keep_kernel(
     if (i < n) {
       const uint32_t sid = ib_sub[qi * ibcap + i];
       pos = ib_pos[qi * ibcap + i];
-      flag = ((bits[sid >> 5] >> (sid & 31)) & 1u) && (l == 1 || label_pass(lab, l, pos, rg)) ? 1u : 0u;
+      flag = ((bits[sid >> 5] >> (sid & 31)) & 1u) && ((l == 1 && md != 3) || label_pass(lab, l, pos, rg)) ? 1u
+                                                                                                        : 0u;
     }
     uint32_t total;
     const uint32_t at = dev::wg_exclusive_scan(flag, tmp, total);
@@ -1380,7 +1398,7 @@ struct Index::Work {
   DevBuf ib_pos, ib_sub, n_ib;                               // in-box points listed by the walk
   DevBuf sub_list, sub_cnt, ns, list_c, cl_beg, nnz_c;       // non-zero sub-clusters / clusters
   DevBuf chosen_c, n_chosen_c, list_s, list_scnt, nnz_s, chosen_s, n_chosen_s, chosen_sid;
-  DevBuf keys, woff, gstat, rstart, rpre, cand, real;
+  DevBuf keys, woff, bsum, gstat, rstart, rpre, cand, real;
   DevBuf mode, kprime, lsize, asize, chunks, pad32, padmax, nL;
   DevBuf L, Lkeys, akeys;
   DevBuf S, feat, budget_T, budget_S, qb16;  // PRECISION 1: scores, model features, budgets, bf16 queries
@@ -1527,6 +1545,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   const uint64_t g_base = q0 / G;
   const Dims D = make_dims(g);
   const bool sparse = cfg_.sparse_count != 0 && S.Nb > 0 && CS <= kSparseMaxCS;
+  // PRECISION 1 refines a walked query from its in-box list, without Morton ranges or PQ (mode 3)
+  const int inbox_mode = (cfg_.precision == 1 && sparse) ? 1 : 0;
 
   // workspace
   w.ranges.reserve(qn * 2 * l * 4);
@@ -1572,6 +1592,15 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.pad32.reserve(qn * 8);
   w.padmax.reserve(qn * 8);
   w.nL.reserve(qn * 4);
+  w.bsum.reserve(3 * kScanBlocks * 8);
+  if (cdiv(qn, 1024) > (uint64_t)kScanBlocks) fail(kUnsupported, "QUERY_CHUNK above 65536");
+  auto scan_u64 = [&](uint64_t* a0, uint64_t* a1, uint64_t* a2, uint64_t n, const uint32_t* n32, const uint32_t* md,
+                      uint32_t skip) {
+    const unsigned nb = (unsigned)cdiv(n, 1024);
+    hipLaunchKernelGGL(scan_local_kernel, dim3(nb), dim3(1024), 0, s, a0, a1, a2, n, n32, md, skip,
+                       w.bsum.as<unsigned long long>());
+    hipLaunchKernelGGL(scan_fix_kernel, dim3(nb), dim3(1024), 0, s, a0, a1, a2, n, w.bsum.as<unsigned long long>());
+  };
   unsigned long long* gstat = w.gstat.as<unsigned long long>() + g_base * 8;
   HDG_HIP(hipMemsetAsync(gstat, 0, ng * 8 * 8, s));
   HDG_HIP(hipMemsetAsync(w.err.data(), 0, 4, s));
@@ -1674,8 +1703,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   };
   // ---- A5: cluster choice ----
   if (!mixed) {
-    hipLaunchKernelGGL(chunk_scan_kernel, dim3(1), dim3(1024), 0, s, w.nnz_c.as<uint32_t>(), nullptr, 0u, qn,
-                       w.woff.as<uint64_t>());
+    scan_u64(w.woff.as<uint64_t>(), nullptr, nullptr, qn, w.nnz_c.as<uint32_t>(), nullptr, 0u);
     hipLaunchKernelGGL(pair_dist_kernel<0>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.centers.as<float>(),
                        w.woff.as<uint64_t>(), qn, w.list_c.as<uint32_t>(), C, nullptr, w.nnz_c.as<uint32_t>(),
                        nullptr, w.keys.as<uint64_t>());
@@ -1703,8 +1731,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   HDG_HIP(hipGetLastError());
   mark();
   if (!mixed) {
-    hipLaunchKernelGGL(chunk_scan_kernel, dim3(1), dim3(1024), 0, s, w.nnz_s.as<uint32_t>(), nullptr, 0u, qn,
-                       w.woff.as<uint64_t>());
+    scan_u64(w.woff.as<uint64_t>(), nullptr, nullptr, qn, w.nnz_s.as<uint32_t>(), nullptr, 0u);
     hipLaunchKernelGGL(pair_dist_kernel<0>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d,
                        S.sub_centers.as<float>(), w.woff.as<uint64_t>(), qn, w.list_s.as<uint32_t>(), (uint64_t)cap,
                        nullptr, w.nnz_s.as<uint32_t>(), nullptr, w.keys.as<uint64_t>());
@@ -1725,16 +1752,16 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      q0, w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.zlo.as<uint64_t>(),
                      w.zhi.as<uint64_t>(), w.chosen_sid.as<uint32_t>(), w.rstart.as<uint32_t>(),
                      w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.real.as<uint64_t>(),
-                     w.gstat.as<unsigned long long>());
+                     w.gstat.as<unsigned long long>(), w.n_ib.as<uint32_t>(), inbox_mode);
   HDG_HIP(hipGetLastError());
   mark();
   hipLaunchKernelGGL(plan_kernel, dim3((unsigned)cdiv(qn, 128)), dim3(128), 0, s, qn, q0, G, l, cfg_.exps0,
                      topk, w.cand.as<uint64_t>(), w.gstat.as<unsigned long long>(), w.mode.as<uint32_t>(),
                      w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(),
-                     w.chunks.as<uint64_t>(), w.pad32.as<uint64_t>(), w.padmax.as<uint64_t>());
+                     w.chunks.as<uint64_t>(), w.pad32.as<uint64_t>(), w.padmax.as<uint64_t>(), w.n_ib.as<uint32_t>(),
+                     inbox_mode);
   HDG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(scan3_kernel, dim3(1), dim3(1024), 0, s, w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(),
-                     w.chunks.as<uint64_t>(), qn);
+  scan_u64(w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(), w.chunks.as<uint64_t>(), qn, nullptr, nullptr, 0);
   HDG_HIP(hipGetLastError());
   // the one host synchronisation of a chunk: sizes of the data-dependent buffers
   HDG_HIP(hipMemcpyAsync(&w.pinned[0], w.lsize.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
@@ -1779,8 +1806,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   }
   mark();
   // ---- A13: exact re-rank of the refine lists, final top-k ----
-  hipLaunchKernelGGL(chunk_scan_kernel, dim3(1), dim3(1024), 0, s, w.nL.as<uint32_t>(), w.mode.as<uint32_t>(), 2u, qn,
-                     w.woff.as<uint64_t>());
+  scan_u64(w.woff.as<uint64_t>(), nullptr, nullptr, qn, w.nL.as<uint32_t>(), w.mode.as<uint32_t>(), 2u);
   hipLaunchKernelGGL(pair_dist_kernel<1>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.vec.as<float>(),
                      w.woff.as<uint64_t>(), qn, w.L.as<uint32_t>(), (uint64_t)0, w.lsize.as<uint64_t>(),
                      w.nL.as<uint32_t>(), S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>());

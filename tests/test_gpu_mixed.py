@@ -57,8 +57,9 @@ def test_coarse_scores_mfma_layout(mixed_index):
 
 def test_mixed_without_pruning_matches_plain(mixed_index):
     """With every non-zero cluster and sub-cluster kept, BF16 scoring changes only the order in which they
-    are visited; in keep-all mode the candidate set, beta and the exact re-rank are those of the plain
-    path, so results are bit-identical."""
+    are visited.  The plain path keeps every label survivor among the Morton-range candidates (keep-all),
+    the mixed path every label survivor among the in-box points: with integer labels these are the same
+    points, and the exact re-rank makes the results bit-identical."""
     m = mixed_index
     idx = m["idx"]
     idx.set_param("SUB_LISTS", 4096)
@@ -72,7 +73,7 @@ def test_mixed_without_pruning_matches_plain(mixed_index):
     idx.set_param("PRECISION", 0)
     idx.set_param("SUB_LISTS", 64)
     assert (st.adc == 0).all() and (st0.adc == 0).all()  # keep-all: no PQ ordering involved
-    np.testing.assert_array_equal(st0.cand, st.cand)
+    np.testing.assert_array_equal(st0.refined, st.refined)
     np.testing.assert_array_equal(i0.cpu().numpy(), i1.cpu().numpy())
     np.testing.assert_array_equal(d0.cpu().numpy(), d1.cpu().numpy())
 
@@ -86,7 +87,7 @@ def test_mixed_probe_budget_tradeoff(mixed_index):
         idx.set_param("PROBE_CLUSTERS", T)
         _, ids = idx.query(m["q"], m["ql"])
         recs.append(datasets.recall(m["gt"], ids.cpu().numpy().astype(np.uint64), 100))
-        cand.append(idx.last_stats(m["q"].shape[0]).cand.mean())
+        cand.append(idx.last_stats(m["q"].shape[0]).refined.mean())
     idx.set_param("PRECISION", 0)
     print("[mixed] recall by probe", recs, "cand", cand)
     assert recs[2] >= recs[0] - 0.01 and cand[2] >= cand[0]
@@ -94,13 +95,15 @@ def test_mixed_probe_budget_tradeoff(mixed_index):
 
 
 def test_mixed_adc_fp16_lut_close_to_fp32():
-    """ADC path with the FP16 LUT: ids overlap the FP32-LUT ids heavily (same candidates, PQ order may differ)."""
+    """ADC path with the FP16 LUT: ids overlap the FP32-LUT ids heavily (same candidates, PQ order may differ).
+    Walked queries of the mixed path re-rank their in-box points without PQ, so the dense count is forced."""
     w = datasets.plain_small(n=30000, nq=200, d=64, components=32, clusters=64, sub_clusters=8, shift=50000)
     idx = H.SuperGridIndex(conf_text=w.conf_text(**FAST), filter_text=w.filter_text)
     idx.build(torch.from_numpy(w.data).to(DEV), torch.from_numpy(w.labels).to(DEV))
     q, ql = torch.from_numpy(w.queries).to(DEV), torch.from_numpy(w.qlabels).to(DEV)
     idx.set_param("EXPS0", 2)
     idx.set_param("SUB_LISTS", 512)  # every non-zero sub-cluster on both paths: same candidates
+    idx.set_param("SPARSE_COUNT", 0)  # dense count: the mixed path takes the Morton-range/PQ route
     idx.set_param("PRECISION", 1)
     idx.set_param("PROBE_CLUSTERS", 0)
     _, i1 = idx.query(q, ql)
