@@ -50,6 +50,8 @@ struct Config {
   uint64_t budget_model = 0;           // BUDGET_MODEL: 1 = per-query budgets from the loaded MLP
   uint64_t sparse_count = 1;           // SPARSE_COUNT: 1 = bucket-grid walk for selective boxes, 0 = dense count
   uint64_t sparse_max_tests = 131072;  // SPARSE_MAX_TESTS: entries a query may test on the walk before the dense count
+  uint64_t adc_per = 32;               // ADC_PER: candidates per thread of an ADC chunk (8, 16, 32, 64)
+  uint64_t adc_wgs = 2048;             // ADC_WGS: persistent ADC workgroups
 };
 
 // Parse with config_utils.h:33-88 semantics: `KEY = value` per line, key case-insensitive and

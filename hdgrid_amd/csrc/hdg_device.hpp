@@ -70,8 +70,20 @@ __device__ inline void wg_sort_lds(uint64_t* keys, uint32_t n) {
 
 // Radix selection: writes the k smallest of the unique keys g[0..n) (global memory) to out[0..k),
 // unordered.  Requires k < n.  `hist` is 2048 words of LDS, `sh` 8 words of LDS.
-__device__ inline void wg_radix_select(const uint64_t* __restrict__ g, uint32_t n, uint32_t k,
-                                       uint64_t* __restrict__ out, uint32_t* hist, uint64_t* sh) {
+// Key sources for the selections: a plain key array, or (f32 distance bits, index) built from a
+// distance array.
+struct KeyArray {
+  const uint64_t* g;
+  __device__ __forceinline__ uint64_t operator()(uint32_t i) const { return g[i]; }
+};
+struct KeyDist {
+  const uint32_t* d;
+  __device__ __forceinline__ uint64_t operator()(uint32_t i) const { return ((uint64_t)d[i] << 32) | i; }
+};
+
+template <class Src>
+__device__ inline void wg_radix_select(Src g, uint32_t n, uint32_t k, uint64_t* __restrict__ out, uint32_t* hist,
+                                       uint64_t* sh) {
   uint64_t prefix = 0, mask = 0;
   uint32_t need = k;  // rank (1-based) of the k-th key inside the current prefix bucket
   const int shifts[6] = {53, 42, 31, 20, 9, 0};
@@ -82,7 +94,7 @@ __device__ inline void wg_radix_select(const uint64_t* __restrict__ g, uint32_t 
     for (int i = threadIdx.x; i < 2048; i += kWG) hist[i] = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += kWG) {
-      uint64_t key = g[i];
+      uint64_t key = g(i);
       if ((key & mask) == prefix) atomicAdd(&hist[(key >> sft) & dmask], 1u);
     }
     __syncthreads();
@@ -107,10 +119,79 @@ __device__ inline void wg_radix_select(const uint64_t* __restrict__ g, uint32_t 
   __syncthreads();
   uint32_t* slot = reinterpret_cast<uint32_t*>(&sh[2]);
   for (uint32_t i = threadIdx.x; i < n; i += kWG) {
-    uint64_t key = g[i];
+    uint64_t key = g(i);
     if (key <= prefix) {
       uint32_t s = atomicAdd(slot, 1u);
       if (s < k) out[s] = key;  // exactly k qualify for unique keys; the guard bounds the write anyway
+    }
+  }
+  __syncthreads();
+}
+
+// Radix selection with compaction: like wg_radix_select, but after the two leading 11-bit passes the keys
+// below the k-th key's bucket are written to out[] at once and the keys inside that bucket are copied to
+// scratch[] (capacity n), so the four trailing passes read only that bucket.  Three passes over g[].
+template <class Src>
+__device__ inline void wg_radix_select_compact(Src g, uint32_t n, uint32_t k, uint64_t* __restrict__ out,
+                                               uint64_t* __restrict__ scratch, uint32_t* hist, uint64_t* sh) {
+  uint64_t prefix = 0, mask = 0;
+  uint32_t need = k;
+  const int shifts[6] = {53, 42, 31, 20, 9, 0};
+  const int widths[6] = {11, 11, 11, 11, 11, 9};
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(&sh[2]);  // [0] selected so far, [1] bucket size
+  for (int pass = 0; pass < 6; ++pass) {
+    if (pass == 2) {
+      // split: below the bucket -> out, inside -> scratch
+      if (threadIdx.x == 0) {
+        cnt[0] = 0;
+        cnt[1] = 0;
+      }
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < n; i += kWG) {
+        const uint64_t key = g(i);
+        const uint64_t top = key & mask;
+        if (top < prefix) {
+          const uint32_t s = atomicAdd(&cnt[0], 1u);
+          if (s < k) out[s] = key;
+        } else if (top == prefix) {
+          scratch[atomicAdd(&cnt[1], 1u)] = key;
+        }
+      }
+      __syncthreads();
+    }
+    const uint32_t len = pass < 2 ? n : cnt[1];
+    const int sft = shifts[pass];
+    const uint64_t dmask = (1ull << widths[pass]) - 1;
+    for (int i = threadIdx.x; i < 2048; i += kWG) hist[i] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < len; i += kWG) {
+      const uint64_t key = pass < 2 ? g(i) : scratch[i];
+      if ((key & mask) == prefix) atomicAdd(&hist[(key >> sft) & dmask], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t cum = 0, b = 0;
+      for (b = 0; b <= dmask; ++b) {
+        if (cum + hist[b] >= need) break;
+        cum += hist[b];
+      }
+      sh[0] = b;
+      sh[1] = cum;
+    }
+    __syncthreads();
+    const uint64_t b = sh[0];
+    need -= (uint32_t)sh[1];
+    prefix |= b << sft;
+    mask |= dmask << sft;
+    __syncthreads();
+  }
+  // prefix is the k-th smallest key: the bucket's keys <= prefix complete out[]
+  const uint32_t len = cnt[1];
+  for (uint32_t i = threadIdx.x; i < len; i += kWG) {
+    const uint64_t key = scratch[i];
+    if (key <= prefix) {
+      const uint32_t s = atomicAdd(&cnt[0], 1u);
+      if (s < k) out[s] = key;
     }
   }
   __syncthreads();

@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
     for (uint32_t j = threadIdx.x; j < n; j += kWG) g[j] = ((uint64_t)ord(row[lst[j]]) << 32) | j;
     __syncthreads();
     if (m > scap) m = scap;
-    dev::wg_radix_select(g, n, m, keys, hist, sh);
+    dev::wg_radix_select(dev::KeyArray{g}, n, m, keys, hist, sh);
     dev::wg_sort_lds(keys, m);
   }
   for (uint32_t j = threadIdx.x; j < m; j += kWG) chosen[qi * out_ld + j] = (uint32_t)keys[j];

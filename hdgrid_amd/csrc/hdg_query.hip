@@ -40,7 +40,6 @@ constexpr float kExactPad = 1e6f;               // refine.cu:53
 constexpr uint64_t kIdPad = 4294967296ull;      // mat_operators.cuh:185
 constexpr uint64_t kU64Max = ~0ull;             // filter.cu:171
 constexpr int kCountSegChunk = 64;              // segments per dense_count_kernel block
-constexpr uint32_t kAdcChunk = 8192;            // candidates per adc_scan block
 
 struct Dims {
   int l, n_valid, max_bits, valid_map;
@@ -374,12 +373,14 @@ __global__ __launch_bounds__(64) void sparse_count_kernel(
 }
 
 // ------------------------------------------------------------------------------------------------
-// A3 (dense), for the queries the sparse pass handed over: a point inside the box has a Morton code
-// in [z(lo), z(hi)] (the bit interleave is monotone in every coordinate) and a sub-cluster's points
-// are sorted by Morton code, so only that code range of each segment is box-tested.  Block = 256
+// A3/A4 (dense), for the queries the sparse pass handed over (wide boxes): which sub-clusters hold at
+// least one in-box point.  A point inside the box has a Morton code in [z(lo), z(hi)] (the bit interleave
+// is monotone in every coordinate) and a sub-cluster's points are sorted by Morton code, so only that code
+// range of each segment is box-tested, and a lane stops at its first hit.  The exact in-box counts are
+// needed only for the chosen sub-clusters (A10's beta) and are taken there (ranges_kernel).  Block = 256
 // lanes = 256 handed-over queries; blockIdx.y = chunk of kCountSegChunk segments.  Each segment is
 // staged in LDS in pieces of kCountPiece points (codes + cells); every lane binary-searches its own
-// code range in LDS and tests the points inside it.  counts[slot][s], slot = position in dense_list.
+// code range in LDS.  counts[slot][s] = 0 / 1, slot = position in dense_list.
 constexpr int kCountPiece = 1024;
 
 template <int NV>
@@ -411,17 +412,17 @@ __global__ __launch_bounds__(256) void dense_count_kernel(const uint32_t* __rest
   const uint64_t s_end = s_begin + kCountSegChunk < CS ? s_begin + kCountSegChunk : CS;
   for (uint64_t s = s_begin; s < s_end; ++s) {
     const uint32_t p0 = seg_off[s], p1 = seg_off[s + 1];
-    uint32_t cnt = 0;
+    bool hit = !live;  // idle lanes never hold the block back
     for (uint32_t base = p0; base < p1; base += kCountPiece) {
       const uint32_t m = p1 - base < (uint32_t)kCountPiece ? p1 - base : (uint32_t)kCountPiece;
-      __syncthreads();
+      if (__syncthreads_count(hit ? 0 : 1) == 0) break;  // every lane has its answer
       for (uint32_t i = threadIdx.x; i < m; i += 256) {
         zs[i] = zcode[base + i];
 #pragma unroll
         for (int j = 0; j < NV; ++j) cs[j][i] = cells[(uint64_t)j * N + base + i];
       }
       __syncthreads();
-      if (za > zs[m - 1] || zb < zs[0]) continue;
+      if (hit || za > zs[m - 1] || zb < zs[0]) continue;
       uint32_t a = 0, b = m;  // first code >= za
       while (a < b) {
         uint32_t mid = (a + b) >> 1;
@@ -436,10 +437,13 @@ __global__ __launch_bounds__(256) void dense_count_kernel(const uint32_t* __rest
         bool in = true;
 #pragma unroll
         for (int j = 0; j < NV; ++j) in = in && (cs[j][i] - lo[j] <= w[j]);
-        cnt += in ? 1u : 0u;
+        if (in) {
+          hit = true;
+          break;
+        }
       }
     }
-    if (live) counts[slot * CS + s] = cnt;
+    if (live) counts[slot * CS + s] = hit ? 1u : 0u;
   }
 }
 
@@ -708,11 +712,12 @@ __global__ __launch_bounds__(1024) void scan_fix_kernel(uint64_t* __restrict__ v
 
 // Sort keys[0..n) (global scratch) ascending into `lds` (capacity `cap` keys, a power of two) when n <= cap,
 // or select the k smallest and sort those (k <= cap).  Returns how many sorted keys are available (min(n,k)).
-__device__ uint32_t select_sorted(uint64_t* __restrict__ g, uint32_t n, uint32_t k, uint64_t* lds, uint32_t cap,
-                                  uint32_t* hist, uint64_t* sh) {
+template <class Src>
+__device__ uint32_t select_sorted(Src g, uint32_t n, uint32_t k, uint64_t* lds, uint32_t cap, uint32_t* hist,
+                                  uint64_t* sh) {
   uint32_t m = n < k ? n : k;
   if (n <= cap) {
-    for (uint32_t i = threadIdx.x; i < n; i += kWG) lds[i] = g[i];
+    for (uint32_t i = threadIdx.x; i < n; i += kWG) lds[i] = g(i);
     __syncthreads();
     dev::wg_sort_lds(lds, n);
     return m;
@@ -765,7 +770,7 @@ __global__ __launch_bounds__(256) void coarse_pick_kernel(uint32_t C, uint32_t n
   const uint32_t n = nnz_c[qi];
   const uint64_t gmax = gstat[((q0 + qi) / G) * 8 + 0];
   const uint32_t kc = (uint32_t)min<uint64_t>(n_list, gmax);
-  const uint32_t m = select_sorted(keys + qi * C, n, kc, sorted, scap, hist, sh);
+  const uint32_t m = select_sorted(dev::KeyArray{keys + qi * C}, n, kc, sorted, scap, hist, sh);
   __syncthreads();
   const uint32_t keep = kept_with_pads(sorted, m, kc, (uint64_t)C - n);
   for (uint32_t j = threadIdx.x; j < keep; j += kWG) chosen_c[qi * n_list + j] = dev::key_tie(sorted[j]);
@@ -845,7 +850,7 @@ __global__ __launch_bounds__(256) void sub_pick_kernel(uint32_t sub_lists, uint6
     ks = b == 0 || b > sub_lists ? sub_lists : b;
     table_len = n;
   }
-  const uint32_t m = select_sorted(keys + qi * cap, n, ks, sorted, scap, hist, sh);
+  const uint32_t m = select_sorted(dev::KeyArray{keys + qi * cap}, n, ks, sorted, scap, hist, sh);
   __syncthreads();
   const uint32_t keep = kept_with_pads(sorted, m, ks, table_len - n);
   for (uint32_t j = threadIdx.x; j < keep; j += kWG) chosen_s[qi * sub_lists + j] = dev::key_tie(sorted[j]);
@@ -858,7 +863,9 @@ __global__ __launch_bounds__(256) void sub_pick_kernel(uint32_t sub_lists, uint6
 // Also sums C_q (candidates) and the in-box count of the chosen sub-clusters (A10 inputs), and
 // records the chosen sub-cluster ids (UINT32_MAX past the end).
 __global__ __launch_bounds__(256) void ranges_kernel(
-    const uint64_t* __restrict__ zcode, const uint32_t* __restrict__ seg_off, uint32_t cap,
+    const uint64_t* __restrict__ zcode, const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ cells,
+    uint64_t N, int n_valid, const uint32_t* __restrict__ box_lo, const uint32_t* __restrict__ box_w, uint64_t qn,
+    const uint64_t* __restrict__ zlo_cnt, uint32_t cap,
     const uint32_t* __restrict__ list_s, const uint32_t* __restrict__ list_scnt, uint32_t sub_lists, uint64_t G,
     uint64_t q0, const uint32_t* __restrict__ chosen_s, const uint32_t* __restrict__ n_chosen_s,
     const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zhi, uint32_t* __restrict__ chosen_sid,
@@ -868,6 +875,7 @@ __global__ __launch_bounds__(256) void ranges_kernel(
   __shared__ uint32_t tmp[8];
   __shared__ uint32_t base_sh;
   __shared__ unsigned long long real_sh;
+  __shared__ uint32_t cs_sh[kWG], cp_sh[kWG];
   const uint64_t qi = blockIdx.x;
   const uint32_t n = n_chosen_s[qi];
   if (inbox_mode && n_ib[qi] != kNone) {
@@ -899,9 +907,16 @@ __global__ __launch_bounds__(256) void ranges_kernel(
     real_sh = 0;
   }
   __syncthreads();
+  const bool dense_q = n_ib[qi] == kNone;
+  uint32_t blo[kMaxDims], bw[kMaxDims];
+  if (dense_q)
+    for (int j2 = 0; j2 < n_valid; ++j2) {
+      blo[j2] = box_lo[(uint64_t)j2 * qn + qi];
+      bw[j2] = box_w[(uint64_t)j2 * qn + qi];
+    }
   for (uint32_t j0 = 0; j0 < sub_lists; j0 += kWG) {
     const uint32_t j = j0 + threadIdx.x;
-    uint32_t cnt = 0, start = 0, sid = kNone;
+    uint32_t cnt = 0, start = 0, sid = kNone, cstart = 0, clen = 0;
     if (j < n) {
       const uint32_t idx = chosen_s[qi * sub_lists + j];
       sid = list_s[qi * cap + idx];
@@ -920,7 +935,44 @@ __global__ __launch_bounds__(256) void ranges_kernel(
       }
       start = lo;
       cnt = lo2 > lo ? lo2 - lo : 0u;  // an empty code range (zlo > zhi) holds no candidate
-      atomicAdd(&real_sh, (unsigned long long)list_scnt[qi * cap + idx]);
+      if (dense_q) {
+        // flagged by the dense pass: the box is counted below over the count range [z(max(lo, 0)), z(hi)]
+        uint32_t c0 = lo;
+        if (zlo_cnt[qi] != a) {
+          uint32_t x = p0, y = p1;
+          while (x < y) {
+            const uint32_t mid = (x + y) >> 1;
+            if (zcode[mid] < zlo_cnt[qi]) x = mid + 1; else y = mid;
+          }
+          c0 = x;
+        }
+        cstart = c0;
+        clen = lo2 > c0 ? lo2 - c0 : 0u;
+      } else {
+        atomicAdd(&real_sh, (unsigned long long)list_scnt[qi * cap + idx]);
+      }
+    }
+    if (dense_q) {
+      // the chunk's count ranges flattened over the workgroup
+      uint32_t ctot;
+      const uint32_t cpre = dev::wg_exclusive_scan(clen, tmp, ctot);
+      cs_sh[threadIdx.x] = cstart;
+      cp_sh[threadIdx.x] = cpre;
+      __syncthreads();
+      uint32_t mine = 0;
+      for (uint32_t t = threadIdx.x; t < ctot; t += kWG) {
+        uint32_t x = 0, y = kWG;  // last owner k with cp_sh[k] <= t
+        while (y - x > 1) {
+          const uint32_t mid = (x + y) >> 1;
+          if (cp_sh[mid] <= t) x = mid; else y = mid;
+        }
+        const uint32_t p = cs_sh[x] + (t - cp_sh[x]);
+        bool in = true;
+        for (int j2 = 0; j2 < n_valid && in; ++j2) in = cells[(uint64_t)j2 * N + p] - blo[j2] <= bw[j2];
+        mine += in ? 1u : 0u;
+      }
+      atomicAdd(&real_sh, (unsigned long long)mine);
+      __syncthreads();
     }
     uint32_t total;
     uint32_t pos = dev::wg_exclusive_scan(cnt, tmp, total);
@@ -962,7 +1014,8 @@ __global__ void plan_kernel(uint64_t qn, uint64_t q0, uint64_t G, int l, uint64_
                             uint32_t* __restrict__ mode, uint64_t* __restrict__ kprime,
                             uint64_t* __restrict__ lsize, uint64_t* __restrict__ asize,
                             uint64_t* __restrict__ chunks, uint64_t* __restrict__ pad32,
-                            uint64_t* __restrict__ padmax, const uint32_t* __restrict__ n_ib, int inbox_mode) {
+                            uint64_t* __restrict__ padmax, const uint32_t* __restrict__ n_ib, int inbox_mode,
+                            uint32_t adc_chunk) {
   uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (qi >= qn) return;
   if (inbox_mode && n_ib[qi] != kNone) {
@@ -987,7 +1040,7 @@ __global__ void plan_kernel(uint64_t qn, uint64_t q0, uint64_t G, int l, uint64_
     mode[qi] = 2;
     lsize[qi] = Cq;
     asize[qi] = Cq;
-    chunks[qi] = cdiv(Cq, kAdcChunk);
+    chunks[qi] = cdiv(Cq, adc_chunk);
     kprime[qi] = kp;
     pad32[qi] = 0;
     padmax[qi] = 0;
@@ -1011,7 +1064,7 @@ __global__ void plan_kernel(uint64_t qn, uint64_t q0, uint64_t G, int l, uint64_
   kprime[qi] = kp;
   lsize[qi] = adc ? kp : Cq;
   asize[qi] = adc ? Cq : 0;
-  chunks[qi] = adc ? cdiv(Cq, kAdcChunk) : 0;
+  chunks[qi] = adc ? cdiv(Cq, adc_chunk) : 0;
   pad32[qi] = p32;
   padmax[qi] = pmax;  // l > 1: set after the post-filter (n_candies - survivors)
 }
@@ -1164,14 +1217,18 @@ __global__ __launch_bounds__(256) void inbox_keep_kernel(
   if (threadIdx.x == 0) nL[qi] = base;
 }
 
-// A11: ADC distances (product-quantization.cu:37-77) written as unique keys (distance bits, column).
+// A11: ADC distances (product-quantization.cu:37-77) written as f32 per column; the selections read them as
+// unique keys (distance bits, column).
 // Mode 1: the query's Morton-range candidates, column = candidate column.  Mode 2 (EXPS0 <= 1): the
-// label-filtered refine list, column = list index.  One block per kAdcChunk items of a query; each
-// thread owns kAdcChunk/256 items and keeps their partial sums in registers while the LUT
-// (product-quantization.cu:4-35, an fma chain over pq_len) passes through LDS in phases of
-// kLutPhaseBytes, so the sum over m stays sequential for any pq_dim.
+// label-filtered refine list, column = list index.  Work = chunks of PER * 256 items of a query
+// (coff[q] = chunks before query q); the grid is persistent and each workgroup takes a contiguous run of
+// chunks, so a query's LUT (product-quantization.cu:4-35, an fma chain over pq_len) is built once per
+// workgroup when it fits one LDS phase of kLutPhaseBytes, else rebuilt phase by phase per chunk.  Each
+// thread owns PER items (lane-consecutive columns, so code rows are read coalesced) and keeps their
+// partial sums in registers; the sum over m stays sequential for any pq_dim.  The query's range table
+// sits in LDS for the column -> z-pool position lookup.
 constexpr uint32_t kLutPhaseBytes = 48 * 1024;
-constexpr int kAdcPer = kAdcChunk / kWG;
+constexpr uint32_t kAdcLdsRanges = 2040;  // LUT phase + range table stay within 64 KiB of LDS
 
 // LUT entry types: f32 (reference-exact) or f16 (PRECISION 1); sums always accumulate in f32.
 __device__ __forceinline__ float lutf(float v) { return v; }
@@ -1180,85 +1237,259 @@ __device__ __forceinline__ void lut_store(float* p, float v) { *p = v; }
 __device__ __forceinline__ void lut_store(__half* p, float v) { *p = __float2half(v); }
 
 template <typename LT>
+__device__ __forceinline__ float adc_word(float sum, uint32_t wv, const LT* lr, uint64_t K) {
+  sum += lutf(lr[wv & 0xFF]);
+  sum += lutf(lr[K + ((wv >> 8) & 0xFF)]);
+  sum += lutf(lr[2 * K + ((wv >> 16) & 0xFF)]);
+  sum += lutf(lr[3 * K + (wv >> 24)]);
+  return sum;
+}
+
+template <typename LT, int PER, int KC, bool STREAM>
 __global__ __launch_bounds__(256) void adc_scan_kernel(
-    uint64_t qn, uint64_t q0, const float* __restrict__ queries, uint64_t d, uint64_t M, uint64_t K,
+    uint64_t qn, uint64_t q0, const float* __restrict__ queries, uint64_t d, uint64_t M, uint64_t K_rt,
     uint64_t pq_len, const float* __restrict__ pq_centers, const uint8_t* __restrict__ codes,
     uint32_t sub_lists, const uint32_t* __restrict__ mode, const uint32_t* __restrict__ n_chosen_s,
     const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ rpre, const uint64_t* __restrict__ cand,
     const uint32_t* __restrict__ L, const uint64_t* __restrict__ loff, const uint32_t* __restrict__ nL,
-    const uint64_t* __restrict__ coff, const uint64_t* __restrict__ aoff, uint64_t* __restrict__ akeys) {
+    const uint64_t* __restrict__ coff, const uint64_t* __restrict__ aoff, uint32_t* __restrict__ adist,
+    uint32_t lut_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const uint64_t K = KC ? (uint64_t)KC : K_rt;  // codebook size; a compile-time 256 folds the LUT row offsets
   LT* lut = reinterpret_cast<LT*>(smem);  // [mp][K]
-  // which query: last qi with coff[qi] <= blockIdx.x
-  const uint64_t blk = blockIdx.x;
-  uint64_t lo = 0, hi = qn;
-  while (hi - lo > 1) {
-    uint64_t mid = (lo + hi) >> 1;
-    if (coff[mid] <= blk) lo = mid; else hi = mid;
-  }
-  const uint64_t qi = lo;
-  const bool from_list = mode[qi] == 2;
-  const uint64_t n_items = from_list ? nL[qi] : cand[qi];
-  const uint64_t c0 = (blk - coff[qi]) * kAdcChunk;
-  if (c0 >= n_items) return;
-  const uint64_t c1 = c0 + kAdcChunk < n_items ? c0 + kAdcChunk : n_items;
-  const float* q = queries + (q0 + qi) * d;
-  const uint32_t nr = n_chosen_s[qi];
-  const uint32_t* rs = rstart + qi * sub_lists;
-  const uint32_t* rp = rpre + qi * (sub_lists + 1);
-  const uint32_t* list = L + loff[qi];
-  uint32_t pos[kAdcPer];
-  float part[kAdcPer];
-#pragma unroll
-  for (int i = 0; i < kAdcPer; ++i) {
-    const uint64_t col = c0 + threadIdx.x + (uint64_t)i * kWG;
-    part[i] = 0.0f;
-    pos[i] = 0;
-    if (col < c1) pos[i] = from_list ? list[col] : col_to_pos(rs, rp, nr, (uint32_t)col);
-  }
+  // the range table is staged in LDS when it fits beside the LUT (sub_lists <= kAdcLdsRanges)
+  const bool table_lds = sub_lists <= kAdcLdsRanges;
+  uint32_t* rs_l = reinterpret_cast<uint32_t*>(smem + lut_bytes);
+  uint32_t* rp_l = rs_l + sub_lists;
+  constexpr uint32_t kChunk = PER * kWG;
+  const uint64_t total = coff[qn];
+  const uint64_t per_wg = (total + gridDim.x - 1) / gridDim.x;
+  uint64_t blk = (uint64_t)blockIdx.x * per_wg;
+  const uint64_t blk_end = blk + per_wg < total ? blk + per_wg : total;
+  if (blk >= blk_end) return;
   uint64_t mp = kLutPhaseBytes / (K * sizeof(LT));
   if (mp > 4) mp &= ~3ull;
+  const bool one_phase = mp >= M;
   const bool words = (M % 4) == 0 && (mp % 4) == 0;
-  for (uint64_t m0 = 0; m0 < M; m0 += mp) {
-    const uint64_t m1 = m0 + mp < M ? m0 + mp : M;
-    for (uint64_t e = threadIdx.x; e < (m1 - m0) * K; e += kWG) {
-      const uint64_t m = m0 + e / K, c = e % K;
-      float ans = 0.0f;
-      for (uint64_t t = 0; t < pq_len; ++t) {
-        if (m * pq_len + t >= d) break;
-        float diff = pq_centers[m * pq_len * K + c * pq_len + t] - q[m * pq_len + t];
-        ans = fmaf(diff, diff, ans);
-      }
-      lut_store(&lut[e], ans);
+  const bool quads = (M % 16) == 0 && (mp % 16) == 0;
+  uint64_t qi = 0;
+  {
+    uint64_t lo = 0, hi = qn;  // last q with coff[q] <= blk
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (coff[mid] <= blk) lo = mid; else hi = mid;
     }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < kAdcPer; ++i) {
-      const uint64_t col = c0 + threadIdx.x + (uint64_t)i * kWG;
-      if (col >= c1) continue;
-      const uint8_t* cd = codes + (uint64_t)pos[i] * M;
-      float sum = part[i];
-      if (words) {
-        for (uint64_t m = m0; m < m1; m += 4) {
-          const uint32_t wv = *reinterpret_cast<const uint32_t*>(cd + m);
-          const LT* lr = lut + (m - m0) * K;
-          sum += lutf(lr[wv & 0xFF]);
-          sum += lutf(lr[K + ((wv >> 8) & 0xFF)]);
-          sum += lutf(lr[2 * K + ((wv >> 16) & 0xFF)]);
-          sum += lutf(lr[3 * K + (wv >> 24)]);
-        }
-      } else {
-        for (uint64_t m = m0; m < m1; ++m) sum += lutf(lut[(m - m0) * K + cd[m]]);
-      }
-      part[i] = sum;
-    }
-    __syncthreads();
+    qi = lo;
   }
-  uint64_t* out = akeys + aoff[qi];
+  uint64_t cur = ~0ull;
+  for (; blk < blk_end; ++blk) {
+    while (coff[qi + 1] <= blk) ++qi;
+    const bool from_list = mode[qi] == 2;
+    const uint32_t n_items = (uint32_t)(from_list ? nL[qi] : cand[qi]);
+    const uint32_t c0 = (uint32_t)(blk - coff[qi]) * kChunk;
+    const uint32_t c1 = c0 + kChunk < n_items ? c0 + kChunk : n_items;
+    const float* q = queries + (q0 + qi) * d;
+    const uint32_t nr = n_chosen_s[qi];
+    const uint32_t* list = L + loff[qi];
+    const bool fresh = qi != cur;
+    const uint32_t* rs = table_lds ? rs_l : rstart + qi * sub_lists;
+    const uint32_t* rp = table_lds ? rp_l : rpre + qi * (sub_lists + 1);
+    __syncthreads();  // the previous chunk is done with the LUT and the range table
+    if (fresh && !from_list && table_lds) {
+      for (uint32_t j = threadIdx.x; j <= nr; j += kWG) {
+        rp_l[j] = rpre[qi * (sub_lists + 1) + j];
+        if (j < nr) rs_l[j] = rstart[qi * sub_lists + j];
+      }
+    }
+    if (fresh && one_phase) {
+      for (uint64_t e = threadIdx.x; e < M * K; e += kWG) {
+        const uint64_t m = e / K, c = e % K;
+        float ans = 0.0f;
+        for (uint64_t t = 0; t < pq_len; ++t) {
+          if (m * pq_len + t >= d) break;
+          const float diff = pq_centers[m * pq_len * K + c * pq_len + t] - q[m * pq_len + t];
+          ans = fmaf(diff, diff, ans);
+        }
+        lut_store(&lut[e], ans);
+      }
+    }
+    cur = qi;
+    __syncthreads();
+    if constexpr (STREAM) {
+      // STREAM (the host picks it when the LUT fits one phase and M % 32 == 0): groups of 4 of this thread's columns, each finished (all M codes) before the next, so
+      // no per-column state stays live and occupancy stays high
+      uint32_t j = 0;
+      if (!from_list) {
+        const uint32_t col = c0 + threadIdx.x;
+        uint32_t hi = nr;
+        while (hi - j > 1) {
+          const uint32_t mid = (j + hi) >> 1;
+          if (rp[mid] <= col) j = mid; else hi = mid;
+        }
+      }
+      uint32_t* out = adist + aoff[qi];
+#pragma unroll 1
+      for (int i0 = 0; i0 < PER; i0 += 4) {
+        uint32_t p4[4];
+        float sum[4];
 #pragma unroll
-  for (int i = 0; i < kAdcPer; ++i) {
-    const uint64_t col = c0 + threadIdx.x + (uint64_t)i * kWG;
-    if (col < c1) out[col] = dev::make_key(part[i], (uint32_t)col);
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t col = c0 + threadIdx.x + (uint32_t)(i0 + u) * kWG;
+          sum[u] = 0.0f;
+          p4[u] = 0;
+          if (col < c1) {
+            if (from_list) {
+              p4[u] = list[col];
+            } else {
+              while (j + 1 < nr && rp[j + 1] <= col) ++j;
+              p4[u] = rs[j] + (col - rp[j]);
+            }
+          }
+        }
+        if (c0 + threadIdx.x + (uint32_t)i0 * kWG >= c1) break;  // the rest of this thread's columns too
+#pragma unroll 1
+        for (uint64_t mb = 0; mb < M; mb += 32) {
+          uint4 v[4][2];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t col = c0 + threadIdx.x + (uint32_t)(i0 + u) * kWG;
+            const uint4* cd = reinterpret_cast<const uint4*>(codes + (uint64_t)p4[u] * M + mb);
+            const bool ok = col < c1;
+            v[u][0] = ok ? cd[0] : make_uint4(0, 0, 0, 0);
+            v[u][1] = ok ? cd[1] : make_uint4(0, 0, 0, 0);
+          }
+          const LT* lr0 = lut + mb * K;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const LT* lr = lr0 + (uint64_t)h * 16 * K;
+              sum[u] = adc_word(sum[u], v[u][h].x, lr, K);
+              sum[u] = adc_word(sum[u], v[u][h].y, lr + 4 * K, K);
+              sum[u] = adc_word(sum[u], v[u][h].z, lr + 8 * K, K);
+              sum[u] = adc_word(sum[u], v[u][h].w, lr + 12 * K, K);
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t col = c0 + threadIdx.x + (uint32_t)(i0 + u) * kWG;
+          if (col < c1) out[col] = __float_as_uint(sum[u]);
+        }
+      }
+      continue;
+    } else {
+    uint32_t pos[PER];
+    float part[PER];
+    if (from_list) {
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const uint32_t col = c0 + threadIdx.x + (uint32_t)i * kWG;
+        part[i] = 0.0f;
+        pos[i] = col < c1 ? list[col] : 0u;
+      }
+    } else {
+      // this thread's columns ascend by kWG: one search for the first, then step through the ranges
+      uint32_t j = 0;
+      {
+        const uint32_t col = c0 + threadIdx.x;
+        uint32_t hi = nr;
+        while (hi - j > 1) {
+          const uint32_t mid = (j + hi) >> 1;
+          if (rp[mid] <= col) j = mid; else hi = mid;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const uint32_t col = c0 + threadIdx.x + (uint32_t)i * kWG;
+        part[i] = 0.0f;
+        pos[i] = 0;
+        if (col < c1) {
+          while (j + 1 < nr && rp[j + 1] <= col) ++j;
+          pos[i] = rs[j] + (col - rp[j]);
+        }
+      }
+    }
+    for (uint64_t m0 = 0; m0 < M; m0 += mp) {
+      const uint64_t m1 = m0 + mp < M ? m0 + mp : M;
+      if (!one_phase) {
+        if (m0 > 0) __syncthreads();
+        for (uint64_t e = threadIdx.x; e < (m1 - m0) * K; e += kWG) {
+          const uint64_t m = m0 + e / K, c = e % K;
+          float ans = 0.0f;
+          for (uint64_t t = 0; t < pq_len; ++t) {
+            if (m * pq_len + t >= d) break;
+            const float diff = pq_centers[m * pq_len * K + c * pq_len + t] - q[m * pq_len + t];
+            ans = fmaf(diff, diff, ans);
+          }
+          lut_store(&lut[e], ans);
+        }
+        __syncthreads();
+      }
+      if (quads && ((m1 - m0) & 31) == 0) {
+        // 32 codes (two 16-byte pieces) per item and step; four items' pieces are in flight before their
+        // sums start; each item still adds its m in ascending order
+#pragma unroll
+        for (int i0 = 0; i0 < PER; i0 += 4) {
+          for (uint64_t mb = m0; mb < m1; mb += 32) {
+            uint4 v[4][2];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const uint32_t col = c0 + threadIdx.x + (uint32_t)(i0 + u) * kWG;
+              const uint4* cd = reinterpret_cast<const uint4*>(codes + (uint64_t)pos[i0 + u] * M + mb);
+              const bool ok = col < c1;
+              v[u][0] = ok ? cd[0] : make_uint4(0, 0, 0, 0);
+              v[u][1] = ok ? cd[1] : make_uint4(0, 0, 0, 0);
+            }
+            const LT* lr0 = lut + (mb - m0) * K;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              float sum = part[i0 + u];
+#pragma unroll
+              for (int h = 0; h < 2; ++h) {
+                const LT* lr = lr0 + (uint64_t)h * 16 * K;
+                sum = adc_word(sum, v[u][h].x, lr, K);
+                sum = adc_word(sum, v[u][h].y, lr + 4 * K, K);
+                sum = adc_word(sum, v[u][h].z, lr + 8 * K, K);
+                sum = adc_word(sum, v[u][h].w, lr + 12 * K, K);
+              }
+              part[i0 + u] = sum;
+            }
+          }
+        }
+        continue;
+      }
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const uint32_t col = c0 + threadIdx.x + (uint32_t)i * kWG;
+        if (col >= c1) continue;
+        const uint8_t* cd = codes + (uint64_t)pos[i] * M;
+        float sum = part[i];
+        if (quads) {
+          for (uint64_t m = m0; m < m1; m += 16) {
+            const uint4 v = *reinterpret_cast<const uint4*>(cd + m);
+            const LT* lr = lut + (m - m0) * K;
+            sum = adc_word(sum, v.x, lr, K);
+            sum = adc_word(sum, v.y, lr + 4 * K, K);
+            sum = adc_word(sum, v.z, lr + 8 * K, K);
+            sum = adc_word(sum, v.w, lr + 12 * K, K);
+          }
+        } else if (words) {
+          for (uint64_t m = m0; m < m1; m += 4)
+            sum = adc_word(sum, *reinterpret_cast<const uint32_t*>(cd + m), lut + (m - m0) * K, K);
+        } else {
+          for (uint64_t m = m0; m < m1; ++m) sum += lutf(lut[(m - m0) * K + cd[m]]);
+        }
+        part[i] = sum;
+      }
+    }
+    uint32_t* out = adist + aoff[qi];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const uint32_t col = c0 + threadIdx.x + (uint32_t)i * kWG;
+      if (col < c1) out[col] = __float_as_uint(part[i]);
+    }
+    }
   }
 }
 
@@ -1270,8 +1501,8 @@ __global__ __launch_bounds__(256) void adc_select_kernel(
     const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ rstart,
     const uint32_t* __restrict__ rpre, const float* __restrict__ lab, const float* __restrict__ ranges,
     const uint64_t* __restrict__ aoff, uint64_t* __restrict__ akeys_sel_scratch,
-    const uint64_t* __restrict__ akeys, const uint64_t* __restrict__ loff, uint32_t* __restrict__ L,
-    uint32_t* __restrict__ nL) {
+    const uint32_t* __restrict__ adist, uint64_t* __restrict__ akeys2, const uint64_t* __restrict__ loff,
+    uint32_t* __restrict__ L, uint32_t* __restrict__ nL) {
   __shared__ uint32_t hist[2048];
   __shared__ uint64_t sh[8];
   __shared__ uint32_t cnt_sh;
@@ -1284,7 +1515,7 @@ __global__ __launch_bounds__(256) void adc_select_kernel(
     return;
   }
   uint64_t* sel = akeys_sel_scratch + loff[qi];  // kp keys
-  dev::wg_radix_select(akeys + aoff[qi], Cq, kp, sel, hist, sh);
+  dev::wg_radix_select_compact(dev::KeyDist{adist + aoff[qi]}, Cq, kp, sel, akeys2 + aoff[qi], hist, sh);
   const uint32_t nr = n_chosen_s[qi];
   const uint32_t* rs = rstart + qi * sub_lists;
   const uint32_t* rp = rpre + qi * (sub_lists + 1);
@@ -1311,7 +1542,7 @@ __global__ __launch_bounds__(256) void final_pick_kernel(
     uint64_t q0, uint32_t scap, const uint32_t* __restrict__ zid, int l, uint64_t topk, const uint32_t* __restrict__ mode,
     const uint64_t* __restrict__ kprime, const uint64_t* __restrict__ loff, const uint32_t* __restrict__ L,
     const uint32_t* __restrict__ nL, const uint64_t* __restrict__ pad32, const uint64_t* __restrict__ padmax,
-    uint64_t* __restrict__ keys, uint64_t* __restrict__ akeys, const uint64_t* __restrict__ aoff,
+    uint64_t* __restrict__ keys, const uint32_t* __restrict__ adist, const uint64_t* __restrict__ aoff,
     float* __restrict__ out_dist, uint64_t* __restrict__ out_ids) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t* sorted = reinterpret_cast<uint64_t*>(smem);
@@ -1320,11 +1551,12 @@ __global__ __launch_bounds__(256) void final_pick_kernel(
   const uint64_t qi = blockIdx.x;
   const uint32_t n = nL[qi];
   const uint32_t* list = L + loff[qi];
-  uint64_t* gk = mode[qi] == 2 ? akeys + aoff[qi] : keys + loff[qi];
+
   float* od = out_dist + (q0 + qi) * topk;
   uint64_t* oi = out_ids + (q0 + qi) * topk;
   const uint32_t k = (uint32_t)topk;
-  const uint32_t m = select_sorted(gk, n, k, sorted, scap, hist, sh);
+  const uint32_t m = mode[qi] == 2 ? select_sorted(dev::KeyDist{adist + aoff[qi]}, n, k, sorted, scap, hist, sh)
+                                   : select_sorted(dev::KeyArray{keys + loff[qi]}, n, k, sorted, scap, hist, sh);
   __syncthreads();
   if (mode[qi] == 2) {
     for (uint32_t j = threadIdx.x; j < k; j += kWG) {
@@ -1400,7 +1632,7 @@ struct Index::Work {
   DevBuf chosen_c, n_chosen_c, list_s, list_scnt, nnz_s, chosen_s, n_chosen_s, chosen_sid;
   DevBuf keys, woff, bsum, gstat, rstart, rpre, cand, real;
   DevBuf mode, kprime, lsize, asize, chunks, pad32, padmax, nL;
-  DevBuf L, Lkeys, akeys;
+  DevBuf L, Lkeys, adist, akeys2;
   DevBuf S, feat, budget_T, budget_S, qb16;  // PRECISION 1: scores, model features, budgets, bf16 queries
   DevBuf st, st_chosen;                // call-wide counters [nq][6] and chosen sub-cluster ids [nq][sub_lists]
   uint64_t st_nq = 0, st_sub_lists = 0;
@@ -1547,6 +1779,9 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   const bool sparse = cfg_.sparse_count != 0 && S.Nb > 0 && CS <= kSparseMaxCS;
   // PRECISION 1 refines a walked query from its in-box list, without Morton ranges or PQ (mode 3)
   const int inbox_mode = (cfg_.precision == 1 && sparse) ? 1 : 0;
+  // ADC work split: items per thread (chunk = 256 * adc_per candidates) and persistent workgroups
+  const uint32_t adc_per = cfg_.adc_per == 8 || cfg_.adc_per == 16 || cfg_.adc_per == 64 ? (uint32_t)cfg_.adc_per : 32u;
+  const uint64_t adc_wgs = cfg_.adc_wgs ? cfg_.adc_wgs : 2048;
 
   // workspace
   w.ranges.reserve(qn * 2 * l * 4);
@@ -1748,7 +1983,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   HDG_HIP(hipGetLastError());
   mark();
   hipLaunchKernelGGL(ranges_kernel, dim3((unsigned)qn), dim3(256), 0, s, S.zcode.as<uint64_t>(),
-                     S.seg_off.as<uint32_t>(), cap, w.list_s.as<uint32_t>(), w.list_scnt.as<uint32_t>(), sub_lists, G,
+                     S.seg_off.as<uint32_t>(), S.cells.as<uint32_t>(), S.N, g.n_valid, w.box_lo.as<uint32_t>(),
+                     w.box_w.as<uint32_t>(), qn, w.zlo_cnt.as<uint64_t>(), cap, w.list_s.as<uint32_t>(), w.list_scnt.as<uint32_t>(), sub_lists, G,
                      q0, w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.zlo.as<uint64_t>(),
                      w.zhi.as<uint64_t>(), w.chosen_sid.as<uint32_t>(), w.rstart.as<uint32_t>(),
                      w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.real.as<uint64_t>(),
@@ -1759,7 +1995,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      topk, w.cand.as<uint64_t>(), w.gstat.as<unsigned long long>(), w.mode.as<uint32_t>(),
                      w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(),
                      w.chunks.as<uint64_t>(), w.pad32.as<uint64_t>(), w.padmax.as<uint64_t>(), w.n_ib.as<uint32_t>(),
-                     inbox_mode);
+                     inbox_mode, adc_per * (uint32_t)kWG);
   HDG_HIP(hipGetLastError());
   scan_u64(w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(), w.chunks.as<uint64_t>(), qn, nullptr, nullptr, 0);
   HDG_HIP(hipGetLastError());
@@ -1774,7 +2010,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   const uint64_t totL = w.pinned[0], totA = w.pinned[1], totChunks = w.pinned[2];
   w.L.reserve(std::max<uint64_t>(totL, 1) * 4);
   w.Lkeys.reserve(std::max<uint64_t>(totL, 1) * 8);
-  w.akeys.reserve(std::max<uint64_t>(totA, 1) * 8);
+  w.adist.reserve(std::max<uint64_t>(totA, 1) * 4);
+  w.akeys2.reserve(std::max<uint64_t>(totA, 1) * 8);
   const int integral = S.labels_integral ? 1 : 0;
   if (sparse)
     hipLaunchKernelGGL(inbox_keep_kernel, dim3((unsigned)qn), dim3(256), 0, s, l, integral, (uint32_t)CS, sub_lists,
@@ -1790,18 +2027,55 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   HDG_HIP(hipGetLastError());
   mark();
   if (totChunks > 0) {
-    auto kern = mixed ? adc_scan_kernel<__half> : adc_scan_kernel<float>;
-    hipLaunchKernelGGL(kern, dim3((unsigned)totChunks), dim3(256), kLutPhaseBytes, s, qn, q0, queries,
-                       d, S.M, S.K, S.pq_len, S.pq_centers.as<float>(), S.codes.as<uint8_t>(), sub_lists,
-                       w.mode.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),
-                       w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.L.as<uint32_t>(), w.lsize.as<uint64_t>(),
-                       w.nL.as<uint32_t>(), w.chunks.as<uint64_t>(), w.asize.as<uint64_t>(), w.akeys.as<uint64_t>());
+    // LDS: the LUT (one phase, at most kLutPhaseBytes) and the range table
+    const uint64_t lut_entry = mixed ? 2 : 4;
+    const uint32_t lut_bytes =
+        (uint32_t)((std::min<uint64_t>(kLutPhaseBytes, S.M * S.K * lut_entry) + 15) & ~15ull);
+    const size_t adc_lds = lut_bytes + (sub_lists <= kAdcLdsRanges ? (2 * (size_t)sub_lists + 1) * 4 : 0);
+    const unsigned adc_grid = (unsigned)std::min<uint64_t>(totChunks, adc_wgs);
+    // the streaming variant needs the whole LUT in one LDS phase and 32-code steps
+    uint64_t mp_h = kLutPhaseBytes / (S.K * lut_entry);
+    if (mp_h > 4) mp_h &= ~3ull;
+    const bool stream = mp_h >= S.M && (S.M % 32) == 0 && (mp_h % 16) == 0;
+#define HDG_ADC1(LT, PER, KC, ST)                                                                                 \
+  hipLaunchKernelGGL((adc_scan_kernel<LT, PER, KC, ST>), dim3(adc_grid), dim3(256), adc_lds, s, qn, q0, queries, d,  \
+                     S.M, S.K, S.pq_len, S.pq_centers.as<float>(), S.codes.as<uint8_t>(), sub_lists,                 \
+                     w.mode.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),                     \
+                     w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.L.as<uint32_t>(), w.lsize.as<uint64_t>(),       \
+                     w.nL.as<uint32_t>(), w.chunks.as<uint64_t>(), w.asize.as<uint64_t>(), w.adist.as<uint32_t>(),   \
+                     lut_bytes)
+#define HDG_ADC(LT, PER)                                                                                          \
+  do {                                                                                                            \
+    if (S.K == 256 && stream) HDG_ADC1(LT, PER, 256, true);                                                       \
+    else if (S.K == 256) HDG_ADC1(LT, PER, 256, false);                                                           \
+    else if (stream) HDG_ADC1(LT, PER, 0, true);                                                                  \
+    else HDG_ADC1(LT, PER, 0, false);                                                                             \
+  } while (0)
+    switch (adc_per) {
+      case 8:
+        if (mixed) HDG_ADC(__half, 8); else HDG_ADC(float, 8);
+        break;
+      case 16:
+        if (mixed) HDG_ADC(__half, 16); else HDG_ADC(float, 16);
+        break;
+      case 64:
+        if (mixed) HDG_ADC(__half, 64); else HDG_ADC(float, 64);
+        break;
+      default:
+        if (mixed) HDG_ADC(__half, 32); else HDG_ADC(float, 32);
+        break;
+    }
+#undef HDG_ADC
+#undef HDG_ADC1
     HDG_HIP(hipGetLastError());
+  }
+  mark();
+  if (totChunks > 0) {
     hipLaunchKernelGGL(adc_select_kernel, dim3((unsigned)qn), dim3(256), 0, s, qn, l, sub_lists,
                        w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.cand.as<uint64_t>(),
                        w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(), w.rpre.as<uint32_t>(), S.lab.as<float>(),
-                       w.ranges.as<float>(), w.asize.as<uint64_t>(), w.Lkeys.as<uint64_t>(), w.akeys.as<uint64_t>(),
-                       w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>());
+                       w.ranges.as<float>(), w.asize.as<uint64_t>(), w.Lkeys.as<uint64_t>(), w.adist.as<uint32_t>(),
+                       w.akeys2.as<uint64_t>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>());
     HDG_HIP(hipGetLastError());
   }
   mark();
@@ -1814,7 +2088,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      topk,
                      w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(),
                      w.nL.as<uint32_t>(), w.pad32.as<uint64_t>(), w.padmax.as<uint64_t>(), w.Lkeys.as<uint64_t>(),
-                     w.akeys.as<uint64_t>(), w.asize.as<uint64_t>(), out_dist, out_ids);
+                     w.adist.as<uint32_t>(), w.asize.as<uint64_t>(), out_dist, out_ids);
   HDG_HIP(hipGetLastError());
   mark();
   hipLaunchKernelGGL(stats_kernel, dim3((unsigned)cdiv(qn, 256)), dim3(256), 0, s, qn, q0, w.cand.as<uint64_t>(),
@@ -1825,10 +2099,10 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                          hipMemcpyDeviceToDevice, s));
   static const char* plain_names[] = {"prep_kernel",     "count_kernels",   "coarse_dist+pick", "sub_expand_kernel",
                                       "sub_dist+pick",   "ranges_kernel",   "plan+sync",        "keep_all_kernel",
-                                      "adc_kernels",     "final_dist+pick"};
+                                      "adc_scan_kernel", "adc_select_kernel", "final_dist+pick"};
   static const char* mixed_names[] = {"prep_kernel",     "count_kernels",   "coarse_gemm+select+budget",
                                       "sub_expand_kernel", "sub_gemm+select", "ranges_kernel", "plan+sync",
-                                      "keep_all_kernel", "adc_kernels",     "final_dist+pick"};
+                                      "keep_all_kernel", "adc_scan_kernel", "adc_select_kernel", "final_dist+pick"};
   w.ev_names = mixed ? mixed_names : plain_names;
   w.ev_used = evn;
 }
