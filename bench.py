@@ -35,6 +35,82 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def traffic_of(kernel, workload):
+    """HBM bytes per launch of a kernel measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by the
+    counters' reading for a copy of known size) on this workload; None when no measurement is recorded."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t[workload][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def batched_leg(args, H, datasets, O, dev, log):
+    """Uneven selectivity: 5M x 128 f32 (1024-component mixture), two integer labels U[0, 1e7], 20k queries
+    with explicit per-query ranges whose joint selectivity is log-uniform in [1e-4, 0.5] (filter type 3,
+    FILTER_DIM 2), n_list 512, one launch sequence over all 20k queries.  Reports plain and mixed QPS,
+    recall@100 on a sample against the FP64 brute force, and the ADC scan's achieved HBM bandwidth
+    (algorithmic bytes C_q * pq_dim per ADC query / the scan kernel's HIP-event time)."""
+    import numpy as np
+    import torch
+    t0 = time.time()
+    w = datasets.explicit_ranges(n=args.batched_n, nq=args.batched_nq, d=128, components=1024, clusters=512,
+                                 sub_clusters=10)
+    conf = w.conf_text(N_LIST=512, QUERY_CHUNK=args.batched_nq, SPARSE_MAX_TESTS=16384)
+    idx = H.SuperGridIndex(conf_text=conf, filter_text=w.filter_text)
+    idx.build(torch.from_numpy(w.data).to(dev), torch.from_numpy(w.labels).to(dev))
+    torch.cuda.synchronize()
+    log(f"batched: {w.name} {w.data.shape}, {w.queries.shape[0]} queries, built in {time.time() - t0:.1f}s")
+    q = torch.from_numpy(w.queries).to(dev)
+    ql = torch.from_numpy(w.qlabels).to(dev)
+    ns = min(300, w.queries.shape[0])
+    ranges = O.recover_filters(w.qlabels[:ns], 2, **w.filter_arrays)
+    _, gt = O.bruteforce(w.data, w.labels, w.queries[:ns], ranges, 100)
+    out = {"workload": f"{w.name}: {args.batched_n} x 128 f32, 2 int labels U[0,1e7], explicit ranges with "
+                       f"selectivity log-uniform in [1e-4, 0.5], {w.queries.shape[0]} queries in one call, "
+                       f"n_list 512, sub_lists 128"}
+    M = idx.get_param("PQ_DIM")
+    best = None
+    for prec, name in [(0, "plain"), (1, "mixed")]:
+        idx.set_param("PRECISION", prec)
+        _, i = idx.query(q[:ns], ql[:ns])
+        torch.cuda.synchronize()
+        rec = datasets.recall(gt, i.cpu().numpy().astype(np.uint64), 100)
+        for _ in range(1):
+            idx.query(q, ql)
+        torch.cuda.synchronize()
+        steps = 3
+        adc_ms = 0.0
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            idx.query(q, ql)
+            adc_ms += idx.last_stage_ms().get("adc_scan_kernel", 0.0)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t1) / steps
+        adc_ms /= steps
+        st = idx.last_stats(w.queries.shape[0])
+        adc_bytes = float(st.adc.sum() * M)
+        gbs = adc_bytes / (adc_ms * 1e-3) / 1e9 if adc_ms > 0 else None
+        out[name] = {"qps": round(w.queries.shape[0] / el, 1), "ms_per_call": round(el * 1e3, 3),
+                     f"recall@100 ({ns}-query sample)": round(rec, 4),
+                     "adc_queries": int((st.adc > 0).sum()), "adc_scan_ms": round(adc_ms, 4),
+                     "adc_scan_gbs": round(gbs, 1) if gbs else None}
+        log(f"batched {name}: {el * 1e3:.2f} ms/call ({w.queries.shape[0] / el:.0f} QPS), recall {rec:.4f}, "
+            f"ADC scan {adc_bytes / 1e9:.2f} GB in {adc_ms:.3f} ms = {gbs:.0f} GB/s")
+        if prec == 0:
+            best = (adc_ms, adc_bytes, gbs)
+    adc_ms, adc_bytes, gbs = best
+    out["adc_roofline"] = {"kernel": "adc_scan_kernel (plain FP32 LUT)", "bound": "hbm", "achieved": round(gbs, 3),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 6),
+                           "traffic": traffic_of("adc_scan", w.name), "avg_ms": round(adc_ms, 4),
+                           "alg_bytes_per_launch": adc_bytes}
+    del idx
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -47,6 +123,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline")
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--paths", default="plain,mixed,learned")
+    ap.add_argument("--no-batched", action="store_true", help="skip the uneven-selectivity leg")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first write and then read 1 GiB (counter scale for rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--batched-n", type=int, default=5_000_000)
+    ap.add_argument("--batched-nq", type=int, default=20_000)
     ap.add_argument("--save-budget", default=None, help="write the trained budget model here")
     ap.add_argument("--load-budget", default=None, help="use this budget model instead of training one")
     args = ap.parse_args()
@@ -63,6 +144,11 @@ def main():
     from oracle import oracle as O
 
     dev = torch.device("cuda:0")
+    if args.calibrate:
+        buf = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)
+        buf.fill_(1.0)
+        log(f"calibration: 1 GiB written, sum {float(buf.sum()):.0f}")
+        del buf
     t0 = time.time()
     w = datasets.mixed_1m(n=args.n, nq=args.nq)
     log(f"workload {w.name}: data {w.data.shape}, queries {w.queries.shape} in {time.time() - t0:.1f}s")
@@ -163,28 +249,26 @@ def main():
     st = H_["st"]
     stage_avg = H_["stage"]
     d_ = w.queries.shape[1]
-    l_ = 3
     M = idx.get_param("PQ_DIM")
-    n_valid = 3
-    N = args.n
-    C, CS = 512, 5120
-    alg = {  # algorithmic HBM bytes per launch (one launch covers all nq queries)
-        "final_dist+pick": float(st.refined.sum() * 4 * d_ + args.nq * 100 * 12),
-        "keep_all_kernel": float(st.cand.sum() * 4 * l_),
-        "adc_kernels": float(st.adc.sum() * M),
-        "sub_gemm+select": float(CS * 2 * ((d_ + 31) // 32 * 32) + args.nq * (4 * d_) + args.nq * CS * 4 * 2),
-        "coarse_gemm+select+budget": float(C * 2 * ((d_ + 31) // 32 * 32) + args.nq * 4 * d_ + args.nq * C * 4 * 2),
-    }
-    dom = max(stage_avg, key=lambda k: stage_avg[k] if k != "plan+sync" else -1)
+    C = idx.get_param("CLUSTERS")
+    # dominant kernel: the exact re-rank (one launch covers all queries); algorithmic bytes = the rows it
+    # re-ranks (R_q * 4d per query) + the queries (4d) + its (distance, id) keys (12 B per row)
+    dom = "final_dist_kernel"
     dom_ms = stage_avg[dom]
-    dom_bytes = alg.get(dom)
-    achieved = (dom_bytes / (dom_ms * 1e-3) / 1e9) if (dom_bytes and dom_ms > 0) else None
-    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 3) if achieved else None,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6) if achieved else None, "traffic": None,
-                "avg_ms": round(dom_ms, 4), "alg_bytes_per_launch": dom_bytes}
-    log(f"reported path {head}: dominant stage {dom} {dom_ms:.3f} ms; C_q={st.cand.mean():.1f} "
-        f"real={st.real.mean():.1f} refined={st.refined.mean():.1f} scored={st.scored.mean():.1f}")
+    dom_bytes = float(st.refined.sum() * (4 * d_ + 12) + args.nq * 4 * d_)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    roofline = {"kernel": "pair_dist_kernel<1> (" + dom + ")", "bound": "hbm", "achieved": round(achieved, 3),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                "traffic": traffic_of("final_dist", w.name), "avg_ms": round(dom_ms, 4),
+                "alg_bytes_per_launch": dom_bytes}
+    log(f"reported path {head}: {dom} {dom_ms:.3f} ms, {achieved:.0f} GB/s; R_q={st.refined.mean():.1f} "
+        f"real={st.real.mean():.1f} scored={st.scored.mean():.1f}")
+    del idx, q_d, ql_d
+    torch.cuda.empty_cache()
+
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg(args, H, datasets, O, dev, log)
 
     cs = min(args.cpu_sample, args.nq)
     threads = O.omp_threads()
@@ -222,6 +306,9 @@ def main():
         "vs_baseline": None, "dtype": "bf16/f16 scoring, f32 re-rank" if head != "plain" else "f32",
         "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu,
     }
+    if batched:
+        out["adc_roofline"] = batched.pop("adc_roofline")
+        out["batched"] = batched
     print(json.dumps(out), flush=True)
 
 

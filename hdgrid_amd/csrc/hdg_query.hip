@@ -2081,9 +2081,11 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   mark();
   // ---- A13: exact re-rank of the refine lists, final top-k ----
   scan_u64(w.woff.as<uint64_t>(), nullptr, nullptr, qn, w.nL.as<uint32_t>(), w.mode.as<uint32_t>(), 2u);
+  mark();
   hipLaunchKernelGGL(pair_dist_kernel<1>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.vec.as<float>(),
                      w.woff.as<uint64_t>(), qn, w.L.as<uint32_t>(), (uint64_t)0, w.lsize.as<uint64_t>(),
                      w.nL.as<uint32_t>(), S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>());
+  mark();
   hipLaunchKernelGGL(final_pick_kernel, dim3((unsigned)qn), dim3(256), scap_f * 8, s, q0, scap_f, S.zid.as<uint32_t>(), l,
                      topk,
                      w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(),
@@ -2099,10 +2101,12 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                          hipMemcpyDeviceToDevice, s));
   static const char* plain_names[] = {"prep_kernel",     "count_kernels",   "coarse_dist+pick", "sub_expand_kernel",
                                       "sub_dist+pick",   "ranges_kernel",   "plan+sync",        "keep_all_kernel",
-                                      "adc_scan_kernel", "adc_select_kernel", "final_dist+pick"};
+                                      "adc_scan_kernel", "adc_select_kernel", "final_scan", "final_dist_kernel",
+                                      "final_pick_kernel"};
   static const char* mixed_names[] = {"prep_kernel",     "count_kernels",   "coarse_gemm+select+budget",
                                       "sub_expand_kernel", "sub_gemm+select", "ranges_kernel", "plan+sync",
-                                      "keep_all_kernel", "adc_scan_kernel", "adc_select_kernel", "final_dist+pick"};
+                                      "keep_all_kernel", "adc_scan_kernel", "adc_select_kernel", "final_scan",
+                                      "final_dist_kernel", "final_pick_kernel"};
   w.ev_names = mixed ? mixed_names : plain_names;
   w.ev_used = evn;
 }
