@@ -286,3 +286,45 @@ def test_bruteforce_pads_when_few_matches():
 def test_find_max_factor_matches_reference():
     for a, b, expect in CFG["factors"]:
         assert O.find_max_factor(a, b) == expect
+
+
+def test_budget_mlp_grads_match_autograd():
+    """oracle/budget_mlp.grads (the checker of the HIP training kernel) against torch autograd in float64 on the
+    same graph: forward activations bf16-rounded (straight-through), loss = mean squared error.  The oracle
+    also rounds the two back-propagated hidden gradients to bf16, so agreement is to bf16 precision; a dropped
+    ReLU mask, a transposed weight or a missing 2/n would be off by O(1)."""
+    torch = pytest.importorskip("torch")
+    from oracle import budget_mlp as BM
+    rng = np.random.default_rng(5)
+    n = 64
+    p = (rng.standard_normal(BM.N_PARAMS) * 0.2).astype(np.float32)
+    X = rng.standard_normal((n, BM.F)).astype(np.float32)
+    Y = rng.uniform(0, 8, n).astype(np.float32)
+    g_or, loss_or = BM.grads(p, X, Y)
+
+    def st(v):  # bf16 rounding in the forward, identity in the backward
+        r = torch.from_numpy(BM.bf16(v.detach().numpy().astype(np.float32)).astype(np.float64))
+        return v + (r - v).detach()
+
+    P = torch.tensor(p.astype(np.float64), requires_grad=True)
+    W1 = P[BM.OFF_W1:BM.OFF_B1].reshape(BM.F, BM.H)
+    b1 = P[BM.OFF_B1:BM.OFF_W2]
+    W2 = P[BM.OFF_W2:BM.OFF_B2].reshape(BM.H, BM.H)
+    b2 = P[BM.OFF_B2:BM.OFF_W3]
+    W3 = P[BM.OFF_W3:BM.OFF_B3]
+    b3 = P[BM.OFF_B3]
+    x = torch.from_numpy(BM.bf16(X).astype(np.float64))
+    h1 = st(torch.relu(x @ W1 + b1))
+    h2 = st(torch.relu(h1 @ W2 + b2))
+    y = h2 @ W3 + b3
+    loss = ((y - torch.from_numpy(Y.astype(np.float64))) ** 2).mean()
+    loss.backward()
+    g_ag = P.grad.numpy()
+    assert abs(loss_or - float(loss)) <= 1e-4 * max(1.0, float(loss))
+    scale = np.abs(g_ag).max()
+    np.testing.assert_allclose(g_or, g_ag, rtol=2e-2, atol=2e-2 * scale)
+    # the comparison is sensitive to each block: perturbing one block's gradient by its own scale fails
+    for a, b in [(BM.OFF_W1, BM.OFF_B1), (BM.OFF_W2, BM.OFF_B2), (BM.OFF_W3, BM.OFF_B3)]:
+        bad = g_or.copy()
+        bad[a:b] *= -1.0
+        assert not np.allclose(bad, g_ag, rtol=2e-2, atol=2e-2 * scale)
