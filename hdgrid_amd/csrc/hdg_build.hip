@@ -360,6 +360,7 @@ void prepare_mixed_centers(IndexState& st, const std::vector<float>& centers, co
 void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* labels, uint64_t label_cols,
                   hipStream_t s) {
   HDG_HIP(hipSetDevice(device_));
+  drop_graphs();
   if (!dataset || !labels) fail(kInvalidArgument, "null dataset or labels");
   if (n == 0 || d == 0) fail(kInvalidArgument, "empty dataset");
   if (n >= 0xFFFFFFFFull) fail(kUnsupported, "more than 2^32-1 base vectors per index");

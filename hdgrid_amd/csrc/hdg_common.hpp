@@ -39,6 +39,13 @@ inline void hip_check(hipError_t e, const char* what, const char* file, int line
 
 [[noreturn]] inline void fail(int st, const std::string& msg) { throw Error(st, msg); }
 
+// Counts device (re)allocations; a captured launch graph holds raw pointers and is valid only while the
+// count it was captured at still holds.
+inline uint64_t& devbuf_generation() {
+  static uint64_t g = 0;
+  return g;
+}
+
 // Owning device allocation (hipMalloc / hipFree), grows on demand and never shrinks.
 class DevBuf {
  public:
@@ -67,6 +74,7 @@ class DevBuf {
     size_t b = bytes < 256 ? 256 : bytes;
     HDG_HIP(hipMalloc(&p_, b));
     bytes_ = b;
+    ++devbuf_generation();
   }
   void release() {
     if (p_) (void)hipFree(p_);

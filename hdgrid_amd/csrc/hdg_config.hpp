@@ -52,6 +52,7 @@ struct Config {
   uint64_t sparse_max_tests = 131072;  // SPARSE_MAX_TESTS: entries a query may test on the walk before the dense count
   uint64_t adc_per = 32;               // ADC_PER: candidates per thread of an ADC chunk (8, 16, 32, 64)
   uint64_t adc_wgs = 2048;             // ADC_WGS: persistent ADC workgroups
+  uint64_t graph = 0;                  // GRAPH: 1 = replay each query chunk as a captured HIP graph
 };
 
 // Parse with config_utils.h:33-88 semantics: `KEY = value` per line, key case-insensitive and

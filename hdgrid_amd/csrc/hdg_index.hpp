@@ -124,6 +124,9 @@ class Index {
 
  private:
   BudgetModel* budget_ = nullptr;
+  uint64_t graph_epoch_ = 0;  // bumped by build / load: captured graphs of an older index state are stale
+  uint64_t config_digest() const;
+  void drop_graphs();
   void query_chunk(const float* queries, uint64_t q0, uint64_t nq, const float* qlabels,
                    uint64_t qcols, uint64_t group, float* out_dist, uint64_t* out_ids,
                    hipStream_t stream);

@@ -161,6 +161,7 @@ void Index::save(const std::string& dir) const {
 
 void Index::load(const std::string& dir, hipStream_t s) {
   HDG_HIP(hipSetDevice(device_));
+  drop_graphs();
   std::ifstream mf(dir + "/meta.json");
   if (!mf) fail(kIoError, "cannot read " + dir + "/meta.json");
   std::stringstream ss;

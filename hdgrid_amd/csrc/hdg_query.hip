@@ -68,6 +68,26 @@ Dims make_dims(const GridShape& g) {
 
 __host__ __device__ inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
+// Zeroes the chunk's reference-batch statistics and flags (a kernel, so that a captured graph holds no
+// memset node).
+__global__ void clear_kernel(unsigned long long* __restrict__ gstat, uint64_t n, uint32_t* __restrict__ err,
+                             uint32_t* __restrict__ n_dense, uint32_t* __restrict__ abort_flag) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) gstat[i] = 0;
+  if (i == 0) {
+    *err = 0;
+    *n_dense = 0;
+    if (abort_flag) *abort_flag = 0;
+  }
+}
+
+// Graph replay: the refine and ADC buffers were captured with fixed capacities; a chunk needing more sets
+// the abort flag, every later kernel of the chunk returns at once, and the host runs it again uncaptured.
+__global__ void cap_check_kernel(const uint64_t* __restrict__ ltot, const uint64_t* __restrict__ atot,
+                                 uint64_t lcap, uint64_t acap, uint32_t* __restrict__ abort_flag) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && (*ltot > lcap || *atot > acap)) *abort_flag = 1u;
+}
+
 // Every query of the chunk goes to the dense count (no bucket grid, or too many sub-clusters for LDS).
 __global__ void iota_kernel(uint64_t qn, uint32_t* __restrict__ dense_list, uint32_t* __restrict__ n_dense,
                             uint32_t* __restrict__ n_ib) {
@@ -548,7 +568,9 @@ __global__ __launch_bounds__(64) void pair_dist_kernel(const float* __restrict__
                                                        uint64_t qn, const uint32_t* __restrict__ list,
                                                        uint64_t lstride, const uint64_t* __restrict__ loff,
                                                        const uint32_t* __restrict__ nrows,
-                                                       const uint32_t* __restrict__ zid, uint64_t* __restrict__ keys) {
+                                                       const uint32_t* __restrict__ zid, uint64_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   __shared__ __align__(16) float qv[512];
   __shared__ __align__(16) float stage[64 * kStageLd];
   __shared__ uint32_t rows[64];
@@ -1103,7 +1125,9 @@ __global__ __launch_bounds__(256) void keep_all_kernel(
     const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zlo_cnt,
     const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ rstart,
     const uint32_t* __restrict__ rpre, const float* __restrict__ lab, const float* __restrict__ ranges,
-    const uint64_t* __restrict__ loff, uint32_t* __restrict__ L, uint32_t* __restrict__ nL) {
+    const uint64_t* __restrict__ loff, uint32_t* __restrict__ L, uint32_t* __restrict__ nL,
+    const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   __shared__ uint32_t tmp[8];
   extern __shared__ uint32_t rsp[];  // rs[sub_lists], rp[sub_lists + 1]
   uint32_t* rs = rsp;
@@ -1180,7 +1204,8 @@ __global__ __launch_bounds__(256) void inbox_keep_kernel(
     const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ chosen_sid, uint32_t ibcap,
     const uint32_t* __restrict__ ib_pos, const uint32_t* __restrict__ ib_sub, const float* __restrict__ lab,
     const float* __restrict__ ranges, const uint64_t* __restrict__ loff, uint32_t* __restrict__ L,
-    uint32_t* __restrict__ nL) {
+    uint32_t* __restrict__ nL, const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   __shared__ uint32_t tmp[8];
   __shared__ uint32_t bits[kSparseMaxCS / 32];
   const uint64_t qi = blockIdx.x;
@@ -1253,7 +1278,8 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
     const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ rpre, const uint64_t* __restrict__ cand,
     const uint32_t* __restrict__ L, const uint64_t* __restrict__ loff, const uint32_t* __restrict__ nL,
     const uint64_t* __restrict__ coff, const uint64_t* __restrict__ aoff, uint32_t* __restrict__ adist,
-    uint32_t lut_bytes) {
+    uint32_t lut_bytes, const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint64_t K = KC ? (uint64_t)KC : K_rt;  // codebook size; a compile-time 256 folds the LUT row offsets
   LT* lut = reinterpret_cast<LT*>(smem);  // [mp][K]
@@ -1502,7 +1528,8 @@ __global__ __launch_bounds__(256) void adc_select_kernel(
     const uint32_t* __restrict__ rpre, const float* __restrict__ lab, const float* __restrict__ ranges,
     const uint64_t* __restrict__ aoff, uint64_t* __restrict__ akeys_sel_scratch,
     const uint32_t* __restrict__ adist, uint64_t* __restrict__ akeys2, const uint64_t* __restrict__ loff,
-    uint32_t* __restrict__ L, uint32_t* __restrict__ nL) {
+    uint32_t* __restrict__ L, uint32_t* __restrict__ nL, const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   __shared__ uint32_t hist[2048];
   __shared__ uint64_t sh[8];
   __shared__ uint32_t cnt_sh;
@@ -1543,7 +1570,8 @@ __global__ __launch_bounds__(256) void final_pick_kernel(
     const uint64_t* __restrict__ kprime, const uint64_t* __restrict__ loff, const uint32_t* __restrict__ L,
     const uint32_t* __restrict__ nL, const uint64_t* __restrict__ pad32, const uint64_t* __restrict__ padmax,
     uint64_t* __restrict__ keys, const uint32_t* __restrict__ adist, const uint64_t* __restrict__ aoff,
-    float* __restrict__ out_dist, uint64_t* __restrict__ out_ids) {
+    float* __restrict__ out_dist, uint64_t* __restrict__ out_ids, const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t* sorted = reinterpret_cast<uint64_t*>(smem);
   __shared__ uint32_t hist[2048];
@@ -1608,7 +1636,8 @@ __global__ void stats_kernel(uint64_t qn, uint64_t q0, const uint64_t* __restric
                              const uint64_t* __restrict__ real, const uint32_t* __restrict__ nL,
                              const uint32_t* __restrict__ nnz_c, const uint32_t* __restrict__ nnz_s,
                              const uint32_t* __restrict__ mode, const uint64_t* __restrict__ kprime,
-                             uint64_t* __restrict__ st) {
+                             uint64_t* __restrict__ st, const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   const uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (qi >= qn) return;
   uint64_t* o = st + (q0 + qi) * 6;
@@ -1636,6 +1665,19 @@ struct Index::Work {
   DevBuf S, feat, budget_T, budget_S, qb16;  // PRECISION 1: scores, model features, budgets, bf16 queries
   DevBuf st, st_chosen;                // call-wide counters [nq][6] and chosen sub-cluster ids [nq][sub_lists]
   uint64_t st_nq = 0, st_sub_lists = 0;
+  // GRAPH 1: per-chunk captured graphs, replayed on an internal stream
+  struct Captured {
+    const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t q0 = 0, qn = 0, G = 0, qcols = 0, cfg = 0, gen = 0;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<Captured> graphs;
+  bool capturing = false;
+  DevBuf abort_flag;
+  uint64_t lcap = 0, acap = 0;  // refine-list and ADC capacities of the captured graphs
+  uint64_t last_totL = 0, last_totA = 0;  // largest totals seen by uncaptured runs
+  hipStream_t gstream = nullptr;
+  hipEvent_t gev[2];
   uint64_t* pinned = nullptr;
   hipEvent_t ev[16];
   bool ev_init = false;
@@ -1675,6 +1717,11 @@ Index::~Index() {
   budget_free();
   if (w_) {
     (void)hipSetDevice(device_);
+    drop_graphs();
+    if (w_->gstream) {
+      (void)hipStreamDestroy(w_->gstream);
+      for (auto& e : w_->gev) (void)hipEventDestroy(e);
+    }
     if (w_->pinned) (void)hipHostFree(w_->pinned);
     if (w_->ev_init)
       for (auto& e : w_->ev) (void)hipEventDestroy(e);
@@ -1750,10 +1797,108 @@ void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* ql
   w.st_nq = nq;
   w.st_sub_lists = cfg_.sub_lists;
   w.ev_used = 0;
-  for (uint64_t q0 = 0; q0 < nq; q0 += chunk) {
-    uint64_t qn = std::min(chunk, nq - q0);
-    query_chunk(queries, q0, qn, qlabels, qcols, G, out_dist, out_ids, s);
+  if (!cfg_.graph) {
+    for (uint64_t q0 = 0; q0 < nq; q0 += chunk) {
+      uint64_t qn = std::min(chunk, nq - q0);
+      query_chunk(queries, q0, qn, qlabels, qcols, G, out_dist, out_ids, s);
+    }
+    return;
   }
+  // GRAPH 1: every chunk's launch sequence is captured once (after an uncaptured run that sizes the buffers)
+  // and replayed on the internal stream, ordered after the caller's stream; one host synchronisation per
+  // chunk at its end reads the abort flag (capacity overflow -> uncaptured rerun and recapture).
+  if (!w.gstream) {
+    HDG_HIP(hipStreamCreateWithFlags(&w.gstream, hipStreamNonBlocking));
+    for (auto& e : w.gev) HDG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  w.abort_flag.reserve(4);
+  const uint64_t digest = config_digest();
+  uint64_t ci = 0;
+  for (uint64_t q0 = 0; q0 < nq; q0 += chunk, ++ci) {
+    const uint64_t qn = std::min(chunk, nq - q0);
+    if (w.graphs.size() <= ci) w.graphs.resize(ci + 1);
+    Work::Captured& c = w.graphs[ci];
+    const void* key[4] = {queries, qlabels, out_dist, out_ids};
+    const bool valid = c.exec && std::equal(key, key + 4, c.key) && c.q0 == q0 && c.qn == qn && c.G == G &&
+                       c.qcols == qcols && c.cfg == digest && c.gen == devbuf_generation();
+    if (valid) {
+      HDG_HIP(hipEventRecord(w.gev[0], s));
+      HDG_HIP(hipStreamWaitEvent(w.gstream, w.gev[0], 0));
+      HDG_HIP(hipGraphLaunch(c.exec, w.gstream));
+      HDG_HIP(hipMemcpyAsync(&w.pinned[4], w.abort_flag.data(), 4, hipMemcpyDeviceToHost, w.gstream));
+      HDG_HIP(hipMemcpyAsync(&w.pinned[5], w.err.data(), 4, hipMemcpyDeviceToHost, w.gstream));
+      HDG_HIP(hipStreamSynchronize(w.gstream));
+      if ((uint32_t)w.pinned[5] & 1u) fail(kInvalidArgument, "map filter label outside interval_map");
+      if (((uint32_t)w.pinned[4] & 1u) == 0) continue;
+    }
+    // uncaptured run: results for this call, buffer sizes for the capture
+    query_chunk(queries, q0, qn, qlabels, qcols, G, out_dist, out_ids, s);
+    w.lcap = std::max(w.lcap, w.last_totL + w.last_totL / 2 + 4096);
+    w.acap = std::max(w.acap, w.last_totA + w.last_totA / 2 + 4096);
+    w.L.reserve(w.lcap * 4);
+    w.Lkeys.reserve(w.lcap * 8);
+    w.adist.reserve(w.acap * 4);
+    w.akeys2.reserve(w.acap * 8);
+    HDG_HIP(hipStreamSynchronize(s));
+    if (c.exec) {
+      (void)hipGraphExecDestroy(c.exec);
+      c.exec = nullptr;
+    }
+    // a buffer that grows while the capture records would leave earlier nodes on the freed block: such a
+    // graph is dropped and the (now large enough) buffers are captured again
+    for (int attempt = 0;; ++attempt) {
+      const uint64_t gen0 = devbuf_generation();
+      hipGraph_t g = nullptr;
+      HDG_HIP(hipStreamBeginCapture(w.gstream, hipStreamCaptureModeThreadLocal));
+      w.capturing = true;
+      try {
+        query_chunk(queries, q0, qn, qlabels, qcols, G, out_dist, out_ids, w.gstream);
+      } catch (...) {
+        w.capturing = false;
+        (void)hipStreamEndCapture(w.gstream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      w.capturing = false;
+      HDG_HIP(hipStreamEndCapture(w.gstream, &g));
+      if (devbuf_generation() != gen0) {
+        (void)hipGraphDestroy(g);
+        if (attempt > 0) fail(kBadState, "query buffers kept growing during graph capture");
+        continue;
+      }
+      const hipError_t ie = hipGraphInstantiate(&c.exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HDG_HIP(ie);
+      break;
+    }
+    std::copy(key, key + 4, c.key);
+    c.q0 = q0;
+    c.qn = qn;
+    c.G = G;
+    c.qcols = qcols;
+    c.cfg = digest;
+    c.gen = devbuf_generation();
+  }
+}
+
+uint64_t Index::config_digest() const {
+  const uint64_t v[] = {cfg_.n_list, cfg_.sub_lists, cfg_.topk, cfg_.exps0, cfg_.precision, cfg_.probe_clusters,
+                        cfg_.budget_model, cfg_.sparse_count, cfg_.sparse_max_tests, cfg_.adc_per, cfg_.adc_wgs,
+                        cfg_.parity_batch, cfg_.query_chunk, cfg_.label_mask, graph_epoch_};
+  uint64_t h = 1469598103934665603ull;
+  for (uint64_t x : v) {
+    h ^= x;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+void Index::drop_graphs() {
+  ++graph_epoch_;
+  if (!w_) return;
+  for (auto& c : w_->graphs)
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+  w_->graphs.clear();
 }
 
 void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const float* qlabels, uint64_t qcols,
@@ -1837,9 +1982,11 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     hipLaunchKernelGGL(scan_fix_kernel, dim3(nb), dim3(1024), 0, s, a0, a1, a2, n, w.bsum.as<unsigned long long>());
   };
   unsigned long long* gstat = w.gstat.as<unsigned long long>() + g_base * 8;
-  HDG_HIP(hipMemsetAsync(gstat, 0, ng * 8 * 8, s));
-  HDG_HIP(hipMemsetAsync(w.err.data(), 0, 4, s));
-  HDG_HIP(hipMemsetAsync(w.n_dense.data(), 0, 4, s));
+  uint32_t* abort_w = w.capturing ? w.abort_flag.as<uint32_t>() : nullptr;
+  const uint32_t* abort_p = abort_w;
+  hipLaunchKernelGGL(clear_kernel, dim3((unsigned)cdiv(ng * 8, 256)), dim3(256), 0, s, gstat, ng * 8,
+                     w.err.as<uint32_t>(), w.n_dense.as<uint32_t>(), abort_w);
+  HDG_HIP(hipGetLastError());
 
   // dynamic LDS of the per-query select kernels: the sort buffer
   const uint32_t scap_c = sort_cap(n_list), scap_s = sort_cap(sub_lists), scap_f = sort_cap(topk);
@@ -1849,7 +1996,9 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     w.ev_init = true;
   }
   int evn = 0;
-  auto mark = [&]() { HDG_HIP(hipEventRecord(w.ev[evn++], s)); };
+  auto mark = [&]() {
+    if (!w.capturing) HDG_HIP(hipEventRecord(w.ev[evn++], s));
+  };
   mark();
   hipLaunchKernelGGL(prep_kernel, dim3((unsigned)cdiv(qn, 128)), dim3(128), 0, s, qlabels, qcols, q0, qn, G, D,
                      f_type_.as<int>(), f_shift_.as<float>(), f_map_.as<float>(), f_map_len_, f_div_.as<int>(),
@@ -1941,7 +2090,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     scan_u64(w.woff.as<uint64_t>(), nullptr, nullptr, qn, w.nnz_c.as<uint32_t>(), nullptr, 0u);
     hipLaunchKernelGGL(pair_dist_kernel<0>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.centers.as<float>(),
                        w.woff.as<uint64_t>(), qn, w.list_c.as<uint32_t>(), C, nullptr, w.nnz_c.as<uint32_t>(),
-                       nullptr, w.keys.as<uint64_t>());
+                       nullptr, w.keys.as<uint64_t>(), nullptr);
     hipLaunchKernelGGL(coarse_pick_kernel, dim3((unsigned)qn), dim3(256), scap_c * 8, s, (uint32_t)C, n_list, G, q0,
                        scap_c,
                        w.nnz_c.as<uint32_t>(), w.gstat.as<unsigned long long>(), w.keys.as<uint64_t>(),
@@ -1969,7 +2118,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     scan_u64(w.woff.as<uint64_t>(), nullptr, nullptr, qn, w.nnz_s.as<uint32_t>(), nullptr, 0u);
     hipLaunchKernelGGL(pair_dist_kernel<0>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d,
                        S.sub_centers.as<float>(), w.woff.as<uint64_t>(), qn, w.list_s.as<uint32_t>(), (uint64_t)cap,
-                       nullptr, w.nnz_s.as<uint32_t>(), nullptr, w.keys.as<uint64_t>());
+                       nullptr, w.nnz_s.as<uint32_t>(), nullptr, w.keys.as<uint64_t>(), nullptr);
     hipLaunchKernelGGL(sub_pick_kernel, dim3((unsigned)qn), dim3(256), scap_s * 8, s, sub_lists, G, q0, cap, scap_s,
                        w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>(), budget_S, w.keys.as<uint64_t>(),
                        w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>());
@@ -1999,15 +2148,30 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   HDG_HIP(hipGetLastError());
   scan_u64(w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(), w.chunks.as<uint64_t>(), qn, nullptr, nullptr, 0);
   HDG_HIP(hipGetLastError());
-  // the one host synchronisation of a chunk: sizes of the data-dependent buffers
-  HDG_HIP(hipMemcpyAsync(&w.pinned[0], w.lsize.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
-  HDG_HIP(hipMemcpyAsync(&w.pinned[1], w.asize.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
-  HDG_HIP(hipMemcpyAsync(&w.pinned[2], w.chunks.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
-  HDG_HIP(hipMemcpyAsync(&w.pinned[3], w.err.data(), 4, hipMemcpyDeviceToHost, s));
-  HDG_HIP(hipStreamSynchronize(s));
-  mark();
-  if ((uint32_t)w.pinned[3] & 1u) fail(kInvalidArgument, "map filter label outside interval_map");
-  const uint64_t totL = w.pinned[0], totA = w.pinned[1], totChunks = w.pinned[2];
+  uint64_t totL, totA, totChunks;
+  if (!w.capturing) {
+    // the one host synchronisation of a chunk: sizes of the data-dependent buffers
+    HDG_HIP(hipMemcpyAsync(&w.pinned[0], w.lsize.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
+    HDG_HIP(hipMemcpyAsync(&w.pinned[1], w.asize.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
+    HDG_HIP(hipMemcpyAsync(&w.pinned[2], w.chunks.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
+    HDG_HIP(hipMemcpyAsync(&w.pinned[3], w.err.data(), 4, hipMemcpyDeviceToHost, s));
+    HDG_HIP(hipStreamSynchronize(s));
+    mark();
+    if ((uint32_t)w.pinned[3] & 1u) fail(kInvalidArgument, "map filter label outside interval_map");
+    totL = w.pinned[0];
+    totA = w.pinned[1];
+    totChunks = w.pinned[2];
+    w.last_totL = std::max(w.last_totL, totL);
+    w.last_totA = std::max(w.last_totA, totA);
+  } else {
+    // captured: fixed capacities; a chunk that needs more raises the abort flag
+    hipLaunchKernelGGL(cap_check_kernel, dim3(1), dim3(64), 0, s, w.lsize.as<uint64_t>() + qn,
+                       w.asize.as<uint64_t>() + qn, w.lcap, w.acap, abort_w);
+    HDG_HIP(hipGetLastError());
+    totL = w.lcap;
+    totA = w.acap;
+    totChunks = 1;  // the persistent ADC grid reads the real count on the device
+  }
   w.L.reserve(std::max<uint64_t>(totL, 1) * 4);
   w.Lkeys.reserve(std::max<uint64_t>(totL, 1) * 8);
   w.adist.reserve(std::max<uint64_t>(totA, 1) * 4);
@@ -2018,12 +2182,12 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                        w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(), w.zlo_cnt.as<uint64_t>(),
                        w.n_chosen_s.as<uint32_t>(), w.chosen_sid.as<uint32_t>(), kInboxCap, w.ib_pos.as<uint32_t>(),
                        w.ib_sub.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
-                       w.L.as<uint32_t>(), w.nL.as<uint32_t>());
+                       w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
   hipLaunchKernelGGL(keep_all_kernel, dim3((unsigned)qn), dim3(256), (2 * sub_lists + 1) * 4, s, qn, l, integral,
                      sub_lists, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
                      w.zlo_cnt.as<uint64_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),
                      w.rpre.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
-                     w.L.as<uint32_t>(), w.nL.as<uint32_t>());
+                     w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
   HDG_HIP(hipGetLastError());
   mark();
   if (totChunks > 0) {
@@ -2032,7 +2196,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     const uint32_t lut_bytes =
         (uint32_t)((std::min<uint64_t>(kLutPhaseBytes, S.M * S.K * lut_entry) + 15) & ~15ull);
     const size_t adc_lds = lut_bytes + (sub_lists <= kAdcLdsRanges ? (2 * (size_t)sub_lists + 1) * 4 : 0);
-    const unsigned adc_grid = (unsigned)std::min<uint64_t>(totChunks, adc_wgs);
+    const unsigned adc_grid = (unsigned)(w.capturing ? adc_wgs : std::min<uint64_t>(totChunks, adc_wgs));
     // the streaming variant needs the whole LUT in one LDS phase and 32-code steps
     uint64_t mp_h = kLutPhaseBytes / (S.K * lut_entry);
     if (mp_h > 4) mp_h &= ~3ull;
@@ -2043,7 +2207,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      w.mode.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),                     \
                      w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.L.as<uint32_t>(), w.lsize.as<uint64_t>(),       \
                      w.nL.as<uint32_t>(), w.chunks.as<uint64_t>(), w.asize.as<uint64_t>(), w.adist.as<uint32_t>(),   \
-                     lut_bytes)
+                     lut_bytes, abort_p)
 #define HDG_ADC(LT, PER)                                                                                          \
   do {                                                                                                            \
     if (S.K == 256 && stream) HDG_ADC1(LT, PER, 256, true);                                                       \
@@ -2075,7 +2239,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                        w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.cand.as<uint64_t>(),
                        w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(), w.rpre.as<uint32_t>(), S.lab.as<float>(),
                        w.ranges.as<float>(), w.asize.as<uint64_t>(), w.Lkeys.as<uint64_t>(), w.adist.as<uint32_t>(),
-                       w.akeys2.as<uint64_t>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>());
+                       w.akeys2.as<uint64_t>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(),
+                       abort_p);
     HDG_HIP(hipGetLastError());
   }
   mark();
@@ -2084,18 +2249,18 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   mark();
   hipLaunchKernelGGL(pair_dist_kernel<1>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.vec.as<float>(),
                      w.woff.as<uint64_t>(), qn, w.L.as<uint32_t>(), (uint64_t)0, w.lsize.as<uint64_t>(),
-                     w.nL.as<uint32_t>(), S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>());
+                     w.nL.as<uint32_t>(), S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>(), abort_p);
   mark();
   hipLaunchKernelGGL(final_pick_kernel, dim3((unsigned)qn), dim3(256), scap_f * 8, s, q0, scap_f, S.zid.as<uint32_t>(), l,
                      topk,
                      w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(),
                      w.nL.as<uint32_t>(), w.pad32.as<uint64_t>(), w.padmax.as<uint64_t>(), w.Lkeys.as<uint64_t>(),
-                     w.adist.as<uint32_t>(), w.asize.as<uint64_t>(), out_dist, out_ids);
+                     w.adist.as<uint32_t>(), w.asize.as<uint64_t>(), out_dist, out_ids, abort_p);
   HDG_HIP(hipGetLastError());
   mark();
   hipLaunchKernelGGL(stats_kernel, dim3((unsigned)cdiv(qn, 256)), dim3(256), 0, s, qn, q0, w.cand.as<uint64_t>(),
                      w.real.as<uint64_t>(), w.nL.as<uint32_t>(), w.nnz_c.as<uint32_t>(), w.nnz_s.as<uint32_t>(),
-                     w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.st.as<uint64_t>());
+                     w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.st.as<uint64_t>(), abort_p);
   HDG_HIP(hipGetLastError());
   HDG_HIP(hipMemcpyAsync(w.st_chosen.as<uint32_t>() + q0 * sub_lists, w.chosen_sid.data(), qn * sub_lists * 4,
                          hipMemcpyDeviceToDevice, s));
@@ -2108,7 +2273,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                                       "keep_all_kernel", "adc_scan_kernel", "adc_select_kernel", "final_scan",
                                       "final_dist_kernel", "final_pick_kernel"};
   w.ev_names = mixed ? mixed_names : plain_names;
-  w.ev_used = evn;
+  w.ev_used = w.capturing ? 0 : evn;
 }
 
 }  // namespace hdg
