@@ -73,6 +73,10 @@ _budget_load = _sig("hdg_budget_load", ctypes.c_int, [_vp, _cp, _vp])
 _budget_grads = _sig("hdg_budget_debug_grads", ctypes.c_int, [_fp, _fp, _fp, ctypes.c_uint32, _fp, _fp])
 _budget_nparams = _sig("hdg_budget_param_count", ctypes.c_int, [])
 _coarse_scores = _sig("hdg_debug_coarse_scores", ctypes.c_int, [_vp, _vp, _u64, _fp, _vp])
+_merge = _sig("hdg_merge_topk", ctypes.c_int, [_vp, _vp, _u64, _u64, _u64, _u64, _u64, _u64, _vp, _vp, _vp])
+_fit_coeff = _sig("hdg_fit_mem_coeff", ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)])
+_autotune = _sig("hdg_autotune", ctypes.c_int,
+                 [_vp, _vp, _u64, _u64, _vp, _u64, _vp, _vp, ctypes.c_uint32, _cp, _u64, _vp])
 
 BUDGET_FEATURES = 48
 
@@ -120,6 +124,30 @@ def split_task(coeff, n_data, n_queries, topk, mem_bound, available, data_width=
     qb, db = _u64(), _u64()
     _check(_split(arr, n_data, n_queries, topk, mem_bound, available, data_width, ctypes.byref(qb), ctypes.byref(db)))
     return qb.value, db.value
+
+
+def fit_mem_coeff(samples):
+    """Memory-predictor coefficients from four (data rows, query rows, bytes used) probes, solved with the
+    reference's gauss (math_utils.h:6-41).  Returns (code, [c0, c1, c2, c3])."""
+    a = (ctypes.c_double * 12)(*[float(x) for row in samples for x in row])
+    c = (ctypes.c_double * 4)()
+    code = _fit_coeff(a, c)
+    return int(code), [c[i] for i in range(4)]
+
+
+def merge_topk(dist, ids, data_batch_size: int, n_data: int, start_offset: int = 0, stream=None):
+    """Device top-k merge of per-data-batch results: dist f32 / ids int64 tensors [n_batches][nq][k] on the
+    GPU (ids batch-local) -> (dist [nq][k], ids [nq][k]) with global ids (merge_res.cu:18-55)."""
+    import torch
+    assert dist.is_cuda and ids.is_cuda and dist.dim() == 3 and dist.shape == ids.shape
+    dist = dist.contiguous()
+    ids = ids.contiguous()
+    nb, nq, k = dist.shape
+    od = torch.empty((nq, k), dtype=torch.float32, device=dist.device)
+    oi = torch.empty((nq, k), dtype=torch.int64, device=dist.device)
+    _check(_merge(_ptr(dist), _ptr(ids), nb, data_batch_size, n_data, start_offset, nq, k, _ptr(od), _ptr(oi),
+                  _stream_ptr(stream)))
+    return od, oi
 
 
 def validate_config_text(conf_text: str, filter_text: str) -> str:
@@ -265,6 +293,22 @@ class SuperGridIndex:
         out = np.zeros((queries.shape[0], self.get_param("CLUSTERS")), np.float32)
         _check(_coarse_scores(self._h, _ptr(queries), queries.shape[0], out.ctypes.data_as(_fp), _stream_ptr(stream)))
         return out
+
+    def autotune(self, queries, query_labels, reps: int = 3, stream=None) -> str:
+        """Tune ADC_PER, ADC_WGS, SPARSE_MAX_TESTS, QUERY_CHUNK and GRAPH on this batch (results unchanged
+        bit for bit); the chosen values stay set.  Returns the trial log."""
+        import torch
+        queries = queries.contiguous()
+        query_labels = query_labels.contiguous()
+        nq = queries.shape[0]
+        k = self.get_param("TOPK")
+        od = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
+        oi = torch.empty((nq, k), dtype=torch.int64, device=queries.device)
+        qcols = query_labels.shape[1] if query_labels.dim() == 2 else 1
+        buf = ctypes.create_string_buffer(8192)
+        _check(_autotune(self._h, _ptr(queries), nq, queries.shape[1], _ptr(query_labels), qcols, _ptr(od), _ptr(oi),
+                         reps, buf, len(buf), _stream_ptr(stream)))
+        return buf.value.decode()
 
     def last_stage_ms(self):
         names = ctypes.create_string_buffer(1024)

@@ -5,12 +5,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cctype>
 #include <cstdio>
 #include <cstring>
 #include <string>
 
 #include "hdg_index.hpp"
+#include "hdg_merge.hpp"
 
 struct hdg_index {
   hdg::Index* impl;
@@ -82,6 +84,34 @@ void split_task(const double* coeff, uint64_t n_data, uint64_t n_queries, uint64
   db = (uint64_t)hdg::find_max_factor(l_d, n_data);
   qb = (uint64_t)hdg::find_max_factor(qb, n_queries);
   if (db * 4 <= n_data) db *= data_width;
+}
+
+// gauss (math_utils.h:6-41) restated: Gauss-Jordan elimination with partial pivoting on the augmented
+// n x (n+1) system, pivots below 1e-30 skipped; returns 0 (unique solution left in column n), 1 (rank
+// deficient, consistent) or 2 (inconsistent).
+int gauss(double a[][5], int n) {
+  const double eps = 1e-30;
+  int r = 0;
+  for (int c = 0; c < n; ++c) {
+    int t = r;
+    for (int i = r; i < n; ++i)
+      if (std::fabs(a[i][c]) > std::fabs(a[t][c])) t = i;
+    if (std::fabs(a[t][c]) < eps) continue;
+    for (int i = c; i <= n; ++i) std::swap(a[t][i], a[r][i]);
+    for (int i = n; i >= c; --i) a[r][i] /= a[r][c];
+    for (int i = r + 1; i < n; ++i)
+      if (std::fabs(a[i][c]) > eps)
+        for (int j = n; j >= c; --j) a[i][j] -= a[i][c] * a[r][j];
+    ++r;
+  }
+  if (r < n) {
+    for (int i = r; i < n; ++i)
+      if (std::fabs(a[i][n]) > eps) return 2;
+    return 1;
+  }
+  for (int i = n - 1; i >= 0; --i)
+    for (int j = i + 1; j < n; ++j) a[i][n] -= a[j][n] * a[i][j];
+  return 0;
 }
 
 }  // namespace
@@ -321,6 +351,43 @@ hdg_status hdg_debug_coarse_scores(hdg_index* index, const float* queries, uint6
   return guarded([&] {
     if (!index || !queries || !out) hdg::fail(HDG_EINVAL, "null argument");
     index->impl->debug_coarse_scores(queries, nq, out, (hipStream_t)stream);
+  });
+}
+
+hdg_status hdg_merge_topk(const float* dist, const uint64_t* ids, uint64_t n_batches, uint64_t data_batch_size,
+                          uint64_t n_data, uint64_t start_offset, uint64_t nq, uint64_t topk, float* out_dist,
+                          uint64_t* out_ids, void* stream) {
+  return guarded([&] {
+    hdg::merge_topk(dist, ids, n_batches, data_batch_size, n_data, start_offset, nq, topk, out_dist, out_ids,
+                    (hipStream_t)stream);
+  });
+}
+
+int hdg_fit_mem_coeff(const double samples[4][3], double coeff[4]) {
+  double a[4][5];
+  for (int i = 0; i < 4; ++i) {
+    a[i][0] = samples[i][0];
+    a[i][1] = samples[i][1];
+    a[i][2] = samples[i][0] * samples[i][1];
+    a[i][3] = 1.0;
+    a[i][4] = samples[i][2];
+  }
+  const int r = gauss(a, 4);
+  for (int i = 0; i < 4; ++i) coeff[i] = a[i][4];
+  return r;
+}
+
+hdg_status hdg_autotune(hdg_index* index, const float* queries, uint64_t nq, uint64_t dim,
+                        const float* query_labels, uint64_t qlabel_cols, float* out_dist, uint64_t* out_ids,
+                        uint32_t reps, char* report, uint64_t cap, void* stream) {
+  return guarded([&] {
+    if (!index || !queries || !query_labels || !out_dist || !out_ids) hdg::fail(HDG_EINVAL, "null argument");
+    const std::string r = index->impl->autotune(queries, nq, dim, query_labels, qlabel_cols, out_dist, out_ids,
+                                                reps, (hipStream_t)stream);
+    if (report && cap) {
+      std::strncpy(report, r.c_str(), cap - 1);
+      report[cap - 1] = 0;
+    }
   });
 }
 

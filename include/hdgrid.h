@@ -133,6 +133,33 @@ int hdg_budget_param_count(void);
  * out[q][c] = |c|^2 - 2 <q, c> computed by the BF16 MFMA kernel.  For checking that kernel. */
 hdg_status hdg_debug_coarse_scores(hdg_index* index, const float* queries, uint64_t nq, float* out, void* stream);
 
+/* Top-k merge of per-data-batch results (device arrays): dist/ids [n_batches][nq][topk], ids local to
+ * their batch; batch b holds rows [b*data_batch_size, min(n_data, (b+1)*data_batch_size)).  Writes
+ * out_dist/out_ids [nq][topk] (device): per query the topk smallest distances, ascending, ties by (batch,
+ * rank); ids become start_offset + b*data_batch_size + local, pads (ids >= the batch's row count) are kept.
+ * topk <= 4096.  Replaces merge_intermediate_result (merge_res.cu:18-55), which adds the offsets as
+ * floats (to pads too) and reads the batches back from files. */
+hdg_status hdg_merge_topk(const float* dist, const uint64_t* ids, uint64_t n_batches, uint64_t data_batch_size,
+                          uint64_t n_data, uint64_t start_offset, uint64_t nq, uint64_t topk, float* out_dist,
+                          uint64_t* out_ids, void* stream);
+
+/* Memory-predictor fit: samples[i] = {data rows, query rows, device bytes used} of four probe runs; solves
+ * c0*d + c1*q + c2*d*q + c3 = used by the reference's gauss (math_utils.h:6-41; calc_mem_coeff,
+ * search-engine-auto-batch.cu:381-428) and writes coeff[4].  Returns gauss's code: 0 unique solution,
+ * 1 rank deficient, 2 inconsistent.  Host only. */
+int hdg_fit_mem_coeff(const double samples[4][3], double coeff[4]);
+
+/* Launch-parameter autotuner.  Times full queries of the given batch (device arrays, as hdg_query;
+ * out_dist/out_ids receive the results) under a ladder of values of each result-invariant knob
+ * (ADC_PER, ADC_WGS, SPARSE_MAX_TESTS, QUERY_CHUNK, GRAPH), one knob at a time, `reps` timed runs per
+ * value (median), and keeps the fastest value whose ids and distances are bit-identical to those of the
+ * starting configuration.  The chosen values stay set on the index; the trial log is written to
+ * report (cap bytes, may be NULL).  No reference counterpart: the reference's launch shapes are
+ * compile-time constants. */
+hdg_status hdg_autotune(hdg_index* index, const float* queries, uint64_t nq, uint64_t dim,
+                        const float* query_labels, uint64_t qlabel_cols, float* out_dist, uint64_t* out_ids,
+                        uint32_t reps, char* report, uint64_t cap, void* stream);
+
 /* Memory-bound batch predictor: split_task (search-engine-auto-batch.cu:430-489) with the
  * reference's model c0*d + c1*q + c2*d*q + c3 <= min(available/5, mem_bound). */
 hdg_status hdg_split_task(const double coeff[4], uint64_t n_data, uint64_t n_queries, uint64_t topk,

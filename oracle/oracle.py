@@ -171,6 +171,25 @@ def find_max_factor(a, b):
     return int(_lib.orc_find_max_factor(_u64(a), _u64(b)))
 
 
+def merge_topk(dist, ids, data_batch_size, n_data, start_offset, k):
+    """merge_intermediate_result (merge_res.cu:18-55): per query, the k smallest distances over the n_batches
+    per-data-batch top-k rows (select_k min, sorted, merge_res.cu:50-52), ids gathered from the same
+    positions (:54-55) after the batch offset start_offset + b * data_batch_size is added (modify_blocks,
+    :2-16).  Restated with integer offsets and pads (ids >= the batch's row count) left unchanged, where the
+    reference adds a float to every id; ties go to the lower position (batch, then rank).
+    dist [nb][nq][kb] (f32, or f64 for exact checks), ids u64 [nb][nq][kb] -> (dist [nq][k], u64 [nq][k])."""
+    dist = np.asarray(dist)
+    ids = np.asarray(ids, np.uint64)
+    nb, nq, kb = dist.shape
+    rows = np.array([max(0, min(data_batch_size, n_data - b * data_batch_size)) for b in range(nb)], np.uint64)
+    base = np.array([start_offset + b * data_batch_size for b in range(nb)], np.uint64)
+    glob = np.where(ids < rows[:, None, None], ids + base[:, None, None], ids)
+    flat_d = dist.transpose(1, 0, 2).reshape(nq, nb * kb)
+    flat_i = glob.transpose(1, 0, 2).reshape(nq, nb * kb)
+    order = np.argsort(flat_d, axis=1, kind="stable")[:, :k]
+    return np.take_along_axis(flat_d, order, 1), np.take_along_axis(flat_i, order, 1)
+
+
 def omp_threads():
     return int(_lib.orc_omp_threads())
 

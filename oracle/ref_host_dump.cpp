@@ -7,6 +7,8 @@
 //   ref_host_dump filter <filter.conf>      -> JSON of the parsed filter_config
 //   ref_host_dump factor <a> <b>            -> findMaxFactor(a, b)
 //   ref_host_dump divide <n> <m> <i>        -> uniformly_divide(n, m, i): size offset
+//   ref_host_dump gauss <d q used> x 4      -> gauss() on calc_mem_coeff's system
+//                                              (search-engine-auto-batch.cu:404-427): code c0 c1 c2 c3
 #include <algorithm>
 #include <cassert>
 #include <cmath>
@@ -105,8 +107,22 @@ int main(int argc, char** argv) {
     uint64_t size = 0, offset = 0;
     uniformly_divide(std::stoull(argv[2]), std::stoull(argv[3]), std::stoull(argv[4]), size, offset);
     std::cout << size << " " << offset << std::endl;
+  } else if (argc >= 14 && !std::strcmp(argv[1], "gauss")) {
+    double a[4][5];
+    for (int i = 0; i < 4; ++i) {
+      const double d = std::stod(argv[2 + 3 * i]), q = std::stod(argv[3 + 3 * i]), used = std::stod(argv[4 + 3 * i]);
+      a[i][0] = d;
+      a[i][1] = q;
+      a[i][2] = d * q;
+      a[i][3] = 1;
+      a[i][4] = used;
+    }
+    const int r = gauss(a, 4);
+    char b[128];
+    std::snprintf(b, sizeof(b), "%d %.17g %.17g %.17g %.17g", r, a[0][4], a[1][4], a[2][4], a[3][4]);
+    std::cout << b << std::endl;
   } else {
-    std::cerr << "usage: ref_host_dump conf|filter <path> | factor <a> <b> | divide <n> <m> <i>" << std::endl;
+    std::cerr << "usage: ref_host_dump conf|filter <path> | factor <a> <b> | divide <n> <m> <i> | gauss <12 numbers>" << std::endl;
     return 2;
   }
   return 0;

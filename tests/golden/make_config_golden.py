@@ -1,5 +1,5 @@
-"""Writes tests/golden/config_parse.json: outputs of the reference's own HDGrid.conf / filter.conf parsers
-and findMaxFactor / uniformly_divide (compiled from the reference tree by oracle/Makefile into
+"""Writes tests/golden/config_parse.json: outputs of the reference's own HDGrid.conf / filter.conf parsers,
+findMaxFactor / uniformly_divide and gauss on calc_mem_coeff's system (compiled from the reference tree by oracle/Makefile into
 oracle/_ref/ref_host_dump) on a fixed set of inputs.  Run from the repo root after `make -C oracle ref`.
 """
 import json
@@ -34,6 +34,17 @@ FILTERS = {
 FACTORS = [(125, 1000), (125, 10000), (125, 20000), (125, 100000), (125, 100), (125, 125), (125, 997),
            (100000, 1000000), (7, 36), (125, 126)]
 DIVIDES = [(1000, 8, 0), (1000, 8, 7), (1003, 8, 2), (1003, 8, 5), (10, 3, 1)]
+# calc_mem_coeff systems: (data rows, query rows, bytes used) x 4 -- the reference's probe sizes with
+# plausible footprints, an exactly affine footprint, and two singular systems (a repeated probe; the same
+# probe with two footprints) where round-off decides what gauss returns
+GAUSS = [
+    [(10000, 5, 1.2e8), (20000, 10, 2.3e8), (30000, 25, 3.9e8), (40000, 20, 4.6e8)],
+    [(10000, 5, 1536 * 10000 + 65536 * 5 + 2**28), (20000, 10, 1536 * 20000 + 65536 * 10 + 2**28),
+     (30000, 25, 1536 * 30000 + 65536 * 25 + 2**28), (40000, 20, 1536 * 40000 + 65536 * 20 + 2**28)],
+    [(10000, 5, 1e8), (10000, 5, 1e8), (30000, 25, 3e8), (40000, 20, 4e8)],
+    [(10000, 5, 1e8), (10000, 5, 2e8), (30000, 25, 3e8), (40000, 20, 4e8)],
+    [(1000, 5, 7.5e6), (2000, 10, 1.1e7), (3000, 25, 2.9e7), (4000, 20, 2.4e7)],
+]
 
 
 def run(*args):
@@ -56,6 +67,12 @@ def main():
     for n, m, i in DIVIDES:
         s, o = run("divide", str(n), str(m), str(i)).split()
         out["divides"].append([n, m, i, int(s), int(o)])
+    out["gauss"] = []
+    for sys_ in GAUSS:
+        args = [repr(float(x)) for row in sys_ for x in row]
+        r = run("gauss", *args).split()
+        out["gauss"].append({"samples": [list(map(float, row)) for row in sys_], "code": int(r[0]),
+                             "coeff": [float(x) for x in r[1:]]})
     with open(os.path.join(ROOT, "tests", "golden", "config_parse.json"), "w") as f:
         json.dump(out, f, indent=1)
 

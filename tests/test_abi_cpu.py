@@ -127,6 +127,26 @@ def test_split_task_model_and_findmaxfactor():
         H.split_task([1.0, 1.0, 1.0, 0.0], 1000, 50, 10, mem_bound=10**9, available=10**12)  # nq < 125
 
 
+def test_fit_mem_coeff_matches_reference_gauss():
+    """hdg_fit_mem_coeff against the reference's own gauss (compiled from math_utils.h) on calc_mem_coeff's
+    4x5 system: return code and solution bit for bit, singular systems included."""
+    assert len(CFG["gauss"]) >= 5
+    for g in CFG["gauss"]:
+        code, coeff = H.fit_mem_coeff(g["samples"])
+        assert code == g["code"]
+        assert coeff == g["coeff"], (coeff, g["coeff"])
+
+
+def test_fit_mem_coeff_recovers_affine_footprint():
+    """An exactly affine footprint c0*d + c1*q + c2*d*q + c3 is recovered (up to round-off)."""
+    c = [1536.0, 65536.0, 12.0, 2.0 ** 28]
+    samples = [(d, q, c[0] * d + c[1] * q + c[2] * d * q + c[3]) for d, q in
+               [(10000, 5), (20000, 10), (30000, 25), (40000, 20)]]
+    code, got = H.fit_mem_coeff(samples)
+    assert code == 0
+    np.testing.assert_allclose(got, c, rtol=1e-9)
+
+
 def test_cli_usage_and_bad_algorithm():
     assert os.path.exists(CLI)
     r = subprocess.run([CLI], capture_output=True, text=True)

@@ -282,6 +282,39 @@ def test_bruteforce_pads_when_few_matches():
     assert list(oi[0][:2]) == [1, 2] and oi[0][2] == np.uint64(2**64 - 1) and np.isinf(od[0][2])
 
 
+# ---- top-k merge across data batches -------------------------------------------------------------------
+def test_merge_of_batch_topk_equals_global_topk():
+    """Merging the exact filtered top-k of every data batch gives the exact filtered top-k of the whole data:
+    a wrong offset, a dropped batch or a reversed order fails this."""
+    rng = np.random.default_rng(11)
+    N, d, nq, k, db = 5000, 12, 16, 20, 1300
+    data = rng.standard_normal((N, d)).astype(np.float32)
+    labels = rng.integers(0, 100, size=(N, 1)).astype(np.float32)
+    queries = rng.standard_normal((nq, d)).astype(np.float32)
+    ranges = np.tile(np.array([[5, 70]], np.float32), (nq, 1))
+    gd, gi = O.bruteforce(data, labels, queries, ranges, k)
+    parts_d, parts_i = [], []
+    for b0 in range(0, N, db):
+        bd, bi = O.bruteforce(data[b0:b0 + db], labels[b0:b0 + db], queries, ranges, k)
+        parts_d.append(bd)
+        parts_i.append(bi)
+    md, mi = O.merge_topk(np.stack(parts_d), np.stack(parts_i), db, N, 0, k)
+    np.testing.assert_array_equal(mi, gi)
+    np.testing.assert_array_equal(md, gd)
+
+
+def test_merge_integer_offsets_keep_pads():
+    """Ids above 2^24 stay exact (the reference's float offset would round them) and pads pass through."""
+    big = 1 << 25
+    dist = np.array([[[1.0, 3.0, 1e6]], [[2.0, 1e6, 1e6]]], np.float32)       # 2 batches, 1 query, k = 3
+    ids = np.array([[[7, 5, 2**32]], [[big - 1, 2**64 - 1, 2**32]]], np.uint64)
+    md, mi = O.merge_topk(dist, ids, big, 2 * big, 0, 3)
+    assert list(mi[0]) == [7, 2 * big - 1, 5]
+    assert list(md[0]) == [1.0, 2.0, 3.0]
+    _, mi = O.merge_topk(dist, ids, big, 2 * big, 0, 5)
+    assert list(mi[0][3:]) == [2**32, 2**64 - 1]           # pads keep their values, ties by position
+
+
 # ---- math helpers against the reference's own compiled host code -----------------------------------------
 def test_find_max_factor_matches_reference():
     for a, b, expect in CFG["factors"]:
