@@ -101,6 +101,28 @@ def batched_leg(args, H, datasets, O, dev, log):
             f"ADC scan {adc_bytes / 1e9:.2f} GB in {adc_ms:.3f} ms = {gbs:.0f} GB/s")
         if prec == 0:
             best = (adc_ms, adc_bytes, gbs)
+    # batched form: launch knobs autotuned (results unchanged), then each query chunk replayed as a captured
+    # HIP graph with fixed-capacity candidate buffers
+    idx.set_param("PRECISION", 1)
+    tune = idx.autotune(q, ql, reps=2)
+    log("batched autotune: " + tune.strip().splitlines()[-1])
+    idx.set_param("GRAPH", 1)
+    nq = w.queries.shape[0]
+    od = torch.empty((nq, 100), dtype=torch.float32, device=dev)
+    oi = torch.empty((nq, 100), dtype=torch.int64, device=dev)
+    for _ in range(2):
+        idx.query(q, ql, od, oi)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(3):
+        idx.query(q, ql, od, oi)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t1) / 3
+    idx.set_param("GRAPH", 0)
+    out["mixed_graph"] = {"qps": round(nq / el, 1), "ms_per_call": round(el * 1e3, 3),
+                          "tuned": {k: idx.get_param(k) for k in ("ADC_PER", "ADC_WGS", "SPARSE_MAX_TESTS",
+                                                                   "QUERY_CHUNK", "SCORE_BLOCK")}}
+    log(f"batched mixed+graph: {el * 1e3:.2f} ms/call ({nq / el:.0f} QPS)")
     adc_ms, adc_bytes, gbs = best
     out["adc_roofline"] = {"kernel": "adc_scan_kernel (plain FP32 LUT)", "bound": "hbm", "achieved": round(gbs, 3),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 6),
@@ -175,22 +197,31 @@ def main():
         torch.cuda.synchronize()
         return datasets.recall(gt_ids, i[:gs].cpu().numpy().astype(np.uint64), 100)
 
-    def timed(label):
+    od = torch.empty((args.nq, 100), dtype=torch.float32, device=dev)
+    oi = torch.empty((args.nq, 100), dtype=torch.int64, device=dev)
+
+    def timed(label, graph=0):
+        """K timed calls of hdg_query over all queries; GRAPH 0 also collects the per-stage HIP-event times
+        (GRAPH 1 replays captured graphs, which carry no stage events)."""
+        idx.set_param("GRAPH", graph)
         for _ in range(args.warmup):
-            idx.query(q_d, ql_d)
+            idx.query(q_d, ql_d, od, oi)
         torch.cuda.synchronize()
         stage_sum = {}
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            idx.query(q_d, ql_d)
-            for k, v in idx.last_stage_ms().items():
-                stage_sum[k] = stage_sum.get(k, 0.0) + v
+            idx.query(q_d, ql_d, od, oi)
+            if not graph:
+                for k, v in idx.last_stage_ms().items():
+                    stage_sum[k] = stage_sum.get(k, 0.0) + v
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         stage = {k: v / args.steps for k, v in stage_sum.items()}
-        log(f"{label}: {el * 1e3 / args.steps:.3f} ms/step, stages: " +
-            ", ".join(f"{k}={v:.3f}" for k, v in sorted(stage.items(), key=lambda kv: -kv[1])))
-        return args.nq * args.steps / el, el * 1e3 / args.steps, stage, idx.last_stats(args.nq)
+        log(f"{label}: {el * 1e3 / args.steps:.3f} ms/step" + (", stages: " + ", ".join(
+            f"{k}={v:.3f}" for k, v in sorted(stage.items(), key=lambda kv: -kv[1])) if stage else ""))
+        st = idx.last_stats(args.nq)
+        idx.set_param("GRAPH", 0)
+        return args.nq * args.steps / el, el * 1e3 / args.steps, stage, st
 
     def sweep(start, factor, cap=4096):
         b = start
@@ -217,6 +248,8 @@ def main():
         b, r = sweep(64, 2 ** 0.5)
         qps, ms, stage, st = timed("mixed")
         res["mixed"] = dict(qps=qps, ms=ms, recall=r, sub_lists=b, stage=stage, st=st)
+        qps, ms, _, _ = timed("mixed+graph", graph=1)
+        res["mixed_graph"] = dict(qps=qps, ms=ms, recall=r, sub_lists=b, stage=stage, st=st)
     # ---- learned budget + mixed ----
     if "learned" in paths:
         cap = max(res["mixed"]["sub_lists"] * 2, 256)
@@ -240,8 +273,17 @@ def main():
         idx.set_param("BUDGET_MODEL", 1)
         r = recall_now()
         log(f"  learned budget: recall@100={r:.4f}")
-        qps, ms, stage, st = timed("learned+mixed")
-        res["learned"] = dict(qps=qps, ms=ms, recall=r, sub_lists=cap, stage=stage, st=st, report=rep)
+        _, _, stage, st = timed("learned+mixed (stage times)")
+        # batched form: launch knobs autotuned on a separate sample of the same distribution (results are
+        # unchanged by them), then every query chunk replayed as one captured HIP graph
+        aq, aql = w.sample_queries(args.nq, 0xA7)
+        tune = idx.autotune(torch.from_numpy(aq).to(dev), torch.from_numpy(aql).to(dev), reps=3)
+        log("autotune: " + tune.strip().splitlines()[-1])
+        qps, ms, _, _ = timed("learned+mixed+graph", graph=1)
+        tuned = {k: idx.get_param(k) for k in ("ADC_PER", "ADC_WGS", "SPARSE_MAX_TESTS", "QUERY_CHUNK",
+                                               "SCORE_BLOCK", "GRAPH")}
+        res["learned"] = dict(qps=qps, ms=ms, recall=r, sub_lists=cap, stage=stage, st=st, report=rep,
+                              tuned=tuned)
         idx.set_param("BUDGET_MODEL", 0)
 
     head = "learned" if "learned" in res else ("mixed" if "mixed" in res else "plain")
@@ -280,12 +322,15 @@ def main():
 
     def summary(k):
         v = res[k]
-        return {"qps": round(v["qps"], 1), "ms_per_step": round(v["ms"], 4), "recall@100": round(v["recall"], 4),
-                "sub_lists": v["sub_lists"]}
+        out = {"qps": round(v["qps"], 1), "ms_per_step": round(v["ms"], 4), "recall@100": round(v["recall"], 4),
+               "sub_lists": v["sub_lists"]}
+        if "tuned" in v:
+            out["tuned"] = v["tuned"]
+        return out
 
     config = {"workload": f"{w.name}: {args.n} x {d_} f32 L2-normalised 512-component mixture, 3 label dims "
                           f"(ts shift [604800,0], len shift [30,30], cat map), {args.nq} queries",
-              "path": {"learned": "learned budget + BF16/FP16 mixed + FP32 re-rank",
+              "path": {"learned": "learned budget + BF16/FP16 mixed + FP32 re-rank, autotuned, graph-replayed",
                        "mixed": "BF16/FP16 mixed + FP32 re-rank, fixed budget",
                        "plain": "plain fp32 (reference-exact)"}[head],
               "topk": 100, "n_list": 512, "clusters": C, "sub_clusters": 10, "pq_dim": M,

@@ -295,7 +295,7 @@ class SuperGridIndex:
         return out
 
     def autotune(self, queries, query_labels, reps: int = 3, stream=None) -> str:
-        """Tune ADC_PER, ADC_WGS, SPARSE_MAX_TESTS, QUERY_CHUNK and GRAPH on this batch (results unchanged
+        """Tune ADC_PER, ADC_WGS, SPARSE_MAX_TESTS, QUERY_CHUNK, SCORE_BLOCK and GRAPH on this batch (results unchanged
         bit for bit); the chosen values stay set.  Returns the trial log."""
         import torch
         queries = queries.contiguous()

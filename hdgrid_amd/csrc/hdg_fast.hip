@@ -12,6 +12,7 @@
 
 #include <cfloat>
 
+#include "hdg_common.hpp"
 #include "hdg_device.hpp"
 #include "hdg_fast.hpp"
 
@@ -43,21 +44,34 @@ __global__ void to_bf16_kernel(const float* __restrict__ queries, uint64_t q0, u
 // supplies A[l%16][8*(l/16)..+7] and B[8*(l/16)..+7][l%16], receives D[4*(l/16)..+3][l%16]).  K advances
 // in steps of 32 through LDS tiles of both operands (rows padded to 40 bf16); the next step's tiles are
 // loaded into registers while the current step's MFMAs run.
+//
+// Tile order: workgroup w runs on XCD w % 8 as that XCD's (w / 8)-th workgroup.  w is mapped to a position
+// p = (w % 8) * per + w / 8 of a grouped order that takes kGemmGroup query tiles for each center tile, so the
+// workgroups resident on one XCD at a time cover a few query tiles x a dozen center tiles and find their
+// operand tiles in that XCD's L2 (one 128 x 128 tile reads 192 KiB of operands).
 constexpr int kGemmTile = 128;
 constexpr int kGemmLd = 40;
+constexpr uint32_t kGemmGroup = 4;
+constexpr uint32_t kXcds = 8;
 
 __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __restrict__ qb16, uint64_t qn, uint64_t Dp,
                                                           const uint16_t* __restrict__ cb16,
                                                           const float* __restrict__ cnorm, uint32_t C,
-                                                          float* __restrict__ S) {
+                                                          float* __restrict__ S, uint32_t QT, uint32_t CT,
+                                                          uint32_t per) {
+  const uint32_t p = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  if (p >= QT * CT) return;
+  const uint32_t span = kGemmGroup * CT, first = (p / span) * kGemmGroup;
+  const uint32_t gsize = QT - first < kGemmGroup ? QT - first : kGemmGroup;
+  const uint32_t inner = p % span;
   // operand tiles, reused by the epilogue as 4 waves x 16 rows x 65 floats
   __shared__ __align__(16) uint16_t smem[2 * kGemmTile * kGemmLd];
   uint16_t* As = smem;
   uint16_t* Bs = smem + kGemmTile * kGemmLd;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = wave >> 1, wn = wave & 1;
-  const uint64_t qbase = (uint64_t)blockIdx.x * kGemmTile;
-  const uint64_t cbase = (uint64_t)blockIdx.y * kGemmTile;
+  const uint64_t qbase = (uint64_t)(first + inner % gsize) * kGemmTile;
+  const uint64_t cbase = (uint64_t)(inner / gsize) * kGemmTile;
   // this thread's two 16-byte pieces of each operand tile: row = idx / 4, piece = idx % 4
   uint4 ra[2], rb[2];
   auto load = [&](uint64_t k0) {
@@ -122,6 +136,16 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
   }
 }
 
+// |q|^2 of a d-dim query by one wave: lane partials over t = lane, lane + 64, ... in order, then a butterfly
+// over the 64 lanes; every kernel that needs the value calls this, so they agree bit for bit.
+__device__ __forceinline__ float wave_sq_norm(const float* q, uint64_t d, int lane) {
+  float p = 0.0f;
+  for (uint64_t t = (uint64_t)lane; t < d; t += 64) p += q[t] * q[t];
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) p += __shfl_xor(p, m, 64);
+  return __shfl(p, 0, 64);
+}
+
 // order-preserving u32 key of a signed f32
 __device__ __forceinline__ uint32_t ord(float v) {
   uint32_t u = __float_as_uint(v);
@@ -140,13 +164,15 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
                                                             uint32_t out_ld, uint32_t* __restrict__ chosen,
                                                             uint32_t* __restrict__ n_chosen,
                                                             float* __restrict__ feat_dist,
-                                                            uint64_t* __restrict__ scratch, uint32_t scap) {
+                                                            uint64_t* __restrict__ scratch, uint32_t scap,
+                                                            int64_t skip_upto) {
   extern __shared__ __align__(16) uint64_t keys[];  // scap keys
   __shared__ uint32_t hist[2048];
   __shared__ uint64_t sh[8];
-  __shared__ float red[kWG];
+  __shared__ float qq_sh;
   const uint64_t qi = blockIdx.x;
   const uint32_t n = n_list_q[qi];
+  if (n <= skip_upto) return;  // handled by scored_select_wave_kernel
   const uint32_t* lst = list + qi * cap;
   const float* row = S + qi * ld;
   uint32_t k = k_default;
@@ -170,15 +196,9 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
   for (uint32_t j = threadIdx.x; j < m; j += kWG) chosen[qi * out_ld + j] = (uint32_t)keys[j];
   if (threadIdx.x == 0) n_chosen[qi] = m;
   if (feat_dist) {
-    const float* q = queries + (q0 + qi) * d;
-    float part = 0.0f;
-    for (uint64_t t = threadIdx.x; t < d; t += kWG) part += q[t] * q[t];
-    red[threadIdx.x] = part;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      float qq = 0.0f;
-      for (int i = 0; i < kWG; ++i) qq += red[i];
-      red[0] = qq;
+    if (threadIdx.x < 64) {
+      const float qq = wave_sq_norm(queries + (q0 + qi) * d, d, threadIdx.x);
+      if (threadIdx.x == 0) qq_sh = qq;
     }
     __syncthreads();
     const uint32_t avail = n < 32 ? n : 32;
@@ -187,9 +207,140 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
       if (avail > 0) {
         const uint32_t o = (uint32_t)(keys[j < avail ? j : avail - 1] >> 32);
         const uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
-        v = __uint_as_float(u) + red[0];
+        v = __uint_as_float(u) + qq_sh;
       }
       feat_dist[qi * kFeatDists + j] = v;
+    }
+  }
+}
+
+// Wave-per-query variant of scored_select_kernel for lists of at most 64 * R items: the keys stay in
+// registers, element e = lane * R + r (consecutive elements in one lane), and a bitonic network sorts them
+// with register swaps for partners in the same lane (j < R) and shuffles for partners in other lanes.  No LDS,
+// no barriers.  It sorts the same keys as the workgroup kernel, so the outputs are identical.  Lists longer
+// than kWaveSelMax items are left to the workgroup kernel.
+constexpr uint32_t kWaveSelMax = 512;
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
+  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+template <int R>
+__device__ __forceinline__ void wave_bitonic(uint64_t (&key)[R], int lane) {
+  constexpr uint32_t N = 64u * R;
+#pragma unroll
+  for (uint32_t k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j < (uint32_t)R) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int p = r ^ (int)j;
+          if (p > r) {
+            const uint32_t e = (uint32_t)lane * R + r;
+            const bool up = (e & k) == 0;
+            const uint64_t a = key[r], b = key[p];
+            if ((a > b) == up) {
+              key[r] = b;
+              key[p] = a;
+            }
+          }
+        }
+      } else {
+        const int lm = (int)(j / R);
+        const bool low = (lane & lm) == 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint32_t e = (uint32_t)lane * R + r;
+          const uint64_t o = shfl_xor_u64(key[r], lm);
+          const bool up = (e & k) == 0;
+          // the lower element of the pair keeps the smaller key when ascending
+          const uint64_t mn = key[r] < o ? key[r] : o, mx = key[r] < o ? o : key[r];
+          key[r] = (low == up) ? mn : mx;
+        }
+      }
+    }
+  }
+}
+
+// sorted key number j (< 64 * R) of a wave-sorted array, broadcast from its lane
+template <int R>
+__device__ __forceinline__ uint64_t wave_key_at(const uint64_t (&key)[R], uint32_t j) {
+  uint64_t v = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint64_t x = shfl_u64(key[r], (int)(j / R));
+    if ((int)(j % R) == r) v = x;
+  }
+  return v;
+}
+
+template <int R>
+__device__ void wave_select(const uint32_t* lst, const float* row, uint32_t n, uint32_t m, int lane,
+                            uint32_t* out, uint32_t avail, uint64_t& feat_key) {
+  uint64_t key[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)lane * R + r;
+    key[r] = e < n ? (((uint64_t)ord(row[lst[e]]) << 32) | e) : ~0ull;
+  }
+  wave_bitonic<R>(key, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)lane * R + r;
+    if (e < m) out[e] = (uint32_t)key[r];
+  }
+  if (avail > 0) {
+    // lane j < 32 takes sorted key min(j, avail - 1)
+    const uint32_t j = (uint32_t)lane < avail ? (uint32_t)lane : avail - 1;
+    feat_key = wave_key_at<R>(key, j & 63u);
+  }
+}
+
+__global__ __launch_bounds__(256) void scored_select_wave_kernel(
+    const float* __restrict__ queries, uint64_t q0, uint64_t qn, uint64_t d, const uint32_t* __restrict__ list,
+    uint32_t cap, const uint32_t* __restrict__ n_list_q, const float* __restrict__ S, uint64_t ld,
+    uint32_t k_default, const uint32_t* __restrict__ budget, uint32_t out_ld, uint32_t* __restrict__ chosen,
+    uint32_t* __restrict__ n_chosen, float* __restrict__ feat_dist) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (qi >= qn) return;
+  const uint32_t n = n_list_q[qi];
+  if (n > kWaveSelMax) return;
+  const uint32_t* lst = list + qi * cap;
+  const float* row = S + qi * ld;
+  uint32_t k = k_default;
+  if (budget) {
+    const uint32_t b = budget[qi];
+    if (b != 0 && b < k) k = b;
+  }
+  const uint32_t m = n < k ? n : k;
+  uint32_t* out = chosen + qi * out_ld;
+  const uint32_t avail = feat_dist ? (n < 32 ? n : 32) : 0;
+  uint64_t fk = 0;
+  if (n <= 64) wave_select<1>(lst, row, n, m, lane, out, avail, fk);
+  else if (n <= 128) wave_select<2>(lst, row, n, m, lane, out, avail, fk);
+  else if (n <= 256) wave_select<4>(lst, row, n, m, lane, out, avail, fk);
+  else wave_select<8>(lst, row, n, m, lane, out, avail, fk);
+  if (lane == 0) n_chosen[qi] = m;
+  if (feat_dist) {
+    const float qq = wave_sq_norm(queries + (q0 + qi) * d, d, lane);
+    if (lane < kFeatDists) {
+      float v = 0.0f;
+      if (avail > 0) {
+        const uint32_t o = (uint32_t)(fk >> 32);
+        const uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+        v = __uint_as_float(u) + qq;
+      }
+      feat_dist[qi * kFeatDists + lane] = v;
     }
   }
 }
@@ -199,11 +350,18 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
 void launch_scored_select(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, const uint32_t* list,
                           uint32_t cap, const uint32_t* n_list_q, const float* S, uint64_t ld, uint32_t k_default,
                           const uint32_t* budget, uint32_t out_ld, uint32_t* chosen, uint32_t* n_chosen,
-                          float* feat_dist, uint64_t* scratch, hipStream_t s) {
+                          float* feat_dist, uint64_t* scratch, bool wave, hipStream_t s) {
   uint32_t scap = 1024;
   while (scap < k_default) scap <<= 1;
+  int64_t skip_upto = -1;
+  if (wave) {
+    hipLaunchKernelGGL(scored_select_wave_kernel, dim3((unsigned)((qn + 3) / 4)), dim3(256), 0, s, queries, q0, qn,
+                       d, list, cap, n_list_q, S, ld, k_default, budget, out_ld, chosen, n_chosen, feat_dist);
+    skip_upto = kWaveSelMax;
+  }
   hipLaunchKernelGGL(scored_select_kernel, dim3((unsigned)qn), dim3(256), scap * 8, s, queries, q0, d, list, cap,
-                     n_list_q, S, ld, k_default, budget, out_ld, chosen, n_chosen, feat_dist, scratch, scap);
+                     n_list_q, S, ld, k_default, budget, out_ld, chosen, n_chosen, feat_dist, scratch, scap,
+                     skip_upto);
 }
 
 void launch_to_bf16(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, uint64_t Dp, uint16_t* out,
@@ -214,8 +372,11 @@ void launch_to_bf16(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, 
 
 void launch_coarse_gemm(const uint16_t* qb16, uint64_t qn, uint64_t Dp, const uint16_t* cb16, const float* cnorm,
                         uint32_t C, float* S, hipStream_t s) {
-  dim3 grid((unsigned)((qn + kGemmTile - 1) / kGemmTile), (unsigned)((C + kGemmTile - 1) / kGemmTile));
-  hipLaunchKernelGGL(coarse_gemm_kernel, grid, dim3(256), 0, s, qb16, qn, Dp, cb16, cnorm, C, S);
+  const uint64_t QT = (qn + kGemmTile - 1) / kGemmTile, CT = (C + kGemmTile - 1) / kGemmTile;
+  if (QT * CT > 0xFFFFFFFFull / 2) fail(kUnsupported, "coarse GEMM grid too large");
+  const uint32_t per = (uint32_t)((QT * CT + kXcds - 1) / kXcds);
+  hipLaunchKernelGGL(coarse_gemm_kernel, dim3(per * kXcds), dim3(256), 0, s, qb16, qn, Dp, cb16, cnorm, C, S,
+                     (uint32_t)QT, (uint32_t)CT, per);
 }
 
 }  // namespace hdg

@@ -19,9 +19,10 @@ void launch_coarse_gemm(const uint16_t* qb16, uint64_t qn, uint64_t Dp, const ui
 // Per query: top-k (k = budget[q] or k_default) of the listed items list[q][0..n_list_q[q]) by S[q][item],
 // ascending; writes the chosen list positions chosen[q][0..m) (row stride out_ld) and n_chosen[q];
 // optional features.
-// scratch: [qn][cap] u64, used when a list exceeds the LDS sort capacity.
+// scratch: [qn][cap] u64, used when a list exceeds the LDS sort capacity.  wave: lists of at most 512 items
+// are sorted by one wave per query in registers (same result as the workgroup sort).
 void launch_scored_select(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, const uint32_t* list,
                           uint32_t cap, const uint32_t* n_list_q, const float* S, uint64_t ld, uint32_t k_default,
                           const uint32_t* budget, uint32_t out_ld, uint32_t* chosen, uint32_t* n_chosen,
-                          float* feat_dist, uint64_t* scratch, hipStream_t s);
+                          float* feat_dist, uint64_t* scratch, bool wave, hipStream_t s);
 }  // namespace hdg

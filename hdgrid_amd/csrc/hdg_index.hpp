@@ -120,8 +120,8 @@ class Index {
   void budget_load(const std::string& path, hipStream_t s);
   void budget_free();
   void debug_coarse_scores(const float* queries, uint64_t nq, float* out_host, hipStream_t s);
-  // Coordinate descent over the result-invariant launch knobs (ADC_PER, ADC_WGS, SPARSE_MAX_TESTS,
-  // QUERY_CHUNK, GRAPH) timed on these queries (hdg_autotune.cpp); keeps the fastest setting whose
+  // Coordinate descent over the launch knobs (ADC_PER, ADC_WGS, SPARSE_MAX_TESTS, QUERY_CHUNK,
+  // SCORE_BLOCK, GRAPH) timed on these queries (hdg_autotune.cpp); keeps the fastest setting whose
   // results equal the starting one's bit for bit and returns the trial log.
   std::string autotune(const float* queries, uint64_t nq, uint64_t d, const float* qlabels, uint64_t qcols,
                        float* out_dist, uint64_t* out_ids, uint32_t reps, hipStream_t s);

@@ -194,7 +194,8 @@ __global__ void prep_kernel(const float* __restrict__ qlabels, uint64_t qcols, u
 // ascending table order with their counts, and the non-zero clusters (filter.cu:242-260 over
 // get_cluster_sum_kernel's sums) fall out as the runs of cluster_of_sub.  A query whose box spans more
 // than kMaxCombos bucket rows or max_tests entries is handed to the dense count instead.
-// One wave per workgroup, persistent over queries; LDS = hist[CS] + bit set[CS/32].
+// One 256-thread workgroup per query (persistent over queries): the four waves share the query's walk and
+// one LDS histogram.  LDS = hist[CS] + bit set[CS/32] + word prefixes[CS/32].
 struct Buckets {
   uint32_t M[kMaxDims], nb[kMaxDims];
   uint64_t stride[kMaxDims];
@@ -202,21 +203,12 @@ struct Buckets {
 
 constexpr uint64_t kMaxCombos = 4096;
 constexpr uint32_t kInboxCap = 4096;  // in-box points listed per walked query
-constexpr uint32_t kSparseMaxCS = 8192;  // LDS histogram: (CS + CS/32) words per wave
-
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  return x;
-}
+constexpr uint32_t kSparseMaxCS = 8192;  // LDS histogram: (CS + 2 CS/32) words per workgroup
 
 template <int NV>
-__global__ __launch_bounds__(64) void sparse_count_kernel(
+__global__ __launch_bounds__(256) void sparse_count_kernel(
     Buckets B, const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ ecell,
-    const uint32_t* __restrict__ esub, uint64_t N, const uint32_t* __restrict__ cluster_of_sub, uint32_t CS,
+    const uint32_t* __restrict__ esub, uint64_t N, const uint32_t* __restrict__ sub_off, uint32_t CS,
     uint32_t C, const uint32_t* __restrict__ box_lo, const uint32_t* __restrict__ box_w, uint64_t qn, uint64_t G,
     uint64_t q0, uint32_t max_tests, const uint32_t* __restrict__ epos, uint32_t ibcap,
     uint32_t* __restrict__ ib_pos, uint32_t* __restrict__ ib_sub, uint32_t* __restrict__ n_ib,
@@ -225,13 +217,14 @@ __global__ __launch_bounds__(64) void sparse_count_kernel(
     uint32_t* __restrict__ nnz_c, uint32_t* __restrict__ dense_list, uint32_t* __restrict__ n_dense,
     unsigned long long* __restrict__ gstat) {
   extern __shared__ __align__(16) uint32_t sm[];
-  uint32_t* hist = sm;       // [CS]
-  uint32_t* bits = sm + CS;  // [nw]
-  __shared__ uint32_t pre_sh[64], e0_sh[64];
   const uint32_t nw = (CS + 31) >> 5;
-  const int lane = threadIdx.x;
-  for (uint32_t i = lane; i < CS; i += 64) hist[i] = 0;
-  for (uint32_t i = lane; i < nw; i += 64) bits[i] = 0;
+  uint32_t* hist = sm;            // [CS]
+  uint32_t* bits = sm + CS;       // [nw]
+  uint32_t* wpre = bits + nw;     // [nw] rank of each word's first set bit among the non-zero sub-clusters
+  __shared__ uint32_t pre_sh[kWG], e0_sh[kWG], tmp[kWG / 64 + 1];
+  const int tid = threadIdx.x;
+  for (uint32_t i = tid; i < CS; i += kWG) hist[i] = 0;
+  for (uint32_t i = tid; i < nw; i += kWG) bits[i] = 0;
   __syncthreads();
   for (uint64_t qi = blockIdx.x; qi < qn; qi += gridDim.x) {
     uint32_t lo[NV], w[NV], blo[NV], bhi[NV];
@@ -251,11 +244,11 @@ __global__ __launch_bounds__(64) void sparse_count_kernel(
 #pragma unroll
     for (int j = 0; j + 1 < NV; ++j) combos *= (uint64_t)(bhi[j] - blo[j] + 1);
     bool to_dense = !empty && combos > kMaxCombos;
-    uint32_t ibn = 0;  // in-box points listed so far (in walk order)
+    uint32_t ibn = 0;  // in-box points listed so far (order: pass, thread, entry of the thread)
     if (!empty && !to_dense) {
       uint64_t tests = 0;
-      for (uint64_t c0 = 0; c0 < combos; c0 += 64) {
-        const uint64_t c = c0 + (uint64_t)lane;
+      for (uint64_t c0 = 0; c0 < combos; c0 += kWG) {
+        const uint64_t c = c0 + (uint64_t)tid;
         uint32_t e0 = 0, len = 0;
         if (c < combos) {
           uint64_t rem = c, base = 0;
@@ -268,23 +261,23 @@ __global__ __launch_bounds__(64) void sparse_count_kernel(
           e0 = bstart[base + blo[NV - 1]];
           len = bstart[base + bhi[NV - 1] + 1] - e0;
         }
-        const uint32_t incl = wave_incl_scan(len, lane);
-        const uint32_t total = __shfl(incl, 63, 64);
-        pre_sh[lane] = incl - len;
-        e0_sh[lane] = e0;
+        uint32_t total;
+        const uint32_t excl = dev::wg_exclusive_scan(len, tmp, total);
+        pre_sh[tid] = excl;
+        e0_sh[tid] = e0;
         __syncthreads();
         tests += total;
         if (tests > max_tests) {
           to_dense = true;
           break;
         }
-        // four entries per lane per pass, so their loads are in flight together
-        for (uint32_t t0 = 0; t0 < total; t0 += 256) {
+        // four entries per thread per pass, so their loads are in flight together
+        for (uint32_t t0 = 0; t0 < total; t0 += 4 * kWG) {
           uint32_t ev[4], cv[4][NV], sv[4], pv[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            const uint32_t t = t0 + (uint32_t)lane + 64u * u;
-            uint32_t a = 0, b = 64;  // last k with pre_sh[k] <= t; that lane's range holds t
+            const uint32_t t = t0 + (uint32_t)tid + (uint32_t)(kWG * u);
+            uint32_t a = 0, b = kWG;  // last k with pre_sh[k] <= t; that thread's range holds t
             while (b - a > 1) {
               const uint32_t m = (a + b) >> 1;
               if (pre_sh[m] <= t) a = m; else b = m;
@@ -299,30 +292,37 @@ __global__ __launch_bounds__(64) void sparse_count_kernel(
             sv[u] = ok ? esub[ev[u]] : 0u;
             pv[u] = ok ? epos[ev[u]] : 0u;
           }
+          bool in[4];
+          uint32_t nin = 0;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            bool in = ev[u] != kNone;
+            in[u] = ev[u] != kNone;
 #pragma unroll
-            for (int j = 0; j < NV; ++j) in = in && (cv[u][j] - lo[j] <= w[j]);
-            const uint64_t mask = __ballot(in);
-            const uint32_t at = ibn + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-            ibn += (uint32_t)__popcll(mask);
-            if (in) {
+            for (int j = 0; j < NV; ++j) in[u] = in[u] && (cv[u][j] - lo[j] <= w[j]);
+            nin += in[u] ? 1u : 0u;
+          }
+          uint32_t pass_tot;
+          uint32_t at = ibn + dev::wg_exclusive_scan(nin, tmp, pass_tot);
+          ibn += pass_tot;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (in[u]) {
               const uint32_t s2 = sv[u];
               if (atomicAdd(&hist[s2], 1u) == 0u) atomicOr(&bits[s2 >> 5], 1u << (s2 & 31));
               if (at < ibcap) {
                 ib_pos[qi * ibcap + at] = pv[u];
                 ib_sub[qi * ibcap + at] = s2;
               }
+              ++at;
             }
           }
         }
-        __syncthreads();
+        __syncthreads();  // pre_sh / e0_sh are rewritten by the next batch of bucket rows
       }
     }
     __syncthreads();
     if (to_dense) {
-      for (uint32_t wi = lane; wi < nw; wi += 64) {
+      for (uint32_t wi = tid; wi < nw; wi += kWG) {
         uint32_t word = bits[wi];
         while (word) {
           const uint32_t b = __ffs(word) - 1;
@@ -331,57 +331,67 @@ __global__ __launch_bounds__(64) void sparse_count_kernel(
         }
         bits[wi] = 0;
       }
-      if (lane == 0) {
+      if (tid == 0) {
         dense_list[atomicAdd(n_dense, 1u)] = (uint32_t)qi;
         n_ib[qi] = kNone;
       }
       __syncthreads();
       continue;
     }
-    // emit the non-zero sub-clusters ascending with their counts, then the cluster runs among them
+    // emit the non-zero sub-clusters ascending with their counts
     uint32_t* sl = sub_list + qi * CS;
     uint32_t* sc = sub_cnt + qi * CS;
-    uint32_t* lc = list_c + qi * C;
-    uint32_t* cb = cl_beg + qi * (C + 1);
     uint32_t base = 0;
-    for (uint32_t w0 = 0; w0 < nw; w0 += 64) {
-      const uint32_t wi = w0 + (uint32_t)lane;
-      uint32_t word = wi < nw ? bits[wi] : 0u;
-      const uint32_t pc = __popc(word);
-      const uint32_t incl = wave_incl_scan(pc, lane);
-      const uint32_t tot = __shfl(incl, 63, 64);
-      uint32_t pos = base + incl - pc;
-      while (word) {
-        const uint32_t b = __ffs(word) - 1;
-        word &= word - 1;
+    for (uint32_t w0 = 0; w0 < nw; w0 += kWG) {
+      const uint32_t wi = w0 + (uint32_t)tid;
+      const uint32_t word = wi < nw ? bits[wi] : 0u;
+      uint32_t tot;
+      uint32_t pos = base + dev::wg_exclusive_scan(__popc(word), tmp, tot);
+      if (wi < nw) wpre[wi] = pos;
+      uint32_t rest = word;
+      while (rest) {
+        const uint32_t b = __ffs(rest) - 1;
+        rest &= rest - 1;
         const uint32_t s2 = wi * 32 + b;
         sl[pos] = s2;
         sc[pos] = hist[s2];
         hist[s2] = 0;
         ++pos;
       }
-      if (wi < nw) bits[wi] = 0;
       base += tot;
     }
-    __syncthreads();  // the wave reads its sub-cluster list back
+    __syncthreads();
+    // the non-zero clusters, ascending, each with the rank of its first non-zero sub-cluster (sub-clusters
+    // are numbered cluster-major, so cluster c owns [sub_off[c], sub_off[c+1]))
+    uint32_t* lc = list_c + qi * C;
+    uint32_t* cb = cl_beg + qi * (C + 1);
     uint32_t ncl = 0;
-    int prev_cl = -1;
-    for (uint32_t i0 = 0; i0 < base; i0 += 64) {
-      const uint32_t i = i0 + (uint32_t)lane;
-      const int cl = i < base ? (int)cluster_of_sub[sl[i]] : -2;
-      int pcl = __shfl_up(cl, 1, 64);
-      if (lane == 0) pcl = prev_cl;
-      const bool start = i < base && cl != pcl;
-      const uint64_t mask = __ballot(start);
-      const uint32_t at = ncl + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-      if (start) {
-        lc[at] = (uint32_t)cl;
-        cb[at] = i;
+    for (uint32_t c0 = 0; c0 < C; c0 += kWG) {
+      const uint32_t c = c0 + (uint32_t)tid;
+      uint32_t first = kNone;
+      if (c < C) {
+        const uint32_t a = sub_off[c], e = sub_off[c + 1];
+        for (uint32_t x = a; x < e && first == kNone;) {
+          const uint32_t wi = x >> 5, sh = x & 31;
+          const uint32_t span = e - x < 32 - sh ? e - x : 32 - sh;
+          const uint32_t m = (bits[wi] >> sh) & (span == 32 ? 0xFFFFFFFFu : ((1u << span) - 1u));
+          if (m) first = x + (uint32_t)(__ffs(m) - 1);
+          x += span;
+        }
       }
-      ncl += (uint32_t)__popcll(mask);
-      prev_cl = __shfl(cl, 63, 64);
+      const uint32_t flag = first != kNone ? 1u : 0u;
+      uint32_t tot;
+      const uint32_t at = ncl + dev::wg_exclusive_scan(flag, tmp, tot);
+      if (flag) {
+        const uint32_t wi = first >> 5, sh = first & 31;
+        lc[at] = c;
+        cb[at] = wpre[wi] + (uint32_t)__popc(bits[wi] & ((1u << sh) - 1u));
+      }
+      ncl += tot;
     }
-    if (lane == 0) {
+    __syncthreads();
+    for (uint32_t wi = tid; wi < nw; wi += kWG) bits[wi] = 0;
+    if (tid == 0) {
       ns_out[qi] = base;
       nnz_c[qi] = ncl;
       cb[ncl] = base;
@@ -1884,7 +1894,8 @@ void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* ql
 uint64_t Index::config_digest() const {
   const uint64_t v[] = {cfg_.n_list, cfg_.sub_lists, cfg_.topk, cfg_.exps0, cfg_.precision, cfg_.probe_clusters,
                         cfg_.budget_model, cfg_.sparse_count, cfg_.sparse_max_tests, cfg_.adc_per, cfg_.adc_wgs,
-                        cfg_.parity_batch, cfg_.query_chunk, cfg_.label_mask, graph_epoch_};
+                        cfg_.parity_batch, cfg_.query_chunk, cfg_.label_mask, cfg_.select_wave, cfg_.score_block,
+                        graph_epoch_};
   uint64_t h = 1469598103934665603ull;
   for (uint64_t x : v) {
     h ^= x;
@@ -2016,8 +2027,11 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     budget_T = w.budget_T.as<uint32_t>();
     budget_S = w.budget_S.as<uint32_t>();
   }
+  // PRECISION 1 scores queries x (sub-)centers in blocks of SCORE_BLOCK queries through one score buffer, small
+  // enough to stay in the memory-side cache between the GEMM that writes it and the selection that reads it
+  const uint64_t score_block = std::min<uint64_t>(qn, std::max<uint64_t>(cfg_.score_block, 128));
   if (mixed) {
-    w.S.reserve(qn * std::max<uint64_t>(C, CS) * 4);
+    w.S.reserve(score_block * std::max<uint64_t>(C, CS) * 4);
     w.feat.reserve(qn * kFeatDists * 4);
     w.qb16.reserve(qn * S.Dp * 2);
     launch_to_bf16(queries, q0, qn, d, S.Dp, w.qb16.as<uint16_t>(), s);
@@ -2033,7 +2047,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
       B.stride[j] = S.bstride[j];
     }
     const uint32_t nw = (uint32_t)((CS + 31) / 32);
-    const size_t lds = (CS + nw) * sizeof(uint32_t);
+    const size_t lds = (CS + 2 * nw) * sizeof(uint32_t);
     const uint32_t max_tests = (uint32_t)std::min<uint64_t>(cfg_.sparse_max_tests, 0xFFFFFFF0ull);
     const unsigned grid = (unsigned)std::min<uint64_t>(qn, 2048);
     const uint32_t* cells = S.cells.as<uint32_t>();
@@ -2042,8 +2056,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
 #define HDG_A3_CASE(NV)                                                                                            \
   case NV:                                                                                                         \
     if (sparse) {                                                                                                  \
-      hipLaunchKernelGGL(sparse_count_kernel<NV>, dim3(grid), dim3(64), lds, s, B, S.bstart.as<uint32_t>(),         \
-                         S.ecell.as<uint32_t>(), S.esub.as<uint32_t>(), S.N, S.cluster_of_sub.as<uint32_t>(),      \
+      hipLaunchKernelGGL(sparse_count_kernel<NV>, dim3(grid), dim3(256), lds, s, B, S.bstart.as<uint32_t>(),        \
+                         S.ecell.as<uint32_t>(), S.esub.as<uint32_t>(), S.N, S.sub_off.as<uint32_t>(),             \
                          (uint32_t)CS, (uint32_t)C, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), qn, G, q0,    \
                          max_tests, S.epos.as<uint32_t>(), kInboxCap, w.ib_pos.as<uint32_t>(),           \
                          w.ib_sub.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.sub_list.as<uint32_t>(), w.sub_cnt.as<uint32_t>(), w.ns.as<uint32_t>(),                 \
@@ -2097,11 +2111,16 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                        w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>());
   } else {
     const uint32_t kc = cfg_.probe_clusters > 0 ? std::min<uint32_t>(n_list, (uint32_t)cfg_.probe_clusters) : n_list;
-    launch_coarse_gemm(w.qb16.as<uint16_t>(), qn, S.Dp, S.centers_bf16.as<uint16_t>(), S.center_norm.as<float>(),
-                       (uint32_t)C, w.S.as<float>(), s);
-    launch_scored_select(queries, q0, qn, d, w.list_c.as<uint32_t>(), (uint32_t)C, w.nnz_c.as<uint32_t>(),
-                         w.S.as<float>(), C, kc, nullptr, n_list, w.chosen_c.as<uint32_t>(),
-                         w.n_chosen_c.as<uint32_t>(), w.feat.as<float>(), w.keys.as<uint64_t>(), s);
+    for (uint64_t b0 = 0; b0 < qn; b0 += score_block) {
+      const uint64_t bn = std::min(score_block, qn - b0);
+      launch_coarse_gemm(w.qb16.as<uint16_t>() + b0 * S.Dp, bn, S.Dp, S.centers_bf16.as<uint16_t>(),
+                         S.center_norm.as<float>(), (uint32_t)C, w.S.as<float>(), s);
+      launch_scored_select(queries, q0 + b0, bn, d, w.list_c.as<uint32_t>() + b0 * C, (uint32_t)C,
+                           w.nnz_c.as<uint32_t>() + b0, w.S.as<float>(), C, kc, nullptr, n_list,
+                           w.chosen_c.as<uint32_t>() + b0 * n_list, w.n_chosen_c.as<uint32_t>() + b0,
+                           w.feat.as<float>() + b0 * kFeatDists, w.keys.as<uint64_t>() + b0 * C,
+                           cfg_.select_wave != 0, s);
+    }
     budgets();
   }
   HDG_HIP(hipGetLastError());
@@ -2123,11 +2142,16 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                        w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>(), budget_S, w.keys.as<uint64_t>(),
                        w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>());
   } else {
-    launch_coarse_gemm(w.qb16.as<uint16_t>(), qn, S.Dp, S.sub_centers_bf16.as<uint16_t>(),
-                       S.sub_center_norm.as<float>(), (uint32_t)CS, w.S.as<float>(), s);
-    launch_scored_select(queries, q0, qn, d, w.list_s.as<uint32_t>(), cap, w.nnz_s.as<uint32_t>(), w.S.as<float>(),
-                         CS, sub_lists, budget_S, sub_lists, w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(),
-                         nullptr, w.keys.as<uint64_t>(), s);
+    for (uint64_t b0 = 0; b0 < qn; b0 += score_block) {
+      const uint64_t bn = std::min(score_block, qn - b0);
+      launch_coarse_gemm(w.qb16.as<uint16_t>() + b0 * S.Dp, bn, S.Dp, S.sub_centers_bf16.as<uint16_t>(),
+                         S.sub_center_norm.as<float>(), (uint32_t)CS, w.S.as<float>(), s);
+      launch_scored_select(queries, q0 + b0, bn, d, w.list_s.as<uint32_t>() + b0 * cap, cap,
+                           w.nnz_s.as<uint32_t>() + b0, w.S.as<float>(), CS, sub_lists,
+                           budget_S ? budget_S + b0 : nullptr, sub_lists, w.chosen_s.as<uint32_t>() + b0 * sub_lists,
+                           w.n_chosen_s.as<uint32_t>() + b0, nullptr, w.keys.as<uint64_t>() + b0 * cap,
+                           cfg_.select_wave != 0, s);
+    }
   }
   HDG_HIP(hipGetLastError());
   mark();

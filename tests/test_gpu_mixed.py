@@ -131,6 +131,32 @@ def test_budget_backward_matches_cpu_oracle():
     assert (z1 <= 0).any() and (z1 > 0).any() and (z2 <= 0).any() and (z2 > 0).any()
 
 
+def test_wave_select_matches_workgroup_select():
+    """SELECT_WAVE 1 (one wave per query, register sort, lists <= 512) and SELECT_WAVE 0 (workgroup LDS sort),
+    and scoring in blocks of 300 or 16384 queries, choose the same clusters and sub-clusters: explicit ranges
+    from 1e-4 to 0.5 selectivity give lists on both sides of 512 (768 sub-clusters)."""
+    w = datasets.explicit_ranges(n=60000, nq=800, d=64, components=64, clusters=96, sub_clusters=8)
+    conf = w.conf_text(N_LIST=96, SUB_LISTS=200, PRECISION=1, **FAST)
+    idx = H.SuperGridIndex(conf_text=conf, filter_text=w.filter_text)
+    idx.build(torch.from_numpy(w.data).to(DEV), torch.from_numpy(w.labels).to(DEV))
+    q, ql = torch.from_numpy(w.queries).to(DEV), torch.from_numpy(w.qlabels).to(DEV)
+    out = {}
+    # (SELECT_WAVE, SCORE_BLOCK): scoring blocks of 300 queries leave a ragged last block
+    for key in ((0, 16384), (1, 16384), (1, 300)):
+        idx.set_param("SELECT_WAVE", key[0])
+        idx.set_param("SCORE_BLOCK", key[1])
+        d, i = idx.query(q, ql)
+        torch.cuda.synchronize()
+        out[key] = (d.cpu().numpy(), i.cpu().numpy(), idx.last_chosen_subs(800), idx.last_stats(800))
+    st = out[(1, 300)][3]
+    assert (st.scored > 512).any() and (st.scored < 256).any()
+    ref = out[(0, 16384)]
+    for key in ((1, 16384), (1, 300)):
+        np.testing.assert_array_equal(ref[2], out[key][2])
+        np.testing.assert_array_equal(ref[1], out[key][1])
+        assert np.array_equal(ref[0].view(np.uint32), out[key][0].view(np.uint32))
+
+
 def test_budget_fit_gain(mixed_index):
     """Train on 2000 queries, evaluate on 400 held-out: the learned budget keeps recall near the target
     while probing fewer clusters than the fixed top budget."""
@@ -153,6 +179,11 @@ def test_budget_fit_gain(mixed_index):
     _, i_l = idx.query(qe, qle)
     st_l = idx.last_stats(ne)
     r_l = datasets.recall(gte, i_l.cpu().numpy().astype(np.uint64), 100)
+    # the budget features (32 smallest coarse scores + |q|^2) agree between the two select kernels
+    idx.set_param("SELECT_WAVE", 0)
+    _, i_wg = idx.query(qe, qle)
+    idx.set_param("SELECT_WAVE", 1)
+    np.testing.assert_array_equal(i_wg.cpu().numpy(), i_l.cpu().numpy())
     idx.set_param("BUDGET_MODEL", 0)
     idx.set_param("PRECISION", 0)
     print(f"[budget] fixed recall {r_fix:.4f} scored {st_fix.scored.mean():.1f} cand {st_fix.cand.mean():.1f}; "
