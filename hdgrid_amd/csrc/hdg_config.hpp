@@ -53,7 +53,8 @@ struct Config {
   uint64_t adc_per = 32;               // ADC_PER: candidates per thread of an ADC chunk (8, 16, 32, 64)
   uint64_t adc_wgs = 2048;             // ADC_WGS: persistent ADC workgroups
   uint64_t graph = 0;                  // GRAPH: 1 = replay each query chunk as a captured HIP graph
-  uint64_t select_wave = 1;            // SELECT_WAVE: 1 = one wave per query sorts lists of <= 512 scored items
+  uint64_t select_wave = 1;            // SELECT_WAVE: 1 = one wave per query for scored selection, in-box keep
+                                       // and the final top-k on lists of <= 512 items (same results)
   uint64_t score_block = 65536;        // SCORE_BLOCK: queries per BF16 scoring block (PRECISION 1; min 128)
 };
 

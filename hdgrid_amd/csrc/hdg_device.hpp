@@ -68,6 +68,72 @@ __device__ inline void wg_sort_lds(uint64_t* keys, uint32_t n) {
   }
 }
 
+// 64-bit shuffles (two 32-bit halves)
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
+  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Bitonic sort of 64 * R keys held by one wave in registers, element e = lane * R + r (consecutive elements
+// in one lane): register swaps for partners in the same lane (j < R), shuffles for partners in other lanes.
+// Ascending; no LDS, no barriers.
+template <int R>
+__device__ __forceinline__ void wave_bitonic(uint64_t (&key)[R], int lane) {
+  constexpr uint32_t N = 64u * R;
+#pragma unroll
+  for (uint32_t k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j < (uint32_t)R) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int p = r ^ (int)j;
+          if (p > r) {
+            const uint32_t e = (uint32_t)lane * R + r;
+            const bool up = (e & k) == 0;
+            const uint64_t a = key[r], b = key[p];
+            if ((a > b) == up) {
+              key[r] = b;
+              key[p] = a;
+            }
+          }
+        }
+      } else {
+        const int lm = (int)(j / R);
+        const bool low = (lane & lm) == 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint32_t e = (uint32_t)lane * R + r;
+          const uint64_t o = shfl_xor_u64(key[r], lm);
+          const bool up = (e & k) == 0;
+          // the lower element of the pair keeps the smaller key when ascending
+          const uint64_t mn = key[r] < o ? key[r] : o, mx = key[r] < o ? o : key[r];
+          key[r] = (low == up) ? mn : mx;
+        }
+      }
+    }
+  }
+}
+
+// sorted key number j (< 64 * R) of a wave-sorted array, broadcast from its lane
+template <int R>
+__device__ __forceinline__ uint64_t wave_key_at(const uint64_t (&key)[R], uint32_t j) {
+  uint64_t v = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint64_t x = shfl_u64(key[r], (int)(j / R));
+    if ((int)(j % R) == r) v = x;
+  }
+  return v;
+}
+
 // Radix selection: writes the k smallest of the unique keys g[0..n) (global memory) to out[0..k),
 // unordered.  Requires k < n.  `hist` is 2048 words of LDS, `sh` 8 words of LDS.
 // Key sources for the selections: a plain key array, or (f32 distance bits, index) built from a

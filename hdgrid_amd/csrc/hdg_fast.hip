@@ -58,7 +58,9 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
                                                           const uint16_t* __restrict__ cb16,
                                                           const float* __restrict__ cnorm, uint32_t C,
                                                           float* __restrict__ S, uint32_t QT, uint32_t CT,
-                                                          uint32_t per) {
+                                                          uint32_t per, const uint32_t* __restrict__ nzb,
+                                                          const uint32_t* __restrict__ nzp, uint32_t nw,
+                                                          uint64_t ld) {
   const uint32_t p = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
   if (p >= QT * CT) return;
   const uint32_t span = kGemmGroup * CT, first = (p / span) * kGemmGroup;
@@ -66,6 +68,8 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
   const uint32_t inner = p % span;
   // operand tiles, reused by the epilogue as 4 waves x 16 rows x 65 floats
   __shared__ __align__(16) uint16_t smem[2 * kGemmTile * kGemmLd];
+  // compact rows: the tile's 128 rows x 4 bit words of non-zero items and their ranks
+  __shared__ uint32_t nz_bits[kGemmTile * 4], nz_rank[kGemmTile * 4];
   uint16_t* As = smem;
   uint16_t* Bs = smem + kGemmTile * kGemmLd;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -88,6 +92,20 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+  // the compact-row words are fetched now and parked in LDS after the main loop
+  uint32_t nzv[4] = {0u, 0u, 0u, 0u};
+  if (nzb) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int idx = tid + i * 256, r = idx >> 2, wd = idx & 3;
+      const uint64_t q = qbase + r;
+      const uint64_t wi = (cbase >> 5) + (uint64_t)wd;
+      if (q < qn && wi < nw) {
+        nzv[2 * i] = nzb[q * nw + wi];
+        nzv[2 * i + 1] = nzp[q * nw + wi];
+      }
+    }
+  }
   load(0);
   const int r16 = lane & 15, kg = lane >> 4;
   for (uint64_t k0 = 0; k0 < Dp; k0 += 32) {
@@ -111,6 +129,14 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
       for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     __syncthreads();
   }
+  if (nzb) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int idx = tid + i * 256;
+      nz_bits[idx] = nzv[2 * i];
+      nz_rank[idx] = nzv[2 * i + 1];
+    }
+  }
   // epilogue: each wave passes its 64 x 64 block through LDS 16 rows at a time, so that every store
   // writes 256 contiguous bytes of a row of S
   float* ep = reinterpret_cast<float*>(smem) + wave * 16 * 65;
@@ -128,9 +154,22 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
 #pragma unroll
       for (int t = 0; t < 4; ++t) ep[(kg * 4 + t) * 65 + j * 16 + r16] = cn[j] - 2.0f * acc[i][j][t];
     __syncthreads();
-    for (int r = 0; r < 16; ++r) {
-      const uint64_t q = qbase + wm * 64 + i * 16 + r;
-      if (q < qn && cw < C) S[q * C + cw] = ep[r * 65 + lane];
+    if (!nzb) {
+      for (int r = 0; r < 16; ++r) {
+        const uint64_t q = qbase + wm * 64 + i * 16 + r;
+        if (q < qn && cw < C) S[q * ld + cw] = ep[r * 65 + lane];
+      }
+    } else {
+      // compact rows: only the query's non-zero items, at their rank among them
+      const uint32_t b = (uint32_t)cw & 31u, wd = (uint32_t)(wn * 2) + (uint32_t)(lane >> 5);
+      for (int r = 0; r < 16; ++r) {
+        const uint32_t tr = (uint32_t)(wm * 64 + i * 16 + r);
+        const uint64_t q = qbase + tr;
+        if (q < qn && cw < C) {
+          const uint32_t word = nz_bits[tr * 4 + wd];
+          if ((word >> b) & 1u) S[q * ld + nz_rank[tr * 4 + wd] + __popc(word & ((1u << b) - 1u))] = ep[r * 65 + lane];
+        }
+      }
     }
     __syncthreads();
   }
@@ -144,6 +183,15 @@ __device__ __forceinline__ float wave_sq_norm(const float* q, uint64_t d, int la
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) p += __shfl_xor(p, m, 64);
   return __shfl(p, 0, 64);
+}
+
+// score of listed item c in its query's score row: dense rows hold every item at its index, compact rows
+// (nzb given) only the query's non-zero items, at their rank (word prefix nzp + bits below in the word)
+__device__ __forceinline__ float item_score(const float* row, uint32_t c, const uint32_t* nzb_q,
+                                            const uint32_t* nzp_q) {
+  if (!nzb_q) return row[c];
+  const uint32_t word = nzb_q[c >> 5], b = c & 31u;
+  return row[nzp_q[c >> 5] + __popc(word & ((1u << b) - 1u))];
 }
 
 // order-preserving u32 key of a signed f32
@@ -165,7 +213,8 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
                                                             uint32_t* __restrict__ n_chosen,
                                                             float* __restrict__ feat_dist,
                                                             uint64_t* __restrict__ scratch, uint32_t scap,
-                                                            int64_t skip_upto) {
+                                                            int64_t skip_upto, const uint32_t* __restrict__ nzb,
+                                                            const uint32_t* __restrict__ nzp, uint32_t nw) {
   extern __shared__ __align__(16) uint64_t keys[];  // scap keys
   __shared__ uint32_t hist[2048];
   __shared__ uint64_t sh[8];
@@ -175,6 +224,8 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
   if (n <= skip_upto) return;  // handled by scored_select_wave_kernel
   const uint32_t* lst = list + qi * cap;
   const float* row = S + qi * ld;
+  const uint32_t* nzb_q = nzb ? nzb + qi * nw : nullptr;
+  const uint32_t* nzp_q = nzb ? nzp + qi * nw : nullptr;
   uint32_t k = k_default;
   if (budget) {
     const uint32_t b = budget[qi];
@@ -182,12 +233,14 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
   }
   uint32_t m = n < k ? n : k;
   if (n <= scap) {
-    for (uint32_t j = threadIdx.x; j < n; j += kWG) keys[j] = ((uint64_t)ord(row[lst[j]]) << 32) | j;
+    for (uint32_t j = threadIdx.x; j < n; j += kWG)
+      keys[j] = ((uint64_t)ord(item_score(row, lst[j], nzb_q, nzp_q)) << 32) | j;
     __syncthreads();
     dev::wg_sort_lds(keys, n);
   } else {
     uint64_t* g = scratch + qi * cap;
-    for (uint32_t j = threadIdx.x; j < n; j += kWG) g[j] = ((uint64_t)ord(row[lst[j]]) << 32) | j;
+    for (uint32_t j = threadIdx.x; j < n; j += kWG)
+      g[j] = ((uint64_t)ord(item_score(row, lst[j], nzb_q, nzp_q)) << 32) | j;
     __syncthreads();
     if (m > scap) m = scap;
     dev::wg_radix_select(dev::KeyArray{g}, n, m, keys, hist, sh);
@@ -221,78 +274,16 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
 // than kWaveSelMax items are left to the workgroup kernel.
 constexpr uint32_t kWaveSelMax = 512;
 
-__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
-  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
-  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
-  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
-  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
-  return ((uint64_t)hi << 32) | lo;
-}
-
 template <int R>
-__device__ __forceinline__ void wave_bitonic(uint64_t (&key)[R], int lane) {
-  constexpr uint32_t N = 64u * R;
-#pragma unroll
-  for (uint32_t k = 2; k <= N; k <<= 1) {
-#pragma unroll
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      if (j < (uint32_t)R) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const int p = r ^ (int)j;
-          if (p > r) {
-            const uint32_t e = (uint32_t)lane * R + r;
-            const bool up = (e & k) == 0;
-            const uint64_t a = key[r], b = key[p];
-            if ((a > b) == up) {
-              key[r] = b;
-              key[p] = a;
-            }
-          }
-        }
-      } else {
-        const int lm = (int)(j / R);
-        const bool low = (lane & lm) == 0;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const uint32_t e = (uint32_t)lane * R + r;
-          const uint64_t o = shfl_xor_u64(key[r], lm);
-          const bool up = (e & k) == 0;
-          // the lower element of the pair keeps the smaller key when ascending
-          const uint64_t mn = key[r] < o ? key[r] : o, mx = key[r] < o ? o : key[r];
-          key[r] = (low == up) ? mn : mx;
-        }
-      }
-    }
-  }
-}
-
-// sorted key number j (< 64 * R) of a wave-sorted array, broadcast from its lane
-template <int R>
-__device__ __forceinline__ uint64_t wave_key_at(const uint64_t (&key)[R], uint32_t j) {
-  uint64_t v = 0;
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const uint64_t x = shfl_u64(key[r], (int)(j / R));
-    if ((int)(j % R) == r) v = x;
-  }
-  return v;
-}
-
-template <int R>
-__device__ void wave_select(const uint32_t* lst, const float* row, uint32_t n, uint32_t m, int lane,
-                            uint32_t* out, uint32_t avail, uint64_t& feat_key) {
+__device__ void wave_select(const uint32_t* lst, const float* row, const uint32_t* nzb_q, const uint32_t* nzp_q,
+                            uint32_t n, uint32_t m, int lane, uint32_t* out, uint32_t avail, uint64_t& feat_key) {
   uint64_t key[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)lane * R + r;
-    key[r] = e < n ? (((uint64_t)ord(row[lst[e]]) << 32) | e) : ~0ull;
+    key[r] = e < n ? (((uint64_t)ord(item_score(row, lst[e], nzb_q, nzp_q)) << 32) | e) : ~0ull;
   }
-  wave_bitonic<R>(key, lane);
+  dev::wave_bitonic<R>(key, lane);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)lane * R + r;
@@ -301,7 +292,7 @@ __device__ void wave_select(const uint32_t* lst, const float* row, uint32_t n, u
   if (avail > 0) {
     // lane j < 32 takes sorted key min(j, avail - 1)
     const uint32_t j = (uint32_t)lane < avail ? (uint32_t)lane : avail - 1;
-    feat_key = wave_key_at<R>(key, j & 63u);
+    feat_key = dev::wave_key_at<R>(key, j & 63u);
   }
 }
 
@@ -309,7 +300,8 @@ __global__ __launch_bounds__(256) void scored_select_wave_kernel(
     const float* __restrict__ queries, uint64_t q0, uint64_t qn, uint64_t d, const uint32_t* __restrict__ list,
     uint32_t cap, const uint32_t* __restrict__ n_list_q, const float* __restrict__ S, uint64_t ld,
     uint32_t k_default, const uint32_t* __restrict__ budget, uint32_t out_ld, uint32_t* __restrict__ chosen,
-    uint32_t* __restrict__ n_chosen, float* __restrict__ feat_dist) {
+    uint32_t* __restrict__ n_chosen, float* __restrict__ feat_dist, const uint32_t* __restrict__ nzb,
+    const uint32_t* __restrict__ nzp, uint32_t nw) {
   const int lane = threadIdx.x & 63;
   const uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (qi >= qn) return;
@@ -317,6 +309,8 @@ __global__ __launch_bounds__(256) void scored_select_wave_kernel(
   if (n > kWaveSelMax) return;
   const uint32_t* lst = list + qi * cap;
   const float* row = S + qi * ld;
+  const uint32_t* nzb_q = nzb ? nzb + qi * nw : nullptr;
+  const uint32_t* nzp_q = nzb ? nzp + qi * nw : nullptr;
   uint32_t k = k_default;
   if (budget) {
     const uint32_t b = budget[qi];
@@ -326,10 +320,10 @@ __global__ __launch_bounds__(256) void scored_select_wave_kernel(
   uint32_t* out = chosen + qi * out_ld;
   const uint32_t avail = feat_dist ? (n < 32 ? n : 32) : 0;
   uint64_t fk = 0;
-  if (n <= 64) wave_select<1>(lst, row, n, m, lane, out, avail, fk);
-  else if (n <= 128) wave_select<2>(lst, row, n, m, lane, out, avail, fk);
-  else if (n <= 256) wave_select<4>(lst, row, n, m, lane, out, avail, fk);
-  else wave_select<8>(lst, row, n, m, lane, out, avail, fk);
+  if (n <= 64) wave_select<1>(lst, row, nzb_q, nzp_q, n, m, lane, out, avail, fk);
+  else if (n <= 128) wave_select<2>(lst, row, nzb_q, nzp_q, n, m, lane, out, avail, fk);
+  else if (n <= 256) wave_select<4>(lst, row, nzb_q, nzp_q, n, m, lane, out, avail, fk);
+  else wave_select<8>(lst, row, nzb_q, nzp_q, n, m, lane, out, avail, fk);
   if (lane == 0) n_chosen[qi] = m;
   if (feat_dist) {
     const float qq = wave_sq_norm(queries + (q0 + qi) * d, d, lane);
@@ -350,18 +344,20 @@ __global__ __launch_bounds__(256) void scored_select_wave_kernel(
 void launch_scored_select(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, const uint32_t* list,
                           uint32_t cap, const uint32_t* n_list_q, const float* S, uint64_t ld, uint32_t k_default,
                           const uint32_t* budget, uint32_t out_ld, uint32_t* chosen, uint32_t* n_chosen,
-                          float* feat_dist, uint64_t* scratch, bool wave, hipStream_t s) {
+                          float* feat_dist, uint64_t* scratch, bool wave, hipStream_t s, const uint32_t* nzb,
+                          const uint32_t* nzp, uint32_t nw) {
   uint32_t scap = 1024;
   while (scap < k_default) scap <<= 1;
   int64_t skip_upto = -1;
   if (wave) {
     hipLaunchKernelGGL(scored_select_wave_kernel, dim3((unsigned)((qn + 3) / 4)), dim3(256), 0, s, queries, q0, qn,
-                       d, list, cap, n_list_q, S, ld, k_default, budget, out_ld, chosen, n_chosen, feat_dist);
+                       d, list, cap, n_list_q, S, ld, k_default, budget, out_ld, chosen, n_chosen, feat_dist, nzb,
+                       nzp, nw);
     skip_upto = kWaveSelMax;
   }
   hipLaunchKernelGGL(scored_select_kernel, dim3((unsigned)qn), dim3(256), scap * 8, s, queries, q0, d, list, cap,
                      n_list_q, S, ld, k_default, budget, out_ld, chosen, n_chosen, feat_dist, scratch, scap,
-                     skip_upto);
+                     skip_upto, nzb, nzp, nw);
 }
 
 void launch_to_bf16(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, uint64_t Dp, uint16_t* out,
@@ -371,12 +367,12 @@ void launch_to_bf16(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, 
 }
 
 void launch_coarse_gemm(const uint16_t* qb16, uint64_t qn, uint64_t Dp, const uint16_t* cb16, const float* cnorm,
-                        uint32_t C, float* S, hipStream_t s) {
+                        uint32_t C, float* S, hipStream_t s, const uint32_t* nzb, const uint32_t* nzp, uint32_t nw) {
   const uint64_t QT = (qn + kGemmTile - 1) / kGemmTile, CT = (C + kGemmTile - 1) / kGemmTile;
   if (QT * CT > 0xFFFFFFFFull / 2) fail(kUnsupported, "coarse GEMM grid too large");
   const uint32_t per = (uint32_t)((QT * CT + kXcds - 1) / kXcds);
   hipLaunchKernelGGL(coarse_gemm_kernel, dim3(per * kXcds), dim3(256), 0, s, qb16, qn, Dp, cb16, cnorm, C, S,
-                     (uint32_t)QT, (uint32_t)CT, per);
+                     (uint32_t)QT, (uint32_t)CT, per, nzb, nzp, nw, (uint64_t)C);
 }
 
 }  // namespace hdg

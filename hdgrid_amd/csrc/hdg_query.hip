@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void sparse_count_kernel(
     uint32_t* __restrict__ sub_list, uint32_t* __restrict__ sub_cnt,
     uint32_t* __restrict__ ns_out, uint32_t* __restrict__ list_c, uint32_t* __restrict__ cl_beg,
     uint32_t* __restrict__ nnz_c, uint32_t* __restrict__ dense_list, uint32_t* __restrict__ n_dense,
-    unsigned long long* __restrict__ gstat) {
+    unsigned long long* __restrict__ gstat, uint32_t* __restrict__ nzb, uint32_t* __restrict__ nzp) {
   extern __shared__ __align__(16) uint32_t sm[];
   const uint32_t nw = (CS + 31) >> 5;
   uint32_t* hist = sm;            // [CS]
@@ -390,7 +390,13 @@ __global__ __launch_bounds__(256) void sparse_count_kernel(
       ncl += tot;
     }
     __syncthreads();
-    for (uint32_t wi = tid; wi < nw; wi += kWG) bits[wi] = 0;
+    for (uint32_t wi = tid; wi < nw; wi += kWG) {
+      if (nzb) {
+        nzb[qi * nw + wi] = bits[wi];
+        nzp[qi * nw + wi] = wpre[wi];
+      }
+      bits[wi] = 0;
+    }
     if (tid == 0) {
       ns_out[qi] = base;
       nnz_c[qi] = ncl;
@@ -488,7 +494,8 @@ __global__ __launch_bounds__(256) void dense_emit_kernel(uint32_t* __restrict__ 
                                                          uint32_t* __restrict__ sub_list, uint32_t* __restrict__ sub_cnt,
                                                          uint32_t* __restrict__ ns_out, uint32_t* __restrict__ list_c,
                                                          uint32_t* __restrict__ cl_beg, uint32_t* __restrict__ nnz_c,
-                                                         unsigned long long* __restrict__ gstat) {
+                                                         unsigned long long* __restrict__ gstat,
+                                                         uint32_t* __restrict__ nzb, uint32_t* __restrict__ nzp) {
   __shared__ uint32_t tmp[8];
   const uint64_t slot = blockIdx.x;
   if (slot >= *n_dense) return;
@@ -503,6 +510,15 @@ __global__ __launch_bounds__(256) void dense_emit_kernel(uint32_t* __restrict__ 
     if (c != 0) {
       sub_list[q * CS + base + pos] = s;
       sub_cnt[q * CS + base + pos] = c;
+    }
+    if (nzb) {
+      // bit words of the non-zero sub-clusters and the rank of each word's first one
+      const uint64_t bal = __ballot(c != 0);
+      const uint32_t lane = threadIdx.x & 63u, nw = (CS + 31) >> 5, wi = s >> 5;
+      if ((lane == 0 || lane == 32) && wi < nw) {
+        nzb[q * nw + wi] = lane == 0 ? (uint32_t)bal : (uint32_t)(bal >> 32);
+        nzp[q * nw + wi] = base + pos;
+      }
     }
     if (s < CS) row[s] = base + pos;
     base += total;
@@ -1252,6 +1268,59 @@ __global__ __launch_bounds__(256) void inbox_keep_kernel(
   if (threadIdx.x == 0) nL[qi] = base;
 }
 
+// inbox_keep_kernel with one wave per query (4 per workgroup), each with its own chosen-sub-cluster bit
+// set in LDS; the kept points are compacted by ballots in list order, so the output equals the workgroup
+// kernel's.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(256) void inbox_keep_wave_kernel(
+    uint64_t qn, int l, int integral, uint32_t CS, uint32_t sub_lists, const uint32_t* __restrict__ mode,
+    const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zlo_cnt,
+    const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ chosen_sid, uint32_t ibcap,
+    const uint32_t* __restrict__ ib_pos, const uint32_t* __restrict__ ib_sub, const float* __restrict__ lab,
+    const float* __restrict__ ranges, const uint64_t* __restrict__ loff, uint32_t* __restrict__ L,
+    uint32_t* __restrict__ nL, const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
+  __shared__ uint32_t bits_all[4][kSparseMaxCS / 32];
+  const int lane = threadIdx.x & 63;
+  const uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (qi >= qn) return;
+  if (!inbox_keeps(qi, l, integral, mode, n_ib, zlo, zlo_cnt)) return;
+  uint32_t* bits = bits_all[threadIdx.x >> 6];
+  const uint32_t nw = (CS + 31) / 32;
+  for (uint32_t i = lane; i < nw; i += 64) bits[i] = 0;
+  wave_lds_sync();
+  const uint32_t nc = n_chosen_s[qi];
+  for (uint32_t j = lane; j < nc; j += 64) {
+    const uint32_t sid = chosen_sid[qi * sub_lists + j];
+    atomicOr(&bits[sid >> 5], 1u << (sid & 31));
+  }
+  wave_lds_sync();
+  float rg[2 * kMaxDims];
+  for (int i = 0; i < 2 * l; ++i) rg[i] = ranges[qi * 2 * l + i];
+  const uint32_t n = n_ib[qi];
+  const uint32_t md = mode[qi];
+  uint32_t* out = L + loff[qi];
+  uint32_t base = 0;
+  for (uint32_t b = 0; b < n; b += 64) {
+    const uint32_t i = b + (uint32_t)lane;
+    uint32_t pos = 0;
+    bool flag = false;
+    if (i < n) {
+      const uint32_t sid = ib_sub[qi * ibcap + i];
+      pos = ib_pos[qi * ibcap + i];
+      flag = ((bits[sid >> 5] >> (sid & 31)) & 1u) && ((l == 1 && md != 3) || label_pass(lab, l, pos, rg));
+    }
+    const uint64_t mask = __ballot(flag);
+    if (flag) out[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = pos;
+    base += (uint32_t)__popcll(mask);
+  }
+  if (lane == 0) nL[qi] = base;
+}
+
 // A11: ADC distances (product-quantization.cu:37-77) written as f32 per column; the selections read them as
 // unique keys (distance bits, column).
 // Mode 1: the query's Morton-range candidates, column = candidate column.  Mode 2 (EXPS0 <= 1): the
@@ -1580,7 +1649,8 @@ __global__ __launch_bounds__(256) void final_pick_kernel(
     const uint64_t* __restrict__ kprime, const uint64_t* __restrict__ loff, const uint32_t* __restrict__ L,
     const uint32_t* __restrict__ nL, const uint64_t* __restrict__ pad32, const uint64_t* __restrict__ padmax,
     uint64_t* __restrict__ keys, const uint32_t* __restrict__ adist, const uint64_t* __restrict__ aoff,
-    float* __restrict__ out_dist, uint64_t* __restrict__ out_ids, const uint32_t* __restrict__ abort_flag) {
+    float* __restrict__ out_dist, uint64_t* __restrict__ out_ids, const uint32_t* __restrict__ abort_flag,
+    int64_t skip_upto) {
   if (abort_flag && *abort_flag) return;
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t* sorted = reinterpret_cast<uint64_t*>(smem);
@@ -1588,6 +1658,7 @@ __global__ __launch_bounds__(256) void final_pick_kernel(
   __shared__ uint64_t sh[8];
   const uint64_t qi = blockIdx.x;
   const uint32_t n = nL[qi];
+  if (n <= skip_upto) return;  // handled by final_pick_wave_kernel
   const uint32_t* list = L + loff[qi];
 
   float* od = out_dist + (q0 + qi) * topk;
@@ -1640,6 +1711,113 @@ __global__ __launch_bounds__(256) void final_pick_kernel(
   }
 }
 
+// final_pick_kernel for refine lists of at most kWavePickMax items, one wave per query: the list's keys are
+// sorted in registers (the same keys and order as the workgroup sort) and every lane writes its own sorted
+// keys to their output slots (shifted past the pad columns), then the pads.  Longer lists are left to
+// final_pick_kernel.
+constexpr uint32_t kWavePickMax = 512;
+
+template <int R, class Src>
+__device__ void wave_pick(Src src, uint32_t n, uint32_t k, int lane, int l, bool adc_mode, uint64_t p32,
+                          uint64_t pmax, const uint32_t* __restrict__ zid, const uint32_t* __restrict__ list,
+                          float* __restrict__ od, uint64_t* __restrict__ oi) {
+  uint64_t key[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)lane * R + r;
+    key[r] = e < n ? src(e) : ~0ull;
+  }
+  dev::wave_bitonic<R>(key, lane);
+  const uint32_t m = n < k ? n : k;
+  if (adc_mode) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t e = (uint32_t)lane * R + r;
+      if (e < m) {
+        od[e] = dev::key_value(key[r]);
+        oi[e] = zid[list[dev::key_tie(key[r])]];
+      }
+    }
+    for (uint32_t j = m + (uint32_t)lane; j < k; j += 64) {
+      od[j] = FLT_MAX;
+      oi[j] = kU64Max;
+    }
+    return;
+  }
+  // items of value <= 1e6 first, then the pad columns, then the remaining real items
+  uint32_t le = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)lane * R + r;
+    le += (e < m && dev::key_value(key[r]) <= kExactPad) ? 1u : 0u;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) le += __shfl_xor(le, o, 64);
+  const uint32_t n_le = le;
+  const uint64_t npad = p32 + pmax;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)lane * R + r;
+    if (e >= m) continue;
+    const uint64_t j = e < n_le ? (uint64_t)e : (uint64_t)e + npad;
+    if (j < k) {
+      od[j] = dev::key_value(key[r]);
+      oi[j] = dev::key_tie(key[r]);
+    }
+  }
+  for (uint64_t j = (uint64_t)n_le + (uint64_t)lane; j < k; j += 64) {
+    const uint64_t t = j - n_le;
+    if (t < p32) {
+      od[j] = kExactPad;
+      oi[j] = kIdPad;
+    } else if (t < npad) {
+      od[j] = kExactPad;
+      oi[j] = kU64Max;
+    } else if (j - npad >= m) {
+      od[j] = kExactPad;  // beyond the real items
+      oi[j] = kU64Max;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void final_pick_wave_kernel(
+    uint64_t q0, uint64_t qn, const uint32_t* __restrict__ zid, int l, uint64_t topk,
+    const uint32_t* __restrict__ mode, const uint64_t* __restrict__ kprime, const uint64_t* __restrict__ loff,
+    const uint32_t* __restrict__ L, const uint32_t* __restrict__ nL, const uint64_t* __restrict__ pad32,
+    const uint64_t* __restrict__ padmax, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ adist,
+    const uint64_t* __restrict__ aoff, float* __restrict__ out_dist, uint64_t* __restrict__ out_ids,
+    const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
+  const int lane = threadIdx.x & 63;
+  const uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (qi >= qn) return;
+  const uint32_t n = nL[qi];
+  if (n > kWavePickMax) return;
+  const uint32_t* list = L + loff[qi];
+  float* od = out_dist + (q0 + qi) * topk;
+  uint64_t* oi = out_ids + (q0 + qi) * topk;
+  const uint32_t k = (uint32_t)topk;
+  const bool adc_mode = mode[qi] == 2;
+  uint64_t p32 = 0, pmax = 0;
+  if (!adc_mode) {
+    p32 = pad32[qi];
+    pmax = padmax[qi];
+    if (l > 1) pmax = kprime[qi] > n ? kprime[qi] - n : 0;
+  }
+#define HDG_WAVE_PICK(R)                                                                                      \
+  do {                                                                                                        \
+    if (adc_mode)                                                                                             \
+      wave_pick<R>(dev::KeyDist{adist + aoff[qi]}, n, k, lane, l, true, p32, pmax, zid, list, od, oi);        \
+    else                                                                                                      \
+      wave_pick<R>(dev::KeyArray{keys + loff[qi]}, n, k, lane, l, false, p32, pmax, zid, list, od, oi);       \
+  } while (0)
+  if (n <= 64) HDG_WAVE_PICK(1);
+  else if (n <= 128) HDG_WAVE_PICK(2);
+  else if (n <= 256) HDG_WAVE_PICK(4);
+  else HDG_WAVE_PICK(8);
+#undef HDG_WAVE_PICK
+}
+
 // Per-query counters of the chunk into the call-wide table st[q][6] = (C_q, in-box count of the chosen
 // sub-clusters, refined, scored, ADC-scanned, k').
 __global__ void stats_kernel(uint64_t qn, uint64_t q0, const uint64_t* __restrict__ cand,
@@ -1673,6 +1851,7 @@ struct Index::Work {
   DevBuf mode, kprime, lsize, asize, chunks, pad32, padmax, nL;
   DevBuf L, Lkeys, adist, akeys2;
   DevBuf S, feat, budget_T, budget_S, qb16;  // PRECISION 1: scores, model features, budgets, bf16 queries
+  DevBuf nzb, nzp;                            // PRECISION 1: non-zero sub-cluster bit words and their ranks
   DevBuf st, st_chosen;                // call-wide counters [nq][6] and chosen sub-cluster ids [nq][sub_lists]
   uint64_t st_nq = 0, st_sub_lists = 0;
   // GRAPH 1: per-chunk captured graphs, replayed on an internal stream
@@ -2030,7 +2209,15 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   // PRECISION 1 scores queries x (sub-)centers in blocks of SCORE_BLOCK queries through one score buffer, small
   // enough to stay in the memory-side cache between the GEMM that writes it and the selection that reads it
   const uint64_t score_block = std::min<uint64_t>(qn, std::max<uint64_t>(cfg_.score_block, 128));
+  // PRECISION 1: per query, bit words of its non-zero sub-clusters and their ranks (compact score rows)
+  const uint32_t nws = (uint32_t)((CS + 31) / 32);
+  uint32_t* nzb = nullptr;
+  uint32_t* nzp = nullptr;
   if (mixed) {
+    w.nzb.reserve(qn * nws * 4);
+    w.nzp.reserve(qn * nws * 4);
+    nzb = w.nzb.as<uint32_t>();
+    nzp = w.nzp.as<uint32_t>();
     w.S.reserve(score_block * std::max<uint64_t>(C, CS) * 4);
     w.feat.reserve(qn * kFeatDists * 4);
     w.qb16.reserve(qn * S.Dp * 2);
@@ -2062,7 +2249,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                          max_tests, S.epos.as<uint32_t>(), kInboxCap, w.ib_pos.as<uint32_t>(),           \
                          w.ib_sub.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.sub_list.as<uint32_t>(), w.sub_cnt.as<uint32_t>(), w.ns.as<uint32_t>(),                 \
                          w.list_c.as<uint32_t>(), w.cl_beg.as<uint32_t>(), w.nnz_c.as<uint32_t>(),                 \
-                         w.dense_list.as<uint32_t>(), w.n_dense.as<uint32_t>(), w.gstat.as<unsigned long long>()); \
+                         w.dense_list.as<uint32_t>(), w.n_dense.as<uint32_t>(), w.gstat.as<unsigned long long>(), \
+                         nzb, nzp);                                                                                \
     } else {                                                                                                       \
       hipLaunchKernelGGL(iota_kernel, dim3((unsigned)cdiv(qn, 256)), dim3(256), 0, s, qn,                          \
                          w.dense_list.as<uint32_t>(), w.n_dense.as<uint32_t>(), w.n_ib.as<uint32_t>());           \
@@ -2089,7 +2277,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                        S.sub_off.as<uint32_t>(), (uint32_t)C, G, q0, w.dense_list.as<uint32_t>(),
                        w.n_dense.as<uint32_t>(), w.sub_list.as<uint32_t>(), w.sub_cnt.as<uint32_t>(),
                        w.ns.as<uint32_t>(), w.list_c.as<uint32_t>(), w.cl_beg.as<uint32_t>(), w.nnz_c.as<uint32_t>(),
-                       w.gstat.as<unsigned long long>());
+                       w.gstat.as<unsigned long long>(), nzb, nzp);
     HDG_HIP(hipGetLastError());
   }
   mark();
@@ -2145,12 +2333,13 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     for (uint64_t b0 = 0; b0 < qn; b0 += score_block) {
       const uint64_t bn = std::min(score_block, qn - b0);
       launch_coarse_gemm(w.qb16.as<uint16_t>() + b0 * S.Dp, bn, S.Dp, S.sub_centers_bf16.as<uint16_t>(),
-                         S.sub_center_norm.as<float>(), (uint32_t)CS, w.S.as<float>(), s);
+                         S.sub_center_norm.as<float>(), (uint32_t)CS, w.S.as<float>(), s, nzb + b0 * nws,
+                         nzp + b0 * nws, nws);
       launch_scored_select(queries, q0 + b0, bn, d, w.list_s.as<uint32_t>() + b0 * cap, cap,
                            w.nnz_s.as<uint32_t>() + b0, w.S.as<float>(), CS, sub_lists,
                            budget_S ? budget_S + b0 : nullptr, sub_lists, w.chosen_s.as<uint32_t>() + b0 * sub_lists,
                            w.n_chosen_s.as<uint32_t>() + b0, nullptr, w.keys.as<uint64_t>() + b0 * cap,
-                           cfg_.select_wave != 0, s);
+                           cfg_.select_wave != 0, s, nzb + b0 * nws, nzp + b0 * nws, nws);
     }
   }
   HDG_HIP(hipGetLastError());
@@ -2201,7 +2390,13 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.adist.reserve(std::max<uint64_t>(totA, 1) * 4);
   w.akeys2.reserve(std::max<uint64_t>(totA, 1) * 8);
   const int integral = S.labels_integral ? 1 : 0;
-  if (sparse)
+  if (sparse && cfg_.select_wave)
+    hipLaunchKernelGGL(inbox_keep_wave_kernel, dim3((unsigned)cdiv(qn, 4)), dim3(256), 0, s, qn, l, integral,
+                       (uint32_t)CS, sub_lists, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
+                       w.zlo_cnt.as<uint64_t>(), w.n_chosen_s.as<uint32_t>(), w.chosen_sid.as<uint32_t>(), kInboxCap,
+                       w.ib_pos.as<uint32_t>(), w.ib_sub.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(),
+                       w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
+  else if (sparse)
     hipLaunchKernelGGL(inbox_keep_kernel, dim3((unsigned)qn), dim3(256), 0, s, l, integral, (uint32_t)CS, sub_lists,
                        w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(), w.zlo_cnt.as<uint64_t>(),
                        w.n_chosen_s.as<uint32_t>(), w.chosen_sid.as<uint32_t>(), kInboxCap, w.ib_pos.as<uint32_t>(),
@@ -2275,11 +2470,20 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      w.woff.as<uint64_t>(), qn, w.L.as<uint32_t>(), (uint64_t)0, w.lsize.as<uint64_t>(),
                      w.nL.as<uint32_t>(), S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>(), abort_p);
   mark();
+  int64_t pick_skip = -1;
+  if (cfg_.select_wave) {
+    hipLaunchKernelGGL(final_pick_wave_kernel, dim3((unsigned)cdiv(qn, 4)), dim3(256), 0, s, q0, qn,
+                       S.zid.as<uint32_t>(), l, topk, w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(),
+                       w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), w.pad32.as<uint64_t>(),
+                       w.padmax.as<uint64_t>(), w.Lkeys.as<uint64_t>(), w.adist.as<uint32_t>(),
+                       w.asize.as<uint64_t>(), out_dist, out_ids, abort_p);
+    pick_skip = kWavePickMax;
+  }
   hipLaunchKernelGGL(final_pick_kernel, dim3((unsigned)qn), dim3(256), scap_f * 8, s, q0, scap_f, S.zid.as<uint32_t>(), l,
                      topk,
                      w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(),
                      w.nL.as<uint32_t>(), w.pad32.as<uint64_t>(), w.padmax.as<uint64_t>(), w.Lkeys.as<uint64_t>(),
-                     w.adist.as<uint32_t>(), w.asize.as<uint64_t>(), out_dist, out_ids, abort_p);
+                     w.adist.as<uint32_t>(), w.asize.as<uint64_t>(), out_dist, out_ids, abort_p, pick_skip);
   HDG_HIP(hipGetLastError());
   mark();
   hipLaunchKernelGGL(stats_kernel, dim3((unsigned)cdiv(qn, 256)), dim3(256), 0, s, qn, q0, w.cand.as<uint64_t>(),
