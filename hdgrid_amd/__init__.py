@@ -267,7 +267,8 @@ class SuperGridIndex:
     # -- learned budget (PRECISION 1, BUDGET_MODEL 1) -----------------------------------------
     def budget_fit(self, queries, query_labels, gt_ids, levels=10, t0=0.0, s0=8.0, ratio=2 ** 0.5, target=0.95,
                    epochs=30, lr=2e-3, seed=0x4844, stream=None):
-        """Label training queries with their minimal budget level and train the MLP; returns the report."""
+        """Label training queries with their minimal sub-cluster budget level and train the MLP; returns the
+        report.  t0 must stay 0 (every level probes all non-zero clusters)."""
         import numpy as np
         queries = queries.contiguous()
         query_labels = query_labels.contiguous()

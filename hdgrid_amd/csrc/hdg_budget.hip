@@ -85,7 +85,8 @@ __device__ float mlp_forward(const float* w, const float* x, float* z1, float* z
   return y;
 }
 
-// Inference: features -> standardise -> MLP -> level -> (clusters, sub-clusters); trims n_chosen_c.
+// Inference: features -> standardise -> MLP -> level -> sub-cluster budget; trims the query's chosen
+// sub-clusters (sorted by score) to the budget.
 // 16 lanes per query: each lane owns 4 hidden units of each layer (every unit's sum runs over its inputs
 // in order, as in mlp_forward), the output sum is taken by the group's first lane.  A workgroup of 256
 // threads serves 16 queries and keeps the bf16 weights in LDS.
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(256) void infer_kernel(uint64_t qn, int l, const fl
                                                     const float* __restrict__ inv_std, float margin, int levels,
                                                     const uint32_t* __restrict__ ladT,
                                                     const uint32_t* __restrict__ ladS,
-                                                    uint32_t* __restrict__ n_chosen_c, uint32_t* __restrict__ bT,
+                                                    uint32_t* __restrict__ n_chosen_s, uint32_t* __restrict__ bT,
                                                     uint32_t* __restrict__ bS) {
   __shared__ float w[kParams];
   __shared__ float xs[kInferQ][kF];
@@ -142,10 +143,10 @@ __global__ __launch_bounds__(256) void infer_kernel(uint64_t qn, int l, const fl
     float t = ceilf(y + margin);
     lev = t < 0.0f ? 0 : (t > (float)(levels - 1) ? levels - 1 : (int)t);
   }
-  const uint32_t T = ladT[lev];
-  bT[qi] = T;
-  bS[qi] = ladS[lev];
-  if (n_chosen_c[qi] > T) n_chosen_c[qi] = T;
+  bT[qi] = ladT[lev];
+  const uint32_t S = ladS[lev];
+  bS[qi] = S;
+  if (n_chosen_s[qi] > S) n_chosen_s[qi] = S;
 }
 
 // Training forward + backward over a mini-batch; gradients accumulated into g (f32, atomics per block).
@@ -238,14 +239,23 @@ struct Index::BudgetModel {
 
 bool Index::budget_model_ready() const { return budget_ && budget_->ready; }
 
+bool Index::budget_capturing() const { return budget_ && budget_->sink; }
+
+bool Index::budget_probes_all(uint32_t C) const {
+  if (!budget_) return true;
+  for (uint32_t t : budget_->ladT)
+    if (t < C) return false;
+  return true;
+}
+
 void Index::run_budget_model(uint64_t qn, int l, const float* ranges, const uint32_t* box_lo, const uint32_t* box_w,
-                             int n_valid, const float* feat_dist, uint32_t* n_chosen_c, uint32_t* budget_T,
+                             int n_valid, const float* feat_dist, uint32_t* n_chosen_s, uint32_t* budget_T,
                              uint32_t* budget_S, hipStream_t s) {
   (void)box_lo;
   BudgetModel& B = *budget_;
   hipLaunchKernelGGL(infer_kernel, dim3((unsigned)((qn + kInferQ - 1) / kInferQ)), dim3(256), 0, s, qn, l, ranges, box_w,
                      n_valid, feat_dist, B.d_w.as<float>(), B.d_mean.as<float>(), B.d_inv.as<float>(), B.margin,
-                     (int)B.ladT.size(), B.d_ladT.as<uint32_t>(), B.d_ladS.as<uint32_t>(), n_chosen_c, budget_T,
+                     (int)B.ladT.size(), B.d_ladT.as<uint32_t>(), B.d_ladS.as<uint32_t>(), n_chosen_s, budget_T,
                      budget_S);
   HDG_HIP(hipGetLastError());
 }
@@ -297,6 +307,9 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
   if (!st_.built) fail(kBadState, "budget fit before build");
   if (k != cfg_.topk) fail(kInvalidArgument, "ground truth width differs from TOPK");
   if (P.levels < 2 || P.levels > 32) fail(kInvalidArgument, "levels must be in [2, 32]");
+  // the budget is applied after the sub-cluster scoring, whose scores are the features: every level probes
+  // all non-zero clusters and only the sub-cluster budget grows
+  if (P.t0 > 0) fail(kUnsupported, "cluster budgets (t0 > 0) are not supported; the model budgets sub-clusters");
   if (!budget_) budget_ = new BudgetModel();
   BudgetModel& B = *budget_;
   B.ready = false;

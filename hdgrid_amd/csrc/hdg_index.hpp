@@ -106,8 +106,11 @@ class Index {
 
   // learned per-query budget (hdg_budget.hip)
   bool budget_model_ready() const;
+  bool budget_capturing() const;                 // budget_fit is recording feature rows
+  bool budget_probes_all(uint32_t C) const;      // every level of the loaded model probes >= C clusters
+  // Per-query sub-cluster budgets from the MLP; trims n_chosen_s (chosen sub-clusters, sorted) to them.
   void run_budget_model(uint64_t qn, int l, const float* ranges, const uint32_t* box_lo, const uint32_t* box_w,
-                        int n_valid, const float* feat_dist, uint32_t* n_chosen_c, uint32_t* budget_T,
+                        int n_valid, const float* feat_dist, uint32_t* n_chosen_s, uint32_t* budget_T,
                         uint32_t* budget_S, hipStream_t s);
   void capture_features(uint64_t q0, uint64_t qn, int l, const float* ranges, const uint32_t* box_w, int n_valid,
                         const float* feat_dist, hipStream_t s);

@@ -2281,12 +2281,24 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     HDG_HIP(hipGetLastError());
   }
   mark();
+  // learned budget (PRECISION 1): the sub-cluster scoring writes the features (its 32 smallest scores and
+  // |q|^2); the model then trims each query's sorted sub-cluster choice to its budget
+  const bool need_feat = mixed && (budget_T || budget_capturing());
   auto budgets = [&]() {
     capture_features(q0, qn, l, w.ranges.as<float>(), w.box_w.as<uint32_t>(), g.n_valid, w.feat.as<float>(), s);
     if (budget_T)
       run_budget_model(qn, l, w.ranges.as<float>(), w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), g.n_valid,
-                       w.feat.as<float>(), w.n_chosen_c.as<uint32_t>(), budget_T, budget_S, s);
+                       w.feat.as<float>(), w.n_chosen_s.as<uint32_t>(), budget_T, budget_S, s);
   };
+  // PRECISION 1 with every non-zero cluster probed (PROBE_CLUSTERS and the budget model's cluster ladder at
+  // least CLUSTERS): the chosen clusters' non-zero sub-clusters are all non-zero sub-clusters, so the walk's
+  // ascending list is scored as it is (the order only breaks exact score ties) and the cluster choice and
+  // expansion are skipped.
+  const uint32_t kc = cfg_.probe_clusters > 0 ? std::min<uint32_t>(n_list, (uint32_t)cfg_.probe_clusters) : n_list;
+  const bool all_clusters = mixed && kc >= C && (!budget_T || budget_probes_all((uint32_t)C));
+  const uint32_t* Ls = all_clusters ? w.sub_list.as<uint32_t>() : w.list_s.as<uint32_t>();
+  const uint32_t* Lcnt = all_clusters ? w.sub_cnt.as<uint32_t>() : w.list_scnt.as<uint32_t>();
+  const uint32_t* Lnnz = all_clusters ? w.ns.as<uint32_t>() : w.nnz_s.as<uint32_t>();
   // ---- A5: cluster choice ----
   if (!mixed) {
     scan_u64(w.woff.as<uint64_t>(), nullptr, nullptr, qn, w.nnz_c.as<uint32_t>(), nullptr, 0u);
@@ -2297,28 +2309,27 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                        scap_c,
                        w.nnz_c.as<uint32_t>(), w.gstat.as<unsigned long long>(), w.keys.as<uint64_t>(),
                        w.chosen_c.as<uint32_t>(), w.n_chosen_c.as<uint32_t>());
-  } else {
-    const uint32_t kc = cfg_.probe_clusters > 0 ? std::min<uint32_t>(n_list, (uint32_t)cfg_.probe_clusters) : n_list;
+  } else if (!all_clusters) {
+    // the coarse scores pick the clusters
     for (uint64_t b0 = 0; b0 < qn; b0 += score_block) {
       const uint64_t bn = std::min(score_block, qn - b0);
       launch_coarse_gemm(w.qb16.as<uint16_t>() + b0 * S.Dp, bn, S.Dp, S.centers_bf16.as<uint16_t>(),
                          S.center_norm.as<float>(), (uint32_t)C, w.S.as<float>(), s);
       launch_scored_select(queries, q0 + b0, bn, d, w.list_c.as<uint32_t>() + b0 * C, (uint32_t)C,
                            w.nnz_c.as<uint32_t>() + b0, w.S.as<float>(), C, kc, nullptr, n_list,
-                           w.chosen_c.as<uint32_t>() + b0 * n_list, w.n_chosen_c.as<uint32_t>() + b0,
-                           w.feat.as<float>() + b0 * kFeatDists, w.keys.as<uint64_t>() + b0 * C,
-                           cfg_.select_wave != 0, s);
+                           w.chosen_c.as<uint32_t>() + b0 * n_list, w.n_chosen_c.as<uint32_t>() + b0, nullptr,
+                           w.keys.as<uint64_t>() + b0 * C, cfg_.select_wave != 0, s);
     }
-    budgets();
   }
   HDG_HIP(hipGetLastError());
   mark();
   // ---- A6: sub-cluster choice ----
-  hipLaunchKernelGGL(sub_expand_kernel, dim3((unsigned)qn), dim3(256), 0, s, (uint32_t)CS, (uint32_t)C,
-                     S.sub_off.as<uint32_t>(), n_list, G, q0, w.list_c.as<uint32_t>(), w.cl_beg.as<uint32_t>(),
-                     w.sub_list.as<uint32_t>(), w.sub_cnt.as<uint32_t>(), w.chosen_c.as<uint32_t>(),
-                     w.n_chosen_c.as<uint32_t>(), cap, w.list_s.as<uint32_t>(), w.list_scnt.as<uint32_t>(),
-                     w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>());
+  if (!all_clusters)
+    hipLaunchKernelGGL(sub_expand_kernel, dim3((unsigned)qn), dim3(256), 0, s, (uint32_t)CS, (uint32_t)C,
+                       S.sub_off.as<uint32_t>(), n_list, G, q0, w.list_c.as<uint32_t>(), w.cl_beg.as<uint32_t>(),
+                       w.sub_list.as<uint32_t>(), w.sub_cnt.as<uint32_t>(), w.chosen_c.as<uint32_t>(),
+                       w.n_chosen_c.as<uint32_t>(), cap, w.list_s.as<uint32_t>(), w.list_scnt.as<uint32_t>(),
+                       w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>());
   HDG_HIP(hipGetLastError());
   mark();
   if (!mixed) {
@@ -2335,18 +2346,19 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
       launch_coarse_gemm(w.qb16.as<uint16_t>() + b0 * S.Dp, bn, S.Dp, S.sub_centers_bf16.as<uint16_t>(),
                          S.sub_center_norm.as<float>(), (uint32_t)CS, w.S.as<float>(), s, nzb + b0 * nws,
                          nzp + b0 * nws, nws);
-      launch_scored_select(queries, q0 + b0, bn, d, w.list_s.as<uint32_t>() + b0 * cap, cap,
-                           w.nnz_s.as<uint32_t>() + b0, w.S.as<float>(), CS, sub_lists,
-                           budget_S ? budget_S + b0 : nullptr, sub_lists, w.chosen_s.as<uint32_t>() + b0 * sub_lists,
-                           w.n_chosen_s.as<uint32_t>() + b0, nullptr, w.keys.as<uint64_t>() + b0 * cap,
-                           cfg_.select_wave != 0, s, nzb + b0 * nws, nzp + b0 * nws, nws);
+      launch_scored_select(queries, q0 + b0, bn, d, Ls + b0 * cap, cap, Lnnz + b0, w.S.as<float>(), CS, sub_lists,
+                           nullptr, sub_lists, w.chosen_s.as<uint32_t>() + b0 * sub_lists,
+                           w.n_chosen_s.as<uint32_t>() + b0, need_feat ? w.feat.as<float>() + b0 * kFeatDists : nullptr,
+                           w.keys.as<uint64_t>() + b0 * cap, cfg_.select_wave != 0, s, nzb + b0 * nws,
+                           nzp + b0 * nws, nws);
     }
+    budgets();
   }
   HDG_HIP(hipGetLastError());
   mark();
   hipLaunchKernelGGL(ranges_kernel, dim3((unsigned)qn), dim3(256), 0, s, S.zcode.as<uint64_t>(),
                      S.seg_off.as<uint32_t>(), S.cells.as<uint32_t>(), S.N, g.n_valid, w.box_lo.as<uint32_t>(),
-                     w.box_w.as<uint32_t>(), qn, w.zlo_cnt.as<uint64_t>(), cap, w.list_s.as<uint32_t>(), w.list_scnt.as<uint32_t>(), sub_lists, G,
+                     w.box_w.as<uint32_t>(), qn, w.zlo_cnt.as<uint64_t>(), cap, Ls, Lcnt, sub_lists, G,
                      q0, w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.zlo.as<uint64_t>(),
                      w.zhi.as<uint64_t>(), w.chosen_sid.as<uint32_t>(), w.rstart.as<uint32_t>(),
                      w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.real.as<uint64_t>(),
@@ -2487,7 +2499,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   HDG_HIP(hipGetLastError());
   mark();
   hipLaunchKernelGGL(stats_kernel, dim3((unsigned)cdiv(qn, 256)), dim3(256), 0, s, qn, q0, w.cand.as<uint64_t>(),
-                     w.real.as<uint64_t>(), w.nL.as<uint32_t>(), w.nnz_c.as<uint32_t>(), w.nnz_s.as<uint32_t>(),
+                     w.real.as<uint64_t>(), w.nL.as<uint32_t>(), w.nnz_c.as<uint32_t>(), Lnnz,
                      w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.st.as<uint64_t>(), abort_p);
   HDG_HIP(hipGetLastError());
   HDG_HIP(hipMemcpyAsync(w.st_chosen.as<uint32_t>() + q0 * sub_lists, w.chosen_sid.data(), qn * sub_lists * 4,

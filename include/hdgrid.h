@@ -113,9 +113,12 @@ hdg_status hdg_last_stage_ms(const hdg_index* index, char* names, uint64_t cap, 
  * hdg_budget_fit labels each of the n training queries (device arrays, like hdg_query) with the
  * smallest budget level whose top-k reaches `target` recall against gt_ids (host [n][TOPK] u64, invalid
  * ids >= 2^63), running this pipeline at every level, then trains the 48-64-64-1 MLP on the device.
- * Ladder: level i probes round(t0*ratio^i) clusters and keeps round(s0*ratio^i) sub-clusters (capped by
- * N_LIST / SUB_LISTS).  report receives 8 floats: final loss, margin, holdout recall, holdout mean level,
- * mean label, recall of the top level, n_train, n_holdout. */
+ * Ladder: level i keeps round(s0*ratio^i) sub-clusters (capped by SUB_LISTS) of all non-zero clusters; t0
+ * must be 0 (cluster budgets are not modelled: HDG_EUNSUPPORTED otherwise).  Features per query: filter
+ * widths, grid-cell extents and the 32 smallest BF16 sub-center scores among its non-zero sub-clusters
+ * (+ |q|^2); at query time the budget trims the score-sorted sub-cluster choice.  report receives 8 floats:
+ * final loss, margin, holdout recall, holdout mean level, mean label, recall of the top level, n_train,
+ * n_holdout. */
 hdg_status hdg_budget_fit(hdg_index* index, const float* queries, uint64_t n, uint64_t dim,
                           const float* query_labels, uint64_t qlabel_cols, const uint64_t* gt_ids, uint64_t k,
                           int levels, double t0, double s0, double ratio, float target, int epochs, float lr,
