@@ -122,6 +122,44 @@ __device__ __forceinline__ void wave_bitonic(uint64_t (&key)[R], int lane) {
   }
 }
 
+// The same network for 32-bit keys.
+template <int R>
+__device__ __forceinline__ void wave_bitonic32(uint32_t (&key)[R], int lane) {
+  constexpr uint32_t N = 64u * R;
+#pragma unroll
+  for (uint32_t k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j < (uint32_t)R) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int p = r ^ (int)j;
+          if (p > r) {
+            const uint32_t e = (uint32_t)lane * R + r;
+            const bool up = (e & k) == 0;
+            const uint32_t a = key[r], b = key[p];
+            if ((a > b) == up) {
+              key[r] = b;
+              key[p] = a;
+            }
+          }
+        }
+      } else {
+        const int lm = (int)(j / R);
+        const bool low = (lane & lm) == 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint32_t e = (uint32_t)lane * R + r;
+          const uint32_t o = (uint32_t)__shfl_xor((int)key[r], lm, 64);
+          const bool up = (e & k) == 0;
+          const uint32_t mn = key[r] < o ? key[r] : o, mx = key[r] < o ? o : key[r];
+          key[r] = (low == up) ? mn : mx;
+        }
+      }
+    }
+  }
+}
+
 // sorted key number j (< 64 * R) of a wave-sorted array, broadcast from its lane
 template <int R>
 __device__ __forceinline__ uint64_t wave_key_at(const uint64_t (&key)[R], uint32_t j) {

@@ -76,7 +76,8 @@ __global__ void clear_kernel(unsigned long long* __restrict__ gstat, uint64_t n,
   if (i < n) gstat[i] = 0;
   if (i == 0) {
     *err = 0;
-    *n_dense = 0;
+    n_dense[0] = 0;  // queries handed to the dense count
+    n_dense[1] = 0;  // queries handed from the wave walk to the workgroup walk
     if (abort_flag) *abort_flag = 0;
   }
 }
@@ -215,7 +216,8 @@ __global__ __launch_bounds__(256) void sparse_count_kernel(
     uint32_t* __restrict__ sub_list, uint32_t* __restrict__ sub_cnt,
     uint32_t* __restrict__ ns_out, uint32_t* __restrict__ list_c, uint32_t* __restrict__ cl_beg,
     uint32_t* __restrict__ nnz_c, uint32_t* __restrict__ dense_list, uint32_t* __restrict__ n_dense,
-    unsigned long long* __restrict__ gstat, uint32_t* __restrict__ nzb, uint32_t* __restrict__ nzp) {
+    unsigned long long* __restrict__ gstat, uint32_t* __restrict__ nzb, uint32_t* __restrict__ nzp,
+    const uint32_t* __restrict__ qlist, const uint32_t* __restrict__ qlist_n) {
   extern __shared__ __align__(16) uint32_t sm[];
   const uint32_t nw = (CS + 31) >> 5;
   uint32_t* hist = sm;            // [CS]
@@ -226,7 +228,10 @@ __global__ __launch_bounds__(256) void sparse_count_kernel(
   for (uint32_t i = tid; i < CS; i += kWG) hist[i] = 0;
   for (uint32_t i = tid; i < nw; i += kWG) bits[i] = 0;
   __syncthreads();
-  for (uint64_t qi = blockIdx.x; qi < qn; qi += gridDim.x) {
+  // all queries, or only those listed in qlist[0 .. *qlist_n)
+  const uint64_t nslots = qlist ? (uint64_t)*qlist_n : qn;
+  for (uint64_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+    const uint64_t qi = qlist ? (uint64_t)qlist[slot] : slot;
     uint32_t lo[NV], w[NV], blo[NV], bhi[NV];
     bool empty = false;
 #pragma unroll
@@ -405,6 +410,261 @@ __global__ __launch_bounds__(256) void sparse_count_kernel(
       atomicMax(&gstat[((q0 + qi) / G) * 8 + 0], (unsigned long long)ncl);
     }
     __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// A3/A4 (sparse), one wave per query: the same walk, in-box list, counts and lists as sparse_count_kernel,
+// without its LDS histogram and barriers.  The wave keeps the sub-cluster ids of its in-box points in a wave
+// buffer of kWaveIb entries, sorts them in registers, and takes the non-zero sub-clusters (ascending, with
+// exact counts) from their runs; an LDS bit set of the distinct ids with word ranks gives the non-zero
+// clusters (cluster c owns sub-clusters [sub_off[c], sub_off[c+1])) and the bit words + ranks.  A query with more in-box points
+// than kWaveIb goes on `redo` for sparse_count_kernel; a wide box goes to the dense count as there.
+constexpr uint32_t kWaveIb = 512;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int R>
+__device__ void wave_sort_buf(uint32_t* buf, uint32_t n, int lane) {
+  uint32_t key[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)lane * R + r;
+    key[r] = e < n ? buf[e] : 0xFFFFFFFFu;
+  }
+  dev::wave_bitonic32<R>(key, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)lane * R + r;
+    if (e < n) buf[e] = key[r];
+  }
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void sparse_count_wave_kernel(
+    Buckets B, const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ ecell,
+    const uint32_t* __restrict__ esub, uint64_t N, const uint32_t* __restrict__ sub_off, uint32_t CS,
+    uint32_t C, const uint32_t* __restrict__ box_lo, const uint32_t* __restrict__ box_w, uint64_t qn, uint64_t G,
+    uint64_t q0, uint32_t max_tests, const uint32_t* __restrict__ epos, uint32_t ibcap,
+    uint32_t* __restrict__ ib_pos, uint32_t* __restrict__ ib_sub, uint32_t* __restrict__ n_ib,
+    uint32_t* __restrict__ sub_list, uint32_t* __restrict__ sub_cnt,
+    uint32_t* __restrict__ ns_out, uint32_t* __restrict__ list_c, uint32_t* __restrict__ cl_beg,
+    uint32_t* __restrict__ nnz_c, uint32_t* __restrict__ dense_list, uint32_t* __restrict__ n_dense,
+    unsigned long long* __restrict__ gstat, uint32_t* __restrict__ nzb, uint32_t* __restrict__ nzp,
+    uint32_t* __restrict__ redo) {
+  __shared__ uint32_t sbuf_all[4][kWaveIb];  // in-box sub-cluster ids; sorted; then first positions of runs
+  __shared__ uint32_t ubuf_all[4][kWaveIb];  // unique sub-cluster ids in rank order
+  __shared__ uint32_t words_all[4][kSparseMaxCS / 32], wrank_all[4][kSparseMaxCS / 32];  // bit set, word ranks
+  __shared__ uint32_t pre_all[4][64], e0_all[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t* sbuf = sbuf_all[wv];
+  uint32_t* ubuf = ubuf_all[wv];
+  uint32_t* words = words_all[wv];
+  uint32_t* wrank = wrank_all[wv];
+  uint32_t* pre_sh = pre_all[wv];
+  uint32_t* e0_sh = e0_all[wv];
+  const uint32_t nw = (CS + 31) >> 5;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (uint64_t qi = (uint64_t)blockIdx.x * 4 + wv; qi < qn; qi += (uint64_t)gridDim.x * 4) {
+    uint32_t lo[NV], w[NV], blo[NV], bhi[NV];
+    bool empty = false;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      lo[j] = box_lo[(uint64_t)j * qn + qi];
+      w[j] = box_w[(uint64_t)j * qn + qi];
+      // the box test (cell - lo) mod 2^32 <= w admits exactly the cells in [max(lo, 0), lo + w]
+      const int lw = (int)lo[j], h = (int)(lo[j] + w[j]);
+      const int a = lw < 0 ? 0 : lw;
+      if (h < a) empty = true;
+      blo[j] = h < a ? 0u : (uint32_t)((uint64_t)a * B.nb[j] / B.M[j]);
+      bhi[j] = h < a ? 0u : (uint32_t)((uint64_t)h * B.nb[j] / B.M[j]);
+    }
+    uint64_t combos = 1;
+#pragma unroll
+    for (int j = 0; j + 1 < NV; ++j) combos *= (uint64_t)(bhi[j] - blo[j] + 1);
+    bool to_dense = !empty && combos > kMaxCombos;
+    uint32_t ibn = 0;  // in-box points listed so far (order: bucket-row batch, entry slot, lane)
+    if (!empty && !to_dense) {
+      uint64_t tests = 0;
+      for (uint64_t c0 = 0; c0 < combos; c0 += 64) {
+        const uint64_t c = c0 + (uint64_t)lane;
+        uint32_t e0 = 0, len = 0;
+        if (c < combos) {
+          uint64_t rem = c, base = 0;
+#pragma unroll
+          for (int j = NV - 2; j >= 0; --j) {
+            const uint64_t wj = bhi[j] - blo[j] + 1;
+            base += (uint64_t)(blo[j] + (uint32_t)(rem % wj)) * B.stride[j];
+            rem /= wj;
+          }
+          e0 = bstart[base + blo[NV - 1]];
+          len = bstart[base + bhi[NV - 1] + 1] - e0;
+        }
+        const uint32_t incl = wave_incl_scan(len, lane);
+        const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+        pre_sh[lane] = incl - len;
+        e0_sh[lane] = e0;
+        wave_lds_fence();
+        tests += total;
+        if (tests > max_tests) {
+          to_dense = true;
+          break;
+        }
+        // four entries per lane per pass, so their loads are in flight together
+        for (uint32_t t0 = 0; t0 < total; t0 += 256) {
+          uint32_t ev[4], cv[4][NV], sv[4], pv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t t = t0 + (uint32_t)lane + 64u * u;
+            uint32_t a = 0, b = 64;  // last k with pre_sh[k] <= t; that lane's range holds t
+            while (b - a > 1) {
+              const uint32_t m = (a + b) >> 1;
+              if (pre_sh[m] <= t) a = m; else b = m;
+            }
+            ev[u] = t < total ? e0_sh[a] + (t - pre_sh[a]) : kNone;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const bool ok = ev[u] != kNone;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) cv[u][j] = ok ? ecell[(uint64_t)j * N + ev[u]] : lo[j] - 1u;
+            sv[u] = ok ? esub[ev[u]] : 0u;
+            pv[u] = ok ? epos[ev[u]] : 0u;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            bool in = ev[u] != kNone;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) in = in && (cv[u][j] - lo[j] <= w[j]);
+            const uint64_t mask = __ballot(in);
+            const uint32_t at = ibn + (uint32_t)__popcll(mask & lt);
+            ibn += (uint32_t)__popcll(mask);
+            if (in) {
+              if (at < ibcap) {
+                ib_pos[qi * ibcap + at] = pv[u];
+                ib_sub[qi * ibcap + at] = sv[u];
+              }
+              if (at < kWaveIb) sbuf[at] = sv[u];
+            }
+          }
+        }
+        wave_lds_fence();  // pre_sh / e0_sh are rewritten by the next batch of bucket rows
+      }
+    }
+    if (to_dense) {
+      if (lane == 0) {
+        dense_list[atomicAdd(&n_dense[0], 1u)] = (uint32_t)qi;
+        n_ib[qi] = kNone;
+      }
+      continue;
+    }
+    if (ibn > kWaveIb) {
+      if (lane == 0) redo[atomicAdd(&n_dense[1], 1u)] = (uint32_t)qi;
+      continue;
+    }
+    wave_lds_fence();
+    const uint32_t n = ibn;
+    if (n > 1) {
+      if (n <= 64) wave_sort_buf<1>(sbuf, n, lane);
+      else if (n <= 128) wave_sort_buf<2>(sbuf, n, lane);
+      else if (n <= 256) wave_sort_buf<4>(sbuf, n, lane);
+      else wave_sort_buf<8>(sbuf, n, lane);
+      wave_lds_fence();
+    }
+    // runs of the sorted ids: ubuf[u] = the u-th distinct id, sbuf[u] = its first position (written in
+    // place: position u <= e, and a batch reads its elements before writing)
+    uint32_t nu = 0, carry = 0xFFFFFFFFu;
+    for (uint32_t b0 = 0; b0 < n; b0 += 64) {
+      const uint32_t e = b0 + (uint32_t)lane;
+      const uint32_t v = e < n ? sbuf[e] : 0xFFFFFFFFu;
+      uint32_t prev = (uint32_t)__shfl_up((int)v, 1, 64);
+      if (lane == 0) prev = carry;
+      const bool start = e < n && (e == 0 || v != prev);
+      carry = (uint32_t)__shfl((int)v, 63, 64);
+      const uint64_t mask = __ballot(start);
+      const uint32_t u = nu + (uint32_t)__popcll(mask & lt);
+      wave_lds_fence();
+      if (start) {
+        ubuf[u] = v;
+        sbuf[u] = e;
+      }
+      nu += (uint32_t)__popcll(mask);
+      wave_lds_fence();
+    }
+    uint32_t* sl = sub_list + qi * CS;
+    uint32_t* sc = sub_cnt + qi * CS;
+    for (uint32_t u = lane; u < nu; u += 64) {
+      sl[u] = ubuf[u];
+      sc[u] = (u + 1 < nu ? sbuf[u + 1] : n) - sbuf[u];
+    }
+    // bit set of the distinct ids and the rank of each word's first one
+    for (uint32_t wi = lane; wi < nw; wi += 64) words[wi] = 0;
+    wave_lds_fence();
+    for (uint32_t u = lane; u < nu; u += 64) atomicOr(&words[ubuf[u] >> 5], 1u << (ubuf[u] & 31));
+    wave_lds_fence();
+    uint32_t wbase = 0;
+    for (uint32_t w0 = 0; w0 < nw; w0 += 64) {
+      const uint32_t wi = w0 + (uint32_t)lane;
+      const uint32_t word = wi < nw ? words[wi] : 0u;
+      const uint32_t pc = (uint32_t)__popc(word);
+      const uint32_t incl = wave_incl_scan(pc, lane);
+      if (wi < nw) {
+        wrank[wi] = wbase + incl - pc;
+        if (nzb) {
+          nzb[qi * nw + wi] = word;
+          nzp[qi * nw + wi] = wbase + incl - pc;
+        }
+      }
+      wbase += (uint32_t)__shfl((int)incl, 63, 64);
+    }
+    wave_lds_fence();
+    // the non-zero clusters, ascending, each with the rank of its first non-zero sub-cluster
+    uint32_t* lc = list_c + qi * C;
+    uint32_t* cb = cl_beg + qi * (C + 1);
+    uint32_t ncl = 0;
+    for (uint32_t c0 = 0; c0 < C; c0 += 64) {
+      const uint32_t c = c0 + (uint32_t)lane;
+      uint32_t first = kNone;
+      if (c < C && nu > 0) {
+        const uint32_t a = sub_off[c], e = sub_off[c + 1];
+        for (uint32_t x = a; x < e && first == kNone;) {
+          const uint32_t wi = x >> 5, sh = x & 31;
+          const uint32_t span = e - x < 32 - sh ? e - x : 32 - sh;
+          const uint32_t m = (words[wi] >> sh) & (span == 32 ? 0xFFFFFFFFu : ((1u << span) - 1u));
+          if (m) first = x + (uint32_t)(__ffs(m) - 1);
+          x += span;
+        }
+      }
+      const bool nz = first != kNone;
+      const uint64_t mask = __ballot(nz);
+      if (nz) {
+        const uint32_t at = ncl + (uint32_t)__popcll(mask & lt);
+        const uint32_t wi = first >> 5, sh = first & 31;
+        lc[at] = c;
+        cb[at] = wrank[wi] + (uint32_t)__popc(words[wi] & ((1u << sh) - 1u));
+      }
+      ncl += (uint32_t)__popcll(mask);
+    }
+    if (lane == 0) {
+      ns_out[qi] = nu;
+      nnz_c[qi] = ncl;
+      cb[ncl] = nu;
+      n_ib[qi] = ibn;
+      atomicMax(&gstat[((q0 + qi) / G) * 8 + 0], (unsigned long long)ncl);
+    }
+    wave_lds_fence();  // the buffers are reused by the wave's next query
   }
 }
 
@@ -1844,6 +2104,7 @@ __global__ void stats_kernel(uint64_t qn, uint64_t q0, const uint64_t* __restric
 struct Index::Work {
   DevBuf ranges, boxes, box_lo, box_w, zlo, zhi, zlo_cnt, err;
   DevBuf dense_list, n_dense, counts;                        // A3 dense hand-over
+  DevBuf redo;                                               // queries the wave walk hands to the workgroup walk
   DevBuf ib_pos, ib_sub, n_ib;                               // in-box points listed by the walk
   DevBuf sub_list, sub_cnt, ns, list_c, cl_beg, nnz_c;       // non-zero sub-clusters / clusters
   DevBuf chosen_c, n_chosen_c, list_s, list_scnt, nnz_s, chosen_s, n_chosen_s, chosen_sid;
@@ -2128,7 +2389,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.zlo_cnt.reserve(qn * 8);
   w.err.reserve(4);
   w.dense_list.reserve(qn * 4);
-  w.n_dense.reserve(4);
+  w.n_dense.reserve(8);
+  w.redo.reserve(qn * 4);
   w.counts.reserve(qn * CS * 4);
   w.ib_pos.reserve(qn * kInboxCap * 4);
   w.ib_sub.reserve(qn * kInboxCap * 4);
@@ -2237,20 +2499,34 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     const size_t lds = (CS + 2 * nw) * sizeof(uint32_t);
     const uint32_t max_tests = (uint32_t)std::min<uint64_t>(cfg_.sparse_max_tests, 0xFFFFFFF0ull);
     const unsigned grid = (unsigned)std::min<uint64_t>(qn, 2048);
+    const unsigned wgrid = (unsigned)std::min<uint64_t>(cdiv(qn, 4), 2048);
+    const unsigned rgrid = (unsigned)std::min<uint64_t>(qn, 256);  // the wave walk's few hand-overs
     const uint32_t* cells = S.cells.as<uint32_t>();
     dim3 dgrid((unsigned)cdiv(qn, 256), (unsigned)cdiv(CS, kCountSegChunk));
     switch (g.n_valid) {
 #define HDG_A3_CASE(NV)                                                                                            \
   case NV:                                                                                                         \
     if (sparse) {                                                                                                  \
-      hipLaunchKernelGGL(sparse_count_kernel<NV>, dim3(grid), dim3(256), lds, s, B, S.bstart.as<uint32_t>(),        \
+      if (cfg_.select_wave)                                                                                        \
+        hipLaunchKernelGGL(sparse_count_wave_kernel<NV>, dim3(wgrid), dim3(256), 0, s, B, S.bstart.as<uint32_t>(), \
+                           S.ecell.as<uint32_t>(), S.esub.as<uint32_t>(), S.N, S.sub_off.as<uint32_t>(),           \
+                           (uint32_t)CS, (uint32_t)C, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), qn, G, q0,  \
+                           max_tests, S.epos.as<uint32_t>(), kInboxCap, w.ib_pos.as<uint32_t>(),                   \
+                           w.ib_sub.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.sub_list.as<uint32_t>(),              \
+                           w.sub_cnt.as<uint32_t>(), w.ns.as<uint32_t>(), w.list_c.as<uint32_t>(),                 \
+                           w.cl_beg.as<uint32_t>(), w.nnz_c.as<uint32_t>(), w.dense_list.as<uint32_t>(),           \
+                           w.n_dense.as<uint32_t>(), w.gstat.as<unsigned long long>(), nzb, nzp,                   \
+                           w.redo.as<uint32_t>());                                                                 \
+      hipLaunchKernelGGL(sparse_count_kernel<NV>, dim3(cfg_.select_wave ? rgrid : grid), dim3(256), lds, s, B,      \
+                         S.bstart.as<uint32_t>(),                                                                  \
                          S.ecell.as<uint32_t>(), S.esub.as<uint32_t>(), S.N, S.sub_off.as<uint32_t>(),             \
                          (uint32_t)CS, (uint32_t)C, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), qn, G, q0,    \
                          max_tests, S.epos.as<uint32_t>(), kInboxCap, w.ib_pos.as<uint32_t>(),           \
                          w.ib_sub.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.sub_list.as<uint32_t>(), w.sub_cnt.as<uint32_t>(), w.ns.as<uint32_t>(),                 \
                          w.list_c.as<uint32_t>(), w.cl_beg.as<uint32_t>(), w.nnz_c.as<uint32_t>(),                 \
                          w.dense_list.as<uint32_t>(), w.n_dense.as<uint32_t>(), w.gstat.as<unsigned long long>(), \
-                         nzb, nzp);                                                                                \
+                         nzb, nzp, cfg_.select_wave ? w.redo.as<uint32_t>() : nullptr,                            \
+                         cfg_.select_wave ? w.n_dense.as<uint32_t>() + 1 : nullptr);                              \
     } else {                                                                                                       \
       hipLaunchKernelGGL(iota_kernel, dim3((unsigned)cdiv(qn, 256)), dim3(256), 0, s, qn,                          \
                          w.dense_list.as<uint32_t>(), w.n_dense.as<uint32_t>(), w.n_ib.as<uint32_t>());           \
