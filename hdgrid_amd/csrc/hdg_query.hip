@@ -1170,6 +1170,36 @@ __global__ __launch_bounds__(256) void sub_pick_kernel(uint32_t sub_lists, uint6
 // z-pool segment (zorder.cu:656-685).  One workgroup per query, one thread per chosen sub-cluster.
 // Also sums C_q (candidates) and the in-box count of the chosen sub-clusters (A10 inputs), and
 // records the chosen sub-cluster ids (UINT32_MAX past the end).
+// ranges_kernel's in-box branch (mode 3 queries: only the chosen sub-cluster ids and the in-box total) with
+// one wave per query.
+__global__ __launch_bounds__(256) void ranges_inbox_wave_kernel(
+    uint64_t qn, uint32_t cap, const uint32_t* __restrict__ list_s, const uint32_t* __restrict__ list_scnt,
+    uint32_t sub_lists, const uint32_t* __restrict__ chosen_s, const uint32_t* __restrict__ n_chosen_s,
+    uint32_t* __restrict__ chosen_sid, uint32_t* __restrict__ rpre, uint64_t* __restrict__ cand,
+    uint64_t* __restrict__ real, const uint32_t* __restrict__ n_ib) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (qi >= qn || n_ib[qi] == kNone) return;
+  const uint32_t n = n_chosen_s[qi];
+  uint64_t r = 0;
+  for (uint32_t j = lane; j < sub_lists; j += 64) {
+    uint32_t sid = kNone;
+    if (j < n) {
+      const uint32_t idx = chosen_s[qi * sub_lists + j];
+      sid = list_s[qi * cap + idx];
+      r += list_scnt[qi * cap + idx];
+    }
+    chosen_sid[qi * sub_lists + j] = sid;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) r += dev::shfl_xor_u64(r, o);
+  if (lane == 0) {
+    cand[qi] = 0;
+    real[qi] = r;
+    rpre[qi * (sub_lists + 1)] = 0;
+  }
+}
+
 __global__ __launch_bounds__(256) void ranges_kernel(
     const uint64_t* __restrict__ zcode, const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ cells,
     uint64_t N, int n_valid, const uint32_t* __restrict__ box_lo, const uint32_t* __restrict__ box_w, uint64_t qn,
@@ -1186,6 +1216,7 @@ __global__ __launch_bounds__(256) void ranges_kernel(
   __shared__ uint32_t cs_sh[kWG], cp_sh[kWG];
   const uint64_t qi = blockIdx.x;
   const uint32_t n = n_chosen_s[qi];
+  if (inbox_mode == 2 && n_ib[qi] != kNone) return;  // done by ranges_inbox_wave_kernel
   if (inbox_mode && n_ib[qi] != kNone) {
     // refined from its in-box list (mode 3): only the chosen ids and the in-box total are needed
     unsigned long long r = 0;
@@ -2632,13 +2663,18 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   }
   HDG_HIP(hipGetLastError());
   mark();
+  if (inbox_mode && cfg_.select_wave)
+    hipLaunchKernelGGL(ranges_inbox_wave_kernel, dim3((unsigned)cdiv(qn, 4)), dim3(256), 0, s, qn, cap, Ls, Lcnt,
+                       sub_lists, w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.chosen_sid.as<uint32_t>(),
+                       w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.real.as<uint64_t>(), w.n_ib.as<uint32_t>());
   hipLaunchKernelGGL(ranges_kernel, dim3((unsigned)qn), dim3(256), 0, s, S.zcode.as<uint64_t>(),
                      S.seg_off.as<uint32_t>(), S.cells.as<uint32_t>(), S.N, g.n_valid, w.box_lo.as<uint32_t>(),
                      w.box_w.as<uint32_t>(), qn, w.zlo_cnt.as<uint64_t>(), cap, Ls, Lcnt, sub_lists, G,
                      q0, w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.zlo.as<uint64_t>(),
                      w.zhi.as<uint64_t>(), w.chosen_sid.as<uint32_t>(), w.rstart.as<uint32_t>(),
                      w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.real.as<uint64_t>(),
-                     w.gstat.as<unsigned long long>(), w.n_ib.as<uint32_t>(), inbox_mode);
+                     w.gstat.as<unsigned long long>(), w.n_ib.as<uint32_t>(),
+                     inbox_mode && cfg_.select_wave ? 2 : inbox_mode);
   HDG_HIP(hipGetLastError());
   mark();
   hipLaunchKernelGGL(plan_kernel, dim3((unsigned)cdiv(qn, 128)), dim3(128), 0, s, qn, q0, G, l, cfg_.exps0,
