@@ -109,27 +109,43 @@ __global__ __launch_bounds__(256) void infer_kernel(uint64_t qn, int l, const fl
   const int g = threadIdx.x & 15, qg = threadIdx.x >> 4;
   const uint64_t qi = (uint64_t)blockIdx.x * kInferQ + qg;
   const bool live = qi < qn;
-  if (live && g == 0) {
-    float f[kF];
-    query_features(qi, l, ranges, box_w, n_valid, qn, feat_dist, f);
-    for (int i = 0; i < kF; ++i) xs[qg][i] = bf16r((f[i] - mean[i]) * inv_std[i]);
+  if (live) {
+    // the query's 48 features (as query_features), three per lane
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int i = g + 16 * k;
+      float v = 0.0f;
+      if (i < 8) {
+        if (i < l) {
+          float wd = ranges[qi * 2 * l + 2 * i + 1] - ranges[qi * 2 * l + 2 * i];
+          wd = wd > 0.0f ? (wd < 1e12f ? wd : 1e12f) : 0.0f;
+          v = log1pf(wd);
+        }
+      } else if (i < 16) {
+        const int j = i - 8;
+        v = j < n_valid ? log1pf((float)box_w[(uint64_t)j * qn + qi] + 1.0f) : 0.0f;
+      } else {
+        v = feat_dist[qi * kFeatDists + (i - 16)];
+      }
+      xs[qg][i] = bf16r((v - mean[i]) * inv_std[i]);
+    }
   }
   __syncthreads();
   if (live) {
-#pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int j = g * 4 + u;
       float a = w[kOffB1 + j];
+#pragma unroll 8
       for (int i = 0; i < kF; ++i) a += xs[qg][i] * w[kOffW1 + i * kH + j];
       h1s[qg][j] = bf16r(a > 0.0f ? a : 0.0f);
     }
   }
   __syncthreads();
   if (live) {
-#pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int j = g * 4 + u;
       float a = w[kOffB2 + j];
+#pragma unroll 8
       for (int i = 0; i < kH; ++i) a += h1s[qg][i] * w[kOffW2 + i * kH + j];
       h2s[qg][j] = bf16r(a > 0.0f ? a : 0.0f);
     }

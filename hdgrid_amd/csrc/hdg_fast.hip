@@ -360,8 +360,33 @@ void launch_scored_select(const float* queries, uint64_t q0, uint64_t qn, uint64
                      skip_upto, nzb, nzp, nw);
 }
 
+// 8 elements per thread (two float4 loads, one 16-byte store) when d and Dp are multiples of 8
+__global__ void to_bf16_x8_kernel(const float* __restrict__ queries, uint64_t q0, uint64_t qn, uint64_t d,
+                                  uint64_t Dp, uint16_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // 8-element group
+  const uint64_t per = Dp / 8;
+  if (i >= qn * per) return;
+  const uint64_t q = i / per, t = (i % per) * 8;
+  uint32_t pk[4] = {0u, 0u, 0u, 0u};
+  if (t < d) {
+    const float4 a = *reinterpret_cast<const float4*>(queries + (q0 + q) * d + t);
+    const float4 b = *reinterpret_cast<const float4*>(queries + (q0 + q) * d + t + 4);
+    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      pk[k] = (uint32_t)(uint16_t)f32_to_bf16(v[2 * k]) | ((uint32_t)(uint16_t)f32_to_bf16(v[2 * k + 1]) << 16);
+  }
+  *reinterpret_cast<uint4*>(out + q * Dp + t) = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+}
+
 void launch_to_bf16(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, uint64_t Dp, uint16_t* out,
                     hipStream_t s) {
+  if (d % 8 == 0 && Dp % 8 == 0) {
+    const uint64_t n = qn * (Dp / 8);
+    hipLaunchKernelGGL(to_bf16_x8_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, queries, q0, qn, d, Dp,
+                       out);
+    return;
+  }
   const uint64_t n = qn * Dp;
   hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, queries, q0, qn, d, Dp, out);
 }
