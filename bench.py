@@ -152,6 +152,9 @@ def main():
     ap.add_argument("--batched-nq", type=int, default=20_000)
     ap.add_argument("--save-budget", default=None, help="write the trained budget model here")
     ap.add_argument("--load-budget", default=None, help="use this budget model instead of training one")
+    ap.add_argument("--budget-levels", type=int, default=20, help="budget ladder levels")
+    ap.add_argument("--budget-s0", type=float, default=16.0, help="sub-cluster budget of the lowest level")
+    ap.add_argument("--budget-ratio", type=float, default=2 ** 0.25, help="budget growth per level")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
@@ -265,8 +268,9 @@ def main():
             _, tgt = O.bruteforce(w.data, w.labels, tq, tr, 100)
             log(f"training set: {args.train} queries, ground truth in {time.time() - t0:.1f}s")
             t0 = time.time()
-            rep = idx.budget_fit(torch.from_numpy(tq).to(dev), torch.from_numpy(tql).to(dev), tgt, levels=12,
-                                 t0=0, s0=8, ratio=2 ** 0.5, target=args.target_recall, epochs=30)
+            rep = idx.budget_fit(torch.from_numpy(tq).to(dev), torch.from_numpy(tql).to(dev), tgt,
+                                 levels=args.budget_levels, t0=0, s0=args.budget_s0, ratio=args.budget_ratio,
+                                 target=args.target_recall, epochs=30)
             log(f"budget model trained in {time.time() - t0:.1f}s: {rep}")
             if args.save_budget:
                 idx.budget_save(args.save_budget)
