@@ -1640,7 +1640,9 @@ __device__ __forceinline__ float adc_word(float sum, uint32_t wv, const LT* lr, 
   return sum;
 }
 
-template <typename LT, int PER, int KC, bool STREAM>
+// STREAM: 0 = per-thread column state over LUT phases; 32 / 8 = streaming, code rows read in steps of 32 codes
+// (16-byte loads, pq_dim % 32 == 0) or 8 codes (8-byte loads, pq_dim % 8 == 0).
+template <typename LT, int PER, int KC, int STREAM>
 __global__ __launch_bounds__(256) void adc_scan_kernel(
     uint64_t qn, uint64_t q0, const float* __restrict__ queries, uint64_t d, uint64_t M, uint64_t K_rt,
     uint64_t pq_len, const float* __restrict__ pq_centers, const uint8_t* __restrict__ codes,
@@ -1711,9 +1713,9 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
     }
     cur = qi;
     __syncthreads();
-    if constexpr (STREAM) {
-      // STREAM (the host picks it when the LUT fits one phase and M % 32 == 0): groups of 4 of this thread's columns, each finished (all M codes) before the next, so
-      // no per-column state stays live and occupancy stays high
+    if constexpr (STREAM != 0) {
+      // streaming (the host picks it when the LUT fits one phase): groups of 4 of this thread's columns, each
+      // finished (all M codes) before the next, so no per-column state stays live and occupancy stays high
       uint32_t j = 0;
       if (!from_list) {
         const uint32_t col = c0 + threadIdx.x;
@@ -1743,6 +1745,28 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
           }
         }
         if (c0 + threadIdx.x + (uint32_t)i0 * kWG >= c1) break;  // the rest of this thread's columns too
+        if constexpr (STREAM == 8) {
+          // a code row of up to 64 codes as 8-byte words, all in flight together; the sum keeps the order over m
+          uint2 v[4][8];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t col = c0 + threadIdx.x + (uint32_t)(i0 + u) * kWG;
+            const uint2* cd = reinterpret_cast<const uint2*>(codes + (uint64_t)p4[u] * M);
+            const bool ok = col < c1;
+#pragma unroll
+            for (int s8 = 0; s8 < 8; ++s8) v[u][s8] = ok && (uint64_t)s8 * 8 < M ? cd[s8] : make_uint2(0, 0);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int s8 = 0; s8 < 8; ++s8) {
+              if ((uint64_t)s8 * 8 >= M) break;
+              const LT* lr = lut + (uint64_t)s8 * 8 * K;
+              sum[u] = adc_word(sum[u], v[u][s8].x, lr, K);
+              sum[u] = adc_word(sum[u], v[u][s8].y, lr + 4 * K, K);
+            }
+          }
+        } else {
 #pragma unroll 1
         for (uint64_t mb = 0; mb < M; mb += 32) {
           uint4 v[4][2];
@@ -1766,6 +1790,7 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
               sum[u] = adc_word(sum[u], v[u][h].w, lr + 12 * K, K);
             }
           }
+        }
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -2743,7 +2768,12 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     // the streaming variant needs the whole LUT in one LDS phase and 32-code steps
     uint64_t mp_h = kLutPhaseBytes / (S.K * lut_entry);
     if (mp_h > 4) mp_h &= ~3ull;
-    const bool stream = mp_h >= S.M && (S.M % 32) == 0 && (mp_h % 16) == 0;
+    // streaming variants: the whole LUT in one phase; 32-code steps (16-byte loads) or, for code rows of at
+    // most 64 codes with pq_dim % 8 == 0, 8-code steps (8-byte loads)
+    const int stream = mp_h < S.M || (mp_h % 16) != 0 ? 0
+                       : (S.M % 32) == 0                ? 32
+                       : ((S.M % 8) == 0 && S.M <= 64) ? 8
+                                                        : 0;
 #define HDG_ADC1(LT, PER, KC, ST)                                                                                 \
   hipLaunchKernelGGL((adc_scan_kernel<LT, PER, KC, ST>), dim3(adc_grid), dim3(256), adc_lds, s, qn, q0, queries, d,  \
                      S.M, S.K, S.pq_len, S.pq_centers.as<float>(), S.codes.as<uint8_t>(), sub_lists,                 \
@@ -2753,10 +2783,12 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      lut_bytes, abort_p)
 #define HDG_ADC(LT, PER)                                                                                          \
   do {                                                                                                            \
-    if (S.K == 256 && stream) HDG_ADC1(LT, PER, 256, true);                                                       \
-    else if (S.K == 256) HDG_ADC1(LT, PER, 256, false);                                                           \
-    else if (stream) HDG_ADC1(LT, PER, 0, true);                                                                  \
-    else HDG_ADC1(LT, PER, 0, false);                                                                             \
+    if (S.K == 256 && stream == 32) HDG_ADC1(LT, PER, 256, 32);                                                   \
+    else if (S.K == 256 && stream == 8) HDG_ADC1(LT, PER, 256, 8);                                                \
+    else if (S.K == 256) HDG_ADC1(LT, PER, 256, 0);                                                               \
+    else if (stream == 32) HDG_ADC1(LT, PER, 0, 32);                                                              \
+    else if (stream == 8) HDG_ADC1(LT, PER, 0, 8);                                                                \
+    else HDG_ADC1(LT, PER, 0, 0);                                                                                 \
   } while (0)
     switch (adc_per) {
       case 8:

@@ -186,6 +186,39 @@ def explicit_ranges(n: int = 200_000, nq: int = 2000, d: int = 128, components: 
     return Workload("explicit-ranges", conf, filt, data, labels, queries, qlabels, fa)
 
 
+def full_pipeline(n: int = 50_000_000, nq: int = 100_000, d: int = 96, components: int = 4096,
+                  seed: int = 0x4D6005, clusters: int = 512, sub_clusters: int = 10, n_list: int = 512,
+                  sub_lists: int = 128) -> Workload:
+    """50M x 96 f32 mixture (4096 components, means N(0,1), sigma 0.3); four label dims: timestamp
+    U[0, 31,536,000) with shift [604800, 0]; length U[0, 2000] with shift [30, 30]; category 1..5 with
+    interval_map [1,2,1,2,3,4,3,5,4,5]; u U[0, 1e6] with div 10 (range [(int)x / 10, FLT_MAX]).
+    MEM_BOUND 200e9, n_list 512, sub_lists 128, PQ_DIM 24 (4 dims per subvector)."""
+    rng = np.random.default_rng(seed)
+    means = rng.standard_normal((components, d), dtype=np.float32)
+    data = gaussian_mixture(rng, n, d, means, 0.3)
+
+    def labs(r, m):
+        ts = r.integers(0, 31_536_000, size=m).astype(np.float32)
+        ln = r.integers(0, 2001, size=m).astype(np.float32)
+        cat = r.integers(1, 6, size=m).astype(np.float32)
+        u = r.integers(0, 1_000_001, size=m).astype(np.float32)
+        return np.stack([ts, ln, cat, u], axis=1)
+
+    labels = labs(rng, n)
+    qrng = np.random.default_rng(seed + 1)
+    queries = gaussian_mixture(qrng, nq, d, means, 0.3)
+    qlabels = labs(qrng, nq)
+    conf = dict(BASE_CONF, PQ_DIM=max(1, d // 4), N_LIST=n_list, SUB_LISTS=sub_lists, CLUSTERS=clusters,
+                SUB_CLUSTERS=sub_clusters, MEM_BOUND=200_000_000_000)
+    filt = ("l = 4\nfilter0:\n\ttype = 0\n\tshift_val = [604800, 0]\n\n"
+            "filter1:\n\ttype = 0\n\tshift_val = [30, 30]\n\n"
+            "filter2:\n\ttype = 1\n\tinterval_map = [1, 2, 1, 2, 3, 4, 3, 5, 4, 5]\n\n"
+            "filter3:\n\ttype = 2\n\tdiv_value = [10]\n")
+    fa = dict(ftype=[0, 0, 1, 2], shift_val=[604800.0, 0.0, 30.0, 30.0, 0.0, 0.0, 0.0, 0.0],
+              imap=[1.0, 2.0, 1.0, 2.0, 3.0, 4.0, 3.0, 5.0, 4.0, 5.0], div_value=[0, 0, 0, 10])
+    return Workload("full-50m-d96-l4", conf, filt, data, labels, queries, qlabels, fa)
+
+
 def recall(gt_ids: np.ndarray, res_ids: np.ndarray, k: int) -> float:
     """compute_recall (verification.cuh:97-153) without the app_reviews_points side file: ground-truth
     ids that are invalid (UINT64_MAX / negative) or repeated do not count; a result id counts once."""

@@ -178,6 +178,17 @@ def test_l2_explicit_adc(l2_explicit):
     assert_parity(res, "explicit ranges exps0=2")
 
 
+@pytest.mark.parametrize("d,pq_dim", [(128, 32), (96, 24), (48, 12)])
+def test_adc_code_widths(d, pq_dim):
+    """ADC parity for every scan variant: 32-code steps (PQ_DIM 32), 8-code steps (24), per-thread state
+    (12, not a multiple of 8)."""
+    w = datasets.explicit_ranges(n=20000, nq=120, d=d, components=32, clusters=32, sub_clusters=6)
+    b = Built(w, PQ_DIM=pq_dim)
+    res = b.run(EXPS0=2)
+    assert (res[2].adc > 0).any(), "scenario must exercise the ADC path"
+    assert_parity(res, f"explicit ranges d={d} pq_dim={pq_dim}")
+
+
 def test_l3_dense_count(l3):
     """Every query through the dense Morton-range count instead of the bucket-grid walk."""
     assert_parity(l3.run(EXPS0=30, SPARSE_COUNT=0), "l3 dense count")
