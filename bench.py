@@ -121,7 +121,7 @@ def batched_leg(args, H, datasets, O, dev, log):
     idx.set_param("GRAPH", 0)
     out["mixed_graph"] = {"qps": round(nq / el, 1), "ms_per_call": round(el * 1e3, 3),
                           "tuned": {k: idx.get_param(k) for k in ("ADC_PER", "ADC_WGS", "SPARSE_MAX_TESTS",
-                                                                   "QUERY_CHUNK", "SCORE_BLOCK")}}
+                                                                   "QUERY_CHUNK", "SCORE_BLOCK", "ADC_SPLIT")}}
     log(f"batched mixed+graph: {el * 1e3:.2f} ms/call ({nq / el:.0f} QPS)")
     adc_ms, adc_bytes, gbs = best
     out["adc_roofline"] = {"kernel": "adc_scan_kernel (plain FP32 LUT)", "bound": "hbm", "achieved": round(gbs, 3),

@@ -296,8 +296,8 @@ class SuperGridIndex:
         return out
 
     def autotune(self, queries, query_labels, reps: int = 3, stream=None) -> str:
-        """Tune ADC_PER, ADC_WGS, SPARSE_MAX_TESTS, QUERY_CHUNK, SCORE_BLOCK and GRAPH on this batch (results unchanged
-        bit for bit); the chosen values stay set.  Returns the trial log."""
+        """Tune ADC_PER, ADC_WGS, SPARSE_MAX_TESTS, QUERY_CHUNK, SCORE_BLOCK, ADC_SPLIT and GRAPH on this batch
+        (results unchanged bit for bit); the chosen values stay set.  Returns the trial log."""
         import torch
         queries = queries.contiguous()
         query_labels = query_labels.contiguous()

@@ -81,6 +81,7 @@ std::string Index::autotune(const float* queries, uint64_t nq, uint64_t d, const
         {"SPARSE_MAX_TESTS", {16384, 32768, 65536, 131072, 262144, 524288}},
         {"QUERY_CHUNK", {}},
         {"SCORE_BLOCK", {512, 1024, 2048, 4096, 16384}},
+        {"ADC_SPLIT", {0, 16384, 65536, 262144}},
         {"GRAPH", {0, 1}},
     };
     for (uint64_t c = 2048; c <= 65536 && c <= chunk_cap * 2; c *= 2) knobs[3].values.push_back(c);
@@ -113,7 +114,7 @@ std::string Index::autotune(const float* queries, uint64_t nq, uint64_t d, const
   (void)hipEventDestroy(e1);
   std::snprintf(line, sizeof line, "best %.4f ms:", best);
   report += line;
-  for (const char* key : {"ADC_PER", "ADC_WGS", "SPARSE_MAX_TESTS", "QUERY_CHUNK", "SCORE_BLOCK", "GRAPH"}) {
+  for (const char* key : {"ADC_PER", "ADC_WGS", "SPARSE_MAX_TESTS", "QUERY_CHUNK", "SCORE_BLOCK", "ADC_SPLIT", "GRAPH"}) {
     std::snprintf(line, sizeof line, " %s=%llu", key, (unsigned long long)get(key));
     report += line;
   }

@@ -56,6 +56,8 @@ struct Config {
   uint64_t select_wave = 1;            // SELECT_WAVE: 1 = one wave per query for scored selection, in-box keep
                                        // and the final top-k on lists of <= 512 items (same results)
   uint64_t score_block = 65536;        // SCORE_BLOCK: queries per BF16 scoring block (PRECISION 1; min 128)
+  uint64_t adc_split = 65536;          // ADC_SPLIT: ADC candidates from which a query's top-k' selection runs
+                                       // over many workgroups (0 = never; same results)
 };
 
 // Parse with config_utils.h:33-88 semantics: `KEY = value` per line, key case-insensitive and

@@ -185,6 +185,30 @@ struct KeyDist {
   __device__ __forceinline__ uint64_t operator()(uint32_t i) const { return ((uint64_t)d[i] << 32) | i; }
 };
 
+// Bin holding the need-th smallest item (1-based, need <= total) of a 2048-bin LDS histogram, by an
+// exclusive scan over 8 bins per thread: sh[0] = the bin, sh[1] = items in lower bins.  Ends with a barrier.
+__device__ inline void wg_hist_find(const uint32_t* hist, uint32_t need, uint64_t* sh) {
+  __shared__ uint32_t tmp[kWG / 64 + 1];
+  static_assert(kWG * 8 == 2048, "8 bins per thread");
+  const uint4 a = reinterpret_cast<const uint4*>(hist)[threadIdx.x * 2];
+  const uint4 b = reinterpret_cast<const uint4*>(hist)[threadIdx.x * 2 + 1];
+  const uint32_t c8[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  uint32_t s8 = 0;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) s8 += c8[t];
+  uint32_t total;
+  uint32_t before = wg_exclusive_scan(s8, tmp, total);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    if (before < need && before + c8[t] >= need) {
+      sh[0] = threadIdx.x * 8 + t;
+      sh[1] = before;
+    }
+    before += c8[t];
+  }
+  __syncthreads();
+}
+
 template <class Src>
 __device__ inline void wg_radix_select(Src g, uint32_t n, uint32_t k, uint64_t* __restrict__ out, uint32_t* hist,
                                        uint64_t* sh) {
@@ -192,6 +216,7 @@ __device__ inline void wg_radix_select(Src g, uint32_t n, uint32_t k, uint64_t* 
   uint32_t need = k;  // rank (1-based) of the k-th key inside the current prefix bucket
   const int shifts[6] = {53, 42, 31, 20, 9, 0};
   const int widths[6] = {11, 11, 11, 11, 11, 9};
+#pragma unroll 1
   for (int pass = 0; pass < 6; ++pass) {
     const int sft = shifts[pass];
     const uint64_t dmask = (1ull << widths[pass]) - 1;
@@ -202,16 +227,7 @@ __device__ inline void wg_radix_select(Src g, uint32_t n, uint32_t k, uint64_t* 
       if ((key & mask) == prefix) atomicAdd(&hist[(key >> sft) & dmask], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t cum = 0, b = 0;
-      for (b = 0; b <= dmask; ++b) {
-        if (cum + hist[b] >= need) break;
-        cum += hist[b];
-      }
-      sh[0] = b;
-      sh[1] = cum;
-    }
-    __syncthreads();
+    wg_hist_find(hist, need, sh);
     const uint64_t b = sh[0];
     need -= (uint32_t)sh[1];
     prefix |= b << sft;
@@ -232,71 +248,76 @@ __device__ inline void wg_radix_select(Src g, uint32_t n, uint32_t k, uint64_t* 
   __syncthreads();
 }
 
-// Radix selection with compaction: like wg_radix_select, but after the two leading 11-bit passes the keys
-// below the k-th key's bucket are written to out[] at once and the keys inside that bucket are copied to
-// scratch[] (capacity n), so the four trailing passes read only that bucket.  Three passes over g[].
-template <class Src>
-__device__ inline void wg_radix_select_compact(Src g, uint32_t n, uint32_t k, uint64_t* __restrict__ out,
-                                               uint64_t* __restrict__ scratch, uint32_t* hist, uint64_t* sh) {
-  uint64_t prefix = 0, mask = 0;
-  uint32_t need = k;
-  const int shifts[6] = {53, 42, 31, 20, 9, 0};
-  const int widths[6] = {11, 11, 11, 11, 11, 9};
-  uint32_t* cnt = reinterpret_cast<uint32_t*>(&sh[2]);  // [0] selected so far, [1] bucket size
-  for (int pass = 0; pass < 6; ++pass) {
-    if (pass == 2) {
-      // split: below the bucket -> out, inside -> scratch
-      if (threadIdx.x == 0) {
-        cnt[0] = 0;
-        cnt[1] = 0;
-      }
-      __syncthreads();
-      for (uint32_t i = threadIdx.x; i < n; i += kWG) {
-        const uint64_t key = g(i);
-        const uint64_t top = key & mask;
-        if (top < prefix) {
-          const uint32_t s = atomicAdd(&cnt[0], 1u);
-          if (s < k) out[s] = key;
-        } else if (top == prefix) {
-          scratch[atomicAdd(&cnt[1], 1u)] = key;
-        }
-      }
-      __syncthreads();
+// Bin shift for distance bits in [mn, mx]: (bits - mn) >> shift stays below 2048 and is monotone in the
+// distance.
+__device__ __forceinline__ uint32_t dist_bin_shift(uint32_t mn, uint32_t mx) {
+  const uint32_t r = mx - mn;
+  const int bits = r ? 32 - __clz((int)r) : 0;
+  return bits > 11 ? (uint32_t)(bits - 11) : 0u;
+}
+
+// Distance items per thread kept in flight by the selection passes.
+constexpr int kSelUnroll = 8;
+
+// Selection of the k smallest keys (d[i] << 32 | i) of a distance array d[0..n) (non-negative f32 bits, all
+// in [mn, mx]) by binning: one histogram pass into the 2048 bins of dist_bin_shift, one split pass (keys in
+// bins below the k-th key's bin go to out[], keys in that bin to scratch[]), then a radix selection inside
+// that one bin.  Bins are monotone in the distance, so out[0..k) holds exactly the k smallest keys
+// (unordered), ties by index.  Requires k < n; hist = 2048 words of LDS, sh = 8 words of LDS, scratch = n
+// keys.  U distance loads per thread stay in flight.
+template <int U>
+__device__ inline void wg_dist_select(const uint32_t* __restrict__ d, uint32_t n, uint32_t k, uint32_t mn,
+                                      uint32_t mx, uint64_t* __restrict__ out, uint64_t* __restrict__ scratch,
+                                      uint32_t* hist, uint64_t* sh) {
+  constexpr uint32_t kStep = kWG * U;
+  const uint32_t sft = dist_bin_shift(mn, mx);
+  for (int i = threadIdx.x; i < 2048; i += kWG) hist[i] = 0;
+  __syncthreads();
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += kStep) {
+    uint32_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * kWG;
+      v[u] = i < n ? d[i] : 0u;
     }
-    const uint32_t len = pass < 2 ? n : cnt[1];
-    const int sft = shifts[pass];
-    const uint64_t dmask = (1ull << widths[pass]) - 1;
-    for (int i = threadIdx.x; i < 2048; i += kWG) hist[i] = 0;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < len; i += kWG) {
-      const uint64_t key = pass < 2 ? g(i) : scratch[i];
-      if ((key & mask) == prefix) atomicAdd(&hist[(key >> sft) & dmask], 1u);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t cum = 0, b = 0;
-      for (b = 0; b <= dmask; ++b) {
-        if (cum + hist[b] >= need) break;
-        cum += hist[b];
-      }
-      sh[0] = b;
-      sh[1] = cum;
-    }
-    __syncthreads();
-    const uint64_t b = sh[0];
-    need -= (uint32_t)sh[1];
-    prefix |= b << sft;
-    mask |= dmask << sft;
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + (uint32_t)u * kWG < n) atomicAdd(&hist[(v[u] - mn) >> sft], 1u);
   }
-  // prefix is the k-th smallest key: the bucket's keys <= prefix complete out[]
-  const uint32_t len = cnt[1];
-  for (uint32_t i = threadIdx.x; i < len; i += kWG) {
-    const uint64_t key = scratch[i];
-    if (key <= prefix) {
-      const uint32_t s = atomicAdd(&cnt[0], 1u);
-      if (s < k) out[s] = key;
+  __syncthreads();
+  wg_hist_find(hist, k, sh);  // sh[0] = the k-th key's bin, sh[1] = keys below it
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(&sh[2]);  // [0] keys below, [1] keys in the bin
+  if (threadIdx.x == 0) {
+    cnt[0] = 0;
+    cnt[1] = 0;
+  }
+  __syncthreads();
+  const uint32_t bstar = (uint32_t)sh[0], below = (uint32_t)sh[1];
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += kStep) {
+    uint32_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * kWG;
+      v[u] = i < n ? d[i] : 0u;
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * kWG;
+      if (i < n) {
+        const uint32_t b = (v[u] - mn) >> sft;
+        const uint64_t key = ((uint64_t)v[u] << 32) | i;
+        if (b < bstar) out[atomicAdd(&cnt[0], 1u)] = key;
+        else if (b == bstar) scratch[atomicAdd(&cnt[1], 1u)] = key;
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t m = cnt[1], need = k - below;
+  __syncthreads();
+  if (need == m) {
+    for (uint32_t i = threadIdx.x; i < m; i += kWG) out[below + i] = scratch[i];
+  } else {
+    wg_radix_select(KeyArray{scratch}, m, need, out + below, hist, sh);
   }
   __syncthreads();
 }

@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
                                                             int64_t skip_upto, const uint32_t* __restrict__ nzb,
                                                             const uint32_t* __restrict__ nzp, uint32_t nw) {
   extern __shared__ __align__(16) uint64_t keys[];  // scap keys
-  __shared__ uint32_t hist[2048];
+  __shared__ __align__(16) uint32_t hist[2048];
   __shared__ uint64_t sh[8];
   __shared__ float qq_sh;
   const uint64_t qi = blockIdx.x;

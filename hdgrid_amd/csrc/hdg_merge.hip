@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void merge_topk_kernel(const float* __restrict
                                                          float* __restrict__ out_dist,
                                                          uint64_t* __restrict__ out_ids) {
   __shared__ uint64_t keys[kMergeCap];
-  __shared__ uint32_t hist[2048];
+  __shared__ __align__(16) uint32_t hist[2048];
   __shared__ uint64_t sh[8];
   const uint64_t q = blockIdx.x;
   const uint32_t n = n_batches * k;

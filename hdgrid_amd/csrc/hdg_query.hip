@@ -78,6 +78,7 @@ __global__ void clear_kernel(unsigned long long* __restrict__ gstat, uint64_t n,
     *err = 0;
     n_dense[0] = 0;  // queries handed to the dense count
     n_dense[1] = 0;  // queries handed from the wave walk to the workgroup walk
+    n_dense[2] = 0;  // selection slots handed out by the plan
     if (abort_flag) *abort_flag = 0;
   }
 }
@@ -1072,7 +1073,7 @@ __global__ __launch_bounds__(256) void coarse_pick_kernel(uint32_t C, uint32_t n
                                                           uint32_t* __restrict__ n_chosen_c) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t* sorted = reinterpret_cast<uint64_t*>(smem);
-  __shared__ uint32_t hist[2048];
+  __shared__ __align__(16) uint32_t hist[2048];
   __shared__ uint64_t sh[8];
   const uint64_t qi = blockIdx.x;
   const uint32_t n = nnz_c[qi];
@@ -1146,7 +1147,7 @@ __global__ __launch_bounds__(256) void sub_pick_kernel(uint32_t sub_lists, uint6
                                                        uint32_t* __restrict__ n_chosen_s) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t* sorted = reinterpret_cast<uint64_t*>(smem);
-  __shared__ uint32_t hist[2048];
+  __shared__ __align__(16) uint32_t hist[2048];
   __shared__ uint64_t sh[8];
   const uint64_t qi = blockIdx.x;
   const uint32_t n = nnz_s[qi];
@@ -1348,15 +1349,32 @@ __device__ __forceinline__ uint64_t f2u_host_dev(float x) {
 // A10 + buffer plan, one thread per query.  mode 0: every candidate reaches the next stage (the PQ
 // top-k' keeps them all); mode 1: ADC scan + top-k'.  lsize = entries of the refine list,
 // asize = ADC keys, chunks = adc_scan blocks.
+// A11 selection over many workgroups.  A mode-1 query with at least ADC_SPLIT ADC candidates gets a slot
+// (kSelSlots per chunk).  The scan records the range [mn, mx] of its distance bits; bins are those of
+// dev::dist_bin_shift (monotone in the distance).  Workgroups of kSelTile candidates add their LDS histograms into the slot's global histogram,
+// then split the candidates into keys below the k'-th key's bin (final) and keys in that bin (scratch); the
+// per-query selection finally ranks the keys of that one bin.
+constexpr uint32_t kSelTile = 16384;
+constexpr uint32_t kSelSlots = 8192;
+constexpr uint32_t kSelBins = 2048;
+
 __global__ void plan_kernel(uint64_t qn, uint64_t q0, uint64_t G, int l, uint64_t exps0, uint64_t topk,
                             const uint64_t* __restrict__ cand, const unsigned long long* __restrict__ gstat,
                             uint32_t* __restrict__ mode, uint64_t* __restrict__ kprime,
                             uint64_t* __restrict__ lsize, uint64_t* __restrict__ asize,
                             uint64_t* __restrict__ chunks, uint64_t* __restrict__ pad32,
                             uint64_t* __restrict__ padmax, const uint32_t* __restrict__ n_ib, int inbox_mode,
-                            uint32_t adc_chunk) {
+                            uint32_t adc_chunk, uint32_t* __restrict__ sslot, uint32_t* __restrict__ smm,
+                            uint32_t* __restrict__ scnt, uint32_t* __restrict__ slot_ctr, uint32_t slot_cap,
+                            uint64_t split_min) {
   uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (qi >= qn) return;
+  sslot[qi] = kNone;
+  smm[2 * qi] = 0xFFFFFFFFu;  // distance-bit range of the ADC scan, filled by the scan
+  smm[2 * qi + 1] = 0u;
+  scnt[4 * qi] = 0u;
+  scnt[4 * qi + 1] = 0u;
+  scnt[4 * qi + 2] = 0u;
   if (inbox_mode && n_ib[qi] != kNone) {
     // mode 3 (mixed precision, walked query): exact re-rank of the chosen sub-clusters' in-box points that
     // pass the label predicate
@@ -1404,6 +1422,10 @@ __global__ void plan_kernel(uint64_t qn, uint64_t q0, uint64_t G, int l, uint64_
   lsize[qi] = adc ? kp : Cq;
   asize[qi] = adc ? Cq : 0;
   chunks[qi] = adc ? cdiv(Cq, adc_chunk) : 0;
+  if (adc && split_min && Cq >= split_min) {
+    const uint32_t sl = atomicAdd(slot_ctr, 1u);
+    if (sl < slot_cap) sslot[qi] = sl;  // beyond the slots: the one-workgroup selection
+  }
   pad32[qi] = p32;
   padmax[qi] = pmax;  // l > 1: set after the post-filter (n_candies - survivors)
 }
@@ -1640,6 +1662,20 @@ __device__ __forceinline__ float adc_word(float sum, uint32_t wv, const LT* lr, 
   return sum;
 }
 
+// Range of a chunk's distance bits: wave reduction, then one atomic min / max per wave.
+__device__ __forceinline__ void sel_range_add(uint32_t* mm, uint32_t mn, uint32_t mx) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t a = (uint32_t)__shfl_xor((int)mn, o, 64), b = (uint32_t)__shfl_xor((int)mx, o, 64);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+  }
+  if ((threadIdx.x & 63) == 0 && mn <= mx) {
+    atomicMin(&mm[0], mn);
+    atomicMax(&mm[1], mx);
+  }
+}
+
 // STREAM: 0 = per-thread column state over LUT phases; 32 / 8 = streaming, code rows read in steps of 32 codes
 // (16-byte loads, pq_dim % 32 == 0) or 8 codes (8-byte loads, pq_dim % 8 == 0).
 template <typename LT, int PER, int KC, int STREAM>
@@ -1650,7 +1686,7 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
     const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ rpre, const uint64_t* __restrict__ cand,
     const uint32_t* __restrict__ L, const uint64_t* __restrict__ loff, const uint32_t* __restrict__ nL,
     const uint64_t* __restrict__ coff, const uint64_t* __restrict__ aoff, uint32_t* __restrict__ adist,
-    uint32_t lut_bytes, const uint32_t* __restrict__ abort_flag) {
+    uint32_t lut_bytes, uint32_t* __restrict__ smm, const uint32_t* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint64_t K = KC ? (uint64_t)KC : K_rt;  // codebook size; a compile-time 256 folds the LUT row offsets
@@ -1726,6 +1762,7 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
         }
       }
       uint32_t* out = adist + aoff[qi];
+      uint32_t tmn = 0xFFFFFFFFu, tmx = 0u;
 #pragma unroll 1
       for (int i0 = 0; i0 < PER; i0 += 4) {
         uint32_t p4[4];
@@ -1795,9 +1832,15 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const uint32_t col = c0 + threadIdx.x + (uint32_t)(i0 + u) * kWG;
-          if (col < c1) out[col] = __float_as_uint(sum[u]);
+          if (col < c1) {
+            const uint32_t b = __float_as_uint(sum[u]);
+            out[col] = b;
+            tmn = b < tmn ? b : tmn;
+            tmx = b > tmx ? b : tmx;
+          }
         }
       }
+      if (!from_list) sel_range_add(smm + 2 * qi, tmn, tmx);
       continue;
     } else {
     uint32_t pos[PER];
@@ -1905,17 +1948,244 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
       }
     }
     uint32_t* out = adist + aoff[qi];
+    uint32_t tmn = 0xFFFFFFFFu, tmx = 0u;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const uint32_t col = c0 + threadIdx.x + (uint32_t)i * kWG;
-      if (col < c1) out[col] = __float_as_uint(part[i]);
+      if (col < c1) {
+        const uint32_t b = __float_as_uint(part[i]);
+        out[col] = b;
+        tmn = b < tmn ? b : tmn;
+        tmx = b > tmx ? b : tmx;
+      }
     }
+    if (!from_list) sel_range_add(smm + 2 * qi, tmn, tmx);
     }
   }
 }
 
+// The tile of this workgroup over the concatenated ADC distances, cut at query boundaries; f(qi, a, e) runs
+// for the candidates [a, e) of each query with a selection slot (uniform over the workgroup).
+template <class F>
+__device__ inline void for_each_sel_segment(uint64_t qn, const uint64_t* __restrict__ aoff,
+                                            const uint32_t* __restrict__ sslot, F f) {
+  const uint64_t total = aoff[qn];
+  const uint64_t t0 = (uint64_t)blockIdx.x * kSelTile;
+  if (t0 >= total) return;
+  const uint64_t t1 = t0 + kSelTile < total ? t0 + kSelTile : total;
+  uint64_t lo = 0, hi = qn;  // last q with aoff[q] <= t0
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (aoff[mid] <= t0) lo = mid; else hi = mid;
+  }
+  uint64_t qi = lo;
+  for (uint64_t x = t0; x < t1;) {
+    while (aoff[qi + 1] <= x) ++qi;
+    const uint64_t e = aoff[qi + 1] < t1 ? aoff[qi + 1] : t1;
+    if (sslot[qi] != kNone) f(qi, (uint32_t)(x - aoff[qi]), (uint32_t)(e - aoff[qi]));
+    x = e;
+  }
+}
+
+// Histogram pass of the many-workgroup selection: LDS counts of a tile, added into the slot's histogram.
+__global__ __launch_bounds__(256) void sel_hist_kernel(uint64_t qn, const uint64_t* __restrict__ aoff,
+                                                       const uint32_t* __restrict__ sslot,
+                                                       const uint32_t* __restrict__ smm,
+                                                       const uint32_t* __restrict__ adist, uint32_t* __restrict__ ghist,
+                                                       const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
+  __shared__ __align__(16) uint32_t hist[kSelBins];
+  constexpr uint32_t kStep = kWG * dev::kSelUnroll;
+  for_each_sel_segment(qn, aoff, sslot, [&](uint64_t qi, uint32_t a, uint32_t e) {
+    const uint32_t mn = smm[2 * qi], sft = dev::dist_bin_shift(mn, smm[2 * qi + 1]);
+    for (uint32_t i = threadIdx.x; i < kSelBins; i += kWG) hist[i] = 0;
+    __syncthreads();
+    const uint32_t* d = adist + aoff[qi];
+    for (uint32_t i0 = a + threadIdx.x; i0 < e; i0 += kStep) {
+      uint32_t v[dev::kSelUnroll];
+#pragma unroll
+      for (int u = 0; u < dev::kSelUnroll; ++u) {
+        const uint32_t i = i0 + (uint32_t)u * kWG;
+        v[u] = i < e ? d[i] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < dev::kSelUnroll; ++u)
+        if (i0 + (uint32_t)u * kWG < e) atomicAdd(&hist[(v[u] - mn) >> sft], 1u);
+    }
+    __syncthreads();
+    uint32_t* g = ghist + (uint64_t)sslot[qi] * kSelBins;
+    for (uint32_t i = threadIdx.x; i < kSelBins; i += kWG) {
+      const uint32_t c = hist[i];
+      if (c) atomicAdd(&g[i], c);
+    }
+    __syncthreads();
+  });
+}
+
+// Items per thread and step of the split (8 would cost occupancy: the label loads hold many registers).
+constexpr int kSplitUnroll = 4;
+static_assert(kSelTile <= kWG * 64, "a tile's items per thread fit the 64-bit masks of the split");
+
+// The split of a query's candidates [a, e) (at most 64 per thread; distance bits d[], bins
+// (bits - mn) >> sft) around the k'-th key's bin bstar, with the A12 post-filter folded in.  Candidates in
+// lower bins are in the top k' for sure: those passing the label filter (label_pass) go straight to the
+// refine list out[] as z-pool positions (a thread's columns ascend, so it walks the query's range table
+// rs/rp forward, and a wave's label reads are contiguous within a range).  Candidates in bin bstar go to
+// scr[] for the final ranking.  A first pass marks both kinds in per-thread bit masks, one workgroup scan and
+// one reservation on the global counters cnt[0] (survivors) / cnt[1] (bin keys) place them, a second pass
+// writes them.  rg: the query's 2 l label bounds.  Called by the whole workgroup.
+__device__ inline void sel_split_range(uint32_t a, uint32_t e, const uint32_t* __restrict__ d, uint32_t mn,
+                                       uint32_t sft, uint32_t bstar, uint32_t nr, const uint32_t* rs,
+                                       const uint32_t* rp, int l, const float* __restrict__ lab, const float* rg,
+                                       uint32_t* __restrict__ out, uint64_t* __restrict__ scr, uint32_t* cnt) {
+  __shared__ uint32_t tmp[kWG / 64 + 1];
+  __shared__ uint32_t res[2];
+  constexpr uint32_t kStep = kWG * kSplitUnroll;
+  uint32_t j0 = 0;  // range of this thread's first column
+  {
+    const uint32_t col = a + threadIdx.x;
+    uint32_t hi = nr;
+    while (hi - j0 > 1) {
+      const uint32_t mid = (j0 + hi) >> 1;
+      if (rp[mid] <= col) j0 = mid; else hi = mid;
+    }
+  }
+  uint32_t j = j0;
+  uint64_t flo = 0, fin = 0;  // bit step * kSplitUnroll + u: listed survivor / key in the bin
+  uint32_t step = 0;
+  for (uint32_t base = a; base < e; base += kStep, ++step) {
+    uint32_t v[kSplitUnroll], pos[kSplitUnroll];
+#pragma unroll
+    for (int u = 0; u < kSplitUnroll; ++u) {
+      const uint32_t i = base + (uint32_t)u * kWG + threadIdx.x;
+      v[u] = i < e ? d[i] : 0u;
+    }
+    uint32_t lo = 0;  // items of this step below the bin
+#pragma unroll
+    for (int u = 0; u < kSplitUnroll; ++u) {
+      const uint32_t i = base + (uint32_t)u * kWG + threadIdx.x;
+      pos[u] = 0;
+      if (i < e) {
+        const uint32_t b = (v[u] - mn) >> sft;
+        if (b < bstar) {
+          while (j + 1 < nr && rp[j + 1] <= i) ++j;
+          pos[u] = rs[j] + (i - rp[j]);
+          lo |= 1u << u;
+        } else if (b == bstar) {
+          fin |= 1ull << (step * kSplitUnroll + u);
+        }
+      }
+    }
+    // the label filter, every item's loads in flight together
+    if (l == 4) {
+      float4 lv[kSplitUnroll];
+#pragma unroll
+      for (int u = 0; u < kSplitUnroll; ++u)
+        lv[u] = (lo >> u) & 1u ? reinterpret_cast<const float4*>(lab)[pos[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int u = 0; u < kSplitUnroll; ++u) {
+        const bool fail = lv[u].x < rg[0] || lv[u].x > rg[1] || lv[u].y < rg[2] || lv[u].y > rg[3] ||
+                          lv[u].z < rg[4] || lv[u].z > rg[5] || lv[u].w < rg[6] || lv[u].w > rg[7];
+        if (fail) lo &= ~(1u << u);
+      }
+    } else if (l > 1) {
+      for (int dm = 0; dm < l; ++dm) {
+        float lv[kSplitUnroll];
+#pragma unroll
+        for (int u = 0; u < kSplitUnroll; ++u) lv[u] = (lo >> u) & 1u ? lab[(uint64_t)pos[u] * l + dm] : 0.f;
+        const float lo_b = rg[2 * dm], hi_b = rg[2 * dm + 1];
+#pragma unroll
+        for (int u = 0; u < kSplitUnroll; ++u)
+          if (lv[u] < lo_b || lv[u] > hi_b) lo &= ~(1u << u);
+      }
+    }
+    flo |= (uint64_t)lo << (step * kSplitUnroll);
+  }
+  // places: survivors in the low 16 bits, bin keys in the high 16 bits (a segment holds at most 16384)
+  const uint32_t c = (uint32_t)__popcll(flo) | ((uint32_t)__popcll(fin) << 16);
+  uint32_t total;
+  const uint32_t excl = dev::wg_exclusive_scan(c, tmp, total);
+  if (threadIdx.x == 0) {
+    res[0] = (total & 0xFFFFu) ? atomicAdd(&cnt[0], total & 0xFFFFu) : 0u;
+    res[1] = (total >> 16) ? atomicAdd(&cnt[1], total >> 16) : 0u;
+  }
+  __syncthreads();
+  uint32_t olo = res[0] + (excl & 0xFFFFu), oin = res[1] + (excl >> 16);
+  j = j0;
+  while (flo) {
+    const int t = __ffsll((long long)flo) - 1;
+    flo &= flo - 1;
+    const uint32_t i = a + (uint32_t)(t / kSplitUnroll) * kStep + (uint32_t)(t % kSplitUnroll) * kWG + threadIdx.x;
+    while (j + 1 < nr && rp[j + 1] <= i) ++j;
+    out[olo++] = rs[j] + (i - rp[j]);
+  }
+  while (fin) {
+    const int t = __ffsll((long long)fin) - 1;
+    fin &= fin - 1;
+    const uint32_t i = a + (uint32_t)(t / kSplitUnroll) * kStep + (uint32_t)(t % kSplitUnroll) * kWG + threadIdx.x;
+    scr[oin++] = ((uint64_t)d[i] << 32) | i;
+  }
+}
+
+// Stages a query's range table (nr <= kSelLdsRanges) and label bounds in LDS.
+constexpr uint32_t kSelLdsRanges = 1024;
+
+__device__ inline void sel_stage_query(uint64_t qi, uint32_t nr, uint32_t sub_lists, int l,
+                                       const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ rpre,
+                                       const float* __restrict__ ranges, uint32_t* rs_l, uint32_t* rp_l, float* rg) {
+  if (nr <= kSelLdsRanges) {
+    for (uint32_t j = threadIdx.x; j <= nr; j += kWG) {
+      rp_l[j] = rpre[qi * (sub_lists + 1) + j];
+      if (j < nr) rs_l[j] = rstart[qi * sub_lists + j];
+    }
+  }
+  if (threadIdx.x < 2 * (uint32_t)l) rg[threadIdx.x] = ranges[qi * 2 * l + threadIdx.x];
+}
+
+// Split pass of the many-workgroup selection over tiles.  Per query, scnt[4 qi + 0] counts the listed
+// survivors, [1] the keys in the k'-th key's bin and [2] holds the keys below that bin.
+__global__ __launch_bounds__(256) void sel_split_kernel(
+    uint64_t qn, int l, uint32_t sub_lists, const uint64_t* __restrict__ aoff, const uint32_t* __restrict__ sslot,
+    const uint32_t* __restrict__ smm, const uint64_t* __restrict__ kprime, const uint32_t* __restrict__ adist,
+    const uint32_t* __restrict__ ghist, const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ rstart,
+    const uint32_t* __restrict__ rpre, const float* __restrict__ lab, const float* __restrict__ ranges,
+    const uint64_t* __restrict__ loff, uint32_t* __restrict__ L, uint64_t* __restrict__ scratch,
+    uint32_t* __restrict__ scnt, const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
+  __shared__ __align__(16) uint32_t hist[kSelBins];
+  __shared__ uint32_t rs_l[kSelLdsRanges], rp_l[kSelLdsRanges + 1];
+  __shared__ float rg[2 * kMaxDims];
+  __shared__ uint64_t sh[8];
+  for_each_sel_segment(qn, aoff, sslot, [&](uint64_t qi, uint32_t a, uint32_t e) {
+    const uint32_t mn = smm[2 * qi], sft = dev::dist_bin_shift(mn, smm[2 * qi + 1]);
+    const uint32_t nr = n_chosen_s[qi];
+    sel_stage_query(qi, nr, sub_lists, l, rstart, rpre, ranges, rs_l, rp_l, rg);
+    const uint4* g = reinterpret_cast<const uint4*>(ghist + (uint64_t)sslot[qi] * kSelBins);
+    for (uint32_t i = threadIdx.x; i < kSelBins / 4; i += kWG) reinterpret_cast<uint4*>(hist)[i] = g[i];
+    __syncthreads();
+    dev::wg_hist_find(hist, (uint32_t)kprime[qi], sh);
+    if (threadIdx.x == 0) scnt[4 * qi + 2] = (uint32_t)sh[1];  // the same value from every tile of the query
+    const bool table_lds = nr <= kSelLdsRanges;
+    sel_split_range(a, e, adist + aoff[qi], mn, sft, (uint32_t)sh[0], nr, table_lds ? rs_l : rstart + qi * sub_lists,
+                    table_lds ? rp_l : rpre + qi * (sub_lists + 1), l, lab, rg, L + loff[qi], scratch + aoff[qi],
+                    scnt + 4 * qi);
+    __syncthreads();
+  });
+}
+
+// Clears the global histograms of the slots handed out by the plan.
+__global__ void sel_zero_kernel(const uint32_t* __restrict__ slot_ctr, uint32_t slot_cap, uint32_t* __restrict__ ghist) {
+  const uint32_t used = *slot_ctr < slot_cap ? *slot_ctr : slot_cap;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (uint64_t)used * (kSelBins / 4)) reinterpret_cast<uint4*>(ghist)[i] = make_uint4(0, 0, 0, 0);
+}
+
+constexpr int kOneWgUnroll = 2;  // distance loads in flight per thread of the one-workgroup selection
+
 // A11 selection + A12 post-filter for mode-1 queries: the k' smallest (pq distance, column) keys
-// (select_k, product-quantization.cu:264), then for l > 1 the exact label filter.
+// (select_k, product-quantization.cu:264), then for l > 1 the exact label filter.  A query with a selection
+// slot arrives split by the tiles (survivors below the k'-th key's bin already listed); the others are
+// selected here by one workgroup.
 __global__ __launch_bounds__(256) void adc_select_kernel(
     uint64_t qn, int l, uint32_t sub_lists, const uint32_t* __restrict__ mode,
     const uint64_t* __restrict__ kprime, const uint64_t* __restrict__ cand,
@@ -1923,9 +2193,10 @@ __global__ __launch_bounds__(256) void adc_select_kernel(
     const uint32_t* __restrict__ rpre, const float* __restrict__ lab, const float* __restrict__ ranges,
     const uint64_t* __restrict__ aoff, uint64_t* __restrict__ akeys_sel_scratch,
     const uint32_t* __restrict__ adist, uint64_t* __restrict__ akeys2, const uint64_t* __restrict__ loff,
-    uint32_t* __restrict__ L, uint32_t* __restrict__ nL, const uint32_t* __restrict__ abort_flag) {
+    uint32_t* __restrict__ L, uint32_t* __restrict__ nL, const uint32_t* __restrict__ sslot,
+    const uint32_t* __restrict__ smm, const uint32_t* __restrict__ scnt, const uint32_t* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
-  __shared__ uint32_t hist[2048];
+  __shared__ __align__(16) uint32_t hist[kSelBins];
   __shared__ uint64_t sh[8];
   __shared__ uint32_t cnt_sh;
   const uint64_t qi = blockIdx.x;
@@ -1936,17 +2207,32 @@ __global__ __launch_bounds__(256) void adc_select_kernel(
     if (threadIdx.x == 0) nL[qi] = 0;
     return;
   }
-  uint64_t* sel = akeys_sel_scratch + loff[qi];  // kp keys
-  dev::wg_radix_select_compact(dev::KeyDist{adist + aoff[qi]}, Cq, kp, sel, akeys2 + aoff[qi], hist, sh);
   const uint32_t nr = n_chosen_s[qi];
   const uint32_t* rs = rstart + qi * sub_lists;
   const uint32_t* rp = rpre + qi * (sub_lists + 1);
+  const uint64_t* sel = akeys_sel_scratch + loff[qi];  // the selected keys still to post-filter
+  uint32_t nsel = kp, listed = 0;                        // their number; refine-list entries already written
+  if (sslot[qi] != kNone) {
+    // the tiles have listed the survivors below the k'-th key's bin: rank that bin's keys
+    listed = scnt[4 * qi];
+    const uint32_t m = scnt[4 * qi + 1];
+    nsel = kp - scnt[4 * qi + 2];
+    const uint64_t* scr = akeys2 + aoff[qi];
+    if (nsel == m) {
+      sel = scr;
+    } else {
+      dev::wg_radix_select(dev::KeyArray{scr}, m, nsel, akeys_sel_scratch + loff[qi], hist, sh);
+    }
+  } else {
+    dev::wg_dist_select<kOneWgUnroll>(adist + aoff[qi], Cq, kp, smm[2 * qi], smm[2 * qi + 1], akeys_sel_scratch + loff[qi],
+                        akeys2 + aoff[qi], hist, sh);
+  }
   float rg[2 * kMaxDims];
   for (int i = 0; i < 2 * l; ++i) rg[i] = ranges[qi * 2 * l + i];
   if (threadIdx.x == 0) cnt_sh = 0;
   __syncthreads();
-  uint32_t* out = L + loff[qi];
-  for (uint32_t e = threadIdx.x; e < kp; e += kWG) {
+  uint32_t* out = L + loff[qi] + listed;
+  for (uint32_t e = threadIdx.x; e < nsel; e += kWG) {
     const uint32_t pos = col_to_pos(rs, rp, nr, dev::key_tie(sel[e]));
     if (l == 1) {
       out[e] = pos;
@@ -1955,7 +2241,7 @@ __global__ __launch_bounds__(256) void adc_select_kernel(
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) nL[qi] = l == 1 ? kp : cnt_sh;
+  if (threadIdx.x == 0) nL[qi] = listed + (l == 1 ? nsel : cnt_sh);
 }
 
 // A13 select: top-k by (exact distance, id) of the refine list with the reference's pad columns
@@ -1970,7 +2256,7 @@ __global__ __launch_bounds__(256) void final_pick_kernel(
   if (abort_flag && *abort_flag) return;
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t* sorted = reinterpret_cast<uint64_t*>(smem);
-  __shared__ uint32_t hist[2048];
+  __shared__ __align__(16) uint32_t hist[2048];
   __shared__ uint64_t sh[8];
   const uint64_t qi = blockIdx.x;
   const uint32_t n = nL[qi];
@@ -2167,6 +2453,7 @@ struct Index::Work {
   DevBuf keys, woff, bsum, gstat, rstart, rpre, cand, real;
   DevBuf mode, kprime, lsize, asize, chunks, pad32, padmax, nL;
   DevBuf L, Lkeys, adist, akeys2;
+  DevBuf sslot, smm, scnt, ghist;  // many-workgroup selection: slots, distance ranges, split counts, histograms
   DevBuf S, feat, budget_T, budget_S, qb16;  // PRECISION 1: scores, model features, budgets, bf16 queries
   DevBuf nzb, nzp;                            // PRECISION 1: non-zero sub-cluster bit words and their ranks
   DevBuf st, st_chosen;                // call-wide counters [nq][6] and chosen sub-cluster ids [nq][sub_lists]
@@ -2391,7 +2678,7 @@ uint64_t Index::config_digest() const {
   const uint64_t v[] = {cfg_.n_list, cfg_.sub_lists, cfg_.topk, cfg_.exps0, cfg_.precision, cfg_.probe_clusters,
                         cfg_.budget_model, cfg_.sparse_count, cfg_.sparse_max_tests, cfg_.adc_per, cfg_.adc_wgs,
                         cfg_.parity_batch, cfg_.query_chunk, cfg_.label_mask, cfg_.select_wave, cfg_.score_block,
-                        graph_epoch_};
+                        cfg_.adc_split, graph_epoch_};
   uint64_t h = 1469598103934665603ull;
   for (uint64_t x : v) {
     h ^= x;
@@ -2445,7 +2732,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.zlo_cnt.reserve(qn * 8);
   w.err.reserve(4);
   w.dense_list.reserve(qn * 4);
-  w.n_dense.reserve(8);
+  w.n_dense.reserve(16);
   w.redo.reserve(qn * 4);
   w.counts.reserve(qn * CS * 4);
   w.ib_pos.reserve(qn * kInboxCap * 4);
@@ -2473,6 +2760,11 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.cand.reserve(qn * 8);
   w.real.reserve(qn * 8);
   w.mode.reserve(qn * 4);
+  const uint32_t sel_slots = (uint32_t)std::min<uint64_t>(qn, kSelSlots);
+  w.sslot.reserve(qn * 4);
+  w.smm.reserve(qn * 8);
+  w.scnt.reserve(qn * 16);
+  w.ghist.reserve((uint64_t)sel_slots * kSelBins * 4);
   w.kprime.reserve(qn * 8);
   w.lsize.reserve((qn + 1) * 8);
   w.asize.reserve((qn + 1) * 8);
@@ -2706,7 +2998,11 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      topk, w.cand.as<uint64_t>(), w.gstat.as<unsigned long long>(), w.mode.as<uint32_t>(),
                      w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(),
                      w.chunks.as<uint64_t>(), w.pad32.as<uint64_t>(), w.padmax.as<uint64_t>(), w.n_ib.as<uint32_t>(),
-                     inbox_mode, adc_per * (uint32_t)kWG);
+                     inbox_mode, adc_per * (uint32_t)kWG, w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(),
+                     w.scnt.as<uint32_t>(), w.n_dense.as<uint32_t>() + 2, sel_slots, cfg_.adc_split);
+  HDG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(sel_zero_kernel, dim3(sel_slots * (kSelBins / 4) / 256), dim3(256), 0, s,
+                     w.n_dense.as<uint32_t>() + 2, sel_slots, w.ghist.as<uint32_t>());
   HDG_HIP(hipGetLastError());
   scan_u64(w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(), w.chunks.as<uint64_t>(), qn, nullptr, nullptr, 0);
   HDG_HIP(hipGetLastError());
@@ -2780,7 +3076,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      w.mode.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),                     \
                      w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.L.as<uint32_t>(), w.lsize.as<uint64_t>(),       \
                      w.nL.as<uint32_t>(), w.chunks.as<uint64_t>(), w.asize.as<uint64_t>(), w.adist.as<uint32_t>(),   \
-                     lut_bytes, abort_p)
+                     lut_bytes, w.smm.as<uint32_t>(), abort_p)
 #define HDG_ADC(LT, PER)                                                                                          \
   do {                                                                                                            \
     if (S.K == 256 && stream == 32) HDG_ADC1(LT, PER, 256, 32);                                                   \
@@ -2810,12 +3106,22 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   }
   mark();
   if (totChunks > 0) {
+    const unsigned sel_grid = (unsigned)std::max<uint64_t>(1, cdiv(totA, kSelTile));
+    hipLaunchKernelGGL(sel_hist_kernel, dim3(sel_grid), dim3(256), 0, s, qn, w.asize.as<uint64_t>(),
+                       w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(), w.adist.as<uint32_t>(), w.ghist.as<uint32_t>(),
+                       abort_p);
+    hipLaunchKernelGGL(sel_split_kernel, dim3(sel_grid), dim3(256), 0, s, qn, l, sub_lists, w.asize.as<uint64_t>(),
+                       w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(), w.kprime.as<uint64_t>(), w.adist.as<uint32_t>(),
+                       w.ghist.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),
+                       w.rpre.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
+                       w.L.as<uint32_t>(), w.akeys2.as<uint64_t>(), w.scnt.as<uint32_t>(), abort_p);
+    HDG_HIP(hipGetLastError());
     hipLaunchKernelGGL(adc_select_kernel, dim3((unsigned)qn), dim3(256), 0, s, qn, l, sub_lists,
                        w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.cand.as<uint64_t>(),
                        w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(), w.rpre.as<uint32_t>(), S.lab.as<float>(),
                        w.ranges.as<float>(), w.asize.as<uint64_t>(), w.Lkeys.as<uint64_t>(), w.adist.as<uint32_t>(),
                        w.akeys2.as<uint64_t>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(),
-                       abort_p);
+                       w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(), w.scnt.as<uint32_t>(), abort_p);
     HDG_HIP(hipGetLastError());
   }
   mark();

@@ -154,8 +154,8 @@ int hdg_fit_mem_coeff(const double samples[4][3], double coeff[4]);
 
 /* Launch-parameter autotuner.  Times full queries of the given batch (device arrays, as hdg_query;
  * out_dist/out_ids receive the results) under a ladder of values of each result-invariant knob
- * (ADC_PER, ADC_WGS, SPARSE_MAX_TESTS, QUERY_CHUNK, SCORE_BLOCK, GRAPH), one knob at a time, `reps` timed runs per
- * value (median), and keeps the fastest value whose ids and distances are bit-identical to those of the
+ * (ADC_PER, ADC_WGS, SPARSE_MAX_TESTS, QUERY_CHUNK, SCORE_BLOCK, ADC_SPLIT, GRAPH), one knob at a time,
+ * `reps` timed runs per value (median), and keeps the fastest value whose ids and distances are bit-identical to those of the
  * starting configuration.  The chosen values stay set on the index; the trial log is written to
  * report (cap bytes, may be NULL).  No reference counterpart: the reference's launch shapes are
  * compile-time constants. */

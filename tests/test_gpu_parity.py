@@ -155,6 +155,17 @@ def test_l1_adc_select(l1_wide):
     assert_parity(res, "l1 wide exps0=2")
 
 
+@pytest.mark.parametrize("split", [0, 256])
+def test_adc_select_split(l1_wide, l2_explicit, split):
+    """Top-k' selection with the ADC queries over many workgroups (ADC_SPLIT 256: each tile of 16384
+    candidates holds several queries' segments) and with none (ADC_SPLIT 0: one workgroup per query)."""
+    for b, label in ((l1_wide, "l1 wide"), (l2_explicit, "explicit ranges")):
+        res = b.run(EXPS0=2, ADC_SPLIT=split)
+        assert (res[2].adc >= 256).any(), "scenario must give queries of at least 256 ADC candidates"
+        assert_parity(res, f"{label} ADC_SPLIT={split}")
+        b.idx.set_param("ADC_SPLIT", 32768)
+
+
 def test_l1_small_lists(l1):
     assert_parity(l1.run(EXPS0=30, N_LIST=8, SUB_LISTS=16), "l1 n_list=8 sub_lists=16")
 
@@ -184,9 +195,10 @@ def test_adc_code_widths(d, pq_dim):
     (12, not a multiple of 8)."""
     w = datasets.explicit_ranges(n=20000, nq=120, d=d, components=32, clusters=32, sub_clusters=6)
     b = Built(w, PQ_DIM=pq_dim)
-    res = b.run(EXPS0=2)
-    assert (res[2].adc > 0).any(), "scenario must exercise the ADC path"
-    assert_parity(res, f"explicit ranges d={d} pq_dim={pq_dim}")
+    for split in (32768, 256):
+        res = b.run(EXPS0=2, ADC_SPLIT=split)
+        assert (res[2].adc > 0).any(), "scenario must exercise the ADC path"
+        assert_parity(res, f"explicit ranges d={d} pq_dim={pq_dim} ADC_SPLIT={split}")
 
 
 def test_l3_dense_count(l3):
