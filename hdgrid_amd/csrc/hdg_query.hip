@@ -2627,7 +2627,8 @@ void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* ql
     // uncaptured run: results for this call, buffer sizes for the capture
     query_chunk(queries, q0, qn, qlabels, qcols, G, out_dist, out_ids, s);
     w.lcap = std::max(w.lcap, w.last_totL + w.last_totL / 2 + 4096);
-    w.acap = std::max(w.acap, w.last_totA + w.last_totA / 2 + 4096);
+    // a chunk that needed no ADC is captured without the ADC stage (capacity 0: any ADC work aborts the replay)
+    if (w.last_totA > 0) w.acap = std::max(w.acap, w.last_totA + w.last_totA / 2 + 4096);
     w.L.reserve(w.lcap * 4);
     w.Lkeys.reserve(w.lcap * 8);
     w.adist.reserve(w.acap * 4);
@@ -3001,9 +3002,6 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      inbox_mode, adc_per * (uint32_t)kWG, w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(),
                      w.scnt.as<uint32_t>(), w.n_dense.as<uint32_t>() + 2, sel_slots, cfg_.adc_split);
   HDG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(sel_zero_kernel, dim3(sel_slots * (kSelBins / 4) / 256), dim3(256), 0, s,
-                     w.n_dense.as<uint32_t>() + 2, sel_slots, w.ghist.as<uint32_t>());
-  HDG_HIP(hipGetLastError());
   scan_u64(w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(), w.chunks.as<uint64_t>(), qn, nullptr, nullptr, 0);
   HDG_HIP(hipGetLastError());
   uint64_t totL, totA, totChunks;
@@ -3028,7 +3026,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     HDG_HIP(hipGetLastError());
     totL = w.lcap;
     totA = w.acap;
-    totChunks = 1;  // the persistent ADC grid reads the real count on the device
+    totChunks = w.acap ? 1 : 0;  // the persistent ADC grid reads the real count on the device
   }
   w.L.reserve(std::max<uint64_t>(totL, 1) * 4);
   w.Lkeys.reserve(std::max<uint64_t>(totL, 1) * 8);
@@ -3107,6 +3105,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   mark();
   if (totChunks > 0) {
     const unsigned sel_grid = (unsigned)std::max<uint64_t>(1, cdiv(totA, kSelTile));
+    hipLaunchKernelGGL(sel_zero_kernel, dim3(sel_slots * (kSelBins / 4) / 256), dim3(256), 0, s,
+                       w.n_dense.as<uint32_t>() + 2, sel_slots, w.ghist.as<uint32_t>());
     hipLaunchKernelGGL(sel_hist_kernel, dim3(sel_grid), dim3(256), 0, s, qn, w.asize.as<uint64_t>(),
                        w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(), w.adist.as<uint32_t>(), w.ghist.as<uint32_t>(),
                        abort_p);
