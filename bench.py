@@ -155,6 +155,8 @@ def main():
     ap.add_argument("--budget-levels", type=int, default=20, help="budget ladder levels")
     ap.add_argument("--budget-s0", type=float, default=16.0, help="sub-cluster budget of the lowest level")
     ap.add_argument("--budget-ratio", type=float, default=2 ** 0.25, help="budget growth per level")
+    ap.add_argument("--learned-cap", type=int, default=0,
+                    help="sub-cluster choice the learned budgets trim (0: twice the fixed budget, at least 256)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
@@ -255,7 +257,7 @@ def main():
         res["mixed_graph"] = dict(qps=qps, ms=ms, recall=r, sub_lists=b, stage=stage, st=st)
     # ---- learned budget + mixed ----
     if "learned" in paths:
-        cap = max(res["mixed"]["sub_lists"] * 2, 256)
+        cap = args.learned_cap or max(res["mixed"]["sub_lists"] * 2, 256)
         idx.set_param("SUB_LISTS", cap)
         if args.load_budget:
             idx.budget_load(args.load_budget)

@@ -91,6 +91,7 @@ __device__ float mlp_forward(const float* w, const float* x, float* z1, float* z
 // in order, as in mlp_forward), the output sum is taken by the group's first lane.  A workgroup of 256
 // threads serves 16 queries and keeps the bf16 weights in LDS.
 constexpr int kInferQ = 16;
+constexpr uint64_t kInferWgs = 1024;  // workgroups of the inference (four fit a CU; each stages the weights once)
 
 __global__ __launch_bounds__(256) void infer_kernel(uint64_t qn, int l, const float* __restrict__ ranges,
                                                     const uint32_t* __restrict__ box_w, int n_valid,
@@ -101,69 +102,100 @@ __global__ __launch_bounds__(256) void infer_kernel(uint64_t qn, int l, const fl
                                                     const uint32_t* __restrict__ ladS,
                                                     uint32_t* __restrict__ n_chosen_s, uint32_t* __restrict__ bT,
                                                     uint32_t* __restrict__ bS) {
-  __shared__ float w[kParams];
+  __shared__ __align__(16) float w[kParams];
   __shared__ float xs[kInferQ][kF];
   __shared__ float h1s[kInferQ][kH];
   __shared__ float h2s[kInferQ][kH];
-  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = wq[i];
+  {
+    // the weights as 16-byte pieces, several in flight per thread
+    constexpr int n4 = kParams / 4;
+    const float4* w4 = reinterpret_cast<const float4*>(wq);
+#pragma unroll 8
+    for (int i = threadIdx.x; i < n4; i += blockDim.x) reinterpret_cast<float4*>(w)[i] = w4[i];
+    for (int i = n4 * 4 + threadIdx.x; i < kParams; i += blockDim.x) w[i] = wq[i];
+  }
   const int g = threadIdx.x & 15, qg = threadIdx.x >> 4;
-  const uint64_t qi = (uint64_t)blockIdx.x * kInferQ + qg;
-  const bool live = qi < qn;
-  if (live) {
-    // the query's 48 features (as query_features), three per lane
+  // persistent over groups of kInferQ queries: the weights are staged once per workgroup
+  const uint64_t groups = (qn + kInferQ - 1) / kInferQ;
+  for (uint64_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+    const uint64_t qi = grp * kInferQ + qg;
+    const bool live = qi < qn;
+    __syncthreads();  // the previous group is done with xs / h1s / h2s (and the weights are staged)
+    if (live) {
+      // the query's 48 features (as query_features), three per lane
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int i = g + 16 * k;
-      float v = 0.0f;
-      if (i < 8) {
-        if (i < l) {
-          float wd = ranges[qi * 2 * l + 2 * i + 1] - ranges[qi * 2 * l + 2 * i];
-          wd = wd > 0.0f ? (wd < 1e12f ? wd : 1e12f) : 0.0f;
-          v = log1pf(wd);
+      for (int k = 0; k < 3; ++k) {
+        const int i = g + 16 * k;
+        float v = 0.0f;
+        if (i < 8) {
+          if (i < l) {
+            float wd = ranges[qi * 2 * l + 2 * i + 1] - ranges[qi * 2 * l + 2 * i];
+            wd = wd > 0.0f ? (wd < 1e12f ? wd : 1e12f) : 0.0f;
+            v = log1pf(wd);
+          }
+        } else if (i < 16) {
+          const int j = i - 8;
+          v = j < n_valid ? log1pf((float)box_w[(uint64_t)j * qn + qi] + 1.0f) : 0.0f;
+        } else {
+          v = feat_dist[qi * kFeatDists + (i - 16)];
         }
-      } else if (i < 16) {
-        const int j = i - 8;
-        v = j < n_valid ? log1pf((float)box_w[(uint64_t)j * qn + qi] + 1.0f) : 0.0f;
-      } else {
-        v = feat_dist[qi * kFeatDists + (i - 16)];
+        xs[qg][i] = bf16r((v - mean[i]) * inv_std[i]);
       }
-      xs[qg][i] = bf16r((v - mean[i]) * inv_std[i]);
     }
-  }
-  __syncthreads();
-  if (live) {
-    for (int u = 0; u < 4; ++u) {
-      const int j = g * 4 + u;
-      float a = w[kOffB1 + j];
+    __syncthreads();
+    // hidden units 4g .. 4g+3 of each layer: four independent chains, each summed in input order, their
+    // weights one 16-byte LDS read per input
+    if (live) {
+      float a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] = w[kOffB1 + g * 4 + u];
 #pragma unroll 8
-      for (int i = 0; i < kF; ++i) a += xs[qg][i] * w[kOffW1 + i * kH + j];
-      h1s[qg][j] = bf16r(a > 0.0f ? a : 0.0f);
+      for (int i = 0; i < kF; ++i) {
+        const float x = xs[qg][i];
+        const float4 wv = *reinterpret_cast<const float4*>(&w[kOffW1 + i * kH + g * 4]);
+        a[0] += x * wv.x;
+        a[1] += x * wv.y;
+        a[2] += x * wv.z;
+        a[3] += x * wv.w;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) h1s[qg][g * 4 + u] = bf16r(a[u] > 0.0f ? a[u] : 0.0f);
     }
-  }
-  __syncthreads();
-  if (live) {
-    for (int u = 0; u < 4; ++u) {
-      const int j = g * 4 + u;
-      float a = w[kOffB2 + j];
+    __syncthreads();
+    if (live) {
+      float a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] = w[kOffB2 + g * 4 + u];
 #pragma unroll 8
-      for (int i = 0; i < kH; ++i) a += h1s[qg][i] * w[kOffW2 + i * kH + j];
-      h2s[qg][j] = bf16r(a > 0.0f ? a : 0.0f);
+      for (int i = 0; i < kH; ++i) {
+        const float x = h1s[qg][i];
+        const float4 wv = *reinterpret_cast<const float4*>(&w[kOffW2 + i * kH + g * 4]);
+        a[0] += x * wv.x;
+        a[1] += x * wv.y;
+        a[2] += x * wv.z;
+        a[3] += x * wv.w;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) h2s[qg][g * 4 + u] = bf16r(a[u] > 0.0f ? a[u] : 0.0f);
+    }
+    __syncthreads();
+    if (live && g == 0) {
+      float y = w[kOffB3];
+#pragma unroll 16
+      for (int i = 0; i < kH; ++i) y += h2s[qg][i] * w[kOffW3 + i];
+      int lev = levels - 1;  // fallback: the fixed budget
+      if (isfinite(y)) {
+        float t = ceilf(y + margin);
+        lev = t < 0.0f ? 0 : (t > (float)(levels - 1) ? levels - 1 : (int)t);
+      }
+      bT[qi] = ladT[lev];
+      const uint32_t S = ladS[lev];
+      bS[qi] = S;
+      if (n_chosen_s[qi] > S) n_chosen_s[qi] = S;
     }
   }
-  __syncthreads();
-  if (!live || g != 0) return;
-  float y = w[kOffB3];
-  for (int i = 0; i < kH; ++i) y += h2s[qg][i] * w[kOffW3 + i];
-  int lev = levels - 1;  // fallback: the fixed budget
-  if (isfinite(y)) {
-    float t = ceilf(y + margin);
-    lev = t < 0.0f ? 0 : (t > (float)(levels - 1) ? levels - 1 : (int)t);
-  }
-  bT[qi] = ladT[lev];
-  const uint32_t S = ladS[lev];
-  bS[qi] = S;
-  if (n_chosen_s[qi] > S) n_chosen_s[qi] = S;
 }
+
 
 // Training forward + backward over a mini-batch; gradients accumulated into g (f32, atomics per block).
 __global__ __launch_bounds__(128) void train_step_kernel(const float* __restrict__ X, const float* __restrict__ Y,
@@ -269,7 +301,7 @@ void Index::run_budget_model(uint64_t qn, int l, const float* ranges, const uint
                              uint32_t* budget_S, hipStream_t s) {
   (void)box_lo;
   BudgetModel& B = *budget_;
-  hipLaunchKernelGGL(infer_kernel, dim3((unsigned)((qn + kInferQ - 1) / kInferQ)), dim3(256), 0, s, qn, l, ranges, box_w,
+  hipLaunchKernelGGL(infer_kernel, dim3((unsigned)std::min<uint64_t>((qn + kInferQ - 1) / kInferQ, kInferWgs)), dim3(256), 0, s, qn, l, ranges, box_w,
                      n_valid, feat_dist, B.d_w.as<float>(), B.d_mean.as<float>(), B.d_inv.as<float>(), B.margin,
                      (int)B.ladT.size(), B.d_ladT.as<uint32_t>(), B.d_ladS.as<uint32_t>(), n_chosen_s, budget_T,
                      budget_S);
