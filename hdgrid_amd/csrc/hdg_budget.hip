@@ -101,7 +101,9 @@ __global__ __launch_bounds__(256) void infer_kernel(uint64_t qn, int l, const fl
                                                     const uint32_t* __restrict__ ladT,
                                                     const uint32_t* __restrict__ ladS,
                                                     uint32_t* __restrict__ n_chosen_s, uint32_t* __restrict__ bT,
-                                                    uint32_t* __restrict__ bS) {
+                                                    uint32_t* __restrict__ bS,
+                                                    const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   __shared__ __align__(16) float w[kParams];
   __shared__ float xs[kInferQ][kF];
   __shared__ float h1s[kInferQ][kH];
@@ -298,13 +300,13 @@ bool Index::budget_probes_all(uint32_t C) const {
 
 void Index::run_budget_model(uint64_t qn, int l, const float* ranges, const uint32_t* box_lo, const uint32_t* box_w,
                              int n_valid, const float* feat_dist, uint32_t* n_chosen_s, uint32_t* budget_T,
-                             uint32_t* budget_S, hipStream_t s) {
+                             uint32_t* budget_S, hipStream_t s, const uint32_t* abort_flag) {
   (void)box_lo;
   BudgetModel& B = *budget_;
   hipLaunchKernelGGL(infer_kernel, dim3((unsigned)std::min<uint64_t>((qn + kInferQ - 1) / kInferQ, kInferWgs)), dim3(256), 0, s, qn, l, ranges, box_w,
                      n_valid, feat_dist, B.d_w.as<float>(), B.d_mean.as<float>(), B.d_inv.as<float>(), B.margin,
                      (int)B.ladT.size(), B.d_ladT.as<uint32_t>(), B.d_ladS.as<uint32_t>(), n_chosen_s, budget_T,
-                     budget_S);
+                     budget_S, abort_flag);
   HDG_HIP(hipGetLastError());
 }
 

@@ -60,7 +60,8 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
                                                           float* __restrict__ S, uint32_t QT, uint32_t CT,
                                                           uint32_t per, const uint32_t* __restrict__ nzb,
                                                           const uint32_t* __restrict__ nzp, uint32_t nw,
-                                                          uint64_t ld) {
+                                                          uint64_t ld, const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   const uint32_t p = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
   if (p >= QT * CT) return;
   const uint32_t span = kGemmGroup * CT, first = (p / span) * kGemmGroup;
@@ -214,7 +215,9 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
                                                             float* __restrict__ feat_dist,
                                                             uint64_t* __restrict__ scratch, uint32_t scap,
                                                             int64_t skip_upto, const uint32_t* __restrict__ nzb,
-                                                            const uint32_t* __restrict__ nzp, uint32_t nw) {
+                                                            const uint32_t* __restrict__ nzp, uint32_t nw,
+                                                            const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   extern __shared__ __align__(16) uint64_t keys[];  // scap keys
   __shared__ __align__(16) uint32_t hist[2048];
   __shared__ uint64_t sh[8];
@@ -301,12 +304,21 @@ __global__ __launch_bounds__(256) void scored_select_wave_kernel(
     uint32_t cap, const uint32_t* __restrict__ n_list_q, const float* __restrict__ S, uint64_t ld,
     uint32_t k_default, const uint32_t* __restrict__ budget, uint32_t out_ld, uint32_t* __restrict__ chosen,
     uint32_t* __restrict__ n_chosen, float* __restrict__ feat_dist, const uint32_t* __restrict__ nzb,
-    const uint32_t* __restrict__ nzp, uint32_t nw) {
+    const uint32_t* __restrict__ nzp, uint32_t nw, uint32_t* __restrict__ abort_flag, uint32_t* __restrict__ left,
+    bool abort_on_left) {
+  if (abort_flag && *abort_flag) return;
   const int lane = threadIdx.x & 63;
   const uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (qi >= qn) return;
   const uint32_t n = n_list_q[qi];
-  if (n > kWaveSelMax) return;
+  if (n > kWaveSelMax) {
+    // left to the workgroup kernel: counted; a replayed graph that left that kernel out is abandoned
+    if (lane == 0) {
+      if (left) atomicAdd(left, 1u);
+      if (abort_on_left) *abort_flag = 1u;
+    }
+    return;
+  }
   const uint32_t* lst = list + qi * cap;
   const float* row = S + qi * ld;
   const uint32_t* nzb_q = nzb ? nzb + qi * nw : nullptr;
@@ -345,19 +357,21 @@ void launch_scored_select(const float* queries, uint64_t q0, uint64_t qn, uint64
                           uint32_t cap, const uint32_t* n_list_q, const float* S, uint64_t ld, uint32_t k_default,
                           const uint32_t* budget, uint32_t out_ld, uint32_t* chosen, uint32_t* n_chosen,
                           float* feat_dist, uint64_t* scratch, bool wave, hipStream_t s, const uint32_t* nzb,
-                          const uint32_t* nzp, uint32_t nw) {
+                          const uint32_t* nzp, uint32_t nw, uint32_t* abort_flag, uint32_t* left,
+                          bool skip_fallback) {
   uint32_t scap = 1024;
   while (scap < k_default) scap <<= 1;
   int64_t skip_upto = -1;
   if (wave) {
     hipLaunchKernelGGL(scored_select_wave_kernel, dim3((unsigned)((qn + 3) / 4)), dim3(256), 0, s, queries, q0, qn,
                        d, list, cap, n_list_q, S, ld, k_default, budget, out_ld, chosen, n_chosen, feat_dist, nzb,
-                       nzp, nw);
+                       nzp, nw, abort_flag, left, skip_fallback && abort_flag);
     skip_upto = kWaveSelMax;
+    if (skip_fallback && abort_flag) return;  // captured without the workgroup kernel
   }
   hipLaunchKernelGGL(scored_select_kernel, dim3((unsigned)qn), dim3(256), scap * 8, s, queries, q0, d, list, cap,
                      n_list_q, S, ld, k_default, budget, out_ld, chosen, n_chosen, feat_dist, scratch, scap,
-                     skip_upto, nzb, nzp, nw);
+                     skip_upto, nzb, nzp, nw, abort_flag);
 }
 
 // 8 elements per thread (two float4 loads, one 16-byte store) when d and Dp are multiples of 8
@@ -392,12 +406,13 @@ void launch_to_bf16(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, 
 }
 
 void launch_coarse_gemm(const uint16_t* qb16, uint64_t qn, uint64_t Dp, const uint16_t* cb16, const float* cnorm,
-                        uint32_t C, float* S, hipStream_t s, const uint32_t* nzb, const uint32_t* nzp, uint32_t nw) {
+                        uint32_t C, float* S, hipStream_t s, const uint32_t* nzb, const uint32_t* nzp, uint32_t nw,
+                        const uint32_t* abort_flag) {
   const uint64_t QT = (qn + kGemmTile - 1) / kGemmTile, CT = (C + kGemmTile - 1) / kGemmTile;
   if (QT * CT > 0xFFFFFFFFull / 2) fail(kUnsupported, "coarse GEMM grid too large");
   const uint32_t per = (uint32_t)((QT * CT + kXcds - 1) / kXcds);
   hipLaunchKernelGGL(coarse_gemm_kernel, dim3(per * kXcds), dim3(256), 0, s, qb16, qn, Dp, cb16, cnorm, C, S,
-                     (uint32_t)QT, (uint32_t)CT, per, nzb, nzp, nw, (uint64_t)C);
+                     (uint32_t)QT, (uint32_t)CT, per, nzb, nzp, nw, (uint64_t)C, abort_flag);
 }
 
 }  // namespace hdg

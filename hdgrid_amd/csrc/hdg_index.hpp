@@ -111,7 +111,7 @@ class Index {
   // Per-query sub-cluster budgets from the MLP; trims n_chosen_s (chosen sub-clusters, sorted) to them.
   void run_budget_model(uint64_t qn, int l, const float* ranges, const uint32_t* box_lo, const uint32_t* box_w,
                         int n_valid, const float* feat_dist, uint32_t* n_chosen_s, uint32_t* budget_T,
-                        uint32_t* budget_S, hipStream_t s);
+                        uint32_t* budget_S, hipStream_t s, const uint32_t* abort_flag = nullptr);
   void capture_features(uint64_t q0, uint64_t qn, int l, const float* ranges, const uint32_t* box_w, int n_valid,
                         const float* feat_dist, hipStream_t s);
   // Labels every training query with its minimal budget level (traces of this pipeline against the

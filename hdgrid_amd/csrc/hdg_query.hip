@@ -79,6 +79,8 @@ __global__ void clear_kernel(unsigned long long* __restrict__ gstat, uint64_t n,
     n_dense[0] = 0;  // queries handed to the dense count
     n_dense[1] = 0;  // queries handed from the wave walk to the workgroup walk
     n_dense[2] = 0;  // selection slots handed out by the plan
+    n_dense[3] = 0;  // queries left to the workgroup scored selection
+    n_dense[4] = 0;  // queries left to the workgroup final pick
     if (abort_flag) *abort_flag = 0;
   }
 }
@@ -218,7 +220,9 @@ __global__ __launch_bounds__(256) void sparse_count_kernel(
     uint32_t* __restrict__ ns_out, uint32_t* __restrict__ list_c, uint32_t* __restrict__ cl_beg,
     uint32_t* __restrict__ nnz_c, uint32_t* __restrict__ dense_list, uint32_t* __restrict__ n_dense,
     unsigned long long* __restrict__ gstat, uint32_t* __restrict__ nzb, uint32_t* __restrict__ nzp,
-    const uint32_t* __restrict__ qlist, const uint32_t* __restrict__ qlist_n) {
+    const uint32_t* __restrict__ qlist, const uint32_t* __restrict__ qlist_n,
+    const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   extern __shared__ __align__(16) uint32_t sm[];
   const uint32_t nw = (CS + 31) >> 5;
   uint32_t* hist = sm;            // [CS]
@@ -464,7 +468,7 @@ __global__ __launch_bounds__(256) void sparse_count_wave_kernel(
     uint32_t* __restrict__ ns_out, uint32_t* __restrict__ list_c, uint32_t* __restrict__ cl_beg,
     uint32_t* __restrict__ nnz_c, uint32_t* __restrict__ dense_list, uint32_t* __restrict__ n_dense,
     unsigned long long* __restrict__ gstat, uint32_t* __restrict__ nzb, uint32_t* __restrict__ nzp,
-    uint32_t* __restrict__ redo) {
+    uint32_t* __restrict__ redo, uint32_t skip, uint32_t* __restrict__ abort_w) {
   __shared__ uint32_t sbuf_all[4][kWaveIb];  // in-box sub-cluster ids; sorted; then first positions of runs
   __shared__ uint32_t ubuf_all[4][kWaveIb];  // unique sub-cluster ids in rank order
   __shared__ uint32_t words_all[4][kSparseMaxCS / 32], wrank_all[4][kSparseMaxCS / 32];  // bit set, word ranks
@@ -564,15 +568,20 @@ __global__ __launch_bounds__(256) void sparse_count_wave_kernel(
         wave_lds_fence();  // pre_sh / e0_sh are rewritten by the next batch of bucket rows
       }
     }
+    // hand-overs; a replayed graph that left the receiving kernels out is abandoned (skip bits 1 / 2)
     if (to_dense) {
       if (lane == 0) {
         dense_list[atomicAdd(&n_dense[0], 1u)] = (uint32_t)qi;
         n_ib[qi] = kNone;
+        if (skip & 1u) *abort_w = 1u;
       }
       continue;
     }
     if (ibn > kWaveIb) {
-      if (lane == 0) redo[atomicAdd(&n_dense[1], 1u)] = (uint32_t)qi;
+      if (lane == 0) {
+        redo[atomicAdd(&n_dense[1], 1u)] = (uint32_t)qi;
+        if (skip & 2u) *abort_w = 1u;
+      }
       continue;
     }
     wave_lds_fence();
@@ -690,7 +699,9 @@ __global__ __launch_bounds__(256) void dense_count_kernel(const uint32_t* __rest
                                                           const uint64_t* __restrict__ zhi, uint64_t qn,
                                                           const uint32_t* __restrict__ dense_list,
                                                           const uint32_t* __restrict__ n_dense,
-                                                          uint32_t* __restrict__ counts) {
+                                                          uint32_t* __restrict__ counts,
+    const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   __shared__ uint64_t zs[kCountPiece];
   __shared__ uint32_t cs[NV][kCountPiece];
   const uint32_t nd = *n_dense;
@@ -756,7 +767,9 @@ __global__ __launch_bounds__(256) void dense_emit_kernel(uint32_t* __restrict__ 
                                                          uint32_t* __restrict__ ns_out, uint32_t* __restrict__ list_c,
                                                          uint32_t* __restrict__ cl_beg, uint32_t* __restrict__ nnz_c,
                                                          unsigned long long* __restrict__ gstat,
-                                                         uint32_t* __restrict__ nzb, uint32_t* __restrict__ nzp) {
+                                                         uint32_t* __restrict__ nzb, uint32_t* __restrict__ nzp,
+    const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   __shared__ uint32_t tmp[8];
   const uint64_t slot = blockIdx.x;
   if (slot >= *n_dense) return;
@@ -983,7 +996,9 @@ __global__ __launch_bounds__(1024) void scan_local_kernel(uint64_t* __restrict__
                                                           uint64_t* __restrict__ v2, uint64_t n,
                                                           const uint32_t* __restrict__ n32,
                                                           const uint32_t* __restrict__ mode, uint32_t skip_mode,
-                                                          unsigned long long* __restrict__ bsum) {
+                                                          unsigned long long* __restrict__ bsum,
+    const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   __shared__ unsigned long long wsum[3][16];
   uint64_t* const vs[3] = {v0, v1, v2};
   const uint64_t i = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
@@ -1001,7 +1016,9 @@ __global__ __launch_bounds__(1024) void scan_local_kernel(uint64_t* __restrict__
 
 __global__ __launch_bounds__(1024) void scan_fix_kernel(uint64_t* __restrict__ v0, uint64_t* __restrict__ v1,
                                                         uint64_t* __restrict__ v2, uint64_t n,
-                                                        const unsigned long long* __restrict__ bsum) {
+                                                        const unsigned long long* __restrict__ bsum,
+    const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   uint64_t* const vs[3] = {v0, v1, v2};
   const uint64_t i = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
   const uint32_t nb = (uint32_t)((n + 1023) / 1024);
@@ -1177,7 +1194,9 @@ __global__ __launch_bounds__(256) void ranges_inbox_wave_kernel(
     uint64_t qn, uint32_t cap, const uint32_t* __restrict__ list_s, const uint32_t* __restrict__ list_scnt,
     uint32_t sub_lists, const uint32_t* __restrict__ chosen_s, const uint32_t* __restrict__ n_chosen_s,
     uint32_t* __restrict__ chosen_sid, uint32_t* __restrict__ rpre, uint64_t* __restrict__ cand,
-    uint64_t* __restrict__ real, const uint32_t* __restrict__ n_ib) {
+    uint64_t* __restrict__ real, const uint32_t* __restrict__ n_ib,
+    const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   const int lane = threadIdx.x & 63;
   const uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (qi >= qn || n_ib[qi] == kNone) return;
@@ -1210,7 +1229,9 @@ __global__ __launch_bounds__(256) void ranges_kernel(
     const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zhi, uint32_t* __restrict__ chosen_sid,
     uint32_t* __restrict__ rstart, uint32_t* __restrict__ rpre, uint64_t* __restrict__ cand,
     uint64_t* __restrict__ real, unsigned long long* __restrict__ gstat, const uint32_t* __restrict__ n_ib,
-    int inbox_mode) {
+    int inbox_mode,
+    const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   __shared__ uint32_t tmp[8];
   __shared__ uint32_t base_sh;
   __shared__ unsigned long long real_sh;
@@ -1366,7 +1387,9 @@ __global__ void plan_kernel(uint64_t qn, uint64_t q0, uint64_t G, int l, uint64_
                             uint64_t* __restrict__ padmax, const uint32_t* __restrict__ n_ib, int inbox_mode,
                             uint32_t adc_chunk, uint32_t* __restrict__ sslot, uint32_t* __restrict__ smm,
                             uint32_t* __restrict__ scnt, uint32_t* __restrict__ slot_ctr, uint32_t slot_cap,
-                            uint64_t split_min) {
+                            uint64_t split_min,
+    const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
   uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (qi >= qn) return;
   sslot[qi] = kNone;
@@ -2388,13 +2411,20 @@ __global__ __launch_bounds__(256) void final_pick_wave_kernel(
     const uint32_t* __restrict__ L, const uint32_t* __restrict__ nL, const uint64_t* __restrict__ pad32,
     const uint64_t* __restrict__ padmax, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ adist,
     const uint64_t* __restrict__ aoff, float* __restrict__ out_dist, uint64_t* __restrict__ out_ids,
-    const uint32_t* __restrict__ abort_flag) {
+    const uint32_t* __restrict__ abort_flag, uint32_t* __restrict__ left, uint32_t* __restrict__ abort_on_left) {
   if (abort_flag && *abort_flag) return;
   const int lane = threadIdx.x & 63;
   const uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (qi >= qn) return;
   const uint32_t n = nL[qi];
-  if (n > kWavePickMax) return;
+  if (n > kWavePickMax) {
+    // left to the workgroup kernel: counted; a replayed graph that left that kernel out is abandoned
+    if (lane == 0) {
+      atomicAdd(left, 1u);
+      if (abort_on_left) *abort_on_left = 1u;
+    }
+    return;
+  }
   const uint32_t* list = L + loff[qi];
   float* od = out_dist + (q0 + qi) * topk;
   uint64_t* oi = out_ids + (q0 + qi) * topk;
@@ -2449,7 +2479,7 @@ struct Index::Work {
   DevBuf redo;                                               // queries the wave walk hands to the workgroup walk
   DevBuf ib_pos, ib_sub, n_ib;                               // in-box points listed by the walk
   DevBuf sub_list, sub_cnt, ns, list_c, cl_beg, nnz_c;       // non-zero sub-clusters / clusters
-  DevBuf chosen_c, n_chosen_c, list_s, list_scnt, nnz_s, chosen_s, n_chosen_s, chosen_sid;
+  DevBuf chosen_c, n_chosen_c, list_s, list_scnt, nnz_s, chosen_s, n_chosen_s;
   DevBuf keys, woff, bsum, gstat, rstart, rpre, cand, real;
   DevBuf mode, kprime, lsize, asize, chunks, pad32, padmax, nL;
   DevBuf L, Lkeys, adist, akeys2;
@@ -2468,6 +2498,7 @@ struct Index::Work {
   bool capturing = false;
   DevBuf abort_flag;
   uint64_t lcap = 0, acap = 0;  // refine-list and ADC capacities of the captured graphs
+  uint32_t need_mask = 0;       // fallback kernels some sizing run needed (bits as the capture's skip mask)
   uint64_t last_totL = 0, last_totA = 0;  // largest totals seen by uncaptured runs
   hipStream_t gstream = nullptr;
   hipEvent_t gev[2];
@@ -2624,8 +2655,9 @@ void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* ql
       if ((uint32_t)w.pinned[5] & 1u) fail(kInvalidArgument, "map filter label outside interval_map");
       if (((uint32_t)w.pinned[4] & 1u) == 0) continue;
     }
-    // uncaptured run: results for this call, buffer sizes for the capture
+    // uncaptured run: results for this call, buffer sizes and the fallback kernels needed for the capture
     query_chunk(queries, q0, qn, qlabels, qcols, G, out_dist, out_ids, s);
+    HDG_HIP(hipMemcpyAsync(&w.pinned[8], w.n_dense.data(), 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     w.lcap = std::max(w.lcap, w.last_totL + w.last_totL / 2 + 4096);
     // a chunk that needed no ADC is captured without the ADC stage (capacity 0: any ADC work aborts the replay)
     if (w.last_totA > 0) w.acap = std::max(w.acap, w.last_totA + w.last_totA / 2 + 4096);
@@ -2634,6 +2666,10 @@ void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* ql
     w.adist.reserve(w.acap * 4);
     w.akeys2.reserve(w.acap * 8);
     HDG_HIP(hipStreamSynchronize(s));
+    {
+      const uint32_t* nd = reinterpret_cast<const uint32_t*>(&w.pinned[8]);
+      w.need_mask |= (nd[0] ? 1u : 0u) | (nd[1] ? 2u : 0u) | (nd[3] ? 4u : 0u) | (nd[4] ? 8u : 0u);
+    }
     if (c.exec) {
       (void)hipGraphExecDestroy(c.exec);
       c.exec = nullptr;
@@ -2733,7 +2769,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.zlo_cnt.reserve(qn * 8);
   w.err.reserve(4);
   w.dense_list.reserve(qn * 4);
-  w.n_dense.reserve(16);
+  w.n_dense.reserve(32);
   w.redo.reserve(qn * 4);
   w.counts.reserve(qn * CS * 4);
   w.ib_pos.reserve(qn * kInboxCap * 4);
@@ -2751,7 +2787,6 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.list_scnt.reserve(qn * cap * 4);
   w.nnz_s.reserve(qn * 4);
   w.chosen_s.reserve(qn * sub_lists * 4);
-  w.chosen_sid.reserve(qn * sub_lists * 4);
   w.n_chosen_s.reserve(qn * 4);
   w.keys.reserve(qn * std::max<uint64_t>(C, cap) * 8);
   w.woff.reserve((qn + 1) * 8);
@@ -2775,16 +2810,19 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.nL.reserve(qn * 4);
   w.bsum.reserve(3 * kScanBlocks * 8);
   if (cdiv(qn, 1024) > (uint64_t)kScanBlocks) fail(kUnsupported, "QUERY_CHUNK above 65536");
-  auto scan_u64 = [&](uint64_t* a0, uint64_t* a1, uint64_t* a2, uint64_t n, const uint32_t* n32, const uint32_t* md,
-                      uint32_t skip) {
-    const unsigned nb = (unsigned)cdiv(n, 1024);
-    hipLaunchKernelGGL(scan_local_kernel, dim3(nb), dim3(1024), 0, s, a0, a1, a2, n, n32, md, skip,
-                       w.bsum.as<unsigned long long>());
-    hipLaunchKernelGGL(scan_fix_kernel, dim3(nb), dim3(1024), 0, s, a0, a1, a2, n, w.bsum.as<unsigned long long>());
-  };
-  unsigned long long* gstat = w.gstat.as<unsigned long long>() + g_base * 8;
   uint32_t* abort_w = w.capturing ? w.abort_flag.as<uint32_t>() : nullptr;
   const uint32_t* abort_p = abort_w;
+  // the chosen sub-cluster ids go straight into the call-wide array read by last_chosen_subs
+  uint32_t* csid = w.st_chosen.as<uint32_t>() + q0 * sub_lists;
+  auto scan_u64 = [&](uint64_t* a0, uint64_t* a1, uint64_t* a2, uint64_t n, const uint32_t* n32, const uint32_t* md,
+                      uint32_t skip_mode) {
+    const unsigned nb = (unsigned)cdiv(n, 1024);
+    hipLaunchKernelGGL(scan_local_kernel, dim3(nb), dim3(1024), 0, s, a0, a1, a2, n, n32, md, skip_mode,
+                       w.bsum.as<unsigned long long>(), abort_p);
+    hipLaunchKernelGGL(scan_fix_kernel, dim3(nb), dim3(1024), 0, s, a0, a1, a2, n, w.bsum.as<unsigned long long>(),
+                       abort_p);
+  };
+  unsigned long long* gstat = w.gstat.as<unsigned long long>() + g_base * 8;
   hipLaunchKernelGGL(clear_kernel, dim3((unsigned)cdiv(ng * 8, 256)), dim3(256), 0, s, gstat, ng * 8,
                      w.err.as<uint32_t>(), w.n_dense.as<uint32_t>(), abort_w);
   HDG_HIP(hipGetLastError());
@@ -2817,6 +2855,19 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     budget_T = w.budget_T.as<uint32_t>();
     budget_S = w.budget_S.as<uint32_t>();
   }
+  // PRECISION 1 with every non-zero cluster probed (PROBE_CLUSTERS and the budget model's cluster ladder at
+  // least CLUSTERS): the chosen clusters' non-zero sub-clusters are all non-zero sub-clusters, so the walk's
+  // ascending list is scored as it is (the order only breaks exact score ties) and the cluster choice and
+  // expansion are skipped.
+  const uint32_t kc = cfg_.probe_clusters > 0 ? std::min<uint32_t>(n_list, (uint32_t)cfg_.probe_clusters) : n_list;
+  const bool all_clusters = mixed && kc >= C && (!budget_T || budget_probes_all((uint32_t)C));
+  // Graph capture leaves out the fallback kernels the sizing runs never needed (bits: 1 dense count, 2 workgroup
+  // walk, 4 workgroup scored selection, 8 workgroup final pick); a kernel that would hand such a kernel a query
+  // sets the abort flag instead, every later kernel returns, and the chunk is rerun uncaptured.  Bits 1, 2 and 4
+  // need every kernel between the walk and the capacity check to honour the flag: the PRECISION 1 path with
+  // in-box refinement, wave selection and every cluster probed.
+  const bool lean_ok = mixed && inbox_mode && cfg_.select_wave && all_clusters;
+  const uint32_t skip = w.capturing && cfg_.select_wave ? (~w.need_mask & (lean_ok ? 0xFu : 0x8u)) : 0u;
   // PRECISION 1 scores queries x (sub-)centers in blocks of SCORE_BLOCK queries through one score buffer, small
   // enough to stay in the memory-side cache between the GEMM that writes it and the selection that reads it
   const uint64_t score_block = std::min<uint64_t>(qn, std::max<uint64_t>(cfg_.score_block, 128));
@@ -2865,7 +2916,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                            w.sub_cnt.as<uint32_t>(), w.ns.as<uint32_t>(), w.list_c.as<uint32_t>(),                 \
                            w.cl_beg.as<uint32_t>(), w.nnz_c.as<uint32_t>(), w.dense_list.as<uint32_t>(),           \
                            w.n_dense.as<uint32_t>(), w.gstat.as<unsigned long long>(), nzb, nzp,                   \
-                           w.redo.as<uint32_t>());                                                                 \
+                           w.redo.as<uint32_t>(), skip & 3u, abort_w);                                             \
+      if (!cfg_.select_wave || !(skip & 2u))                                                                       \
       hipLaunchKernelGGL(sparse_count_kernel<NV>, dim3(cfg_.select_wave ? rgrid : grid), dim3(256), lds, s, B,      \
                          S.bstart.as<uint32_t>(),                                                                  \
                          S.ecell.as<uint32_t>(), S.esub.as<uint32_t>(), S.N, S.sub_off.as<uint32_t>(),             \
@@ -2875,15 +2927,16 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                          w.list_c.as<uint32_t>(), w.cl_beg.as<uint32_t>(), w.nnz_c.as<uint32_t>(),                 \
                          w.dense_list.as<uint32_t>(), w.n_dense.as<uint32_t>(), w.gstat.as<unsigned long long>(), \
                          nzb, nzp, cfg_.select_wave ? w.redo.as<uint32_t>() : nullptr,                            \
-                         cfg_.select_wave ? w.n_dense.as<uint32_t>() + 1 : nullptr);                              \
+                         cfg_.select_wave ? w.n_dense.as<uint32_t>() + 1 : nullptr, abort_p);                     \
     } else {                                                                                                       \
       hipLaunchKernelGGL(iota_kernel, dim3((unsigned)cdiv(qn, 256)), dim3(256), 0, s, qn,                          \
                          w.dense_list.as<uint32_t>(), w.n_dense.as<uint32_t>(), w.n_ib.as<uint32_t>());           \
     }                                                                                                              \
+    if (!(skip & 1u))                                                                                              \
     hipLaunchKernelGGL(dense_count_kernel<NV>, dgrid, dim3(256), 0, s, cells, S.N, S.zcode.as<uint64_t>(),          \
                        S.seg_off.as<uint32_t>(), CS, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(),              \
                        w.zlo_cnt.as<uint64_t>(), w.zhi.as<uint64_t>(), qn, w.dense_list.as<uint32_t>(),            \
-                       w.n_dense.as<uint32_t>(), w.counts.as<uint32_t>());                                         \
+                       w.n_dense.as<uint32_t>(), w.counts.as<uint32_t>(), abort_p);                                \
     break;
       HDG_A3_CASE(1)
       HDG_A3_CASE(2)
@@ -2898,11 +2951,12 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
         fail(kBadState, "no valid label dims");
     }
     HDG_HIP(hipGetLastError());
+    if (!(skip & 1u))
     hipLaunchKernelGGL(dense_emit_kernel, dim3((unsigned)qn), dim3(256), 0, s, w.counts.as<uint32_t>(), (uint32_t)CS,
                        S.sub_off.as<uint32_t>(), (uint32_t)C, G, q0, w.dense_list.as<uint32_t>(),
                        w.n_dense.as<uint32_t>(), w.sub_list.as<uint32_t>(), w.sub_cnt.as<uint32_t>(),
                        w.ns.as<uint32_t>(), w.list_c.as<uint32_t>(), w.cl_beg.as<uint32_t>(), w.nnz_c.as<uint32_t>(),
-                       w.gstat.as<unsigned long long>(), nzb, nzp);
+                       w.gstat.as<unsigned long long>(), nzb, nzp, abort_p);
     HDG_HIP(hipGetLastError());
   }
   mark();
@@ -2913,14 +2967,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     capture_features(q0, qn, l, w.ranges.as<float>(), w.box_w.as<uint32_t>(), g.n_valid, w.feat.as<float>(), s);
     if (budget_T)
       run_budget_model(qn, l, w.ranges.as<float>(), w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), g.n_valid,
-                       w.feat.as<float>(), w.n_chosen_s.as<uint32_t>(), budget_T, budget_S, s);
+                       w.feat.as<float>(), w.n_chosen_s.as<uint32_t>(), budget_T, budget_S, s, abort_p);
   };
-  // PRECISION 1 with every non-zero cluster probed (PROBE_CLUSTERS and the budget model's cluster ladder at
-  // least CLUSTERS): the chosen clusters' non-zero sub-clusters are all non-zero sub-clusters, so the walk's
-  // ascending list is scored as it is (the order only breaks exact score ties) and the cluster choice and
-  // expansion are skipped.
-  const uint32_t kc = cfg_.probe_clusters > 0 ? std::min<uint32_t>(n_list, (uint32_t)cfg_.probe_clusters) : n_list;
-  const bool all_clusters = mixed && kc >= C && (!budget_T || budget_probes_all((uint32_t)C));
   const uint32_t* Ls = all_clusters ? w.sub_list.as<uint32_t>() : w.list_s.as<uint32_t>();
   const uint32_t* Lcnt = all_clusters ? w.sub_cnt.as<uint32_t>() : w.list_scnt.as<uint32_t>();
   const uint32_t* Lnnz = all_clusters ? w.ns.as<uint32_t>() : w.nnz_s.as<uint32_t>();
@@ -2939,11 +2987,12 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     for (uint64_t b0 = 0; b0 < qn; b0 += score_block) {
       const uint64_t bn = std::min(score_block, qn - b0);
       launch_coarse_gemm(w.qb16.as<uint16_t>() + b0 * S.Dp, bn, S.Dp, S.centers_bf16.as<uint16_t>(),
-                         S.center_norm.as<float>(), (uint32_t)C, w.S.as<float>(), s);
+                         S.center_norm.as<float>(), (uint32_t)C, w.S.as<float>(), s, nullptr, nullptr, 0, abort_p);
       launch_scored_select(queries, q0 + b0, bn, d, w.list_c.as<uint32_t>() + b0 * C, (uint32_t)C,
                            w.nnz_c.as<uint32_t>() + b0, w.S.as<float>(), C, kc, nullptr, n_list,
                            w.chosen_c.as<uint32_t>() + b0 * n_list, w.n_chosen_c.as<uint32_t>() + b0, nullptr,
-                           w.keys.as<uint64_t>() + b0 * C, cfg_.select_wave != 0, s);
+                           w.keys.as<uint64_t>() + b0 * C, cfg_.select_wave != 0, s, nullptr, nullptr, 0, abort_w,
+                           w.n_dense.as<uint32_t>() + 3, (skip & 4u) != 0);
     }
   }
   HDG_HIP(hipGetLastError());
@@ -2970,12 +3019,12 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
       const uint64_t bn = std::min(score_block, qn - b0);
       launch_coarse_gemm(w.qb16.as<uint16_t>() + b0 * S.Dp, bn, S.Dp, S.sub_centers_bf16.as<uint16_t>(),
                          S.sub_center_norm.as<float>(), (uint32_t)CS, w.S.as<float>(), s, nzb + b0 * nws,
-                         nzp + b0 * nws, nws);
+                         nzp + b0 * nws, nws, abort_p);
       launch_scored_select(queries, q0 + b0, bn, d, Ls + b0 * cap, cap, Lnnz + b0, w.S.as<float>(), CS, sub_lists,
                            nullptr, sub_lists, w.chosen_s.as<uint32_t>() + b0 * sub_lists,
                            w.n_chosen_s.as<uint32_t>() + b0, need_feat ? w.feat.as<float>() + b0 * kFeatDists : nullptr,
                            w.keys.as<uint64_t>() + b0 * cap, cfg_.select_wave != 0, s, nzb + b0 * nws,
-                           nzp + b0 * nws, nws);
+                           nzp + b0 * nws, nws, abort_w, w.n_dense.as<uint32_t>() + 3, (skip & 4u) != 0);
     }
     budgets();
   }
@@ -2983,16 +3032,21 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   mark();
   if (inbox_mode && cfg_.select_wave)
     hipLaunchKernelGGL(ranges_inbox_wave_kernel, dim3((unsigned)cdiv(qn, 4)), dim3(256), 0, s, qn, cap, Ls, Lcnt,
-                       sub_lists, w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.chosen_sid.as<uint32_t>(),
-                       w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.real.as<uint64_t>(), w.n_ib.as<uint32_t>());
+                       sub_lists, w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), csid,
+                       w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.real.as<uint64_t>(), w.n_ib.as<uint32_t>(),
+                       abort_p);
+  // without dense-counted or workgroup-walked queries every query was refined from its in-box list by the
+  // wave kernels: the Morton-range kernels below have nothing to do
+  const bool all_walked = (skip & 3u) == 3u;
+  if (!all_walked)
   hipLaunchKernelGGL(ranges_kernel, dim3((unsigned)qn), dim3(256), 0, s, S.zcode.as<uint64_t>(),
                      S.seg_off.as<uint32_t>(), S.cells.as<uint32_t>(), S.N, g.n_valid, w.box_lo.as<uint32_t>(),
                      w.box_w.as<uint32_t>(), qn, w.zlo_cnt.as<uint64_t>(), cap, Ls, Lcnt, sub_lists, G,
                      q0, w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.zlo.as<uint64_t>(),
-                     w.zhi.as<uint64_t>(), w.chosen_sid.as<uint32_t>(), w.rstart.as<uint32_t>(),
+                     w.zhi.as<uint64_t>(), csid, w.rstart.as<uint32_t>(),
                      w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.real.as<uint64_t>(),
                      w.gstat.as<unsigned long long>(), w.n_ib.as<uint32_t>(),
-                     inbox_mode && cfg_.select_wave ? 2 : inbox_mode);
+                     inbox_mode && cfg_.select_wave ? 2 : inbox_mode, abort_p);
   HDG_HIP(hipGetLastError());
   mark();
   hipLaunchKernelGGL(plan_kernel, dim3((unsigned)cdiv(qn, 128)), dim3(128), 0, s, qn, q0, G, l, cfg_.exps0,
@@ -3000,7 +3054,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(),
                      w.chunks.as<uint64_t>(), w.pad32.as<uint64_t>(), w.padmax.as<uint64_t>(), w.n_ib.as<uint32_t>(),
                      inbox_mode, adc_per * (uint32_t)kWG, w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(),
-                     w.scnt.as<uint32_t>(), w.n_dense.as<uint32_t>() + 2, sel_slots, cfg_.adc_split);
+                     w.scnt.as<uint32_t>(), w.n_dense.as<uint32_t>() + 2, sel_slots, cfg_.adc_split, abort_p);
   HDG_HIP(hipGetLastError());
   scan_u64(w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(), w.chunks.as<uint64_t>(), qn, nullptr, nullptr, 0);
   HDG_HIP(hipGetLastError());
@@ -3036,15 +3090,16 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   if (sparse && cfg_.select_wave)
     hipLaunchKernelGGL(inbox_keep_wave_kernel, dim3((unsigned)cdiv(qn, 4)), dim3(256), 0, s, qn, l, integral,
                        (uint32_t)CS, sub_lists, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
-                       w.zlo_cnt.as<uint64_t>(), w.n_chosen_s.as<uint32_t>(), w.chosen_sid.as<uint32_t>(), kInboxCap,
+                       w.zlo_cnt.as<uint64_t>(), w.n_chosen_s.as<uint32_t>(), csid, kInboxCap,
                        w.ib_pos.as<uint32_t>(), w.ib_sub.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(),
                        w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
   else if (sparse)
     hipLaunchKernelGGL(inbox_keep_kernel, dim3((unsigned)qn), dim3(256), 0, s, l, integral, (uint32_t)CS, sub_lists,
                        w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(), w.zlo_cnt.as<uint64_t>(),
-                       w.n_chosen_s.as<uint32_t>(), w.chosen_sid.as<uint32_t>(), kInboxCap, w.ib_pos.as<uint32_t>(),
+                       w.n_chosen_s.as<uint32_t>(), csid, kInboxCap, w.ib_pos.as<uint32_t>(),
                        w.ib_sub.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
                        w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
+  if (!all_walked)
   hipLaunchKernelGGL(keep_all_kernel, dim3((unsigned)qn), dim3(256), (2 * sub_lists + 1) * 4, s, qn, l, integral,
                      sub_lists, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
                      w.zlo_cnt.as<uint64_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),
@@ -3138,9 +3193,11 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                        S.zid.as<uint32_t>(), l, topk, w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(),
                        w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), w.pad32.as<uint64_t>(),
                        w.padmax.as<uint64_t>(), w.Lkeys.as<uint64_t>(), w.adist.as<uint32_t>(),
-                       w.asize.as<uint64_t>(), out_dist, out_ids, abort_p);
+                       w.asize.as<uint64_t>(), out_dist, out_ids, abort_p, w.n_dense.as<uint32_t>() + 4,
+                       (skip & 8u) ? abort_w : nullptr);
     pick_skip = kWavePickMax;
   }
+  if (!(skip & 8u))
   hipLaunchKernelGGL(final_pick_kernel, dim3((unsigned)qn), dim3(256), scap_f * 8, s, q0, scap_f, S.zid.as<uint32_t>(), l,
                      topk,
                      w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(),
@@ -3152,8 +3209,6 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      w.real.as<uint64_t>(), w.nL.as<uint32_t>(), w.nnz_c.as<uint32_t>(), Lnnz,
                      w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.st.as<uint64_t>(), abort_p);
   HDG_HIP(hipGetLastError());
-  HDG_HIP(hipMemcpyAsync(w.st_chosen.as<uint32_t>() + q0 * sub_lists, w.chosen_sid.data(), qn * sub_lists * 4,
-                         hipMemcpyDeviceToDevice, s));
   static const char* plain_names[] = {"prep_kernel",     "count_kernels",   "coarse_dist+pick", "sub_expand_kernel",
                                       "sub_dist+pick",   "ranges_kernel",   "plan+sync",        "keep_all_kernel",
                                       "adc_scan_kernel", "adc_select_kernel", "final_scan", "final_dist_kernel",
