@@ -85,13 +85,6 @@ __global__ void clear_kernel(unsigned long long* __restrict__ gstat, uint64_t n,
   }
 }
 
-// Graph replay: the refine and ADC buffers were captured with fixed capacities; a chunk needing more sets
-// the abort flag, every later kernel of the chunk returns at once, and the host runs it again uncaptured.
-__global__ void cap_check_kernel(const uint64_t* __restrict__ ltot, const uint64_t* __restrict__ atot,
-                                 uint64_t lcap, uint64_t acap, uint32_t* __restrict__ abort_flag) {
-  if (threadIdx.x == 0 && blockIdx.x == 0 && (*ltot > lcap || *atot > acap)) *abort_flag = 1u;
-}
-
 // Every query of the chunk goes to the dense count (no bucket grid, or too many sub-clusters for LDS).
 __global__ void iota_kernel(uint64_t qn, uint32_t* __restrict__ dense_list, uint32_t* __restrict__ n_dense,
                             uint32_t* __restrict__ n_ib) {
@@ -1017,6 +1010,7 @@ __global__ __launch_bounds__(1024) void scan_local_kernel(uint64_t* __restrict__
 __global__ __launch_bounds__(1024) void scan_fix_kernel(uint64_t* __restrict__ v0, uint64_t* __restrict__ v1,
                                                         uint64_t* __restrict__ v2, uint64_t n,
                                                         const unsigned long long* __restrict__ bsum,
+                                                        uint64_t cap0, uint64_t cap1, uint32_t* __restrict__ cap_abort,
     const uint32_t* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
   uint64_t* const vs[3] = {v0, v1, v2};
@@ -1032,7 +1026,11 @@ __global__ __launch_bounds__(1024) void scan_fix_kernel(uint64_t* __restrict__ v
       all += t;
     }
     if (i < n) vs[a][i] += before;
-    if (blockIdx.x == 0 && threadIdx.x == 0) vs[a][n] = all;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      vs[a][n] = all;
+      // graph replay: arrays 0 / 1 are the refine-list and ADC sizes, captured with capacities cap0 / cap1
+      if (cap_abort && ((a == 0 && all > cap0) || (a == 1 && all > cap1))) *cap_abort = 1u;
+    }
   }
 }
 
@@ -2411,12 +2409,20 @@ __global__ __launch_bounds__(256) void final_pick_wave_kernel(
     const uint32_t* __restrict__ L, const uint32_t* __restrict__ nL, const uint64_t* __restrict__ pad32,
     const uint64_t* __restrict__ padmax, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ adist,
     const uint64_t* __restrict__ aoff, float* __restrict__ out_dist, uint64_t* __restrict__ out_ids,
-    const uint32_t* __restrict__ abort_flag, uint32_t* __restrict__ left, uint32_t* __restrict__ abort_on_left) {
+    const uint32_t* __restrict__ abort_flag, uint32_t* __restrict__ left, uint32_t* __restrict__ abort_on_left,
+    const uint64_t* __restrict__ cand, const uint64_t* __restrict__ real, const uint32_t* __restrict__ nnz_c,
+    const uint32_t* __restrict__ nnz_s, uint64_t* __restrict__ st) {
   if (abort_flag && *abort_flag) return;
   const int lane = threadIdx.x & 63;
   const uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (qi >= qn) return;
   const uint32_t n = nL[qi];
+  if (lane < 6) {
+    // the query's counters (as stats_kernel)
+    const uint64_t v[6] = {cand[qi], real[qi], n, (uint64_t)nnz_c[qi] + nnz_s[qi], mode[qi] == 1 ? cand[qi] : 0,
+                           kprime[qi]};
+    st[(q0 + qi) * 6 + lane] = v[lane];
+  }
   if (n > kWavePickMax) {
     // left to the workgroup kernel: counted; a replayed graph that left that kernel out is abandoned
     if (lane == 0) {
@@ -2815,12 +2821,12 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   // the chosen sub-cluster ids go straight into the call-wide array read by last_chosen_subs
   uint32_t* csid = w.st_chosen.as<uint32_t>() + q0 * sub_lists;
   auto scan_u64 = [&](uint64_t* a0, uint64_t* a1, uint64_t* a2, uint64_t n, const uint32_t* n32, const uint32_t* md,
-                      uint32_t skip_mode) {
+                      uint32_t skip_mode, uint32_t* cap_abort = nullptr) {
     const unsigned nb = (unsigned)cdiv(n, 1024);
     hipLaunchKernelGGL(scan_local_kernel, dim3(nb), dim3(1024), 0, s, a0, a1, a2, n, n32, md, skip_mode,
                        w.bsum.as<unsigned long long>(), abort_p);
     hipLaunchKernelGGL(scan_fix_kernel, dim3(nb), dim3(1024), 0, s, a0, a1, a2, n, w.bsum.as<unsigned long long>(),
-                       abort_p);
+                       w.lcap, w.acap, cap_abort, abort_p);
   };
   unsigned long long* gstat = w.gstat.as<unsigned long long>() + g_base * 8;
   hipLaunchKernelGGL(clear_kernel, dim3((unsigned)cdiv(ng * 8, 256)), dim3(256), 0, s, gstat, ng * 8,
@@ -3056,7 +3062,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      inbox_mode, adc_per * (uint32_t)kWG, w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(),
                      w.scnt.as<uint32_t>(), w.n_dense.as<uint32_t>() + 2, sel_slots, cfg_.adc_split, abort_p);
   HDG_HIP(hipGetLastError());
-  scan_u64(w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(), w.chunks.as<uint64_t>(), qn, nullptr, nullptr, 0);
+  // captured: fixed capacities; a chunk that needs more raises the abort flag in the scan's fix-up
+  scan_u64(w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(), w.chunks.as<uint64_t>(), qn, nullptr, nullptr, 0, abort_w);
   HDG_HIP(hipGetLastError());
   uint64_t totL, totA, totChunks;
   if (!w.capturing) {
@@ -3074,10 +3081,6 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     w.last_totL = std::max(w.last_totL, totL);
     w.last_totA = std::max(w.last_totA, totA);
   } else {
-    // captured: fixed capacities; a chunk that needs more raises the abort flag
-    hipLaunchKernelGGL(cap_check_kernel, dim3(1), dim3(64), 0, s, w.lsize.as<uint64_t>() + qn,
-                       w.asize.as<uint64_t>() + qn, w.lcap, w.acap, abort_w);
-    HDG_HIP(hipGetLastError());
     totL = w.lcap;
     totA = w.acap;
     totChunks = w.acap ? 1 : 0;  // the persistent ADC grid reads the real count on the device
@@ -3194,7 +3197,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                        w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), w.pad32.as<uint64_t>(),
                        w.padmax.as<uint64_t>(), w.Lkeys.as<uint64_t>(), w.adist.as<uint32_t>(),
                        w.asize.as<uint64_t>(), out_dist, out_ids, abort_p, w.n_dense.as<uint32_t>() + 4,
-                       (skip & 8u) ? abort_w : nullptr);
+                       (skip & 8u) ? abort_w : nullptr, w.cand.as<uint64_t>(), w.real.as<uint64_t>(),
+                       w.nnz_c.as<uint32_t>(), Lnnz, w.st.as<uint64_t>());
     pick_skip = kWavePickMax;
   }
   if (!(skip & 8u))
@@ -3205,6 +3209,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      w.adist.as<uint32_t>(), w.asize.as<uint64_t>(), out_dist, out_ids, abort_p, pick_skip);
   HDG_HIP(hipGetLastError());
   mark();
+  if (!cfg_.select_wave)  // the wave final pick writes the counters itself
   hipLaunchKernelGGL(stats_kernel, dim3((unsigned)cdiv(qn, 256)), dim3(256), 0, s, qn, q0, w.cand.as<uint64_t>(),
                      w.real.as<uint64_t>(), w.nL.as<uint32_t>(), w.nnz_c.as<uint32_t>(), Lnnz,
                      w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.st.as<uint64_t>(), abort_p);
