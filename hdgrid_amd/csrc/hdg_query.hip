@@ -1318,17 +1318,40 @@ __global__ __launch_bounds__(256) void ranges_kernel(
       cs_sh[threadIdx.x] = cstart;
       cp_sh[threadIdx.x] = cpre;
       __syncthreads();
+      // a thread's positions t ascend: one search for the owner of the first, then step forward; four
+      // positions per step, each dim's four cell loads in flight together
       uint32_t mine = 0;
-      for (uint32_t t = threadIdx.x; t < ctot; t += kWG) {
-        uint32_t x = 0, y = kWG;  // last owner k with cp_sh[k] <= t
+      uint32_t x = 0;  // last owner k with cp_sh[k] <= t
+      {
+        uint32_t y = kWG;
         while (y - x > 1) {
           const uint32_t mid = (x + y) >> 1;
-          if (cp_sh[mid] <= t) x = mid; else y = mid;
+          if (cp_sh[mid] <= threadIdx.x) x = mid; else y = mid;
         }
-        const uint32_t p = cs_sh[x] + (t - cp_sh[x]);
-        bool in = true;
-        for (int j2 = 0; j2 < n_valid && in; ++j2) in = cells[(uint64_t)j2 * N + p] - blo[j2] <= bw[j2];
-        mine += in ? 1u : 0u;
+      }
+      for (uint32_t t0 = threadIdx.x; t0 < ctot; t0 += 4 * kWG) {
+        uint32_t p[4];
+        bool in[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t t = t0 + (uint32_t)u * kWG;
+          in[u] = t < ctot;
+          p[u] = 0;
+          if (in[u]) {
+            while (x + 1 < (uint32_t)kWG && cp_sh[x + 1] <= t) ++x;
+            p[u] = cs_sh[x] + (t - cp_sh[x]);
+          }
+        }
+        for (int j2 = 0; j2 < n_valid; ++j2) {
+          const uint32_t* cb = cells + (uint64_t)j2 * N;
+          uint32_t c[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) c[u] = in[u] ? cb[p[u]] : 0u;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) in[u] = in[u] && c[u] - blo[j2] <= bw[j2];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) mine += in[u] ? 1u : 0u;
       }
       atomicAdd(&real_sh, (unsigned long long)mine);
       __syncthreads();
