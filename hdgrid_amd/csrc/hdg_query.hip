@@ -1534,8 +1534,9 @@ __global__ __launch_bounds__(256) void keep_all_kernel(
     if (threadIdx.x == 0) nL[qi] = Cq;
     return;
   }
-  float rg[2 * kMaxDims];
-  for (int i = 0; i < 2 * l; ++i) rg[i] = ranges[qi * 2 * l + i];
+  __shared__ float rg[2 * kMaxDims];
+  if (threadIdx.x < 2 * (uint32_t)l) rg[threadIdx.x] = ranges[qi * 2 * l + threadIdx.x];
+  __syncthreads();
   // each thread owns kKeepPer consecutive columns, so all their label loads are in flight together and
   // one scan per kWG * kKeepPer columns keeps the column order
   constexpr int kKeepPer = 8;
@@ -1557,13 +1558,37 @@ __global__ __launch_bounds__(256) void keep_all_kernel(
         while (j + 1 < nr && rp[j + 1] <= col) ++j;
         pos[i] = rs[j] + (col - rp[j]);
       }
+      // the label filter (label_pass) with every item's loads issued before the tests
 #pragma unroll
-      for (int i = 0; i < kKeepPer; ++i) {
-        if (c0 + i < Cq && label_pass(lab, l, pos[i], rg)) {
-          flags |= 1u << i;
-          ++cnt;
+      for (int i = 0; i < kKeepPer; ++i)
+        if (c0 + i < Cq) flags |= 1u << i;
+      if (l == 4) {
+#pragma unroll
+        for (int i = 0; i < kKeepPer; i += 4) {
+          float4 lv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            lv[u] = (flags >> (i + u)) & 1u ? reinterpret_cast<const float4*>(lab)[pos[i + u]]
+                                            : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const bool fail = lv[u].x < rg[0] || lv[u].x > rg[1] || lv[u].y < rg[2] || lv[u].y > rg[3] ||
+                              lv[u].z < rg[4] || lv[u].z > rg[5] || lv[u].w < rg[6] || lv[u].w > rg[7];
+            if (fail) flags &= ~(1u << (i + u));
+          }
+        }
+      } else {
+        for (int dm = 0; dm < l; ++dm) {
+          float lv[kKeepPer];
+#pragma unroll
+          for (int i = 0; i < kKeepPer; ++i) lv[i] = (flags >> i) & 1u ? lab[(uint64_t)pos[i] * l + dm] : 0.f;
+          const float lo_b = rg[2 * dm], hi_b = rg[2 * dm + 1];
+#pragma unroll
+          for (int i = 0; i < kKeepPer; ++i)
+            if (lv[i] < lo_b || lv[i] > hi_b) flags &= ~(1u << i);
         }
       }
+      cnt = (uint32_t)__popc(flags);
     }
     uint32_t total;
     uint32_t at = base + dev::wg_exclusive_scan(cnt, tmp, total);
