@@ -173,12 +173,17 @@ __global__ void prep_kernel(const float* __restrict__ qlabels, uint64_t qcols, u
     box_lo[(uint64_t)j * qn + qi] = (uint32_t)lo_c[i];
     box_w[(uint64_t)j * qn + qi] = (uint32_t)hi_c[i] - (uint32_t)lo_c[i];
   }
-  zlo[qi] = morton_code(lo_c, l, D.max_bits, D.valid_map);
+  const uint64_t za = morton_code(lo_c, l, D.max_bits, D.valid_map);
+  zlo[qi] = za;
   zhi[qi] = morton_code(hi_c, l, D.max_bits, D.valid_map);
   // Morton range used by the count: a lower corner that wrapped negative (a truncated u64 cell id)
   // still admits every cell >= 0, so its code is taken at 0.
-  for (int i = 0; i < l; ++i) lo_c[i] = lo_c[i] < 0 ? 0 : lo_c[i];
-  zlo_cnt[qi] = morton_code(lo_c, l, D.max_bits, D.valid_map);
+  bool wrapped = false;
+  for (int i = 0; i < l; ++i) {
+    wrapped = wrapped || lo_c[i] < 0;
+    lo_c[i] = lo_c[i] < 0 ? 0 : lo_c[i];
+  }
+  zlo_cnt[qi] = wrapped ? morton_code(lo_c, l, D.max_bits, D.valid_map) : za;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -2893,7 +2898,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     if (!w.capturing) HDG_HIP(hipEventRecord(w.ev[evn++], s));
   };
   mark();
-  hipLaunchKernelGGL(prep_kernel, dim3((unsigned)cdiv(qn, 128)), dim3(128), 0, s, qlabels, qcols, q0, qn, G, D,
+  // one thread per query, 64-thread blocks so that a 10k-query chunk spreads over more CUs
+  hipLaunchKernelGGL(prep_kernel, dim3((unsigned)cdiv(qn, 64)), dim3(64), 0, s, qlabels, qcols, q0, qn, G, D,
                      f_type_.as<int>(), f_shift_.as<float>(), f_map_.as<float>(), f_map_len_, f_div_.as<int>(),
                      w.ranges.as<float>(), w.boxes.as<int>(), w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(),
                      w.zlo.as<uint64_t>(), w.zhi.as<uint64_t>(), w.zlo_cnt.as<uint64_t>(), w.err.as<uint32_t>());
@@ -3103,7 +3109,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      inbox_mode && cfg_.select_wave ? 2 : inbox_mode, abort_p);
   HDG_HIP(hipGetLastError());
   mark();
-  hipLaunchKernelGGL(plan_kernel, dim3((unsigned)cdiv(qn, 128)), dim3(128), 0, s, qn, q0, G, l, cfg_.exps0,
+  hipLaunchKernelGGL(plan_kernel, dim3((unsigned)cdiv(qn, 64)), dim3(64), 0, s, qn, q0, G, l, cfg_.exps0,
                      topk, w.cand.as<uint64_t>(), w.gstat.as<unsigned long long>(), w.mode.as<uint32_t>(),
                      w.kprime.as<uint64_t>(), w.lsize.as<uint64_t>(), w.asize.as<uint64_t>(),
                      w.chunks.as<uint64_t>(), w.pad32.as<uint64_t>(), w.padmax.as<uint64_t>(), w.n_ib.as<uint32_t>(),
