@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
                                                             uint64_t* __restrict__ scratch, uint32_t scap,
                                                             int64_t skip_upto, const uint32_t* __restrict__ nzb,
                                                             const uint32_t* __restrict__ nzp, uint32_t nw,
-                                                            const uint32_t* __restrict__ abort_flag) {
+                                                            const uint32_t* __restrict__ abort_flag, bool by_pos) {
   if (abort_flag && *abort_flag) return;
   extern __shared__ __align__(16) uint64_t keys[];  // scap keys
   __shared__ __align__(16) uint32_t hist[2048];
@@ -237,13 +237,13 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
   uint32_t m = n < k ? n : k;
   if (n <= scap) {
     for (uint32_t j = threadIdx.x; j < n; j += kWG)
-      keys[j] = ((uint64_t)ord(item_score(row, lst[j], nzb_q, nzp_q)) << 32) | j;
+      keys[j] = ((uint64_t)ord(by_pos ? row[j] : item_score(row, lst[j], nzb_q, nzp_q)) << 32) | j;
     __syncthreads();
     dev::wg_sort_lds(keys, n);
   } else {
     uint64_t* g = scratch + qi * cap;
     for (uint32_t j = threadIdx.x; j < n; j += kWG)
-      g[j] = ((uint64_t)ord(item_score(row, lst[j], nzb_q, nzp_q)) << 32) | j;
+      g[j] = ((uint64_t)ord(by_pos ? row[j] : item_score(row, lst[j], nzb_q, nzp_q)) << 32) | j;
     __syncthreads();
     if (m > scap) m = scap;
     dev::wg_radix_select(dev::KeyArray{g}, n, m, keys, hist, sh);
@@ -279,12 +279,14 @@ constexpr uint32_t kWaveSelMax = 512;
 
 template <int R>
 __device__ void wave_select(const uint32_t* lst, const float* row, const uint32_t* nzb_q, const uint32_t* nzp_q,
-                            uint32_t n, uint32_t m, int lane, uint32_t* out, uint32_t avail, uint64_t& feat_key) {
+                            bool by_pos, uint32_t n, uint32_t m, int lane, uint32_t* out, uint32_t avail,
+                            uint64_t& feat_key) {
   uint64_t key[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)lane * R + r;
-    key[r] = e < n ? (((uint64_t)ord(item_score(row, lst[e], nzb_q, nzp_q)) << 32) | e) : ~0ull;
+    const float v = e < n ? (by_pos ? row[e] : item_score(row, lst[e], nzb_q, nzp_q)) : 0.0f;
+    key[r] = e < n ? (((uint64_t)ord(v) << 32) | e) : ~0ull;
   }
   dev::wave_bitonic<R>(key, lane);
 #pragma unroll
@@ -305,7 +307,7 @@ __global__ __launch_bounds__(256) void scored_select_wave_kernel(
     uint32_t k_default, const uint32_t* __restrict__ budget, uint32_t out_ld, uint32_t* __restrict__ chosen,
     uint32_t* __restrict__ n_chosen, float* __restrict__ feat_dist, const uint32_t* __restrict__ nzb,
     const uint32_t* __restrict__ nzp, uint32_t nw, uint32_t* __restrict__ abort_flag, uint32_t* __restrict__ left,
-    bool abort_on_left) {
+    bool abort_on_left, bool by_pos) {
   if (abort_flag && *abort_flag) return;
   const int lane = threadIdx.x & 63;
   const uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -332,10 +334,10 @@ __global__ __launch_bounds__(256) void scored_select_wave_kernel(
   uint32_t* out = chosen + qi * out_ld;
   const uint32_t avail = feat_dist ? (n < 32 ? n : 32) : 0;
   uint64_t fk = 0;
-  if (n <= 64) wave_select<1>(lst, row, nzb_q, nzp_q, n, m, lane, out, avail, fk);
-  else if (n <= 128) wave_select<2>(lst, row, nzb_q, nzp_q, n, m, lane, out, avail, fk);
-  else if (n <= 256) wave_select<4>(lst, row, nzb_q, nzp_q, n, m, lane, out, avail, fk);
-  else wave_select<8>(lst, row, nzb_q, nzp_q, n, m, lane, out, avail, fk);
+  if (n <= 64) wave_select<1>(lst, row, nzb_q, nzp_q, by_pos, n, m, lane, out, avail, fk);
+  else if (n <= 128) wave_select<2>(lst, row, nzb_q, nzp_q, by_pos, n, m, lane, out, avail, fk);
+  else if (n <= 256) wave_select<4>(lst, row, nzb_q, nzp_q, by_pos, n, m, lane, out, avail, fk);
+  else wave_select<8>(lst, row, nzb_q, nzp_q, by_pos, n, m, lane, out, avail, fk);
   if (lane == 0) n_chosen[qi] = m;
   if (feat_dist) {
     const float qq = wave_sq_norm(queries + (q0 + qi) * d, d, lane);
@@ -358,20 +360,20 @@ void launch_scored_select(const float* queries, uint64_t q0, uint64_t qn, uint64
                           const uint32_t* budget, uint32_t out_ld, uint32_t* chosen, uint32_t* n_chosen,
                           float* feat_dist, uint64_t* scratch, bool wave, hipStream_t s, const uint32_t* nzb,
                           const uint32_t* nzp, uint32_t nw, uint32_t* abort_flag, uint32_t* left,
-                          bool skip_fallback) {
+                          bool skip_fallback, bool by_pos) {
   uint32_t scap = 1024;
   while (scap < k_default) scap <<= 1;
   int64_t skip_upto = -1;
   if (wave) {
     hipLaunchKernelGGL(scored_select_wave_kernel, dim3((unsigned)((qn + 3) / 4)), dim3(256), 0, s, queries, q0, qn,
                        d, list, cap, n_list_q, S, ld, k_default, budget, out_ld, chosen, n_chosen, feat_dist, nzb,
-                       nzp, nw, abort_flag, left, skip_fallback && abort_flag);
+                       nzp, nw, abort_flag, left, skip_fallback && abort_flag, by_pos);
     skip_upto = kWaveSelMax;
     if (skip_fallback && abort_flag) return;  // captured without the workgroup kernel
   }
   hipLaunchKernelGGL(scored_select_kernel, dim3((unsigned)qn), dim3(256), scap * 8, s, queries, q0, d, list, cap,
                      n_list_q, S, ld, k_default, budget, out_ld, chosen, n_chosen, feat_dist, scratch, scap,
-                     skip_upto, nzb, nzp, nw, abort_flag);
+                     skip_upto, nzb, nzp, nw, abort_flag, by_pos);
 }
 
 // 8 elements per thread (two float4 loads, one 16-byte store) when d and Dp are multiples of 8

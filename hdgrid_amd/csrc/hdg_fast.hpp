@@ -28,11 +28,13 @@ void launch_coarse_gemm(const uint16_t* qb16, uint64_t qn, uint64_t Dp, const ui
 // compact rows (launch_coarse_gemm) and every listed item is one of the query's non-zero items.
 // abort_flag: every kernel returns at once when it is set (graph replay); `left` counts the queries the wave
 // kernel leaves to the workgroup kernel; skip_fallback (with abort_flag) launches no workgroup kernel and
-// sets the flag instead when a query needs it.
+// sets the flag instead when a query needs it.  by_pos: the list is exactly the query's non-zero items in
+// ascending order, so item e's compact score sits at row position e (no rank lookup).
 void launch_scored_select(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, const uint32_t* list,
                           uint32_t cap, const uint32_t* n_list_q, const float* S, uint64_t ld, uint32_t k_default,
                           const uint32_t* budget, uint32_t out_ld, uint32_t* chosen, uint32_t* n_chosen,
                           float* feat_dist, uint64_t* scratch, bool wave, hipStream_t s,
                           const uint32_t* nzb = nullptr, const uint32_t* nzp = nullptr, uint32_t nw = 0,
-                          uint32_t* abort_flag = nullptr, uint32_t* left = nullptr, bool skip_fallback = false);
+                          uint32_t* abort_flag = nullptr, uint32_t* left = nullptr, bool skip_fallback = false,
+                          bool by_pos = false);
 }  // namespace hdg

@@ -3084,7 +3084,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                            nullptr, sub_lists, w.chosen_s.as<uint32_t>() + b0 * sub_lists,
                            w.n_chosen_s.as<uint32_t>() + b0, need_feat ? w.feat.as<float>() + b0 * kFeatDists : nullptr,
                            w.keys.as<uint64_t>() + b0 * cap, cfg_.select_wave != 0, s, nzb + b0 * nws,
-                           nzp + b0 * nws, nws, abort_w, w.n_dense.as<uint32_t>() + 3, (skip & 4u) != 0);
+                           nzp + b0 * nws, nws, abort_w, w.n_dense.as<uint32_t>() + 3, (skip & 4u) != 0,
+                           all_clusters);
     }
     budgets();
   }
