@@ -39,8 +39,9 @@ __global__ void to_bf16_kernel(const float* __restrict__ queries, uint64_t q0, u
   out[i] = (uint16_t)f32_to_bf16(t < d ? queries[(q0 + q) * d + t] : 0.0f);
 }
 
-// S[q][c] = cnorm[c] - 2 * <q, c> with bf16 operands and f32 accumulation.  Block = 4 waves = 128 queries
-// x 128 centers; wave (wm, wn) owns 64 x 64 as 4 x 4 accumulators of v_mfma_f32_16x16x32_bf16 (lane l
+// S[q][c] = cnorm[c] - 2 * <q, c> with bf16 operands and f32 accumulation.  Block = 8 waves = 256 queries
+// x 128 centers (each center piece staged once for twice the queries); wave (wm, wn) owns 64 x 64 as 4 x 4
+// accumulators of v_mfma_f32_16x16x32_bf16 (lane l
 // supplies A[l%16][8*(l/16)..+7] and B[8*(l/16)..+7][l%16], receives D[4*(l/16)..+3][l%16]).  K advances
 // in steps of 32 through LDS tiles of both operands (rows padded to 40 bf16); the next step's tiles are
 // loaded into registers while the current step's MFMAs run.
@@ -48,13 +49,15 @@ __global__ void to_bf16_kernel(const float* __restrict__ queries, uint64_t q0, u
 // Tile order: workgroup w runs on XCD w % 8 as that XCD's (w / 8)-th workgroup.  w is mapped to a position
 // p = (w % 8) * per + w / 8 of a grouped order that takes kGemmGroup query tiles for each center tile, so the
 // workgroups resident on one XCD at a time cover a few query tiles x a dozen center tiles and find their
-// operand tiles in that XCD's L2 (one 128 x 128 tile reads 192 KiB of operands).
-constexpr int kGemmTile = 128;
+// operand tiles in that XCD's L2 (one 256 x 128 tile reads 288 KiB of operands).
+constexpr int kGemmTile = 128;   // centers per tile
+constexpr int kGemmTileQ = 256;  // queries per tile: 8 waves as 4 x 2
+constexpr int kGemmThreads = 512;
 constexpr int kGemmLd = 40;
 constexpr uint32_t kGemmGroup = 4;
 constexpr uint32_t kXcds = 8;
 
-__global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __restrict__ qb16, uint64_t qn, uint64_t Dp,
+__global__ __launch_bounds__(512) void coarse_gemm_kernel(const uint16_t* __restrict__ qb16, uint64_t qn, uint64_t Dp,
                                                           const uint16_t* __restrict__ cb16,
                                                           const float* __restrict__ cnorm, uint32_t C,
                                                           float* __restrict__ S, uint32_t QT, uint32_t CT,
@@ -68,25 +71,28 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
   const uint32_t gsize = QT - first < kGemmGroup ? QT - first : kGemmGroup;
   const uint32_t inner = p % span;
   // operand tiles, reused by the epilogue as 4 waves x 16 rows x 65 floats
-  __shared__ __align__(16) uint16_t smem[2 * kGemmTile * kGemmLd];
-  // compact rows: the tile's 128 rows x 4 bit words of non-zero items and their ranks
-  __shared__ uint32_t nz_bits[kGemmTile * 4], nz_rank[kGemmTile * 4];
+  constexpr int kOperands = (kGemmTileQ + kGemmTile) * kGemmLd, kEpilogue = 8 * 16 * 65 * 2;
+  __shared__ __align__(16) uint16_t smem[kOperands > kEpilogue ? kOperands : kEpilogue];
+  // compact rows: the tile's 256 rows x 4 bit words of non-zero items and their ranks
+  __shared__ uint32_t nz_bits[kGemmTileQ * 4], nz_rank[kGemmTileQ * 4];
   uint16_t* As = smem;
-  uint16_t* Bs = smem + kGemmTile * kGemmLd;
+  uint16_t* Bs = smem + kGemmTileQ * kGemmLd;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = wave >> 1, wn = wave & 1;
-  const uint64_t qbase = (uint64_t)(first + inner % gsize) * kGemmTile;
+  const uint64_t qbase = (uint64_t)(first + inner % gsize) * kGemmTileQ;
   const uint64_t cbase = (uint64_t)(inner / gsize) * kGemmTile;
   // this thread's two 16-byte pieces of each operand tile: row = idx / 4, piece = idx % 4
-  uint4 ra[2], rb[2];
+  // this thread's 16-byte pieces of the operand tiles: two of A (256 rows x 4), one of B (128 rows x 4)
+  uint4 ra[2], rb;
   auto load = [&](uint64_t k0) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      const int idx = tid + i * 256, r = idx >> 2, pc = idx & 3;
-      const uint64_t q = qbase + r, c = cbase + r;
+      const int idx = tid + i * kGemmThreads, r = idx >> 2, pc = idx & 3;
+      const uint64_t q = qbase + r;
       ra[i] = q < qn ? *reinterpret_cast<const uint4*>(qb16 + q * Dp + k0 + pc * 8) : make_uint4(0, 0, 0, 0);
-      rb[i] = c < C ? *reinterpret_cast<const uint4*>(cb16 + c * Dp + k0 + pc * 8) : make_uint4(0, 0, 0, 0);
     }
+    const uint64_t c = cbase + (tid >> 2);
+    rb = c < C ? *reinterpret_cast<const uint4*>(cb16 + c * Dp + k0 + (tid & 3) * 8) : make_uint4(0, 0, 0, 0);
   };
   f32x4 acc[4][4];
 #pragma unroll
@@ -98,7 +104,7 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
   if (nzb) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      const int idx = tid + i * 256, r = idx >> 2, wd = idx & 3;
+      const int idx = tid + i * kGemmThreads, r = idx >> 2, wd = idx & 3;
       const uint64_t q = qbase + r;
       const uint64_t wi = (cbase >> 5) + (uint64_t)wd;
       if (q < qn && wi < nw) {
@@ -112,10 +118,10 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
   for (uint64_t k0 = 0; k0 < Dp; k0 += 32) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      const int idx = tid + i * 256, r = idx >> 2, pc = idx & 3;
+      const int idx = tid + i * kGemmThreads, r = idx >> 2, pc = idx & 3;
       *reinterpret_cast<uint4*>(As + r * kGemmLd + pc * 8) = ra[i];
-      *reinterpret_cast<uint4*>(Bs + r * kGemmLd + pc * 8) = rb[i];
     }
+    *reinterpret_cast<uint4*>(Bs + (tid >> 2) * kGemmLd + (tid & 3) * 8) = rb;
     __syncthreads();
     if (k0 + 32 < Dp) load(k0 + 32);
     bf16x8 a[4], b[4];
@@ -133,7 +139,7 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const uint16_t* __rest
   if (nzb) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      const int idx = tid + i * 256;
+      const int idx = tid + i * kGemmThreads;
       nz_bits[idx] = nzv[2 * i];
       nz_rank[idx] = nzv[2 * i + 1];
     }
@@ -410,10 +416,10 @@ void launch_to_bf16(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, 
 void launch_coarse_gemm(const uint16_t* qb16, uint64_t qn, uint64_t Dp, const uint16_t* cb16, const float* cnorm,
                         uint32_t C, float* S, hipStream_t s, const uint32_t* nzb, const uint32_t* nzp, uint32_t nw,
                         const uint32_t* abort_flag) {
-  const uint64_t QT = (qn + kGemmTile - 1) / kGemmTile, CT = (C + kGemmTile - 1) / kGemmTile;
+  const uint64_t QT = (qn + kGemmTileQ - 1) / kGemmTileQ, CT = (C + kGemmTile - 1) / kGemmTile;
   if (QT * CT > 0xFFFFFFFFull / 2) fail(kUnsupported, "coarse GEMM grid too large");
   const uint32_t per = (uint32_t)((QT * CT + kXcds - 1) / kXcds);
-  hipLaunchKernelGGL(coarse_gemm_kernel, dim3(per * kXcds), dim3(256), 0, s, qb16, qn, Dp, cb16, cnorm, C, S,
+  hipLaunchKernelGGL(coarse_gemm_kernel, dim3(per * kXcds), dim3(kGemmThreads), 0, s, qb16, qn, Dp, cb16, cnorm, C, S,
                      (uint32_t)QT, (uint32_t)CT, per, nzb, nzp, nw, (uint64_t)C, abort_flag);
 }
 
