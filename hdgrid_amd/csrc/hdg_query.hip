@@ -282,15 +282,18 @@ __global__ __launch_bounds__(256) void sparse_count_kernel(
         // four entries per thread per pass, so their loads are in flight together
         for (uint32_t t0 = 0; t0 < total; t0 += 4 * kWG) {
           uint32_t ev[4], cv[4][NV], sv[4], pv[4];
+          // last k with pre_sh[k] <= t (that thread's range holds t): the four searches step together
+          static_assert(kWG == 256, "eight search steps");
+          uint32_t a[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+          for (uint32_t step = kWG / 2; step > 0; step >>= 1)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (pre_sh[a[u] + step] <= t0 + (uint32_t)tid + (uint32_t)(kWG * u)) a[u] += step;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const uint32_t t = t0 + (uint32_t)tid + (uint32_t)(kWG * u);
-            uint32_t a = 0, b = kWG;  // last k with pre_sh[k] <= t; that thread's range holds t
-            while (b - a > 1) {
-              const uint32_t m = (a + b) >> 1;
-              if (pre_sh[m] <= t) a = m; else b = m;
-            }
-            ev[u] = t < total ? e0_sh[a] + (t - pre_sh[a]) : kNone;
+            ev[u] = t < total ? e0_sh[a[u]] + (t - pre_sh[a[u]]) : kNone;
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
@@ -528,15 +531,17 @@ __global__ __launch_bounds__(256) void sparse_count_wave_kernel(
         // four entries per lane per pass, so their loads are in flight together
         for (uint32_t t0 = 0; t0 < total; t0 += 256) {
           uint32_t ev[4], cv[4][NV], sv[4], pv[4];
+          // last k with pre_sh[k] <= t (that lane's range holds t): the four searches step together
+          uint32_t a[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+          for (uint32_t step = 32; step > 0; step >>= 1)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (pre_sh[a[u] + step] <= t0 + (uint32_t)lane + 64u * u) a[u] += step;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const uint32_t t = t0 + (uint32_t)lane + 64u * u;
-            uint32_t a = 0, b = 64;  // last k with pre_sh[k] <= t; that lane's range holds t
-            while (b - a > 1) {
-              const uint32_t m = (a + b) >> 1;
-              if (pre_sh[m] <= t) a = m; else b = m;
-            }
-            ev[u] = t < total ? e0_sh[a] + (t - pre_sh[a]) : kNone;
+            ev[u] = t < total ? e0_sh[a[u]] + (t - pre_sh[a[u]]) : kNone;
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
