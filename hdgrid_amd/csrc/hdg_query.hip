@@ -908,8 +908,15 @@ __global__ __launch_bounds__(64) void pair_dist_kernel(const float* __restrict__
           const uint32_t e = lane + it * 64;
           const uint32_t r = e >> 4, c4 = (e & 15) * 4;
           const uint32_t row = rows[r];
-          pre[it] = (row != kNone && t0 + c4 < d) ? *reinterpret_cast<const float4*>(X + (uint64_t)row * d + t0 + c4)
-                                                  : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          // re-rank (KIND 1): data rows are rarely read twice, so they are streamed past the caches; the
+          // (sub-)centers of KIND 0 are shared by many queries and stay cached
+          typedef float nt4 __attribute__((ext_vector_type(4)));
+          nt4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+          if (row != kNone && t0 + c4 < d) {
+            const nt4* src = reinterpret_cast<const nt4*>(X + (uint64_t)row * d + t0 + c4);
+            v = KIND == 1 ? __builtin_nontemporal_load(src) : *src;
+          }
+          pre[it] = make_float4(v.x, v.y, v.z, v.w);
         }
       };
       load_stage(0);
