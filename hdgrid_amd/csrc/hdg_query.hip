@@ -1711,7 +1711,21 @@ __global__ __launch_bounds__(256) void inbox_keep_wave_kernel(
     if (i < n) {
       const uint32_t sid = ib_sub[qi * ibcap + i];
       pos = ib_pos[qi * ibcap + i];
-      flag = ((bits[sid >> 5] >> (sid & 31)) & 1u) && ((l == 1 && md != 3) || label_pass(lab, l, pos, rg));
+      flag = ((bits[sid >> 5] >> (sid & 31)) & 1u) != 0;
+      if (flag && !(l == 1 && md != 3)) {
+        if (l <= 4) {
+          // label_pass with the point's labels loaded together
+          const float* lp = lab + (uint64_t)pos * l;
+          float v[4];
+#pragma unroll
+          for (int dm = 0; dm < 4; ++dm) v[dm] = dm < l ? lp[dm] : 0.0f;
+#pragma unroll
+          for (int dm = 0; dm < 4; ++dm)
+            if (dm < l && (v[dm] < rg[2 * dm] || v[dm] > rg[2 * dm + 1])) flag = false;
+        } else {
+          flag = label_pass(lab, l, pos, rg);
+        }
+      }
     }
     const uint64_t mask = __ballot(flag);
     if (flag) out[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = pos;
