@@ -1502,10 +1502,23 @@ __device__ __forceinline__ uint32_t col_to_pos(const uint32_t* rs, const uint32_
   return rs[lo] + (col - rp[lo]);
 }
 
+// The exact label predicate lo <= label <= hi in every dim (filter.cu:30-64).  Up to four dims are loaded
+// together (one memory round trip); more dims are tested one by one.
 __device__ __forceinline__ bool label_pass(const float* __restrict__ lab, int l, uint32_t pos,
                                            const float* rg) {
+  const float* lp = lab + (uint64_t)pos * l;
+  if (l <= 4) {
+    float v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = i < l ? lp[i] : 0.0f;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < l && (v[i] < rg[2 * i] || v[i] > rg[2 * i + 1])) ok = false;
+    return ok;
+  }
   for (int i = 0; i < l; ++i) {
-    float v = lab[(uint64_t)pos * l + i];
+    const float v = lp[i];
     if (v < rg[2 * i] || v > rg[2 * i + 1]) return false;
   }
   return true;
@@ -1711,21 +1724,7 @@ __global__ __launch_bounds__(256) void inbox_keep_wave_kernel(
     if (i < n) {
       const uint32_t sid = ib_sub[qi * ibcap + i];
       pos = ib_pos[qi * ibcap + i];
-      flag = ((bits[sid >> 5] >> (sid & 31)) & 1u) != 0;
-      if (flag && !(l == 1 && md != 3)) {
-        if (l <= 4) {
-          // label_pass with the point's labels loaded together
-          const float* lp = lab + (uint64_t)pos * l;
-          float v[4];
-#pragma unroll
-          for (int dm = 0; dm < 4; ++dm) v[dm] = dm < l ? lp[dm] : 0.0f;
-#pragma unroll
-          for (int dm = 0; dm < 4; ++dm)
-            if (dm < l && (v[dm] < rg[2 * dm] || v[dm] > rg[2 * dm + 1])) flag = false;
-        } else {
-          flag = label_pass(lab, l, pos, rg);
-        }
-      }
+      flag = ((bits[sid >> 5] >> (sid & 31)) & 1u) && ((l == 1 && md != 3) || label_pass(lab, l, pos, rg));
     }
     const uint64_t mask = __ballot(flag);
     if (flag) out[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = pos;
