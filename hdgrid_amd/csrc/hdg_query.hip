@@ -865,7 +865,11 @@ __device__ __forceinline__ float sq_chain4(float acc, float4 qq, float4 x) {
   return acc;
 }
 
-template <int KIND>
+// Rows of at least this many dims are streamed past the caches by the exact re-rank (measured: 1,536-byte rows
+// of the 1M x 384 workload gain, 512-byte rows of the 5M x 128 one, which neighbouring queries share, lose).
+constexpr uint64_t kStreamRowsMinDim = 256;
+
+template <int KIND, bool STREAM = false>
 __global__ __launch_bounds__(64) void pair_dist_kernel(const float* __restrict__ queries, uint64_t q0, uint64_t d,
                                                        const float* __restrict__ X, const uint64_t* __restrict__ woff,
                                                        uint64_t qn, const uint32_t* __restrict__ list,
@@ -908,13 +912,14 @@ __global__ __launch_bounds__(64) void pair_dist_kernel(const float* __restrict__
           const uint32_t e = lane + it * 64;
           const uint32_t r = e >> 4, c4 = (e & 15) * 4;
           const uint32_t row = rows[r];
-          // re-rank (KIND 1): data rows are rarely read twice, so they are streamed past the caches; the
-          // (sub-)centers of KIND 0 are shared by many queries and stay cached
+          // STREAM (re-rank of long rows): data rows are rarely read twice, so they are streamed past the
+          // caches; (sub-)centers and short rows that neighbouring queries share stay cached
           typedef float nt4 __attribute__((ext_vector_type(4)));
           nt4 v = {0.0f, 0.0f, 0.0f, 0.0f};
           if (row != kNone && t0 + c4 < d) {
             const nt4* src = reinterpret_cast<const nt4*>(X + (uint64_t)row * d + t0 + c4);
-            v = KIND == 1 ? __builtin_nontemporal_load(src) : *src;
+            if constexpr (STREAM) v = __builtin_nontemporal_load(src);
+            else v = *src;
           }
           pre[it] = make_float4(v.x, v.y, v.z, v.w);
         }
@@ -2329,10 +2334,23 @@ __global__ __launch_bounds__(256) void adc_select_kernel(
   float rg[2 * kMaxDims];
   for (int i = 0; i < 2 * l; ++i) rg[i] = ranges[qi * 2 * l + i];
   if (threadIdx.x == 0) cnt_sh = 0;
+  // the range table in LDS (the histogram's space, free after the selection) for the column lookups
+  const bool table_lds = 2 * nr + 1 <= (uint32_t)kSelBins;
+  const uint32_t* rs_c = rs;
+  const uint32_t* rp_c = rp;
+  if (table_lds) {
+    __syncthreads();  // the selection is done with the histogram
+    for (uint32_t j = threadIdx.x; j <= nr; j += kWG) {
+      hist[nr + j] = rp[j];
+      if (j < nr) hist[j] = rs[j];
+    }
+    rs_c = hist;
+    rp_c = hist + nr;
+  }
   __syncthreads();
   uint32_t* out = L + loff[qi] + listed;
   for (uint32_t e = threadIdx.x; e < nsel; e += kWG) {
-    const uint32_t pos = col_to_pos(rs, rp, nr, dev::key_tie(sel[e]));
+    const uint32_t pos = col_to_pos(rs_c, rp_c, nr, dev::key_tie(sel[e]));
     if (l == 1) {
       out[e] = pos;
     } else if (label_pass(lab, l, pos, rg)) {
@@ -3266,9 +3284,14 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   // ---- A13: exact re-rank of the refine lists, final top-k ----
   scan_u64(w.woff.as<uint64_t>(), nullptr, nullptr, qn, w.nL.as<uint32_t>(), w.mode.as<uint32_t>(), 2u);
   mark();
-  hipLaunchKernelGGL(pair_dist_kernel<1>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.vec.as<float>(),
-                     w.woff.as<uint64_t>(), qn, w.L.as<uint32_t>(), (uint64_t)0, w.lsize.as<uint64_t>(),
-                     w.nL.as<uint32_t>(), S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>(), abort_p);
+  if (d >= kStreamRowsMinDim)
+    hipLaunchKernelGGL((pair_dist_kernel<1, true>), dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.vec.as<float>(),
+                       w.woff.as<uint64_t>(), qn, w.L.as<uint32_t>(), (uint64_t)0, w.lsize.as<uint64_t>(),
+                       w.nL.as<uint32_t>(), S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>(), abort_p);
+  else
+    hipLaunchKernelGGL((pair_dist_kernel<1, false>), dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.vec.as<float>(),
+                       w.woff.as<uint64_t>(), qn, w.L.as<uint32_t>(), (uint64_t)0, w.lsize.as<uint64_t>(),
+                       w.nL.as<uint32_t>(), S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>(), abort_p);
   mark();
   int64_t pick_skip = -1;
   if (cfg_.select_wave) {
