@@ -282,6 +282,7 @@ void build_bucket_grid(IndexState& st, const std::vector<uint32_t>& h_cells, hip
   const int nv = g.n_valid;
   const uint64_t N = st.N;
   st.Nb = 0;
+  st.sat_per = 0;
   if (nv == 0 || N == 0) return;
   // about four points per bucket; the share is handed out from the dim with the fewest cells up, so a
   // dim with few cells keeps one bucket per cell and leaves the rest to the wide dims
@@ -323,6 +324,33 @@ void build_bucket_grid(IndexState& st, const std::vector<uint32_t>& h_cells, hip
   upload(st.epos, epos, s);
   upload(st.ecell, ecell, s);
   upload(st.esub, esub, s);
+  // coarse occupancy sums: 2^(12 / n_valid) coarse cells per dim (at most the dim's cells), counted per
+  // sub-cluster, then prefix-summed along each dim
+  const int gb = std::max(1, 12 / nv);
+  for (int j = 0; j < nv; ++j) st.sg[j] = std::min<uint32_t>(1u << gb, st.Mv[j]);
+  st.sstride[nv - 1] = 1;
+  for (int j = nv - 2; j >= 0; --j) st.sstride[j] = st.sstride[j + 1] * (st.sg[j + 1] + 1);
+  const uint64_t per = st.sstride[0] * (st.sg[0] + 1);
+  if (per * st.CS > (1ull << 26)) return;  // more than 256 MiB: the dense count scans without it
+  std::vector<uint32_t> t(per * st.CS, 0);
+  sub = 0;
+  for (uint64_t p = 0; p < N; ++p) {
+    while (st.h_seg_off[sub + 1] <= p) ++sub;
+    uint64_t x = 0;
+    for (int j = 0; j < nv; ++j)
+      x += ((uint64_t)h_cells[(uint64_t)j * N + p] * st.sg[j] / st.Mv[j] + 1) * st.sstride[j];
+    t[sub * per + x]++;
+  }
+  for (int j = 0; j < nv; ++j) {
+    const uint64_t w = st.sg[j] + 1, sj = st.sstride[j];
+    for (uint64_t c = 0; c < st.CS; ++c) {
+      uint32_t* tc = t.data() + c * per;
+      for (uint64_t x = 0; x < per; ++x)
+        if ((x / sj) % w != 0) tc[x] += tc[x - sj];
+    }
+  }
+  upload(st.sat, t, s);
+  st.sat_per = per;
 }
 
 // rows [n][d] f32 -> bf16 [n][Dp] (round to nearest even, zero padded) and |row_bf16|^2 in f32.

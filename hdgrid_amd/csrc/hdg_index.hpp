@@ -59,6 +59,13 @@ struct IndexState {
   DevBuf ecell;       // u32 [n_valid][N] cells of the entries
   DevBuf esub;        // u32 [N] sub-cluster (table position) of the entries
   DevBuf epos;        // u32 [N] z-pool position of the entries
+  // coarse occupancy sums per sub-cluster for the dense count: coarse cell g_j = cell_j * sg[j] / Mv[j];
+  // table [CS][sat_per] of inclusive prefix sums over the (sg + 1)^n_valid grid shifted by one per dim
+  // (entry x counts the points with coarse cells < x); sat_per = 0 when the table would be too large
+  uint32_t sg[kMaxDims] = {0};
+  uint64_t sstride[kMaxDims] = {0};
+  uint64_t sat_per = 0;
+  DevBuf sat;         // u32 [CS][sat_per]
   // every label value is an integer: then a point passing the exact label predicate lies in the query's
   // cell box, so the in-box points of the chosen sub-clusters are a superset of the keep-all survivors
   bool labels_integral = false;

@@ -692,6 +692,38 @@ __global__ __launch_bounds__(256) void sparse_count_wave_kernel(
 // code range in LDS.  counts[slot][s] = 0 / 1, slot = position in dense_list.
 constexpr int kCountPiece = 1024;
 
+// Coarse occupancy sums of the index (per sub-cluster inclusive prefix sums over coarse cells, shifted by one).
+struct CoarseSums {
+  const uint32_t* t = nullptr;  // [CS][per], or null: no table
+  uint64_t per = 0;
+  uint32_t sg[kMaxDims] = {0}, Mv[kMaxDims] = {0};
+  uint64_t stride[kMaxDims] = {0};
+};
+
+// Points of a sub-cluster whose coarse cells lie in [a_j, b_j] in every dim: inclusion-exclusion over the
+// 2^NV corners of its prefix-sum table t.
+template <int NV>
+__device__ __forceinline__ uint32_t coarse_rect(const uint32_t* __restrict__ t, const uint64_t* stride,
+                                                const uint32_t* a, const uint32_t* b) {
+  int64_t sum = 0;
+#pragma unroll
+  for (uint32_t m = 0; m < (1u << NV); ++m) {
+    uint64_t x = 0;
+    int low = 0;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      if ((m >> j) & 1u) {
+        x += ((uint64_t)b[j] + 1) * stride[j];
+      } else {
+        x += (uint64_t)a[j] * stride[j];
+        ++low;
+      }
+    }
+    sum += (low & 1) ? -(int64_t)t[x] : (int64_t)t[x];
+  }
+  return (uint32_t)sum;
+}
+
 template <int NV>
 __global__ __launch_bounds__(256) void dense_count_kernel(const uint32_t* __restrict__ cells, uint64_t N,
                                                           const uint64_t* __restrict__ zcode,
@@ -703,7 +735,7 @@ __global__ __launch_bounds__(256) void dense_count_kernel(const uint32_t* __rest
                                                           const uint32_t* __restrict__ dense_list,
                                                           const uint32_t* __restrict__ n_dense,
                                                           uint32_t* __restrict__ counts,
-    const uint32_t* __restrict__ abort_flag) {
+    const uint32_t* __restrict__ abort_flag, CoarseSums sums) {
   if (abort_flag && *abort_flag) return;
   __shared__ uint64_t zs[kCountPiece];
   __shared__ uint32_t cs[NV][kCountPiece];
@@ -719,21 +751,48 @@ __global__ __launch_bounds__(256) void dense_count_kernel(const uint32_t* __rest
     w[j] = live ? box_w[(uint64_t)j * qn + q] : 0u;
   }
   const uint64_t za = live ? zlo[q] : ~0ull, zb = live ? zhi[q] : 0ull;
+  // the box in coarse cells: every coarse cell it touches (o) and the ones it covers whole (i)
+  uint32_t oa[NV], ob[NV], ia[NV], ib[NV];
+  bool interior = true, none = !live;
+  if (sums.t) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const int lw = (int)lo[j], h = (int)(lo[j] + w[j]);
+      const int a = lw < 0 ? 0 : lw;  // the box test admits the cells [max(lo, 0), lo + w]
+      if (h < a) none = true;
+      const uint64_t G = sums.sg[j], M = sums.Mv[j];
+      oa[j] = (uint32_t)((uint64_t)(uint32_t)a * G / M);
+      ob[j] = h < a ? oa[j] : (uint32_t)((uint64_t)(uint32_t)h * G / M);
+      // coarse cell g holds the cells [ceil(g M / G), ceil((g + 1) M / G) - 1]
+      const uint64_t first = ((uint64_t)oa[j] * M + G - 1) / G, last = (((uint64_t)ob[j] + 1) * M + G - 1) / G - 1;
+      ia[j] = first >= (uint64_t)a ? oa[j] : oa[j] + 1;
+      const int64_t ibj = last <= (uint64_t)(int64_t)h ? (int64_t)ob[j] : (int64_t)ob[j] - 1;
+      if (ibj < (int64_t)ia[j]) interior = false;
+      ib[j] = ibj < 0 ? 0u : (uint32_t)ibj;
+    }
+  }
   const uint64_t s_begin = (uint64_t)blockIdx.y * kCountSegChunk;
   const uint64_t s_end = s_begin + kCountSegChunk < CS ? s_begin + kCountSegChunk : CS;
   for (uint64_t s = s_begin; s < s_end; ++s) {
     const uint32_t p0 = seg_off[s], p1 = seg_off[s + 1];
-    bool hit = !live;  // idle lanes never hold the block back
+    bool hit = !live;    // idle lanes never hold the block back
+    bool done = none;    // settled without a hit
+    if (sums.t && live && !none) {
+      // a point in a covered coarse cell is in the box; no point in a touched one: none is
+      const uint32_t* t = sums.t + s * sums.per;
+      if (interior && coarse_rect<NV>(t, sums.stride, ia, ib) > 0) hit = true;
+      else if (coarse_rect<NV>(t, sums.stride, oa, ob) == 0) done = true;
+    }
     for (uint32_t base = p0; base < p1; base += kCountPiece) {
       const uint32_t m = p1 - base < (uint32_t)kCountPiece ? p1 - base : (uint32_t)kCountPiece;
-      if (__syncthreads_count(hit ? 0 : 1) == 0) break;  // every lane has its answer
+      if (__syncthreads_count(hit || done ? 0 : 1) == 0) break;  // every lane has its answer
       for (uint32_t i = threadIdx.x; i < m; i += 256) {
         zs[i] = zcode[base + i];
 #pragma unroll
         for (int j = 0; j < NV; ++j) cs[j][i] = cells[(uint64_t)j * N + base + i];
       }
       __syncthreads();
-      if (hit || za > zs[m - 1] || zb < zs[0]) continue;
+      if (hit || done || za > zs[m - 1] || zb < zs[0]) continue;
       uint32_t a = 0, b = m;  // first code >= za
       while (a < b) {
         uint32_t mid = (a + b) >> 1;
@@ -3006,6 +3065,16 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     const unsigned rgrid = (unsigned)std::min<uint64_t>(qn, 256);  // the wave walk's few hand-overs
     const uint32_t* cells = S.cells.as<uint32_t>();
     dim3 dgrid((unsigned)cdiv(qn, 256), (unsigned)cdiv(CS, kCountSegChunk));
+    CoarseSums sums;
+    if (S.sat_per) {
+      sums.t = S.sat.as<uint32_t>();
+      sums.per = S.sat_per;
+      for (int j = 0; j < g.n_valid; ++j) {
+        sums.sg[j] = S.sg[j];
+        sums.Mv[j] = S.Mv[j];
+        sums.stride[j] = S.sstride[j];
+      }
+    }
     switch (g.n_valid) {
 #define HDG_A3_CASE(NV)                                                                                            \
   case NV:                                                                                                         \
@@ -3039,7 +3108,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     hipLaunchKernelGGL(dense_count_kernel<NV>, dgrid, dim3(256), 0, s, cells, S.N, S.zcode.as<uint64_t>(),          \
                        S.seg_off.as<uint32_t>(), CS, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(),              \
                        w.zlo_cnt.as<uint64_t>(), w.zhi.as<uint64_t>(), qn, w.dense_list.as<uint32_t>(),            \
-                       w.n_dense.as<uint32_t>(), w.counts.as<uint32_t>(), abort_p);                                \
+                       w.n_dense.as<uint32_t>(), w.counts.as<uint32_t>(), abort_p, sums);                          \
     break;
       HDG_A3_CASE(1)
       HDG_A3_CASE(2)
