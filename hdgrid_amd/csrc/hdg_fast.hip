@@ -41,10 +41,13 @@ __global__ void to_bf16_kernel(const float* __restrict__ queries, uint64_t q0, u
 
 // S[q][c] = cnorm[c] - 2 * <q, c> with bf16 operands and f32 accumulation.  Block = 8 waves = 256 queries
 // x 128 centers (each center piece staged once for twice the queries); wave (wm, wn) owns 64 x 64 as 4 x 4
-// accumulators of v_mfma_f32_16x16x32_bf16 (lane l
-// supplies A[l%16][8*(l/16)..+7] and B[8*(l/16)..+7][l%16], receives D[4*(l/16)..+3][l%16]).  K advances
-// in steps of 32 through LDS tiles of both operands (rows padded to 40 bf16); the next step's tiles are
-// loaded into registers while the current step's MFMAs run.
+// accumulators of v_mfma_f32_16x16x32_bf16 (lane l supplies A[l%16][8*(l/16)..+7] and B[8*(l/16)..+7][l%16],
+// receives D[4*(l/16)..+3][l%16]).  K advances in steps of 32.  The operand tiles of a step go from global
+// memory straight into LDS (global_load_lds, 16 bytes per lane) through a ring of three step buffers, so the
+// loads of the next two steps are in flight while a step's MFMAs run, without holding them in registers.
+// A step buffer is 16-byte pieces: row r's piece p (bf16 k0 + 8p .. +7) at slot 4r + (p ^ swz(r)) -- the
+// rows have no padding and the XOR swizzle keeps the fragment reads (16 rows x one piece per 16 lanes) on
+// distinct banks.
 //
 // Tile order: workgroup w runs on XCD w % 8 as that XCD's (w / 8)-th workgroup.  w is mapped to a position
 // p = (w % 8) * per + w / 8 of a grouped order that takes kGemmGroup query tiles for each center tile, so the
@@ -53,9 +56,16 @@ __global__ void to_bf16_kernel(const float* __restrict__ queries, uint64_t q0, u
 constexpr int kGemmTile = 128;   // centers per tile
 constexpr int kGemmTileQ = 256;  // queries per tile: 8 waves as 4 x 2
 constexpr int kGemmThreads = 512;
-constexpr int kGemmLd = 40;
+constexpr int kGemmStages = 3;   // step buffers in the LDS ring
+constexpr int kGemmStepPieces = (kGemmTileQ + kGemmTile) * 4;  // 16-byte pieces of one K step
 constexpr uint32_t kGemmGroup = 4;
 constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ int gemm_swz(int r) { return ((r >> 1) & 3) ^ ((r >> 3) & 1); }
+
+typedef __attribute__((address_space(3))) void lds_void;
+typedef const __attribute__((address_space(1))) void global_cvoid;
+
 
 __global__ __launch_bounds__(512) void coarse_gemm_kernel(const uint16_t* __restrict__ qb16, uint64_t qn, uint64_t Dp,
                                                           const uint16_t* __restrict__ cb16,
@@ -70,29 +80,40 @@ __global__ __launch_bounds__(512) void coarse_gemm_kernel(const uint16_t* __rest
   const uint32_t span = kGemmGroup * CT, first = (p / span) * kGemmGroup;
   const uint32_t gsize = QT - first < kGemmGroup ? QT - first : kGemmGroup;
   const uint32_t inner = p % span;
-  // operand tiles, reused by the epilogue as 4 waves x 16 rows x 65 floats
-  constexpr int kOperands = (kGemmTileQ + kGemmTile) * kGemmLd, kEpilogue = 8 * 16 * 65 * 2;
-  __shared__ __align__(16) uint16_t smem[kOperands > kEpilogue ? kOperands : kEpilogue];
+  // the step ring, reused by the epilogue as 8 waves x 16 rows x 65 floats
+  static_assert(kGemmStages * kGemmStepPieces * 16 >= 8 * 16 * 65 * 4, "epilogue does not fit the ring");
+  __shared__ __align__(16) uint4 ring[kGemmStages * kGemmStepPieces];
   // compact rows: the tile's 256 rows x 4 bit words of non-zero items and their ranks
   __shared__ uint32_t nz_bits[kGemmTileQ * 4], nz_rank[kGemmTileQ * 4];
-  uint16_t* As = smem;
-  uint16_t* Bs = smem + kGemmTileQ * kGemmLd;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave >> 1, wn = wave & 1;
   const uint64_t qbase = (uint64_t)(first + inner % gsize) * kGemmTileQ;
   const uint64_t cbase = (uint64_t)(inner / gsize) * kGemmTile;
-  // this thread's two 16-byte pieces of each operand tile: row = idx / 4, piece = idx % 4
-  // this thread's 16-byte pieces of the operand tiles: two of A (256 rows x 4), one of B (128 rows x 4)
-  uint4 ra[2], rb;
-  auto load = [&](uint64_t k0) {
+  // this lane's global sources: two A pieces (slots wave*64 + lane and (wave+8)*64 + lane) and one B piece
+  // (slot wave*64 + lane); rows past the operands read the last row (their results are never stored)
+  const uint16_t* srcA[2];
+  const uint16_t* srcB;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int idx = tid + i * kGemmThreads, r = idx >> 2, pc = idx & 3;
-      const uint64_t q = qbase + r;
-      ra[i] = q < qn ? *reinterpret_cast<const uint4*>(qb16 + q * Dp + k0 + pc * 8) : make_uint4(0, 0, 0, 0);
-    }
-    const uint64_t c = cbase + (tid >> 2);
-    rb = c < C ? *reinterpret_cast<const uint4*>(cb16 + c * Dp + k0 + (tid & 3) * 8) : make_uint4(0, 0, 0, 0);
+  for (int i = 0; i < 2; ++i) {
+    const int sl = (wave + i * 8) * 64 + lane, r = sl >> 2, pc = (sl & 3) ^ gemm_swz(r);
+    const uint64_t q = qbase + r < qn ? qbase + r : qn - 1;
+    srcA[i] = qb16 + q * Dp + pc * 8;
+  }
+  {
+    const int sl = wave * 64 + lane, r = sl >> 2, pc = (sl & 3) ^ gemm_swz(r);
+    const uint64_t c = cbase + r < C ? cbase + r : C - 1;
+    srcB = cb16 + c * Dp + pc * 8;
+  }
+  auto issue = [&](uint32_t ks) {
+    uint4* buf = ring + (ks % kGemmStages) * kGemmStepPieces;
+    const uint64_t k0 = (uint64_t)ks * 32;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      __builtin_amdgcn_global_load_lds((global_cvoid*)(srcA[i] + k0), (lds_void*)(buf + (wave + i * 8) * 64), 16,
+                                       0, 0);
+    __builtin_amdgcn_global_load_lds((global_cvoid*)(srcB + k0), (lds_void*)(buf + kGemmTileQ * 4 + wave * 64), 16,
+                                     0, 0);
   };
   f32x4 acc[4][4];
 #pragma unroll
@@ -113,29 +134,43 @@ __global__ __launch_bounds__(512) void coarse_gemm_kernel(const uint16_t* __rest
       }
     }
   }
-  load(0);
+  const uint32_t KS = (uint32_t)(Dp / 32);
+  issue(0);
+  if (KS > 1) issue(1);
   const int r16 = lane & 15, kg = lane >> 4;
-  for (uint64_t k0 = 0; k0 < Dp; k0 += 32) {
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int idx = tid + i * kGemmThreads, r = idx >> 2, pc = idx & 3;
-      *reinterpret_cast<uint4*>(As + r * kGemmLd + pc * 8) = ra[i];
-    }
-    *reinterpret_cast<uint4*>(Bs + (tid >> 2) * kGemmLd + (tid & 3) * 8) = rb;
-    __syncthreads();
-    if (k0 + 32 < Dp) load(k0 + 32);
+  const int frag = r16 * 4 + (kg ^ gemm_swz(r16));  // slot of this lane's piece in a 16-row group
+  const uint32_t ring_lds = (uint32_t)(uintptr_t)(lds_void*)ring;
+  const uint32_t offA = (uint32_t)((wm * 64 * 4 + frag) * 16), offB = (uint32_t)((kGemmTileQ * 4 + wn * 64 * 4 + frag) * 16);
+  for (uint32_t ks = 0; ks < KS; ++ks) {
+    // step ks has landed once at most the next step's three loads of this lane are outstanding; the barrier
+    // makes every lane's pieces visible and ends every wave's reads of step ks - 1, whose buffer step ks + 2
+    // now refills.  Wait, barrier and fragment reads are written out (the compiler's fence and its alias
+    // check against the LDS loads in flight would wait for every outstanding load instead); the reads wait
+    // for their data in the same block.
+    if (ks + 1 < KS) asm volatile("s_waitcnt vmcnt(3)\n\ts_barrier" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+    if (ks + 2 < KS) issue(ks + 2);
+    const uint32_t bufb = ring_lds + (ks % kGemmStages) * (kGemmStepPieces * 16);
     bf16x8 a[4], b[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      a[i] = *reinterpret_cast<const bf16x8*>(As + (wm * 64 + i * 16 + r16) * kGemmLd + kg * 8);
-      b[i] = *reinterpret_cast<const bf16x8*>(Bs + (wn * 64 + i * 16 + r16) * kGemmLd + kg * 8);
-    }
+    asm volatile(
+        "ds_read_b128 %0, %8\n\t"
+        "ds_read_b128 %1, %8 offset:1024\n\t"
+        "ds_read_b128 %2, %8 offset:2048\n\t"
+        "ds_read_b128 %3, %8 offset:3072\n\t"
+        "ds_read_b128 %4, %9\n\t"
+        "ds_read_b128 %5, %9 offset:1024\n\t"
+        "ds_read_b128 %6, %9 offset:2048\n\t"
+        "ds_read_b128 %7, %9 offset:3072\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(a[0]), "=&v"(a[1]), "=&v"(a[2]), "=&v"(a[3]), "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3])
+        : "v"(bufb + offA), "v"(bufb + offB)
+        : "memory");
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    __syncthreads();
   }
+  __syncthreads();  // every wave is done with the ring before the epilogue reuses it
   if (nzb) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -146,7 +181,7 @@ __global__ __launch_bounds__(512) void coarse_gemm_kernel(const uint16_t* __rest
   }
   // epilogue: each wave passes its 64 x 64 block through LDS 16 rows at a time, so that every store
   // writes 256 contiguous bytes of a row of S
-  float* ep = reinterpret_cast<float*>(smem) + wave * 16 * 65;
+  float* ep = reinterpret_cast<float*>(ring) + wave * 16 * 65;
   float cn[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
