@@ -171,23 +171,47 @@ __global__ __launch_bounds__(512) void coarse_gemm_kernel(const uint16_t* __rest
       for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
   }
   __syncthreads();  // every wave is done with the ring before the epilogue reuses it
-  if (nzb) {
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int idx = tid + i * kGemmThreads;
-      nz_bits[idx] = nzv[2 * i];
-      nz_rank[idx] = nzv[2 * i + 1];
-    }
-  }
-  // epilogue: each wave passes its 64 x 64 block through LDS 16 rows at a time, so that every store
-  // writes 256 contiguous bytes of a row of S
-  float* ep = reinterpret_cast<float*>(ring) + wave * 16 * 65;
   float cn[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const uint64_t c = cbase + wn * 64 + j * 16 + r16;
     cn[j] = c < C ? cnorm[c] : 0.0f;
   }
+  if (nzb) {
+    // compact rows: only the query's non-zero items, at their rank among them.  The stores scatter anyway,
+    // so each lane writes its accumulators directly: row kg*4 + t of block i, column j*16 + r16 of the wave's
+    // 64, which lie in bit words wn*2 (j < 2) and wn*2 + 1 (j >= 2) of the tile's four
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int idx = tid + i * kGemmThreads;
+      nz_bits[idx] = nzv[2 * i];
+      nz_rank[idx] = nzv[2 * i + 1];
+    }
+    __syncthreads();
+    const uint32_t wd0 = (uint32_t)(wn * 2);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint32_t tr = (uint32_t)(wm * 64 + i * 16 + kg * 4 + t);
+        const uint64_t q = qbase + tr;
+        if (q >= qn) continue;
+        const uint32_t w2[2] = {nz_bits[tr * 4 + wd0], nz_bits[tr * 4 + wd0 + 1]};
+        const uint32_t p2[2] = {nz_rank[tr * 4 + wd0], nz_rank[tr * 4 + wd0 + 1]};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t word = w2[j >> 1], bit = (uint32_t)((j & 1) * 16 + r16);
+          const uint64_t c = cbase + wn * 64 + j * 16 + r16;
+          if (c < C && ((word >> bit) & 1u))
+            S[q * ld + p2[j >> 1] + __popc(word & ((1u << bit) - 1u))] = cn[j] - 2.0f * acc[i][j][t];
+        }
+      }
+    }
+    return;
+  }
+  // dense rows: each wave passes its 64 x 64 block through LDS 16 rows at a time, so that every store
+  // writes 256 contiguous bytes of a row of S
+  float* ep = reinterpret_cast<float*>(ring) + wave * 16 * 65;
   const uint64_t cw = cbase + wn * 64 + lane;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -196,22 +220,9 @@ __global__ __launch_bounds__(512) void coarse_gemm_kernel(const uint16_t* __rest
 #pragma unroll
       for (int t = 0; t < 4; ++t) ep[(kg * 4 + t) * 65 + j * 16 + r16] = cn[j] - 2.0f * acc[i][j][t];
     __syncthreads();
-    if (!nzb) {
-      for (int r = 0; r < 16; ++r) {
-        const uint64_t q = qbase + wm * 64 + i * 16 + r;
-        if (q < qn && cw < C) S[q * ld + cw] = ep[r * 65 + lane];
-      }
-    } else {
-      // compact rows: only the query's non-zero items, at their rank among them
-      const uint32_t b = (uint32_t)cw & 31u, wd = (uint32_t)(wn * 2) + (uint32_t)(lane >> 5);
-      for (int r = 0; r < 16; ++r) {
-        const uint32_t tr = (uint32_t)(wm * 64 + i * 16 + r);
-        const uint64_t q = qbase + tr;
-        if (q < qn && cw < C) {
-          const uint32_t word = nz_bits[tr * 4 + wd];
-          if ((word >> b) & 1u) S[q * ld + nz_rank[tr * 4 + wd] + __popc(word & ((1u << b) - 1u))] = ep[r * 65 + lane];
-        }
-      }
+    for (int r = 0; r < 16; ++r) {
+      const uint64_t q = qbase + wm * 64 + i * 16 + r;
+      if (q < qn && cw < C) S[q * ld + cw] = ep[r * 65 + lane];
     }
     __syncthreads();
   }
