@@ -123,6 +123,8 @@ def batched_leg(args, H, datasets, O, dev, log):
                           "tuned": {k: idx.get_param(k) for k in ("ADC_PER", "ADC_WGS", "SPARSE_MAX_TESTS",
                                                                    "QUERY_CHUNK", "SCORE_BLOCK", "ADC_SPLIT")}}
     log(f"batched mixed+graph: {el * 1e3:.2f} ms/call ({nq / el:.0f} QPS)")
+    if args.batched_train > 0:
+        out["learned_vs_fixed"] = batched_learned(args, idx, w, q, ql, od, oi, H, datasets, O, dev, log)
     adc_ms, adc_bytes, gbs = best
     out["adc_roofline"] = {"kernel": "adc_scan_kernel (plain FP32 LUT)", "bound": "hbm", "achieved": round(gbs, 3),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 6),
@@ -131,6 +133,79 @@ def batched_leg(args, H, datasets, O, dev, log):
     del idx
     torch.cuda.empty_cache()
     return out
+
+
+def batched_learned(args, idx, w, q, ql, od, oi, H, datasets, O, dev, log):
+    """Learned budget on the uneven-selectivity workload, where the model's inputs (filter widths, grid-cell
+    counts, sub-center scores) vary from query to query.  Both paths run mixed precision with the autotuned launch
+    knobs and graph replay: the fixed budget at the smallest sub_lists (x sqrt 2 from 64) whose recall@100 on a
+    1,000-query evaluation sample reaches the target, and the learned budget trimming a 256 sub-cluster choice
+    (model trained on seeded training queries of the same distribution, labelled against the FP64 brute force)."""
+    import numpy as np
+    import torch
+    nq = w.queries.shape[0]
+    ne = min(1000, nq)
+    t0 = time.time()
+    er = O.recover_filters(w.qlabels[:ne], 2, **w.filter_arrays)
+    _, egt = O.bruteforce(w.data, w.labels, w.queries[:ne], er, 100)
+    tq, tql = w.sample_queries(args.batched_train, 0x7E58)
+    tr = O.recover_filters(tql, 2, **w.filter_arrays)
+    _, tgt = O.bruteforce(w.data, w.labels, tq, tr, 100)
+    log(f"batched learned: ground truth for {ne} evaluation + {args.batched_train} training queries in "
+        f"{time.time() - t0:.1f}s")
+    idx.set_param("PRECISION", 1)
+
+    def recall_at():
+        _, i = idx.query(q[:ne], ql[:ne])
+        torch.cuda.synchronize()
+        return datasets.recall(egt, i.cpu().numpy().astype(np.uint64), 100)
+
+    def graph_time():
+        idx.set_param("GRAPH", 1)
+        for _ in range(2):
+            idx.query(q, ql, od, oi)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            idx.query(q, ql, od, oi)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t1) / 3
+        idx.set_param("GRAPH", 0)
+        return el
+
+    b = 64.0
+    while True:
+        idx.set_param("SUB_LISTS", int(b))
+        r_fix = recall_at()
+        log(f"batched fixed budget sub_lists={int(b)}: recall@100={r_fix:.4f}")
+        if r_fix >= args.target_recall or b >= 1024:
+            break
+        b *= 2 ** 0.5
+    b_fix = int(b)
+    el_fix = graph_time()
+    st_fix = idx.last_stats(nq)
+    idx.set_param("SUB_LISTS", 256)
+    t0 = time.time()
+    rep = idx.budget_fit(torch.from_numpy(tq).to(dev), torch.from_numpy(tql).to(dev), tgt,
+                         levels=args.budget_levels, t0=0, s0=args.budget_s0, ratio=args.budget_ratio,
+                         target=args.target_recall, epochs=30)
+    log(f"batched learned: model trained in {time.time() - t0:.1f}s: {rep}")
+    idx.set_param("BUDGET_MODEL", 1)
+    r_l = recall_at()
+    el_l = graph_time()
+    st_l = idx.last_stats(nq)
+    idx.set_param("BUDGET_MODEL", 0)
+    idx.set_param("SUB_LISTS", 128)
+    log(f"batched learned: recall@100 {r_l:.4f}, {el_l * 1e3:.2f} ms/call ({nq / el_l:.0f} QPS) vs fixed "
+        f"sub_lists {b_fix}: recall {r_fix:.4f}, {el_fix * 1e3:.2f} ms/call ({nq / el_fix:.0f} QPS)")
+    return {"eval_sample": ne, "train_queries": args.batched_train,
+            "fixed": {"sub_lists": b_fix, "recall@100": round(r_fix, 4), "qps": round(nq / el_fix, 1),
+                      "ms_per_call": round(el_fix * 1e3, 3), "mean_refined": round(float(st_fix.refined.mean()), 1),
+                      "mean_candidates": round(float(st_fix.cand.mean()), 1)},
+            "learned": {"cap": 256, "recall@100": round(r_l, 4), "qps": round(nq / el_l, 1),
+                        "ms_per_call": round(el_l * 1e3, 3), "mean_refined": round(float(st_l.refined.mean()), 1),
+                        "mean_candidates": round(float(st_l.cand.mean()), 1)},
+            "learned_vs_fixed": round(el_fix / el_l, 3), "budget_model": rep}
 
 
 def main():
@@ -150,6 +225,8 @@ def main():
                     help="first write and then read 1 GiB (counter scale for rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--batched-n", type=int, default=5_000_000)
     ap.add_argument("--batched-nq", type=int, default=20_000)
+    ap.add_argument("--batched-train", type=int, default=4000,
+                    help="budget-model training queries of the uneven-selectivity leg (0: no learned leg there)")
     ap.add_argument("--save-budget", default=None, help="write the trained budget model here")
     ap.add_argument("--load-budget", default=None, help="use this budget model instead of training one")
     ap.add_argument("--budget-levels", type=int, default=20, help="budget ladder levels")

@@ -170,20 +170,25 @@ def explicit_ranges(n: int = 200_000, nq: int = 2000, d: int = 128, components: 
     means = rng.standard_normal((components, d), dtype=np.float32)
     data = gaussian_mixture(rng, n, d, means, 0.3)
     labels = rng.integers(0, 10_000_001, size=(n, 2)).astype(np.float32)
-    qrng = np.random.default_rng(seed + 1)
-    queries = gaussian_mixture(qrng, nq, d, means, 0.3)
-    sel = np.exp(qrng.uniform(np.log(1e-4), np.log(0.5), size=nq))
-    half = 1e7 * np.sqrt(sel) / 2
-    centers = qrng.uniform(0, 1e7, size=(nq, 2))
-    qlabels = np.zeros((nq, 4), np.float32)
-    for i in range(2):
-        qlabels[:, 2 * i] = np.maximum(centers[:, i] - half, 0).astype(np.float32)
-        qlabels[:, 2 * i + 1] = np.minimum(centers[:, i] + half, 1e7).astype(np.float32)
+
+    def sampler(m, s):
+        r = np.random.default_rng(s)
+        qs = gaussian_mixture(r, m, d, means, 0.3)
+        sel = np.exp(r.uniform(np.log(1e-4), np.log(0.5), size=m))
+        half = 1e7 * np.sqrt(sel) / 2
+        centers = r.uniform(0, 1e7, size=(m, 2))
+        ql = np.zeros((m, 4), np.float32)
+        for i in range(2):
+            ql[:, 2 * i] = np.maximum(centers[:, i] - half, 0).astype(np.float32)
+            ql[:, 2 * i + 1] = np.minimum(centers[:, i] + half, 1e7).astype(np.float32)
+        return qs, ql
+
+    queries, qlabels = sampler(nq, seed + 1)
     conf = dict(BASE_CONF, PQ_DIM=max(1, d // 4), N_LIST=64, SUB_LISTS=128, FILTER_DIM=2, CLUSTERS=clusters,
                 SUB_CLUSTERS=sub_clusters)
     filt = "l = 2\nfilter0:\n\ttype = 3\n\nfilter1:\n\ttype = 3\n"
     fa = dict(ftype=[3, 3], shift_val=[0.0] * 4, imap=[], div_value=[0, 0])
-    return Workload("explicit-ranges", conf, filt, data, labels, queries, qlabels, fa)
+    return Workload("explicit-ranges", conf, filt, data, labels, queries, qlabels, fa, sampler=sampler)
 
 
 def full_pipeline(n: int = 50_000_000, nq: int = 100_000, d: int = 96, components: int = 4096,
