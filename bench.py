@@ -13,7 +13,10 @@ brute force (on a 2,000-query sample):
   mixed   -- PRECISION 1: BF16 MFMA cluster and sub-cluster scoring, FP16 LUT, FP32 re-rank, fixed budget;
   learned -- PRECISION 1 + BUDGET_MODEL 1: per-query sub-cluster budget from the MLP trained on traces of
              this pipeline over --train seeded training queries (labels = minimal budget level per query).
-`value` is the learned+mixed QPS; the plain and mixed QPS are reported in `config`.
+`value` is the learned+mixed QPS; the plain and mixed QPS are reported in `config`.  A second workload (5M x 128,
+two labels with per-query selectivity from 1e-4 to 0.5, 20k queries per call) reports plain / mixed / autotuned
+graph-replayed QPS, the ADC scan's HBM roofline, and the learned budget against the smallest fixed budget that
+reaches the target (`batched.learned_vs_fixed`).
 
 The CPU baseline is the oracle's FP64 filtered brute force (a C++ restatement of the reference's CPU search,
 ground_truth.cuh:166-233, with filter.conf predicates) on a bounded query sample, all host threads.
@@ -138,7 +141,7 @@ def batched_leg(args, H, datasets, O, dev, log):
 def batched_learned(args, idx, w, q, ql, od, oi, H, datasets, O, dev, log):
     """Learned budget on the uneven-selectivity workload, where the model's inputs (filter widths, grid-cell
     counts, sub-center scores) vary from query to query.  Both paths run mixed precision with the autotuned launch
-    knobs and graph replay: the fixed budget at the smallest sub_lists (x sqrt 2 from 64) whose recall@100 on a
+    knobs and graph replay: the fixed budget at the smallest sub_lists (x 2^(1/4) from 64) whose recall@100 on a
     1,000-query evaluation sample reaches the target, and the learned budget trimming a 256 sub-cluster choice
     (model trained on seeded training queries of the same distribution, labelled against the FP64 brute force)."""
     import numpy as np
@@ -180,7 +183,7 @@ def batched_learned(args, idx, w, q, ql, od, oi, H, datasets, O, dev, log):
         log(f"batched fixed budget sub_lists={int(b)}: recall@100={r_fix:.4f}")
         if r_fix >= args.target_recall or b >= 1024:
             break
-        b *= 2 ** 0.5
+        b *= 2 ** 0.25
     b_fix = int(b)
     el_fix = graph_time()
     st_fix = idx.last_stats(nq)
