@@ -287,19 +287,82 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
     if (b != 0 && b < k) k = b;
   }
   uint32_t m = n < k ? n : k;
+  // keys selected: the m written, and at least the 32 smallest when the features are written
+  uint32_t msel = m;
+  if (feat_dist && msel < (n < 32u ? n : 32u)) msel = n < 32u ? n : 32u;
   if (n <= scap) {
     for (uint32_t j = threadIdx.x; j < n; j += kWG)
       keys[j] = ((uint64_t)ord(by_pos ? row[j] : item_score(row, lst[j], nzb_q, nzp_q)) << 32) | j;
     __syncthreads();
     dev::wg_sort_lds(keys, n);
   } else {
+    // the keys and their range go to global scratch; one histogram pass over 2048 bins of the range and one
+    // split pass (keys in bins below the msel-th key's bin to LDS, keys in its bin compacted in place at the
+    // front of the scratch) leave a radix selection inside that one bin.  The msel smallest keys are unique,
+    // so the result equals a full sort's first msel.
+    __shared__ uint32_t mm[2][kWG / 64];
     uint64_t* g = scratch + qi * cap;
-    for (uint32_t j = threadIdx.x; j < n; j += kWG)
-      g[j] = ((uint64_t)ord(by_pos ? row[j] : item_score(row, lst[j], nzb_q, nzp_q)) << 32) | j;
+    uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+    for (uint32_t j = threadIdx.x; j < n; j += kWG) {
+      const uint32_t o = ord(by_pos ? row[j] : item_score(row, lst[j], nzb_q, nzp_q));
+      g[j] = ((uint64_t)o << 32) | j;
+      mn = o < mn ? o : mn;
+      mx = o > mx ? o : mx;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint32_t a = (uint32_t)__shfl_xor((int)mn, o, 64), b = (uint32_t)__shfl_xor((int)mx, o, 64);
+      mn = a < mn ? a : mn;
+      mx = b > mx ? b : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+      mm[0][threadIdx.x >> 6] = mn;
+      mm[1][threadIdx.x >> 6] = mx;
+    }
+    for (int i = threadIdx.x; i < 2048; i += kWG) hist[i] = 0;
     __syncthreads();
-    if (m > scap) m = scap;
-    dev::wg_radix_select(dev::KeyArray{g}, n, m, keys, hist, sh);
-    dev::wg_sort_lds(keys, m);
+    mn = mm[0][0];
+    mx = mm[1][0];
+#pragma unroll
+    for (int w = 1; w < kWG / 64; ++w) {
+      mn = mm[0][w] < mn ? mm[0][w] : mn;
+      mx = mm[1][w] > mx ? mm[1][w] : mx;
+    }
+    if (msel > scap) msel = scap;
+    if (m > msel) m = msel;
+    const uint32_t sft = dev::dist_bin_shift(mn, mx);
+    for (uint32_t j = threadIdx.x; j < n; j += kWG) atomicAdd(&hist[((uint32_t)(g[j] >> 32) - mn) >> sft], 1u);
+    __syncthreads();
+    dev::wg_hist_find(hist, msel, sh);  // sh[0] = the msel-th key's bin, sh[1] = keys below it
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(&sh[2]);
+    if (threadIdx.x == 0) {
+      cnt[0] = 0;
+      cnt[1] = 0;
+    }
+    __syncthreads();
+    const uint32_t bstar = (uint32_t)sh[0], below = (uint32_t)sh[1];
+    for (uint32_t r0 = 0; r0 < n; r0 += kWG) {
+      const uint32_t j = r0 + threadIdx.x;
+      const uint64_t key = j < n ? g[j] : 0ull;
+      // every key of this round is read before any is written back; writes land below r0 + kWG, reads of the
+      // next rounds at or above it
+      __syncthreads();
+      if (j < n) {
+        const uint32_t b = ((uint32_t)(key >> 32) - mn) >> sft;
+        if (b < bstar) keys[atomicAdd(&cnt[0], 1u)] = key;
+        else if (b == bstar) g[atomicAdd(&cnt[1], 1u)] = key;
+      }
+    }
+    __syncthreads();
+    const uint32_t mb = cnt[1], need = msel - below;
+    __syncthreads();  // sh is reused by the radix selection
+    if (need == mb) {
+      for (uint32_t i = threadIdx.x; i < mb; i += kWG) keys[below + i] = g[i];
+    } else {
+      dev::wg_radix_select(dev::KeyArray{g}, mb, need, keys + below, hist, sh);
+    }
+    __syncthreads();
+    dev::wg_sort_lds(keys, msel);
   }
   for (uint32_t j = threadIdx.x; j < m; j += kWG) chosen[qi * out_ld + j] = (uint32_t)keys[j];
   if (threadIdx.x == 0) n_chosen[qi] = m;
