@@ -195,6 +195,9 @@ def batched_learned(args, idx, w, q, ql, od, oi, H, datasets, O, dev, log):
     log(f"batched learned: model trained in {time.time() - t0:.1f}s: {rep}")
     idx.set_param("BUDGET_MODEL", 1)
     r_l = recall_at()
+    # the learned path's own launch knobs (results unchanged by them), as the fixed path had its own
+    tune = idx.autotune(q, ql, reps=2)
+    log("batched learned autotune: " + tune.strip().splitlines()[-1])
     el_l = graph_time()
     st_l = idx.last_stats(nq)
     idx.set_param("BUDGET_MODEL", 0)
@@ -206,6 +209,8 @@ def batched_learned(args, idx, w, q, ql, od, oi, H, datasets, O, dev, log):
                       "ms_per_call": round(el_fix * 1e3, 3), "mean_refined": round(float(st_fix.refined.mean()), 1),
                       "mean_candidates": round(float(st_fix.cand.mean()), 1)},
             "learned": {"cap": 256, "recall@100": round(r_l, 4), "qps": round(nq / el_l, 1),
+                        "tuned": {k: idx.get_param(k) for k in ("ADC_PER", "ADC_WGS", "SPARSE_MAX_TESTS",
+                                                                 "QUERY_CHUNK", "SCORE_BLOCK", "ADC_SPLIT")},
                         "ms_per_call": round(el_l * 1e3, 3), "mean_refined": round(float(st_l.refined.mean()), 1),
                         "mean_candidates": round(float(st_l.cand.mean()), 1)},
             "learned_vs_fixed": round(el_fix / el_l, 3), "budget_model": rep}
