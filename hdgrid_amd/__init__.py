@@ -57,6 +57,7 @@ _set_param = _sig("hdg_set_param", ctypes.c_int, [_vp, _cp, _cp])
 _get_param = _sig("hdg_get_param", ctypes.c_int, [_vp, _cp, _u64p])
 _save = _sig("hdg_save", ctypes.c_int, [_vp, _cp])
 _load = _sig("hdg_load", ctypes.c_int, [_vp, _cp, _vp])
+_validate_ckpt = _sig("hdg_validate_checkpoint", ctypes.c_int, [_cp, ctypes.c_int])
 _stats = _sig("hdg_last_query_stats", ctypes.c_int, [_vp, _u64, _u64p, _u64p, _u64p, _u64p, _u64p, _u64p])
 _chosen = _sig("hdg_last_chosen_subs", ctypes.c_int, [_vp, _u64, ctypes.POINTER(ctypes.c_uint32)])
 _stage_ms = _sig("hdg_last_stage_ms", ctypes.c_int, [_vp, _cp, _u64, ctypes.POINTER(ctypes.c_float), _u64p])
@@ -111,6 +112,12 @@ def _check(st: int):
 
 def library_path() -> str:
     return LIB_PATH
+
+
+def validate_checkpoint(directory: str, label_dims: int) -> None:
+    """Checks a saved index state (hdg_save layout) without a device; raises HDGridError (HDG_EIO) when it is
+    inconsistent."""
+    _check(_validate_ckpt(directory.encode(), int(label_dims)))
 
 
 def find_max_factor(a: int, b: int) -> int:
@@ -296,8 +303,9 @@ class SuperGridIndex:
         return out
 
     def autotune(self, queries, query_labels, reps: int = 3, stream=None) -> str:
-        """Tune ADC_PER, ADC_WGS, SPARSE_MAX_TESTS, QUERY_CHUNK, SCORE_BLOCK, ADC_SPLIT and GRAPH on this batch
-        (results unchanged bit for bit); the chosen values stay set.  Returns the trial log."""
+        """Tune ADC_PER, ADC_WGS, QUERY_CHUNK, SCORE_BLOCK, ADC_SPLIT and GRAPH (and SPARSE_MAX_TESTS under
+        PRECISION 0) on this batch (results unchanged bit for bit); the chosen values stay set.  Returns the
+        trial log."""
         import torch
         queries = queries.contiguous()
         query_labels = query_labels.contiguous()

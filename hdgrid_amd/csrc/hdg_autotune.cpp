@@ -1,6 +1,9 @@
 // Launch-parameter autotuner: coordinate descent over the query knobs that do not change results.
 //
-// Each knob is tried over a fixed ladder of values with the others held at the best setting so far; a
+// The knobs: ADC_PER, ADC_WGS, QUERY_CHUNK, SCORE_BLOCK, ADC_SPLIT and GRAPH never change a result;
+// SPARSE_MAX_TESTS does not under PRECISION 0, but under PRECISION 1 it decides which queries refine
+// straight from their in-box lists (instead of the FP16-LUT Morton route), so it is tuned only under
+// PRECISION 0.  Each knob is tried over a fixed ladder of values with the others held at the best setting so far; a
 // candidate is timed as the median of `reps` full queries (HIP events on the caller's stream, one
 // untimed query first, which also captures the graphs of GRAPH 1) and accepted only when it is faster
 // AND its ids and distances equal those of the starting configuration bit for bit.  The final setting
@@ -85,6 +88,7 @@ std::string Index::autotune(const float* queries, uint64_t nq, uint64_t d, const
         {"GRAPH", {0, 1}},
     };
     for (uint64_t c = 2048; c <= 65536 && c <= chunk_cap * 2; c *= 2) knobs[3].values.push_back(c);
+    if (cfg_.precision != 0) knobs[2].values.clear();  // result-changing under PRECISION 1
     for (const Knob& kb : knobs) {
       const uint64_t keep = get(kb.key);
       uint64_t best_v = keep;

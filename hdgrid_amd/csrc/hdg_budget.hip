@@ -542,6 +542,7 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
   upload(B.d_inv, B.inv_std, s);
   upload(B.d_ladT, B.ladT, s);
   upload(B.d_ladS, B.ladS, s);
+  drop_graphs();  // captured graphs hold the previous model's margin and level count
   B.ready = true;
 }
 
@@ -568,6 +569,7 @@ void Index::budget_save(const std::string& path) const {
 }
 
 void Index::budget_load(const std::string& path, hipStream_t s) {
+  HDG_HIP(hipSetDevice(device_));
   std::ifstream f(path, std::ios::binary);
   if (!f) fail(kIoError, "cannot read " + path);
   uint32_t magic = 0, levels = 0, params = 0;
@@ -576,35 +578,44 @@ void Index::budget_load(const std::string& path, hipStream_t s) {
   f.read((char*)&levels, 4);
   f.read((char*)&params, 4);
   f.read((char*)&margin, 4);
-  if (magic != 0x48444742u || params != (uint32_t)kParams || levels < 2 || levels > 32)
+  if (!f || magic != 0x48444742u || params != (uint32_t)kParams || levels < 2 || levels > 32)
     fail(kIoError, "not a budget model file: " + path);
-  if (!budget_) budget_ = new BudgetModel();
-  BudgetModel& B = *budget_;
-  B.ladT.resize(levels);
-  B.ladS.resize(levels);
-  B.mean.resize(kF);
-  B.inv_std.resize(kF);
-  B.master.resize(kParams);
-  f.read((char*)B.ladT.data(), levels * 4);
-  f.read((char*)B.ladS.data(), levels * 4);
-  f.read((char*)B.mean.data(), kF * 4);
-  f.read((char*)B.inv_std.data(), kF * 4);
-  f.read((char*)B.master.data(), kParams * 4);
+  // the whole file is read and checked before anything of the current model changes
+  std::vector<uint32_t> ladT(levels), ladS(levels);
+  std::vector<float> mean(kF), inv_std(kF), master(kParams);
+  f.read((char*)ladT.data(), levels * 4);
+  f.read((char*)ladS.data(), levels * 4);
+  f.read((char*)mean.data(), kF * 4);
+  f.read((char*)inv_std.data(), kF * 4);
+  f.read((char*)master.data(), kParams * 4);
   if (!f) fail(kIoError, "short budget model file: " + path);
-  B.margin = margin;
+  if (!std::isfinite(margin)) fail(kIoError, "budget model margin is not finite: " + path);
+  for (uint32_t i = 0; i < levels; ++i)
+    if (ladS[i] == 0 || ladT[i] == 0) fail(kIoError, "budget model ladder holds a zero budget: " + path);
   std::vector<float> wq(kParams);
   for (int i = 0; i < kParams; ++i) {
     uint32_t u;
-    std::memcpy(&u, &B.master[i], 4);
+    std::memcpy(&u, &master[i], 4);
     u += 0x7FFFu + ((u >> 16) & 1u);
     u &= 0xFFFF0000u;
     std::memcpy(&wq[i], &u, 4);
   }
+  // captured graphs hold the old model's margin and level count as kernel arguments
+  drop_graphs();
+  if (!budget_) budget_ = new BudgetModel();
+  BudgetModel& B = *budget_;
+  B.ready = false;
   upload(B.d_w, wq, s);
-  upload(B.d_mean, B.mean, s);
-  upload(B.d_inv, B.inv_std, s);
-  upload(B.d_ladT, B.ladT, s);
-  upload(B.d_ladS, B.ladS, s);
+  upload(B.d_mean, mean, s);
+  upload(B.d_inv, inv_std, s);
+  upload(B.d_ladT, ladT, s);
+  upload(B.d_ladS, ladS, s);
+  B.ladT = std::move(ladT);
+  B.ladS = std::move(ladS);
+  B.mean = std::move(mean);
+  B.inv_std = std::move(inv_std);
+  B.master = std::move(master);
+  B.margin = margin;
   B.ready = true;
 }
 

@@ -48,6 +48,12 @@ void check_algorithm(const char* algorithm) {
   if (a != "super-grid") hdg::fail(HDG_ECONFIG, "Unknown algorithm: " + std::string(algorithm));
 }
 
+// IS_SPLIT = 1 selects the reference's Morton split mode (zorder.cu:474-654), which this library does
+// not implement; refusing it at creation keeps a drop-in HDGrid.conf from silently running the no-split path.
+void check_supported(const hdg::Config& cfg) {
+  if (cfg.is_split) hdg::fail(HDG_EUNSUPPORTED, "IS_SPLIT = 1 (Morton split mode) is not supported");
+}
+
 void check_device(int device) {
   int n = 0;
   HDG_HIP(hipGetDeviceCount(&n));
@@ -126,9 +132,10 @@ hdg_status hdg_create(const char* algorithm, const char* conf_path, const char* 
     if (!out || !conf_path) hdg::fail(HDG_EINVAL, "null argument");
     *out = nullptr;
     check_algorithm(algorithm);
-    check_device(device);
     hdg::Config cfg;
     hdg::parse_config_file(conf_path, cfg);
+    check_supported(cfg);
+    check_device(device);
     std::string fpath = filter_conf_path && *filter_conf_path ? std::string(filter_conf_path)
                                                                : cfg.path + "filter.conf";
     hdg::FilterConfig fc;
@@ -143,9 +150,10 @@ hdg_status hdg_create_from_text(const char* algorithm, const char* conf_text, co
     if (!out || !conf_text || !filter_text) hdg::fail(HDG_EINVAL, "null argument");
     *out = nullptr;
     check_algorithm(algorithm);
-    check_device(device);
     hdg::Config cfg;
     hdg::parse_config_text(conf_text, cfg);
+    check_supported(cfg);
+    check_device(device);
     hdg::FilterConfig fc;
     hdg::parse_filter_text(filter_text, fc);
     *out = new hdg_index{new hdg::Index(cfg, fc, device)};
@@ -225,8 +233,11 @@ hdg_status hdg_destroy(hdg_index* index) {
 hdg_status hdg_set_param(hdg_index* index, const char* key, const char* value) {
   return guarded([&] {
     if (!index || !key || !value) hdg::fail(HDG_EINVAL, "null argument");
-    if (!hdg::set_config_value(index->impl->config(), key, value))
+    hdg::Config trial = index->impl->config();
+    if (!hdg::set_config_value(trial, key, value))
       hdg::fail(HDG_ECONFIG, std::string("unknown HDGrid.conf key: ") + key);
+    check_supported(trial);
+    index->impl->config() = trial;
   });
 }
 
@@ -249,6 +260,13 @@ hdg_status hdg_load(hdg_index* index, const char* dir, void* stream) {
   return guarded([&] {
     if (!index || !dir) hdg::fail(HDG_EINVAL, "null argument");
     index->impl->load(dir, (hipStream_t)stream);
+  });
+}
+
+hdg_status hdg_validate_checkpoint(const char* dir, int label_dims) {
+  return guarded([&] {
+    if (!dir) hdg::fail(HDG_EINVAL, "null argument");
+    (void)hdg::read_checkpoint(dir, label_dims);
   });
 }
 

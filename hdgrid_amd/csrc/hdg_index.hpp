@@ -72,6 +72,18 @@ struct IndexState {
   bool built = false;
 };
 
+// A saved index state read into host memory and checked (hdg_io.cpp): the host part of IndexState plus the
+// arrays that go to the device.
+struct HostCheckpoint {
+  IndexState S;
+  std::vector<float> centers, subc, pqc, vec, lab;
+  std::vector<uint8_t> codes;
+  std::vector<uint32_t> cells;
+};
+// Reads and validates the checkpoint in `dir` for an index with l_expected label dims; throws kIoError on
+// any inconsistency (shapes, offsets, permutations, value ranges).  Touches no device.
+HostCheckpoint read_checkpoint(const std::string& dir, int l_expected);
+
 // Bucket grid from the z-pool-ordered cells ([n_valid][N]) and sub-cluster offsets; sets the bucket fields.
 void build_bucket_grid(IndexState& st, const std::vector<uint32_t>& h_cells, hipStream_t s);
 
@@ -130,8 +142,8 @@ class Index {
   void budget_load(const std::string& path, hipStream_t s);
   void budget_free();
   void debug_coarse_scores(const float* queries, uint64_t nq, float* out_host, hipStream_t s);
-  // Coordinate descent over the launch knobs (ADC_PER, ADC_WGS, SPARSE_MAX_TESTS, QUERY_CHUNK,
-  // SCORE_BLOCK, GRAPH) timed on these queries (hdg_autotune.cpp); keeps the fastest setting whose
+  // Coordinate descent over the launch knobs (ADC_PER, ADC_WGS, QUERY_CHUNK, SCORE_BLOCK, ADC_SPLIT, GRAPH;
+  // SPARSE_MAX_TESTS under PRECISION 0 only) timed on these queries (hdg_autotune.cpp); keeps the fastest setting whose
   // results equal the starting one's bit for bit and returns the trial log.
   std::string autotune(const float* queries, uint64_t nq, uint64_t d, const float* qlabels, uint64_t qcols,
                        float* out_dist, uint64_t* out_ids, uint32_t reps, hipStream_t s);

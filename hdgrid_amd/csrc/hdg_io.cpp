@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -53,24 +54,29 @@ void write_npy(const std::string& path, const T* data, const std::vector<uint64_
 
 template <class T>
 std::vector<T> read_npy(const std::string& path, std::vector<uint64_t>& shape) {
-  std::ifstream f(path, std::ios::binary);
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
   if (!f) fail(kIoError, "cannot read " + path);
+  const uint64_t fsize = (uint64_t)f.tellg();
+  f.seekg(0);
   char magic[8];
   f.read(magic, 8);
-  if (std::memcmp(magic, "\x93NUMPY", 6) != 0) fail(kIoError, "not an npy file: " + path);
+  if (!f || std::memcmp(magic, "\x93NUMPY", 6) != 0) fail(kIoError, "not an npy file: " + path);
   uint32_t hl = 0;
   if (magic[6] == 1) {
-    uint16_t h16;
+    uint16_t h16 = 0;
     f.read(reinterpret_cast<char*>(&h16), 2);
     hl = h16;
   } else {
     f.read(reinterpret_cast<char*>(&hl), 4);
   }
+  if (!f || hl > fsize) fail(kIoError, "bad npy header in " + path);
   std::string hdr(hl, ' ');
   f.read(&hdr[0], hl);
+  if (!f) fail(kIoError, "short npy header in " + path);
   if (hdr.find(npy_descr<T>()) == std::string::npos) fail(kIoError, "unexpected dtype in " + path);
   if (hdr.find("'fortran_order': False") == std::string::npos) fail(kIoError, "fortran order in " + path);
-  size_t a = hdr.find('('), b = hdr.find(')');
+  const size_t a = hdr.find('('), b = hdr.find(')');
+  if (a == std::string::npos || b == std::string::npos || b < a) fail(kIoError, "bad npy shape in " + path);
   shape.clear();
   std::string dims = hdr.substr(a + 1, b - a - 1);
   std::stringstream ss(dims);
@@ -79,13 +85,24 @@ std::vector<T> read_npy(const std::string& path, std::vector<uint64_t>& shape) {
   while (std::getline(ss, tok, ',')) {
     size_t p = tok.find_first_not_of(' ');
     if (p == std::string::npos) continue;
+    if (tok.find_first_not_of("0123456789 ", p) != std::string::npos) fail(kIoError, "bad npy shape in " + path);
     shape.push_back(std::stoull(tok.substr(p)));
+    if (shape.back() && n > ~0ull / shape.back()) fail(kIoError, "npy shape overflows in " + path);
     n *= shape.back();
   }
+  // the data must be exactly what the shape says: a truncated or padded file is refused before allocating
+  const uint64_t data_off = (uint64_t)f.tellg();
+  if (n > ~0ull / sizeof(T) || fsize - data_off != n * sizeof(T))
+    fail(kIoError, "npy data size does not match its shape in " + path);
   std::vector<T> v(n);
   f.read(reinterpret_cast<char*>(v.data()), n * sizeof(T));
   if (!f) fail(kIoError, "short read " + path);
   return v;
+}
+
+// Throws kIoError unless an array read from `name` has exactly the expected shape.
+void expect_shape(const std::vector<uint64_t>& got, const std::vector<uint64_t>& want, const std::string& name) {
+  if (got != want) fail(kIoError, "array " + name + " of the saved index has the wrong shape");
 }
 
 template <class T>
@@ -110,19 +127,33 @@ std::string json_array(const int* v, int n) {
 std::vector<uint64_t> json_get(const std::string& js, const std::string& key) {
   size_t p = js.find("\"" + key + "\"");
   if (p == std::string::npos) fail(kIoError, "meta.json lacks " + key);
-  p = js.find(':', p) + 1;
-  while (js[p] == ' ') ++p;
+  p = js.find(':', p);
+  if (p == std::string::npos) fail(kIoError, "meta.json: no value for " + key);
+  ++p;
+  while (p < js.size() && js[p] == ' ') ++p;
   std::vector<uint64_t> out;
-  if (js[p] == '[') {
+  auto num = [&](const std::string& t) -> uint64_t {
+    const size_t q = t.find_first_not_of(" \n\r\t");
+    if (q == std::string::npos || !std::isdigit((unsigned char)t[q])) fail(kIoError, "meta.json: bad value of " + key);
+    return std::stoull(t.substr(q));
+  };
+  if (p < js.size() && js[p] == '[') {
     size_t e = js.find(']', p);
+    if (e == std::string::npos) fail(kIoError, "meta.json: unterminated array " + key);
     std::stringstream ss(js.substr(p + 1, e - p - 1));
     std::string tok;
     while (std::getline(ss, tok, ','))
-      if (tok.find_first_not_of(' ') != std::string::npos) out.push_back(std::stoull(tok));
+      if (tok.find_first_not_of(' ') != std::string::npos) out.push_back(num(tok));
   } else {
-    out.push_back(std::stoull(js.substr(p)));
+    out.push_back(num(js.substr(p, 32)));
   }
   return out;
+}
+
+uint64_t json_scalar(const std::string& js, const std::string& key) {
+  auto v = json_get(js, key);
+  if (v.size() != 1) fail(kIoError, "meta.json: " + key + " is not a scalar");
+  return v[0];
 }
 
 }  // namespace
@@ -159,51 +190,120 @@ void Index::save(const std::string& dir) const {
   write_npy(dir + "/lab.npy", download<float>(S.lab, S.N * (uint64_t)S.l).data(), {S.N, (uint64_t)S.l});
 }
 
-void Index::load(const std::string& dir, hipStream_t s) {
-  HDG_HIP(hipSetDevice(device_));
-  drop_graphs();
+// Reads a saved index state into host memory.  Every array's shape is checked against meta.json and every
+// value that is used as an index or offset is range-checked, so a truncated or inconsistent checkpoint fails
+// with kIoError here instead of reaching the query kernels.  Touches no device.
+HostCheckpoint read_checkpoint(const std::string& dir, int l_expected) {
+  HostCheckpoint H;
+  IndexState& S = H.S;
   std::ifstream mf(dir + "/meta.json");
   if (!mf) fail(kIoError, "cannot read " + dir + "/meta.json");
   std::stringstream ss;
   ss << mf.rdbuf();
   const std::string js = ss.str();
-  IndexState S;
-  S.N = json_get(js, "N")[0];
-  S.d = json_get(js, "d")[0];
-  S.C = json_get(js, "C")[0];
-  S.CS = json_get(js, "CS")[0];
-  S.M = json_get(js, "M")[0];
-  S.K = json_get(js, "K")[0];
-  S.pq_len = json_get(js, "pq_len")[0];
-  S.l = (int)json_get(js, "l")[0];
-  if (S.l != fc_.l) fail(kBadState, "saved index has a different number of label dims than filter.conf");
+  S.N = json_scalar(js, "N");
+  S.d = json_scalar(js, "d");
+  S.C = json_scalar(js, "C");
+  S.CS = json_scalar(js, "CS");
+  S.M = json_scalar(js, "M");
+  S.K = json_scalar(js, "K");
+  S.pq_len = json_scalar(js, "pq_len");
+  const uint64_t l64 = json_scalar(js, "l");
+  auto bad = [&](const std::string& what) { fail(kIoError, "saved index in " + dir + ": " + what); };
+  if (l64 < 1 || l64 > (uint64_t)kMaxDims) bad("label dims outside [1, 8]");
+  S.l = (int)l64;
+  if (S.l != l_expected) fail(kBadState, "saved index has a different number of label dims than filter.conf");
+  if (S.N == 0 || S.N >= (1ull << 32)) bad("row count outside [1, 2^32)");
+  if (S.d == 0 || S.C == 0 || S.CS < S.C || S.CS >= (1ull << 32) || S.M == 0 || S.M > S.d) bad("bad sizes");
+  if (S.K == 0 || S.K > 256) bad("PQ codewords outside [1, 256]");
+  if (S.pq_len != (S.d + S.M - 1) / S.M) bad("pq_len differs from ceil(d / M)");
   GridShape& g = S.grid;
   g.l = S.l;
   auto dims = json_get(js, "dims"), span = json_get(js, "span"), left = json_get(js, "left"),
        shuf = json_get(js, "shuffle");
+  if (dims.size() != l64 || span.size() != l64 || left.size() != l64 || shuf.size() != l64)
+    bad("grid arrays of meta.json do not have l entries");
+  uint32_t seen = 0;
   for (int i = 0; i < g.l; ++i) {
+    if (dims[i] == 0 || dims[i] > 0x7fffffffull || shuf[i] >= l64 || ((seen >> shuf[i]) & 1))
+      bad("bad grid shape");
+    seen |= 1u << shuf[i];
     g.dims[i] = (int)dims[i];
     g.span[i] = span[i];
     g.left[i] = left[i];
     g.shuffle[i] = (int)shuf[i];
   }
-  g.max_bits = (int)json_get(js, "max_bits")[0];
-  g.valid_map = (int)json_get(js, "valid_map")[0];
-  g.tot_grids = json_get(js, "tot_grids")[0];
+  const uint64_t max_bits = json_scalar(js, "max_bits"), valid_map = json_scalar(js, "valid_map");
+  if (max_bits < 1 || max_bits > 32 || valid_map == 0 || valid_map >= (1ull << g.l)) bad("bad Morton configuration");
+  g.max_bits = (int)max_bits;
+  g.valid_map = (int)valid_map;
+  g.tot_grids = json_scalar(js, "tot_grids");
   g.n_valid = 0;
   for (int i = 0; i < g.l; ++i)
     if ((g.valid_map >> i) & 1) g.valid_dims[g.n_valid++] = i;
+  for (int j = 0; j < g.n_valid; ++j) {
+    if (g.span[g.valid_dims[j]] == 0) bad("a valid dim has a zero label span");
+    if (((uint64_t)g.dims[g.valid_dims[j]] - 1) >> max_bits) bad("a valid dim has more cells than its Morton bits hold");
+  }
+  if ((uint64_t)g.n_valid * max_bits > 64) bad("Morton codes wider than 64 bits");
   std::vector<uint64_t> sh;
   S.h_zid = read_npy<uint32_t>(dir + "/zid.npy", sh);
+  expect_shape(sh, {S.N}, "zid");
   S.h_zcode = read_npy<uint64_t>(dir + "/zcode.npy", sh);
+  expect_shape(sh, {S.N}, "zcode");
   S.h_seg_off = read_npy<uint64_t>(dir + "/seg_off.npy", sh);
+  expect_shape(sh, {S.CS + 1}, "seg_off");
   S.h_sub_off = read_npy<uint32_t>(dir + "/sub_off.npy", sh);
-  if (S.h_seg_off.size() != S.CS + 1 || S.h_sub_off.size() != S.C + 1 || S.h_zid.size() != S.N)
-    fail(kIoError, "inconsistent index arrays in " + dir);
-  for (uint64_t c = 0; c < S.C; ++c)
+  expect_shape(sh, {S.C + 1}, "sub_off");
+  if (S.h_seg_off[0] != 0 || S.h_seg_off[S.CS] != S.N) bad("seg_off does not span [0, N]");
+  if (S.h_sub_off[0] != 0 || S.h_sub_off[S.C] != S.CS) bad("sub_off does not span [0, CS]");
+  for (uint64_t c = 0; c < S.C; ++c) {
+    if (S.h_sub_off[c + 1] < S.h_sub_off[c]) bad("sub_off is not monotone");
     S.max_subs_per_cluster = std::max<int>(S.max_subs_per_cluster, (int)(S.h_sub_off[c + 1] - S.h_sub_off[c]));
-  for (uint64_t i = 0; i < S.CS; ++i)
+  }
+  for (uint64_t i = 0; i < S.CS; ++i) {
+    if (S.h_seg_off[i + 1] < S.h_seg_off[i]) bad("seg_off is not monotone");
     S.max_seg_len = std::max<uint32_t>(S.max_seg_len, (uint32_t)(S.h_seg_off[i + 1] - S.h_seg_off[i]));
+    for (uint64_t p = S.h_seg_off[i] + 1; p < S.h_seg_off[i + 1]; ++p)
+      if (S.h_zcode[p] < S.h_zcode[p - 1]) bad("Morton codes are not ascending inside a sub-cluster");
+  }
+  {
+    std::vector<uint8_t> hit(S.N, 0);
+    for (uint32_t z : S.h_zid) {
+      if (z >= S.N || hit[z]) bad("zid is not a permutation of the rows");
+      hit[z] = 1;
+    }
+  }
+  H.centers = read_npy<float>(dir + "/centers.npy", sh);
+  expect_shape(sh, {S.C, S.d}, "centers");
+  H.subc = read_npy<float>(dir + "/sub_centers.npy", sh);
+  expect_shape(sh, {S.CS, S.d}, "sub_centers");
+  H.pqc = read_npy<float>(dir + "/pq_centers.npy", sh);
+  expect_shape(sh, {S.M, S.K, S.pq_len}, "pq_centers");
+  H.codes = read_npy<uint8_t>(dir + "/codes.npy", sh);
+  expect_shape(sh, {S.N, S.M}, "codes");
+  for (uint8_t c : H.codes)
+    if (c >= S.K) bad("a PQ code is not below the codeword count");
+  H.cells = read_npy<uint32_t>(dir + "/cells.npy", sh);
+  expect_shape(sh, {(uint64_t)g.n_valid, S.N}, "cells");
+  for (int j = 0; j < g.n_valid; ++j) {
+    const uint32_t lim = (uint32_t)g.dims[g.valid_dims[j]];
+    for (uint64_t p = 0; p < S.N; ++p)
+      if (H.cells[(uint64_t)j * S.N + p] >= lim) bad("a grid cell lies outside its dim");
+  }
+  H.vec = read_npy<float>(dir + "/vec.npy", sh);
+  expect_shape(sh, {S.N, S.d}, "vec");
+  H.lab = read_npy<float>(dir + "/lab.npy", sh);
+  expect_shape(sh, {S.N, (uint64_t)S.l}, "lab");
+  return H;
+}
+
+void Index::load(const std::string& dir, hipStream_t s) {
+  HDG_HIP(hipSetDevice(device_));
+  HostCheckpoint H = read_checkpoint(dir, fc_.l);
+  IndexState& S = H.S;
+  // everything is consistent: only now is the current index replaced
+  drop_graphs();
   upload(S.zid, S.h_zid, s);
   upload(S.zcode, S.h_zcode, s);
   upload(S.sub_off, S.h_sub_off, s);
@@ -215,27 +315,16 @@ void Index::load(const std::string& dir, hipStream_t s) {
       for (uint32_t p = S.h_sub_off[c]; p < S.h_sub_off[c + 1]; ++p) cos[p] = (uint32_t)c;
     upload(S.cluster_of_sub, cos, s);
   }
-  {
-    std::vector<float> centers = read_npy<float>(dir + "/centers.npy", sh);
-    std::vector<float> subc = read_npy<float>(dir + "/sub_centers.npy", sh);
-    upload(S.centers, centers, s);
-    upload(S.sub_centers, subc, s);
-    prepare_mixed_centers(S, centers, subc, s);
-  }
-  upload(S.pq_centers, read_npy<float>(dir + "/pq_centers.npy", sh), s);
-  upload(S.codes, read_npy<uint8_t>(dir + "/codes.npy", sh), s);
-  {
-    std::vector<uint32_t> cells = read_npy<uint32_t>(dir + "/cells.npy", sh);
-    if (cells.size() != (uint64_t)g.n_valid * S.N) fail(kIoError, "inconsistent cells array in " + dir);
-    upload(S.cells, cells, s);
-    build_bucket_grid(S, cells, s);
-  }
-  upload(S.vec, read_npy<float>(dir + "/vec.npy", sh), s);
-  {
-    std::vector<float> lab = read_npy<float>(dir + "/lab.npy", sh);
-    S.labels_integral = std::all_of(lab.begin(), lab.end(), [](float v) { return v == std::floor(v); });
-    upload(S.lab, lab, s);
-  }
+  upload(S.centers, H.centers, s);
+  upload(S.sub_centers, H.subc, s);
+  prepare_mixed_centers(S, H.centers, H.subc, s);
+  upload(S.pq_centers, H.pqc, s);
+  upload(S.codes, H.codes, s);
+  upload(S.cells, H.cells, s);
+  build_bucket_grid(S, H.cells, s);
+  upload(S.vec, H.vec, s);
+  S.labels_integral = std::all_of(H.lab.begin(), H.lab.end(), [](float v) { return v == std::floor(v); });
+  upload(S.lab, H.lab, s);
   S.built = true;
   st_ = std::move(S);
 }

@@ -2769,6 +2769,7 @@ void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* ql
   if (d != st_.d) fail(kInvalidArgument, "query dimension differs from the index");
   if (nq == 0) return;
   if (!queries || !qlabels || !out_dist || !out_ids) fail(kInvalidArgument, "null query/output pointer");
+  if (cfg_.is_split) fail(kUnsupported, "IS_SPLIT = 1 (Morton split mode) is not supported");
   const int l = st_.l;
   if (qcols < (uint64_t)l) fail(kInvalidArgument, "query label columns < l");
   for (int i = 0; i < l; ++i)

@@ -91,7 +91,13 @@ hdg_status hdg_get_param(const hdg_index* index, const char* key, uint64_t* valu
  * counterpart (the reference rebuilds the index per data batch): checkpoint/resume and the shared
  * state that the CPU oracle is run on. */
 hdg_status hdg_save(const hdg_index* index, const char* dir);
+/* hdg_load checks the whole state before it replaces the index: every array's shape against meta.json,
+ * monotone offsets spanning the rows, zid a permutation, Morton codes ascending per sub-cluster, cells inside
+ * their dims, PQ codes below the codeword count; HDG_EIO otherwise (the index is left as it was). */
 hdg_status hdg_load(hdg_index* index, const char* dir, void* stream);
+/* The same checks without an index or a device: HDG_OK when `dir` holds a consistent state for
+ * `label_dims` label dims, HDG_EIO otherwise. */
+hdg_status hdg_validate_checkpoint(const char* dir, int label_dims);
 
 /* Per-query counters of the last hdg_query (host arrays of nq entries; any may be NULL):
  * cand = Morton-range candidates C_q (Super-Grid-IVF_PQ.cu:1184), real = in-box count of the chosen
@@ -154,7 +160,8 @@ int hdg_fit_mem_coeff(const double samples[4][3], double coeff[4]);
 
 /* Launch-parameter autotuner.  Times full queries of the given batch (device arrays, as hdg_query;
  * out_dist/out_ids receive the results) under a ladder of values of each result-invariant knob
- * (ADC_PER, ADC_WGS, SPARSE_MAX_TESTS, QUERY_CHUNK, SCORE_BLOCK, ADC_SPLIT, GRAPH), one knob at a time,
+ * (ADC_PER, ADC_WGS, QUERY_CHUNK, SCORE_BLOCK, ADC_SPLIT, GRAPH, and SPARSE_MAX_TESTS under PRECISION 0 only:
+ * under PRECISION 1 it changes which queries refine from their in-box lists), one knob at a time,
  * `reps` timed runs per value (median), and keeps the fastest value whose ids and distances are bit-identical to those of the
  * starting configuration.  The chosen values stay set on the index; the trial log is written to
  * report (cap bytes, may be NULL).  No reference counterpart: the reference's launch shapes are

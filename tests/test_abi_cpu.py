@@ -179,3 +179,108 @@ def test_recall_definition():
     res = np.array([[2, 2, 1, 7]], dtype=np.uint64)              # duplicate result counts once
     assert datasets.recall(gt, res, 4) == 1.0
     assert datasets.recall(gt, np.array([[7, 8, 9, 2]], np.uint64), 4) == 0.5
+
+
+def _write_checkpoint(td, N=40, d=8, C=2, S=2, M=4, K=16, l=2, seed=0):
+    """A small consistent index state in the hdg_save layout (meta.json + .npy)."""
+    rng = np.random.default_rng(seed)
+    CS = C * S
+    dims, max_bits = [4, 8], 3
+    seg = np.sort(rng.choice(np.arange(1, N), CS - 1, replace=False)).astype(np.uint64)
+    seg_off = np.concatenate([[0], seg, [N]]).astype(np.uint64)
+    cells = np.stack([rng.integers(0, dims[0], N), rng.integers(0, dims[1], N)]).astype(np.uint32)
+    zcode = np.zeros(N, np.uint64)
+    for i in range(CS):  # ascending codes inside every sub-cluster
+        a, b = int(seg_off[i]), int(seg_off[i + 1])
+        zcode[a:b] = np.sort(rng.integers(0, 1 << (2 * max_bits), b - a)).astype(np.uint64)
+    meta = {"format": 1, "N": N, "d": d, "C": C, "CS": CS, "M": M, "K": K, "pq_len": d // M, "l": l,
+            "dims": dims, "span": [100, 1000], "left": [0, 0], "shuffle": [1, 0], "max_bits": max_bits,
+            "valid_map": 3, "tot_grids": dims[0] * dims[1], "n_valid": 2}
+    json.dump(meta, open(os.path.join(td, "meta.json"), "w"))
+    arrays = {"zid": rng.permutation(N).astype(np.uint32), "zcode": zcode, "seg_off": seg_off,
+              "sub_off": np.arange(0, CS + 1, S).astype(np.uint32),
+              "centers": rng.standard_normal((C, d)).astype(np.float32),
+              "sub_centers": rng.standard_normal((CS, d)).astype(np.float32),
+              "pq_centers": rng.standard_normal((M, K, d // M)).astype(np.float32),
+              "codes": rng.integers(0, K, (N, M)).astype(np.uint8), "cells": cells,
+              "vec": rng.standard_normal((N, d)).astype(np.float32),
+              "lab": rng.integers(0, 100, (N, l)).astype(np.float32)}
+    for k, v in arrays.items():
+        np.save(os.path.join(td, k + ".npy"), v, allow_pickle=False)
+    return meta, arrays
+
+
+def _corruptions():
+    """(name, mutate(td, meta, arrays)) pairs, each making the checkpoint inconsistent in one way."""
+    def npy(name, value):
+        return lambda td, m, a: np.save(os.path.join(td, name + ".npy"), value(a[name]), allow_pickle=False)
+
+    def meta(key, value):
+        def f(td, m, a):
+            m = dict(m)
+            m[key] = value
+            json.dump(m, open(os.path.join(td, "meta.json"), "w"))
+        return f
+
+    def truncate(name):
+        def f(td, m, a):
+            p = os.path.join(td, name + ".npy")
+            data = open(p, "rb").read()
+            open(p, "wb").write(data[:-7])
+        return f
+
+    return [
+        ("sub_off past CS", npy("sub_off", lambda v: np.where(np.arange(v.size) == 1, 99, v).astype(np.uint32))),
+        ("sub_off end", npy("sub_off", lambda v: np.concatenate([v[:-1], [v[-1] - 1]]).astype(np.uint32))),
+        ("seg_off end below N", npy("seg_off", lambda v: np.concatenate([v[:-1], [v[-1] - 1]]).astype(np.uint64))),
+        ("seg_off not monotone", npy("seg_off", lambda v: v[[0, 2, 1] + list(range(3, v.size))].astype(np.uint64))),
+        ("zid out of range", npy("zid", lambda v: np.where(v == 0, 40, v).astype(np.uint32))),
+        ("zid repeated", npy("zid", lambda v: np.where(v == 0, 1, v).astype(np.uint32))),
+        ("cell outside dim", npy("cells", lambda v: np.where(np.arange(v.size).reshape(v.shape) == 3, 4, v)
+                                 .astype(np.uint32))),
+        ("cells shape", npy("cells", lambda v: v[:1].copy())),
+        ("codes above K", npy("codes", lambda v: np.where(v == v.max(), 200, v).astype(np.uint8))),
+        ("codes shape", npy("codes", lambda v: v[:, :3].copy())),
+        ("centers shape", npy("centers", lambda v: v[:1].copy())),
+        ("sub_centers dtype", npy("sub_centers", lambda v: v.astype(np.float64))),
+        ("pq_centers shape", npy("pq_centers", lambda v: v[:, :8].copy())),
+        ("vec truncated", truncate("vec")),
+        ("codes truncated", truncate("codes")),
+        ("lab shape", npy("lab", lambda v: v[:, :1].copy())),
+        ("zcode descending", npy("zcode", lambda v: v[::-1].copy())),
+        ("meta dims short", meta("dims", [4])),
+        ("meta shuffle repeated", meta("shuffle", [0, 0])),
+        ("meta max_bits too small", meta("max_bits", 2)),
+        ("meta pq_len", meta("pq_len", 3)),
+        ("meta K above 256", meta("K", 300)),
+        ("meta N mismatch", meta("N", 41)),
+    ]
+
+
+def test_checkpoint_validation_accepts_consistent_state():
+    with tempfile.TemporaryDirectory() as td:
+        _write_checkpoint(td)
+        H.validate_checkpoint(td, 2)
+        with pytest.raises(H.HDGridError) as e:   # label dims differ from filter.conf's l
+            H.validate_checkpoint(td, 3)
+        assert e.value.status == 6
+
+
+@pytest.mark.parametrize("case", range(len(_corruptions())), ids=[c[0] for c in _corruptions()])
+def test_checkpoint_validation_rejects_corruption(case):
+    """Truncated or inconsistent checkpoints fail with HDG_EIO before anything reaches a device."""
+    name, mutate = _corruptions()[case]
+    with tempfile.TemporaryDirectory() as td:
+        m, a = _write_checkpoint(td)
+        mutate(td, m, a)
+        with pytest.raises(H.HDGridError) as e:
+            H.validate_checkpoint(td, 2)
+        assert e.value.status == 3, name
+
+
+def test_is_split_is_refused_before_any_device_call():
+    """IS_SPLIT = 1 (the reference's Morton split mode) is refused at creation with HDG_EUNSUPPORTED instead of
+    silently running the no-split path."""
+    with pytest.raises(H.HDGridError) as e:
+        H.SuperGridIndex(conf_text="topk = 10\nis_split = 1\n", filter_text="l = 1\nfilter0:\n\ttype = 3\n")
+    assert e.value.status == 7
