@@ -44,6 +44,12 @@ constexpr float kExactPad = 1e6f;                 // refine.cu:53
 
 thread_local std::string g_err;
 
+// Planted mistakes for the tests that show the composition checks are sensitive (0 = the faithful
+// restatement).  1: beta and W over the whole call instead of per reference batch; 2: n_candies without
+// the min(., W); 3: l = 1 pass-through padded with UINT64_MAX only (no 2^32 candidate pads); 4: EXPS0 <= 1
+// path takes the PQ top-k before the label filter; 5: beta inverted (in-box / candidates).
+int g_plant = 0;
+
 // f32 -> u64 conversion as the GPU performs it (cvt.rzi.u64.f32 saturates; NaN -> 0).
 inline uint64_t f2u_sat(float x) {
   if (!(x > 0.0f)) return 0;                       // negative, zero and NaN
@@ -97,6 +103,8 @@ inline bool key_less(const Key& a, const Key& b) {
 extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
+
+void orc_plant_fault(int f) { g_plant = f; }
 
 // =============================================================================================
 // A1: filter recovery -- preprocessing.cu:3-30 (denormalize_labels_kernel), :206-230.
@@ -647,7 +655,7 @@ int orc_query(const OrcState* Sp, const float* queries, const float* qlabels, ui
   // z-pool cells: cells of point zids[p]
   std::vector<float> ranges(nq * 2 * l);
   std::vector<int> boxes(nq * 2 * l);
-  if (batch == 0) batch = nq;
+  if (batch == 0 || g_plant == 1) batch = nq;
   for (uint64_t b0 = 0; b0 < nq; b0 += batch) {
     const uint64_t b1 = std::min(nq, b0 + batch);
     const uint64_t nb = b1 - b0;
@@ -777,8 +785,9 @@ int orc_query(const OrcState* Sp, const float* queries, const float* qlabels, ui
         real_c += real[qi];
         coarse_c += cands[qi].size();
       }
-      float beta = (float)coarse_c / (float)real_c;
+      float beta = g_plant == 5 ? (float)real_c / (float)coarse_c : (float)coarse_c / (float)real_c;
       n_candies = std::min<uint64_t>(f2u_host((float)(refine_ratio * k) * beta), W);
+      if (g_plant == 2) n_candies = f2u_host((float)(refine_ratio * k) * beta);
     }
     // ---- A11..A13 per query ----
 #pragma omp parallel for schedule(dynamic, 1)
@@ -810,6 +819,14 @@ int orc_query(const OrcState* Sp, const float* queries, const float* qlabels, ui
         std::vector<uint64_t> surv;
         for (uint64_t c = 0; c < Cq; ++c)
           if (passes(cand[c])) surv.push_back(cand[c]);
+        if (g_plant == 4) {  // PQ top-k over every candidate first, then the label filter
+          std::vector<Key> all;
+          for (uint64_t c = 0; c < Cq; ++c) all.push_back({orc_adc(S.codes, S.N, cand[c], lut.data(), S.M, S.K), c});
+          topk_sorted(all, k);
+          surv.clear();
+          for (auto& it : all)
+            if (passes(cand[it.col])) surv.push_back(cand[it.col]);
+        }
         for (uint64_t j = 0; j < surv.size(); ++j)
           items.push_back({orc_adc(S.codes, S.N, surv[j], lut.data(), S.M, S.K), j});
         topk_sorted(items, k);
@@ -838,7 +855,8 @@ int orc_query(const OrcState* Sp, const float* queries, const float* qlabels, ui
           for (auto& it : items) L.push_back(it.col < Cq ? cand[it.col] : kIdPad);
         } else {
           // stride mismatch (product-quantization.cu:254-264): every column passes through
-          for (uint64_t c = 0; c < kp; ++c) L.push_back(c < Cq ? cand[c] : (c < W ? kIdPad : kU64Max));
+          for (uint64_t c = 0; c < kp; ++c)
+            L.push_back(c < Cq ? cand[c] : (c < W && g_plant != 3 ? kIdPad : kU64Max));
         }
       } else {
         kp = n_candies;

@@ -190,6 +190,11 @@ def merge_topk(dist, ids, data_batch_size, n_data, start_offset, k):
     return np.take_along_axis(flat_d, order, 1), np.take_along_axis(flat_i, order, 1)
 
 
+def plant_fault(f: int) -> None:
+    """Switch the oracle to a deliberately wrong variant (tests of the tests; 0 restores the faithful one)."""
+    _lib.orc_plant_fault(ctypes.c_int(f))
+
+
 def omp_threads():
     return int(_lib.orc_omp_threads())
 
