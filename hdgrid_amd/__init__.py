@@ -52,6 +52,7 @@ _create = _sig("hdg_create", ctypes.c_int, [_cp, _cp, _cp, ctypes.c_int, ctypes.
 _create_text = _sig("hdg_create_from_text", ctypes.c_int, [_cp, _cp, _cp, ctypes.c_int, ctypes.POINTER(_vp)])
 _build = _sig("hdg_build", ctypes.c_int, [_vp, _vp, _u64, _u64, _vp, _u64, _vp])
 _query = _sig("hdg_query", ctypes.c_int, [_vp, _vp, _u64, _u64, _vp, _u64, _vp, _vp, _vp])
+_query_noalloc = _sig("hdg_query_noalloc", ctypes.c_int, [_vp, _vp, _u64, _u64, _vp, _u64, _vp, _vp, _vp])
 _destroy = _sig("hdg_destroy", ctypes.c_int, [_vp])
 _set_param = _sig("hdg_set_param", ctypes.c_int, [_vp, _cp, _cp])
 _get_param = _sig("hdg_get_param", ctypes.c_int, [_vp, _cp, _u64p])
@@ -248,6 +249,17 @@ class SuperGridIndex:
         qcols = query_labels.shape[1] if query_labels.dim() == 2 else 1
         _check(_query(self._h, _ptr(queries), nq, queries.shape[1], _ptr(query_labels), qcols, _ptr(out_dist),
                       _ptr(out_ids), _stream_ptr(stream)))
+        return out_dist, out_ids
+
+    def query_noalloc(self, queries, query_labels, out_dist, out_ids, stream=None):
+        """hdg_query_noalloc: the same search with the workspace carved from the library's fixed per-device
+        scratch array (no device allocation).  The outputs must be given."""
+        assert queries.is_cuda and query_labels.is_cuda and out_dist.is_cuda and out_ids.is_cuda
+        queries = queries.contiguous()
+        query_labels = query_labels.contiguous()
+        qcols = query_labels.shape[1] if query_labels.dim() == 2 else 1
+        _check(_query_noalloc(self._h, _ptr(queries), queries.shape[0], queries.shape[1], _ptr(query_labels), qcols,
+                              _ptr(out_dist), _ptr(out_ids), _stream_ptr(stream)))
         return out_dist, out_ids
 
     # -- state / counters ----------------------------------------------------------------------

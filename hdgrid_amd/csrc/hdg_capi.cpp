@@ -222,6 +222,15 @@ hdg_status hdg_query(hdg_index* index, const float* queries, uint64_t nq, uint64
   });
 }
 
+hdg_status hdg_query_noalloc(hdg_index* index, const float* queries, uint64_t nq, uint64_t dim,
+                             const float* query_labels, uint64_t qlabel_cols, float* out_dist, uint64_t* out_ids,
+                             void* stream) {
+  return guarded([&] {
+    if (!index) hdg::fail(HDG_EINVAL, "null index");
+    index->impl->query_noalloc(queries, nq, dim, query_labels, qlabel_cols, out_dist, out_ids, (hipStream_t)stream);
+  });
+}
+
 hdg_status hdg_destroy(hdg_index* index) {
   return guarded([&] {
     if (!index) return;

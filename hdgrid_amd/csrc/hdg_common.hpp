@@ -46,23 +46,45 @@ inline uint64_t& devbuf_generation() {
   return g;
 }
 
-// Owning device allocation (hipMalloc / hipFree), grows on demand and never shrinks.
+// Bump allocator over a device region the caller provides (the allocation-free query path carves its whole
+// workspace out of one fixed array).  While an arena is active on this thread, DevBuf::reserve carves from it
+// instead of calling hipMalloc, and the carved buffers own nothing.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+};
+inline Arena*& active_arena() {
+  static thread_local Arena* a = nullptr;
+  return a;
+}
+// Thrown by DevBuf::reserve when the active arena cannot hold the request; `need` = bytes the arena would need.
+struct ArenaOverflow : Error {
+  size_t need;
+  ArenaOverflow(size_t n, const std::string& msg) : Error(kOutOfMemory, msg), need(n) {}
+};
+
+// Device buffer that grows on demand and never shrinks: an owning hipMalloc allocation, or (while an arena is
+// active) a non-owning piece of the arena.
 class DevBuf {
  public:
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_) {
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_), owned_(o.owned_) {
     o.p_ = nullptr;
     o.bytes_ = 0;
+    o.owned_ = false;
   }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
       p_ = o.p_;
       bytes_ = o.bytes_;
+      owned_ = o.owned_;
       o.p_ = nullptr;
       o.bytes_ = 0;
+      o.owned_ = false;
     }
     return *this;
   }
@@ -70,16 +92,29 @@ class DevBuf {
 
   void reserve(size_t bytes) {
     if (bytes <= bytes_) return;
+    if (Arena* a = active_arena()) {
+      if (owned_) fail(kBadState, "an owning device buffer would be reallocated inside an allocation-free call");
+      const size_t b = ((bytes < 256 ? 256 : bytes) + 255) & ~(size_t)255;
+      if (b > a->cap - a->used)
+        throw ArenaOverflow(a->used + b, "workspace needs " + std::to_string(a->used + b) + " bytes of the " +
+                                             std::to_string(a->cap) + "-byte scratch array");
+      p_ = a->base + a->used;
+      a->used += b;
+      bytes_ = b;
+      return;
+    }
     release();
     size_t b = bytes < 256 ? 256 : bytes;
     HDG_HIP(hipMalloc(&p_, b));
     bytes_ = b;
+    owned_ = true;
     ++devbuf_generation();
   }
   void release() {
-    if (p_) (void)hipFree(p_);
+    if (p_ && owned_) (void)hipFree(p_);
     p_ = nullptr;
     bytes_ = 0;
+    owned_ = false;
   }
   template <class T>
   T* as() const {
@@ -91,6 +126,7 @@ class DevBuf {
  private:
   void* p_ = nullptr;
   size_t bytes_ = 0;
+  bool owned_ = false;
 };
 
 template <class T>

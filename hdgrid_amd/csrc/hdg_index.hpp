@@ -105,6 +105,13 @@ class Index {
   void query(const float* queries, uint64_t nq, uint64_t d, const float* qlabels, uint64_t qcols,
              float* out_dist, uint64_t* out_ids, hipStream_t stream);
 
+  // query() with its whole workspace carved from this device's fixed scratch array (kNoAllocScratchBytes):
+  // no device or pinned allocation, no graph capture, no stage timing.  Chunks of whole reference batches are
+  // halved until one fits; a batch that does not fit, or call-wide counters beyond a quarter of the array, fail.
+  // Calls that overlap on one device race on the shared array.
+  void query_noalloc(const float* queries, uint64_t nq, uint64_t d, const float* qlabels, uint64_t qcols,
+                     float* out_dist, uint64_t* out_ids, hipStream_t stream);
+
   void save(const std::string& dir) const;
   void load(const std::string& dir, hipStream_t stream);
 
@@ -157,6 +164,9 @@ class Index {
   void query_chunk(const float* queries, uint64_t q0, uint64_t nq, const float* qlabels,
                    uint64_t qcols, uint64_t group, float* out_dist, uint64_t* out_ids,
                    hipStream_t stream);
+  // argument checks shared by query() and query_noalloc(); returns the reference batch size G
+  uint64_t check_query(const float* queries, uint64_t nq, uint64_t d, const float* qlabels, uint64_t qcols,
+                       float* out_dist, uint64_t* out_ids) const;
   Config cfg_;
   FilterConfig fc_;
   int device_;
@@ -167,7 +177,13 @@ class Index {
   // query workspace (grown on demand, reused across calls)
   struct Work;
   Work* w_;
+  Work* wn_ = nullptr;      // workspace of query_noalloc (every buffer a piece of the scratch array)
+  Work* last_w_ = nullptr;  // workspace of the last query call (read by last_stats / last_chosen_subs)
+  void* scratch_ = nullptr; // device address of this device's scratch array
 };
+
+// Size of the per-device scratch array behind Index::query_noalloc.
+constexpr uint64_t kNoAllocScratchBytes = 64ull << 20;
 
 // findMaxFactor (math_utils.h:40-60).
 int find_max_factor(uint64_t a, uint64_t b);

@@ -17,6 +17,8 @@
 // reference's compiler contracts is an explicit fmaf().
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
@@ -2664,11 +2666,35 @@ struct Index::Work {
   hipStream_t gstream = nullptr;
   hipEvent_t gev[2];
   uint64_t* pinned = nullptr;
+  bool noalloc = false;  // workspace of query_noalloc: no events, every buffer carved from the scratch array
   hipEvent_t ev[16];
   bool ev_init = false;
   int ev_used = 0;
   const char* const* ev_names = nullptr;
 };
+
+// Workspace of the allocation-free query path: one fixed array per device, held until the process ends.  Host
+// code never uses this name as an address: the device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices] = {};
+
+// Device address of the scratch array on `device`, which must be the current device.
+void* scratch_address(int device) {
+  if (device < 0 || device >= kScratchDevices) fail(kUnsupported, "device ordinal above 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    HDG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(kHipError, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
 
 int find_max_factor(uint64_t a, uint64_t b) {
   if (b <= a) return (int)b;
@@ -2684,8 +2710,14 @@ int find_max_factor(uint64_t a, uint64_t b) {
 
 Index::Index(const Config& cfg, const FilterConfig& fc, int device) : cfg_(cfg), fc_(fc), device_(device) {
   w_ = new Work();
+  last_w_ = w_;
   HDG_HIP(hipSetDevice(device_));
   HDG_HIP(hipHostMalloc(reinterpret_cast<void**>(&w_->pinned), 64 * sizeof(uint64_t), hipHostMallocDefault));
+  // set-up of the allocation-free path: its host-side workspace, pinned read-back words and the scratch address
+  wn_ = new Work();
+  wn_->noalloc = true;
+  HDG_HIP(hipHostMalloc(reinterpret_cast<void**>(&wn_->pinned), 64 * sizeof(uint64_t), hipHostMallocDefault));
+  scratch_ = scratch_address(device_);
   std::vector<int> ft(fc_.filter_type.begin(), fc_.filter_type.end());
   std::vector<float> sv(fc_.shift_val.begin(), fc_.shift_val.end());
   std::vector<float> mp = fc_.flat_map();
@@ -2712,10 +2744,14 @@ Index::~Index() {
       for (auto& e : w_->ev) (void)hipEventDestroy(e);
     delete w_;
   }
+  if (wn_) {
+    if (wn_->pinned) (void)hipHostFree(wn_->pinned);
+    delete wn_;  // its buffers are pieces of the scratch array and own nothing
+  }
 }
 
 void Index::last_stats(uint64_t nq, uint64_t* const out[6]) const {
-  Work& w = *w_;
+  Work& w = *last_w_;
   if (nq != w.st_nq) fail(kInvalidArgument, "nq differs from the last query call");
   if (nq == 0) return;
   HDG_HIP(hipSetDevice(device_));
@@ -2727,7 +2763,7 @@ void Index::last_stats(uint64_t nq, uint64_t* const out[6]) const {
 }
 
 void Index::last_chosen_subs(uint64_t nq, uint32_t* out) const {
-  Work& w = *w_;
+  Work& w = *last_w_;
   if (nq != w.st_nq) fail(kInvalidArgument, "nq differs from the last query call");
   if (nq == 0) return;
   HDG_HIP(hipSetDevice(device_));
@@ -2735,7 +2771,7 @@ void Index::last_chosen_subs(uint64_t nq, uint32_t* out) const {
 }
 
 std::vector<std::pair<std::string, float>> Index::stage_times() const {
-  Work& w = *w_;
+  Work& w = *last_w_;
   std::vector<std::pair<std::string, float>> r;
   if (w.ev_used < 2) return r;
   HDG_HIP(hipSetDevice(device_));
@@ -2762,12 +2798,11 @@ void Index::debug_coarse_scores(const float* queries, uint64_t nq, float* out_ho
   HDG_HIP(hipStreamSynchronize(s));
 }
 
-void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* qlabels, uint64_t qcols,
-                  float* out_dist, uint64_t* out_ids, hipStream_t s) {
-  HDG_HIP(hipSetDevice(device_));
+uint64_t Index::check_query(const float* queries, uint64_t nq, uint64_t d, const float* qlabels, uint64_t qcols,
+                            float* out_dist, uint64_t* out_ids) const {
   if (!st_.built) fail(kBadState, "query before build");
   if (d != st_.d) fail(kInvalidArgument, "query dimension differs from the index");
-  if (nq == 0) return;
+  if (nq == 0) return 0;
   if (!queries || !qlabels || !out_dist || !out_ids) fail(kInvalidArgument, "null query/output pointer");
   if (cfg_.is_split) fail(kUnsupported, "IS_SPLIT = 1 (Morton split mode) is not supported");
   const int l = st_.l;
@@ -2775,8 +2810,89 @@ void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* ql
   for (int i = 0; i < l; ++i)
     if (fc_.filter_type[i] == 3 && qcols < 2 * (uint64_t)l)
       fail(kInvalidArgument, "explicit-range filter needs 2*l query label columns");
-  uint64_t G = cfg_.parity_batch ? cfg_.parity_batch : (uint64_t)find_max_factor(125, nq);
+  return cfg_.parity_batch ? cfg_.parity_batch : (uint64_t)find_max_factor(125, nq);
+}
+
+void Index::query_noalloc(const float* queries, uint64_t nq, uint64_t d, const float* qlabels, uint64_t qcols,
+                          float* out_dist, uint64_t* out_ids, hipStream_t s) {
+  HDG_HIP(hipSetDevice(device_));
+  const uint64_t G = check_query(queries, nq, d, qlabels, qcols, out_dist, out_ids);
+  if (nq == 0) return;
+  if (!scratch_) fail(kBadState, "no scratch array on this device");
+  if (budget_capturing()) fail(kBadState, "allocation-free query during a budget fit");
+  // the call-wide counters and chosen sub-cluster ids stay in the array for the whole call
+  const uint64_t call_bytes = ((nq * 6 * 8 + 255) & ~255ull) + ((nq * cfg_.sub_lists * 4 + 255) & ~255ull);
+  if (call_bytes > kNoAllocScratchBytes / 4)
+    fail(kUnsupported, "allocation-free query: " + std::to_string(nq) + " queries need " +
+                           std::to_string(call_bytes) + " bytes of call-wide counters, above a quarter of the " +
+                           std::to_string(kNoAllocScratchBytes) + "-byte scratch array; split the call");
+  Arena arena{reinterpret_cast<char*>(scratch_), (size_t)kNoAllocScratchBytes, 0};
+  Work* const normal = w_;
+  struct Restore {
+    Index* ix;
+    Work* w;
+    ~Restore() {
+      active_arena() = nullptr;
+      ix->w_ = w;
+    }
+  } restore{this, normal};
+  // forget every piece of the previous call (their memory is the same array)
+  {
+    Work fresh;
+    fresh.pinned = wn_->pinned;
+    fresh.noalloc = true;
+    *wn_ = std::move(fresh);
+  }
+  w_ = wn_;
+  last_w_ = wn_;
+  active_arena() = &arena;
+  Work& w = *wn_;
+  w.st.reserve(nq * 6 * 8);
+  w.st_chosen.reserve(nq * cfg_.sub_lists * 4);
+  w.st_nq = nq;
+  w.st_sub_lists = cfg_.sub_lists;
+  w.ev_used = 0;
+  const size_t base = arena.used;
   uint64_t chunk = std::max<uint64_t>(G, (cfg_.query_chunk / G) * G);
+  for (uint64_t q0 = 0; q0 < nq;) {
+    const uint64_t qn = std::min(chunk, nq - q0);
+    // every chunk carves its workspace again from the same place; the kernels of the previous chunk are ahead
+    // of this one's on the stream, so reusing the memory is ordered
+    {
+      Work fresh;
+      fresh.pinned = wn_->pinned;
+      fresh.noalloc = true;
+      fresh.st = std::move(wn_->st);
+      fresh.st_chosen = std::move(wn_->st_chosen);
+      fresh.st_nq = nq;
+      fresh.st_sub_lists = cfg_.sub_lists;
+      *wn_ = std::move(fresh);
+    }
+    arena.used = base;
+    try {
+      query_chunk(queries, q0, qn, qlabels, qcols, G, out_dist, out_ids, s);
+    } catch (const ArenaOverflow& e) {
+      // halve the chunk (whole reference batches) and run it again; the overwritten outputs of the failed try
+      // are written again
+      if (qn > G) {
+        chunk = std::max<uint64_t>(G, (qn / 2 / G) * G);
+        continue;
+      }
+      fail(kOutOfMemory, "allocation-free query: one reference batch of " + std::to_string(G) +
+                             " queries needs " + std::to_string(e.need) + " bytes, above the " +
+                             std::to_string(kNoAllocScratchBytes) + "-byte scratch array");
+    }
+    q0 += qn;
+  }
+}
+
+void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* qlabels, uint64_t qcols,
+                  float* out_dist, uint64_t* out_ids, hipStream_t s) {
+  HDG_HIP(hipSetDevice(device_));
+  const uint64_t G = check_query(queries, nq, d, qlabels, qcols, out_dist, out_ids);
+  if (nq == 0) return;
+  uint64_t chunk = std::max<uint64_t>(G, (cfg_.query_chunk / G) * G);
+  last_w_ = w_;
   Work& w = *w_;
   w.st.reserve(nq * 6 * 8);
   w.st_chosen.reserve(nq * cfg_.sub_lists * 4);
@@ -2992,13 +3108,13 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   // dynamic LDS of the per-query select kernels: the sort buffer
   const uint32_t scap_c = sort_cap(n_list), scap_s = sort_cap(sub_lists), scap_f = sort_cap(topk);
   const unsigned dist_grid = 4096;
-  if (!w.ev_init) {
+  if (!w.ev_init && !w.noalloc) {
     for (auto& e : w.ev) HDG_HIP(hipEventCreate(&e));
     w.ev_init = true;
   }
   int evn = 0;
   auto mark = [&]() {
-    if (!w.capturing) HDG_HIP(hipEventRecord(w.ev[evn++], s));
+    if (!w.capturing && w.ev_init) HDG_HIP(hipEventRecord(w.ev[evn++], s));
   };
   mark();
   // one thread per query, 64-thread blocks so that a 10k-query chunk spreads over more CUs

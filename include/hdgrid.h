@@ -79,6 +79,22 @@ hdg_status hdg_query(hdg_index* index, const float* queries, uint64_t nq, uint64
                      const float* query_labels, uint64_t qlabel_cols, float* out_dist,
                      uint64_t* out_ids, void* stream);
 
+/* hdg_query with no allocation: no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and errors as hdg_query, but the whole workspace is carved from one fixed 64 MiB array per device
+ * (a static device array the library holds until the process ends; its address is looked up once per device
+ * when an index is created): the call makes no hipMalloc / hipFree / pinned allocation, never captures a
+ * graph (GRAPH is ignored) and records no stage times.  Queries run in chunks of whole PARITY_BATCH groups,
+ * halved until one chunk's workspace fits.  Limits: the call-wide counters (nq * (48 + 4 * SUB_LISTS) bytes)
+ * must fit in a quarter of the array, else HDG_EUNSUPPORTED; one PARITY_BATCH group whose workspace exceeds the
+ * array fails with HDG_ENOMEM.  Every call on a device shares that device's array and overwrites it: calls
+ * that overlap on one device (other threads, other streams, other indexes) race, and hdg_last_query_stats /
+ * hdg_last_chosen_subs after this call read the array, so they are valid until the next allocation-free call
+ * on the device. */
+hdg_status hdg_query_noalloc(hdg_index* index, const float* queries, uint64_t nq, uint64_t dim,
+                             const float* query_labels, uint64_t qlabel_cols, float* out_dist,
+                             uint64_t* out_ids, void* stream);
+
 /* Releases the index and its device memory (the reference frees MEM_INDEX, mmr.cuh:73-80). */
 hdg_status hdg_destroy(hdg_index* index);
 
