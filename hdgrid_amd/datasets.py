@@ -79,6 +79,19 @@ def gaussian_mixture(rng: np.random.Generator, n: int, d: int, means: np.ndarray
     return out
 
 
+def gaussian_mixture_device(gen, n: int, d: int, means, sigma: float, device, chunk: int = 1 << 22):
+    """gaussian_mixture drawn on a device with a seeded torch generator (for the 50M workload, where the host
+    generator takes tens of seconds); returns a device tensor."""
+    import torch
+    out = torch.empty((n, d), dtype=torch.float32, device=device)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        comp = torch.randint(0, means.shape[0], (e - s,), generator=gen, device=device)
+        x = torch.randn((e - s, d), generator=gen, device=device, dtype=torch.float32)
+        out[s:e] = x.mul_(sigma).add_(means[comp])
+    return out
+
+
 @dataclass
 class Workload:
     name: str
@@ -91,6 +104,8 @@ class Workload:
     filter_arrays: Dict[str, list] = field(default_factory=dict)  # ftype, shift_val, imap, div_value
     notes: List[str] = field(default_factory=list)
     sampler: Optional[object] = None  # sampler(n, seed) -> (queries, qlabels) from the same distribution
+    data_dev: Optional[object] = None    # device copies when the workload was generated on a device
+    labels_dev: Optional[object] = None
 
     def sample_queries(self, n: int, seed: int):
         """Further queries from the workload's distribution (e.g. training queries for the budget model)."""
@@ -193,14 +208,31 @@ def explicit_ranges(n: int = 200_000, nq: int = 2000, d: int = 128, components: 
 
 def full_pipeline(n: int = 50_000_000, nq: int = 100_000, d: int = 96, components: int = 4096,
                   seed: int = 0x4D6005, clusters: int = 512, sub_clusters: int = 10, n_list: int = 512,
-                  sub_lists: int = 128) -> Workload:
+                  sub_lists: int = 128, device=None) -> Workload:
     """50M x 96 f32 mixture (4096 components, means N(0,1), sigma 0.3); four label dims: timestamp
     U[0, 31,536,000) with shift [604800, 0]; length U[0, 2000] with shift [30, 30]; category 1..5 with
     interval_map [1,2,1,2,3,4,3,5,4,5]; u U[0, 1e6] with div 10 (range [(int)x / 10, FLT_MAX]).
-    MEM_BOUND 200e9, n_list 512, sub_lists 128, PQ_DIM 24 (4 dims per subvector)."""
-    rng = np.random.default_rng(seed)
-    means = rng.standard_normal((components, d), dtype=np.float32)
-    data = gaussian_mixture(rng, n, d, means, 0.3)
+    MEM_BOUND 200e9, n_list 512, sub_lists 128, PQ_DIM 24 (4 dims per subvector).
+    device: draw the base vectors and labels there with a seeded torch generator (the host copies are
+    still returned; data_dev / labels_dev keep the device tensors)."""
+    if device is not None:
+        import torch
+        gen = torch.Generator(device=device)
+        gen.manual_seed(seed)
+        means_t = torch.randn((components, d), generator=gen, device=device, dtype=torch.float32)
+        data_dev = gaussian_mixture_device(gen, n, d, means_t, 0.3, device)
+        hi = [31_536_000, 2001, 6, 1_000_001]
+        lo = [0, 0, 1, 0]
+        labels_dev = torch.stack([torch.randint(lo[i], hi[i], (n,), generator=gen, device=device)
+                                  for i in range(4)], 1).to(torch.float32)
+        means = means_t.cpu().numpy()
+        data = data_dev.cpu().numpy()
+        labels = labels_dev.cpu().numpy()
+    else:
+        rng = np.random.default_rng(seed)
+        means = rng.standard_normal((components, d), dtype=np.float32)
+        data = gaussian_mixture(rng, n, d, means, 0.3)
+        data_dev = labels_dev = None
 
     def labs(r, m):
         ts = r.integers(0, 31_536_000, size=m).astype(np.float32)
@@ -209,7 +241,8 @@ def full_pipeline(n: int = 50_000_000, nq: int = 100_000, d: int = 96, component
         u = r.integers(0, 1_000_001, size=m).astype(np.float32)
         return np.stack([ts, ln, cat, u], axis=1)
 
-    labels = labs(rng, n)
+    if device is None:
+        labels = labs(rng, n)
     qrng = np.random.default_rng(seed + 1)
     queries = gaussian_mixture(qrng, nq, d, means, 0.3)
     qlabels = labs(qrng, nq)
@@ -221,7 +254,12 @@ def full_pipeline(n: int = 50_000_000, nq: int = 100_000, d: int = 96, component
             "filter3:\n\ttype = 2\n\tdiv_value = [10]\n")
     fa = dict(ftype=[0, 0, 1, 2], shift_val=[604800.0, 0.0, 30.0, 30.0, 0.0, 0.0, 0.0, 0.0],
               imap=[1.0, 2.0, 1.0, 2.0, 3.0, 4.0, 3.0, 5.0, 4.0, 5.0], div_value=[0, 0, 0, 10])
-    return Workload("full-50m-d96-l4", conf, filt, data, labels, queries, qlabels, fa)
+    def sampler(m, s):
+        r = np.random.default_rng(s)
+        return gaussian_mixture(r, m, d, means, 0.3), labs(r, m)
+
+    return Workload("full-50m-d96-l4", conf, filt, data, labels, queries, qlabels, fa, sampler=sampler,
+                    data_dev=data_dev, labels_dev=labels_dev)
 
 
 def recall(gt_ids: np.ndarray, res_ids: np.ndarray, k: int) -> float:

@@ -652,7 +652,11 @@ int orc_query(const OrcState* Sp, const float* queries, const float* qlabels, ui
     }
   std::vector<int> cells;
   point_cells(S, cells);
-  // z-pool cells: cells of point zids[p]
+  // cells in z-pool order, so the in-box count below reads them sequentially
+  std::vector<int> zcells(S.N * l);
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < (int64_t)S.N; ++p)
+    for (int i = 0; i < l; ++i) zcells[(uint64_t)p * l + i] = cells[S.zids[p] * l + i];
   std::vector<float> ranges(nq * 2 * l);
   std::vector<int> boxes(nq * 2 * l);
   if (batch == 0 || g_plant == 1) batch = nq;
@@ -673,7 +677,7 @@ int orc_query(const OrcState* Sp, const float* queries, const float* qlabels, ui
       for (uint64_t s = 0; s < S.CS; ++s) {
         uint64_t c = 0;
         for (uint64_t p = S.seg_off[s]; p < S.seg_off[s] + S.seg_len[s]; ++p) {
-          const int* pc = &cells[S.zids[p] * l];
+          const int* pc = &zcells[p * l];
           bool in = true;
           for (int i = 0; i < l && in; ++i) {
             if (!((S.valid_map >> i) & 1)) continue;

@@ -237,7 +237,9 @@ def load_state(directory: str) -> dict:
     st = dict(meta=meta)
     for name in ["zid", "zcode", "seg_off", "sub_off", "centers", "sub_centers", "pq_centers", "codes", "cells",
                  "vec", "lab"]:
-        st[name] = np.load(os.path.join(directory, name + ".npy"), allow_pickle=False)
+        path = os.path.join(directory, name + ".npy")
+        # memory-mapped: the query restatement reads only some arrays (never vec), and only parts of big ones
+        st[name] = np.load(path, allow_pickle=False, mmap_mode="r") if os.path.exists(path) else None
     return st
 
 

@@ -28,14 +28,26 @@ DEV = torch.device("cuda:0")
 BUILD_FAST = dict(KMEANS_ITERS=6, BALANCE_ITERS=8, KMEANS_TRAIN=65536, PQ_TRAIN=16384)
 
 
+def _big_tmp_root():
+    """A RAM-backed temporary root for index states of several GB, when the machine has one with room."""
+    import shutil
+    if os.path.isdir("/dev/shm") and shutil.disk_usage("/dev/shm").free > 64 * 2**30:
+        return "/dev/shm"
+    return None
+
+
 class Built:
-    def __init__(self, w, **conf):
+    def __init__(self, w, tmp_root=None, **conf):
         self.w = w
         self.conf = dict(BUILD_FAST, **conf)
         self.idx = H.SuperGridIndex(conf_text=w.conf_text(**self.conf), filter_text=w.filter_text)
         print(f"[parity] build {w.name} n={w.data.shape[0]} d={w.data.shape[1]} conf={conf}", flush=True)
-        self.idx.build(torch.from_numpy(w.data).to(DEV), torch.from_numpy(w.labels).to(DEV))
-        self.tmp = tempfile.TemporaryDirectory()
+        data = w.data_dev if w.data_dev is not None else torch.from_numpy(w.data).to(DEV)
+        labels = w.labels_dev if w.labels_dev is not None else torch.from_numpy(w.labels).to(DEV)
+        self.idx.build(data, labels)
+        torch.cuda.synchronize()
+        print(f"[parity] built {w.name}; saving the state", flush=True)
+        self.tmp = tempfile.TemporaryDirectory(dir=tmp_root)
         self.idx.save(self.tmp.name)
         self.state = O.load_state(self.tmp.name)
         self.q = torch.from_numpy(w.queries).to(DEV)
@@ -234,3 +246,84 @@ def test_recall_vs_bruteforce(l3):
     r = datasets.recall(bi, gi, 100)
     print(f"[parity] recall@100 l3 = {r:.4f}")
     assert r > 0.5
+
+
+# ---- the workloads at their own sizes ------------------------------------------------------------------
+def test_config1_plain_full_size():
+    """100k x 128, one int label with shift +-1000, 1,000 queries, n_list 64, CLUSTERS 512 x 10: every query
+    bit-exact against the oracle, with PQ_DIM 32 (4 dims per sub-vector) and PQ_DIM 4 (the README sample)."""
+    w = datasets.plain_small()
+    for pq_dim in (32, 4):
+        b = Built(w, PQ_DIM=pq_dim)
+        if pq_dim == 32:
+            check_state(b)
+        assert_parity(b.run(EXPS0=30), f"config 1 PQ_DIM={pq_dim}")
+        del b
+
+
+def test_config2_full_index_subset():
+    """The 1M x 384 three-label index at full size, 200 queries in two reference batches of 100."""
+    w = datasets.mixed_1m(nq=200)
+    b = Built(w, tmp_root=_big_tmp_root())
+    res = b.run(EXPS0=30, PARITY_BATCH=100)
+    assert_parity(res, "config 2 full index")
+
+
+def test_config3_full_index_subset():
+    """The 5M x 128 uneven-selectivity index (explicit ranges, selectivity 1e-4..0.5) at full size, 200 queries in
+    two reference batches of 100, with the benchmark's walk limit; the PQ top-k' stage runs for the wide boxes."""
+    w = datasets.explicit_ranges(n=5_000_000, nq=200, d=128, components=1024, clusters=512, sub_clusters=10)
+    b = Built(w, tmp_root=_big_tmp_root(), N_LIST=512, SPARSE_MAX_TESTS=16384)
+    res = b.run(EXPS0=30, PARITY_BATCH=100)
+    assert (res[2].adc > 0).any(), "the wide boxes must reach the ADC stage"
+    assert_parity(res, "config 3 full index")
+
+
+def test_config5_full_size():
+    """50M x 96 with four label dims (shift, shift, interval map, div 10), CLUSTERS 512 x 10, n_list 512,
+    sub_lists 128.  The first 100 queries (one reference batch) bit-exact against the oracle; all 100k queries
+    through the size-independent properties: the first batch of the full call equals the subset call, every
+    returned id passes the exact four-dim predicate once, distances ascend and equal the oracle's FP32 L2 bit for
+    bit on a sample, and a recall@100 floor against the FP64 brute force on the first 100 queries."""
+    w = datasets.full_pipeline(device=DEV)
+    b = Built(w, tmp_root=_big_tmp_root(), PARITY_BATCH=100)
+    n_sub = 100
+    q_all = torch.from_numpy(w.queries).to(DEV)
+    ql_all = torch.from_numpy(w.qlabels).to(DEV)
+    print("[parity] config 5: 100k queries", flush=True)
+    d_all, i_all = b.idx.query(q_all, ql_all)
+    torch.cuda.synchronize()
+    d_all, i_all = d_all.cpu().numpy(), i_all.cpu().numpy().astype(np.uint64)
+    st_all = b.idx.last_stats(w.queries.shape[0])
+    # the oracle subset: queries 0..99, one reference batch
+    sub = datasets.Workload(w.name, w.conf, w.filter_text, w.data, w.labels, w.queries[:n_sub].copy(),
+                            w.qlabels[:n_sub].copy(), w.filter_arrays)
+    b.w = sub
+    b.q, b.ql = q_all[:n_sub].contiguous(), ql_all[:n_sub].contiguous()
+    print("[parity] config 5: oracle on the first reference batch", flush=True)
+    res = b.run(EXPS0=30)
+    assert (res[2].adc > 0).any() and (res[2].kprime > 0).any()
+    assert_parity(res, "config 5 first batch")
+    np.testing.assert_array_equal(i_all[:n_sub], res[1], err_msg="first batch of the full call")
+    assert np.array_equal(d_all[:n_sub].view(np.uint32), res[0].view(np.uint32))
+    # predicate, uniqueness, order over all 100k queries
+    ranges = O.recover_filters(w.qlabels, 4, **w.filter_arrays)
+    valid = i_all < np.uint64(w.data.shape[0])
+    lab = w.labels[np.where(valid, i_all, 0).astype(np.int64)]
+    ok = np.all((lab >= ranges[:, None, 0::2]) & (lab <= ranges[:, None, 1::2]), axis=2)
+    assert np.all(ok | ~valid), "a returned id fails the label predicate"
+    for r in range(0, i_all.shape[0], 997):
+        v = i_all[r][valid[r]]
+        assert np.unique(v).size == v.size, f"query {r}: repeated id"
+    dv = np.where(valid, d_all, np.float32(3e38))
+    assert np.all(np.diff(dv, axis=1)[valid[:, 1:]] >= 0), "distances not ascending"
+    rng = np.random.default_rng(5)
+    for r in rng.choice(w.queries.shape[0], 40, replace=False):
+        for j in np.nonzero(valid[r])[0][:10]:
+            exact = O.exact_l2(w.queries[r], w.data[int(i_all[r, j])])
+            assert np.float32(exact).view(np.uint32) == d_all[r, j].view(np.uint32), f"query {r} rank {j}"
+    _, gt = O.bruteforce(w.data, w.labels, w.queries[:n_sub], ranges[:n_sub], 100)
+    rec = datasets.recall(gt, i_all[:n_sub], 100)
+    print(f"[parity] config 5 recall@100 (first 100 queries) = {rec:.4f}, mean C_q = {st_all.cand.mean():.0f}, "
+          f"mean k' = {st_all.kprime.mean():.0f}", flush=True)
+    assert rec > 0.6  # a sanity floor: SUB_LISTS 128 on this fast-built index gives about 0.75
