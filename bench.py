@@ -1,9 +1,9 @@
 """Super-Grid filtered-ANN benchmark on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): 1M base vectors, d=384 f32, L2-normalised
-Gaussian mixture (512 components); three label dims -- timestamp U[0, 31,536,000) with shift [604800, 0],
-length U[0, 2000] with shift [30, 30], category 1..5 with interval_map [1,2,1,2,3,4,3,5,4,5]; 10k queries,
-topk 100, n_list 512, clusters 512 x sub_clusters 10, PQ_DIM 96 (4 dims per subvector), 256 codewords.
+Workload of the headline `value` (the configuration the metric is quoted on): 1M base vectors, d=384 f32,
+L2-normalised Gaussian mixture (512 components); three label dims -- timestamp U[0, 31,536,000) with shift
+[604800, 0], length U[0, 2000] with shift [30, 30], category 1..5 with interval_map [1,2,1,2,3,4,3,5,4,5]; 10k
+queries, topk 100, n_list 512, clusters 512 x sub_clusters 10, PQ_DIM 96 (4 dims per subvector), 256 codewords.
 Synthetic data, seeded.
 
 A step is one hdg_query over all 10k queries with the index and the queries resident in HBM.  Three paths
@@ -11,15 +11,20 @@ are measured, each at the smallest sub-cluster budget reaching recall@100 >= 0.9
 brute force (on a 2,000-query sample):
   plain   -- PRECISION 0, the reference-exact FP32 path (sub_lists doubled from 128 until the target);
   mixed   -- PRECISION 1: BF16 MFMA cluster and sub-cluster scoring, FP16 LUT, FP32 re-rank, fixed budget;
-  learned -- PRECISION 1 + BUDGET_MODEL 1: per-query sub-cluster budget from the MLP trained on traces of
-             this pipeline over --train seeded training queries (labels = minimal budget level per query).
-`value` is the learned+mixed QPS; the plain and mixed QPS are reported in `config`.  A second workload (5M x 128,
-two labels with per-query selectivity from 1e-4 to 0.5, 20k queries per call) reports plain / mixed / autotuned
-graph-replayed QPS, the ADC scan's HBM roofline, and the learned budget against the smallest fixed budget that
-reaches the target (`batched.learned_vs_fixed`).
+  learned -- PRECISION 1 + BUDGET_MODEL 1: per-query sub-cluster budget from the MLP trained on --train seeded
+             training queries (labels = each query's minimal budget level), launch knobs autotuned, graph replay;
+             its recall is measured again after the autotuning, in exactly the timed configuration.
+`value` is the learned+mixed QPS; the other paths are reported in `config`, each with the error of its returned
+FP32 distances against FP64 recomputed ones.
+
+Further legs (in `config.legs`): the 100k x 128 one-label plain workload; the 5M x 128 uneven-selectivity
+workload (plain / mixed / autotuned graph-replayed QPS, the ADC scan's HBM roofline, learned vs the smallest
+fixed budget); the 50M x 96 four-label workload (plain / mixed / learned+mixed+graph at recall >= 0.95 and the
+ADC scan's roofline on its 1.2 GB code array, above the 256 MB memory-side cache).
 
 The CPU baseline is the oracle's FP64 filtered brute force (a C++ restatement of the reference's CPU search,
-ground_truth.cuh:166-233, with filter.conf predicates) on a bounded query sample, all host threads.
+ground_truth.cuh:166-233, with filter.conf predicates) on a bounded query sample, all host threads; the oracle's
+restatement of the plain Super-Grid query path is timed beside it on one core and on all cores.
 """
 import argparse
 import json
@@ -50,6 +55,226 @@ def traffic_of(kernel, workload):
         return None
 
 
+def dist_error(data, queries, ids, dists, rows=256):
+    """Error of returned FP32 distances against FP64 distances recomputed from the f32 vectors, over every valid
+    (query, rank) pair: max and mean relative error, max absolute error."""
+    import numpy as np
+    ids = np.asarray(ids).astype(np.uint64)
+    valid = ids < np.uint64(data.shape[0])
+    mx_rel = mx_abs = s_rel = 0.0
+    n = 0
+    for r0 in range(0, ids.shape[0], rows):
+        vv = valid[r0:r0 + rows]
+        if not vv.any():
+            continue
+        idx = np.where(vv, ids[r0:r0 + rows], 0).astype(np.int64)
+        x = data[idx].astype(np.float64)
+        q = queries[r0:r0 + rows, None, :].astype(np.float64)
+        d64 = ((x - q) ** 2).sum(-1)[vv]
+        err = np.abs(np.asarray(dists[r0:r0 + rows], np.float64)[vv] - d64)
+        rel = err / np.maximum(d64, 1e-30)
+        mx_rel = max(mx_rel, float(rel.max()))
+        mx_abs = max(mx_abs, float(err.max()))
+        s_rel += float(rel.sum())
+        n += int(vv.sum())
+    return {"max_rel": float(f"{mx_rel:.3e}"), "mean_rel": float(f"{(s_rel / max(n, 1)):.3e}"),
+            "max_abs": float(f"{mx_abs:.3e}"), "pairs": n}
+
+
+def plain_small_leg(args, H, datasets, O, dev, log):
+    """The 100k x 128 one-label workload (shift +-1000 on a label U[0, 1e6), n_list 64, clusters 512 x 10,
+    PQ_DIM 32, 1,000 queries): the plain path at the smallest sub_lists (doubling from 128) reaching the target
+    on all 1,000 queries, QPS of one call over the 1,000 queries, and its FP32 distance error."""
+    import numpy as np
+    import torch
+    w = datasets.plain_small()
+    idx = H.SuperGridIndex(conf_text=w.conf_text(), filter_text=w.filter_text)
+    idx.build(torch.from_numpy(w.data).to(dev), torch.from_numpy(w.labels).to(dev))
+    q, ql = torch.from_numpy(w.queries).to(dev), torch.from_numpy(w.qlabels).to(dev)
+    ranges = O.recover_filters(w.qlabels, 1, **w.filter_arrays)
+    _, gt = O.bruteforce(w.data, w.labels, w.queries, ranges, 100)
+    b = 128
+    while True:
+        idx.set_param("SUB_LISTS", b)
+        d, i = idx.query(q, ql)
+        torch.cuda.synchronize()
+        rec = datasets.recall(gt, i.cpu().numpy().astype(np.uint64), 100)
+        if rec >= args.target_recall or b >= 2048:
+            break
+        b *= 2
+    for _ in range(3):
+        idx.query(q, ql)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        idx.query(q, ql)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / 20
+    err = dist_error(w.data, w.queries, i.cpu().numpy(), d.cpu().numpy())
+    log(f"plain 100k: sub_lists {b}, recall {rec:.4f}, {el * 1e3:.3f} ms per 1,000 queries, dist err {err}")
+    del idx
+    return {"workload": "plain-100k-d128-l1: 100000 x 128 f32 64-component mixture, 1 label U[0,1e6) shift "
+                        "+-1000, 1000 queries, n_list 64, clusters 512 x 10, PQ_DIM 32",
+            "plain": {"qps": round(1000 / el, 1), "ms_per_call": round(el * 1e3, 4), "recall@100": round(rec, 4),
+                      "recall_sample": 1000, "sub_lists": b, "dist_err_vs_fp64": err}}
+
+
+def full_leg(args, H, datasets, O, dev, log):
+    """The 50M x 96 four-label workload (timestamp shift, length shift, category map, u div 10), clusters 512 x 10,
+    n_list 512, PQ_DIM 24, 100k queries per call.  Plain (PRECISION 0), mixed (PRECISION 1, fixed budget) and the
+    learned budget + mixed + graph replay, each at the smallest budget reaching the target on a 2,000-query FP64
+    sample; the ADC scan's roofline from the plain call (code array 1.2 GB, above the 256 MB memory-side cache)."""
+    import numpy as np
+    import torch
+    t0 = time.time()
+    w = datasets.full_pipeline(n=args.full_n, nq=args.full_nq, device=dev)
+    log(f"full: {w.name} {w.data.shape} generated in {time.time() - t0:.1f}s")
+    idx = H.SuperGridIndex(conf_text=w.conf_text(), filter_text=w.filter_text)
+    t0 = time.time()
+    idx.build(w.data_dev, w.labels_dev)
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    w.data_dev = w.labels_dev = None
+    torch.cuda.empty_cache()
+    nq = w.queries.shape[0]
+    gs = min(args.full_gt, nq)
+    t0 = time.time()
+    ranges = O.recover_filters(w.qlabels[:gs], 4, **w.filter_arrays)
+    _, gt = O.bruteforce(w.data, w.labels, w.queries[:gs], ranges, 100)
+    log(f"full: built in {build_s:.1f}s; ground truth for {gs} queries in {time.time() - t0:.1f}s")
+    q, ql = torch.from_numpy(w.queries).to(dev), torch.from_numpy(w.qlabels).to(dev)
+    od = torch.empty((nq, 100), dtype=torch.float32, device=dev)
+    oi = torch.empty((nq, 100), dtype=torch.int64, device=dev)
+    M = idx.get_param("PQ_DIM")
+
+    def recall_now():
+        d, i = idx.query(q[:gs], ql[:gs])
+        torch.cuda.synchronize()
+        d, i = d.cpu().numpy(), i.cpu().numpy()
+        return datasets.recall(gt, i.astype(np.uint64), 100), d, i
+
+    def sweep(start, factor, cap=2048):
+        b = start
+        while True:
+            idx.set_param("SUB_LISTS", int(b))
+            r, d, i = recall_now()
+            log(f"full   sub_lists={int(b)}: recall@100={r:.4f}")
+            if r >= args.target_recall or b >= cap:
+                return int(b), r, d, i
+            b *= factor
+
+    def timed(graph=0, steps=None):
+        steps = steps or args.full_steps
+        idx.set_param("GRAPH", graph)
+        idx.query(q, ql, od, oi)
+        torch.cuda.synchronize()
+        stage = {}
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            idx.query(q, ql, od, oi)
+            if not graph:
+                for k, v in idx.last_stage_ms().items():
+                    stage[k] = stage.get(k, 0.0) + v / steps
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t1) / steps
+        st = idx.last_stats(nq)
+        idx.set_param("GRAPH", 0)
+        return el, stage, st
+
+    out = {"workload": f"{w.name}: {args.full_n} x 96 f32 4096-component mixture, 4 label dims (ts shift "
+                       f"[604800,0], len shift [30,30], cat map, u div 10), {nq} queries per call, n_list 512, "
+                       f"clusters 512 x 10, PQ_DIM {M}", "build_s": round(build_s, 1), "recall_sample": gs}
+    def one_chunk(n):
+        """Stage times of one call that is a single chunk (stage events cover one chunk): the first n queries."""
+        keep = idx.get_param("QUERY_CHUNK")
+        idx.set_param("QUERY_CHUNK", n)
+        for _ in range(2):
+            idx.query(q[:n], ql[:n])
+        stage = {}
+        for _ in range(args.full_steps):
+            idx.query(q[:n], ql[:n])
+            for k, v in idx.last_stage_ms().items():
+                stage[k] = stage.get(k, 0.0) + v / args.full_steps
+        torch.cuda.synchronize()
+        st = idx.last_stats(n)
+        idx.set_param("QUERY_CHUNK", keep)
+        return stage, st
+
+    idx.set_param("PRECISION", 0)
+    b, r, d, i = sweep(128, 2)
+    el, _, st_all = timed()
+    n1 = min(nq, 16300)  # one chunk of whole reference batches of 100
+    stage, st = one_chunk(n1)
+    adc_ms = stage.get("adc_scan_kernel", 0.0)
+    adc_bytes = float(st.adc.sum() * M)
+    out["plain"] = {"qps": round(nq / el, 1), "ms_per_call": round(el * 1e3, 3), "recall@100": round(r, 4),
+                    "sub_lists": b, "mean_candidates": round(float(st_all.cand.mean()), 1),
+                    "mean_in_box": round(float(st_all.real.mean()), 1),
+                    "mean_kprime": round(float(st_all.kprime.mean()), 1),
+                    "mean_refined": round(float(st_all.refined.mean()), 1),
+                    "dist_err_vs_fp64": dist_error(w.data, w.queries[:gs], i, d),
+                    f"stages_ms ({n1} queries, one chunk)": {k: round(v, 3) for k, v in
+                                                             sorted(stage.items(), key=lambda kv: -kv[1])}}
+    log(f"full plain: sub_lists {b} recall {r:.4f}: {el * 1e3:.1f} ms/call ({nq / el:.0f} QPS); ADC "
+        f"{adc_bytes / 1e9:.1f} GB in {adc_ms:.2f} ms")
+    gbs = adc_bytes / (adc_ms * 1e-3) / 1e9 if adc_ms > 0 else 0.0
+    out["adc_roofline"] = {"kernel": "adc_scan_kernel (plain FP32 LUT, 24-code rows)", "bound": "hbm",
+                           "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(gbs / HBM_PEAK_GBS, 6), "traffic": traffic_of("adc_scan", w.name),
+                           "avg_ms": round(adc_ms, 4), "alg_bytes_per_launch": adc_bytes,
+                           "code_array_bytes": int(args.full_n) * M}
+    idx.set_param("PRECISION", 1)
+    b, r, d, i = sweep(128, 2 ** 0.5)
+    el, _, st = timed()
+    el_g, _, _ = timed(graph=1)
+    stage_m, _ = one_chunk(n1)
+    out["mixed"] = {"qps": round(nq / el, 1), "ms_per_call": round(el * 1e3, 3), "recall@100": round(r, 4),
+                    "sub_lists": b, "mean_candidates": round(float(st.cand.mean()), 1),
+                    "mean_refined": round(float(st.refined.mean()), 1),
+                    "dist_err_vs_fp64": dist_error(w.data, w.queries[:gs], i, d),
+                    f"stages_ms ({n1} queries, one chunk)": {k: round(v, 3) for k, v in
+                                                             sorted(stage_m.items(), key=lambda kv: -kv[1])}}
+    out["mixed_graph"] = {"qps": round(nq / el_g, 1), "ms_per_call": round(el_g * 1e3, 3)}
+    log(f"full mixed: sub_lists {b} recall {r:.4f}: {el * 1e3:.1f} ms/call; graph {el_g * 1e3:.1f} ms/call")
+    if args.full_train > 0:
+        cap = max(2 * b, 256)
+        idx.set_param("SUB_LISTS", cap)
+        t0 = time.time()
+        tq, tql = w.sample_queries(args.full_train, 0x7E5A)
+        tr = O.recover_filters(tql, 4, **w.filter_arrays)
+        _, tgt = O.bruteforce(w.data, w.labels, tq, tr, 100)
+        log(f"full learned: ground truth for {args.full_train} training queries in {time.time() - t0:.1f}s")
+        t0 = time.time()
+        rep = idx.budget_fit(torch.from_numpy(tq).to(dev), torch.from_numpy(tql).to(dev), tgt,
+                             levels=args.budget_levels, t0=0, s0=args.budget_s0, ratio=args.budget_ratio,
+                             target=args.target_recall, epochs=30)
+        log(f"full learned: model trained in {time.time() - t0:.1f}s: {rep}")
+        idx.set_param("BUDGET_MODEL", 1)
+        sq = min(nq, 20000)
+        tune = idx.autotune(q[:sq], ql[:sq], reps=2)
+        log("full learned autotune: " + tune.strip().splitlines()[-1])
+        idx.set_param("GRAPH", 1)
+        r, d, i = recall_now()  # the timed configuration: tuned knobs, graph replay
+        idx.set_param("GRAPH", 0)
+        el, _, st = timed(graph=1)
+        out["learned"] = {"qps": round(nq / el, 1), "ms_per_call": round(el * 1e3, 3), "recall@100": round(r, 4),
+                          "cap": cap, "mean_candidates": round(float(st.cand.mean()), 1),
+                          "dist_err_vs_fp64": dist_error(w.data, w.queries[:gs], i, d),
+                          "tuned": {k: idx.get_param(k) for k in ("ADC_PER", "ADC_WGS", "SPARSE_MAX_TESTS",
+                                                                   "QUERY_CHUNK", "SCORE_BLOCK", "ADC_SPLIT")},
+                          "graph": 1, "train_queries": args.full_train,
+                          "budget_model": {k: round(v, 4) for k, v in rep.items()}}
+        out["learned_vs_plain"] = round(out["learned"]["qps"] / out["plain"]["qps"], 3)
+        idx.set_param("BUDGET_MODEL", 0)
+        log(f"full learned+mixed+graph: recall {r:.4f}, {el * 1e3:.1f} ms/call ({nq / el:.0f} QPS), "
+            f"{out['learned_vs_plain']}x plain")
+    out["mixed_vs_plain"] = round(out["mixed"]["qps"] / out["plain"]["qps"], 3)
+    out["mixed_graph_vs_plain"] = round(out["mixed_graph"]["qps"] / out["plain"]["qps"], 3)
+    del idx
+    torch.cuda.empty_cache()
+    return out
+
+
 def batched_leg(args, H, datasets, O, dev, log):
     """Uneven selectivity: 5M x 128 f32 (1024-component mixture), two integer labels U[0, 1e7], 20k queries
     with explicit per-query ranges whose joint selectivity is log-uniform in [1e-4, 0.5] (filter type 3,
@@ -78,9 +303,10 @@ def batched_leg(args, H, datasets, O, dev, log):
     best = None
     for prec, name in [(0, "plain"), (1, "mixed")]:
         idx.set_param("PRECISION", prec)
-        _, i = idx.query(q[:ns], ql[:ns])
+        d_s, i = idx.query(q[:ns], ql[:ns])
         torch.cuda.synchronize()
         rec = datasets.recall(gt, i.cpu().numpy().astype(np.uint64), 100)
+        derr = dist_error(w.data, w.queries[:ns], i.cpu().numpy(), d_s.cpu().numpy())
         for _ in range(1):
             idx.query(q, ql)
         torch.cuda.synchronize()
@@ -99,7 +325,7 @@ def batched_leg(args, H, datasets, O, dev, log):
         out[name] = {"qps": round(w.queries.shape[0] / el, 1), "ms_per_call": round(el * 1e3, 3),
                      f"recall@100 ({ns}-query sample)": round(rec, 4),
                      "adc_queries": int((st.adc > 0).sum()), "adc_scan_ms": round(adc_ms, 4),
-                     "adc_scan_gbs": round(gbs, 1) if gbs else None}
+                     "adc_scan_gbs": round(gbs, 1) if gbs else None, "dist_err_vs_fp64": derr}
         log(f"batched {name}: {el * 1e3:.2f} ms/call ({w.queries.shape[0] / el:.0f} QPS), recall {rec:.4f}, "
             f"ADC scan {adc_bytes / 1e9:.2f} GB in {adc_ms:.3f} ms = {gbs:.0f} GB/s")
         if prec == 0:
@@ -216,6 +442,56 @@ def batched_learned(args, idx, w, q, ql, od, oi, H, datasets, O, dev, log):
             "learned_vs_fixed": round(el_fix / el_l, 3), "budget_model": rep}
 
 
+def cpu_supergrid(args, idx, w, O, sub_lists, log):
+    """The oracle's restatement of the plain Super-Grid query path (orc_query: A1-A13 on the saved index state)
+    timed on the first --cpu-sg-sample queries, on one core and on all cores, against the GPU plain result of
+    the same queries."""
+    import tempfile
+
+    import numpy as np
+
+    import hdgrid_amd as H
+    n = min(args.cpu_sg_sample, w.queries.shape[0])
+    if n <= 0:
+        return None
+    root = "/dev/shm" if os.path.isdir("/dev/shm") else None
+    with tempfile.TemporaryDirectory(dir=root) as td:
+        idx.save(td)
+        state = O.load_state(td)
+        batch = idx.get_param("PARITY_BATCH") or H.find_max_factor(125, n)
+        ps = O.prepare(state, w.data, w.labels, n_list=idx.get_param("N_LIST"), sub_lists=sub_lists,
+                       exps0=idx.get_param("EXPS0"), topk=100, **w.filter_arrays)
+        out = {}
+        for label, th in (("one_core", 1), ("all_cores", all_threads_of(O))):
+            O.set_threads(th)
+            t0 = time.perf_counter()
+            od, oi = O.run_query(ps, w.queries[:n], w.qlabels[:n], batch=batch)
+            el = time.perf_counter() - t0
+            out[label] = {"qps": round(n / el, 2), "threads": th}
+        O.set_threads(all_threads_of(O))
+    import torch
+    qd = torch.from_numpy(w.queries[:n]).to("cuda:0")
+    qld = torch.from_numpy(w.qlabels[:n]).to("cuda:0")
+    gd, gi = idx.query(qd, qld)
+    torch.cuda.synchronize()
+    same = bool(np.array_equal(gi.cpu().numpy().astype(np.uint64), oi) and
+                np.array_equal(gd.cpu().numpy().view(np.uint32), od.view(np.uint32)))
+    log(f"CPU Super-Grid restatement: {out}; equals the GPU plain result: {same}")
+    return {"value": out["all_cores"]["qps"], "unit": "queries/s", "cores": out["all_cores"]["threads"],
+            "single_core_qps": out["one_core"]["qps"], "kind": "port",
+            "sample": f"first {n} queries, plain path (sub_lists {sub_lists}), reference batches of {batch}",
+            "equals_gpu_plain": same}
+
+
+_ALL_THREADS = []
+
+
+def all_threads_of(O):
+    if not _ALL_THREADS:
+        _ALL_THREADS.append(O.omp_threads())
+    return _ALL_THREADS[0]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +518,15 @@ def main():
     ap.add_argument("--budget-ratio", type=float, default=2 ** 0.25, help="budget growth per level")
     ap.add_argument("--learned-cap", type=int, default=0,
                     help="sub-cluster choice the learned budgets trim (0: twice the fixed budget, at least 256)")
+    ap.add_argument("--no-small", action="store_true", help="skip the 100k x 128 one-label leg")
+    ap.add_argument("--no-full", action="store_true", help="skip the 50M x 96 four-label leg")
+    ap.add_argument("--full-n", type=int, default=50_000_000)
+    ap.add_argument("--full-nq", type=int, default=100_000)
+    ap.add_argument("--full-gt", type=int, default=2000, help="FP64 ground-truth sample of the 50M leg")
+    ap.add_argument("--full-train", type=int, default=4000, help="budget-model training queries of the 50M leg")
+    ap.add_argument("--full-steps", type=int, default=3)
+    ap.add_argument("--cpu-sg-sample", type=int, default=200,
+                    help="queries of the CPU Super-Grid restatement timing (one and all cores)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
@@ -256,6 +541,8 @@ def main():
     from oracle import oracle as O
 
     dev = torch.device("cuda:0")
+    all_threads = all_threads_of(O)
+    cpu_sg = None
     if args.calibrate:
         buf = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)
         buf.fill_(1.0)
@@ -282,10 +569,16 @@ def main():
     _, gt_ids = O.bruteforce(w.data, w.labels, w.queries[:gs], ranges[:gs], 100)
     log(f"ground truth for {gs} queries in {time.time() - t0:.1f}s")
 
+    last = {}
+
     def recall_now():
-        d, i = idx.query(q_d, ql_d)
+        d, i = idx.query(q_d[:gs], ql_d[:gs])
         torch.cuda.synchronize()
-        return datasets.recall(gt_ids, i[:gs].cpu().numpy().astype(np.uint64), 100)
+        last["d"], last["i"] = d.cpu().numpy(), i.cpu().numpy()
+        return datasets.recall(gt_ids, last["i"].astype(np.uint64), 100)
+
+    def derr():
+        return dist_error(w.data, w.queries[:gs], last["i"], last["d"])
 
     od = torch.empty((args.nq, 100), dtype=torch.float32, device=dev)
     oi = torch.empty((args.nq, 100), dtype=torch.int64, device=dev)
@@ -329,17 +622,20 @@ def main():
     if "plain" in paths:
         idx.set_param("PRECISION", 0)
         b, r = sweep(128, 2)
+        err = derr()
         qps, ms, stage, st = timed("plain")
-        res["plain"] = dict(qps=qps, ms=ms, recall=r, sub_lists=b, stage=stage, st=st)
+        res["plain"] = dict(qps=qps, ms=ms, recall=r, sub_lists=b, stage=stage, st=st, err=err)
+        cpu_sg = cpu_supergrid(args, idx, w, O, b, log)
     # ---- mixed, fixed budget ----
     if "mixed" in paths or "learned" in paths:
         idx.set_param("PRECISION", 1)
         idx.set_param("PROBE_CLUSTERS", 0)
         b, r = sweep(64, 2 ** 0.5)
+        err = derr()
         qps, ms, stage, st = timed("mixed")
-        res["mixed"] = dict(qps=qps, ms=ms, recall=r, sub_lists=b, stage=stage, st=st)
+        res["mixed"] = dict(qps=qps, ms=ms, recall=r, sub_lists=b, stage=stage, st=st, err=err)
         qps, ms, _, _ = timed("mixed+graph", graph=1)
-        res["mixed_graph"] = dict(qps=qps, ms=ms, recall=r, sub_lists=b, stage=stage, st=st)
+        res["mixed_graph"] = dict(qps=qps, ms=ms, recall=r, sub_lists=b, stage=stage, st=st, err=err)
     # ---- learned budget + mixed ----
     if "learned" in paths:
         cap = args.learned_cap or max(res["mixed"]["sub_lists"] * 2, 256)
@@ -362,19 +658,26 @@ def main():
             if args.save_budget:
                 idx.budget_save(args.save_budget)
         idx.set_param("BUDGET_MODEL", 1)
-        r = recall_now()
-        log(f"  learned budget: recall@100={r:.4f}")
+        r0 = recall_now()
+        log(f"  learned budget: recall@100={r0:.4f}")
         _, _, stage, st = timed("learned+mixed (stage times)")
-        # batched form: launch knobs autotuned on a separate sample of the same distribution (results are
-        # unchanged by them), then every query chunk replayed as one captured HIP graph
+        # batched form: launch knobs autotuned on a separate sample of the same distribution (they do not change
+        # results), then every query chunk replayed as one captured HIP graph; the recall is measured again in
+        # exactly that configuration
         aq, aql = w.sample_queries(args.nq, 0xA7)
         tune = idx.autotune(torch.from_numpy(aq).to(dev), torch.from_numpy(aql).to(dev), reps=3)
         log("autotune: " + tune.strip().splitlines()[-1])
-        qps, ms, _, _ = timed("learned+mixed+graph", graph=1)
         tuned = {k: idx.get_param(k) for k in ("ADC_PER", "ADC_WGS", "SPARSE_MAX_TESTS", "QUERY_CHUNK",
-                                               "SCORE_BLOCK", "GRAPH")}
+                                               "SCORE_BLOCK", "ADC_SPLIT")}
+        tuned["GRAPH"] = 1
+        idx.set_param("GRAPH", 1)
+        r = recall_now()
+        err = derr()
+        idx.set_param("GRAPH", 0)
+        log(f"  learned budget, timed configuration: recall@100={r:.4f} (before autotuning {r0:.4f})")
+        qps, ms, _, _ = timed("learned+mixed+graph", graph=1)
         res["learned"] = dict(qps=qps, ms=ms, recall=r, sub_lists=cap, stage=stage, st=st, report=rep,
-                              tuned=tuned)
+                              tuned=tuned, err=err, recall_before_tune=r0)
         idx.set_param("BUDGET_MODEL", 0)
 
     head = "learned" if "learned" in res else ("mixed" if "mixed" in res else "plain")
@@ -399,12 +702,19 @@ def main():
     del idx, q_d, ql_d
     torch.cuda.empty_cache()
 
+    legs = {}
+    if not args.no_small:
+        legs["plain_100k"] = plain_small_leg(args, H, datasets, O, dev, log)
+        torch.cuda.empty_cache()
     batched = None
     if not args.no_batched:
         batched = batched_leg(args, H, datasets, O, dev, log)
+    if not args.no_full:
+        legs["full_50m"] = full_leg(args, H, datasets, O, dev, log)
 
     cs = min(args.cpu_sample, args.nq)
-    threads = O.omp_threads()
+    threads = all_threads
+    O.set_threads(threads)
     t0 = time.perf_counter()
     O.bruteforce(w.data, w.labels, w.queries[:cs], ranges[:cs], 100, threads=threads)
     cpu_s = time.perf_counter() - t0
@@ -414,9 +724,10 @@ def main():
     def summary(k):
         v = res[k]
         out = {"qps": round(v["qps"], 1), "ms_per_step": round(v["ms"], 4), "recall@100": round(v["recall"], 4),
-               "sub_lists": v["sub_lists"]}
+               "sub_lists": v["sub_lists"], "dist_err_vs_fp64": v.get("err")}
         if "tuned" in v:
             out["tuned"] = v["tuned"]
+            out["recall@100 before autotuning"] = round(v["recall_before_tune"], 4)
         return out
 
     config = {"workload": f"{w.name}: {args.n} x {d_} f32 L2-normalised 512-component mixture, 3 label dims "
@@ -435,6 +746,8 @@ def main():
     if "learned" in res and res["learned"]["report"]:
         config["budget_model"] = {k: round(v, 4) for k, v in res["learned"]["report"].items()}
         config["budget_model"]["train_queries"] = args.train
+    if legs:
+        config["legs"] = legs
 
     out = {
         "metric": METRIC, "value": round(H_["qps"], 2), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
@@ -442,6 +755,8 @@ def main():
         "vs_baseline": None, "dtype": "bf16/f16 scoring, f32 re-rank" if head != "plain" else "f32",
         "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu,
     }
+    if cpu_sg:
+        out["cpu_supergrid"] = cpu_sg
     if batched:
         out["adc_roofline"] = batched.pop("adc_roofline")
         out["batched"] = batched

@@ -106,6 +106,14 @@ const char* orc_last_error() { return g_err.c_str(); }
 
 void orc_plant_fault(int f) { g_plant = f; }
 
+// OpenMP threads of the parallel loops that follow (orc_query over queries, orc_bruteforce over query tiles).
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#endif
+  (void)n;
+}
+
 // =============================================================================================
 // A1: filter recovery -- preprocessing.cu:3-30 (denormalize_labels_kernel), :206-230.
 // qlabels is the flat [nq][qcols] array; types 0..2 read element t*l+i and type 3 reads elements
@@ -919,40 +927,51 @@ int orc_bruteforce(const float* data, const float* labels, uint64_t N, uint64_t 
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
 #endif
+  // queries in tiles of kTile: one sweep over the labels serves the whole tile (each query still visits the
+  // points in ascending order, so its heap and result are those of a per-query sweep)
+  constexpr uint64_t kTile = 16;
+  const int64_t tiles = (int64_t)((nq + kTile - 1) / kTile);
 #pragma omp parallel for schedule(dynamic, 1)
-  for (int64_t qi = 0; qi < (int64_t)nq; ++qi) {
-    const float* q = queries + qi * d;
-    const float* rg = ranges + qi * 2 * l;
-    std::priority_queue<std::pair<double, uint64_t>> heap;
+  for (int64_t ti = 0; ti < tiles; ++ti) {
+    const uint64_t q0 = (uint64_t)ti * kTile, qn = std::min<uint64_t>(kTile, nq - q0);
+    std::vector<std::priority_queue<std::pair<double, uint64_t>>> heaps(qn);
     for (uint64_t p = 0; p < N; ++p) {
-      bool ok = true;
-      for (int i = 0; i < l && ok; ++i) {
-        float lab = labels[p * l + i];
-        if (lab < rg[2 * i] || lab > rg[2 * i + 1]) ok = false;
-      }
-      if (!ok) continue;
-      double s = 0.0;
-      const float* x = data + p * d;
-      for (uint64_t t = 0; t < d; ++t) {
-        double df = (double)x[t] - (double)q[t];
-        s += df * df;
-      }
-      if (heap.size() < k) {
-        heap.push({s, p});
-      } else if (std::make_pair(s, p) < heap.top()) {
-        heap.pop();
-        heap.push({s, p});
+      const float* lp = labels + p * l;
+      for (uint64_t j = 0; j < qn; ++j) {
+        const float* rg = ranges + (q0 + j) * 2 * l;
+        bool ok = true;
+        for (int i = 0; i < l && ok; ++i)
+          if (lp[i] < rg[2 * i] || lp[i] > rg[2 * i + 1]) ok = false;
+        if (!ok) continue;
+        const float* q = queries + (q0 + j) * d;
+        double s = 0.0;
+        const float* x = data + p * d;
+        for (uint64_t t = 0; t < d; ++t) {
+          double df = (double)x[t] - (double)q[t];
+          s += df * df;
+        }
+        auto& heap = heaps[j];
+        if (heap.size() < k) {
+          heap.push({s, p});
+        } else if (std::make_pair(s, p) < heap.top()) {
+          heap.pop();
+          heap.push({s, p});
+        }
       }
     }
-    uint64_t n = heap.size();
-    for (int64_t j = (int64_t)k - 1; j >= 0; --j) {
-      if ((uint64_t)j < n) {
-        out_dist[qi * k + j] = heap.top().first;
-        out_ids[qi * k + j] = heap.top().second;
-        heap.pop();
-      } else {
-        out_dist[qi * k + j] = INFINITY;
-        out_ids[qi * k + j] = kU64Max;
+    for (uint64_t j = 0; j < qn; ++j) {
+      const uint64_t qi = q0 + j;
+      auto& heap = heaps[j];
+      uint64_t n = heap.size();
+      for (int64_t r = (int64_t)k - 1; r >= 0; --r) {
+        if ((uint64_t)r < n) {
+          out_dist[qi * k + r] = heap.top().first;
+          out_ids[qi * k + r] = heap.top().second;
+          heap.pop();
+        } else {
+          out_dist[qi * k + r] = INFINITY;
+          out_ids[qi * k + r] = kU64Max;
+        }
       }
     }
   }

@@ -195,6 +195,11 @@ def plant_fault(f: int) -> None:
     _lib.orc_plant_fault(ctypes.c_int(f))
 
 
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's parallel loops from now on."""
+    _lib.orc_set_threads(ctypes.c_int(int(n)))
+
+
 def omp_threads():
     return int(_lib.orc_omp_threads())
 
@@ -243,9 +248,16 @@ def load_state(directory: str) -> dict:
     return st
 
 
-def query(state: dict, data, labels, queries, qlabels, *, n_list, sub_lists, exps0, topk, ftype, shift_val,
-          imap, div_value, batch=0, trace=False):
-    """Run the restated Super-Grid query path on an index state (original-order data and labels)."""
+class PreparedState:
+    """An index state laid out for orc_query (the reference's layouts: PQ codes [M][N] by original id), with the
+    arrays it points to kept alive."""
+
+    def __init__(self, s, keep, sub_lists, topk):
+        self.s, self.keep, self.sub_lists, self.topk = s, keep, sub_lists, topk
+
+
+def prepare(state: dict, data, labels, *, n_list, sub_lists, exps0, topk, ftype, shift_val, imap, div_value):
+    """State preparation of `query` (no query work): returns a PreparedState for run_query."""
     meta = state["meta"]
     N, d, C, CS = meta["N"], meta["d"], meta["C"], meta["CS"]
     M, K, pq_len, l = meta["M"], meta["K"], meta["pq_len"], meta["l"]
@@ -287,6 +299,12 @@ def query(state: dict, data, labels, queries, qlabels, *, n_list, sub_lists, exp
     s.zids = arr(zid, np.uint64)
     s.pq_centers = arr(state["pq_centers"], np.float32)
     s.codes = arr(codes_ref, np.uint8)
+    return PreparedState(s, keep, sub_lists, topk)
+
+
+def run_query(ps: PreparedState, queries, qlabels, batch=0, trace=False):
+    """orc_query on a prepared state."""
+    s, sub_lists, topk = ps.s, ps.sub_lists, ps.topk
     queries = _c(queries, np.float32)
     qlabels = _c(qlabels, np.float32)
     nq = queries.shape[0]
@@ -308,3 +326,11 @@ def query(state: dict, data, labels, queries, qlabels, *, n_list, sub_lists, exp
     if st:
         raise ValueError(_err())
     return (od, oi, tr_arrays) if trace else (od, oi)
+
+
+def query(state: dict, data, labels, queries, qlabels, *, n_list, sub_lists, exps0, topk, ftype, shift_val,
+          imap, div_value, batch=0, trace=False):
+    """Run the restated Super-Grid query path on an index state (original-order data and labels)."""
+    ps = prepare(state, data, labels, n_list=n_list, sub_lists=sub_lists, exps0=exps0, topk=topk, ftype=ftype,
+                 shift_val=shift_val, imap=imap, div_value=div_value)
+    return run_query(ps, queries, qlabels, batch=batch, trace=trace)
