@@ -58,6 +58,8 @@ struct Config {
   uint64_t score_block = 65536;        // SCORE_BLOCK: queries per BF16 scoring block (PRECISION 1; min 128)
   uint64_t adc_split = 65536;          // ADC_SPLIT: ADC candidates from which a query's top-k' selection runs
                                        // over many workgroups (0 = never; same results)
+  uint64_t inbox_cap = 4096;           // INBOX_CAP: in-box points the bucket-grid walk lists per query (512..2^20);
+                                       // under PRECISION 1 a walked query whose list fits refines from it
 };
 
 // Parse with config_utils.h:33-88 semantics: `KEY = value` per line, key case-insensitive and

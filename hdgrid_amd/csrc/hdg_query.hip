@@ -206,7 +206,6 @@ struct Buckets {
 };
 
 constexpr uint64_t kMaxCombos = 4096;
-constexpr uint32_t kInboxCap = 4096;  // in-box points listed per walked query
 constexpr uint32_t kSparseMaxCS = 8192;  // LDS histogram: (CS + 2 CS/32) words per workgroup
 
 template <int NV>
@@ -2993,7 +2992,7 @@ uint64_t Index::config_digest() const {
   const uint64_t v[] = {cfg_.n_list, cfg_.sub_lists, cfg_.topk, cfg_.exps0, cfg_.precision, cfg_.probe_clusters,
                         cfg_.budget_model, cfg_.sparse_count, cfg_.sparse_max_tests, cfg_.adc_per, cfg_.adc_wgs,
                         cfg_.parity_batch, cfg_.query_chunk, cfg_.label_mask, cfg_.select_wave, cfg_.score_block,
-                        cfg_.adc_split, graph_epoch_};
+                        cfg_.adc_split, cfg_.inbox_cap, graph_epoch_};
   uint64_t h = 1469598103934665603ull;
   for (uint64_t x : v) {
     h ^= x;
@@ -3050,8 +3049,10 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.n_dense.reserve(32);
   w.redo.reserve(qn * 4);
   w.counts.reserve(qn * CS * 4);
-  w.ib_pos.reserve(qn * kInboxCap * 4);
-  w.ib_sub.reserve(qn * kInboxCap * 4);
+  if (cfg_.inbox_cap < 512 || cfg_.inbox_cap > (1u << 20)) fail(kUnsupported, "INBOX_CAP must be in [512, 2^20]");
+  const uint32_t ibcap = (uint32_t)cfg_.inbox_cap;
+  w.ib_pos.reserve(qn * ibcap * 4);
+  w.ib_sub.reserve(qn * ibcap * 4);
   w.n_ib.reserve(qn * 4);
   w.sub_list.reserve(qn * CS * 4);
   w.sub_cnt.reserve(qn * CS * 4);
@@ -3200,7 +3201,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
         hipLaunchKernelGGL(sparse_count_wave_kernel<NV>, dim3(wgrid), dim3(256), 0, s, B, S.bstart.as<uint32_t>(), \
                            S.ecell.as<uint32_t>(), S.esub.as<uint32_t>(), S.N, S.sub_off.as<uint32_t>(),           \
                            (uint32_t)CS, (uint32_t)C, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), qn, G, q0,  \
-                           max_tests, S.epos.as<uint32_t>(), kInboxCap, w.ib_pos.as<uint32_t>(),                   \
+                           max_tests, S.epos.as<uint32_t>(), ibcap, w.ib_pos.as<uint32_t>(),                   \
                            w.ib_sub.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.sub_list.as<uint32_t>(),              \
                            w.sub_cnt.as<uint32_t>(), w.ns.as<uint32_t>(), w.list_c.as<uint32_t>(),                 \
                            w.cl_beg.as<uint32_t>(), w.nnz_c.as<uint32_t>(), w.dense_list.as<uint32_t>(),           \
@@ -3211,7 +3212,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                          S.bstart.as<uint32_t>(),                                                                  \
                          S.ecell.as<uint32_t>(), S.esub.as<uint32_t>(), S.N, S.sub_off.as<uint32_t>(),             \
                          (uint32_t)CS, (uint32_t)C, w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), qn, G, q0,    \
-                         max_tests, S.epos.as<uint32_t>(), kInboxCap, w.ib_pos.as<uint32_t>(),           \
+                         max_tests, S.epos.as<uint32_t>(), ibcap, w.ib_pos.as<uint32_t>(),           \
                          w.ib_sub.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.sub_list.as<uint32_t>(), w.sub_cnt.as<uint32_t>(), w.ns.as<uint32_t>(),                 \
                          w.list_c.as<uint32_t>(), w.cl_beg.as<uint32_t>(), w.nnz_c.as<uint32_t>(),                 \
                          w.dense_list.as<uint32_t>(), w.n_dense.as<uint32_t>(), w.gstat.as<unsigned long long>(), \
@@ -3377,13 +3378,13 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   if (sparse && cfg_.select_wave)
     hipLaunchKernelGGL(inbox_keep_wave_kernel, dim3((unsigned)cdiv(qn, 4)), dim3(256), 0, s, qn, l, integral,
                        (uint32_t)CS, sub_lists, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
-                       w.zlo_cnt.as<uint64_t>(), w.n_chosen_s.as<uint32_t>(), csid, kInboxCap,
+                       w.zlo_cnt.as<uint64_t>(), w.n_chosen_s.as<uint32_t>(), csid, ibcap,
                        w.ib_pos.as<uint32_t>(), w.ib_sub.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(),
                        w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
   else if (sparse)
     hipLaunchKernelGGL(inbox_keep_kernel, dim3((unsigned)qn), dim3(256), 0, s, l, integral, (uint32_t)CS, sub_lists,
                        w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(), w.zlo_cnt.as<uint64_t>(),
-                       w.n_chosen_s.as<uint32_t>(), csid, kInboxCap, w.ib_pos.as<uint32_t>(),
+                       w.n_chosen_s.as<uint32_t>(), csid, ibcap, w.ib_pos.as<uint32_t>(),
                        w.ib_sub.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
                        w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
   if (!all_walked)

@@ -212,7 +212,7 @@ def full_pipeline(n: int = 50_000_000, nq: int = 100_000, d: int = 96, component
     """50M x 96 f32 mixture (4096 components, means N(0,1), sigma 0.3); four label dims: timestamp
     U[0, 31,536,000) with shift [604800, 0]; length U[0, 2000] with shift [30, 30]; category 1..5 with
     interval_map [1,2,1,2,3,4,3,5,4,5]; u U[0, 1e6] with div 10 (range [(int)x / 10, FLT_MAX]).
-    MEM_BOUND 200e9, n_list 512, sub_lists 128, PQ_DIM 24 (4 dims per subvector).
+    MEM_BOUND 200e9, n_list 512, sub_lists 128, PQ_DIM 24 (4 dims per subvector), INBOX_CAP 65536.
     device: draw the base vectors and labels there with a seeded torch generator (the host copies are
     still returned; data_dev / labels_dev keep the device tensors)."""
     if device is not None:
@@ -246,8 +246,11 @@ def full_pipeline(n: int = 50_000_000, nq: int = 100_000, d: int = 96, component
     qrng = np.random.default_rng(seed + 1)
     queries = gaussian_mixture(qrng, nq, d, means, 0.3)
     qlabels = labs(qrng, nq)
+    # in-box lists of up to 65536 points: a query's box holds ~40k points over all sub-clusters, so PRECISION 1
+    # refines it from that list instead of its ~200k Morton-range candidates (INBOX_CAP changes nothing under
+    # PRECISION 0)
     conf = dict(BASE_CONF, PQ_DIM=max(1, d // 4), N_LIST=n_list, SUB_LISTS=sub_lists, CLUSTERS=clusters,
-                SUB_CLUSTERS=sub_clusters, MEM_BOUND=200_000_000_000)
+                SUB_CLUSTERS=sub_clusters, MEM_BOUND=200_000_000_000, INBOX_CAP=65536)
     filt = ("l = 4\nfilter0:\n\ttype = 0\n\tshift_val = [604800, 0]\n\n"
             "filter1:\n\ttype = 0\n\tshift_val = [30, 30]\n\n"
             "filter2:\n\ttype = 1\n\tinterval_map = [1, 2, 1, 2, 3, 4, 3, 5, 4, 5]\n\n"
