@@ -231,7 +231,7 @@ def full_leg(args, H, datasets, O, dev, log):
     out["mixed"] = {"qps": round(nq / el, 1), "ms_per_call": round(el * 1e3, 3), "recall@100": round(r, 4),
                     "sub_lists": b, "mean_candidates": round(float(st.cand.mean()), 1),
                     "mean_refined": round(float(st.refined.mean()), 1),
-                    "share_refined_from_in_box_lists": round(float((st.adc == 0).mean()), 4),
+                    "share_refined_from_in_box_lists": round(float((st.cand == 0).mean()), 4),
                     "dist_err_vs_fp64": dist_error(w.data, w.queries[:gs], i, d),
                     f"stages_ms ({n1} queries, one chunk)": {k: round(v, 3) for k, v in
                                                              sorted(stage_m.items(), key=lambda kv: -kv[1])}}
