@@ -581,8 +581,36 @@ void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* lab
       std::memcpy(&h_zcodes[p * M], &h_codes[(uint64_t)st_.h_zid[p] * M], M);
     upload(st_.codes, h_zcodes, s);
   }
+  prepare_blocked_codes(st_, s);
   HDG_HIP(hipStreamSynchronize(s));
   st_.built = true;
+}
+
+namespace {
+// One thread per row: its sixteen-byte words and its eight-byte tail move to their places in the 64-row block.
+__global__ void block_codes_kernel(const uint8_t* __restrict__ codes, uint64_t n, uint64_t M,
+                                   uint8_t* __restrict__ out) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint64_t n16 = M / 16;
+  const uint8_t* row = codes + p * M;  // rows are 8-byte aligned (M % 8 == 0)
+  uint8_t* blk = out + (p >> 6) * 64 * M;
+  for (uint64_t w = 0; w < n16; ++w) {
+    const uint2 a = reinterpret_cast<const uint2*>(row)[2 * w], b = reinterpret_cast<const uint2*>(row)[2 * w + 1];
+    reinterpret_cast<uint4*>(blk + w * 1024)[p & 63] = make_uint4(a.x, a.y, b.x, b.y);
+  }
+  if (M & 15) reinterpret_cast<uint2*>(blk + n16 * 1024)[p & 63] = reinterpret_cast<const uint2*>(row)[2 * n16];
+}
+}  // namespace
+
+void prepare_blocked_codes(IndexState& st, hipStream_t s) {
+  st.codes_wb = (st.M % 8) == 0 ? 16u : 0u;
+  if (!st.codes_wb) return;
+  const uint64_t rows = (st.N + 63) / 64 * 64;
+  st.codes_blk.reserve(rows * st.M);
+  hipLaunchKernelGGL(block_codes_kernel, dim3((unsigned)((st.N + 255) / 256)), dim3(256), 0, s,
+                     st.codes.as<uint8_t>(), st.N, st.M, st.codes_blk.as<uint8_t>());
+  HDG_HIP(hipGetLastError());
 }
 
 }  // namespace hdg

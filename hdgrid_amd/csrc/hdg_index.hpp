@@ -36,6 +36,12 @@ struct IndexState {
   DevBuf zcode;       // u64 [N]
   DevBuf zid;         // u32 [N] original row id
   DevBuf codes;       // u8  [N][M]
+  // the codes again, rows in blocks of 64 (block b at byte b * 64 * M): first the M / 16 sixteen-byte words of
+  // the 64 rows (word w of row p at w * 1024 + (p % 64) * 16), then, when M % 16 == 8, their last eight codes
+  // (at (M / 16) * 1024 + (p % 64) * 8); so 64 consecutive rows' word is one contiguous piece for the streaming
+  // ADC scan.  codes_wb = 16 when M % 8 == 0 (the layout exists), else 0.
+  DevBuf codes_blk;
+  uint32_t codes_wb = 0;
   DevBuf pq_centers;  // f32 [M][K][pq_len]
   DevBuf centers;     // f32 [C][d]
   DevBuf sub_centers; // f32 [CS][d] cluster-major
@@ -86,6 +92,9 @@ HostCheckpoint read_checkpoint(const std::string& dir, int l_expected);
 
 // Bucket grid from the z-pool-ordered cells ([n_valid][N]) and sub-cluster offsets; sets the bucket fields.
 void build_bucket_grid(IndexState& st, const std::vector<uint32_t>& h_cells, hipStream_t s);
+
+// Fills codes_blk / codes_wb from the row-layout codes (device).
+void prepare_blocked_codes(IndexState& st, hipStream_t s);
 
 // Fills the bf16 center copies, their norms and Dp from host centers [C][d] and sub-centers [CS][d].
 void prepare_mixed_centers(IndexState& st, const std::vector<float>& centers, const std::vector<float>& sub_centers,

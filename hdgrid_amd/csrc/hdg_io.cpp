@@ -320,6 +320,7 @@ void Index::load(const std::string& dir, hipStream_t s) {
   prepare_mixed_centers(S, H.centers, H.subc, s);
   upload(S.pq_centers, H.pqc, s);
   upload(S.codes, H.codes, s);
+  prepare_blocked_codes(S, s);
   upload(S.cells, H.cells, s);
   build_bucket_grid(S, H.cells, s);
   upload(S.vec, H.vec, s);

@@ -1474,12 +1474,12 @@ __device__ __forceinline__ uint64_t f2u_host_dev(float x) {
 // top-k' keeps them all); mode 1: ADC scan + top-k'.  lsize = entries of the refine list,
 // asize = ADC keys, chunks = adc_scan blocks.
 // A11 selection over many workgroups.  A mode-1 query with at least ADC_SPLIT ADC candidates gets a slot
-// (kSelSlots per chunk).  The scan records the range [mn, mx] of its distance bits; bins are those of
+// (kSelSlots per chunk: every query of the largest chunk).  The scan records the range [mn, mx] of its distance bits; bins are those of
 // dev::dist_bin_shift (monotone in the distance).  Workgroups of kSelTile candidates add their LDS histograms into the slot's global histogram,
 // then split the candidates into keys below the k'-th key's bin (final) and keys in that bin (scratch); the
 // per-query selection finally ranks the keys of that one bin.
 constexpr uint32_t kSelTile = 16384;
-constexpr uint32_t kSelSlots = 8192;
+constexpr uint32_t kSelSlots = 65536;
 constexpr uint32_t kSelBins = 2048;
 
 __global__ void plan_kernel(uint64_t qn, uint64_t q0, uint64_t G, int l, uint64_t exps0, uint64_t topk,
@@ -1947,37 +1947,48 @@ __global__ __launch_bounds__(256) void adc_scan_kernel(
         }
         if (c0 + threadIdx.x + (uint32_t)i0 * kWG >= c1) break;  // the rest of this thread's columns too
         if constexpr (STREAM == 8) {
-          // a code row of up to 64 codes as 8-byte words, all in flight together; the sum keeps the order over m
-          uint2 v[4][8];
+          // a code row of 8, 16 or 24 codes: one 16-byte word and/or one 8-byte tail from the 64-row blocked
+          // codes, all four rows' pieces in flight together; the sum keeps the order over m
+          uint4 a[4];
+          uint2 b[4];
+          const bool has16 = M >= 16, has8 = (M & 15) != 0;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const uint32_t col = c0 + threadIdx.x + (uint32_t)(i0 + u) * kWG;
-            const uint2* cd = reinterpret_cast<const uint2*>(codes + (uint64_t)p4[u] * M);
+            const uint64_t p = p4[u];
+            const uint8_t* blk = codes + (p >> 6) * 64 * M;
             const bool ok = col < c1;
-#pragma unroll
-            for (int s8 = 0; s8 < 8; ++s8) v[u][s8] = ok && (uint64_t)s8 * 8 < M ? cd[s8] : make_uint2(0, 0);
+            a[u] = ok && has16 ? reinterpret_cast<const uint4*>(blk)[p & 63] : make_uint4(0, 0, 0, 0);
+            b[u] = ok && has8 ? reinterpret_cast<const uint2*>(blk + (M / 16) * 1024)[p & 63] : make_uint2(0, 0);
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-#pragma unroll
-            for (int s8 = 0; s8 < 8; ++s8) {
-              if ((uint64_t)s8 * 8 >= M) break;
-              const LT* lr = lut + (uint64_t)s8 * 8 * K;
-              sum[u] = adc_word(sum[u], v[u][s8].x, lr, K);
-              sum[u] = adc_word(sum[u], v[u][s8].y, lr + 4 * K, K);
+            const LT* lr = lut;
+            if (has16) {
+              sum[u] = adc_word(sum[u], a[u].x, lr, K);
+              sum[u] = adc_word(sum[u], a[u].y, lr + 4 * K, K);
+              sum[u] = adc_word(sum[u], a[u].z, lr + 8 * K, K);
+              sum[u] = adc_word(sum[u], a[u].w, lr + 12 * K, K);
+              lr += 16 * K;
+            }
+            if (has8) {
+              sum[u] = adc_word(sum[u], b[u].x, lr, K);
+              sum[u] = adc_word(sum[u], b[u].y, lr + 4 * K, K);
             }
           }
         } else {
+        // codes in 64-row blocks of 16-byte words: word w of row p at block (p / 64), lane slot p % 64
 #pragma unroll 1
         for (uint64_t mb = 0; mb < M; mb += 32) {
           uint4 v[4][2];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const uint32_t col = c0 + threadIdx.x + (uint32_t)(i0 + u) * kWG;
-            const uint4* cd = reinterpret_cast<const uint4*>(codes + (uint64_t)p4[u] * M + mb);
+            const uint64_t p = p4[u];
+            const uint4* cd = reinterpret_cast<const uint4*>(codes + (p >> 6) * 64 * M) + (mb / 16) * 64 + (p & 63);
             const bool ok = col < c1;
             v[u][0] = ok ? cd[0] : make_uint4(0, 0, 0, 0);
-            v[u][1] = ok ? cd[1] : make_uint4(0, 0, 0, 0);
+            v[u][1] = ok ? cd[64] : make_uint4(0, 0, 0, 0);
           }
           const LT* lr0 = lut + mb * K;
 #pragma unroll
@@ -2693,6 +2704,16 @@ void* scratch_address(int device) {
   }
   return g_scratch_ptr[device];
 }
+// Workgroups of 256 threads of `kernel` (with `lds` bytes of dynamic LDS) resident on the current device at
+// once, capped at `want`: the grid of a persistent kernel whose work is split evenly over its workgroups.
+unsigned resident_grid(const void* kernel, size_t lds, unsigned want) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  HDG_HIP(hipGetDevice(&dev));
+  HDG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  HDG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds));
+  const uint64_t cap = (uint64_t)(per_cu > 0 ? per_cu : 1) * (uint64_t)(ncu > 0 ? ncu : 1);
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(want, cap));
+}
 }  // namespace
 
 int find_max_factor(uint64_t a, uint64_t b) {
@@ -3407,13 +3428,20 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     if (mp_h > 4) mp_h &= ~3ull;
     // streaming variants: the whole LUT in one phase; 32-code steps (16-byte loads) or, for code rows of at
     // most 64 codes with pq_dim % 8 == 0, 8-code steps (8-byte loads)
-    const int stream = mp_h < S.M || (mp_h % 16) != 0 ? 0
-                       : (S.M % 32) == 0                ? 32
-                       : ((S.M % 8) == 0 && S.M <= 64) ? 8
-                                                        : 0;
+    int stream = mp_h < S.M || (mp_h % 16) != 0 ? 0
+                 : (S.M % 32) == 0                ? 32
+                 : ((S.M % 8) == 0 && S.M <= 24) ? 8
+                                                  : 0;
+    // the streaming variants read the 64-row blocked codes
+    if (S.codes_wb != 16 || !S.codes_blk.data()) stream = 0;
+    const uint8_t* codes_arg = stream ? S.codes_blk.as<uint8_t>() : S.codes.as<uint8_t>();
+    // the persistent grid is at most what is resident at once, so every workgroup's share starts at launch
 #define HDG_ADC1(LT, PER, KC, ST)                                                                                 \
-  hipLaunchKernelGGL((adc_scan_kernel<LT, PER, KC, ST>), dim3(adc_grid), dim3(256), adc_lds, s, qn, q0, queries, d,  \
-                     S.M, S.K, S.pq_len, S.pq_centers.as<float>(), S.codes.as<uint8_t>(), sub_lists,                 \
+  hipLaunchKernelGGL((adc_scan_kernel<LT, PER, KC, ST>),                                                          \
+                     dim3(resident_grid(reinterpret_cast<const void*>(&adc_scan_kernel<LT, PER, KC, ST>), adc_lds,  \
+                                        adc_grid)),                                                               \
+                     dim3(256), adc_lds, s, qn, q0, queries, d,                                                    \
+                     S.M, S.K, S.pq_len, S.pq_centers.as<float>(), codes_arg, sub_lists,                              \
                      w.mode.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),                     \
                      w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.L.as<uint32_t>(), w.lsize.as<uint64_t>(),       \
                      w.nL.as<uint32_t>(), w.chunks.as<uint64_t>(), w.asize.as<uint64_t>(), w.adist.as<uint32_t>(),   \
