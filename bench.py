@@ -267,6 +267,18 @@ def full_leg(args, H, datasets, O, dev, log):
                           "budget_model": {k: round(v, 4) for k, v in rep.items()}}
         out["learned_vs_plain"] = round(out["learned"]["qps"] / out["plain"]["qps"], 3)
         idx.set_param("BUDGET_MODEL", 0)
+        # the fixed mixed budget at matched recall (smallest of the x sqrt(2) ladder reaching the learned recall)
+        b_m = 128.0
+        while True:
+            idx.set_param("SUB_LISTS", int(b_m))
+            r_m, _, _ = recall_now()
+            if r_m >= r or b_m >= 2048:
+                break
+            b_m *= 2 ** 0.5
+        el_m, _, _ = timed(graph=1)
+        out["mixed_at_learned_recall"] = {"sub_lists": int(b_m), "recall@100": round(r_m, 4),
+                                          "qps": round(nq / el_m, 1)}
+        out["learned_vs_mixed_at_matched_recall"] = round(el_m / el, 3)
         log(f"full learned+mixed+graph: recall {r:.4f}, {el * 1e3:.1f} ms/call ({nq / el:.0f} QPS), "
             f"{out['learned_vs_plain']}x plain")
     out["mixed_vs_plain"] = round(out["mixed"]["qps"] / out["plain"]["qps"], 3)
@@ -428,9 +440,20 @@ def batched_learned(args, idx, w, q, ql, od, oi, H, datasets, O, dev, log):
     el_l = graph_time()
     st_l = idx.last_stats(nq)
     idx.set_param("BUDGET_MODEL", 0)
+    # matched recall: the smallest fixed budget of the same ladder whose recall is at least the learned one
+    b = 64.0
+    while True:
+        idx.set_param("SUB_LISTS", int(b))
+        r_m = recall_at()
+        if r_m >= r_l or b >= 1024:
+            break
+        b *= 2 ** 0.25
+    b_m = int(b)
+    el_m = graph_time()
     idx.set_param("SUB_LISTS", 128)
     log(f"batched learned: recall@100 {r_l:.4f}, {el_l * 1e3:.2f} ms/call ({nq / el_l:.0f} QPS) vs fixed "
-        f"sub_lists {b_fix}: recall {r_fix:.4f}, {el_fix * 1e3:.2f} ms/call ({nq / el_fix:.0f} QPS)")
+        f"sub_lists {b_fix}: recall {r_fix:.4f}, {el_fix * 1e3:.2f} ms/call ({nq / el_fix:.0f} QPS); fixed at "
+        f"matched recall: sub_lists {b_m}, recall {r_m:.4f}, {el_m * 1e3:.2f} ms/call")
     return {"eval_sample": ne, "train_queries": args.batched_train,
             "fixed": {"sub_lists": b_fix, "recall@100": round(r_fix, 4), "qps": round(nq / el_fix, 1),
                       "ms_per_call": round(el_fix * 1e3, 3), "mean_refined": round(float(st_fix.refined.mean()), 1),
@@ -440,7 +463,10 @@ def batched_learned(args, idx, w, q, ql, od, oi, H, datasets, O, dev, log):
                                                                  "QUERY_CHUNK", "SCORE_BLOCK", "ADC_SPLIT")},
                         "ms_per_call": round(el_l * 1e3, 3), "mean_refined": round(float(st_l.refined.mean()), 1),
                         "mean_candidates": round(float(st_l.cand.mean()), 1)},
-            "learned_vs_fixed": round(el_fix / el_l, 3), "budget_model": rep}
+            "fixed_at_learned_recall": {"sub_lists": b_m, "recall@100": round(r_m, 4), "qps": round(nq / el_m, 1),
+                                        "ms_per_call": round(el_m * 1e3, 3)},
+            "learned_vs_fixed": round(el_fix / el_l, 3), "learned_vs_fixed_at_matched_recall": round(el_m / el_l, 3),
+            "budget_model": rep}
 
 
 def cpu_supergrid(args, idx, w, O, sub_lists, log):
@@ -680,6 +706,17 @@ def main():
         res["learned"] = dict(qps=qps, ms=ms, recall=r, sub_lists=cap, stage=stage, st=st, report=rep,
                               tuned=tuned, err=err, recall_before_tune=r0)
         idx.set_param("BUDGET_MODEL", 0)
+        # the fixed mixed budget at matched recall, with the same tuned knobs and graph replay
+        b_m = 64.0
+        while True:
+            idx.set_param("SUB_LISTS", int(b_m))
+            r_m = recall_now()
+            if r_m >= r or b_m >= 4096:
+                break
+            b_m *= 2 ** 0.25
+        qps_m, ms_m, _, _ = timed("mixed at the learned recall (graph)", graph=1)
+        res["learned"]["fixed_at_matched_recall"] = {"sub_lists": int(b_m), "recall@100": round(r_m, 4),
+                                                     "qps": round(qps_m, 1)}
 
     head = "learned" if "learned" in res else ("mixed" if "mixed" in res else "plain")
     H_ = res[head]
@@ -729,6 +766,9 @@ def main():
         if "tuned" in v:
             out["tuned"] = v["tuned"]
             out["recall@100 before autotuning"] = round(v["recall_before_tune"], 4)
+        if "fixed_at_matched_recall" in v:
+            out["fixed_mixed_at_matched_recall"] = v["fixed_at_matched_recall"]
+            out["vs_fixed_at_matched_recall"] = round(v["qps"] / v["fixed_at_matched_recall"]["qps"], 3)
         return out
 
     config = {"workload": f"{w.name}: {args.n} x {d_} f32 L2-normalised 512-component mixture, 3 label dims "

@@ -74,6 +74,7 @@ _budget_save = _sig("hdg_budget_save", ctypes.c_int, [_vp, _cp])
 _budget_load = _sig("hdg_budget_load", ctypes.c_int, [_vp, _cp, _vp])
 _budget_grads = _sig("hdg_budget_debug_grads", ctypes.c_int, [_fp, _fp, _fp, ctypes.c_uint32, _fp, _fp])
 _budget_nparams = _sig("hdg_budget_param_count", ctypes.c_int, [])
+_budget_feats = _sig("hdg_budget_debug_features", ctypes.c_int, [_vp, _vp, _u64, _u64, _vp, _u64, _fp, _vp])
 _coarse_scores = _sig("hdg_debug_coarse_scores", ctypes.c_int, [_vp, _vp, _u64, _fp, _vp])
 _merge = _sig("hdg_merge_topk", ctypes.c_int, [_vp, _vp, _u64, _u64, _u64, _u64, _u64, _u64, _vp, _vp, _vp])
 _fit_coeff = _sig("hdg_fit_mem_coeff", ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)])
@@ -300,6 +301,17 @@ class SuperGridIndex:
         keys = ["final_loss", "margin", "holdout_recall", "holdout_level", "mean_label", "fixed_recall", "n_train",
                 "n_holdout"]
         return {k: float(rep[i]) for i, k in enumerate(keys)}
+
+    def budget_features(self, queries, query_labels, stream=None):
+        """Raw budget-model input rows [n][48] the PRECISION 1 pipeline computes for these queries."""
+        import numpy as np
+        queries = queries.contiguous()
+        query_labels = query_labels.contiguous()
+        out = np.zeros((queries.shape[0], BUDGET_FEATURES), np.float32)
+        qcols = query_labels.shape[1] if query_labels.dim() == 2 else 1
+        _check(_budget_feats(self._h, _ptr(queries), queries.shape[0], queries.shape[1], _ptr(query_labels), qcols,
+                             out.ctypes.data_as(_fp), _stream_ptr(stream)))
+        return out
 
     def budget_save(self, path: str) -> None:
         _check(_budget_save(self._h, path.encode()))

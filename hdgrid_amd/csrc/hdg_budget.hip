@@ -1,14 +1,15 @@
 // Learned per-query probe budget for PRECISION 1 (BUDGET_MODEL = 1).
 //
-// Budget ladder: level i probes T[i] clusters and keeps S[i] sub-clusters (both grow geometrically).
-// A query's label is the smallest level whose results reach the target recall@k against the ground
-// truth supplied by the caller (the traces come from running this pipeline at every level).
+// Budget ladder: level i keeps S[i] sub-clusters (growing geometrically) of all non-zero clusters.  A training
+// query's label is the smallest level at which the labelling path (BUDGET_LABELS 0: the plain reference-exact
+// path; 1: the mixed path) reaches the target recall@k against the ground truth supplied by the caller.
 // Model: 48 features -> 64 ReLU -> 64 ReLU -> 1 (predicted level).  Features: log1p of the filter width of
-// each label dim (8), log1p of the grid-cell extent of each valid dim (8), the 32 smallest coarse
-// distances from the BF16 coarse scores (32); standardised with training mean / std.
+// each label dim (8), log1p of the grid-cell extent of each valid dim (8), the 32 smallest BF16 coarse-centroid
+// scores among the query's non-zero clusters, plus |q|^2 (32); standardised with training mean / std.
 // Training: BF16 operands (inputs, weights and activations rounded to bf16), FP32 accumulation, FP32
-// master weights, Adam, MSE loss.  Inference adds a margin calibrated on a held-out split; a non-finite
-// prediction falls back to the fixed budget (the highest level).
+// master weights, Adam, MSE loss.  Inference adds a margin: the smallest one at which the deployed path (mixed
+// precision with the model) reaches the target mean recall on a held-out share of the training queries; a
+// non-finite prediction falls back to the fixed budget (the highest level).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -374,10 +375,13 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
     B.ladT[i] = P.t0 > 0 ? (uint32_t)std::min<double>(n_list, std::max(1.0, std::round(P.t0 * f))) : n_list;
     B.ladS[i] = (uint32_t)std::min<double>(sub_lists, std::max(1.0, std::round(P.s0 * f)));
   }
-  // traces: recall of every training query at every level, features from one extra pass
+  // traces: recall of every training query at every level of the labelling path (BUDGET_LABELS 0: the plain
+  // reference-exact path, the minimal plain budget of each query; 1: the mixed path), then one mixed pass at the
+  // top level that writes the feature rows
   const Config saved = cfg_;
-  cfg_.precision = 1;
+  cfg_.precision = cfg_.budget_labels ? 1 : 0;
   cfg_.budget_model = 0;
+  cfg_.graph = 0;
   DevBuf d_dist, d_ids, d_feat;
   d_dist.reserve(n * k * 4);
   d_ids.reserve(n * k * 8);
@@ -388,15 +392,15 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
     for (int i = 0; i < P.levels; ++i) {
       cfg_.probe_clusters = P.t0 > 0 ? B.ladT[i] : 0;
       cfg_.sub_lists = B.ladS[i];
-      if (i == P.levels - 1) {
-        B.sink = d_feat.as<float>();
-        B.sink_rows = n;
-      }
       query(queries, n, d, qlabels, qcols, d_dist.as<float>(), d_ids.as<uint64_t>(), s);
-      B.sink = nullptr;
       HDG_HIP(hipMemcpy(ids.data(), d_ids.data(), n * k * 8, hipMemcpyDeviceToHost));
       rec[i] = recall_per_query(ids, gt_host, n, k);
     }
+    cfg_.precision = 1;
+    B.sink = d_feat.as<float>();
+    B.sink_rows = n;
+    query(queries, n, d, qlabels, qcols, d_dist.as<float>(), d_ids.as<uint64_t>(), s);
+    B.sink = nullptr;
   } catch (...) {
     cfg_ = saved;
     B.sink = nullptr;
@@ -497,7 +501,7 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
   }
   HDG_HIP(hipMemcpy(B.master.data(), dMaster.data(), kParams * 4, hipMemcpyDeviceToHost));
   HDG_HIP(hipMemcpy(wq.data(), dWq.data(), kParams * 4, hipMemcpyDeviceToHost));
-  // predictions for all samples; calibrate the margin on the holdout
+  // predictions for all samples (mean level of the holdout at the chosen margin)
   DevBuf dPred;
   dPred.reserve(n * 4);
   hipLaunchKernelGGL(predict_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, dX.as<float>(), n,
@@ -510,40 +514,95 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
     float t = std::ceil(yhat + margin);
     return t < 0 ? 0 : (t > P.levels - 1 ? P.levels - 1 : (int)t);
   };
-  float chosen = P.max_margin;
-  double hold_rec = 0.0, hold_lev = 0.0;
-  // smallest margin (negative ones included) at which the holdout's mean recall reaches the target: the
-  // per-query labels each meet the target on their own, so the mean can afford to undershoot some
-  for (float m = -P.max_margin; m <= P.max_margin + 1e-6f; m += 0.125f) {
-    double rsum = 0, lsum = 0;
-    for (uint64_t t = n_train; t < n; ++t) {
-      const uint32_t q = perm[t];
-      const int lev = level_of(pred[q], m);
-      rsum += rec[lev][q];
-      lsum += lev;
-    }
-    hold_rec = rsum / n_hold;
-    hold_lev = lsum / n_hold;
-    if (hold_rec >= P.target) {
-      chosen = m;
-      break;
-    }
-  }
-  B.margin = chosen;
-  double mean_label = 0, fixed_rec = 0;
-  for (uint64_t q = 0; q < n; ++q) {
-    mean_label += y[q];
-    fixed_rec += rec[P.levels - 1][q];
-  }
-  B.report = BudgetReport{last_loss, B.margin, (float)hold_rec, (float)hold_lev, (float)(mean_label / n),
-                          (float)(fixed_rec / n), (uint64_t)n_train, (uint64_t)n_hold};
   upload(B.d_w, wq, s);
   upload(B.d_mean, B.mean, s);
   upload(B.d_inv, B.inv_std, s);
   upload(B.d_ladT, B.ladT, s);
   upload(B.d_ladS, B.ladS, s);
   drop_graphs();  // captured graphs hold the previous model's margin and level count
+  // margin: the smallest one (negative ones included, steps of 1/8 level) at which the deployed path -- mixed
+  // precision with this model's budgets -- reaches the target mean recall on the holdout queries, run end to
+  // end against the caller's ground truth
+  const uint64_t n_hold_rows = n - n_train;
+  std::vector<float> hq(n_hold_rows * d), hql(n_hold_rows * qcols), allq(n * d), allql(n * qcols);
+  HDG_HIP(hipMemcpy(allq.data(), queries, n * d * 4, hipMemcpyDeviceToHost));
+  HDG_HIP(hipMemcpy(allql.data(), qlabels, n * qcols * 4, hipMemcpyDeviceToHost));
+  std::vector<uint64_t> hgt(n_hold_rows * k);
+  for (uint64_t t = 0; t < n_hold_rows; ++t) {
+    const uint64_t q = perm[n_train + t];
+    std::memcpy(&hq[t * d], &allq[q * d], d * 4);
+    std::memcpy(&hql[t * qcols], &allql[q * qcols], qcols * 4);
+    std::memcpy(&hgt[t * k], &gt_host[q * k], k * 8);
+  }
+  DevBuf dhq, dhql;
+  upload(dhq, hq, s);
+  upload(dhql, hql, s);
+  std::vector<uint64_t> hids(n_hold_rows * k);
+  float chosen = P.max_margin;
+  double hold_rec = 0.0, hold_lev = 0.0;
+  cfg_.precision = 1;
+  cfg_.budget_model = 1;
+  cfg_.graph = 0;
+  cfg_.probe_clusters = 0;
   B.ready = true;
+  try {
+    for (float m = -P.max_margin; m <= P.max_margin + 1e-6f; m += 0.125f) {
+      B.margin = m;
+      query(dhq.as<float>(), n_hold_rows, d, dhql.as<float>(), qcols, d_dist.as<float>(), d_ids.as<uint64_t>(), s);
+      HDG_HIP(hipMemcpy(hids.data(), d_ids.data(), n_hold_rows * k * 8, hipMemcpyDeviceToHost));
+      const std::vector<float> r = recall_per_query(hids, hgt.data(), n_hold_rows, k);
+      double rsum = 0;
+      for (float v : r) rsum += v;
+      hold_rec = rsum / n_hold_rows;
+      chosen = m;
+      if (hold_rec >= P.target) break;
+    }
+  } catch (...) {
+    cfg_ = saved;
+    B.ready = false;
+    throw;
+  }
+  cfg_ = saved;
+  B.margin = chosen;
+  for (uint64_t t = n_train; t < n; ++t) hold_lev += level_of(pred[perm[t]], chosen);
+  hold_lev /= (double)n_hold_rows;
+  double mean_label = 0, fixed_rec = 0;
+  for (uint64_t q = 0; q < n; ++q) {
+    mean_label += y[q];
+    fixed_rec += rec[P.levels - 1][q];
+  }
+  B.report = BudgetReport{last_loss, B.margin, (float)hold_rec, (float)hold_lev, (float)(mean_label / n),
+                          (float)(fixed_rec / n), (uint64_t)n_train, (uint64_t)n_hold_rows};
+  drop_graphs();
+  B.ready = true;
+}
+
+void Index::budget_debug_features(const float* queries, uint64_t n, uint64_t d, const float* qlabels,
+                                  uint64_t qcols, float* out_host, hipStream_t s) {
+  HDG_HIP(hipSetDevice(device_));
+  if (!st_.built) fail(kBadState, "feature capture before build");
+  if (!budget_) budget_ = new BudgetModel();
+  BudgetModel& B = *budget_;
+  DevBuf d_feat, d_dist, d_ids;
+  d_feat.reserve(n * kF * 4);
+  d_dist.reserve(n * cfg_.topk * 4);
+  d_ids.reserve(n * cfg_.topk * 8);
+  const Config saved = cfg_;
+  cfg_.precision = 1;
+  cfg_.budget_model = 0;
+  cfg_.graph = 0;
+  B.sink = d_feat.as<float>();
+  B.sink_rows = n;
+  try {
+    query(queries, n, d, qlabels, qcols, d_dist.as<float>(), d_ids.as<uint64_t>(), s);
+  } catch (...) {
+    B.sink = nullptr;
+    cfg_ = saved;
+    throw;
+  }
+  B.sink = nullptr;
+  cfg_ = saved;
+  HDG_HIP(hipMemcpy(out_host, d_feat.data(), n * kF * 4, hipMemcpyDeviceToHost));
 }
 
 const BudgetReport& Index::budget_report() const {

@@ -374,6 +374,14 @@ hdg_status hdg_budget_debug_grads(const float* params, const float* X, const flo
 
 int hdg_budget_param_count(void) { return hdg::budget_param_count(); }
 
+hdg_status hdg_budget_debug_features(hdg_index* index, const float* queries, uint64_t n, uint64_t dim,
+                                     const float* query_labels, uint64_t qlabel_cols, float* out, void* stream) {
+  return guarded([&] {
+    if (!index || !queries || !query_labels || !out) hdg::fail(HDG_EINVAL, "null argument");
+    index->impl->budget_debug_features(queries, n, dim, query_labels, qlabel_cols, out, (hipStream_t)stream);
+  });
+}
+
 hdg_status hdg_debug_coarse_scores(hdg_index* index, const float* queries, uint64_t nq, float* out, void* stream) {
   return guarded([&] {
     if (!index || !queries || !out) hdg::fail(HDG_EINVAL, "null argument");

@@ -60,6 +60,8 @@ struct Config {
                                        // over many workgroups (0 = never; same results)
   uint64_t inbox_cap = 4096;           // INBOX_CAP: in-box points the bucket-grid walk lists per query (512..2^20);
                                        // under PRECISION 1 a walked query whose list fits refines from it
+  uint64_t budget_labels = 0;          // BUDGET_LABELS: path whose traces label the budget model's training
+                                       // queries: 0 = plain (PRECISION 0), 1 = mixed (PRECISION 1)
 };
 
 // Parse with config_utils.h:33-88 semantics: `KEY = value` per line, key case-insensitive and

@@ -154,6 +154,9 @@ class Index {
   void budget_fit(const float* queries, uint64_t n, uint64_t d, const float* qlabels, uint64_t qcols,
                   const uint64_t* gt_host, uint64_t k, const BudgetFitParams& p, hipStream_t s);
   const BudgetReport& budget_report() const;
+  // The raw (unstandardised) feature rows [n][48] the PRECISION 1 pipeline computes for these queries.
+  void budget_debug_features(const float* queries, uint64_t n, uint64_t d, const float* qlabels, uint64_t qcols,
+                             float* out_host, hipStream_t s);
   void budget_save(const std::string& path) const;
   void budget_load(const std::string& path, hipStream_t s);
   void budget_free();

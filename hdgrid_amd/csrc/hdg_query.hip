@@ -3271,9 +3271,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     HDG_HIP(hipGetLastError());
   }
   mark();
-  // learned budget (PRECISION 1): the sub-cluster scoring writes the features (its 32 smallest scores and
-  // |q|^2); the model then trims each query's sorted sub-cluster choice to its budget
-  const bool need_feat = mixed && (budget_T || budget_capturing());
+  // learned budget (PRECISION 1): the cluster scoring has written the features (the 32 smallest coarse scores
+  // and |q|^2); after the sub-cluster selection the model trims each query's sorted choice to its budget
   auto budgets = [&]() {
     capture_features(q0, qn, l, w.ranges.as<float>(), w.box_w.as<uint32_t>(), g.n_valid, w.feat.as<float>(), s);
     if (budget_T)
@@ -3283,6 +3282,9 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   const uint32_t* Ls = all_clusters ? w.sub_list.as<uint32_t>() : w.list_s.as<uint32_t>();
   const uint32_t* Lcnt = all_clusters ? w.sub_cnt.as<uint32_t>() : w.list_scnt.as<uint32_t>();
   const uint32_t* Lnnz = all_clusters ? w.ns.as<uint32_t>() : w.nnz_s.as<uint32_t>();
+  // learned budget (PRECISION 1): features of each query -- filter widths, grid-cell extents and the 32 smallest
+  // BF16 coarse scores among its non-zero clusters (+ |q|^2), written by the cluster selection
+  const bool need_feat = mixed && (budget_T || budget_capturing());
   // ---- A5: cluster choice ----
   if (!mixed) {
     scan_u64(w.woff.as<uint64_t>(), nullptr, nullptr, qn, w.nnz_c.as<uint32_t>(), nullptr, 0u);
@@ -3301,9 +3303,24 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                          S.center_norm.as<float>(), (uint32_t)C, w.S.as<float>(), s, nullptr, nullptr, 0, abort_p);
       launch_scored_select(queries, q0 + b0, bn, d, w.list_c.as<uint32_t>() + b0 * C, (uint32_t)C,
                            w.nnz_c.as<uint32_t>() + b0, w.S.as<float>(), C, kc, nullptr, n_list,
-                           w.chosen_c.as<uint32_t>() + b0 * n_list, w.n_chosen_c.as<uint32_t>() + b0, nullptr,
+                           w.chosen_c.as<uint32_t>() + b0 * n_list, w.n_chosen_c.as<uint32_t>() + b0,
+                           need_feat ? w.feat.as<float>() + b0 * kFeatDists : nullptr,
                            w.keys.as<uint64_t>() + b0 * C, cfg_.select_wave != 0, s, nullptr, nullptr, 0, abort_w,
                            w.n_dense.as<uint32_t>() + 3, (skip & 4u) != 0);
+    }
+  } else if (need_feat) {
+    // every non-zero cluster is probed, so nothing is chosen here, but the budget features still come from the
+    // coarse scores of the query's non-zero clusters
+    for (uint64_t b0 = 0; b0 < qn; b0 += score_block) {
+      const uint64_t bn = std::min(score_block, qn - b0);
+      launch_coarse_gemm(w.qb16.as<uint16_t>() + b0 * S.Dp, bn, S.Dp, S.centers_bf16.as<uint16_t>(),
+                         S.center_norm.as<float>(), (uint32_t)C, w.S.as<float>(), s, nullptr, nullptr, 0, abort_p);
+      launch_scored_select(queries, q0 + b0, bn, d, w.list_c.as<uint32_t>() + b0 * C, (uint32_t)C,
+                           w.nnz_c.as<uint32_t>() + b0, w.S.as<float>(), C, 1, nullptr, n_list,
+                           w.chosen_c.as<uint32_t>() + b0 * n_list, w.n_chosen_c.as<uint32_t>() + b0,
+                           w.feat.as<float>() + b0 * kFeatDists, w.keys.as<uint64_t>() + b0 * C,
+                           cfg_.select_wave != 0, s, nullptr, nullptr, 0, abort_w, w.n_dense.as<uint32_t>() + 3,
+                           (skip & 4u) != 0);
     }
   }
   HDG_HIP(hipGetLastError());
@@ -3333,7 +3350,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                          nzp + b0 * nws, nws, abort_p);
       launch_scored_select(queries, q0 + b0, bn, d, Ls + b0 * cap, cap, Lnnz + b0, w.S.as<float>(), CS, sub_lists,
                            nullptr, sub_lists, w.chosen_s.as<uint32_t>() + b0 * sub_lists,
-                           w.n_chosen_s.as<uint32_t>() + b0, need_feat ? w.feat.as<float>() + b0 * kFeatDists : nullptr,
+                           w.n_chosen_s.as<uint32_t>() + b0, nullptr,
                            w.keys.as<uint64_t>() + b0 * cap, cfg_.select_wave != 0, s, nzb + b0 * nws,
                            nzp + b0 * nws, nws, abort_w, w.n_dense.as<uint32_t>() + 3, (skip & 4u) != 0,
                            all_clusters);

@@ -136,11 +136,13 @@ hdg_status hdg_last_stage_ms(const hdg_index* index, char* names, uint64_t cap, 
  * smallest budget level whose top-k reaches `target` recall against gt_ids (host [n][TOPK] u64, invalid
  * ids >= 2^63), running this pipeline at every level, then trains the 48-64-64-1 MLP on the device.
  * Ladder: level i keeps round(s0*ratio^i) sub-clusters (capped by SUB_LISTS) of all non-zero clusters; t0
- * must be 0 (cluster budgets are not modelled: HDG_EUNSUPPORTED otherwise).  Features per query: filter
- * widths, grid-cell extents and the 32 smallest BF16 sub-center scores among its non-zero sub-clusters
- * (+ |q|^2); at query time the budget trims the score-sorted sub-cluster choice.  report receives 8 floats:
- * final loss, margin, holdout recall, holdout mean level, mean label, recall of the top level, n_train,
- * n_holdout. */
+ * must be 0 (cluster budgets are not modelled: HDG_EUNSUPPORTED otherwise).  Labels come from the plain path
+ * (BUDGET_LABELS = 0, the default) or the mixed one (1).  Features per query: filter widths, grid-cell extents
+ * and the 32 smallest BF16 coarse-centroid scores among its non-zero clusters (+ |q|^2); at query time the
+ * budget trims the score-sorted sub-cluster choice.  The margin added to the prediction is the smallest one at
+ * which the deployed path (PRECISION 1 with the model) reaches `target` mean recall on the held-out 10% of the
+ * training queries.  report receives 8 floats: final loss, margin, holdout recall, holdout mean level, mean
+ * label, recall of the top level, n_train, n_holdout. */
 hdg_status hdg_budget_fit(hdg_index* index, const float* queries, uint64_t n, uint64_t dim,
                           const float* query_labels, uint64_t qlabel_cols, const uint64_t* gt_ids, uint64_t k,
                           int levels, double t0, double s0, double ratio, float target, int epochs, float lr,
@@ -153,6 +155,12 @@ hdg_status hdg_budget_load(hdg_index* index, const char* path, void* stream);
 hdg_status hdg_budget_debug_grads(const float* params, const float* X, const float* Y, uint32_t n, float* grads,
                                   float* loss);
 int hdg_budget_param_count(void);
+/* The budget model's raw (unstandardised) input rows for n queries (device arrays, as hdg_query), written to
+ * out [n][48] (host): log1p of each label dim's filter width (8), log1p(extent + 1) of each valid grid dim's cell
+ * box (8), the 32 smallest BF16 coarse scores |c|^2 - 2<q,c> among the query's non-zero clusters plus |q|^2 (32).
+ * Runs the PRECISION 1 pipeline once.  For checking the feature and inference kernels. */
+hdg_status hdg_budget_debug_features(hdg_index* index, const float* queries, uint64_t n, uint64_t dim,
+                                     const float* query_labels, uint64_t qlabel_cols, float* out, void* stream);
 
 /* Coarse scores of the PRECISION 1 path for nq queries (device array): out [nq][CLUSTERS] host f32 with
  * out[q][c] = |c|^2 - 2 <q, c> computed by the BF16 MFMA kernel.  For checking that kernel. */
