@@ -48,10 +48,11 @@ void check_algorithm(const char* algorithm) {
   if (a != "super-grid") hdg::fail(HDG_ECONFIG, "Unknown algorithm: " + std::string(algorithm));
 }
 
-// IS_SPLIT = 1 selects the reference's Morton split mode (zorder.cu:474-654), which this library does
-// not implement; refusing it at creation keeps a drop-in HDGrid.conf from silently running the no-split path.
+// IS_SPLIT = 1 selects the Morton split mode (zorder.cu:474-654) with N_SPLIT pieces per range; the range table
+// of a query holds SUB_LISTS * N_SPLIT pieces.
 void check_supported(const hdg::Config& cfg) {
-  if (cfg.is_split) hdg::fail(HDG_EUNSUPPORTED, "IS_SPLIT = 1 (Morton split mode) is not supported");
+  if (cfg.is_split && (cfg.n_split == 0 || cfg.n_split * cfg.sub_lists > 8000))
+    hdg::fail(HDG_EUNSUPPORTED, "IS_SPLIT = 1 needs N_SPLIT >= 1 and SUB_LISTS * N_SPLIT <= 8000");
 }
 
 void check_device(int device) {

@@ -1309,7 +1309,7 @@ __global__ __launch_bounds__(256) void ranges_kernel(
     const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zhi, uint32_t* __restrict__ chosen_sid,
     uint32_t* __restrict__ rstart, uint32_t* __restrict__ rpre, uint64_t* __restrict__ cand,
     uint64_t* __restrict__ real, unsigned long long* __restrict__ gstat, const uint32_t* __restrict__ n_ib,
-    int inbox_mode,
+    int inbox_mode, int split,
     const uint32_t* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
   __shared__ uint32_t tmp[8];
@@ -1441,8 +1441,11 @@ __global__ __launch_bounds__(256) void ranges_kernel(
     uint32_t total;
     uint32_t pos = dev::wg_exclusive_scan(cnt, tmp, total);
     if (j < sub_lists) {
-      rstart[qi * sub_lists + j] = start;
-      rpre[qi * (sub_lists + 1) + j] = base_sh + pos;
+      // split mode: the range table is written by split_ranges_kernel
+      if (!split) {
+        rstart[qi * sub_lists + j] = start;
+        rpre[qi * (sub_lists + 1) + j] = base_sh + pos;
+      }
       chosen_sid[qi * sub_lists + j] = sid;
     }
     __syncthreads();
@@ -1450,13 +1453,155 @@ __global__ __launch_bounds__(256) void ranges_kernel(
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    rpre[qi * (sub_lists + 1) + sub_lists] = base_sh;
-    cand[qi] = base_sh;
-    real[qi] = real_sh;
     const uint64_t g = (q0 + qi) / G;
-    atomicAdd(&gstat[g * 8 + 3], (unsigned long long)base_sh);
+    real[qi] = real_sh;
     atomicAdd(&gstat[g * 8 + 4], real_sh);
-    atomicMax(&gstat[g * 8 + 5], (unsigned long long)base_sh);
+    if (!split) {
+      rpre[qi * (sub_lists + 1) + sub_lists] = base_sh;
+      cand[qi] = base_sh;
+      atomicAdd(&gstat[g * 8 + 3], (unsigned long long)base_sh);
+      atomicMax(&gstat[g * 8 + 5], (unsigned long long)base_sh);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// A8 in Morton split mode (is_split = 1, zorder.cu:474-654): each chosen sub-cluster's Morton range is cut into
+// N_SPLIT near-equal shares (get_query_end_points_kernel, :591-647); each share starts at its first point and
+// jumps at most three times to the next in-box code (BIGMIN) until it reaches an in-box point (the share then
+// starts there) or no in-box code is left before its last code (the share is dropped: the reference compares
+// `next >= last code`).  The candidate sub-cluster is the query's own (the reference's kernel reads candidates[c],
+// H23).  The query's range table holds sub_lists * N_SPLIT pieces, chosen sub-cluster major; n_rng = pieces in use.
+// BIGMIN here is a bit scan over the interleaved code (Tropf and Herzog): the smallest code above `cur` whose
+// point lies in the box [zmin, zmax]; the reference's step computes the same code for every cur below the largest.
+__device__ __forceinline__ bool bigmin_next(uint64_t cur, uint64_t zmin, uint64_t zmax, int nb, int a,
+                                            const uint64_t* dmask, uint64_t& out) {
+  if (cur == ~0ull || (nb < 64 && cur + 1 >= (1ull << nb))) return false;  // no code above the largest one
+  const uint64_t z = cur + 1;
+  uint64_t best = 0;
+  bool have = false;
+  for (int b = nb - 1; b >= 0; --b) {
+    const uint64_t bit = 1ull << b;
+    const bool zb = (z >> b) & 1ull, lb = (zmin >> b) & 1ull, hb = (zmax >> b) & 1ull;
+    const uint64_t m = dmask[(nb - 1 - b) % a] & ((bit << 1) - 1ull);  // this dim's bits at or below b
+    if (!zb && !lb && hb) {
+      best = (zmin & ~m) | bit;            // lowest code of the box's upper half in this dim
+      have = true;
+      zmax = (zmax & ~m) | (m & ~bit);     // the box's lower half in this dim
+    } else if (!zb && lb && hb) {
+      out = zmin;
+      return true;
+    } else if (zb && !lb && !hb) {
+      out = best;
+      return have;
+    } else if (zb && !lb && hb) {
+      zmin = (zmin & ~m) | bit;            // the box's upper half in this dim
+    }
+  }
+  out = z;  // z itself lies in the box
+  return true;
+}
+
+__global__ __launch_bounds__(256) void split_ranges_kernel(
+    const uint64_t* __restrict__ zcode, const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ cells,
+    uint64_t N, int n_valid, int max_bits, const uint32_t* __restrict__ box_lo, const uint32_t* __restrict__ box_w,
+    uint64_t qn, const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zhi,
+    const uint64_t* __restrict__ zlo_cnt, uint32_t sub_lists, uint32_t n_split,
+    const uint32_t* __restrict__ chosen_sid, const uint32_t* __restrict__ n_chosen_s,
+    const uint32_t* __restrict__ n_ib, int inbox_mode, uint32_t* __restrict__ rstart, uint32_t* __restrict__ rpre,
+    uint32_t* __restrict__ n_rng, uint64_t* __restrict__ cand, unsigned long long* __restrict__ gstat, uint64_t G,
+    uint64_t q0, uint32_t* __restrict__ err, const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
+  __shared__ uint32_t tmp[8];
+  const uint64_t qi = blockIdx.x;
+  if (inbox_mode && n_ib[qi] != kNone) {  // refined from its in-box list (mode 3)
+    if (threadIdx.x == 0) n_rng[qi] = 0;
+    return;
+  }
+  const uint32_t R = sub_lists * n_split;
+  const uint32_t items = n_chosen_s[qi] * n_split;
+  const uint64_t za = zlo[qi], zb = zhi[qi];
+  // a lower corner beyond the label range wrapped: the reference's step then compares mismatched boxes
+  if (threadIdx.x == 0 && za != zlo_cnt[qi]) atomicOr(err, 2u);
+  const int nb = n_valid * max_bits;
+  uint64_t dmask[kMaxDims];
+  for (int j = 0; j < n_valid; ++j) {
+    uint64_t m = 0;
+    for (int p = 0; p < nb; ++p)
+      if ((nb - 1 - p) % n_valid == j) m |= 1ull << p;
+    dmask[j] = m;
+  }
+  uint32_t blo[kMaxDims], bw[kMaxDims];
+  for (int j = 0; j < n_valid; ++j) {
+    blo[j] = box_lo[(uint64_t)j * qn + qi];
+    bw[j] = box_w[(uint64_t)j * qn + qi];
+  }
+  uint32_t base = 0;
+  for (uint32_t t0 = 0; t0 < R; t0 += kWG) {
+    const uint32_t t = t0 + threadIdx.x;
+    uint32_t start = 0, len = 0;
+    if (t < items) {
+      const uint32_t j = t / n_split, sp = t - j * n_split;
+      const uint32_t sid = chosen_sid[qi * sub_lists + j];
+      const uint32_t p0 = seg_off[sid], p1 = seg_off[sid + 1];
+      uint32_t lo = p0, hi = p1, lo2 = p0, hi2 = p1;
+      while (lo < hi || lo2 < hi2) {
+        const uint32_t m1 = (lo + hi) >> 1, m2 = (lo2 + hi2) >> 1;
+        const uint64_t z1 = lo < hi ? zcode[m1] : 0, z2 = lo2 < hi2 ? zcode[m2] : 0;
+        if (lo < hi) {
+          if (z1 < za) lo = m1 + 1; else hi = m1;
+        }
+        if (lo2 < hi2) {
+          if (z2 < zb + 1) lo2 = m2 + 1; else hi2 = m2;
+        }
+      }
+      const uint32_t L = lo2 > lo ? lo2 - lo : 0u;
+      const uint32_t ns = L < n_split ? L : n_split;
+      if (sp < ns) {
+        const uint32_t qt = L / ns, r = L - qt * ns;
+        uint32_t a = lo + sp * qt + (sp < r ? sp : r);
+        const uint32_t b = a + qt + (sp < r ? 1u : 0u);
+        const uint64_t zlast = zcode[b - 1];
+        uint64_t cur = zcode[a];
+        bool valid = true;
+        for (int jumps = 0; jumps < 3;) {
+          uint32_t x = a, y = b;  // lower_bound(cur) in [a, b)
+          while (x < y) {
+            const uint32_t mid = (x + y) >> 1;
+            if (zcode[mid] < cur) x = mid + 1; else y = mid;
+          }
+          if (x >= b) break;
+          a = x;
+          bool in = true;
+          for (int jd = 0; jd < n_valid; ++jd) in = in && (cells[(uint64_t)jd * N + x] - blo[jd] <= bw[jd]);
+          ++jumps;
+          if (in) break;
+          uint64_t nz = 0;
+          if (!bigmin_next(zcode[x], za, zb, nb, n_valid, dmask, nz) || nz >= zlast) {
+            valid = false;
+            break;
+          }
+          cur = nz;
+        }
+        start = a;
+        len = valid ? b - a : 0u;
+      }
+    }
+    uint32_t total;
+    const uint32_t pos = dev::wg_exclusive_scan(len, tmp, total);
+    if (t < R) {
+      rstart[qi * R + t] = start;
+      rpre[qi * (R + 1) + t] = base + pos;
+    }
+    base += total;
+  }
+  if (threadIdx.x == 0) {
+    rpre[qi * (R + 1) + R] = base;
+    cand[qi] = base;
+    n_rng[qi] = items;
+    const uint64_t g = (q0 + qi) / G;
+    atomicAdd(&gstat[g * 8 + 3], (unsigned long long)base);
+    atomicMax(&gstat[g * 8 + 5], (unsigned long long)base);
   }
 }
 
@@ -1595,7 +1740,9 @@ __device__ __forceinline__ bool label_pass(const float* __restrict__ lab, int l,
 __device__ __forceinline__ bool inbox_keeps(uint64_t qi, int l, int integral, const uint32_t* __restrict__ mode,
                                             const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo,
                                             const uint64_t* __restrict__ zlo_cnt) {
-  return mode[qi] == 3 || (mode[qi] == 0 && n_ib[qi] != kNone && zlo[qi] == zlo_cnt[qi] && (l == 1 || integral));
+  // integral < 0 (Morton split mode): the keep-all candidates are the split pieces, so they are never replaced
+  return mode[qi] == 3 ||
+         (mode[qi] == 0 && integral >= 0 && n_ib[qi] != kNone && zlo[qi] == zlo_cnt[qi] && (l == 1 || integral));
 }
 
 __global__ __launch_bounds__(256) void keep_all_kernel(
@@ -2659,6 +2806,7 @@ struct Index::Work {
   DevBuf sslot, smm, scnt, ghist;  // many-workgroup selection: slots, distance ranges, split counts, histograms
   DevBuf S, feat, budget_T, budget_S, qb16;  // PRECISION 1: scores, model features, budgets, bf16 queries
   DevBuf nzb, nzp;                            // PRECISION 1: non-zero sub-cluster bit words and their ranks
+  DevBuf nrng;                         // Morton split mode: range-table pieces in use per query
   DevBuf st, st_chosen;                // call-wide counters [nq][6] and chosen sub-cluster ids [nq][sub_lists]
   uint64_t st_nq = 0, st_sub_lists = 0;
   // GRAPH 1: per-chunk captured graphs, replayed on an internal stream
@@ -2824,7 +2972,8 @@ uint64_t Index::check_query(const float* queries, uint64_t nq, uint64_t d, const
   if (d != st_.d) fail(kInvalidArgument, "query dimension differs from the index");
   if (nq == 0) return 0;
   if (!queries || !qlabels || !out_dist || !out_ids) fail(kInvalidArgument, "null query/output pointer");
-  if (cfg_.is_split) fail(kUnsupported, "IS_SPLIT = 1 (Morton split mode) is not supported");
+  if (cfg_.is_split && (cfg_.n_split == 0 || cfg_.n_split * cfg_.sub_lists > 8000))
+    fail(kUnsupported, "IS_SPLIT = 1 needs N_SPLIT >= 1 and SUB_LISTS * N_SPLIT <= 8000");
   const int l = st_.l;
   if (qcols < (uint64_t)l) fail(kInvalidArgument, "query label columns < l");
   for (int i = 0; i < l; ++i)
@@ -2951,6 +3100,8 @@ void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* ql
       HDG_HIP(hipMemcpyAsync(&w.pinned[5], w.err.data(), 4, hipMemcpyDeviceToHost, w.gstream));
       HDG_HIP(hipStreamSynchronize(w.gstream));
       if ((uint32_t)w.pinned[5] & 1u) fail(kInvalidArgument, "map filter label outside interval_map");
+      if ((uint32_t)w.pinned[5] & 2u)
+        fail(kUnsupported, "Morton split mode: a query box whose lower corner lies beyond the label range");
       if (((uint32_t)w.pinned[4] & 1u) == 0) continue;
     }
     // uncaptured run: results for this call, buffer sizes and the fallback kernels needed for the capture
@@ -3013,7 +3164,7 @@ uint64_t Index::config_digest() const {
   const uint64_t v[] = {cfg_.n_list, cfg_.sub_lists, cfg_.topk, cfg_.exps0, cfg_.precision, cfg_.probe_clusters,
                         cfg_.budget_model, cfg_.sparse_count, cfg_.sparse_max_tests, cfg_.adc_per, cfg_.adc_wgs,
                         cfg_.parity_batch, cfg_.query_chunk, cfg_.label_mask, cfg_.select_wave, cfg_.score_block,
-                        cfg_.adc_split, cfg_.inbox_cap, graph_epoch_};
+                        cfg_.adc_split, cfg_.inbox_cap, cfg_.is_split, cfg_.n_split, graph_epoch_};
   uint64_t h = 1469598103934665603ull;
   for (uint64_t x : v) {
     h ^= x;
@@ -3091,8 +3242,13 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.keys.reserve(qn * std::max<uint64_t>(C, cap) * 8);
   w.woff.reserve((qn + 1) * 8);
   w.gstat.reserve((ng + g_base + 1) * 8 * 8);
-  w.rstart.reserve(qn * sub_lists * 4);
-  w.rpre.reserve(qn * (sub_lists + 1) * 4);
+  // Morton split mode: each chosen sub-cluster's range becomes N_SPLIT pieces of the query's range table
+  const uint32_t n_split = cfg_.is_split ? (uint32_t)cfg_.n_split : 0u;
+  const uint32_t R = n_split ? sub_lists * n_split : sub_lists;
+  w.rstart.reserve(qn * R * 4);
+  w.rpre.reserve(qn * (R + 1) * 4);
+  w.nrng.reserve(qn * 4);
+  const uint32_t* nr_arg = n_split ? w.nrng.as<uint32_t>() : w.n_chosen_s.as<uint32_t>();
   w.cand.reserve(qn * 8);
   w.real.reserve(qn * 8);
   w.mode.reserve(qn * 4);
@@ -3375,8 +3531,18 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      w.zhi.as<uint64_t>(), csid, w.rstart.as<uint32_t>(),
                      w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.real.as<uint64_t>(),
                      w.gstat.as<unsigned long long>(), w.n_ib.as<uint32_t>(),
-                     inbox_mode && cfg_.select_wave ? 2 : inbox_mode, abort_p);
+                     inbox_mode && cfg_.select_wave ? 2 : inbox_mode, n_split ? 1 : 0, abort_p);
   HDG_HIP(hipGetLastError());
+  if (n_split && !all_walked) {
+    hipLaunchKernelGGL(split_ranges_kernel, dim3((unsigned)qn), dim3(256), 0, s, S.zcode.as<uint64_t>(),
+                       S.seg_off.as<uint32_t>(), S.cells.as<uint32_t>(), S.N, g.n_valid, g.max_bits,
+                       w.box_lo.as<uint32_t>(), w.box_w.as<uint32_t>(), qn, w.zlo.as<uint64_t>(),
+                       w.zhi.as<uint64_t>(), w.zlo_cnt.as<uint64_t>(), sub_lists, n_split, csid,
+                       w.n_chosen_s.as<uint32_t>(), w.n_ib.as<uint32_t>(), inbox_mode, w.rstart.as<uint32_t>(),
+                       w.rpre.as<uint32_t>(), w.nrng.as<uint32_t>(), w.cand.as<uint64_t>(),
+                       w.gstat.as<unsigned long long>(), G, q0, w.err.as<uint32_t>(), abort_p);
+    HDG_HIP(hipGetLastError());
+  }
   mark();
   hipLaunchKernelGGL(plan_kernel, dim3((unsigned)cdiv(qn, 64)), dim3(64), 0, s, qn, q0, G, l, cfg_.exps0,
                      topk, w.cand.as<uint64_t>(), w.gstat.as<unsigned long long>(), w.mode.as<uint32_t>(),
@@ -3398,6 +3564,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     HDG_HIP(hipStreamSynchronize(s));
     mark();
     if ((uint32_t)w.pinned[3] & 1u) fail(kInvalidArgument, "map filter label outside interval_map");
+    if ((uint32_t)w.pinned[3] & 2u)
+      fail(kUnsupported, "Morton split mode: a query box whose lower corner lies beyond the label range");
     totL = w.pinned[0];
     totA = w.pinned[1];
     totChunks = w.pinned[2];
@@ -3412,7 +3580,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.Lkeys.reserve(std::max<uint64_t>(totL, 1) * 8);
   w.adist.reserve(std::max<uint64_t>(totA, 1) * 4);
   w.akeys2.reserve(std::max<uint64_t>(totA, 1) * 8);
-  const int integral = S.labels_integral ? 1 : 0;
+  // split mode keeps the keep-all candidates its pieces give (no in-box replacement)
+  const int integral = n_split ? -1 : (S.labels_integral ? 1 : 0);
   if (sparse && cfg_.select_wave)
     hipLaunchKernelGGL(inbox_keep_wave_kernel, dim3((unsigned)cdiv(qn, 4)), dim3(256), 0, s, qn, l, integral,
                        (uint32_t)CS, sub_lists, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
@@ -3426,9 +3595,9 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                        w.ib_sub.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
                        w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
   if (!all_walked)
-  hipLaunchKernelGGL(keep_all_kernel, dim3((unsigned)qn), dim3(256), (2 * sub_lists + 1) * 4, s, qn, l, integral,
-                     sub_lists, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
-                     w.zlo_cnt.as<uint64_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),
+  hipLaunchKernelGGL(keep_all_kernel, dim3((unsigned)qn), dim3(256), (2 * R + 1) * 4, s, qn, l, integral,
+                     R, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
+                     w.zlo_cnt.as<uint64_t>(), nr_arg, w.rstart.as<uint32_t>(),
                      w.rpre.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
                      w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
   HDG_HIP(hipGetLastError());
@@ -3438,7 +3607,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     const uint64_t lut_entry = mixed ? 2 : 4;
     const uint32_t lut_bytes =
         (uint32_t)((std::min<uint64_t>(kLutPhaseBytes, S.M * S.K * lut_entry) + 15) & ~15ull);
-    const size_t adc_lds = lut_bytes + (sub_lists <= kAdcLdsRanges ? (2 * (size_t)sub_lists + 1) * 4 : 0);
+    const size_t adc_lds = lut_bytes + (R <= kAdcLdsRanges ? (2 * (size_t)R + 1) * 4 : 0);
     const unsigned adc_grid = (unsigned)(w.capturing ? adc_wgs : std::min<uint64_t>(totChunks, adc_wgs));
     // the streaming variant needs the whole LUT in one LDS phase and 32-code steps
     uint64_t mp_h = kLutPhaseBytes / (S.K * lut_entry);
@@ -3458,8 +3627,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
                      dim3(resident_grid(reinterpret_cast<const void*>(&adc_scan_kernel<LT, PER, KC, ST>), adc_lds,  \
                                         adc_grid)),                                                               \
                      dim3(256), adc_lds, s, qn, q0, queries, d,                                                    \
-                     S.M, S.K, S.pq_len, S.pq_centers.as<float>(), codes_arg, sub_lists,                              \
-                     w.mode.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),                     \
+                     S.M, S.K, S.pq_len, S.pq_centers.as<float>(), codes_arg, R,                                      \
+                     w.mode.as<uint32_t>(), nr_arg, w.rstart.as<uint32_t>(),                                          \
                      w.rpre.as<uint32_t>(), w.cand.as<uint64_t>(), w.L.as<uint32_t>(), w.lsize.as<uint64_t>(),       \
                      w.nL.as<uint32_t>(), w.chunks.as<uint64_t>(), w.asize.as<uint64_t>(), w.adist.as<uint32_t>(),   \
                      lut_bytes, w.smm.as<uint32_t>(), abort_p)
@@ -3498,15 +3667,15 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     hipLaunchKernelGGL(sel_hist_kernel, dim3(sel_grid), dim3(256), 0, s, qn, w.asize.as<uint64_t>(),
                        w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(), w.adist.as<uint32_t>(), w.ghist.as<uint32_t>(),
                        abort_p);
-    hipLaunchKernelGGL(sel_split_kernel, dim3(sel_grid), dim3(256), 0, s, qn, l, sub_lists, w.asize.as<uint64_t>(),
+    hipLaunchKernelGGL(sel_split_kernel, dim3(sel_grid), dim3(256), 0, s, qn, l, R, w.asize.as<uint64_t>(),
                        w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(), w.kprime.as<uint64_t>(), w.adist.as<uint32_t>(),
-                       w.ghist.as<uint32_t>(), w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(),
+                       w.ghist.as<uint32_t>(), nr_arg, w.rstart.as<uint32_t>(),
                        w.rpre.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
                        w.L.as<uint32_t>(), w.akeys2.as<uint64_t>(), w.scnt.as<uint32_t>(), abort_p);
     HDG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(adc_select_kernel, dim3((unsigned)qn), dim3(256), 0, s, qn, l, sub_lists,
+    hipLaunchKernelGGL(adc_select_kernel, dim3((unsigned)qn), dim3(256), 0, s, qn, l, R,
                        w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.cand.as<uint64_t>(),
-                       w.n_chosen_s.as<uint32_t>(), w.rstart.as<uint32_t>(), w.rpre.as<uint32_t>(), S.lab.as<float>(),
+                       nr_arg, w.rstart.as<uint32_t>(), w.rpre.as<uint32_t>(), S.lab.as<float>(),
                        w.ranges.as<float>(), w.asize.as<uint64_t>(), w.Lkeys.as<uint64_t>(), w.adist.as<uint32_t>(),
                        w.akeys2.as<uint64_t>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(),
                        w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(), w.scnt.as<uint32_t>(), abort_p);

@@ -106,6 +106,7 @@ const char* orc_last_error() { return g_err.c_str(); }
 
 void orc_plant_fault(int f) { g_plant = f; }
 
+
 // OpenMP threads of the parallel loops that follow (orc_query over queries, orc_bruteforce over query tiles).
 void orc_set_threads(int n) {
 #ifdef _OPENMP
@@ -618,6 +619,7 @@ struct OrcState {
   const uint64_t* zids;      // [N] z-pool
   const float* pq_centers;   // [M][K][pq_len]
   const uint8_t* codes;      // [M][N] by original id
+  uint64_t n_split;          // 0: no split (is_split = 0); else is_split = 1 with n_split pieces per range
 };
 
 // Per-query trace for tests: sizes and checks of every intermediate stage.
@@ -631,6 +633,186 @@ struct OrcTrace {
   uint64_t* chosen_subs;    // [nq][sub_lists] chosen sub-cluster ids, UINT64_MAX pad (may be null)
   uint64_t* sub_counts;     // [nq][CS] in-box counts (may be null)
 };
+
+// ---------------------------------------------------------------------------------------------
+// Morton split mode (is_split = 1): zorder.cu:17-34 (morton_decode), :83-244 (getNextZValueBaseline),
+// :474-580 (zorder_range_query_kernel), :591-647 (get_query_end_points_kernel).
+// Decisions: the candidate sub-cluster is read from the query's own row (H23: the kernel reads candidates[c]);
+// the pieces of a query row are laid out [chosen sub-cluster][split] with the row width of the chosen list
+// (the reference allocates sub_lists * n_split columns before clamping sub_lists, which only agrees when the
+// batch's candidate width equals sub_lists).  getNextZValueBaseline equals the true BIGMIN -- the smallest
+// Morton code above `cur` inside the box -- for every cur below the largest code (tests/test_oracle_split.py
+// against brute force); above the largest code no in-box code exists and the piece is dropped.
+// Pinned by: brute-force BIGMIN on small grids; every piece a subset of its Morton range that keeps all of its
+// in-box points except, as in the reference, one that is the piece's last code and the first in-box one.
+static void orc_morton_decode(uint64_t code, int* coords, int dims, int max_bits, int valid_map) {
+  for (int i = 0; i < dims; ++i) coords[i] = 0;
+  int active = 0;
+  for (int i = 0; i < dims; ++i) active += (valid_map >> i) & 1;
+  for (int b = 0; b < max_bits; ++b) {
+    int j = 0;
+    for (int i = 0; i < dims; ++i) {
+      if (!((valid_map >> i) & 1)) continue;
+      const int shift = (max_bits - 1 - b) * active + (active - 1 - j);
+      coords[i] |= (int)((code >> shift) & 1) << (max_bits - 1 - b);
+      ++j;
+    }
+  }
+}
+
+static bool orc_in_box(const int* c, const int* box, int dims) {
+  for (int i = 0; i < dims; ++i)
+    if (c[i] < box[2 * i] || c[i] > box[2 * i + 1]) return false;
+  return true;
+}
+
+static uint64_t low_k_bits(uint64_t target, uint64_t source, int k) {
+  const uint64_t m = (1ull << k) - 1;
+  return (target & ~m) | (source & m);
+}
+
+// getNextZValueBaseline (zorder.cu:83-244): the next Morton code after cur whose point lies in the box.
+static bool orc_next_z_impl(uint64_t cur, const int* box, int dims, int max_bits, int valid_map, uint64_t& next) {
+  const uint64_t nisp = cur + 1;
+  int coords[kMaxDims];
+  orc_morton_decode(nisp, coords, dims, max_bits, valid_map);
+  if (orc_in_box(coords, box, dims)) {
+    next = nisp;
+    return true;
+  }
+  int flag[kMaxDims] = {0}, out_step[kMaxDims], save_min[kMaxDims], save_max[kMaxDims], vbits[kMaxDims];
+  int active = 0;
+  for (int i = 0; i < dims; ++i) {
+    if (!((valid_map >> i) & 1)) continue;
+    vbits[active++] = i;
+    out_step[i] = save_min[i] = save_max[i] = max_bits;
+    for (int st = 0; st < max_bits; ++st) {  // left violation: first step where the point's bit is below lo's
+      const int sh = max_bits - 1 - st;
+      const int lb = (box[2 * i] >> sh) & 1, cb = (coords[i] >> sh) & 1;
+      if (flag[i] == 0 && cb < lb) {
+        flag[i] = -1;
+        out_step[i] = st;
+        break;
+      }
+      if (cb > lb) break;
+    }
+    if (flag[i] == 0)
+      for (int st = 0; st < max_bits; ++st) {  // right violation
+        const int sh = max_bits - 1 - st;
+        const int hb = (box[2 * i + 1] >> sh) & 1, cb = (coords[i] >> sh) & 1;
+        if (cb > hb) {
+          flag[i] = 1;
+          out_step[i] = st;
+          break;
+        }
+        if (cb < hb) break;
+      }
+    if (flag[i] != -1)
+      for (int st = 0; st < max_bits; ++st) {
+        const int sh = max_bits - 1 - st;
+        if (((coords[i] >> sh) & 1) > ((box[2 * i] >> sh) & 1)) {
+          save_min[i] = st;
+          break;
+        }
+      }
+    if (flag[i] != 1)
+      for (int st = 0; st < max_bits; ++st) {
+        const int sh = max_bits - 1 - st;
+        if (((coords[i] >> sh) & 1) < ((box[2 * i + 1] >> sh) & 1)) {
+          save_max[i] = st;
+          break;
+        }
+      }
+  }
+  int change = 0;
+  for (int i = 0, j = 0; i < dims; ++i) {
+    if (!((valid_map >> i) & 1)) continue;
+    if (out_step[i] < max_bits) {
+      const int bp = (max_bits - 1 - out_step[i]) * active + (active - 1 - j);
+      if (bp > change) change = bp;
+    }
+    ++j;
+  }
+  int dsel = vbits[active - 1 - (change % active)];
+  if (flag[dsel] == 1) {
+    int found = -1;
+    for (int bp = change + 1; bp < max_bits * active; ++bp) {
+      const int jd = active - 1 - (bp % active);
+      const int td = vbits[jd];
+      if (bp <= (max_bits - 1 - save_max[td]) * active + (active - 1 - jd) && ((nisp >> bp) & 1ull) == 0) {
+        found = bp;
+        break;
+      }
+    }
+    if (found == -1) return false;
+    change = found;
+    dsel = vbits[active - 1 - (change % active)];
+    save_min[dsel] = max_bits - 1 - (change / active);
+    flag[dsel] = 0;
+  }
+  const int blocks = change / active;
+  for (int i = 0, j = 0; i < dims; ++i) {
+    if (!((valid_map >> i) & 1)) continue;
+    if (flag[i] >= 0) {
+      int bits = blocks;
+      if (vbits[j] > dsel) bits++;
+      if (change <= (max_bits - 1 - save_min[i]) * active + (active - 1 - j))
+        coords[i] = (int)low_k_bits((uint64_t)(uint32_t)coords[i], 0, bits);
+      else
+        coords[i] = (int)low_k_bits((uint64_t)(uint32_t)coords[i], (uint64_t)(uint32_t)box[2 * i], bits);
+    } else {
+      coords[i] = box[2 * i];
+    }
+    ++j;
+  }
+  next = orc_morton(coords, dims, max_bits, valid_map) | (1ull << change);
+  return true;
+}
+
+// One piece of a query's Morton range in sub-cluster s (get_query_end_points_kernel + zorder_range_query_kernel):
+// out = (start position in the z pool, length), length 0 for an empty or dropped piece.
+static void orc_split_piece(const OrcState& S, uint64_t sub, uint64_t zmin, uint64_t zmax, const int* box,
+                            uint64_t split, uint64_t& out_start, uint64_t& out_len) {
+  out_start = kU64Max;
+  out_len = 0;
+  const uint64_t start = S.seg_off[sub], size = S.seg_len[sub];
+  if (size == 0) return;
+  const uint64_t* zc = S.zcodes + start;
+  uint64_t lower = (uint64_t)(std::lower_bound(zc, zc + size, zmin) - zc);
+  uint64_t upper = (uint64_t)(std::lower_bound(zc, zc + size, zmax + 1) - zc);
+  const uint64_t len = upper - lower;
+  const uint64_t ns = std::min<uint64_t>(len, S.n_split);
+  if (split >= ns) return;  // fewer candidates than splits (also len == 0): an empty piece
+  const uint64_t qt = len / ns, r = len % ns;
+  lower += split * qt + std::min<uint64_t>(split, r);
+  upper = lower + qt + (split < r ? 1 : 0);
+  lower += start;
+  upper += start;
+  if (lower == upper) return;
+  const uint64_t zmax_p = S.zcodes[upper - 1];
+  uint64_t cur = S.zcodes[lower];
+  bool valid = true;
+  for (int jumps = 0; jumps < 3;) {
+    const uint64_t idx = (uint64_t)(std::lower_bound(S.zcodes + lower, S.zcodes + upper, cur) - S.zcodes);
+    if (idx >= upper) break;
+    lower = idx;
+    int coords[kMaxDims];
+    orc_morton_decode(S.zcodes[idx], coords, S.l, S.max_bits, S.valid_map);
+    ++jumps;
+    if (orc_in_box(coords, box, S.l)) break;
+    uint64_t nz = 0;
+    // no code above the largest one: no in-box code is left in the piece
+    const bool largest = S.zcodes[idx] + 1 == (1ull << (S.max_bits * __builtin_popcount(S.valid_map)));
+    const bool ok = !largest && orc_next_z_impl(S.zcodes[idx], box, S.l, S.max_bits, S.valid_map, nz);
+    if (!ok || nz >= zmax_p) {
+      valid = false;
+      break;
+    }
+    cur = nz;
+  }
+  out_start = lower;
+  out_len = valid ? upper - lower : 0;
+}
 
 static void point_cells(const OrcState& S, std::vector<int>& cells) {
   cells.resize(S.N * S.l);
@@ -770,12 +952,20 @@ int orc_query(const OrcState* Sp, const float* queries, const float* qlabels, ui
       uint64_t zhi = orc_morton(hi_c, l, S.max_bits, S.valid_map);
       for (uint64_t s : chosen_subs[qi]) {
         if (s >= S.CS) continue;  // start UINT64_MAX, count 0
+        real[qi] += counts[qi * S.CS + s];
+        if (S.n_split) {
+          for (uint64_t sp = 0; sp < S.n_split; ++sp) {
+            uint64_t ps = 0, pl = 0;
+            orc_split_piece(S, s, zlo, zhi, box, sp, ps, pl);
+            for (uint64_t p = ps; p < ps + pl; ++p) cands[qi].push_back(S.zids[p]);
+          }
+          continue;
+        }
         const uint64_t* zc = S.zcodes + S.seg_off[s];
         uint64_t n = S.seg_len[s];
         uint64_t lower = (uint64_t)(std::lower_bound(zc, zc + n, zlo) - zc);
         uint64_t upper = (uint64_t)(std::lower_bound(zc, zc + n, zhi + 1) - zc);
         for (uint64_t p = lower; p < upper; ++p) cands[qi].push_back(S.zids[S.seg_off[s] + p]);
-        real[qi] += counts[qi * S.CS + s];
       }
       W = std::max<uint64_t>(W, cands[qi].size());
       if (trace && trace->cand_count) trace->cand_count[b0 + qi] = (int64_t)cands[qi].size();
@@ -914,6 +1104,23 @@ int orc_query(const OrcState* Sp, const float* queries, const float* qlabels, ui
     }
   }
   return 0;
+}
+
+// getNextZValueBaseline restated (zorder.cu:83-244) for the pinning tests: returns 1 and writes next, or 0.
+int orc_next_z(uint64_t cur, const int* box, int dims, int max_bits, int valid_map, uint64_t* next) {
+  uint64_t nz = 0;
+  const bool ok = orc_next_z_impl(cur, box, dims, max_bits, valid_map, nz);
+  *next = nz;
+  return ok ? 1 : 0;
+}
+
+// One split-mode piece of sub-cluster `sub` for a query box (test hook of orc_split_piece).
+void orc_split_piece_of(const void* state, uint64_t sub, uint64_t zmin, uint64_t zmax, const int* box,
+                        uint64_t split, uint64_t* start, uint64_t* len) {
+  uint64_t a = 0, b = 0;
+  orc_split_piece(*reinterpret_cast<const OrcState*>(state), sub, zmin, zmax, box, split, a, b);
+  *start = a;
+  *len = b;
 }
 
 // =============================================================================================

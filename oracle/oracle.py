@@ -190,6 +190,26 @@ def merge_topk(dist, ids, data_batch_size, n_data, start_offset, k):
     return np.take_along_axis(flat_d, order, 1), np.take_along_axis(flat_i, order, 1)
 
 
+def next_z(cur, box, dims, max_bits, valid_map):
+    """getNextZValueBaseline (zorder.cu:83-244): the next Morton code after cur inside the int box [lo, hi] per
+    dim, or None."""
+    b = _c(box, np.int32)
+    out = np.zeros(1, np.uint64)
+    ok = _lib.orc_next_z(_u64(int(cur)), _p(b), ctypes.c_int(dims), ctypes.c_int(max_bits), ctypes.c_int(valid_map),
+                         _p(out))
+    return int(out[0]) if ok else None
+
+
+def split_piece(ps, sub, zmin, zmax, box, split):
+    """One split-mode piece (start, length) of sub-cluster `sub` for a query box, on a PreparedState."""
+    b = _c(box, np.int32)
+    st = np.zeros(1, np.uint64)
+    ln = np.zeros(1, np.uint64)
+    _lib.orc_split_piece_of(ctypes.byref(ps.s), _u64(int(sub)), _u64(int(zmin)), _u64(int(zmax)), _p(b),
+                            _u64(int(split)), _p(st), _p(ln))
+    return int(st[0]), int(ln[0])
+
+
 def plant_fault(f: int) -> None:
     """Switch the oracle to a deliberately wrong variant (tests of the tests; 0 restores the faithful one)."""
     _lib.orc_plant_fault(ctypes.c_int(f))
@@ -227,7 +247,8 @@ class _State(ctypes.Structure):
         ("valid_map", ctypes.c_int),
         ("ftype", _vp), ("shift_val", _vp), ("imap", _vp), ("imap_len", _u64), ("div_value", _vp),
         ("data", _vp), ("labels", _vp), ("centers", _vp), ("sub_centers", _vp), ("sub_off", _vp), ("sub_cnt", _vp),
-        ("seg_off", _vp), ("seg_len", _vp), ("zcodes", _vp), ("zids", _vp), ("pq_centers", _vp), ("codes", _vp)]]
+        ("seg_off", _vp), ("seg_len", _vp), ("zcodes", _vp), ("zids", _vp), ("pq_centers", _vp), ("codes", _vp),
+        ("n_split", _u64)]]
 
 
 class _Trace(ctypes.Structure):
@@ -256,7 +277,8 @@ class PreparedState:
         self.s, self.keep, self.sub_lists, self.topk = s, keep, sub_lists, topk
 
 
-def prepare(state: dict, data, labels, *, n_list, sub_lists, exps0, topk, ftype, shift_val, imap, div_value):
+def prepare(state: dict, data, labels, *, n_list, sub_lists, exps0, topk, ftype, shift_val, imap, div_value,
+            n_split=0):
     """State preparation of `query` (no query work): returns a PreparedState for run_query."""
     meta = state["meta"]
     N, d, C, CS = meta["N"], meta["d"], meta["C"], meta["CS"]
@@ -299,6 +321,7 @@ def prepare(state: dict, data, labels, *, n_list, sub_lists, exps0, topk, ftype,
     s.zids = arr(zid, np.uint64)
     s.pq_centers = arr(state["pq_centers"], np.float32)
     s.codes = arr(codes_ref, np.uint8)
+    s.n_split = int(n_split)
     return PreparedState(s, keep, sub_lists, topk)
 
 
@@ -329,8 +352,9 @@ def run_query(ps: PreparedState, queries, qlabels, batch=0, trace=False):
 
 
 def query(state: dict, data, labels, queries, qlabels, *, n_list, sub_lists, exps0, topk, ftype, shift_val,
-          imap, div_value, batch=0, trace=False):
-    """Run the restated Super-Grid query path on an index state (original-order data and labels)."""
+          imap, div_value, batch=0, trace=False, n_split=0):
+    """Run the restated Super-Grid query path on an index state (original-order data and labels); n_split > 0
+    selects the Morton split mode (is_split = 1) with that many pieces per range."""
     ps = prepare(state, data, labels, n_list=n_list, sub_lists=sub_lists, exps0=exps0, topk=topk, ftype=ftype,
-                 shift_val=shift_val, imap=imap, div_value=div_value)
+                 shift_val=shift_val, imap=imap, div_value=div_value, n_split=n_split)
     return run_query(ps, queries, qlabels, batch=batch, trace=trace)

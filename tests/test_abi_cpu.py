@@ -278,9 +278,10 @@ def test_checkpoint_validation_rejects_corruption(case):
         assert e.value.status == 3, name
 
 
-def test_is_split_is_refused_before_any_device_call():
-    """IS_SPLIT = 1 (the reference's Morton split mode) is refused at creation with HDG_EUNSUPPORTED instead of
-    silently running the no-split path."""
-    with pytest.raises(H.HDGridError) as e:
-        H.SuperGridIndex(conf_text="topk = 10\nis_split = 1\n", filter_text="l = 1\nfilter0:\n\ttype = 3\n")
-    assert e.value.status == 7
+def test_split_mode_limits_are_checked_before_any_device_call():
+    """IS_SPLIT = 1 needs N_SPLIT >= 1 and a range table of at most 8000 pieces (SUB_LISTS * N_SPLIT):
+    HDG_EUNSUPPORTED at creation otherwise."""
+    for extra in ("is_split = 1\nn_split = 0\n", "is_split = 1\nn_split = 100\nsub_lists = 128\n"):
+        with pytest.raises(H.HDGridError) as e:
+            H.SuperGridIndex(conf_text="topk = 10\n" + extra, filter_text="l = 1\nfilter0:\n\ttype = 3\n")
+        assert e.value.status == 7

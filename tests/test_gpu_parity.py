@@ -67,9 +67,10 @@ class Built:
         st = self.idx.last_stats(nq)
         chosen = self.idx.last_chosen_subs(nq)
         batch = self.param("PARITY_BATCH") or H.find_max_factor(125, nq)
+        n_split = self.param("N_SPLIT") if self.param("IS_SPLIT") else 0
         od, oi, tr = O.query(self.state, w.data, w.labels, w.queries, w.qlabels, n_list=self.param("N_LIST"),
                              sub_lists=self.param("SUB_LISTS"), exps0=self.param("EXPS0"),
-                             topk=self.param("TOPK"), batch=batch, trace=True, **w.filter_arrays)
+                             topk=self.param("TOPK"), batch=batch, trace=True, n_split=n_split, **w.filter_arrays)
         return dist.cpu().numpy(), ids.cpu().numpy().astype(np.uint64), st, chosen, od, oi, tr
 
 
@@ -246,6 +247,40 @@ def test_recall_vs_bruteforce(l3):
     r = datasets.recall(bi, gi, 100)
     print(f"[parity] recall@100 l3 = {r:.4f}")
     assert r > 0.5
+
+
+# ---- Morton split mode (IS_SPLIT = 1) ---------------------------------------------------------------------
+@pytest.mark.parametrize("n_split", [1, 4])
+def test_split_mode_l3(l3, n_split):
+    """Split pieces with BIGMIN jumps: keep-all (EXPS0 30) and PQ top-k' (EXPS0 3, batches of 20) bit-exact, and
+    the rows are no longer than the no-split ones."""
+    base = l3.run(EXPS0=3, PARITY_BATCH=20, IS_SPLIT=0)
+    for exps0 in (30, 3):
+        res = l3.run(EXPS0=exps0, PARITY_BATCH=20, IS_SPLIT=1, N_SPLIT=n_split)
+        assert_parity(res, f"l3 split {n_split} exps0={exps0}")
+    assert np.all(res[2].cand <= base[2].cand) and (res[2].cand < base[2].cand).any()
+    l3.idx.set_param("IS_SPLIT", 0)
+    l3.idx.set_param("PARITY_BATCH", 0)
+
+
+def test_split_mode_explicit_adc(l2_explicit):
+    res = l2_explicit.run(EXPS0=2, IS_SPLIT=1, N_SPLIT=8)
+    assert (res[2].adc > 0).any()
+    assert_parity(res, "explicit ranges split 8")
+    l2_explicit.idx.set_param("IS_SPLIT", 0)
+
+
+def test_split_mode_four_labels_div():
+    """Four label dims with the div filter (the 50M workload's filters on 60k points)."""
+    w = datasets.full_pipeline(n=60000, nq=200, d=32, components=64, clusters=32, sub_clusters=6, n_list=32,
+                               sub_lists=64)
+    b = Built(w)
+    for n_split in (2, 16):
+        res = b.run(EXPS0=30, IS_SPLIT=1, N_SPLIT=n_split)
+        assert_parity(res, f"four labels split {n_split}")
+    base = b.run(EXPS0=30, IS_SPLIT=0)
+    assert_parity(base, "four labels no split")
+    assert res[2].cand.sum() < base[2].cand.sum()
 
 
 # ---- the workloads at their own sizes ------------------------------------------------------------------
