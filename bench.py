@@ -223,6 +223,30 @@ def full_leg(args, H, datasets, O, dev, log):
                            "frac": round(gbs / HBM_PEAK_GBS, 6), "traffic": traffic_of("adc_scan", w.name),
                            "avg_ms": round(adc_ms, 4), "alg_bytes_per_launch": adc_bytes,
                            "code_array_bytes": int(args.full_n) * M}
+    # Morton split mode (IS_SPLIT 1) on the same budget: candidate reduction, recall, one-chunk time
+    split = {}
+    for ns in (4, 16):
+        idx.set_param("IS_SPLIT", 1)
+        idx.set_param("N_SPLIT", ns)
+        r_s, _, _ = recall_now()
+        keep = idx.get_param("QUERY_CHUNK")
+        idx.set_param("QUERY_CHUNK", n1)
+        idx.query(q[:n1], ql[:n1])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        idx.query(q[:n1], ql[:n1])
+        torch.cuda.synchronize()
+        el_s = time.perf_counter() - t1
+        st_s = idx.last_stats(n1)
+        idx.set_param("QUERY_CHUNK", keep)
+        split[f"n_split_{ns}"] = {"recall@100": round(r_s, 4), "mean_candidates": round(float(st_s.cand.mean()), 1),
+                                  "candidates_vs_no_split": round(float(st_s.cand.sum() / max(1, st.cand.sum())), 4),
+                                  f"ms per {n1} queries": round(el_s * 1e3, 2)}
+        log(f"full plain split {ns}: recall {r_s:.4f}, candidates x{split[f'n_split_{ns}']['candidates_vs_no_split']}, "
+            f"{el_s * 1e3:.1f} ms per {n1} queries")
+    idx.set_param("IS_SPLIT", 0)
+    out["plain"][f"ms per {n1} queries (one chunk)"] = round(sum(stage.values()), 2)
+    out["plain_split_mode"] = split
     idx.set_param("PRECISION", 1)
     b, r, d, i = sweep(128, 2 ** 0.5)
     el, _, st = timed()
