@@ -1,13 +1,20 @@
 // Index build: Super-Grid-IVF_PQ.cu:420-646 (index::build) and product-quantization.cu:147-212
-// (build_pq), with the RAFT k-means calls replaced by this file's own k-means:
-//   * assignment on the GPU (tiled FP32 ||c||^2 - 2 x.c, ties to the lower center id);
-//   * centroid update accumulated in FP64 on the GPU, divided on the host;
-//   * empty clusters re-seeded from random training rows.
-// The deterministic parts (grid shape, cells, Morton codes, cluster-major ordering, per-sub-cluster
-// Morton sort) follow the reference exactly; the k-means outcome is not pinned by the reference.
+// (build_pq), with the RAFT k-means calls replaced by this file's own k-means, all on the device:
+//   * assignment: tiled FP32 ||c||^2 - 2 x.c, ties to the lower center id;
+//   * centroid update: sums in 64-bit fixed point with integer atomics (associative, so independent of
+//     the order rows arrive in), divided in FP64;
+//   * empty clusters re-seeded from random training rows drawn on the host.
+// Cells, Morton codes, the z-pool order (two stable radix sorts), the gathers into z-pool order, the
+// bucket grid and the coarse occupancy sums run on the device too.  The deterministic parts (grid shape,
+// cells, Morton codes, cluster-major ordering, per-sub-cluster Morton sort) follow the reference exactly;
+// the k-means outcome is not pinned by the reference, but the same input and config give the same index
+// state bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -38,9 +45,9 @@ __global__ __launch_bounds__(256) void assign_kernel(const float* __restrict__ X
   const uint64_t row0 = (uint64_t)blockIdx.x * kTile;
   float best_v[4];
   uint32_t best_i[4];
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < 4; ++r) {  // a row whose scores are all infinite or NaN keeps center 0
     best_v[r] = INFINITY;
-    best_i[r] = 0xFFFFFFFFu;
+    best_i[r] = 0;
   }
   for (uint32_t c0 = 0; c0 < k; c0 += kTile) {
     float acc[4][4] = {};
@@ -103,16 +110,56 @@ __global__ __launch_bounds__(256) void assign_kernel(const float* __restrict__ X
   }
 }
 
-// FP64 centroid sums: sums[label][t] += X[row][t]; counts[label] += 1.
+// Largest |x| over the rows, as float bits (non-negative floats order like their bit patterns).
+__global__ void absmax_kernel(const float* __restrict__ X, uint64_t n, uint64_t ld, uint32_t w,
+                              unsigned int* __restrict__ out) {
+  unsigned int m = 0;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * w; e += (uint64_t)gridDim.x * blockDim.x) {
+    const float v = fabsf(X[(e / w) * ld + e % w]);
+    const unsigned int b = v != v ? 0x7F800001u : __float_as_uint(v);  // NaN counts as non-finite
+    m = b > m ? b : m;
+  }
+  atomicMax(out, m);
+}
+
+// Centroid sums in 64-bit fixed point: each coordinate is rounded once to a multiple of 1 / scale, where
+// scale = 2^(62 - e) and 2^e > n * max|x|, so no partial sum leaves int64; the integer additions are
+// associative, so sums[label][t] and counts[label] come out the same whatever order the rows arrive in.
 __global__ void accumulate_kernel(const float* __restrict__ X, uint64_t n, uint64_t ld, uint32_t w,
-                                  const uint32_t* __restrict__ label, double* __restrict__ sums,
-                                  unsigned long long* __restrict__ counts) {
+                                  const uint32_t* __restrict__ label, double scale,
+                                  unsigned long long* __restrict__ sums, unsigned long long* __restrict__ counts) {
   uint64_t row = (uint64_t)blockIdx.x * blockDim.y + threadIdx.y;
   if (row >= n) return;
   uint32_t c = label[row];
-  for (uint32_t t = threadIdx.x; t < w; t += blockDim.x)
-    atomicAdd(&sums[(uint64_t)c * w + t], (double)X[row * ld + t]);
+  for (uint32_t t = threadIdx.x; t < w; t += blockDim.x) {
+    const long long v = __double2ll_rn((double)X[row * ld + t] * scale);
+    atomicAdd(&sums[(uint64_t)c * w + t], (unsigned long long)v);  // two's complement: a signed sum
+  }
   if (threadIdx.x == 0) atomicAdd(&counts[c], 1ull);
+}
+
+// centers[c][t] = sums[c][t] / scale / counts[c] for the non-empty clusters.
+__global__ void center_update_kernel(const unsigned long long* __restrict__ sums,
+                                     const unsigned long long* __restrict__ counts, uint32_t k, uint32_t w,
+                                     double inv_scale, float* __restrict__ centers) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (uint64_t)k * w) return;
+  const unsigned long long cnt = counts[e / w];
+  if (cnt == 0) return;
+  centers[e] = (float)((double)(long long)sums[e] * inv_scale / (double)cnt);
+}
+
+// |c|^2 of each center, summed in FP64 over the dims in order (one thread per center).
+__global__ void center_norm_kernel(const float* __restrict__ centers, uint32_t k, uint32_t w,
+                                   float* __restrict__ norm) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  double a = 0.0;
+  for (uint32_t t = 0; t < w; ++t) {
+    const double v = centers[(uint64_t)c * w + t];
+    a += v * v;
+  }
+  norm[c] = (float)a;
 }
 
 // dst[i] = src[idx[i]] for rows of `words` u32 words.
@@ -126,6 +173,15 @@ __global__ void gather_rows_kernel(const uint32_t* __restrict__ src, uint64_t sr
   for (uint32_t t = threadIdx.x; t < words; t += blockDim.x) o[t] = s[t];
 }
 
+// dst[idx[i]] = src[i] for rows of w floats.
+__global__ void scatter_rows_kernel(const float* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t n,
+                                    uint32_t w, float* __restrict__ dst) {
+  uint64_t row = (uint64_t)blockIdx.x * blockDim.y + threadIdx.y;
+  if (row >= n) return;
+  float* o = dst + (uint64_t)idx[row] * w;
+  for (uint32_t t = threadIdx.x; t < w; t += blockDim.x) o[t] = src[row * w + t];
+}
+
 // Strided sub-matrix copy: dst[i][0..w) = src[i*ld + off + 0..w).
 __global__ void slice_kernel(const float* __restrict__ src, uint64_t n, uint64_t ld, uint64_t off,
                              uint32_t w, float* __restrict__ dst) {
@@ -133,6 +189,207 @@ __global__ void slice_kernel(const float* __restrict__ src, uint64_t n, uint64_t
   if (e >= n * w) return;
   uint64_t r = e / w, t = e % w;
   dst[e] = src[r * ld + off + t];
+}
+
+// PQ codes of every row for all subspaces in one pass over the data (product-quantization.cu:182-211 encodes
+// one subspace at a time).  Thread = row.  For each subspace the block stages the K centers [K][pq_len]
+// (zero padded past the subspace's width) and their norms in LDS; the code is the center with the smallest
+// |c|^2 - 2 x.c, x.c an FMA chain over the dims in order, ties to the lower id: assign_kernel's rule, so
+// the codes equal a per-subspace assign_kernel pass.  PQL >= pq_len is the register width of x.
+template <int PQL>
+__global__ __launch_bounds__(256) void pq_encode_kernel(const float* __restrict__ X, uint64_t n, uint64_t d,
+                                                        uint32_t M, uint32_t K, uint32_t pq_len,
+                                                        const float* __restrict__ pqc,
+                                                        const float* __restrict__ pqn,
+                                                        uint8_t* __restrict__ codes) {
+  extern __shared__ float sm[];  // [K * pq_len] centers, then [K] norms
+  const uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t m = 0; m < M; ++m) {
+    const uint64_t off = (uint64_t)m * pq_len;
+    const uint32_t w = (uint32_t)(off < d ? (d - off < pq_len ? d - off : pq_len) : 0);
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < K * pq_len; e += blockDim.x) sm[e] = pqc[(uint64_t)m * K * pq_len + e];
+    for (uint32_t e = threadIdx.x; e < K; e += blockDim.x) sm[K * pq_len + e] = pqn[(uint64_t)m * K + e];
+    __syncthreads();
+    if (row >= n) continue;
+    if (w == 0) {
+      codes[row * M + m] = 0;
+      continue;
+    }
+    float x[PQL];
+#pragma unroll
+    for (int t = 0; t < PQL; ++t) x[t] = (uint32_t)t < w ? X[row * d + off + t] : 0.0f;
+    float best = INFINITY;
+    uint32_t bi = 0;
+    for (uint32_t c = 0; c < K; ++c) {
+      const float* cc = sm + c * pq_len;
+      float acc = 0.0f;
+#pragma unroll
+      for (int t = 0; t < PQL; ++t)
+        if ((uint32_t)t < pq_len) acc = fmaf(x[t], cc[t], acc);
+      const float sc = sm[K * pq_len + c] - 2.0f * acc;
+      if (sc < best) {
+        best = sc;
+        bi = c;
+      }
+    }
+    codes[row * M + m] = (uint8_t)bi;
+  }
+}
+
+// codes[row][m] = label[row] (the per-subspace encoder for wide subspaces).
+__global__ void code_column_kernel(const uint32_t* __restrict__ label, uint64_t n, uint32_t M, uint32_t m,
+                                   uint8_t* __restrict__ codes) {
+  const uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row < n) codes[row * M + m] = (uint8_t)label[row];
+}
+
+// Grid arguments for the cell / Morton kernel (the build's GridShape, by value).
+struct GridArgs {
+  int l, max_bits, valid_map, n_valid;
+  int dims[kMaxDims], shuffle[kMaxDims], valid_dims[kMaxDims];
+  uint64_t left[kMaxDims], span[kMaxDims];
+};
+
+// B2/B5 per row (Super-Grid-IVF_PQ.cu:559-600): the cell of each sorted dim (get_grid_id clamped to the
+// last cell), the Morton code over the valid dims, the valid dims' cells ([n_valid][n], original order),
+// the row's sub-cluster table position and its id (the value the sorts carry).
+__global__ void cells_morton_kernel(const float* __restrict__ labels, uint64_t n, GridArgs g,
+                                    const uint32_t* __restrict__ fine_label, const uint32_t* __restrict__ pos_of_fine,
+                                    uint32_t* __restrict__ cells, uint64_t* __restrict__ zc,
+                                    uint32_t* __restrict__ seg, uint32_t* __restrict__ ids) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int cp[kMaxDims];
+  for (int i = 0; i < g.l; ++i) {
+    const int c = grid_cell(labels[p * g.l + g.shuffle[i]], g.left[i], g.span[i], g.dims[i]);
+    cp[i] = c < g.dims[i] - 1 ? c : g.dims[i] - 1;
+  }
+  zc[p] = morton_code(cp, g.l, g.max_bits, g.valid_map);
+  for (int j = 0; j < g.n_valid; ++j) cells[(uint64_t)j * n + p] = (uint32_t)cp[g.valid_dims[j]];
+  seg[p] = pos_of_fine[fine_label[p]];
+  ids[p] = (uint32_t)p;
+}
+
+// out[i] = in[idx[i]] for elements of type T.
+template <class T>
+__global__ void gather_kernel(const T* __restrict__ in, const uint32_t* __restrict__ idx, uint64_t n,
+                              T* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[idx[i]];
+}
+
+// dst[i][j] = src[idx[i]][j] for rows of `bytes` bytes.
+__global__ void gather_bytes_kernel(const uint8_t* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t n,
+                                    uint32_t bytes, uint8_t* __restrict__ dst) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * bytes) return;
+  const uint64_t i = e / bytes, j = e % bytes;
+  dst[e] = src[(uint64_t)idx[i] * bytes + j];
+}
+
+// start[b] = first i with sorted[i] >= b, for b in [0, nkeys] (sorted ascending, n entries).
+__global__ void lower_bounds_kernel(const uint32_t* __restrict__ sorted, uint64_t n, uint64_t nkeys,
+                                    uint32_t* __restrict__ start) {
+  const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nkeys) return;
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (sorted[mid] < b) lo = mid + 1;
+    else hi = mid;
+  }
+  start[b] = (uint32_t)lo;
+}
+
+// The sub-cluster holding z-pool position p: the last s with seg_off[s] <= p (seg_off ascending, [CS+1]).
+__device__ inline uint32_t segment_of(const uint32_t* __restrict__ seg_off, uint64_t CS, uint32_t p) {
+  uint64_t lo = 0, hi = CS + 1;  // first s with seg_off[s] > p, minus one
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (seg_off[mid] <= p) lo = mid + 1;
+    else hi = mid;
+  }
+  return (uint32_t)(lo - 1);
+}
+
+// Bucket grid arguments (the IndexState's bucket fields, by value).
+struct BucketArgs {
+  int nv;
+  uint32_t Mv[kMaxDims], nb[kMaxDims], sg[kMaxDims];
+  uint64_t bstride[kMaxDims], sstride[kMaxDims];
+};
+
+// bid[p] = bucket of z-pool position p; pos[p] = p.
+__global__ void bucket_id_kernel(const uint32_t* __restrict__ cells, uint64_t N, BucketArgs a,
+                                 uint32_t* __restrict__ bid, uint32_t* __restrict__ pos) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  uint64_t b = 0;
+  for (int j = 0; j < a.nv; ++j) b += ((uint64_t)cells[(uint64_t)j * N + p] * a.nb[j] / a.Mv[j]) * a.bstride[j];
+  bid[p] = (uint32_t)b;
+  pos[p] = (uint32_t)p;
+}
+
+// Entry e of the bucket grid: its cells and its sub-cluster, from the z-pool position epos[e].
+__global__ void entry_fill_kernel(const uint32_t* __restrict__ cells, const uint32_t* __restrict__ seg_off,
+                                  uint64_t CS, uint64_t N, int nv, const uint32_t* __restrict__ epos,
+                                  uint32_t* __restrict__ ecell, uint32_t* __restrict__ esub) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N) return;
+  const uint32_t p = epos[e];
+  for (int j = 0; j < nv; ++j) ecell[(uint64_t)j * N + e] = cells[(uint64_t)j * N + p];
+  esub[e] = segment_of(seg_off, CS, p);
+}
+
+// Coarse occupancy counts: t[sub][x] += 1 at x = sum_j (coarse cell_j + 1) * sstride[j].
+__global__ void sat_count_kernel(const uint32_t* __restrict__ cells, const uint32_t* __restrict__ seg_off,
+                                 uint64_t CS, uint64_t N, BucketArgs a, uint64_t per, uint32_t* __restrict__ t) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  uint64_t x = 0;
+  for (int j = 0; j < a.nv; ++j)
+    x += ((uint64_t)cells[(uint64_t)j * N + p] * a.sg[j] / a.Mv[j] + 1) * a.sstride[j];
+  atomicAdd(&t[(uint64_t)segment_of(seg_off, CS, (uint32_t)p) * per + x], 1u);
+}
+
+// Inclusive prefix sums along dim j of every sub-cluster's table: one thread per line of the dim.
+__global__ void sat_prefix_kernel(uint32_t* __restrict__ t, uint64_t lines, uint64_t per, uint64_t wj,
+                                  uint64_t sj) {
+  const uint64_t L = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (L >= lines) return;
+  const uint64_t per_line = per / wj;  // lines of one sub-cluster's table
+  const uint64_t c = L / per_line, r = L % per_line;
+  uint32_t* base = t + c * per + (r / sj) * (wj * sj) + r % sj;
+  uint32_t acc = base[0];
+  for (uint64_t k = 1; k < wj; ++k) {
+    acc += base[k * sj];
+    base[k * sj] = acc;
+  }
+}
+
+// Stable radix sort of (key, value) pairs on bits [0, end_bit) of the key; end_bit 0 copies.
+template <class K>
+void sort_pairs(const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint64_t n, int end_bit, hipStream_t s) {
+  if (n == 0) return;
+  if (end_bit <= 0) {
+    HDG_HIP(hipMemcpyAsync(kout, kin, n * sizeof(K), hipMemcpyDeviceToDevice, s));
+    HDG_HIP(hipMemcpyAsync(vout, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    return;
+  }
+  size_t bytes = 0;
+  HDG_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)end_bit, s));
+  DevBuf tmp;
+  tmp.reserve(std::max<size_t>(bytes, 256));
+  HDG_HIP(rocprim::radix_sort_pairs(tmp.data(), bytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)end_bit, s));
+  HDG_HIP(hipStreamSynchronize(s));  // tmp is released on return
+}
+
+// Bits needed for keys in [0, nkeys).
+int key_bits(uint64_t nkeys) {
+  int b = 0;
+  while (b < 64 && (nkeys - 1) >> b) ++b;
+  return nkeys <= 1 ? 0 : b;
 }
 
 uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
@@ -145,82 +402,107 @@ void launch_assign(const float* X, uint64_t n, uint64_t ld, uint32_t w, const fl
   HDG_HIP(hipGetLastError());
 }
 
-// Lloyd k-means on X (device, rows n, width w, stride ld).  centers (host, [k][w]) are seeded here
-// from distinct random rows.  Returns the final centers on the host.
+void gather_rows(const float* X, uint64_t ld, const DevBuf& d_idx, uint64_t rows, uint32_t w, float* out,
+                 hipStream_t s) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)ceil_div(rows, 4)), dim3(64, 4), 0, s,
+                     reinterpret_cast<const uint32_t*>(X), ld, d_idx.as<uint32_t>(), rows, w,
+                     reinterpret_cast<uint32_t*>(out));
+  HDG_HIP(hipGetLastError());
+}
+
+// Lloyd k-means on X (device, rows n, width w, stride ld), seeded from distinct random rows; the centers
+// stay on the device between iterations and come back to the host at the end.  Every step is
+// order-independent (assignment ties go to the lower id, sums are fixed point), so the same input and seed
+// give the same centers bit for bit.
 void kmeans(const float* X, uint64_t n, uint64_t ld, uint32_t w, uint32_t k, int iters, uint64_t seed,
             std::vector<float>& centers, hipStream_t s) {
   if (n < k) fail(kInvalidArgument, "k-means needs at least as many rows as centers");
   std::mt19937_64 rng(seed);
-  centers.assign((size_t)k * w, 0.0f);
   // distinct random seed rows
-  std::vector<uint64_t> pick;
-  {
-    std::vector<uint64_t> perm;
-    if (n <= 4ull * k) {
-      perm.resize(n);
-      std::iota(perm.begin(), perm.end(), 0);
-      std::shuffle(perm.begin(), perm.end(), rng);
-      pick.assign(perm.begin(), perm.begin() + k);
-    } else {
-      std::vector<char> used;
-      std::uniform_int_distribution<uint64_t> U(0, n - 1);
-      std::vector<uint64_t> tmp;
-      tmp.reserve(k);
-      std::vector<uint64_t> sorted;
-      while (tmp.size() < k) {
-        uint64_t r = U(rng);
-        auto it = std::lower_bound(sorted.begin(), sorted.end(), r);
-        if (it != sorted.end() && *it == r) continue;
-        sorted.insert(it, r);
-        tmp.push_back(r);
-      }
-      pick = tmp;
+  std::vector<uint32_t> pick;
+  if (n <= 4ull * k) {
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::shuffle(perm.begin(), perm.end(), rng);
+    pick.assign(perm.begin(), perm.begin() + k);
+  } else {
+    std::uniform_int_distribution<uint64_t> U(0, n - 1);
+    std::vector<uint64_t> sorted;
+    pick.reserve(k);
+    while (pick.size() < k) {
+      uint64_t r = U(rng);
+      auto it = std::lower_bound(sorted.begin(), sorted.end(), r);
+      if (it != sorted.end() && *it == r) continue;
+      sorted.insert(it, r);
+      pick.push_back((uint32_t)r);
     }
   }
-  auto fetch_row = [&](uint64_t r, float* dst) {
-    HDG_HIP(hipMemcpyAsync(dst, X + r * ld, w * sizeof(float), hipMemcpyDeviceToHost, s));
-  };
-  for (uint32_t c = 0; c < k; ++c) fetch_row(pick[c], &centers[(size_t)c * w]);
-  HDG_HIP(hipStreamSynchronize(s));
-
-  DevBuf d_c, d_norm, d_label, d_sums, d_cnt;
+  DevBuf d_c, d_norm, d_label, d_sums, d_cnt, d_idx, d_max;
   d_c.reserve((size_t)k * w * sizeof(float));
   d_norm.reserve(k * sizeof(float));
   d_label.reserve(n * sizeof(uint32_t));
-  d_sums.reserve((size_t)k * w * sizeof(double));
+  d_sums.reserve((size_t)k * w * sizeof(unsigned long long));
   d_cnt.reserve(k * sizeof(unsigned long long));
-  std::vector<float> norm(k);
-  std::vector<double> sums((size_t)k * w);
+  upload(d_idx, pick, s);
+  gather_rows(X, ld, d_idx, k, w, d_c.as<float>(), s);
+
+  // fixed-point scale from the largest coordinate
+  upload(d_max, std::vector<unsigned int>{0u}, s);
+  hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)std::min<uint64_t>(4096, ceil_div(n * w, 256))), dim3(256), 0, s,
+                     X, n, ld, w, d_max.as<unsigned int>());
+  HDG_HIP(hipGetLastError());
+  unsigned int maxbits = 0;
+  HDG_HIP(hipMemcpyAsync(&maxbits, d_max.data(), sizeof(maxbits), hipMemcpyDeviceToHost, s));
+  HDG_HIP(hipStreamSynchronize(s));
+  float maxabs;
+  std::memcpy(&maxabs, &maxbits, sizeof(maxabs));
+  if (!std::isfinite(maxabs)) fail(kInvalidArgument, "k-means input holds a non-finite value");
+  int e = 0;
+  std::frexp((double)maxabs * (double)n, &e);  // n * max|x| < 2^e
+  const double scale = maxabs > 0.0f ? std::ldexp(1.0, 62 - e) : 1.0;
+
   std::vector<unsigned long long> cnt(k);
   std::uniform_int_distribution<uint64_t> U(0, n - 1);
   for (int it = 0; it < iters; ++it) {
-    for (uint32_t c = 0; c < k; ++c) {
-      double a = 0;
-      for (uint32_t t = 0; t < w; ++t) a += (double)centers[(size_t)c * w + t] * centers[(size_t)c * w + t];
-      norm[c] = (float)a;
-    }
-    HDG_HIP(hipMemcpyAsync(d_c.data(), centers.data(), (size_t)k * w * sizeof(float), hipMemcpyHostToDevice, s));
-    HDG_HIP(hipMemcpyAsync(d_norm.data(), norm.data(), k * sizeof(float), hipMemcpyHostToDevice, s));
-    launch_assign(X, n, ld, w, d_c.as<float>(), k, d_norm.as<float>(), d_label.as<uint32_t>(), nullptr, s);
-    HDG_HIP(hipMemsetAsync(d_sums.data(), 0, (size_t)k * w * sizeof(double), s));
-    HDG_HIP(hipMemsetAsync(d_cnt.data(), 0, k * sizeof(unsigned long long), s));
-    dim3 blk(32, 8);
-    hipLaunchKernelGGL(accumulate_kernel, dim3((unsigned)ceil_div(n, 8)), blk, 0, s, X, n, ld, w,
-                       d_label.as<uint32_t>(), d_sums.as<double>(), d_cnt.as<unsigned long long>());
+    hipLaunchKernelGGL(center_norm_kernel, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, s, d_c.as<float>(), k, w,
+                       d_norm.as<float>());
     HDG_HIP(hipGetLastError());
-    HDG_HIP(hipMemcpyAsync(sums.data(), d_sums.data(), (size_t)k * w * sizeof(double), hipMemcpyDeviceToHost, s));
+    launch_assign(X, n, ld, w, d_c.as<float>(), k, d_norm.as<float>(), d_label.as<uint32_t>(), nullptr, s);
+    HDG_HIP(hipMemsetAsync(d_sums.data(), 0, (size_t)k * w * sizeof(unsigned long long), s));
+    HDG_HIP(hipMemsetAsync(d_cnt.data(), 0, k * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(accumulate_kernel, dim3((unsigned)ceil_div(n, 8)), dim3(32, 8), 0, s, X, n, ld, w,
+                       d_label.as<uint32_t>(), scale, d_sums.as<unsigned long long>(),
+                       d_cnt.as<unsigned long long>());
+    HDG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(center_update_kernel, dim3((unsigned)ceil_div((uint64_t)k * w, 256)), dim3(256), 0, s,
+                       d_sums.as<unsigned long long>(), d_cnt.as<unsigned long long>(), k, w, 1.0 / scale,
+                       d_c.as<float>());
+    HDG_HIP(hipGetLastError());
     HDG_HIP(hipMemcpyAsync(cnt.data(), d_cnt.data(), k * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HDG_HIP(hipStreamSynchronize(s));
-    for (uint32_t c = 0; c < k; ++c) {
+    // re-seed the empty clusters from random rows, in ascending cluster order
+    std::vector<uint32_t> empty_c, empty_r;
+    for (uint32_t c = 0; c < k; ++c)
       if (cnt[c] == 0) {
-        fetch_row(U(rng), &centers[(size_t)c * w]);  // re-seed an empty cluster
-        continue;
+        empty_c.push_back(c);
+        empty_r.push_back((uint32_t)U(rng));
       }
-      for (uint32_t t = 0; t < w; ++t)
-        centers[(size_t)c * w + t] = (float)(sums[(size_t)c * w + t] / (double)cnt[c]);
+    if (!empty_c.empty()) {
+      DevBuf d_ec, d_er, d_tmp;
+      upload(d_er, empty_r, s);
+      d_tmp.reserve(empty_r.size() * w * sizeof(float));
+      gather_rows(X, ld, d_er, empty_r.size(), w, d_tmp.as<float>(), s);
+      upload(d_ec, empty_c, s);  // centers[empty_c[i]] = row empty_r[i]
+      hipLaunchKernelGGL(scatter_rows_kernel, dim3((unsigned)ceil_div(empty_c.size(), 4)), dim3(64, 4), 0, s,
+                         d_tmp.as<float>(), d_ec.as<uint32_t>(), (uint64_t)empty_c.size(), w, d_c.as<float>());
+      HDG_HIP(hipGetLastError());
+      HDG_HIP(hipStreamSynchronize(s));
     }
-    HDG_HIP(hipStreamSynchronize(s));
   }
+  centers.resize((size_t)k * w);
+  HDG_HIP(hipMemcpyAsync(centers.data(), d_c.data(), (size_t)k * w * sizeof(float), hipMemcpyDeviceToHost, s));
+  HDG_HIP(hipStreamSynchronize(s));
 }
 
 // Assign every row of X to its nearest center (centers on host).
@@ -277,7 +559,7 @@ std::vector<uint32_t> sample_rows(uint64_t n, uint64_t want, uint64_t seed) {
 
 }  // namespace
 
-void build_bucket_grid(IndexState& st, const std::vector<uint32_t>& h_cells, hipStream_t s) {
+void build_bucket_grid(IndexState& st, hipStream_t s) {
   const GridShape& g = st.grid;
   const int nv = g.n_valid;
   const uint64_t N = st.N;
@@ -301,55 +583,60 @@ void build_bucket_grid(IndexState& st, const std::vector<uint32_t>& h_cells, hip
   for (int j = nv - 2; j >= 0; --j) st.bstride[j] = st.bstride[j + 1] * st.nb[j + 1];
   st.Nb = st.bstride[0] * st.nb[0];
   if (st.Nb >= (1ull << 32)) fail(kUnsupported, "bucket grid too large");
-  // stable counting sort of z-pool positions by bucket
-  std::vector<uint32_t> start(st.Nb + 1, 0), bid(N);
-  for (uint64_t p = 0; p < N; ++p) {
-    uint64_t b = 0;
-    for (int j = 0; j < nv; ++j)
-      b += ((uint64_t)h_cells[(uint64_t)j * N + p] * st.nb[j] / st.Mv[j]) * st.bstride[j];
-    bid[p] = (uint32_t)b;
-    start[b + 1]++;
-  }
-  for (uint64_t b = 0; b < st.Nb; ++b) start[b + 1] += start[b];
-  std::vector<uint32_t> cur(start.begin(), start.end() - 1), ecell((uint64_t)nv * N), esub(N), epos(N);
-  uint64_t sub = 0;
-  for (uint64_t p = 0; p < N; ++p) {
-    while (st.h_seg_off[sub + 1] <= p) ++sub;
-    const uint32_t e = cur[bid[p]]++;
-    for (int j = 0; j < nv; ++j) ecell[(uint64_t)j * N + e] = h_cells[(uint64_t)j * N + p];
-    esub[e] = (uint32_t)sub;
-    epos[e] = (uint32_t)p;
-  }
-  upload(st.bstart, start, s);
-  upload(st.epos, epos, s);
-  upload(st.ecell, ecell, s);
-  upload(st.esub, esub, s);
-  // coarse occupancy sums: 2^(12 / n_valid) coarse cells per dim (at most the dim's cells), counted per
-  // sub-cluster, then prefix-summed along each dim
+  // coarse occupancy grid: 2^(12 / n_valid) coarse cells per dim (at most the dim's cells)
   const int gb = std::max(1, 12 / nv);
   for (int j = 0; j < nv; ++j) st.sg[j] = std::min<uint32_t>(1u << gb, st.Mv[j]);
   st.sstride[nv - 1] = 1;
   for (int j = nv - 2; j >= 0; --j) st.sstride[j] = st.sstride[j + 1] * (st.sg[j + 1] + 1);
+  BucketArgs a{};
+  a.nv = nv;
+  for (int j = 0; j < nv; ++j) {
+    a.Mv[j] = st.Mv[j];
+    a.nb[j] = st.nb[j];
+    a.sg[j] = st.sg[j];
+    a.bstride[j] = st.bstride[j];
+    a.sstride[j] = st.sstride[j];
+  }
+  const unsigned blocks = (unsigned)ceil_div(N, 256);
+  // entries: z-pool positions stably sorted by bucket (ascending position inside a bucket)
+  {
+    DevBuf bid, pos, bid_sorted;
+    bid.reserve(N * 4);
+    pos.reserve(N * 4);
+    bid_sorted.reserve(N * 4);
+    st.epos.reserve(N * 4);
+    hipLaunchKernelGGL(bucket_id_kernel, dim3(blocks), dim3(256), 0, s, st.cells.as<uint32_t>(), N, a,
+                       bid.as<uint32_t>(), pos.as<uint32_t>());
+    HDG_HIP(hipGetLastError());
+    sort_pairs(bid.as<uint32_t>(), bid_sorted.as<uint32_t>(), pos.as<uint32_t>(), st.epos.as<uint32_t>(), N,
+               key_bits(st.Nb), s);
+    st.bstart.reserve((st.Nb + 1) * 4);
+    hipLaunchKernelGGL(lower_bounds_kernel, dim3((unsigned)ceil_div(st.Nb + 1, 256)), dim3(256), 0, s,
+                       bid_sorted.as<uint32_t>(), N, st.Nb, st.bstart.as<uint32_t>());
+    HDG_HIP(hipGetLastError());
+    st.ecell.reserve((uint64_t)nv * N * 4);
+    st.esub.reserve(N * 4);
+    hipLaunchKernelGGL(entry_fill_kernel, dim3(blocks), dim3(256), 0, s, st.cells.as<uint32_t>(),
+                       st.seg_off.as<uint32_t>(), st.CS, N, nv, st.epos.as<uint32_t>(), st.ecell.as<uint32_t>(),
+                       st.esub.as<uint32_t>());
+    HDG_HIP(hipGetLastError());
+    HDG_HIP(hipStreamSynchronize(s));
+  }
+  // coarse occupancy sums per sub-cluster, prefix-summed along each dim
   const uint64_t per = st.sstride[0] * (st.sg[0] + 1);
   if (per * st.CS > (1ull << 26)) return;  // more than 256 MiB: the dense count scans without it
-  std::vector<uint32_t> t(per * st.CS, 0);
-  sub = 0;
-  for (uint64_t p = 0; p < N; ++p) {
-    while (st.h_seg_off[sub + 1] <= p) ++sub;
-    uint64_t x = 0;
-    for (int j = 0; j < nv; ++j)
-      x += ((uint64_t)h_cells[(uint64_t)j * N + p] * st.sg[j] / st.Mv[j] + 1) * st.sstride[j];
-    t[sub * per + x]++;
-  }
+  st.sat.reserve(per * st.CS * 4);
+  HDG_HIP(hipMemsetAsync(st.sat.data(), 0, per * st.CS * 4, s));
+  hipLaunchKernelGGL(sat_count_kernel, dim3(blocks), dim3(256), 0, s, st.cells.as<uint32_t>(),
+                     st.seg_off.as<uint32_t>(), st.CS, N, a, per, st.sat.as<uint32_t>());
+  HDG_HIP(hipGetLastError());
   for (int j = 0; j < nv; ++j) {
-    const uint64_t w = st.sg[j] + 1, sj = st.sstride[j];
-    for (uint64_t c = 0; c < st.CS; ++c) {
-      uint32_t* tc = t.data() + c * per;
-      for (uint64_t x = 0; x < per; ++x)
-        if ((x / sj) % w != 0) tc[x] += tc[x - sj];
-    }
+    const uint64_t wj = st.sg[j] + 1, lines = st.CS * (per / wj);
+    hipLaunchKernelGGL(sat_prefix_kernel, dim3((unsigned)ceil_div(lines, 256)), dim3(256), 0, s,
+                       st.sat.as<uint32_t>(), lines, per, wj, st.sstride[j]);
+    HDG_HIP(hipGetLastError());
   }
-  upload(st.sat, t, s);
+  HDG_HIP(hipStreamSynchronize(s));
   st.sat_per = per;
 }
 
@@ -411,12 +698,23 @@ void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* lab
   st_.pq_len = pq_len;
   st_.l = l;
 
+  // per-stage wall times at log level 1 (HDG_LOG=1)
+  auto t_last = std::chrono::steady_clock::now();
+  auto stage_done = [&](const char* what) {
+    if (log_level() < 1) return;
+    HDG_HIP(hipStreamSynchronize(s));
+    const auto now = std::chrono::steady_clock::now();
+    log_line(1, "build %s: %.1f ms", what, std::chrono::duration<double, std::milli>(now - t_last).count());
+    t_last = now;
+  };
+
   // ---- B1: grid shape from the labels (host copy) ----
   std::vector<float> h_lab(n * l);
   HDG_HIP(hipMemcpyAsync(h_lab.data(), labels, n * l * sizeof(float), hipMemcpyDeviceToHost, s));
   HDG_HIP(hipStreamSynchronize(s));
   st_.grid = compute_grid_shape(h_lab.data(), n, l, cfg_.max_grids, cfg_.label_mask);
   const GridShape& g = st_.grid;
+  stage_done("grid shape");
 
   // ---- B6a: PQ codebooks per subspace (raw vectors, product-quantization.cu:182-211) ----
   std::vector<float> pq_centers(M * K * pq_len, 0.0f);
@@ -424,43 +722,68 @@ void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* lab
   d_codes_orig.reserve(n * M);
   {
     std::vector<uint32_t> rows = sample_rows(n, std::max<uint64_t>(cfg_.pq_train, K), cfg_.seed ^ 0x5151);
-    DevBuf d_sub, d_label;
-    d_sub.reserve(std::max<uint64_t>(rows.size(), n) * pq_len * sizeof(float));
-    DevBuf d_train;
+    DevBuf d_train, d_sub;
     gather_rows_dev(dataset, d, rows, (uint32_t)d, d_train, s);
-    std::vector<uint8_t> codes_col;  // temporary
-    DevBuf d_lab32;
-    std::vector<uint32_t> h_l32(n);
-    std::vector<uint8_t> h_codes(n * M);
+    d_sub.reserve(rows.size() * pq_len * sizeof(float));
     for (uint64_t m = 0; m < M; ++m) {
       const uint64_t off = m * pq_len;
       const uint32_t w = (uint32_t)std::min<uint64_t>(pq_len, d - std::min(d, off));
-      std::vector<float> cent;
-      if (w == 0) {
-        cent.assign(K * pq_len, 0.0f);
-      } else {
-        // training slice [rows][w]
-        hipLaunchKernelGGL(slice_kernel, dim3((unsigned)ceil_div(rows.size() * w, 256)), dim3(256), 0, s,
-                           d_train.as<float>(), (uint64_t)rows.size(), d, off, w, d_sub.as<float>());
-        HDG_HIP(hipGetLastError());
-        std::vector<float> cw;
-        kmeans(d_sub.as<float>(), rows.size(), w, w, (uint32_t)K, (int)cfg_.kmeans_iters,
-               cfg_.seed * 1315423911ull + m, cw, s);
-        cent.assign(K * pq_len, 0.0f);
-        for (uint64_t c = 0; c < K; ++c)
-          for (uint32_t t = 0; t < w; ++t) cent[c * pq_len + t] = cw[c * w + t];
-        // encode all n rows of this subspace
-        hipLaunchKernelGGL(slice_kernel, dim3((unsigned)ceil_div(n * w, 256)), dim3(256), 0, s, dataset, n, d,
-                           off, w, d_sub.as<float>());
-        HDG_HIP(hipGetLastError());
-        assign_all(d_sub.as<float>(), n, w, w, cw, (uint32_t)K, d_lab32, s);
-        HDG_HIP(hipMemcpy(h_l32.data(), d_lab32.data(), n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-      }
-      for (uint64_t p = 0; p < n; ++p) h_codes[p * M + m] = w == 0 ? 0 : (uint8_t)h_l32[p];
-      std::copy(cent.begin(), cent.end(), pq_centers.begin() + m * K * pq_len);
+      if (w == 0) continue;  // an empty subspace keeps zero centers
+      hipLaunchKernelGGL(slice_kernel, dim3((unsigned)ceil_div(rows.size() * w, 256)), dim3(256), 0, s,
+                         d_train.as<float>(), (uint64_t)rows.size(), d, off, w, d_sub.as<float>());
+      HDG_HIP(hipGetLastError());
+      std::vector<float> cw;
+      kmeans(d_sub.as<float>(), rows.size(), w, w, (uint32_t)K, (int)cfg_.kmeans_iters,
+             cfg_.seed * 1315423911ull + m, cw, s);
+      for (uint64_t c = 0; c < K; ++c)
+        for (uint32_t t = 0; t < w; ++t) pq_centers[(m * K + c) * pq_len + t] = cw[c * w + t];
     }
-    upload(d_codes_orig, h_codes, s);
+    // encode all n rows: the one-pass encoder when a subspace fits its registers, else per subspace
+    std::vector<float> pq_norm(M * K);
+    for (uint64_t e = 0; e < M * K; ++e) {
+      double a = 0;
+      for (uint64_t t = 0; t < pq_len; ++t) a += (double)pq_centers[e * pq_len + t] * pq_centers[e * pq_len + t];
+      pq_norm[e] = (float)a;
+    }
+    DevBuf d_pqc, d_pqn;
+    upload(d_pqc, pq_centers, s);
+    upload(d_pqn, pq_norm, s);
+    const size_t lds = (K * pq_len + K) * sizeof(float);
+    const dim3 grid((unsigned)ceil_div(n, 256));
+    auto enc = [&](auto kern) {
+      hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, dataset, n, d, (uint32_t)M, (uint32_t)K, (uint32_t)pq_len,
+                         d_pqc.as<float>(), d_pqn.as<float>(), d_codes_orig.as<uint8_t>());
+      HDG_HIP(hipGetLastError());
+    };
+    if (pq_len <= 4) enc(pq_encode_kernel<4>);
+    else if (pq_len <= 8) enc(pq_encode_kernel<8>);
+    else if (pq_len <= 16) enc(pq_encode_kernel<16>);
+    else {
+      DevBuf d_all, d_lab32;
+      d_all.reserve(n * pq_len * sizeof(float));
+      for (uint64_t m = 0; m < M; ++m) {
+        const uint64_t off = m * pq_len;
+        const uint32_t w = (uint32_t)std::min<uint64_t>(pq_len, d - std::min(d, off));
+        std::vector<float> cw(K * (uint64_t)std::max<uint32_t>(w, 1), 0.0f);
+        if (w > 0) {
+          for (uint64_t c = 0; c < K; ++c)
+            for (uint32_t t = 0; t < w; ++t) cw[c * w + t] = pq_centers[(m * K + c) * pq_len + t];
+          hipLaunchKernelGGL(slice_kernel, dim3((unsigned)ceil_div(n * w, 256)), dim3(256), 0, s, dataset, n, d, off,
+                             w, d_all.as<float>());
+          HDG_HIP(hipGetLastError());
+          assign_all(d_all.as<float>(), n, w, w, cw, (uint32_t)K, d_lab32, s);
+        } else {
+          d_lab32.reserve(n * sizeof(uint32_t));
+          HDG_HIP(hipMemsetAsync(d_lab32.data(), 0, n * sizeof(uint32_t), s));
+        }
+        hipLaunchKernelGGL(code_column_kernel, grid, dim3(256), 0, s, d_lab32.as<uint32_t>(), n, (uint32_t)M,
+                           (uint32_t)m, d_codes_orig.as<uint8_t>());
+        HDG_HIP(hipGetLastError());
+      }
+    }
+    HDG_HIP(hipStreamSynchronize(s));
   }
+  stage_done("PQ codebooks and codes");
 
   // ---- B6b: fine k-means (CS centers), Super-Grid-IVF_PQ.cu:431-463 ----
   std::vector<float> fine;
@@ -473,8 +796,7 @@ void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* lab
            cfg_.seed + 7, fine, s);
     assign_all(dataset, n, d, (uint32_t)d, fine, (uint32_t)CS, d_fine_label, s);
   }
-  std::vector<uint32_t> fine_label(n);
-  HDG_HIP(hipMemcpy(fine_label.data(), d_fine_label.data(), n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  stage_done("fine k-means and assignment");
 
   // ---- B6c: group sub-centers into C clusters (stands in for kmeans_balanced, :466-476) ----
   std::vector<float> coarse;
@@ -487,6 +809,7 @@ void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* lab
     assign_all(d_fine.as<float>(), CS, d, (uint32_t)d, coarse, (uint32_t)C, d_g, s);
     HDG_HIP(hipMemcpy(group.data(), d_g.data(), CS * sizeof(uint32_t), hipMemcpyDeviceToHost));
   }
+  stage_done("sub-center grouping");
 
   // ---- cluster-major sub-cluster order (group_by_cluster_id keeps ascending ids), :481-504 ----
   std::vector<uint32_t> sub_order;  // position -> fine id
@@ -511,78 +834,100 @@ void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* lab
   for (uint64_t c = 0; c < C; ++c)
     for (uint32_t p = st_.h_sub_off[c]; p < st_.h_sub_off[c + 1]; ++p) cluster_of_sub[p] = (uint32_t)c;
 
-  // ---- B2/B5: cells and Morton codes; z-pool = per sub-cluster ascending (zcode, id) ----
-  std::vector<int> coords(n * l);
-  std::vector<uint64_t> zc(n);
-  for (uint64_t p = 0; p < n; ++p) {
-    int* cp = &coords[p * l];
-    for (int i = 0; i < l; ++i) {
-      int gcell = grid_cell(h_lab[p * l + g.shuffle[i]], g.left[i], g.span[i], g.dims[i]);
-      cp[i] = std::min(gcell, g.dims[i] - 1);
-    }
-    zc[p] = morton_code(cp, l, g.max_bits, g.valid_map);
-  }
+  // ---- B2/B5 on the device: cells and Morton codes; z-pool = per sub-cluster ascending (zcode, id) ----
+  // Two stable radix sorts: (zcode -> id) with ids ascending, then (sub-cluster -> id) on that order.
+  const int nv = g.n_valid;
   st_.h_seg_off.assign(CS + 1, 0);
   {
-    std::vector<uint64_t> cnt(CS, 0);
-    for (uint64_t p = 0; p < n; ++p) cnt[pos_of_fine[fine_label[p]]]++;
-    for (uint64_t s2 = 0; s2 < CS; ++s2) st_.h_seg_off[s2 + 1] = st_.h_seg_off[s2] + cnt[s2];
-    st_.h_zid.assign(n, 0);
-    std::vector<uint64_t> fillp(st_.h_seg_off.begin(), st_.h_seg_off.end() - 1);
-    for (uint64_t p = 0; p < n; ++p) st_.h_zid[fillp[pos_of_fine[fine_label[p]]]++] = (uint32_t)p;
-    for (uint64_t s2 = 0; s2 < CS; ++s2) {
-      auto b = st_.h_zid.begin() + st_.h_seg_off[s2], e = st_.h_zid.begin() + st_.h_seg_off[s2 + 1];
-      std::sort(b, e, [&](uint32_t a, uint32_t c) { return zc[a] != zc[c] ? zc[a] < zc[c] : a < c; });
-      st_.max_seg_len = std::max<uint32_t>(st_.max_seg_len, (uint32_t)(e - b));
+    GridArgs ga{};
+    ga.l = l;
+    ga.max_bits = g.max_bits;
+    ga.valid_map = g.valid_map;
+    ga.n_valid = nv;
+    for (int i = 0; i < kMaxDims; ++i) {
+      ga.dims[i] = g.dims[i];
+      ga.shuffle[i] = g.shuffle[i];
+      ga.valid_dims[i] = g.valid_dims[i];
+      ga.left[i] = g.left[i];
+      ga.span[i] = g.span[i];
     }
+    DevBuf d_pos_of_fine, cells_orig, zc, zc_s, seg, seg_s, ids, ids_s;
+    upload(d_pos_of_fine, pos_of_fine, s);
+    cells_orig.reserve((uint64_t)nv * n * 4);
+    zc.reserve(n * 8);
+    zc_s.reserve(n * 8);
+    seg.reserve(n * 4);
+    seg_s.reserve(n * 4);
+    ids.reserve(n * 4);
+    ids_s.reserve(n * 4);
+    const dim3 grid((unsigned)ceil_div(n, 256));
+    hipLaunchKernelGGL(cells_morton_kernel, grid, dim3(256), 0, s, labels, n, ga, d_fine_label.as<uint32_t>(),
+                       d_pos_of_fine.as<uint32_t>(), cells_orig.as<uint32_t>(), zc.as<uint64_t>(), seg.as<uint32_t>(),
+                       ids.as<uint32_t>());
+    HDG_HIP(hipGetLastError());
+    const int zbits = std::min(64, nv * g.max_bits);
+    sort_pairs(zc.as<uint64_t>(), zc_s.as<uint64_t>(), ids.as<uint32_t>(), ids_s.as<uint32_t>(), n, zbits, s);
+    // seg in (zcode, id) order, then the stable sort by sub-cluster: ids_s -> st_.zid
+    hipLaunchKernelGGL(gather_kernel<uint32_t>, grid, dim3(256), 0, s, seg.as<uint32_t>(), ids_s.as<uint32_t>(), n,
+                       ids.as<uint32_t>());  // ids now holds the gathered sub-cluster keys
+    HDG_HIP(hipGetLastError());
+    st_.zid.reserve(n * 4);
+    sort_pairs(ids.as<uint32_t>(), seg_s.as<uint32_t>(), ids_s.as<uint32_t>(), st_.zid.as<uint32_t>(), n,
+               key_bits(CS), s);
+    st_.seg_off.reserve((CS + 1) * 4);
+    hipLaunchKernelGGL(lower_bounds_kernel, dim3((unsigned)ceil_div(CS + 1, 256)), dim3(256), 0, s,
+                       seg_s.as<uint32_t>(), n, CS, st_.seg_off.as<uint32_t>());
+    HDG_HIP(hipGetLastError());
+    // per-point arrays in z-pool order
+    st_.zcode.reserve(n * 8);
+    hipLaunchKernelGGL(gather_kernel<uint64_t>, grid, dim3(256), 0, s, zc.as<uint64_t>(), st_.zid.as<uint32_t>(), n,
+                       st_.zcode.as<uint64_t>());
+    HDG_HIP(hipGetLastError());
+    st_.cells.reserve((uint64_t)nv * n * 4);
+    for (int j = 0; j < nv; ++j) {
+      hipLaunchKernelGGL(gather_kernel<uint32_t>, grid, dim3(256), 0, s, cells_orig.as<uint32_t>() + (uint64_t)j * n,
+                         st_.zid.as<uint32_t>(), n, st_.cells.as<uint32_t>() + (uint64_t)j * n);
+      HDG_HIP(hipGetLastError());
+    }
+    st_.lab.reserve(n * l * sizeof(float));
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)ceil_div(n, 4)), dim3(64, 4), 0, s,
+                       reinterpret_cast<const uint32_t*>(labels), (uint64_t)l, st_.zid.as<uint32_t>(), n, (uint32_t)l,
+                       st_.lab.as<uint32_t>());
+    HDG_HIP(hipGetLastError());
+    st_.vec.reserve(n * d * sizeof(float));
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)ceil_div(n, 4)), dim3(64, 4), 0, s,
+                       reinterpret_cast<const uint32_t*>(dataset), d, st_.zid.as<uint32_t>(), n, (uint32_t)d,
+                       st_.vec.as<uint32_t>());
+    HDG_HIP(hipGetLastError());
+    st_.codes.reserve(n * M);
+    hipLaunchKernelGGL(gather_bytes_kernel, dim3((unsigned)ceil_div(n * M, 256)), dim3(256), 0, s,
+                       d_codes_orig.as<uint8_t>(), st_.zid.as<uint32_t>(), n, (uint32_t)M, st_.codes.as<uint8_t>());
+    HDG_HIP(hipGetLastError());
+    // host copies for save and the per-query limits
+    std::vector<uint32_t> so(CS + 1);
+    st_.h_zid.resize(n);
     st_.h_zcode.resize(n);
-    for (uint64_t p = 0; p < n; ++p) st_.h_zcode[p] = zc[st_.h_zid[p]];
+    HDG_HIP(hipMemcpyAsync(so.data(), st_.seg_off.data(), (CS + 1) * 4, hipMemcpyDeviceToHost, s));
+    HDG_HIP(hipMemcpyAsync(st_.h_zid.data(), st_.zid.data(), n * 4, hipMemcpyDeviceToHost, s));
+    HDG_HIP(hipMemcpyAsync(st_.h_zcode.data(), st_.zcode.data(), n * 8, hipMemcpyDeviceToHost, s));
+    HDG_HIP(hipStreamSynchronize(s));
+    for (uint64_t i = 0; i <= CS; ++i) st_.h_seg_off[i] = so[i];
+    for (uint64_t i = 0; i < CS; ++i)
+      st_.max_seg_len = std::max<uint32_t>(st_.max_seg_len, (uint32_t)(so[i + 1] - so[i]));
   }
-
-  // ---- device arrays in z-pool order ----
-  std::vector<uint32_t> h_cells((uint64_t)g.n_valid * n);
-  std::vector<float> h_zlab(n * l);
-  for (uint64_t p = 0; p < n; ++p) {
-    uint32_t id = st_.h_zid[p];
-    for (int j = 0; j < g.n_valid; ++j) h_cells[(uint64_t)j * n + p] = (uint32_t)coords[(uint64_t)id * l + g.valid_dims[j]];
-    for (int i = 0; i < l; ++i) h_zlab[p * l + i] = h_lab[(uint64_t)id * l + i];
-  }
-  upload(st_.cells, h_cells, s);
-  build_bucket_grid(st_, h_cells, s);
+  stage_done("cells, Morton codes, z-pool sort and gathers");
+  build_bucket_grid(st_, s);
+  stage_done("bucket grid and occupancy sums");
   st_.labels_integral = std::all_of(h_lab.begin(), h_lab.end(), [](float v) { return v == std::floor(v); });
-  upload(st_.lab, h_zlab, s);
-  upload(st_.zcode, st_.h_zcode, s);
-  upload(st_.zid, st_.h_zid, s);
   upload(st_.pq_centers, pq_centers, s);
   upload(st_.centers, coarse, s);
   prepare_mixed_centers(st_, coarse, sub_centers, s);
   upload(st_.sub_centers, sub_centers, s);
   upload(st_.sub_off, st_.h_sub_off, s);
-  {
-    std::vector<uint32_t> so(CS + 1);
-    for (uint64_t i = 0; i <= CS; ++i) so[i] = (uint32_t)st_.h_seg_off[i];
-    upload(st_.seg_off, so, s);
-  }
   upload(st_.cluster_of_sub, cluster_of_sub, s);
-  // vectors and PQ codes gathered into z-pool order on the device
-  {
-    DevBuf d_zid;
-    upload(d_zid, st_.h_zid, s);
-    st_.vec.reserve(n * d * sizeof(float));
-    dim3 blk(64, 4);
-    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)ceil_div(n, 4)), blk, 0, s,
-                       reinterpret_cast<const uint32_t*>(dataset), d, d_zid.as<uint32_t>(), n, (uint32_t)d,
-                       st_.vec.as<uint32_t>());
-    HDG_HIP(hipGetLastError());
-    std::vector<uint8_t> h_codes(n * M), h_zcodes(n * M);
-    HDG_HIP(hipMemcpyAsync(h_codes.data(), d_codes_orig.data(), n * M, hipMemcpyDeviceToHost, s));
-    HDG_HIP(hipStreamSynchronize(s));
-    for (uint64_t p = 0; p < n; ++p)
-      std::memcpy(&h_zcodes[p * M], &h_codes[(uint64_t)st_.h_zid[p] * M], M);
-    upload(st_.codes, h_zcodes, s);
-  }
   prepare_blocked_codes(st_, s);
   HDG_HIP(hipStreamSynchronize(s));
+  stage_done("centers and blocked codes");
   st_.built = true;
 }
 

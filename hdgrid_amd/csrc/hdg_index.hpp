@@ -90,8 +90,9 @@ struct HostCheckpoint {
 // any inconsistency (shapes, offsets, permutations, value ranges).  Touches no device.
 HostCheckpoint read_checkpoint(const std::string& dir, int l_expected);
 
-// Bucket grid from the z-pool-ordered cells ([n_valid][N]) and sub-cluster offsets; sets the bucket fields.
-void build_bucket_grid(IndexState& st, const std::vector<uint32_t>& h_cells, hipStream_t s);
+// Bucket grid on the device from the z-pool-ordered cells (st.cells, [n_valid][N]) and sub-cluster offsets
+// (st.seg_off); sets the bucket fields.
+void build_bucket_grid(IndexState& st, hipStream_t s);
 
 // Fills codes_blk / codes_wb from the row-layout codes (device).
 void prepare_blocked_codes(IndexState& st, hipStream_t s);
