@@ -928,8 +928,14 @@ __device__ __forceinline__ float sq_chain4(float acc, float4 qq, float4 x) {
 // Rows of at least this many dims are streamed past the caches by the exact re-rank (measured: 1,536-byte rows
 // of the 1M x 384 workload gain, 512-byte rows of the 5M x 128 one, which neighbouring queries share, lose).
 constexpr uint64_t kStreamRowsMinDim = 256;
+// Largest vector dimension: the WIDE exact-distance kernel holds the query (padded to 512) in 32 KiB of LDS.
+constexpr uint64_t kMaxDim = 8192;
 
-template <int KIND, bool STREAM = false>
+// WIDE (d > 512): refine.cu runs min(d, 512) = 512 threads; thread t's sum over dims t, t + 512, ... is an fma
+// chain (nvcc contracts `sum += diff * diff`), and thread 0 adds the 512 partials in order.  The lane keeps the
+// 64 partials of one block of t in registers while the stages for t0 + 512 c (c = 0, 1, ...) pass through LDS,
+// then adds them to its total; the query vector sits in dynamic LDS, zero padded to a multiple of 512.
+template <int KIND, bool STREAM = false, bool WIDE = false>
 __global__ __launch_bounds__(64) void pair_dist_kernel(const float* __restrict__ queries, uint64_t q0, uint64_t d,
                                                        const float* __restrict__ X, const uint64_t* __restrict__ woff,
                                                        uint64_t qn, const uint32_t* __restrict__ list,
@@ -938,7 +944,9 @@ __global__ __launch_bounds__(64) void pair_dist_kernel(const float* __restrict__
                                                        const uint32_t* __restrict__ zid, uint64_t* __restrict__ keys,
                                                        const uint32_t* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
-  __shared__ __align__(16) float qv[512];
+  __shared__ __align__(16) float qv_s[WIDE ? 4 : 512];
+  extern __shared__ __align__(16) float qv_dyn[];
+  float* const qv = WIDE ? qv_dyn : qv_s;
   __shared__ __align__(16) float stage[64 * kStageLd];
   __shared__ uint32_t rows[64];
   const int lane = threadIdx.x;
@@ -956,7 +964,8 @@ __global__ __launch_bounds__(64) void pair_dist_kernel(const float* __restrict__
     const uint64_t lb = KIND == 1 ? loff[q] : q * lstride;
     const uint32_t n = nrows[q];
     if (q != cur) {
-      for (uint64_t t = lane; t < d; t += 64) qv[t] = queries[(q0 + q) * d + t];
+      const uint64_t qlen = WIDE ? (d + 511) / 512 * 512 : d;
+      for (uint64_t t = lane; t < qlen; t += 64) qv[t] = t < d ? queries[(q0 + q) * d + t] : 0.0f;
       cur = q;
     }
     const uint32_t j = (uint32_t)chunk * 64 + lane;
@@ -964,7 +973,70 @@ __global__ __launch_bounds__(64) void pair_dist_kernel(const float* __restrict__
     rows[lane] = myrow;
     __syncthreads();
     float acc = 0.0f;
-    if (vec4) {
+    if constexpr (WIDE) {
+      const uint32_t nper = (uint32_t)((d + 511) / 512), nst = 8 * nper;
+      // stage si covers dims [t0, t0 + 64) with t0 = 512 (si % nper) + 64 (si / nper)
+      auto stage_t0 = [&](uint32_t si) -> uint64_t { return (uint64_t)(si % nper) * 512 + (uint64_t)(si / nper) * 64; };
+      float4 pre[16];
+      auto load_stage = [&](uint64_t t0) {
+#pragma unroll
+        for (int it = 0; it < 16; ++it) {
+          const uint32_t e = lane + it * 64;
+          const uint32_t r = e >> 4, c4 = (e & 15) * 4;
+          const uint32_t row = rows[r];
+          float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          if (row != kNone) {
+            const float* src = X + (uint64_t)row * d + t0 + c4;
+            if (vec4) {
+              if (t0 + c4 < d) v = *reinterpret_cast<const float4*>(src);
+            } else {
+              if (t0 + c4 < d) v.x = src[0];
+              if (t0 + c4 + 1 < d) v.y = src[1];
+              if (t0 + c4 + 2 < d) v.z = src[2];
+              if (t0 + c4 + 3 < d) v.w = src[3];
+            }
+          }
+          pre[it] = v;
+        }
+      };
+      float part[64];
+      load_stage(stage_t0(0));
+      for (uint32_t si = 0; si < nst; ++si) {
+        const uint64_t t0 = stage_t0(si);
+#pragma unroll
+        for (int it = 0; it < 16; ++it) {
+          const uint32_t e = lane + it * 64;
+          *reinterpret_cast<float4*>(stage + (e >> 4) * kStageLd + (e & 15) * 4) = pre[it];
+        }
+        __syncthreads();
+        if (si + 1 < nst) load_stage(stage_t0(si + 1));
+        if (myrow != kNone) {
+          const float4* s4 = reinterpret_cast<const float4*>(stage + lane * kStageLd);
+          const float4* q4 = reinterpret_cast<const float4*>(qv + t0);
+          if (si % nper == 0) {
+#pragma unroll
+            for (int c = 0; c < 64; ++c) part[c] = 0.0f;
+          }
+#pragma unroll
+          for (int c = 0; c < 16; ++c) {
+            const float4 a = q4[c], b = s4[c];
+            float df = a.x - b.x;
+            part[4 * c] = fmaf(df, df, part[4 * c]);
+            df = a.y - b.y;
+            part[4 * c + 1] = fmaf(df, df, part[4 * c + 1]);
+            df = a.z - b.z;
+            part[4 * c + 2] = fmaf(df, df, part[4 * c + 2]);
+            df = a.w - b.w;
+            part[4 * c + 3] = fmaf(df, df, part[4 * c + 3]);
+          }
+          if (si % nper == nper - 1) {
+#pragma unroll
+            for (int c = 0; c < 64; ++c) acc = acc + part[c];
+          }
+        }
+        __syncthreads();
+      }
+    } else if (vec4) {
       float4 pre[16];
       auto load_stage = [&](uint64_t t0) {
 #pragma unroll
@@ -1028,6 +1100,19 @@ __global__ __launch_bounds__(64) void pair_dist_kernel(const float* __restrict__
     }
     if (myrow != kNone) keys[lb + j] = dev::make_key(acc, KIND == 1 ? zid[myrow] : j);
   }
+}
+
+// pair_dist_kernel launch: the WIDE form (dynamic LDS for the padded query) when d > 512.
+template <int KIND, bool STREAM = false>
+void launch_pair_dist(unsigned grid, hipStream_t s, const float* queries, uint64_t q0, uint64_t d, const float* X,
+                      const uint64_t* woff, uint64_t qn, const uint32_t* list, uint64_t lstride, const uint64_t* loff,
+                      const uint32_t* nrows, const uint32_t* zid, uint64_t* keys, const uint32_t* abort_flag) {
+  if (d > 512)
+    hipLaunchKernelGGL((pair_dist_kernel<KIND, STREAM, true>), dim3(grid), dim3(64), (d + 511) / 512 * 512 * 4, s,
+                       queries, q0, d, X, woff, qn, list, lstride, loff, nrows, zid, keys, abort_flag);
+  else
+    hipLaunchKernelGGL((pair_dist_kernel<KIND, STREAM, false>), dim3(grid), dim3(64), 0, s, queries, q0, d, X, woff,
+                       qn, list, lstride, loff, nrows, zid, keys, abort_flag);
 }
 
 // Exclusive scans in two passes over blocks of 1024 entries: scan_local_kernel scans every block and
@@ -3191,7 +3276,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   const uint32_t n_list = (uint32_t)std::min<uint64_t>(cfg_.n_list, C);
   const uint32_t sub_lists = (uint32_t)cfg_.sub_lists;
   const uint64_t topk = cfg_.topk;
-  if (d > 512) fail(kUnsupported, "vector dimension above 512");
+  if (d > kMaxDim) fail(kUnsupported, "vector dimension above " + std::to_string(kMaxDim));
   if (C > (uint64_t)dev::kSortCap) fail(kUnsupported, "clusters above 4096");
   if (cfg_.n_list > 1024) fail(kUnsupported, "n_list above 1024");
   if (topk > (uint64_t)dev::kSortCap || topk == 0) fail(kUnsupported, "topk must be in [1, 4096]");
@@ -3444,9 +3529,9 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   // ---- A5: cluster choice ----
   if (!mixed) {
     scan_u64(w.woff.as<uint64_t>(), nullptr, nullptr, qn, w.nnz_c.as<uint32_t>(), nullptr, 0u);
-    hipLaunchKernelGGL(pair_dist_kernel<0>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.centers.as<float>(),
-                       w.woff.as<uint64_t>(), qn, w.list_c.as<uint32_t>(), C, nullptr, w.nnz_c.as<uint32_t>(),
-                       nullptr, w.keys.as<uint64_t>(), nullptr);
+    launch_pair_dist<0>(dist_grid, s, queries, q0, d, S.centers.as<float>(), w.woff.as<uint64_t>(), qn,
+                        w.list_c.as<uint32_t>(), C, nullptr, w.nnz_c.as<uint32_t>(), nullptr, w.keys.as<uint64_t>(),
+                        nullptr);
     hipLaunchKernelGGL(coarse_pick_kernel, dim3((unsigned)qn), dim3(256), scap_c * 8, s, (uint32_t)C, n_list, G, q0,
                        scap_c,
                        w.nnz_c.as<uint32_t>(), w.gstat.as<unsigned long long>(), w.keys.as<uint64_t>(),
@@ -3492,9 +3577,9 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   mark();
   if (!mixed) {
     scan_u64(w.woff.as<uint64_t>(), nullptr, nullptr, qn, w.nnz_s.as<uint32_t>(), nullptr, 0u);
-    hipLaunchKernelGGL(pair_dist_kernel<0>, dim3(dist_grid), dim3(64), 0, s, queries, q0, d,
-                       S.sub_centers.as<float>(), w.woff.as<uint64_t>(), qn, w.list_s.as<uint32_t>(), (uint64_t)cap,
-                       nullptr, w.nnz_s.as<uint32_t>(), nullptr, w.keys.as<uint64_t>(), nullptr);
+    launch_pair_dist<0>(dist_grid, s, queries, q0, d, S.sub_centers.as<float>(), w.woff.as<uint64_t>(), qn,
+                        w.list_s.as<uint32_t>(), (uint64_t)cap, nullptr, w.nnz_s.as<uint32_t>(), nullptr,
+                        w.keys.as<uint64_t>(), nullptr);
     hipLaunchKernelGGL(sub_pick_kernel, dim3((unsigned)qn), dim3(256), scap_s * 8, s, sub_lists, G, q0, cap, scap_s,
                        w.nnz_s.as<uint32_t>(), w.gstat.as<unsigned long long>(), budget_S, w.keys.as<uint64_t>(),
                        w.chosen_s.as<uint32_t>(), w.n_chosen_s.as<uint32_t>());
@@ -3686,13 +3771,13 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   scan_u64(w.woff.as<uint64_t>(), nullptr, nullptr, qn, w.nL.as<uint32_t>(), w.mode.as<uint32_t>(), 2u);
   mark();
   if (d >= kStreamRowsMinDim)
-    hipLaunchKernelGGL((pair_dist_kernel<1, true>), dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.vec.as<float>(),
-                       w.woff.as<uint64_t>(), qn, w.L.as<uint32_t>(), (uint64_t)0, w.lsize.as<uint64_t>(),
-                       w.nL.as<uint32_t>(), S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>(), abort_p);
+    launch_pair_dist<1, true>(dist_grid, s, queries, q0, d, S.vec.as<float>(), w.woff.as<uint64_t>(), qn,
+                              w.L.as<uint32_t>(), (uint64_t)0, w.lsize.as<uint64_t>(), w.nL.as<uint32_t>(),
+                              S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>(), abort_p);
   else
-    hipLaunchKernelGGL((pair_dist_kernel<1, false>), dim3(dist_grid), dim3(64), 0, s, queries, q0, d, S.vec.as<float>(),
-                       w.woff.as<uint64_t>(), qn, w.L.as<uint32_t>(), (uint64_t)0, w.lsize.as<uint64_t>(),
-                       w.nL.as<uint32_t>(), S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>(), abort_p);
+    launch_pair_dist<1, false>(dist_grid, s, queries, q0, d, S.vec.as<float>(), w.woff.as<uint64_t>(), qn,
+                               w.L.as<uint32_t>(), (uint64_t)0, w.lsize.as<uint64_t>(), w.nL.as<uint32_t>(),
+                               S.zid.as<uint32_t>(), w.Lkeys.as<uint64_t>(), abort_p);
   mark();
   int64_t pick_skip = -1;
   if (cfg_.select_wave) {

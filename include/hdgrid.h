@@ -39,7 +39,7 @@ typedef int hdg_status;
 #define HDG_EHIP 4         /* HIP runtime error */
 #define HDG_ENOMEM 5       /* device allocation failed */
 #define HDG_ESTATE 6       /* call order (query before build, ...) */
-#define HDG_EUNSUPPORTED 7 /* outside the supported envelope (d > 512, clusters > 4096, ...) */
+#define HDG_EUNSUPPORTED 7 /* outside the supported envelope (d > 8192, clusters > 4096, ...) */
 
 /* Thread-local description of the last failure of a call on this thread. */
 const char* hdg_last_error(void);

@@ -214,6 +214,29 @@ def test_adc_code_widths(d, pq_dim):
         assert_parity(res, f"explicit ranges d={d} pq_dim={pq_dim} ADC_SPLIT={split}")
 
 
+@pytest.mark.parametrize("d,pq_dim", [(600, 150), (530, 106)])
+def test_wide_vectors(d, pq_dim):
+    """d > 512: the exact distances of the coarse, sub-cluster and re-rank stages take refine.cu's 512-thread
+    order (per-thread fma chains over dims t, t + 512, ..., partials added in order).  600 dims load as float4,
+    530 (not a multiple of 4) element by element.  Plain parity with the oracle, then the mixed path's
+    returned distances against the oracle's exact_l2."""
+    w = datasets.explicit_ranges(n=12000, nq=100, d=d, components=24, clusters=24, sub_clusters=5)
+    b = Built(w, PQ_DIM=pq_dim)
+    assert_parity(b.run(EXPS0=2), f"explicit ranges d={d}")
+    b.idx.set_param("PRECISION", 1)
+    dist, ids = b.idx.query(b.q, b.ql)
+    torch.cuda.synchronize()
+    dist, ids = dist.cpu().numpy(), ids.cpu().numpy().astype(np.uint64)  # pads: 2^32 or UINT64_MAX
+    b.idx.set_param("PRECISION", 0)
+    checked = 0
+    for r in range(0, 100, 7):
+        for j in range(0, 100, 9):
+            if ids[r, j] < np.uint64(w.data.shape[0]):
+                assert np.float32(O.exact_l2(w.queries[r], w.data[int(ids[r, j])])) == dist[r, j]
+                checked += 1
+    assert checked > 20
+
+
 def test_l3_dense_count(l3):
     """Every query through the dense Morton-range count instead of the bucket-grid walk."""
     assert_parity(l3.run(EXPS0=30, SPARSE_COUNT=0), "l3 dense count")

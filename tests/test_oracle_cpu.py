@@ -255,6 +255,57 @@ def test_exact_l2_sequential_order_and_fp64_bound():
             assert got == s
 
 
+def _fma32(a, b, c):
+    """fmaf by exact rational arithmetic: a * b + c rounded once to the nearest float32 (ties to even)."""
+    from fractions import Fraction
+    exact = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    cand = np.float32(float(exact))  # within one float32 step of the exact value
+    best = None
+    for v in (np.nextafter(cand, np.float32(-np.inf)), cand, np.nextafter(cand, np.float32(np.inf))):
+        err = abs(Fraction(float(v)) - exact)
+        if best is None or err < best[0] or (err == best[0] and (int(v.view(np.uint32)) & 1) == 0):
+            best = (err, v)
+    return np.float32(best[1])
+
+
+def test_exact_l2_wide_follows_512_thread_partials():
+    """d > 512 (refine.cu:58-73 with 512 threads): thread t's sum over dims t, t + 512, ... is an fma chain
+    (nvcc contracts `sum += diff * diff`), the partials are added in thread order.  Restated with an exact
+    fma; a sequential sum, unfused squares or a different partition each change some of these values."""
+    rng = np.random.default_rng(11)
+    hits = {"seq": 0, "nofma": 0}
+    for d, sparse in [(513, False), (600, False), (1030, False), (600, True), (1100, True)]:
+        for _ in range(4):
+            q = rng.standard_normal(d).astype(np.float32)
+            x = rng.standard_normal(d).astype(np.float32)
+            if sparse:  # differences only at a few dims t, t + 512, ...: the partials are not drowned in the total
+                keep = np.zeros(d, bool)
+                for t in rng.choice(d - 512, 3, replace=False):
+                    keep[t::512] = True
+                x = np.where(keep, x, q).astype(np.float32)
+            diff = (q - x).astype(np.float32)
+            total = np.float32(0)
+            for t in range(512):
+                part = np.float32(0)
+                for i in range(t, d, 512):
+                    part = _fma32(diff[i], diff[i], part)
+                total = np.float32(total + part)
+            got = np.float32(O.exact_l2(q, x))
+            assert got == total, f"d={d}"
+            seq = np.float32(0)
+            for i in range(d):
+                seq = np.float32(seq + np.float32(diff[i] * diff[i]))
+            hits["seq"] += int(seq != total)
+            nofma = np.float32(0)
+            for t in range(512):
+                part = np.float32(0)
+                for i in range(t, d, 512):
+                    part = np.float32(part + np.float32(diff[i] * diff[i]))
+                nofma = np.float32(nofma + part)
+            hits["nofma"] += int(nofma != total)
+    assert hits["seq"] > 0 and hits["nofma"] > 0, hits
+
+
 # ---- brute force ------------------------------------------------------------------------------------------
 def test_bruteforce_matches_numpy():
     rng = np.random.default_rng(6)
