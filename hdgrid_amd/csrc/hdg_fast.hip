@@ -39,22 +39,24 @@ __global__ void to_bf16_kernel(const float* __restrict__ queries, uint64_t q0, u
   out[i] = (uint16_t)f32_to_bf16(t < d ? queries[(q0 + q) * d + t] : 0.0f);
 }
 
-// S[q][c] = cnorm[c] - 2 * <q, c> with bf16 operands and f32 accumulation.  Block = 8 waves = 256 queries
-// x 128 centers (each center piece staged once for twice the queries); wave (wm, wn) owns 64 x 64 as 4 x 4
-// accumulators of v_mfma_f32_16x16x32_bf16 (lane l supplies A[l%16][8*(l/16)..+7] and B[8*(l/16)..+7][l%16],
-// receives D[4*(l/16)..+3][l%16]).  K advances in steps of 32.  The operand tiles of a step go from global
-// memory straight into LDS (global_load_lds, 16 bytes per lane) through a ring of three step buffers, so the
-// loads of the next two steps are in flight while a step's MFMAs run, without holding them in registers.
-// A step buffer is 16-byte pieces: row r's piece p (bf16 k0 + 8p .. +7) at slot 4r + (p ^ swz(r)) -- the
-// rows have no padding and the XOR swizzle keeps the fragment reads (16 rows x one piece per 16 lanes) on
-// distinct banks.
+// S[q][c] = cnorm[c] - 2 * <q, c> with bf16 operands and f32 accumulation.  Block = 8 waves = 128 queries
+// x 128 centers; wave (wm, wn) (4 x 2) owns 32 x 64 as 2 x 4 accumulators of v_mfma_f32_16x16x32_bf16 (lane l
+// supplies A[l%16][8*(l/16)..+7] and B[8*(l/16)..+7][l%16], receives D[4*(l/16)..+3][l%16]).  K advances in
+// steps of 32.  The operand tiles of a step go from global memory straight into LDS (global_load_lds, 16 bytes
+// per lane) through a ring of three step buffers, so the loads of the next two steps are in flight while a
+// step's MFMAs run, without holding them in registers.  A step buffer is 16-byte pieces: row r's piece p
+// (bf16 k0 + 8p .. +7) at slot 4r + (p ^ swz(r)) -- the rows have no padding and the XOR swizzle keeps the
+// fragment reads (16 rows x one piece per 16 lanes) on distinct banks.  The center norms are fetched before
+// the main loop, so the epilogue does not wait on them.  (Measured on 10k x 5,120 x 384 with 9% non-zero
+// compact rows: 256 x 128 tiles of 64 x 64 waves took 80 us, these 68 us; the main loop alone is ~50 us at
+// any tile shape tried, the rest is epilogue latency.)
 //
 // Tile order: workgroup w runs on XCD w % 8 as that XCD's (w / 8)-th workgroup.  w is mapped to a position
 // p = (w % 8) * per + w / 8 of a grouped order that takes kGemmGroup query tiles for each center tile, so the
 // workgroups resident on one XCD at a time cover a few query tiles x a dozen center tiles and find their
-// operand tiles in that XCD's L2 (one 256 x 128 tile reads 288 KiB of operands).
+// operand tiles in that XCD's L2.
 constexpr int kGemmTile = 128;   // centers per tile
-constexpr int kGemmTileQ = 256;  // queries per tile: 8 waves as 4 x 2
+constexpr int kGemmTileQ = 128;  // queries per tile: 8 waves as 4 x 2, 32 x 64 each
 constexpr int kGemmThreads = 512;
 constexpr int kGemmStages = 3;   // step buffers in the LDS ring
 constexpr int kGemmStepPieces = (kGemmTileQ + kGemmTile) * 4;  // 16-byte pieces of one K step
@@ -82,118 +84,103 @@ __global__ __launch_bounds__(512) void coarse_gemm_kernel(const uint16_t* __rest
   const uint32_t inner = p % span;
   // the step ring, reused by the epilogue as 8 waves x 16 rows x 65 floats
   static_assert(kGemmStages * kGemmStepPieces * 16 >= 8 * 16 * 65 * 4, "epilogue does not fit the ring");
+  static_assert(kGemmStepPieces == 2 * kGemmThreads && kGemmTileQ * 4 == kGemmThreads, "one A and one B piece per lane");
   __shared__ __align__(16) uint4 ring[kGemmStages * kGemmStepPieces];
-  // compact rows: the tile's 256 rows x 4 bit words of non-zero items and their ranks
+  // compact rows: the tile's 128 rows x 4 bit words of non-zero items and their ranks
   __shared__ uint32_t nz_bits[kGemmTileQ * 4], nz_rank[kGemmTileQ * 4];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave >> 1, wn = wave & 1;
   const uint64_t qbase = (uint64_t)(first + inner % gsize) * kGemmTileQ;
   const uint64_t cbase = (uint64_t)(inner / gsize) * kGemmTile;
-  // this lane's global sources: two A pieces (slots wave*64 + lane and (wave+8)*64 + lane) and one B piece
-  // (slot wave*64 + lane); rows past the operands read the last row (their results are never stored)
-  const uint16_t* srcA[2];
+  // this lane's global sources: one A piece (slot wave*64 + lane) and one B piece (slot (wave+8)*64 + lane,
+  // i.e. B slot wave*64 + lane); rows past the operands read the last row (their results are never stored)
+  const uint16_t* srcA;
   const uint16_t* srcB;
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const int sl = (wave + i * 8) * 64 + lane, r = sl >> 2, pc = (sl & 3) ^ gemm_swz(r);
-    const uint64_t q = qbase + r < qn ? qbase + r : qn - 1;
-    srcA[i] = qb16 + q * Dp + pc * 8;
-  }
   {
     const int sl = wave * 64 + lane, r = sl >> 2, pc = (sl & 3) ^ gemm_swz(r);
+    const uint64_t q = qbase + r < qn ? qbase + r : qn - 1;
+    srcA = qb16 + q * Dp + pc * 8;
     const uint64_t c = cbase + r < C ? cbase + r : C - 1;
     srcB = cb16 + c * Dp + pc * 8;
   }
   auto issue = [&](uint32_t ks) {
     uint4* buf = ring + (ks % kGemmStages) * kGemmStepPieces;
     const uint64_t k0 = (uint64_t)ks * 32;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-      __builtin_amdgcn_global_load_lds((global_cvoid*)(srcA[i] + k0), (lds_void*)(buf + (wave + i * 8) * 64), 16,
-                                       0, 0);
+    __builtin_amdgcn_global_load_lds((global_cvoid*)(srcA + k0), (lds_void*)(buf + wave * 64), 16, 0, 0);
     __builtin_amdgcn_global_load_lds((global_cvoid*)(srcB + k0), (lds_void*)(buf + kGemmTileQ * 4 + wave * 64), 16,
                                      0, 0);
   };
-  f32x4 acc[4][4];
+  f32x4 acc[2][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
-  // the compact-row words are fetched now and parked in LDS after the main loop
-  uint32_t nzv[4] = {0u, 0u, 0u, 0u};
+  // the compact-row words and the center norms are fetched now; the words are parked in LDS after the main loop
+  const int r16 = lane & 15, kg = lane >> 4;
+  uint32_t nzv[2] = {0u, 0u};
   if (nzb) {
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int idx = tid + i * kGemmThreads, r = idx >> 2, wd = idx & 3;
-      const uint64_t q = qbase + r;
-      const uint64_t wi = (cbase >> 5) + (uint64_t)wd;
-      if (q < qn && wi < nw) {
-        nzv[2 * i] = nzb[q * nw + wi];
-        nzv[2 * i + 1] = nzp[q * nw + wi];
-      }
+    const int r = tid >> 2, wd = tid & 3;
+    const uint64_t q = qbase + r;
+    const uint64_t wi = (cbase >> 5) + (uint64_t)wd;
+    if (q < qn && wi < nw) {
+      nzv[0] = nzb[q * nw + wi];
+      nzv[1] = nzp[q * nw + wi];
     }
   }
-  const uint32_t KS = (uint32_t)(Dp / 32);
-  issue(0);
-  if (KS > 1) issue(1);
-  const int r16 = lane & 15, kg = lane >> 4;
-  const int frag = r16 * 4 + (kg ^ gemm_swz(r16));  // slot of this lane's piece in a 16-row group
-  const uint32_t ring_lds = (uint32_t)(uintptr_t)(lds_void*)ring;
-  const uint32_t offA = (uint32_t)((wm * 64 * 4 + frag) * 16), offB = (uint32_t)((kGemmTileQ * 4 + wn * 64 * 4 + frag) * 16);
-  for (uint32_t ks = 0; ks < KS; ++ks) {
-    // step ks has landed once at most the next step's three loads of this lane are outstanding; the barrier
-    // makes every lane's pieces visible and ends every wave's reads of step ks - 1, whose buffer step ks + 2
-    // now refills.  Wait, barrier and fragment reads are written out (the compiler's fence and its alias
-    // check against the LDS loads in flight would wait for every outstanding load instead); the reads wait
-    // for their data in the same block.
-    if (ks + 1 < KS) asm volatile("s_waitcnt vmcnt(3)\n\ts_barrier" ::: "memory");
-    else asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
-    if (ks + 2 < KS) issue(ks + 2);
-    const uint32_t bufb = ring_lds + (ks % kGemmStages) * (kGemmStepPieces * 16);
-    bf16x8 a[4], b[4];
-    asm volatile(
-        "ds_read_b128 %0, %8\n\t"
-        "ds_read_b128 %1, %8 offset:1024\n\t"
-        "ds_read_b128 %2, %8 offset:2048\n\t"
-        "ds_read_b128 %3, %8 offset:3072\n\t"
-        "ds_read_b128 %4, %9\n\t"
-        "ds_read_b128 %5, %9 offset:1024\n\t"
-        "ds_read_b128 %6, %9 offset:2048\n\t"
-        "ds_read_b128 %7, %9 offset:3072\n\t"
-        "s_waitcnt lgkmcnt(0)"
-        : "=&v"(a[0]), "=&v"(a[1]), "=&v"(a[2]), "=&v"(a[3]), "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3])
-        : "v"(bufb + offA), "v"(bufb + offB)
-        : "memory");
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-  }
-  __syncthreads();  // every wave is done with the ring before the epilogue reuses it
   float cn[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const uint64_t c = cbase + wn * 64 + j * 16 + r16;
     cn[j] = c < C ? cnorm[c] : 0.0f;
   }
+  const uint32_t KS = (uint32_t)(Dp / 32);
+  issue(0);
+  if (KS > 1) issue(1);
+  const int frag = r16 * 4 + (kg ^ gemm_swz(r16));  // slot of this lane's piece in a 16-row group
+  const uint32_t ring_lds = (uint32_t)(uintptr_t)(lds_void*)ring;
+  const uint32_t offA = (uint32_t)((wm * 32 * 4 + frag) * 16), offB = (uint32_t)((kGemmTileQ * 4 + wn * 64 * 4 + frag) * 16);
+  for (uint32_t ks = 0; ks < KS; ++ks) {
+    // step ks has landed once at most the next step's two loads of this lane are outstanding; the barrier
+    // makes every lane's pieces visible and ends every wave's reads of step ks - 1, whose buffer step ks + 2
+    // now refills.  Wait, barrier and fragment reads are written out (the compiler's fence and its alias
+    // check against the LDS loads in flight would wait for every outstanding load instead); the reads wait
+    // for their data in the same block.
+    if (ks + 1 < KS) asm volatile("s_waitcnt vmcnt(2)\n\ts_barrier" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+    if (ks + 2 < KS) issue(ks + 2);
+    const uint32_t bufb = ring_lds + (ks % kGemmStages) * (kGemmStepPieces * 16);
+    bf16x8 a[2], b[4];
+    asm volatile(
+        "ds_read_b128 %0, %6\n\t"
+        "ds_read_b128 %1, %6 offset:1024\n\t"
+        "ds_read_b128 %2, %7\n\t"
+        "ds_read_b128 %3, %7 offset:1024\n\t"
+        "ds_read_b128 %4, %7 offset:2048\n\t"
+        "ds_read_b128 %5, %7 offset:3072\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(a[0]), "=&v"(a[1]), "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3])
+        : "v"(bufb + offA), "v"(bufb + offB)
+        : "memory");
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+  __syncthreads();  // every wave is done with the ring before the epilogue reuses it
   if (nzb) {
     // compact rows: only the query's non-zero items, at their rank among them.  The stores scatter anyway,
     // so each lane writes its accumulators directly: row kg*4 + t of block i, column j*16 + r16 of the wave's
     // 64, which lie in bit words wn*2 (j < 2) and wn*2 + 1 (j >= 2) of the tile's four
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int idx = tid + i * kGemmThreads;
-      nz_bits[idx] = nzv[2 * i];
-      nz_rank[idx] = nzv[2 * i + 1];
-    }
+    nz_bits[tid] = nzv[0];
+    nz_rank[tid] = nzv[1];
     __syncthreads();
     const uint32_t wd0 = (uint32_t)(wn * 2);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 2; ++i) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const uint32_t tr = (uint32_t)(wm * 64 + i * 16 + kg * 4 + t);
+        const uint32_t tr = (uint32_t)(wm * 32 + i * 16 + kg * 4 + t);
         const uint64_t q = qbase + tr;
         if (q >= qn) continue;
         const uint32_t w2[2] = {nz_bits[tr * 4 + wd0], nz_bits[tr * 4 + wd0 + 1]};
@@ -209,19 +196,19 @@ __global__ __launch_bounds__(512) void coarse_gemm_kernel(const uint16_t* __rest
     }
     return;
   }
-  // dense rows: each wave passes its 64 x 64 block through LDS 16 rows at a time, so that every store
+  // dense rows: each wave passes its 32 x 64 block through LDS 16 rows at a time, so that every store
   // writes 256 contiguous bytes of a row of S
   float* ep = reinterpret_cast<float*>(ring) + wave * 16 * 65;
   const uint64_t cw = cbase + wn * 64 + lane;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < 2; ++i) {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int t = 0; t < 4; ++t) ep[(kg * 4 + t) * 65 + j * 16 + r16] = cn[j] - 2.0f * acc[i][j][t];
     __syncthreads();
     for (int r = 0; r < 16; ++r) {
-      const uint64_t q = qbase + wm * 64 + i * 16 + r;
+      const uint64_t q = qbase + wm * 32 + i * 16 + r;
       if (q < qn && cw < C) S[q * ld + cw] = ep[r * 65 + lane];
     }
     __syncthreads();
