@@ -529,8 +529,11 @@ __global__ __launch_bounds__(256) void sparse_count_wave_kernel(
           to_dense = true;
           break;
         }
-        // four entries per lane per pass, so their loads are in flight together
-        for (uint32_t t0 = 0; t0 < total; t0 += 256) {
+        // four entries per lane per pass, so their loads are in flight together.  Once the in-box points
+        // overflow the wave buffer the query goes to the workgroup walk, which walks the box again: the
+        // entries are not tested further, but the remaining bucket rows are still summed, so that a box
+        // over max_tests entries goes to the dense count as before
+        for (uint32_t t0 = 0; t0 < total && ibn <= kWaveIb; t0 += 256) {
           uint32_t ev[4], cv[4][NV], sv[4], pv[4];
           // last k with pre_sh[k] <= t (that lane's range holds t): the four searches step together
           uint32_t a[4] = {0u, 0u, 0u, 0u};
@@ -3442,7 +3445,9 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     const uint32_t max_tests = (uint32_t)std::min<uint64_t>(cfg_.sparse_max_tests, 0xFFFFFFF0ull);
     const unsigned grid = (unsigned)std::min<uint64_t>(qn, 2048);
     const unsigned wgrid = (unsigned)std::min<uint64_t>(cdiv(qn, 4), 2048);
-    const unsigned rgrid = (unsigned)std::min<uint64_t>(qn, 256);  // the wave walk's few hand-overs
+    // the wave walk's hand-overs: few on most workloads, nearly every query when boxes hold thousands of
+    // points; workgroups without a hand-over exit at once, so the grid is sized for the many
+    const unsigned rgrid = grid;
     const uint32_t* cells = S.cells.as<uint32_t>();
     dim3 dgrid((unsigned)cdiv(qn, 256), (unsigned)cdiv(CS, kCountSegChunk));
     CoarseSums sums;
