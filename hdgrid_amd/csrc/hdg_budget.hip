@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -200,53 +201,120 @@ __global__ __launch_bounds__(256) void infer_kernel(uint64_t qn, int l, const fl
 }
 
 
-// Training forward + backward over a mini-batch; gradients accumulated into g (f32, atomics per block).
-__global__ __launch_bounds__(128) void train_step_kernel(const float* __restrict__ X, const float* __restrict__ Y,
+// Training forward + backward over a mini-batch, kTrainTile samples per workgroup of 256 threads.  Phase 1:
+// four threads per sample, thread t owning hidden units 16t .. 16t + 15 of each layer (every unit's sum runs
+// over its inputs in order, as in mlp_forward), leave the sample's bf16 inputs, activations and back-propagated
+// errors in LDS; the output and its error are one thread's sequential sum.  Phase 2: each thread takes
+// parameters and sums their gradient over the tile's samples in order; one global atomic per parameter and
+// workgroup.  Samples past the batch carry zero errors.  LDS: the weights plus six tile arrays (dynamic).
+constexpr int kTrainTile = 64;
+constexpr int kTrainThreads = 256;
+constexpr int kUnitsPer = kH / (kTrainThreads / kTrainTile);  // 16
+constexpr int kParamsPad = (kParams + 3) / 4 * 4;
+constexpr size_t kTrainLds = (size_t)(kParamsPad + kTrainTile * (kF + 4 * kH) + kTrainTile) * sizeof(float);
+
+__global__ __launch_bounds__(256) void train_step_kernel(const float* __restrict__ X, const float* __restrict__ Y,
                                                          const uint32_t* __restrict__ idx, uint32_t bsz,
                                                          const float* __restrict__ wq, float* __restrict__ g,
                                                          float* __restrict__ loss) {
-  __shared__ float w[kParams];
-  __shared__ float gs[kParams];
-  for (int i = threadIdx.x; i < kParams; i += blockDim.x) {
-    w[i] = wq[i];
-    gs[i] = 0.0f;
+  extern __shared__ __align__(16) float tsm[];
+  float* w = tsm;                                 // [kParams]
+  float* xs = w + kParamsPad;                     // [tile][kF] bf16-rounded inputs
+  float* h1s = xs + kTrainTile * kF;              // [tile][kH]
+  float* h2s = h1s + kTrainTile * kH;             // [tile][kH]
+  float* g2s = h2s + kTrainTile * kH;             // [tile][kH] errors at layer-2 pre-activations
+  float* g1s = g2s + kTrainTile * kH;             // [tile][kH] errors at layer-1 pre-activations
+  float* gys = g1s + kTrainTile * kH;             // [tile] output errors
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kParams; i += kTrainThreads) w[i] = wq[i];
+  const int sl = tid >> 2, part = tid & 3;        // sample of the tile, quarter of the units
+  const uint32_t b = blockIdx.x * kTrainTile + (uint32_t)sl;
+  const bool valid = b < bsz;
+  const uint32_t smp = valid ? idx[b] : 0u;
+  for (int f = part; f < kF; f += 4) xs[sl * kF + f] = valid ? bf16r(X[(uint64_t)smp * kF + f]) : 0.0f;
+  __syncthreads();
+  float z1[kUnitsPer], z2[kUnitsPer];
+#pragma unroll
+  for (int u = 0; u < kUnitsPer; ++u) {
+    const int j = part * kUnitsPer + u;
+    float a = w[kOffB1 + j];
+    for (int i = 0; i < kF; ++i) a += xs[sl * kF + i] * w[kOffW1 + i * kH + j];
+    z1[u] = a;
+    h1s[sl * kH + j] = valid ? bf16r(a > 0.0f ? a : 0.0f) : 0.0f;
   }
   __syncthreads();
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < bsz) {
-    const uint32_t s = idx[b];
-    float x[kF];
-    for (int i = 0; i < kF; ++i) x[i] = bf16r(X[(uint64_t)s * kF + i]);
-    float z1[kH], z2[kH];
-    const float y = mlp_forward(w, x, z1, z2);
-    const float err = y - Y[s];
-    atomicAdd(loss, err * err / (float)bsz);
-    const float gy = 2.0f * err / (float)bsz;
-    float h1[kH], h2[kH], g2[kH];
-    for (int j = 0; j < kH; ++j) {
-      h1[j] = bf16r(z1[j] > 0.0f ? z1[j] : 0.0f);
-      h2[j] = bf16r(z2[j] > 0.0f ? z2[j] : 0.0f);
-    }
-    atomicAdd(&gs[kOffB3], gy);
-    for (int j = 0; j < kH; ++j) {
-      atomicAdd(&gs[kOffW3 + j], gy * h2[j]);
-      g2[j] = z2[j] > 0.0f ? bf16r(gy * w[kOffW3 + j]) : 0.0f;
-    }
-    for (int i = 0; i < kH; ++i) {
-      float back = 0.0f;
-      for (int j = 0; j < kH; ++j) {
-        atomicAdd(&gs[kOffW2 + i * kH + j], h1[i] * g2[j]);
-        back += w[kOffW2 + i * kH + j] * g2[j];
-      }
-      const float g1 = z1[i] > 0.0f ? bf16r(back) : 0.0f;
-      atomicAdd(&gs[kOffB1 + i], g1);
-      for (int f = 0; f < kF; ++f) atomicAdd(&gs[kOffW1 + f * kH + i], x[f] * g1);
-    }
-    for (int j = 0; j < kH; ++j) atomicAdd(&gs[kOffB2 + j], g2[j]);
+#pragma unroll
+  for (int u = 0; u < kUnitsPer; ++u) {
+    const int j = part * kUnitsPer + u;
+    float a = w[kOffB2 + j];
+    for (int i = 0; i < kH; ++i) a += h1s[sl * kH + i] * w[kOffW2 + i * kH + j];
+    z2[u] = a;
+    h2s[sl * kH + j] = valid ? bf16r(a > 0.0f ? a : 0.0f) : 0.0f;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kParams; i += blockDim.x)
-    if (gs[i] != 0.0f) atomicAdd(&g[i], gs[i]);
+  if (part == 0) {
+    float gy = 0.0f;
+    if (valid) {
+      float y = w[kOffB3];
+      for (int i = 0; i < kH; ++i) y += h2s[sl * kH + i] * w[kOffW3 + i];
+      const float err = y - Y[smp];
+      atomicAdd(loss, err * err / (float)bsz);
+      gy = 2.0f * err / (float)bsz;
+    }
+    gys[sl] = gy;
+  }
+  __syncthreads();
+  const float gy = gys[sl];
+#pragma unroll
+  for (int u = 0; u < kUnitsPer; ++u) {
+    const int j = part * kUnitsPer + u;
+    g2s[sl * kH + j] = z2[u] > 0.0f ? bf16r(gy * w[kOffW3 + j]) : 0.0f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kUnitsPer; ++u) {
+    const int i = part * kUnitsPer + u;
+    float back = 0.0f;
+    for (int j = 0; j < kH; ++j) back += w[kOffW2 + i * kH + j] * g2s[sl * kH + j];
+    g1s[sl * kH + i] = z1[u] > 0.0f ? bf16r(back) : 0.0f;
+  }
+  __syncthreads();
+  // phase 2: parameter gradients summed over the tile's samples
+  for (int p = tid; p < kParams; p += kTrainThreads) {
+    float sum = 0.0f;
+    if (p < kOffB1) {
+      const int f = p / kH, i = p % kH;
+      for (int t = 0; t < kTrainTile; ++t) sum += xs[t * kF + f] * g1s[t * kH + i];
+    } else if (p < kOffW2) {
+      const int i = p - kOffB1;
+      for (int t = 0; t < kTrainTile; ++t) sum += g1s[t * kH + i];
+    } else if (p < kOffB2) {
+      const int i = (p - kOffW2) / kH, j = (p - kOffW2) % kH;
+      for (int t = 0; t < kTrainTile; ++t) sum += h1s[t * kH + i] * g2s[t * kH + j];
+    } else if (p < kOffW3) {
+      const int j = p - kOffB2;
+      for (int t = 0; t < kTrainTile; ++t) sum += g2s[t * kH + j];
+    } else if (p < kOffB3) {
+      const int j = p - kOffW3;
+      for (int t = 0; t < kTrainTile; ++t) sum += gys[t] * h2s[t * kH + j];
+    } else {
+      for (int t = 0; t < kTrainTile; ++t) sum += gys[t];
+    }
+    if (sum != 0.0f) atomicAdd(&g[p], sum);
+  }
+}
+
+void launch_train_step(const float* X, const float* Y, const uint32_t* idx, uint32_t bsz, const float* wq, float* g,
+                       float* loss, hipStream_t s) {
+  static const bool attr = [] {
+    HDG_HIP(hipFuncSetAttribute((const void*)train_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kTrainLds));
+    return true;
+  }();
+  (void)attr;
+  hipLaunchKernelGGL(train_step_kernel, dim3((bsz + kTrainTile - 1) / kTrainTile), dim3(kTrainThreads), kTrainLds, s,
+                     X, Y, idx, bsz, wq, g, loss);
+  HDG_HIP(hipGetLastError());
 }
 
 // Adam on the f32 master weights; writes the bf16-rounded working copy.
@@ -320,35 +388,45 @@ void Index::capture_features(uint64_t q0, uint64_t qn, int l, const float* range
   HDG_HIP(hipGetLastError());
 }
 
-static std::vector<float> recall_per_query(const std::vector<uint64_t>& res, const uint64_t* gt, uint64_t n,
-                                           uint64_t k) {
-  std::vector<float> r(n);
-  std::vector<uint64_t> a(k), b(k);
-  for (uint64_t q = 0; q < n; ++q) {
-    uint64_t nb = 0;
-    for (uint64_t j = 0; j < k; ++j) {
-      uint64_t x = gt[q * k + j];
-      if (x < (1ull << 63)) b[nb++] = x;
-    }
-    std::sort(b.begin(), b.begin() + nb);
-    nb = (uint64_t)(std::unique(b.begin(), b.begin() + nb) - b.begin());
-    for (uint64_t j = 0; j < k; ++j) a[j] = res[q * k + j];
-    std::sort(a.begin(), a.end());
-    uint64_t na = (uint64_t)(std::unique(a.begin(), a.end()) - a.begin());
-    uint64_t hit = 0, i = 0, j = 0;
-    while (i < na && j < nb) {
-      if (a[i] == b[j]) {
-        ++hit;
-        ++i;
-        ++j;
-      } else if (a[i] < b[j]) {
-        ++i;
-      } else {
-        ++j;
-      }
-    }
-    r[q] = nb ? (float)hit / (float)nb : 1.0f;
+// recall@k of every query on the device (one wave per query): distinct valid ground-truth ids (< 2^63) found among
+// the result ids, over the distinct valid ground-truth ids (1 when there are none).  Pairwise scans (k^2 per
+// query), which is nothing next to the query calls it scores.
+__global__ __launch_bounds__(256) void recall_kernel(const uint64_t* __restrict__ res, const uint64_t* __restrict__ gt,
+                                                     uint64_t n, uint64_t k, float* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t q = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= n) return;
+  const uint64_t* a = res + q * k;
+  const uint64_t* b = gt + q * k;
+  uint32_t hit = 0, nb = 0;
+  for (uint64_t j = (uint64_t)lane; j < k; j += 64) {
+    const uint64_t x = b[j];
+    bool first = x < (1ull << 63);
+    for (uint64_t i = 0; i < j && first; ++i) first = b[i] != x;
+    if (!first) continue;
+    ++nb;  // a distinct valid ground-truth id: found among the results?
+    bool found = false;
+    for (uint64_t i = 0; i < k && !found; ++i) found = a[i] == x;
+    hit += found ? 1u : 0u;
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    hit += (uint32_t)__shfl_xor((int)hit, o, 64);
+    nb += (uint32_t)__shfl_xor((int)nb, o, 64);
+  }
+  if (lane == 0) out[q] = nb ? (float)hit / (float)nb : 1.0f;
+}
+
+// recall_kernel over n queries: results on the device, ground truth on the device.
+static std::vector<float> recall_on_device(const uint64_t* d_res, const uint64_t* d_gt, uint64_t n, uint64_t k,
+                                           DevBuf& d_out, hipStream_t s) {
+  d_out.reserve(n * 4);
+  hipLaunchKernelGGL(recall_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, d_res, d_gt, n, k,
+                     d_out.as<float>());
+  HDG_HIP(hipGetLastError());
+  std::vector<float> r(n);
+  HDG_HIP(hipMemcpyAsync(r.data(), d_out.data(), n * 4, hipMemcpyDeviceToHost, s));
+  HDG_HIP(hipStreamSynchronize(s));
   return r;
 }
 
@@ -364,6 +442,15 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
   if (!budget_) budget_ = new BudgetModel();
   BudgetModel& B = *budget_;
   B.ready = false;
+  // phase wall times at log level 1 (HDG_LOG=1)
+  auto t_last = std::chrono::steady_clock::now();
+  auto phase_done = [&](const char* what) {
+    if (log_level() < 1) return;
+    HDG_HIP(hipStreamSynchronize(s));
+    const auto now = std::chrono::steady_clock::now();
+    log_line(1, "budget fit %s: %.1f ms", what, std::chrono::duration<double, std::milli>(now - t_last).count());
+    t_last = now;
+  };
   // ladder
   const uint32_t n_list = (uint32_t)std::min<uint64_t>(cfg_.n_list, st_.C);
   const uint32_t sub_lists = (uint32_t)cfg_.sub_lists;
@@ -386,15 +473,17 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
   d_dist.reserve(n * k * 4);
   d_ids.reserve(n * k * 8);
   d_feat.reserve(n * kF * 4);
-  std::vector<uint64_t> ids(n * k);
   std::vector<std::vector<float>> rec(P.levels);
+  DevBuf d_gt, d_rec;
+  d_gt.reserve(n * k * 8);
+  HDG_HIP(hipMemcpyAsync(d_gt.data(), gt_host, n * k * 8, hipMemcpyHostToDevice, s));
+  HDG_HIP(hipStreamSynchronize(s));
   try {
     for (int i = 0; i < P.levels; ++i) {
       cfg_.probe_clusters = P.t0 > 0 ? B.ladT[i] : 0;
       cfg_.sub_lists = B.ladS[i];
       query(queries, n, d, qlabels, qcols, d_dist.as<float>(), d_ids.as<uint64_t>(), s);
-      HDG_HIP(hipMemcpy(ids.data(), d_ids.data(), n * k * 8, hipMemcpyDeviceToHost));
-      rec[i] = recall_per_query(ids, gt_host, n, k);
+      rec[i] = recall_on_device(d_ids.as<uint64_t>(), d_gt.as<uint64_t>(), n, k, d_rec, s);
     }
     cfg_.precision = 1;
     B.sink = d_feat.as<float>();
@@ -407,6 +496,7 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
     throw;
   }
   cfg_ = saved;
+  phase_done("labelling runs and feature capture");
   // labels: smallest level reaching the target recall (the top level when none does)
   std::vector<float> y(n);
   for (uint64_t q = 0; q < n; ++q) {
@@ -442,6 +532,7 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
   }
   for (uint64_t q = 0; q < n; ++q)
     for (int f = 0; f < kF; ++f) X[q * kF + f] = (X[q * kF + f] - B.mean[f]) * B.inv_std[f];
+  phase_done("labels and standardisation");
   // init (He) and train on the GPU
   B.master.assign(kParams, 0.0f);
   {
@@ -478,29 +569,28 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
   for (int ep = 0; ep < P.epochs; ++ep) {
     std::shuffle(order.begin(), order.end(), rng);
     upload(dIdx, order, s);
-    float ep_loss = 0.0f;
     int nb = 0;
+    HDG_HIP(hipMemsetAsync(dLoss.data(), 0, 4, s));  // the epoch's batch losses add up here
     for (uint64_t b0 = 0; b0 + bsz <= n_train; b0 += bsz) {
       HDG_HIP(hipMemsetAsync(dG.data(), 0, kParams * 4, s));
-      HDG_HIP(hipMemsetAsync(dLoss.data(), 0, 4, s));
-      hipLaunchKernelGGL(train_step_kernel, dim3((bsz + 127) / 128), dim3(128), 0, s, dX.as<float>(), dY.as<float>(),
-                         dIdx.as<uint32_t>() + b0, bsz, dWq.as<float>(), dG.as<float>(), dLoss.as<float>());
+      launch_train_step(dX.as<float>(), dY.as<float>(), dIdx.as<uint32_t>() + b0, bsz, dWq.as<float>(), dG.as<float>(),
+                        dLoss.as<float>(), s);
       ++step;
       const float c1 = 1.0f - std::pow(0.9f, (float)step), c2 = 1.0f - std::pow(0.999f, (float)step);
       hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, dMaster.as<float>(),
                          dM.as<float>(), dV.as<float>(), dG.as<float>(), dWq.as<float>(), P.lr, 0.9f, 0.999f, 1e-8f,
                          c1, c2);
       HDG_HIP(hipGetLastError());
-      float lv = 0.0f;
-      HDG_HIP(hipMemcpyAsync(&lv, dLoss.data(), 4, hipMemcpyDeviceToHost, s));
-      HDG_HIP(hipStreamSynchronize(s));
-      ep_loss += lv;
       ++nb;
     }
-    last_loss = nb ? ep_loss / nb : 0.0f;
+    float lv = 0.0f;
+    HDG_HIP(hipMemcpyAsync(&lv, dLoss.data(), 4, hipMemcpyDeviceToHost, s));
+    HDG_HIP(hipStreamSynchronize(s));  // also orders the next epoch's index upload after this epoch's steps
+    last_loss = nb ? lv / nb : 0.0f;
   }
   HDG_HIP(hipMemcpy(B.master.data(), dMaster.data(), kParams * 4, hipMemcpyDeviceToHost));
   HDG_HIP(hipMemcpy(wq.data(), dWq.data(), kParams * 4, hipMemcpyDeviceToHost));
+  phase_done("training");
   // predictions for all samples (mean level of the holdout at the chosen margin)
   DevBuf dPred;
   dPred.reserve(n * 4);
@@ -534,10 +624,10 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
     std::memcpy(&hql[t * qcols], &allql[q * qcols], qcols * 4);
     std::memcpy(&hgt[t * k], &gt_host[q * k], k * 8);
   }
-  DevBuf dhq, dhql;
+  DevBuf dhq, dhql, dhgt;
   upload(dhq, hq, s);
   upload(dhql, hql, s);
-  std::vector<uint64_t> hids(n_hold_rows * k);
+  upload(dhgt, hgt, s);
   float chosen = P.max_margin;
   double hold_rec = 0.0, hold_lev = 0.0;
   cfg_.precision = 1;
@@ -549,8 +639,7 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
     for (float m = -P.max_margin; m <= P.max_margin + 1e-6f; m += 0.125f) {
       B.margin = m;
       query(dhq.as<float>(), n_hold_rows, d, dhql.as<float>(), qcols, d_dist.as<float>(), d_ids.as<uint64_t>(), s);
-      HDG_HIP(hipMemcpy(hids.data(), d_ids.data(), n_hold_rows * k * 8, hipMemcpyDeviceToHost));
-      const std::vector<float> r = recall_per_query(hids, hgt.data(), n_hold_rows, k);
+      const std::vector<float> r = recall_on_device(d_ids.as<uint64_t>(), dhgt.as<uint64_t>(), n_hold_rows, k, d_rec, s);
       double rsum = 0;
       for (float v : r) rsum += v;
       hold_rec = rsum / n_hold_rows;
@@ -563,6 +652,7 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
     throw;
   }
   cfg_ = saved;
+  phase_done("margin calibration");
   B.margin = chosen;
   for (uint64_t t = n_train; t < n; ++t) hold_lev += level_of(pred[perm[t]], chosen);
   hold_lev /= (double)n_hold_rows;
@@ -697,9 +787,8 @@ void budget_debug_grads(const float* params, const float* X, const float* Y, uin
   dL.reserve(4);
   HDG_HIP(hipMemset(dG.data(), 0, kParams * 4));
   HDG_HIP(hipMemset(dL.data(), 0, 4));
-  hipLaunchKernelGGL(train_step_kernel, dim3((n + 127) / 128), dim3(128), 0, nullptr, dX.as<float>(), dY.as<float>(),
-                     dIdx.as<uint32_t>(), n, dW.as<float>(), dG.as<float>(), dL.as<float>());
-  HDG_HIP(hipGetLastError());
+  launch_train_step(dX.as<float>(), dY.as<float>(), dIdx.as<uint32_t>(), n, dW.as<float>(), dG.as<float>(),
+                    dL.as<float>(), nullptr);
   HDG_HIP(hipDeviceSynchronize());
   HDG_HIP(hipMemcpy(grads, dG.data(), kParams * 4, hipMemcpyDeviceToHost));
   HDG_HIP(hipMemcpy(loss, dL.data(), 4, hipMemcpyDeviceToHost));

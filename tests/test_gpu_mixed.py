@@ -167,6 +167,19 @@ def test_budget_fit_gain(mixed_index):
     idx.set_param("PROBE_CLUSTERS", 0)
     rep = idx.budget_fit(m["q"][:ntr], m["ql"][:ntr], m["gt"][:ntr], levels=8, t0=0, s0=4, epochs=40, target=0.95)
     print("[budget] report", rep)
+    # the fit's per-query recall (on the device) at the top ladder level of the labelling path (plain, sub_lists
+    # round(4 * sqrt(2)^7) = 45) equals the per-query recall computed here, averaged
+    idx.set_param("PRECISION", 0)
+    idx.set_param("SUB_LISTS", 45)
+    _, i_top = idx.query(m["q"][:ntr], m["ql"][:ntr])
+    i_top = i_top.cpu().numpy().astype(np.uint64)
+    per_q = []
+    for g, r in zip(m["gt"][:ntr], i_top):
+        gs = {int(x) for x in g[:100] if int(x) < (1 << 63)}
+        per_q.append(len(gs & {int(x) for x in r}) / len(gs) if gs else 1.0)
+    assert abs(rep["fixed_recall"] - float(np.mean(per_q))) < 1e-5, (rep["fixed_recall"], float(np.mean(per_q)))
+    idx.set_param("SUB_LISTS", 64)
+    idx.set_param("PRECISION", 1)
     qe, qle, gte = m["q"][ntr:], m["ql"][ntr:], m["gt"][ntr:]
     ne = qe.shape[0]
     # fixed budget = the configured sub_lists (64), all non-zero clusters probed
