@@ -2015,20 +2015,45 @@ __global__ __launch_bounds__(256) void inbox_keep_wave_kernel(
   for (int i = 0; i < 2 * l; ++i) rg[i] = ranges[qi * 2 * l + i];
   const uint32_t n = n_ib[qi];
   const uint32_t md = mode[qi];
+  const bool filt = !(l == 1 && md != 3);
   uint32_t* out = L + loff[qi];
   uint32_t base = 0;
-  for (uint32_t b = 0; b < n; b += 64) {
-    const uint32_t i = b + (uint32_t)lane;
-    uint32_t pos = 0;
-    bool flag = false;
-    if (i < n) {
-      const uint32_t sid = ib_sub[qi * ibcap + i];
-      pos = ib_pos[qi * ibcap + i];
-      flag = ((bits[sid >> 5] >> (sid & 31)) & 1u) && ((l == 1 && md != 3) || label_pass(lab, l, pos, rg));
+  // kKeepBatches batches of 64 list entries per pass: their list loads, then the label loads of the entries in
+  // chosen sub-clusters, are in flight together; the kept entries are compacted batch by batch in list order
+  constexpr int kKeepBatches = 8;
+  for (uint32_t b0 = 0; b0 < n; b0 += 64u * kKeepBatches) {
+    uint32_t sid[kKeepBatches], pos[kKeepBatches];
+#pragma unroll
+    for (int u = 0; u < kKeepBatches; ++u) {
+      const uint32_t i = b0 + 64u * u + (uint32_t)lane;
+      sid[u] = i < n ? ib_sub[qi * ibcap + i] : kNone;
+      pos[u] = i < n ? ib_pos[qi * ibcap + i] : 0u;
     }
-    const uint64_t mask = __ballot(flag);
-    if (flag) out[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = pos;
-    base += (uint32_t)__popcll(mask);
+    bool flag[kKeepBatches];
+#pragma unroll
+    for (int u = 0; u < kKeepBatches; ++u)
+      flag[u] = sid[u] != kNone && ((bits[sid[u] >> 5] >> (sid[u] & 31)) & 1u);
+    if (filt) {
+      if (l == 4) {
+        float4 lv[kKeepBatches];
+#pragma unroll
+        for (int u = 0; u < kKeepBatches; ++u)
+          lv[u] = flag[u] ? reinterpret_cast<const float4*>(lab)[pos[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < kKeepBatches; ++u)
+          flag[u] = flag[u] && !(lv[u].x < rg[0] || lv[u].x > rg[1] || lv[u].y < rg[2] || lv[u].y > rg[3] ||
+                                 lv[u].z < rg[4] || lv[u].z > rg[5] || lv[u].w < rg[6] || lv[u].w > rg[7]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < kKeepBatches; ++u) flag[u] = flag[u] && label_pass(lab, l, pos[u], rg);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kKeepBatches; ++u) {
+      const uint64_t mask = __ballot(flag[u]);
+      if (flag[u]) out[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = pos[u];
+      base += (uint32_t)__popcll(mask);
+    }
   }
   if (lane == 0) nL[qi] = base;
 }
