@@ -331,15 +331,18 @@ __global__ void bucket_id_kernel(const uint32_t* __restrict__ cells, uint64_t N,
   pos[p] = (uint32_t)p;
 }
 
-// Entry e of the bucket grid: its cells and its sub-cluster, from the z-pool position epos[e].
+// Entry e of the bucket grid: its cells (type CT) and its sub-cluster, from the z-pool position epos[e]; with
+// shift >= 0 the sub-cluster and the position are packed into one word, sub << shift | (p - seg_off[sub]).
+template <class CT>
 __global__ void entry_fill_kernel(const uint32_t* __restrict__ cells, const uint32_t* __restrict__ seg_off,
-                                  uint64_t CS, uint64_t N, int nv, const uint32_t* __restrict__ epos,
-                                  uint32_t* __restrict__ ecell, uint32_t* __restrict__ esub) {
+                                  uint64_t CS, uint64_t N, int nv, const uint32_t* __restrict__ epos, int shift,
+                                  CT* __restrict__ ecell, uint32_t* __restrict__ esub) {
   const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= N) return;
   const uint32_t p = epos[e];
-  for (int j = 0; j < nv; ++j) ecell[(uint64_t)j * N + e] = cells[(uint64_t)j * N + p];
-  esub[e] = segment_of(seg_off, CS, p);
+  for (int j = 0; j < nv; ++j) ecell[(uint64_t)j * N + e] = (CT)cells[(uint64_t)j * N + p];
+  const uint32_t sub = segment_of(seg_off, CS, p);
+  esub[e] = shift >= 0 ? (sub << shift) | (p - seg_off[sub]) : sub;
 }
 
 // Coarse occupancy counts: t[sub][x] += 1 at x = sum_j (coarse cell_j + 1) * sstride[j].
@@ -614,13 +617,27 @@ void build_bucket_grid(IndexState& st, hipStream_t s) {
     hipLaunchKernelGGL(lower_bounds_kernel, dim3((unsigned)ceil_div(st.Nb + 1, 256)), dim3(256), 0, s,
                        bid_sorted.as<uint32_t>(), N, st.Nb, st.bstart.as<uint32_t>());
     HDG_HIP(hipGetLastError());
-    st.ecell.reserve((uint64_t)nv * N * 4);
+    // compact entries: u16 cells when every valid dim has at most 65536 cells; sub-cluster and position in one
+    // word when their bits fit (the walk then reads 4 bytes fewer per tested entry and lists 4 bytes per point)
+    st.cells16 = true;
+    for (int j = 0; j < nv; ++j) st.cells16 = st.cells16 && st.Mv[j] <= 65536u;
+    const int sub_bits = key_bits(st.CS), off_bits = key_bits(st.max_seg_len);
+    st.entries_packed = sub_bits + off_bits <= 32 && off_bits < 32;
+    st.pk_shift = st.entries_packed ? (uint32_t)off_bits : 0u;
+    const int shift = st.entries_packed ? off_bits : -1;
+    st.ecell.reserve((uint64_t)nv * N * (st.cells16 ? 2 : 4));
     st.esub.reserve(N * 4);
-    hipLaunchKernelGGL(entry_fill_kernel, dim3(blocks), dim3(256), 0, s, st.cells.as<uint32_t>(),
-                       st.seg_off.as<uint32_t>(), st.CS, N, nv, st.epos.as<uint32_t>(), st.ecell.as<uint32_t>(),
-                       st.esub.as<uint32_t>());
+    if (st.cells16)
+      hipLaunchKernelGGL(entry_fill_kernel<uint16_t>, dim3(blocks), dim3(256), 0, s, st.cells.as<uint32_t>(),
+                         st.seg_off.as<uint32_t>(), st.CS, N, nv, st.epos.as<uint32_t>(), shift,
+                         st.ecell.as<uint16_t>(), st.esub.as<uint32_t>());
+    else
+      hipLaunchKernelGGL(entry_fill_kernel<uint32_t>, dim3(blocks), dim3(256), 0, s, st.cells.as<uint32_t>(),
+                         st.seg_off.as<uint32_t>(), st.CS, N, nv, st.epos.as<uint32_t>(), shift,
+                         st.ecell.as<uint32_t>(), st.esub.as<uint32_t>());
     HDG_HIP(hipGetLastError());
     HDG_HIP(hipStreamSynchronize(s));
+    if (st.entries_packed) st.epos.release();
   }
   // coarse occupancy sums per sub-cluster, prefix-summed along each dim
   const uint64_t per = st.sstride[0] * (st.sg[0] + 1);

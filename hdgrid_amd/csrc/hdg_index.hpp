@@ -62,9 +62,13 @@ struct IndexState {
   uint64_t bstride[kMaxDims] = {0};
   uint64_t Nb = 0;
   DevBuf bstart;      // u32 [Nb+1]
-  DevBuf ecell;       // u32 [n_valid][N] cells of the entries
-  DevBuf esub;        // u32 [N] sub-cluster (table position) of the entries
-  DevBuf epos;        // u32 [N] z-pool position of the entries
+  DevBuf ecell;       // [n_valid][N] cells of the entries: u16 when cells16 (every valid dim <= 65536 cells), else u32
+  DevBuf esub;        // u32 [N] sub-cluster (table position) of the entries; packed entries: sub << pk_shift |
+                      // (z-pool position - seg_off[sub])
+  DevBuf epos;        // u32 [N] z-pool position of the entries (empty when the entries are packed)
+  bool cells16 = false;
+  bool entries_packed = false;  // bits of CS - 1 plus bits of max_seg_len - 1 fit 32
+  uint32_t pk_shift = 0;
   // coarse occupancy sums per sub-cluster for the dense count: coarse cell g_j = cell_j * sg[j] / Mv[j];
   // table [CS][sat_per] of inclusive prefix sums over the (sg + 1)^n_valid grid shifted by one per dim
   // (entry x counts the points with coarse cells < x); sat_per = 0 when the table would be too large

@@ -203,7 +203,17 @@ __global__ void prep_kernel(const float* __restrict__ qlabels, uint64_t qcols, u
 struct Buckets {
   uint32_t M[kMaxDims], nb[kMaxDims];
   uint64_t stride[kMaxDims];
+  uint32_t cells16;   // entry cells are u16 (else u32)
+  uint32_t packed;    // esub holds sub << pk_shift | (position - seg_off[sub]); epos is unused
+  uint32_t pk_shift;
 };
+
+// Cell of valid dim j of bucket entry e.
+__device__ __forceinline__ uint32_t entry_cell(const Buckets& B, const uint32_t* __restrict__ ecell, uint64_t N,
+                                               int j, uint32_t e) {
+  return B.cells16 ? (uint32_t)reinterpret_cast<const uint16_t*>(ecell)[(uint64_t)j * N + e]
+                   : ecell[(uint64_t)j * N + e];
+}
 
 constexpr uint64_t kMaxCombos = 4096;
 constexpr uint32_t kSparseMaxCS = 8192;  // LDS histogram: (CS + 2 CS/32) words per workgroup
@@ -300,9 +310,10 @@ __global__ __launch_bounds__(256) void sparse_count_kernel(
           for (int u = 0; u < 4; ++u) {
             const bool ok = ev[u] != kNone;
 #pragma unroll
-            for (int j = 0; j < NV; ++j) cv[u][j] = ok ? ecell[(uint64_t)j * N + ev[u]] : lo[j] - 1u;
-            sv[u] = ok ? esub[ev[u]] : 0u;
-            pv[u] = ok ? epos[ev[u]] : 0u;
+            for (int j = 0; j < NV; ++j) cv[u][j] = ok ? entry_cell(B, ecell, N, j, ev[u]) : lo[j] - 1u;
+            const uint32_t se = ok ? esub[ev[u]] : 0u;
+            sv[u] = B.packed ? se >> B.pk_shift : se;
+            pv[u] = B.packed ? se : (ok ? epos[ev[u]] : 0u);
           }
           bool in[4];
           uint32_t nin = 0;
@@ -323,7 +334,7 @@ __global__ __launch_bounds__(256) void sparse_count_kernel(
               if (atomicAdd(&hist[s2], 1u) == 0u) atomicOr(&bits[s2 >> 5], 1u << (s2 & 31));
               if (at < ibcap) {
                 ib_pos[qi * ibcap + at] = pv[u];
-                ib_sub[qi * ibcap + at] = s2;
+                if (!B.packed) ib_sub[qi * ibcap + at] = s2;
               }
               ++at;
             }
@@ -551,9 +562,10 @@ __global__ __launch_bounds__(256) void sparse_count_wave_kernel(
           for (int u = 0; u < 4; ++u) {
             const bool ok = ev[u] != kNone;
 #pragma unroll
-            for (int j = 0; j < NV; ++j) cv[u][j] = ok ? ecell[(uint64_t)j * N + ev[u]] : lo[j] - 1u;
-            sv[u] = ok ? esub[ev[u]] : 0u;
-            pv[u] = ok ? epos[ev[u]] : 0u;
+            for (int j = 0; j < NV; ++j) cv[u][j] = ok ? entry_cell(B, ecell, N, j, ev[u]) : lo[j] - 1u;
+            const uint32_t se = ok ? esub[ev[u]] : 0u;
+            sv[u] = B.packed ? se >> B.pk_shift : se;
+            pv[u] = B.packed ? se : (ok ? epos[ev[u]] : 0u);
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
@@ -566,7 +578,7 @@ __global__ __launch_bounds__(256) void sparse_count_wave_kernel(
             if (in) {
               if (at < ibcap) {
                 ib_pos[qi * ibcap + at] = pv[u];
-                ib_sub[qi * ibcap + at] = sv[u];
+                if (!B.packed) ib_sub[qi * ibcap + at] = sv[u];
               }
               if (at < kWaveIb) sbuf[at] = sv[u];
             }
@@ -1940,9 +1952,10 @@ __global__ __launch_bounds__(256) void inbox_keep_kernel(
     int l, int integral, uint32_t CS, uint32_t sub_lists, const uint32_t* __restrict__ mode,
     const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zlo_cnt,
     const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ chosen_sid, uint32_t ibcap,
-    const uint32_t* __restrict__ ib_pos, const uint32_t* __restrict__ ib_sub, const float* __restrict__ lab,
-    const float* __restrict__ ranges, const uint64_t* __restrict__ loff, uint32_t* __restrict__ L,
-    uint32_t* __restrict__ nL, const uint32_t* __restrict__ abort_flag) {
+    const uint32_t* __restrict__ ib_pos, const uint32_t* __restrict__ ib_sub, const uint32_t* __restrict__ seg_off,
+    uint32_t pk_shift, const float* __restrict__ lab, const float* __restrict__ ranges,
+    const uint64_t* __restrict__ loff, uint32_t* __restrict__ L, uint32_t* __restrict__ nL,
+    const uint32_t* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
   __shared__ uint32_t tmp[8];
   __shared__ uint32_t bits[kSparseMaxCS / 32];
@@ -1967,10 +1980,12 @@ __global__ __launch_bounds__(256) void inbox_keep_kernel(
     const uint32_t i = b + threadIdx.x;
     uint32_t pos = 0, flag = 0;
     if (i < n) {
-      const uint32_t sid = ib_sub[qi * ibcap + i];
-      pos = ib_pos[qi * ibcap + i];
-      flag = ((bits[sid >> 5] >> (sid & 31)) & 1u) && ((l == 1 && md != 3) || label_pass(lab, l, pos, rg)) ? 1u
-                                                                                                        : 0u;
+      const uint32_t v = ib_pos[qi * ibcap + i];
+      const uint32_t sid = ib_sub ? ib_sub[qi * ibcap + i] : v >> pk_shift;
+      if ((bits[sid >> 5] >> (sid & 31)) & 1u) {
+        pos = ib_sub ? v : seg_off[sid] + (v & ((1u << pk_shift) - 1u));
+        flag = ((l == 1 && md != 3) || label_pass(lab, l, pos, rg)) ? 1u : 0u;
+      }
     }
     uint32_t total;
     const uint32_t at = dev::wg_exclusive_scan(flag, tmp, total);
@@ -1980,7 +1995,7 @@ __global__ __launch_bounds__(256) void inbox_keep_kernel(
   if (threadIdx.x == 0) nL[qi] = base;
 }
 
-// inbox_keep_kernel with one wave per query (4 per workgroup), each with its own chosen-sub-cluster bit
+// inbox_keep_kernel with one wave per query (kKeepWaves per workgroup), each with its own chosen-sub-cluster bit
 // set in LDS; the kept points are compacted by ballots in list order, so the output equals the workgroup
 // kernel's.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -1988,17 +2003,25 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-__global__ __launch_bounds__(256) void inbox_keep_wave_kernel(
+constexpr int kKeepWaves = 8;  // queries (waves) per workgroup: they share the staged seg_off
+
+__global__ __launch_bounds__(64 * kKeepWaves) void inbox_keep_wave_kernel(
     uint64_t qn, int l, int integral, uint32_t CS, uint32_t sub_lists, const uint32_t* __restrict__ mode,
     const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zlo_cnt,
     const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ chosen_sid, uint32_t ibcap,
-    const uint32_t* __restrict__ ib_pos, const uint32_t* __restrict__ ib_sub, const float* __restrict__ lab,
-    const float* __restrict__ ranges, const uint64_t* __restrict__ loff, uint32_t* __restrict__ L,
-    uint32_t* __restrict__ nL, const uint32_t* __restrict__ abort_flag) {
+    const uint32_t* __restrict__ ib_pos, const uint32_t* __restrict__ ib_sub, const uint32_t* __restrict__ seg_off,
+    uint32_t pk_shift, const float* __restrict__ lab, const float* __restrict__ ranges,
+    const uint64_t* __restrict__ loff, uint32_t* __restrict__ L, uint32_t* __restrict__ nL,
+    const uint32_t* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
-  __shared__ uint32_t bits_all[4][kSparseMaxCS / 32];
+  __shared__ uint32_t bits_all[kKeepWaves][kSparseMaxCS / 32];
+  extern __shared__ uint32_t seg_lds[];  // packed entries: seg_off[0 .. CS) for the position of a kept entry
+  if (!ib_sub) {
+    for (uint32_t i = threadIdx.x; i < CS; i += blockDim.x) seg_lds[i] = seg_off[i];
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 63;
-  const uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint64_t qi = (uint64_t)blockIdx.x * kKeepWaves + (threadIdx.x >> 6);
   if (qi >= qn) return;
   if (!inbox_keeps(qi, l, integral, mode, n_ib, zlo, zlo_cnt)) return;
   uint32_t* bits = bits_all[threadIdx.x >> 6];
@@ -2026,13 +2049,19 @@ __global__ __launch_bounds__(256) void inbox_keep_wave_kernel(
 #pragma unroll
     for (int u = 0; u < kKeepBatches; ++u) {
       const uint32_t i = b0 + 64u * u + (uint32_t)lane;
-      sid[u] = i < n ? ib_sub[qi * ibcap + i] : kNone;
       pos[u] = i < n ? ib_pos[qi * ibcap + i] : 0u;
+      sid[u] = i >= n ? kNone : (ib_sub ? ib_sub[qi * ibcap + i] : pos[u] >> pk_shift);
     }
     bool flag[kKeepBatches];
 #pragma unroll
     for (int u = 0; u < kKeepBatches; ++u)
       flag[u] = sid[u] != kNone && ((bits[sid[u] >> 5] >> (sid[u] & 31)) & 1u);
+    if (!ib_sub) {
+      // packed entries: the position of the kept ones from their sub-cluster's z-pool offset
+#pragma unroll
+      for (int u = 0; u < kKeepBatches; ++u)
+        if (flag[u]) pos[u] = seg_lds[sid[u]] + (pos[u] & ((1u << pk_shift) - 1u));
+    }
     if (filt) {
       if (l == 4) {
         float4 lv[kKeepBatches];
@@ -3337,7 +3366,7 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   if (cfg_.inbox_cap < 512 || cfg_.inbox_cap > (1u << 20)) fail(kUnsupported, "INBOX_CAP must be in [512, 2^20]");
   const uint32_t ibcap = (uint32_t)cfg_.inbox_cap;
   w.ib_pos.reserve(qn * ibcap * 4);
-  w.ib_sub.reserve(qn * ibcap * 4);
+  if (!S.entries_packed) w.ib_sub.reserve(qn * ibcap * 4);  // packed entries list sub and position in one word
   w.n_ib.reserve(qn * 4);
   w.sub_list.reserve(qn * CS * 4);
   w.sub_cnt.reserve(qn * CS * 4);
@@ -3465,6 +3494,9 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
       B.nb[j] = S.nb[j];
       B.stride[j] = S.bstride[j];
     }
+    B.cells16 = S.cells16 ? 1u : 0u;
+    B.packed = S.entries_packed ? 1u : 0u;
+    B.pk_shift = S.pk_shift;
     const uint32_t nw = (uint32_t)((CS + 31) / 32);
     const size_t lds = (CS + 2 * nw) * sizeof(uint32_t);
     const uint32_t max_tests = (uint32_t)std::min<uint64_t>(cfg_.sparse_max_tests, 0xFFFFFFF0ull);
@@ -3697,18 +3729,20 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   w.akeys2.reserve(std::max<uint64_t>(totA, 1) * 8);
   // split mode keeps the keep-all candidates its pieces give (no in-box replacement)
   const int integral = n_split ? -1 : (S.labels_integral ? 1 : 0);
+  const uint32_t* ib_sub_arg = S.entries_packed ? nullptr : w.ib_sub.as<uint32_t>();
   if (sparse && cfg_.select_wave)
-    hipLaunchKernelGGL(inbox_keep_wave_kernel, dim3((unsigned)cdiv(qn, 4)), dim3(256), 0, s, qn, l, integral,
+    hipLaunchKernelGGL(inbox_keep_wave_kernel, dim3((unsigned)cdiv(qn, kKeepWaves)), dim3(64 * kKeepWaves),
+                       S.entries_packed ? CS * 4 : 0, s, qn, l, integral,
                        (uint32_t)CS, sub_lists, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
                        w.zlo_cnt.as<uint64_t>(), w.n_chosen_s.as<uint32_t>(), csid, ibcap,
-                       w.ib_pos.as<uint32_t>(), w.ib_sub.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(),
-                       w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
+                       w.ib_pos.as<uint32_t>(), ib_sub_arg, S.seg_off.as<uint32_t>(), S.pk_shift, S.lab.as<float>(),
+                       w.ranges.as<float>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
   else if (sparse)
     hipLaunchKernelGGL(inbox_keep_kernel, dim3((unsigned)qn), dim3(256), 0, s, l, integral, (uint32_t)CS, sub_lists,
                        w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(), w.zlo_cnt.as<uint64_t>(),
-                       w.n_chosen_s.as<uint32_t>(), csid, ibcap, w.ib_pos.as<uint32_t>(),
-                       w.ib_sub.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
-                       w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
+                       w.n_chosen_s.as<uint32_t>(), csid, ibcap, w.ib_pos.as<uint32_t>(), ib_sub_arg,
+                       S.seg_off.as<uint32_t>(), S.pk_shift, S.lab.as<float>(), w.ranges.as<float>(),
+                       w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
   if (!all_walked)
   hipLaunchKernelGGL(keep_all_kernel, dim3((unsigned)qn), dim3(256), (2 * R + 1) * 4, s, qn, l, integral,
                      R, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
