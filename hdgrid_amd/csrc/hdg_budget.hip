@@ -8,8 +8,8 @@
 // scores among the query's non-zero clusters, plus |q|^2 (32); standardised with training mean / std.
 // Training: BF16 operands (inputs, weights and activations rounded to bf16), FP32 accumulation, FP32
 // master weights, Adam, MSE loss.  Inference adds a margin: the smallest one at which the deployed path (mixed
-// precision with the model) reaches the target mean recall on a held-out share of the training queries; a
-// non-finite prediction falls back to the fixed budget (the highest level).
+// precision with the model) reaches the target mean recall plus one standard error of that mean on a held-out
+// share of the training queries; a non-finite prediction falls back to the fixed budget (the highest level).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -640,11 +640,18 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
       B.margin = m;
       query(dhq.as<float>(), n_hold_rows, d, dhql.as<float>(), qcols, d_dist.as<float>(), d_ids.as<uint64_t>(), s);
       const std::vector<float> r = recall_on_device(d_ids.as<uint64_t>(), dhgt.as<uint64_t>(), n_hold_rows, k, d_rec, s);
-      double rsum = 0;
-      for (float v : r) rsum += v;
+      double rsum = 0, rsq = 0;
+      for (float v : r) {
+        rsum += v;
+        rsq += (double)v * v;
+      }
       hold_rec = rsum / n_hold_rows;
+      // the holdout mean must clear the target by one standard error, so that the recall on unseen queries of
+      // the same distribution meets it more often than not despite the holdout's sampling noise
+      const double var = std::max(0.0, rsq / n_hold_rows - hold_rec * hold_rec);
+      const double se = std::sqrt(var / (double)n_hold_rows);
       chosen = m;
-      if (hold_rec >= P.target) break;
+      if (hold_rec >= (double)P.target + se) break;
     }
   } catch (...) {
     cfg_ = saved;

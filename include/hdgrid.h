@@ -140,9 +140,9 @@ hdg_status hdg_last_stage_ms(const hdg_index* index, char* names, uint64_t cap, 
  * (BUDGET_LABELS = 0, the default) or the mixed one (1).  Features per query: filter widths, grid-cell extents
  * and the 32 smallest BF16 coarse-centroid scores among its non-zero clusters (+ |q|^2); at query time the
  * budget trims the score-sorted sub-cluster choice.  The margin added to the prediction is the smallest one at
- * which the deployed path (PRECISION 1 with the model) reaches `target` mean recall on the held-out 10% of the
- * training queries.  report receives 8 floats: final loss, margin, holdout recall, holdout mean level, mean
- * label, recall of the top level, n_train, n_holdout. */
+ * which the deployed path (PRECISION 1 with the model) reaches `target` mean recall plus one standard error of
+ * that mean on the held-out 10% of the training queries.  report receives 8 floats: final loss, margin, holdout
+ * recall, holdout mean level, mean label, recall of the top level, n_train, n_holdout. */
 hdg_status hdg_budget_fit(hdg_index* index, const float* queries, uint64_t n, uint64_t dim,
                           const float* query_labels, uint64_t qlabel_cols, const uint64_t* gt_ids, uint64_t k,
                           int levels, double t0, double s0, double ratio, float target, int epochs, float lr,
