@@ -77,6 +77,9 @@ _budget_nparams = _sig("hdg_budget_param_count", ctypes.c_int, [])
 _budget_feats = _sig("hdg_budget_debug_features", ctypes.c_int, [_vp, _vp, _u64, _u64, _vp, _u64, _fp, _vp])
 _coarse_scores = _sig("hdg_debug_coarse_scores", ctypes.c_int, [_vp, _vp, _u64, _fp, _vp])
 _merge = _sig("hdg_merge_topk", ctypes.c_int, [_vp, _vp, _u64, _u64, _u64, _u64, _u64, _u64, _vp, _vp, _vp])
+_kmeans_bal = _sig("hdg_debug_kmeans_balanced", ctypes.c_int,
+                  [_vp, _u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _fp, ctypes.POINTER(ctypes.c_uint32),
+                   ctypes.c_int, _vp])
 _fit_coeff = _sig("hdg_fit_mem_coeff", ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)])
 _autotune = _sig("hdg_autotune", ctypes.c_int,
                  [_vp, _vp, _u64, _u64, _vp, _u64, _vp, _vp, ctypes.c_uint32, _cp, _u64, _vp])
@@ -157,6 +160,20 @@ def merge_topk(dist, ids, data_batch_size: int, n_data: int, start_offset: int =
     _check(_merge(_ptr(dist), _ptr(ids), nb, data_batch_size, n_data, start_offset, nq, k, _ptr(od), _ptr(oi),
                   _stream_ptr(stream)))
     return od, oi
+
+
+def kmeans_balanced(rows, k: int, n_iters: int, stream=None):
+    """The build's balanced k-means (the grouping of the fine centers) on rows [n][w] (a float32 device
+    tensor): returns (centers [k][w], labels [n]) as numpy arrays."""
+    import numpy as np
+    rows = rows.contiguous()
+    n, w = rows.shape
+    centers = np.zeros((k, w), np.float32)
+    labels = np.zeros(n, np.uint32)
+    _check(_kmeans_bal(_ptr(rows), n, w, k, n_iters, centers.ctypes.data_as(_fp),
+                       labels.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), rows.device.index or 0,
+                       _stream_ptr(stream)))
+    return centers, labels
 
 
 def validate_config_text(conf_text: str, filter_text: str) -> str:

@@ -138,15 +138,39 @@ __global__ void accumulate_kernel(const float* __restrict__ X, uint64_t n, uint6
   if (threadIdx.x == 0) atomicAdd(&counts[c], 1ull);
 }
 
-// centers[c][t] = sums[c][t] / scale / counts[c] for the non-empty clusters.
+// centers[c][t] = sums[c][t] / scale / counts[c] for the non-empty clusters; an empty cluster keeps its
+// center, or gets the zero vector when zero_empty is set (the balanced k-means' division rule).
 __global__ void center_update_kernel(const unsigned long long* __restrict__ sums,
                                      const unsigned long long* __restrict__ counts, uint32_t k, uint32_t w,
-                                     double inv_scale, float* __restrict__ centers) {
+                                     double inv_scale, float* __restrict__ centers, int zero_empty = 0) {
   const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (uint64_t)k * w) return;
   const unsigned long long cnt = counts[e / w];
-  if (cnt == 0) return;
+  if (cnt == 0) {
+    if (zero_empty) centers[e] = 0.0f;
+    return;
+  }
   centers[e] = (float)((double)(long long)sums[e] * inv_scale / (double)cnt);
+}
+
+// label[i] = i mod k: the balanced k-means' starting assignment.
+__global__ void mod_label_kernel(uint32_t* __restrict__ label, uint64_t n, uint32_t k) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) label[i] = (uint32_t)(i % k);
+}
+
+// One balancing move per blockIdx.y: centers[l] = (wc * centers[li] + X[i]) / (wc + 1), each step rounded
+// to f32.  The moved clusters are small and the sources large, so no block reads a center another writes.
+__global__ void adjust_move_kernel(float* __restrict__ centers, uint32_t w, const uint32_t* __restrict__ moves,
+                                   const float* __restrict__ wcs, const float* __restrict__ X, uint64_t ld) {
+  const uint32_t m = blockIdx.y;
+  const uint32_t l = moves[3 * m], li = moves[3 * m + 1], i = moves[3 * m + 2];
+  const float wc = wcs[m];
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < w; t += gridDim.x * blockDim.x) {
+    float v = __fmul_rn(wc, centers[(uint64_t)li * w + t]);
+    v = __fadd_rn(v, X[(uint64_t)i * ld + t]);
+    centers[(uint64_t)l * w + t] = __fdiv_rn(v, __fadd_rn(wc, 1.0f));
+  }
 }
 
 // |c|^2 of each center, summed in FP64 over the dims in order (one thread per center).
@@ -525,6 +549,239 @@ void assign_all(const float* X, uint64_t n, uint64_t ld, uint32_t w, const std::
   HDG_HIP(hipStreamSynchronize(s));
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// Balanced hierarchical k-means: RAFT's kmeans_balanced::fit_predict as Super-Grid-IVF_PQ.cu:466-476 calls it
+// (L2, n_iters 200) to group the clusters*sub_clusters fine centers into `clusters` groups.  RAFT is not
+// vendored (rapids-cmake fetches it, CMakeLists.txt:18-34); this follows its published algorithm:
+//   fit = mesoclusters (round(sqrt(k)) of them) trained on all rows; fine cluster counts handed to the
+//         mesoclusters in proportion to their sizes; each mesocluster's rows clustered into its share; then
+//         max(n_iters / 10, 2) balancing EM iterations over all rows with all k centers;
+//   predict = nearest center.
+// Training a set of clusters starts from labels i mod k and their means, then runs balancing EM: before
+// every E step but the first, clusters of at most floor(average * threshold) rows (average = n / k) are
+// moved next to a row of a cluster of at least `average` rows (centers[l] = (w * centers[li] + x_i) / (w + 1),
+// w = min(size, 7)); every `pullback`-th iteration with such a move adds an iteration.  The row is found by
+// stepping a cursor through the rows with a prime stride that does not divide n (the cursor and the stride
+// index carry over between calls within one fit, in RAFT across the whole process).
+// Arithmetic as this file's k-means: assignment by ||c||^2 - 2 x.c (ties to the lower id), fixed-point sums
+// divided in FP64 (RAFT sums in f32), an empty cluster's center set to zero (RAFT divides by a zero count to
+// zero).  Deterministic: the same rows and settings give the same centers and labels bit for bit.
+void gather_rows_dev(const float* X, uint64_t ld_floats, const std::vector<uint32_t>& rows, uint32_t w,
+                     DevBuf& out, hipStream_t s);
+
+struct BalancedFit {
+  hipStream_t s;
+  uint64_t cursor = 0;  // row cursor of the balancing moves
+  uint32_t prime_j = 0;  // index of the current stride prime
+  DevBuf d_norm, d_sums, d_cnt, d_moves, d_wc;
+  std::vector<unsigned long long> cnt;
+  std::vector<uint32_t> lab;
+};
+
+// Stride primes: the smallest prime >= 29 + 70 j for j = 0..39 (29 ... 2,767).
+uint64_t stride_prime(uint32_t j) {
+  uint64_t p = 29 + 70ull * j;
+  for (;; ++p) {
+    bool prime = p > 1;
+    for (uint64_t q = 2; q * q <= p && prime; ++q) prime = p % q != 0;
+    if (prime) return p;
+  }
+}
+constexpr uint32_t kStridePrimes = 40;
+constexpr float kAdjustWeight = 7.0f;
+
+double fixed_point_scale(const float* X, uint64_t n, uint64_t ld, uint32_t w, hipStream_t s) {
+  DevBuf d_max;
+  upload(d_max, std::vector<unsigned int>{0u}, s);
+  if (n > 0 && w > 0) {
+    hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)std::min<uint64_t>(4096, ceil_div(n * w, 256))), dim3(256), 0,
+                       s, X, n, ld, w, d_max.as<unsigned int>());
+    HDG_HIP(hipGetLastError());
+  }
+  unsigned int maxbits = 0;
+  HDG_HIP(hipMemcpyAsync(&maxbits, d_max.data(), sizeof(maxbits), hipMemcpyDeviceToHost, s));
+  HDG_HIP(hipStreamSynchronize(s));
+  float maxabs;
+  std::memcpy(&maxabs, &maxbits, sizeof(maxabs));
+  if (!std::isfinite(maxabs)) fail(kInvalidArgument, "k-means input holds a non-finite value");
+  int e = 0;
+  std::frexp((double)maxabs * (double)n, &e);  // n * max|x| < 2^e
+  return maxabs > 0.0f ? std::ldexp(1.0, 62 - e) : 1.0;
+}
+
+// E step: label = nearest center.
+void bal_predict(BalancedFit& f, const float* X, uint64_t n, uint64_t ld, uint32_t w, uint32_t k,
+                 const float* centers, uint32_t* label) {
+  f.d_norm.reserve(k * sizeof(float));
+  hipLaunchKernelGGL(center_norm_kernel, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, f.s, centers, k, w,
+                     f.d_norm.as<float>());
+  HDG_HIP(hipGetLastError());
+  launch_assign(X, n, ld, w, centers, k, f.d_norm.as<float>(), label, nullptr, f.s);
+}
+
+// M step: centers = means of their rows (zero when empty), sizes in f.d_cnt.
+void bal_centers(BalancedFit& f, const float* X, uint64_t n, uint64_t ld, uint32_t w, uint32_t k,
+                 const uint32_t* label, double scale, float* centers) {
+  f.d_sums.reserve((size_t)k * w * sizeof(unsigned long long));
+  f.d_cnt.reserve(k * sizeof(unsigned long long));
+  HDG_HIP(hipMemsetAsync(f.d_sums.data(), 0, (size_t)k * w * sizeof(unsigned long long), f.s));
+  HDG_HIP(hipMemsetAsync(f.d_cnt.data(), 0, k * sizeof(unsigned long long), f.s));
+  if (n > 0) {
+    hipLaunchKernelGGL(accumulate_kernel, dim3((unsigned)ceil_div(n, 8)), dim3(32, 8), 0, f.s, X, n, ld, w, label,
+                       scale, f.d_sums.as<unsigned long long>(), f.d_cnt.as<unsigned long long>());
+    HDG_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(center_update_kernel, dim3((unsigned)ceil_div((uint64_t)k * w, 256)), dim3(256), 0, f.s,
+                     f.d_sums.as<unsigned long long>(), f.d_cnt.as<unsigned long long>(), k, w, 1.0 / scale,
+                     centers, 1);
+  HDG_HIP(hipGetLastError());
+}
+
+// Balancing move of every small cluster; true when a center moved.
+bool bal_adjust(BalancedFit& f, const float* X, uint64_t n, uint64_t ld, uint32_t w, uint32_t k,
+                const uint32_t* label, float threshold, float* centers) {
+  if (k == 0 || n == 0) return false;
+  // threshold < 1 keeps the moved clusters (<= floor(average * threshold) rows) apart from the source clusters
+  // (>= average rows), so the moves can run in parallel
+  if (!(threshold < 1.0f)) fail(kBadState, "balancing threshold must be below 1");
+  const uint64_t average = n / k;
+  uint64_t ofst;
+  do {
+    f.prime_j = (f.prime_j + 1) % kStridePrimes;
+    ofst = stride_prime(f.prime_j);
+  } while (n % ofst == 0);
+  const uint64_t small = (uint64_t)((float)average * threshold);
+  f.cnt.resize(k);
+  HDG_HIP(hipMemcpyAsync(f.cnt.data(), f.d_cnt.data(), k * sizeof(unsigned long long), hipMemcpyDeviceToHost, f.s));
+  HDG_HIP(hipStreamSynchronize(f.s));
+  bool any = false;
+  for (uint32_t l = 0; l < k && !any; ++l) any = f.cnt[l] <= small;
+  if (!any) return false;
+  f.lab.resize(n);
+  HDG_HIP(hipMemcpyAsync(f.lab.data(), label, n * sizeof(uint32_t), hipMemcpyDeviceToHost, f.s));
+  HDG_HIP(hipStreamSynchronize(f.s));
+  std::vector<uint32_t> moves;
+  std::vector<float> wcs;
+  for (uint32_t l = 0; l < k; ++l) {
+    if (f.cnt[l] > small) continue;
+    do {
+      f.cursor = (f.cursor + ofst) % n;
+    } while (f.cnt[f.lab[f.cursor]] < average);
+    moves.insert(moves.end(), {l, f.lab[f.cursor], (uint32_t)f.cursor});
+    wcs.push_back(std::min((float)f.cnt[l], kAdjustWeight));
+  }
+  upload(f.d_moves, moves, f.s);
+  upload(f.d_wc, wcs, f.s);
+  hipLaunchKernelGGL(adjust_move_kernel, dim3((unsigned)ceil_div(w, 64), (unsigned)wcs.size()), dim3(64), 0, f.s,
+                     centers, w, f.d_moves.as<uint32_t>(), f.d_wc.as<float>(), X, ld);
+  HDG_HIP(hipGetLastError());
+  return true;
+}
+
+void bal_em(BalancedFit& f, const float* X, uint64_t n, uint64_t ld, uint32_t w, uint32_t k, uint32_t n_iters,
+            uint32_t pullback, float threshold, double scale, float* centers, uint32_t* label) {
+  uint32_t counter = pullback;
+  for (uint32_t iter = 0; iter < n_iters; ++iter) {
+    if (iter > 0 && bal_adjust(f, X, n, ld, w, k, label, threshold, centers)) {
+      if (counter++ >= pullback) {
+        counter -= pullback;
+        ++n_iters;
+      }
+    }
+    bal_predict(f, X, n, ld, w, k, centers, label);
+    bal_centers(f, X, n, ld, w, k, label, scale, centers);
+  }
+}
+
+// Train k clusters on X from labels i mod k (pullback 2, threshold 0.25).
+void bal_build_clusters(BalancedFit& f, const float* X, uint64_t n, uint64_t ld, uint32_t w, uint32_t k,
+                        uint32_t n_iters, float* centers, uint32_t* label) {
+  const double scale = fixed_point_scale(X, n, ld, w, f.s);
+  if (n > 0) {
+    hipLaunchKernelGGL(mod_label_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, f.s, label, n, k);
+    HDG_HIP(hipGetLastError());
+  }
+  bal_centers(f, X, n, ld, w, k, label, scale, centers);
+  bal_em(f, X, n, ld, w, k, n_iters, 2, 0.25f, scale, centers, label);
+}
+
+}  // namespace
+
+// Fine cluster counts per mesocluster: proportional to the mesocluster sizes, rounded half up, at least one
+// for a non-empty mesocluster while leaving one for each non-empty one after it; the last takes the rest.
+std::vector<uint32_t> arrange_fine_clusters(uint32_t k, const std::vector<uint64_t>& meso_sizes, uint64_t n) {
+  const size_t nm = meso_sizes.size();
+  std::vector<uint32_t> nums(nm, 0);
+  uint64_t lists_rem = k, rows_rem = n, nonempty_rem = 0;
+  for (uint64_t sz : meso_sizes) nonempty_rem += sz > 0;
+  for (size_t i = 0; i < nm; ++i) {
+    if (i + 1 < nm) {
+      if (meso_sizes[i] > 0) {
+        --nonempty_rem;
+        uint64_t sh = (uint64_t)((double)(lists_rem * meso_sizes[i]) / (double)rows_rem + 0.5);
+        sh = std::min<uint64_t>(sh, lists_rem - nonempty_rem);
+        nums[i] = (uint32_t)std::max<uint64_t>(sh, 1);
+      }
+    } else {
+      nums[i] = (uint32_t)lists_rem;
+    }
+    lists_rem -= nums[i];
+    rows_rem -= meso_sizes[i];
+  }
+  return nums;
+}
+
+void kmeans_balanced(const float* X, uint64_t n, uint32_t w, uint32_t k, uint32_t n_iters,
+                     std::vector<float>& centers, std::vector<uint32_t>& labels, hipStream_t s) {
+  if (k == 0 || n < k) fail(kInvalidArgument, "balanced k-means needs at least as many rows as clusters");
+  BalancedFit f;
+  f.s = s;
+  // mesoclusters over all rows
+  const uint32_t nm = std::min<uint32_t>(k, (uint32_t)(std::sqrt((double)k) + 0.5));
+  DevBuf d_meso_c, d_meso_l;
+  d_meso_c.reserve((size_t)nm * w * sizeof(float));
+  d_meso_l.reserve(n * sizeof(uint32_t));
+  bal_build_clusters(f, X, n, w, w, nm, n_iters, d_meso_c.as<float>(), d_meso_l.as<uint32_t>());
+  std::vector<uint32_t> meso_l(n);
+  HDG_HIP(hipMemcpyAsync(meso_l.data(), d_meso_l.data(), n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HDG_HIP(hipStreamSynchronize(s));
+  std::vector<uint64_t> meso_sizes(nm, 0);
+  for (uint32_t v : meso_l) ++meso_sizes[v];
+  const std::vector<uint32_t> nums = arrange_fine_clusters(k, meso_sizes, n);
+  // fine clusters inside each mesocluster (rows in ascending order)
+  DevBuf d_c, d_l, d_sub, d_sub_l;
+  d_c.reserve((size_t)k * w * sizeof(float));
+  HDG_HIP(hipMemsetAsync(d_c.data(), 0, (size_t)k * w * sizeof(float), s));
+  d_l.reserve(n * sizeof(uint32_t));
+  uint64_t done = 0;
+  for (uint32_t i = 0; i < nm; ++i) {
+    std::vector<uint32_t> rows;
+    for (uint64_t j = 0; j < n; ++j)
+      if (meso_l[j] == i) rows.push_back((uint32_t)j);
+    if (rows.empty() || nums[i] == 0) {  // an empty last mesocluster leaves its share at zero centers
+      done += nums[i];
+      continue;
+    }
+    gather_rows_dev(X, w, rows, w, d_sub, s);
+    d_sub_l.reserve(rows.size() * sizeof(uint32_t));
+    bal_build_clusters(f, d_sub.as<float>(), rows.size(), w, w, nums[i], n_iters, d_c.as<float>() + done * w,
+                       d_sub_l.as<uint32_t>());
+    done += nums[i];
+  }
+  // balancing EM over all rows with all k centers, then the final assignment
+  const double scale = fixed_point_scale(X, n, w, w, s);
+  bal_em(f, X, n, w, w, k, std::max<uint32_t>(n_iters / 10, 2), 5, 0.2f, scale, d_c.as<float>(),
+         d_l.as<uint32_t>());
+  bal_predict(f, X, n, w, w, k, d_c.as<float>(), d_l.as<uint32_t>());
+  centers.resize((size_t)k * w);
+  labels.resize(n);
+  HDG_HIP(hipMemcpyAsync(centers.data(), d_c.data(), (size_t)k * w * sizeof(float), hipMemcpyDeviceToHost, s));
+  HDG_HIP(hipMemcpyAsync(labels.data(), d_l.data(), n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HDG_HIP(hipStreamSynchronize(s));
+}
+
+namespace {
+
 // Gather `rows` (host list) of X into a contiguous device matrix.
 void gather_rows_dev(const float* X, uint64_t ld_floats, const std::vector<uint32_t>& rows, uint32_t w,
                      DevBuf& out, hipStream_t s) {
@@ -815,16 +1072,13 @@ void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* lab
   }
   stage_done("fine k-means and assignment");
 
-  // ---- B6c: group sub-centers into C clusters (stands in for kmeans_balanced, :466-476) ----
+  // ---- B6c: group sub-centers into C clusters with the balanced k-means (kmeans_balanced, :466-476) ----
   std::vector<float> coarse;
-  std::vector<uint32_t> group(CS);
+  std::vector<uint32_t> group;
   {
     DevBuf d_fine;
     upload(d_fine, fine, s);
-    kmeans(d_fine.as<float>(), CS, d, (uint32_t)d, (uint32_t)C, (int)cfg_.balance_iters, cfg_.seed + 11, coarse, s);
-    DevBuf d_g;
-    assign_all(d_fine.as<float>(), CS, d, (uint32_t)d, coarse, (uint32_t)C, d_g, s);
-    HDG_HIP(hipMemcpy(group.data(), d_g.data(), CS * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    kmeans_balanced(d_fine.as<float>(), CS, (uint32_t)d, (uint32_t)C, (uint32_t)cfg_.balance_iters, coarse, group, s);
   }
   stage_done("sub-center grouping");
 

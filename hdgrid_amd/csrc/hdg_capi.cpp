@@ -390,6 +390,20 @@ hdg_status hdg_debug_coarse_scores(hdg_index* index, const float* queries, uint6
   });
 }
 
+hdg_status hdg_debug_kmeans_balanced(const float* rows, uint64_t n, uint32_t w, uint32_t k, uint32_t n_iters,
+                                     float* centers, uint32_t* labels, int device, void* stream) {
+  return guarded([&] {
+    if (!rows || !centers || !labels) hdg::fail(HDG_EINVAL, "null argument");
+    if (w == 0) hdg::fail(HDG_EINVAL, "zero row width");
+    HDG_HIP(hipSetDevice(device));
+    std::vector<float> c;
+    std::vector<uint32_t> lab;
+    hdg::kmeans_balanced(rows, n, w, k, n_iters, c, lab, (hipStream_t)stream);
+    std::copy(c.begin(), c.end(), centers);
+    std::copy(lab.begin(), lab.end(), labels);
+  });
+}
+
 hdg_status hdg_merge_topk(const float* dist, const uint64_t* ids, uint64_t n_batches, uint64_t data_batch_size,
                           uint64_t n_data, uint64_t start_offset, uint64_t nq, uint64_t topk, float* out_dist,
                           uint64_t* out_ids, void* stream) {

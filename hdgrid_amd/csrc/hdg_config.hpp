@@ -42,7 +42,7 @@ struct Config {
   uint64_t kmeans_iters = 16;          // KMEANS_ITERS: Lloyd iterations (fine + PQ)
   uint64_t kmeans_train = 262144;      // KMEANS_TRAIN: max training rows for the fine k-means
   uint64_t pq_train = 65536;           // PQ_TRAIN: max training rows per PQ subspace
-  uint64_t balance_iters = 25;         // BALANCE_ITERS: iterations of the sub-center grouping
+  uint64_t balance_iters = 200;        // BALANCE_ITERS: balanced k-means iterations of the grouping
   uint64_t parity_batch = 0;           // PARITY_BATCH: beta group size (0 = findMaxFactor(125, nq))
   uint64_t query_chunk = 16384;        // QUERY_CHUNK: queries processed per device pass
   uint64_t precision = 0;              // PRECISION: 0 reference-exact FP32, 1 BF16 coarse + FP16 LUT

@@ -98,6 +98,13 @@ HostCheckpoint read_checkpoint(const std::string& dir, int l_expected);
 // (st.seg_off); sets the bucket fields.
 void build_bucket_grid(IndexState& st, hipStream_t s);
 
+// Balanced hierarchical k-means (RAFT kmeans_balanced::fit_predict, Super-Grid-IVF_PQ.cu:466-476) of n rows
+// X [n][w] (device) into k clusters: centers [k][w] and labels [n] returned on the host.
+void kmeans_balanced(const float* X, uint64_t n, uint32_t w, uint32_t k, uint32_t n_iters,
+                     std::vector<float>& centers, std::vector<uint32_t>& labels, hipStream_t s);
+// Fine cluster counts of the mesoclusters (sizes meso_sizes, n rows in all) for k clusters.
+std::vector<uint32_t> arrange_fine_clusters(uint32_t k, const std::vector<uint64_t>& meso_sizes, uint64_t n);
+
 // Fills codes_blk / codes_wb from the row-layout codes (device).
 void prepare_blocked_codes(IndexState& st, hipStream_t s);
 

@@ -166,6 +166,13 @@ hdg_status hdg_budget_debug_features(hdg_index* index, const float* queries, uin
  * out[q][c] = |c|^2 - 2 <q, c> computed by the BF16 MFMA kernel.  For checking that kernel. */
 hdg_status hdg_debug_coarse_scores(hdg_index* index, const float* queries, uint64_t nq, float* out, void* stream);
 
+/* Balanced hierarchical k-means of n rows (device array rows [n][w]) into k clusters, as the index build
+ * groups its clusters*sub_clusters fine centers into `clusters` groups (replaces the RAFT
+ * kmeans_balanced::fit_predict call at Super-Grid-IVF_PQ.cu:466-476; n_iters = BALANCE_ITERS there).
+ * Writes centers [k][w] and labels [n] (host).  For checking the build's grouping. */
+hdg_status hdg_debug_kmeans_balanced(const float* rows, uint64_t n, uint32_t w, uint32_t k, uint32_t n_iters,
+                                     float* centers, uint32_t* labels, int device, void* stream);
+
 /* Top-k merge of per-data-batch results (device arrays): dist/ids [n_batches][nq][topk], ids local to
  * their batch; batch b holds rows [b*data_batch_size, min(n_data, (b+1)*data_batch_size)).  Writes
  * out_dist/out_ids [nq][topk] (device): per query the topk smallest distances, ascending, ties by (batch,

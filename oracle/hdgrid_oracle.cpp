@@ -1199,6 +1199,214 @@ int orc_find_max_factor(uint64_t a, uint64_t b) {
   return mf;
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// B6c: balanced hierarchical k-means, RAFT kmeans_balanced::fit_predict as Super-Grid-IVF_PQ.cu:466-476 calls
+// it (L2, n_iters 200) on the clusters*sub_clusters fine centers.  RAFT is not vendored (rapids-cmake fetch,
+// CMakeLists.txt:18-34, no version pin in the tree); restated from its published algorithm
+// (cluster/detail/kmeans_balanced.cuh: build_hierarchical, arrange_fine_clusters, build_fine_clusters,
+// build_clusters, balancing_em_iters, adjust_centers, calc_centers_and_sizes, predict) with the build's
+// deterministic arithmetic:
+//   * E step: label = argmin_c (||c||^2 - 2 x.c), ||c||^2 an FP64 sum rounded to f32, x.c an f32 fma chain over
+//     the dims in order, ties to the lower center id;
+//   * M step: sums of round(x * 2^(62-e)) in int64 (2^e > n * max|x|), divided in FP64, rounded to f32; an
+//     empty cluster's center is zero (RAFT divides by a zero count to zero);
+//   * balancing move of cluster l (size <= floor(average * threshold), average = n / k): a cursor steps through
+//     the rows by a prime stride (smallest prime >= 29 + 70 j, j cycling over 40 values, skipping strides that
+//     divide n) until it reaches a row i of a cluster of >= average rows; centers[l] = (wc * centers[label[i]]
+//     + x_i) / (wc + 1), wc = min(size, 7), each operation rounded to f32.  RAFT's cursor is process-wide; here
+//     it lives for one fit.
+// Parity: pinned by tests/test_oracle_kmeans.py (hand-derived arrangements and balancing moves, EM fixed points
+// on separated blobs, k-means invariants); RAFT's own outputs are unpinned (the library is absent).
+namespace {
+namespace kb {
+
+struct Fit {
+  uint64_t cursor = 0;
+  uint32_t prime_j = 0;
+};
+
+uint64_t stride_prime(uint32_t j) {
+  for (uint64_t p = 29 + 70ull * j;; ++p) {
+    bool prime = p > 1;
+    for (uint64_t q = 2; q * q <= p && prime; ++q) prime = p % q != 0;
+    if (prime) return p;
+  }
+}
+
+double fixed_scale(const float* X, uint64_t n, uint32_t w) {
+  float m = 0.0f;
+  for (uint64_t e = 0; e < n * w; ++e) m = std::max(m, std::fabs(X[e]));
+  int ex = 0;
+  std::frexp((double)m * (double)n, &ex);
+  return m > 0.0f ? std::ldexp(1.0, 62 - ex) : 1.0;
+}
+
+void predict(const float* X, uint64_t n, uint32_t w, uint32_t k, const float* C, uint32_t* lab) {
+  std::vector<float> cn(k);
+  for (uint32_t c = 0; c < k; ++c) {
+    double a = 0.0;
+    for (uint32_t t = 0; t < w; ++t) a += (double)C[(uint64_t)c * w + t] * (double)C[(uint64_t)c * w + t];
+    cn[c] = (float)a;
+  }
+#pragma omp parallel for schedule(static) if (n * k * w > (1u << 20))
+  for (uint64_t i = 0; i < n; ++i) {
+    float best = INFINITY;
+    uint32_t bi = 0;
+    for (uint32_t c = 0; c < k; ++c) {
+      float acc = 0.0f;
+      for (uint32_t t = 0; t < w; ++t) acc = std::fmaf(X[i * w + t], C[(uint64_t)c * w + t], acc);
+      const float sc = cn[c] - 2.0f * acc;
+      if (sc < best) {
+        best = sc;
+        bi = c;
+      }
+    }
+    lab[i] = bi;
+  }
+}
+
+void centers_of(const float* X, uint64_t n, uint32_t w, uint32_t k, const uint32_t* lab, double scale, float* C,
+                std::vector<uint64_t>& size) {
+  std::vector<int64_t> sum((size_t)k * w, 0);
+  size.assign(k, 0);
+  for (uint64_t i = 0; i < n; ++i) {
+    ++size[lab[i]];
+    for (uint32_t t = 0; t < w; ++t) sum[(size_t)lab[i] * w + t] += std::llrint((double)X[i * w + t] * scale);
+  }
+  for (uint32_t c = 0; c < k; ++c)
+    for (uint32_t t = 0; t < w; ++t)
+      C[(size_t)c * w + t] =
+          size[c] ? (float)((double)sum[(size_t)c * w + t] * (1.0 / scale) / (double)size[c]) : 0.0f;
+}
+
+bool adjust(Fit& f, const float* X, uint64_t n, uint32_t w, uint32_t k, const uint32_t* lab,
+            const std::vector<uint64_t>& size, float threshold, float* C) {
+  if (k == 0 || n == 0) return false;
+  const uint64_t average = n / k;
+  uint64_t ofst;
+  do {
+    f.prime_j = (f.prime_j + 1) % 40;
+    ofst = stride_prime(f.prime_j);
+  } while (n % ofst == 0);
+  const uint64_t small = (uint64_t)((float)average * threshold);
+  bool moved = false;
+  for (uint32_t l = 0; l < k; ++l) {
+    if (size[l] > small) continue;
+    do {
+      f.cursor = (f.cursor + ofst) % n;
+    } while (size[lab[f.cursor]] < average);
+    const uint32_t li = lab[f.cursor];
+    const float wc = std::min((float)size[l], 7.0f);
+    for (uint32_t t = 0; t < w; ++t) {
+      float v = wc * C[(size_t)li * w + t];
+      v = v + X[f.cursor * w + t];
+      C[(size_t)l * w + t] = v / (wc + 1.0f);
+    }
+    moved = true;
+  }
+  return moved;
+}
+
+void em(Fit& f, const float* X, uint64_t n, uint32_t w, uint32_t k, uint32_t n_iters, uint32_t pullback,
+        float threshold, double scale, float* C, uint32_t* lab, std::vector<uint64_t>& size) {
+  uint32_t counter = pullback;
+  for (uint32_t iter = 0; iter < n_iters; ++iter) {
+    if (iter > 0 && adjust(f, X, n, w, k, lab, size, threshold, C)) {
+      if (counter++ >= pullback) {
+        counter -= pullback;
+        ++n_iters;
+      }
+    }
+    predict(X, n, w, k, C, lab);
+    centers_of(X, n, w, k, lab, scale, C, size);
+  }
+}
+
+void build_clusters(Fit& f, const float* X, uint64_t n, uint32_t w, uint32_t k, uint32_t n_iters, float* C,
+                    uint32_t* lab, std::vector<uint64_t>& size) {
+  const double scale = fixed_scale(X, n, w);
+  for (uint64_t i = 0; i < n; ++i) lab[i] = (uint32_t)(i % k);
+  centers_of(X, n, w, k, lab, scale, C, size);
+  em(f, X, n, w, k, n_iters, 2, 0.25f, scale, C, lab, size);
+}
+
+void arrange(uint32_t k, const uint64_t* sizes, uint32_t nm, uint64_t n, uint32_t* nums) {
+  uint64_t lists_rem = k, rows_rem = n, nonempty_rem = 0;
+  for (uint32_t i = 0; i < nm; ++i) nonempty_rem += sizes[i] > 0;
+  for (uint32_t i = 0; i < nm; ++i) {
+    nums[i] = 0;
+    if (i + 1 < nm) {
+      if (sizes[i] > 0) {
+        --nonempty_rem;
+        uint64_t sh = (uint64_t)((double)(lists_rem * sizes[i]) / (double)rows_rem + 0.5);
+        sh = std::min<uint64_t>(sh, lists_rem - nonempty_rem);
+        nums[i] = (uint32_t)std::max<uint64_t>(sh, 1);
+      }
+    } else {
+      nums[i] = (uint32_t)lists_rem;
+    }
+    lists_rem -= nums[i];
+    rows_rem -= sizes[i];
+  }
+}
+
+}  // namespace kb
+}  // namespace
+
+// Fine cluster counts per mesocluster (arrange_fine_clusters).
+void orc_arrange_fine(uint32_t k, const uint64_t* sizes, uint32_t nm, uint64_t n, uint32_t* out) {
+  kb::arrange(k, sizes, nm, n, out);
+}
+
+// One balancing step over C (in/out) with the given labels and sizes; cursor / prime index carried in and out.
+int orc_kb_adjust(const float* X, uint64_t n, uint32_t w, uint32_t k, const uint32_t* lab, const uint64_t* sizes,
+                  float threshold, float* C, uint64_t* cursor, uint32_t* prime_j) {
+  kb::Fit f;
+  f.cursor = *cursor;
+  f.prime_j = *prime_j;
+  std::vector<uint64_t> sz(sizes, sizes + k);
+  const bool moved = kb::adjust(f, X, n, w, k, lab, sz, threshold, C);
+  *cursor = f.cursor;
+  *prime_j = f.prime_j;
+  return moved ? 1 : 0;
+}
+
+// Whole fit_predict: centers [k][w], labels [n].  Returns 0, or -1 when n < k or k == 0.
+int orc_kmeans_balanced(const float* X, uint64_t n, uint32_t w, uint32_t k, uint32_t n_iters, float* centers,
+                        uint32_t* labels) {
+  if (k == 0 || n < k) {
+    g_err = "balanced k-means needs at least as many rows as clusters";
+    return -1;
+  }
+  kb::Fit f;
+  const uint32_t nm = std::min<uint32_t>(k, (uint32_t)(std::sqrt((double)k) + 0.5));
+  std::vector<float> mc((size_t)nm * w);
+  std::vector<uint32_t> ml(n);
+  std::vector<uint64_t> size;
+  kb::build_clusters(f, X, n, w, nm, n_iters, mc.data(), ml.data(), size);
+  std::vector<uint64_t> msz(nm, 0);
+  for (uint32_t v : ml) ++msz[v];
+  std::vector<uint32_t> nums(nm);
+  kb::arrange(k, msz.data(), nm, n, nums.data());
+  std::fill(centers, centers + (size_t)k * w, 0.0f);
+  uint64_t done = 0;
+  for (uint32_t i = 0; i < nm; ++i) {
+    std::vector<float> sub;
+    for (uint64_t j = 0; j < n; ++j)
+      if (ml[j] == i) sub.insert(sub.end(), X + j * w, X + (j + 1) * w);
+    const uint64_t rows = sub.size() / w;
+    if (rows > 0 && nums[i] > 0) {
+      std::vector<uint32_t> sl(rows);
+      kb::build_clusters(f, sub.data(), rows, w, nums[i], n_iters, centers + done * w, sl.data(), size);
+    }
+    done += nums[i];
+  }
+  const double scale = kb::fixed_scale(X, n, w);
+  kb::em(f, X, n, w, k, std::max<uint32_t>(n_iters / 10, 2), 5, 0.2f, scale, centers, labels, size);
+  kb::predict(X, n, w, k, centers, labels);
+  return 0;
+}
+
 int orc_omp_threads() {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -358,3 +358,37 @@ def query(state: dict, data, labels, queries, qlabels, *, n_list, sub_lists, exp
     ps = prepare(state, data, labels, n_list=n_list, sub_lists=sub_lists, exps0=exps0, topk=topk, ftype=ftype,
                  shift_val=shift_val, imap=imap, div_value=div_value, n_split=n_split)
     return run_query(ps, queries, qlabels, batch=batch, trace=trace)
+
+
+def arrange_fine_clusters(k, meso_sizes, n):
+    """Fine cluster counts of the mesoclusters (balanced k-means, arrange_fine_clusters)."""
+    sz = _c(meso_sizes, np.uint64)
+    out = np.zeros(sz.shape[0], np.uint32)
+    _lib.orc_arrange_fine(ctypes.c_uint32(k), _p(sz), ctypes.c_uint32(sz.shape[0]), _u64(n), _p(out))
+    return out
+
+
+def balance_adjust(X, labels, sizes, threshold, centers, cursor=0, prime_j=0):
+    """One balancing step of the balanced k-means; returns (moved, centers, cursor, prime_j)."""
+    X = _c(X, np.float32)
+    C = np.array(centers, np.float32, copy=True, order="C")
+    lab = _c(labels, np.uint32)
+    sz = _c(sizes, np.uint64)
+    cur = _u64(cursor)
+    pj = ctypes.c_uint32(prime_j)
+    moved = _lib.orc_kb_adjust(_p(X), _u64(X.shape[0]), ctypes.c_uint32(X.shape[1]), ctypes.c_uint32(C.shape[0]),
+                               _p(lab), _p(sz), ctypes.c_float(threshold), _p(C), ctypes.byref(cur), ctypes.byref(pj))
+    return bool(moved), C, int(cur.value), int(pj.value)
+
+
+def kmeans_balanced(X, k, n_iters):
+    """Balanced hierarchical k-means fit_predict: (centers [k][w], labels [n])."""
+    X = _c(X, np.float32)
+    n, w = X.shape
+    C = np.zeros((k, w), np.float32)
+    lab = np.zeros(n, np.uint32)
+    st = _lib.orc_kmeans_balanced(_p(X), _u64(n), ctypes.c_uint32(w), ctypes.c_uint32(k), ctypes.c_uint32(n_iters),
+                                  _p(C), _p(lab))
+    if st != 0:
+        raise ValueError(_err())
+    return C, lab

@@ -97,3 +97,18 @@ def test_build_refuses_non_finite_rows():
     with pytest.raises(H.HDGridError) as e:
         idx.build(torch.from_numpy(data).to(DEV), torch.from_numpy(w.labels).to(DEV))
     assert "non-finite" in str(e.value)
+
+
+@pytest.mark.parametrize("n,w,k,iters,seed", [(5120, 96, 512, 200, 0), (3000, 40, 48, 20, 1), (517, 33, 23, 8, 2),
+                                               (64, 5, 64, 10, 3), (700, 384, 70, 30, 4)])
+def test_balanced_kmeans_matches_oracle(n, w, k, iters, seed):
+    """The grouping's balanced k-means on the GPU equals the oracle's restatement bit for bit (centers and
+    labels), from the configurations' own shape (5,120 fine centers into 512 clusters, 200 iterations) down
+    to one row per cluster."""
+    rng = np.random.default_rng(seed)
+    X = (rng.standard_normal((n, w)) + rng.integers(0, 4, (n, 1))).astype(np.float32)
+    print(f"[kmeans_balanced] n={n} w={w} k={k} iters={iters}", flush=True)
+    gc, gl = H.kmeans_balanced(torch.from_numpy(X).to(DEV), k, iters)
+    oc, ol = O.kmeans_balanced(X, k, iters)
+    assert np.array_equal(gl, ol)
+    assert np.array_equal(gc.view(np.uint32), oc.view(np.uint32))
