@@ -2960,6 +2960,10 @@ struct Index::Work {
   std::vector<Captured> graphs;
   bool capturing = false;
   DevBuf abort_flag;
+  // Capacities and fallback kernels of the captured graphs, from the uncaptured runs under the configuration
+  // (digest cap_cfg) they are captured for: runs under another configuration (e.g. the plain-path labelling of
+  // a budget fit) do not size them
+  uint64_t cap_cfg = 0;
   uint64_t lcap = 0, acap = 0;  // refine-list and ADC capacities of the captured graphs
   uint32_t need_mask = 0;       // fallback kernels some sizing run needed (bits as the capture's skip mask)
   uint64_t last_totL = 0, last_totA = 0;  // largest totals seen by uncaptured runs
@@ -3226,6 +3230,12 @@ void Index::query(const float* queries, uint64_t nq, uint64_t d, const float* ql
   }
   w.abort_flag.reserve(4);
   const uint64_t digest = config_digest();
+  if (w.cap_cfg != digest) {
+    w.cap_cfg = digest;
+    w.lcap = w.acap = 0;
+    w.last_totL = w.last_totA = 0;
+    w.need_mask = 0;
+  }
   uint64_t ci = 0;
   for (uint64_t q0 = 0; q0 < nq; q0 += chunk, ++ci) {
     const uint64_t qn = std::min(chunk, nq - q0);
