@@ -80,6 +80,8 @@ _merge = _sig("hdg_merge_topk", ctypes.c_int, [_vp, _vp, _u64, _u64, _u64, _u64,
 _kmeans_bal = _sig("hdg_debug_kmeans_balanced", ctypes.c_int,
                   [_vp, _u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _fp, ctypes.POINTER(ctypes.c_uint32),
                    ctypes.c_int, _vp])
+_coarse_feat = _sig("hdg_debug_coarse_features", ctypes.c_int,
+                    [_vp, _u64, _u64, _vp, ctypes.c_uint32, _vp, _vp, _u64, _vp, ctypes.c_int, _vp])
 _fit_coeff = _sig("hdg_fit_mem_coeff", ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)])
 _autotune = _sig("hdg_autotune", ctypes.c_int,
                  [_vp, _vp, _u64, _u64, _vp, _u64, _vp, _vp, ctypes.c_uint32, _cp, _u64, _vp])
@@ -174,6 +176,18 @@ def kmeans_balanced(rows, k: int, n_iters: int, stream=None):
                        labels.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), rows.device.index or 0,
                        _stream_ptr(stream)))
     return centers, labels
+
+
+def coarse_features(queries, lists, n_list, scores, stream=None):
+    """The query path's coarse-score budget features (device tensors: queries [nq][d] float32, lists [nq][cap]
+    uint32-valued int32, n_list [nq] int32, scores [nq][ld] float32): returns feat [nq][32] float32 (device)."""
+    import torch
+    queries, lists, n_list, scores = (t.contiguous() for t in (queries, lists, n_list, scores))
+    nq, d = queries.shape
+    feat = torch.empty((nq, 32), dtype=torch.float32, device=queries.device)
+    _check(_coarse_feat(_ptr(queries), nq, d, _ptr(lists), lists.shape[1], _ptr(n_list), _ptr(scores),
+                        scores.shape[1], _ptr(feat), queries.device.index or 0, _stream_ptr(stream)))
+    return feat
 
 
 def validate_config_text(conf_text: str, filter_text: str) -> str:

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <string>
 
+#include "hdg_fast.hpp"
 #include "hdg_index.hpp"
 #include "hdg_merge.hpp"
 
@@ -387,6 +388,19 @@ hdg_status hdg_debug_coarse_scores(hdg_index* index, const float* queries, uint6
   return guarded([&] {
     if (!index || !queries || !out) hdg::fail(HDG_EINVAL, "null argument");
     index->impl->debug_coarse_scores(queries, nq, out, (hipStream_t)stream);
+  });
+}
+
+hdg_status hdg_debug_coarse_features(const float* queries, uint64_t nq, uint64_t dim, const uint32_t* list,
+                                     uint32_t cap, const uint32_t* n_list, const float* scores, uint64_t ld,
+                                     float* feat, int device, void* stream) {
+  return guarded([&] {
+    if (nq && (!queries || !list || !n_list || !scores || !feat)) hdg::fail(HDG_EINVAL, "null argument");
+    if (dim == 0) hdg::fail(HDG_EINVAL, "zero dimension");
+    HDG_HIP(hipSetDevice(device));
+    hdg::launch_coarse_features(queries, 0, nq, dim, list, cap, n_list, scores, ld, feat, (hipStream_t)stream,
+                                nullptr);
+    HDG_HIP(hipGetLastError());
   });
 }
 

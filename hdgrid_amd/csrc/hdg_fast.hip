@@ -455,6 +455,104 @@ __global__ __launch_bounds__(256) void scored_select_wave_kernel(
   }
 }
 
+// Budget-model features on the path that probes every non-zero cluster: per query, the 32 smallest coarse
+// scores among its non-zero clusters, ascending, plus |q|^2 (lane j < 32 writes the min(j, avail - 1)-th,
+// avail = min(n, 32)) -- the values scored_select_wave_kernel writes as its features, without its full sort of
+// 64-bit (score, position) keys.  One wave per query, lists of at most 512 items (8 keys per lane, lane-major):
+// each lane sorts its 8 keys; T = the 8th smallest of the lanes' 4th smallest keys, so at least 32 keys are
+// <= T and the 32 smallest are among them; those keys (normally at most 64) are compacted and sorted.  When more
+// than 64 keys are <= T, all 512 are sorted.  Keys are order-preserving u32 of the f32 scores, so equal scores
+// give equal features whatever their order.
+__device__ __forceinline__ void cas_up(uint32_t& a, uint32_t& b) {
+  const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+  a = lo;
+  b = hi;
+}
+
+__device__ __forceinline__ void sort8_up(uint32_t (&k)[8]) {  // Batcher's odd-even merge network, 19 steps
+  cas_up(k[0], k[1]); cas_up(k[2], k[3]); cas_up(k[4], k[5]); cas_up(k[6], k[7]);
+  cas_up(k[0], k[2]); cas_up(k[1], k[3]); cas_up(k[4], k[6]); cas_up(k[5], k[7]);
+  cas_up(k[1], k[2]); cas_up(k[5], k[6]);
+  cas_up(k[0], k[4]); cas_up(k[1], k[5]); cas_up(k[2], k[6]); cas_up(k[3], k[7]);
+  cas_up(k[2], k[4]); cas_up(k[3], k[5]);
+  cas_up(k[1], k[2]); cas_up(k[3], k[4]); cas_up(k[5], k[6]);
+}
+
+__global__ __launch_bounds__(256) void coarse_features_kernel(const float* __restrict__ queries, uint64_t q0,
+                                                              uint64_t qn, uint64_t d,
+                                                              const uint32_t* __restrict__ list, uint32_t cap,
+                                                              const uint32_t* __restrict__ n_list_q,
+                                                              const float* __restrict__ S, uint64_t ld,
+                                                              float* __restrict__ feat,
+                                                              const uint32_t* __restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
+  __shared__ uint32_t cbuf_all[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t qi = (uint64_t)blockIdx.x * 4 + wv;
+  if (qi >= qn) return;
+  uint32_t* cbuf = cbuf_all[wv];
+  const uint32_t n = n_list_q[qi] < cap ? n_list_q[qi] : cap;  // cap <= 512
+  const uint32_t* lst = list + qi * cap;
+  const float* row = S + qi * ld;
+  uint32_t v;  // lane j: the j-th smallest key (j < min(n, 64))
+  if (n <= 64) {
+    uint32_t k1[1] = {(uint32_t)lane < n ? ord(row[lst[lane]]) : 0xFFFFFFFFu};
+    dev::wave_bitonic32<1>(k1, lane);
+    v = k1[0];
+  } else {
+    uint32_t key[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const uint32_t e = (uint32_t)lane * 8 + r;
+      key[r] = e < n ? ord(row[lst[e]]) : 0xFFFFFFFFu;
+    }
+    sort8_up(key);
+    uint32_t t[1] = {key[3]};
+    dev::wave_bitonic32<1>(t, lane);
+    const uint32_t T = (uint32_t)__shfl((int)t[0], 7, 64);
+    uint32_t c = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) c += key[r] <= T ? 1u : 0u;
+    uint32_t incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+    if (total <= 64) {
+      const uint32_t off = incl - c;
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        if ((uint32_t)r < c) cbuf[off + r] = key[r];  // the lane's keys are ascending: its first c are <= T
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      uint32_t k1[1] = {(uint32_t)lane < total ? cbuf[lane] : 0xFFFFFFFFu};
+      dev::wave_bitonic32<1>(k1, lane);
+      v = k1[0];
+    } else {
+      dev::wave_bitonic32<8>(key, lane);
+      // rank j lies in lane j / 8, register j % 8
+      const uint32_t src = (uint32_t)lane >> 3, rr = (uint32_t)lane & 7u;
+      uint32_t mine = 0;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const uint32_t x = (uint32_t)__shfl((int)key[r], (int)src, 64);
+        mine = (uint32_t)r == rr ? x : mine;
+      }
+      v = mine;
+    }
+  }
+  const uint32_t avail = n < (uint32_t)kFeatDists ? n : (uint32_t)kFeatDists;
+  const uint32_t j = avail == 0 ? 0u : ((uint32_t)lane < avail ? (uint32_t)lane : avail - 1);
+  const uint32_t o = (uint32_t)__shfl((int)v, (int)j, 64);
+  const float qq = wave_sq_norm(queries + (q0 + qi) * d, d, lane);
+  if (lane < kFeatDists) {
+    const uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+    feat[qi * kFeatDists + lane] = avail > 0 ? __uint_as_float(u) + qq : 0.0f;
+  }
+}
+
 }  // namespace
 
 void launch_scored_select(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, const uint32_t* list,
@@ -476,6 +574,15 @@ void launch_scored_select(const float* queries, uint64_t q0, uint64_t qn, uint64
   hipLaunchKernelGGL(scored_select_kernel, dim3((unsigned)qn), dim3(256), scap * 8, s, queries, q0, d, list, cap,
                      n_list_q, S, ld, k_default, budget, out_ld, chosen, n_chosen, feat_dist, scratch, scap,
                      skip_upto, nzb, nzp, nw, abort_flag, by_pos);
+}
+
+void launch_coarse_features(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, const uint32_t* list,
+                            uint32_t cap, const uint32_t* n_list_q, const float* S, uint64_t ld, float* feat,
+                            hipStream_t s, const uint32_t* abort_flag) {
+  if (cap > 512) fail(kUnsupported, "coarse features take lists of at most 512 items");
+  if (qn == 0) return;
+  hipLaunchKernelGGL(coarse_features_kernel, dim3((unsigned)((qn + 3) / 4)), dim3(256), 0, s, queries, q0, qn, d,
+                     list, cap, n_list_q, S, ld, feat, abort_flag);
 }
 
 // 8 elements per thread (two float4 loads, one 16-byte store) when d and Dp are multiples of 8

@@ -20,6 +20,12 @@ void launch_to_bf16(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, 
 void launch_coarse_gemm(const uint16_t* qb16, uint64_t qn, uint64_t Dp, const uint16_t* cb16, const float* cnorm,
                         uint32_t C, float* S, hipStream_t s, const uint32_t* nzb = nullptr,
                         const uint32_t* nzp = nullptr, uint32_t nw = 0, const uint32_t* abort_flag = nullptr);
+// Budget-model features of queries whose every non-zero cluster is probed: feat [qn][kFeatDists] = the 32
+// smallest of S[q][list[q][0 .. n_list_q[q])] ascending (the last repeated when fewer), each + |q|^2; the
+// values launch_scored_select writes as features.  cap (the list row stride) <= 512.
+void launch_coarse_features(const float* queries, uint64_t q0, uint64_t qn, uint64_t d, const uint32_t* list,
+                            uint32_t cap, const uint32_t* n_list_q, const float* S, uint64_t ld, float* feat,
+                            hipStream_t s, const uint32_t* abort_flag);
 // Per query: top-k (k = budget[q] or k_default) of the listed items list[q][0..n_list_q[q]) by S[q][item],
 // ascending; writes the chosen list positions chosen[q][0..m) (row stride out_ld) and n_chosen[q];
 // optional features.

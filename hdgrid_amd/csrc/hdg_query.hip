@@ -3628,6 +3628,12 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
       const uint64_t bn = std::min(score_block, qn - b0);
       launch_coarse_gemm(w.qb16.as<uint16_t>() + b0 * S.Dp, bn, S.Dp, S.centers_bf16.as<uint16_t>(),
                          S.center_norm.as<float>(), (uint32_t)C, w.S.as<float>(), s, nullptr, nullptr, 0, abort_p);
+      if (C <= 512) {  // one wave per query, 32 smallest only
+        launch_coarse_features(queries, q0 + b0, bn, d, w.list_c.as<uint32_t>() + b0 * C, (uint32_t)C,
+                               w.nnz_c.as<uint32_t>() + b0, w.S.as<float>(), C, w.feat.as<float>() + b0 * kFeatDists,
+                               s, abort_w);
+        continue;
+      }
       launch_scored_select(queries, q0 + b0, bn, d, w.list_c.as<uint32_t>() + b0 * C, (uint32_t)C,
                            w.nnz_c.as<uint32_t>() + b0, w.S.as<float>(), C, 1, nullptr, n_list,
                            w.chosen_c.as<uint32_t>() + b0 * n_list, w.n_chosen_c.as<uint32_t>() + b0,

@@ -166,6 +166,16 @@ hdg_status hdg_budget_debug_features(hdg_index* index, const float* queries, uin
  * out[q][c] = |c|^2 - 2 <q, c> computed by the BF16 MFMA kernel.  For checking that kernel. */
 hdg_status hdg_debug_coarse_scores(hdg_index* index, const float* queries, uint64_t nq, float* out, void* stream);
 
+/* Budget-model coarse-score features of nq queries (device arrays): feat [nq][32] = the 32 smallest of
+ * scores[q][list[q][0 .. n_list[q])] ascending (the last repeated when fewer; 0 when the list is empty), each
+ * plus |queries[q]|^2 (queries [nq][dim]; list rows cap apart, cap <= 512; scores rows ld apart) -- the features
+ * the query path computes when every non-zero cluster is probed (the learned budget replaces the fixed
+ * SUB_LISTS choice of Super-Grid-IVF_PQ.cu's search; its features have no counterpart in the reference).
+ * For checking that kernel on hand-made lists and ties. */
+hdg_status hdg_debug_coarse_features(const float* queries, uint64_t nq, uint64_t dim, const uint32_t* list,
+                                     uint32_t cap, const uint32_t* n_list, const float* scores, uint64_t ld,
+                                     float* feat, int device, void* stream);
+
 /* Balanced hierarchical k-means of n rows (device array rows [n][w]) into k clusters, as the index build
  * groups its clusters*sub_clusters fine centers into `clusters` groups (replaces the RAFT
  * kmeans_balanced::fit_predict call at Super-Grid-IVF_PQ.cu:466-476; n_iters = BALANCE_ITERS there).

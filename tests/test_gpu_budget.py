@@ -174,3 +174,60 @@ def test_budget_gain_uneven_selectivity_matched_recall():
           f"refined {st_f.refined.mean():.0f}", flush=True)
     assert r_l >= 0.95
     assert work_l < 0.8 * work_f
+
+
+def _features_case(rng, nq, cap, ld, n_list, values, zero_queries=True):
+    d = 24
+    queries = np.zeros((nq, d), np.float32) if zero_queries else rng.standard_normal((nq, d)).astype(np.float32)
+    S = values(rng, (nq, ld)).astype(np.float32)
+    lists = np.full((nq, cap), 0xFFFFFFFF, np.uint32)
+    for q in range(nq):
+        lists[q, :n_list[q]] = rng.permutation(ld)[:n_list[q]]
+    got = H.coarse_features(torch.from_numpy(queries).to(DEV), torch.from_numpy(lists.view(np.int32)).to(DEV),
+                            torch.from_numpy(np.asarray(n_list, np.int32)).to(DEV), torch.from_numpy(S).to(DEV))
+    torch.cuda.synchronize()
+    got = got.cpu().numpy()
+    exp = np.zeros((nq, 32), np.float64)
+    for q in range(nq):
+        n = int(n_list[q])
+        if n == 0:
+            continue
+        srt = np.sort(S[q, lists[q, :n]].astype(np.float64))
+        qq = float((queries[q].astype(np.float64) ** 2).sum())
+        exp[q] = srt[np.minimum(np.arange(32), min(n, 32) - 1)] + qq
+    return got, exp
+
+
+@pytest.mark.parametrize("kind", ["distinct", "ties", "one_value", "norms"])
+def test_coarse_features_kernel(kind):
+    """The 32-smallest coarse-score features on hand-made lists: short lists (one sort of 64), long lists through
+    the threshold compaction, heavy ties that overflow it into the full sort of 512, empty lists, list rows
+    narrower than the score rows, and |q|^2 added (the exact sort against float64 + the float32 sum)."""
+    rng = np.random.default_rng(["distinct", "ties", "one_value", "norms"].index(kind))
+    ns = [0, 1, 2, 31, 32, 33, 63, 64, 65, 100, 127, 128, 129, 255, 256, 257, 300, 400, 511, 512]
+    ns += list(rng.integers(0, 513, 23))  # 43 queries: not a multiple of the 4 queries per workgroup
+    values = {
+        "distinct": lambda r, s: r.standard_normal(s) * 1e3,
+        "ties": lambda r, s: r.integers(-1, 2, s).astype(np.float64),
+        "one_value": lambda r, s: np.full(s, -7.25),
+        "norms": lambda r, s: r.standard_normal(s) * 10.0,
+    }[kind]
+    got, exp = _features_case(rng, len(ns), 512, 700, ns, values, zero_queries=(kind != "norms"))
+    if kind == "norms":
+        np.testing.assert_allclose(got, exp, rtol=1e-6, atol=1e-4)
+    else:
+        np.testing.assert_array_equal(got, exp.astype(np.float32))
+    # a narrower list row (cap 100) over score rows 130 apart
+    ns2 = [0, 5, 64, 65, 99, 100, 77]
+    got, exp = _features_case(rng, len(ns2), 100, 130, ns2, values, zero_queries=(kind != "norms"))
+    if kind == "norms":
+        np.testing.assert_allclose(got, exp, rtol=1e-6, atol=1e-4)
+    else:
+        np.testing.assert_array_equal(got, exp.astype(np.float32))
+
+
+def test_coarse_features_refuse_long_lists():
+    z = torch.zeros((4, 8), dtype=torch.float32, device=DEV)
+    with pytest.raises(Exception):
+        H.coarse_features(z, torch.zeros((4, 513), dtype=torch.int32, device=DEV),
+                          torch.zeros(4, dtype=torch.int32, device=DEV), torch.zeros((4, 513), device=DEV))
