@@ -205,18 +205,21 @@ __global__ __launch_bounds__(256) void infer_kernel(uint64_t qn, int l, const fl
 // four threads per sample, thread t owning hidden units 16t .. 16t + 15 of each layer (every unit's sum runs
 // over its inputs in order, as in mlp_forward), leave the sample's bf16 inputs, activations and back-propagated
 // errors in LDS; the output and its error are one thread's sequential sum.  Phase 2: each thread takes
-// parameters and sums their gradient over the tile's samples in order; one global atomic per parameter and
-// workgroup.  Samples past the batch carry zero errors.  LDS: the weights plus six tile arrays (dynamic).
+// parameters and sums their gradient over the tile's samples in order into the workgroup's row of gpart
+// [workgroups][kParams]; the tile's squared errors / bsz are summed in sample order into lpart[workgroup].
+// grad_reduce_kernel then adds the rows in workgroup order, so a step's gradient (and the whole fit) does not
+// depend on which workgroup finishes first.  Samples past the batch carry zero errors.  LDS: the weights plus
+// seven tile arrays (dynamic).
 constexpr int kTrainTile = 64;
 constexpr int kTrainThreads = 256;
 constexpr int kUnitsPer = kH / (kTrainThreads / kTrainTile);  // 16
 constexpr int kParamsPad = (kParams + 3) / 4 * 4;
-constexpr size_t kTrainLds = (size_t)(kParamsPad + kTrainTile * (kF + 4 * kH) + kTrainTile) * sizeof(float);
+constexpr size_t kTrainLds = (size_t)(kParamsPad + kTrainTile * (kF + 4 * kH) + 2 * kTrainTile) * sizeof(float);
 
 __global__ __launch_bounds__(256) void train_step_kernel(const float* __restrict__ X, const float* __restrict__ Y,
                                                          const uint32_t* __restrict__ idx, uint32_t bsz,
-                                                         const float* __restrict__ wq, float* __restrict__ g,
-                                                         float* __restrict__ loss) {
+                                                         const float* __restrict__ wq, float* __restrict__ gpart,
+                                                         float* __restrict__ lpart) {
   extern __shared__ __align__(16) float tsm[];
   float* w = tsm;                                 // [kParams]
   float* xs = w + kParamsPad;                     // [tile][kF] bf16-rounded inputs
@@ -225,6 +228,7 @@ __global__ __launch_bounds__(256) void train_step_kernel(const float* __restrict
   float* g2s = h2s + kTrainTile * kH;             // [tile][kH] errors at layer-2 pre-activations
   float* g1s = g2s + kTrainTile * kH;             // [tile][kH] errors at layer-1 pre-activations
   float* gys = g1s + kTrainTile * kH;             // [tile] output errors
+  float* les = gys + kTrainTile;                  // [tile] squared errors / bsz
   const int tid = threadIdx.x;
   for (int i = tid; i < kParams; i += kTrainThreads) w[i] = wq[i];
   const int sl = tid >> 2, part = tid & 3;        // sample of the tile, quarter of the units
@@ -253,17 +257,23 @@ __global__ __launch_bounds__(256) void train_step_kernel(const float* __restrict
   }
   __syncthreads();
   if (part == 0) {
-    float gy = 0.0f;
+    float gy = 0.0f, le = 0.0f;
     if (valid) {
       float y = w[kOffB3];
       for (int i = 0; i < kH; ++i) y += h2s[sl * kH + i] * w[kOffW3 + i];
       const float err = y - Y[smp];
-      atomicAdd(loss, err * err / (float)bsz);
+      le = err * err / (float)bsz;
       gy = 2.0f * err / (float)bsz;
     }
     gys[sl] = gy;
+    les[sl] = le;
   }
   __syncthreads();
+  if (tid == 0) {
+    float l = 0.0f;
+    for (int t = 0; t < kTrainTile; ++t) l += les[t];
+    lpart[blockIdx.x] = l;
+  }
   const float gy = gys[sl];
 #pragma unroll
   for (int u = 0; u < kUnitsPer; ++u) {
@@ -300,20 +310,47 @@ __global__ __launch_bounds__(256) void train_step_kernel(const float* __restrict
     } else {
       for (int t = 0; t < kTrainTile; ++t) sum += gys[t];
     }
-    if (sum != 0.0f) atomicAdd(&g[p], sum);
+    gpart[(uint64_t)blockIdx.x * kParams + p] = sum;
   }
 }
 
+// g[p] = the workgroups' partial gradients of parameter p added in workgroup order; loss += their losses in order.
+__global__ void grad_reduce_kernel(const float* __restrict__ gpart, const float* __restrict__ lpart, uint32_t nblk,
+                                   float* __restrict__ g, float* __restrict__ loss) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < (uint32_t)kParams) {
+    float sum = 0.0f;
+    for (uint32_t b = 0; b < nblk; ++b) sum += gpart[(uint64_t)b * kParams + p];
+    g[p] = sum;
+  }
+  if (p == 0) {
+    float l = 0.0f;
+    for (uint32_t b = 0; b < nblk; ++b) l += lpart[b];
+    *loss += l;
+  }
+}
+
+uint64_t train_step_workspace_floats(uint32_t bsz) {
+  const uint64_t nblk = (bsz + kTrainTile - 1) / kTrainTile;
+  return nblk * kParams + nblk;
+}
+
+// g = the batch's gradient (overwritten), *loss += the batch's mean squared error; work holds
+// train_step_workspace_floats(bsz) floats.
 void launch_train_step(const float* X, const float* Y, const uint32_t* idx, uint32_t bsz, const float* wq, float* g,
-                       float* loss, hipStream_t s) {
+                       float* loss, float* work, hipStream_t s) {
   static const bool attr = [] {
     HDG_HIP(hipFuncSetAttribute((const void*)train_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)kTrainLds));
     return true;
   }();
   (void)attr;
-  hipLaunchKernelGGL(train_step_kernel, dim3((bsz + kTrainTile - 1) / kTrainTile), dim3(kTrainThreads), kTrainLds, s,
-                     X, Y, idx, bsz, wq, g, loss);
+  const uint32_t nblk = (bsz + kTrainTile - 1) / kTrainTile;
+  float* gpart = work;
+  float* lpart = work + (uint64_t)nblk * kParams;
+  hipLaunchKernelGGL(train_step_kernel, dim3(nblk), dim3(kTrainThreads), kTrainLds, s, X, Y, idx, bsz, wq, gpart,
+                     lpart);
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, gpart, lpart, nblk, g, loss);
   HDG_HIP(hipGetLastError());
 }
 
@@ -544,7 +581,7 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
     for (uint64_t t = 0; t < n_train; ++t) ymean += y[perm[t]];
     B.master[kOffB3] = ymean / (float)n_train;
   }
-  DevBuf dX, dY, dIdx, dMaster, dM, dV, dG, dWq, dLoss;
+  DevBuf dX, dY, dIdx, dMaster, dM, dV, dG, dWq, dLoss, dWork;
   upload(dX, X, s);
   upload(dY, y, s);
   upload(dMaster, B.master, s);
@@ -564,6 +601,7 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
   dIdx.reserve(n_train * 4);
   std::vector<uint32_t> order(perm.begin(), perm.begin() + n_train);
   const uint32_t bsz = (uint32_t)std::min<uint64_t>(P.batch, n_train);
+  dWork.reserve(train_step_workspace_floats(bsz) * 4);
   int step = 0;
   float last_loss = 0.0f;
   for (int ep = 0; ep < P.epochs; ++ep) {
@@ -572,9 +610,8 @@ void Index::budget_fit(const float* queries, uint64_t n, uint64_t d, const float
     int nb = 0;
     HDG_HIP(hipMemsetAsync(dLoss.data(), 0, 4, s));  // the epoch's batch losses add up here
     for (uint64_t b0 = 0; b0 + bsz <= n_train; b0 += bsz) {
-      HDG_HIP(hipMemsetAsync(dG.data(), 0, kParams * 4, s));
       launch_train_step(dX.as<float>(), dY.as<float>(), dIdx.as<uint32_t>() + b0, bsz, dWq.as<float>(), dG.as<float>(),
-                        dLoss.as<float>(), s);
+                        dLoss.as<float>(), dWork.as<float>(), s);
       ++step;
       const float c1 = 1.0f - std::pow(0.9f, (float)step), c2 = 1.0f - std::pow(0.999f, (float)step);
       hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, dMaster.as<float>(),
@@ -782,7 +819,7 @@ void Index::budget_free() {
 
 // One forward/backward of the training kernel on a given batch (gradient check against a CPU oracle).
 void budget_debug_grads(const float* params, const float* X, const float* Y, uint32_t n, float* grads, float* loss) {
-  DevBuf dX, dY, dIdx, dW, dG, dL;
+  DevBuf dX, dY, dIdx, dW, dG, dL, dWork;
   std::vector<float> x(X, X + (size_t)n * kF), y(Y, Y + n), w(params, params + kParams);
   std::vector<uint32_t> idx(n);
   std::iota(idx.begin(), idx.end(), 0u);
@@ -792,10 +829,10 @@ void budget_debug_grads(const float* params, const float* X, const float* Y, uin
   upload(dW, w, nullptr);
   dG.reserve(kParams * 4);
   dL.reserve(4);
-  HDG_HIP(hipMemset(dG.data(), 0, kParams * 4));
+  dWork.reserve(train_step_workspace_floats(n) * 4);
   HDG_HIP(hipMemset(dL.data(), 0, 4));
   launch_train_step(dX.as<float>(), dY.as<float>(), dIdx.as<uint32_t>(), n, dW.as<float>(), dG.as<float>(),
-                    dL.as<float>(), nullptr);
+                    dL.as<float>(), dWork.as<float>(), nullptr);
   HDG_HIP(hipDeviceSynchronize());
   HDG_HIP(hipMemcpy(grads, dG.data(), kParams * 4, hipMemcpyDeviceToHost));
   HDG_HIP(hipMemcpy(loss, dL.data(), 4, hipMemcpyDeviceToHost));

@@ -231,3 +231,25 @@ def test_coarse_features_refuse_long_lists():
     with pytest.raises(Exception):
         H.coarse_features(z, torch.zeros((4, 513), dtype=torch.int32, device=DEV),
                           torch.zeros(4, dtype=torch.int32, device=DEV), torch.zeros((4, 513), device=DEV))
+
+
+def test_budget_fit_is_reproducible():
+    """Two fits on the same training queries and seed give the same model file byte for byte: the per-workgroup
+    gradients of a step are added in workgroup order, never by atomics."""
+    w = datasets.explicit_ranges(n=30000, nq=50, d=48, components=32, clusters=32, sub_clusters=6)
+    idx = H.SuperGridIndex(conf_text=w.conf_text(N_LIST=32, SUB_LISTS=64, PRECISION=1, **FAST),
+                           filter_text=w.filter_text)
+    idx.build(torch.from_numpy(w.data).to(DEV), torch.from_numpy(w.labels).to(DEV))
+    tq, tql = w.sample_queries(3000, 0x77)
+    _, tgt = O.bruteforce(w.data, w.labels, tq, O.recover_filters(tql, 2, **w.filter_arrays), 100)
+    tq, tql = torch.from_numpy(tq).to(DEV), torch.from_numpy(tql).to(DEV)
+    with tempfile.TemporaryDirectory() as td:
+        reps, blobs = [], []
+        for i in range(2):
+            reps.append(idx.budget_fit(tq, tql, tgt, levels=8, t0=0, s0=6, epochs=6, target=0.95))
+            idx.budget_save(os.path.join(td, f"m{i}.bin"))
+            with open(os.path.join(td, f"m{i}.bin"), "rb") as f:
+                blobs.append(f.read())
+    print("[budget] reports", reps, flush=True)
+    assert reps[0] == reps[1]
+    assert blobs[0] == blobs[1]
