@@ -82,6 +82,9 @@ _kmeans_bal = _sig("hdg_debug_kmeans_balanced", ctypes.c_int,
                    ctypes.c_int, _vp])
 _coarse_feat = _sig("hdg_debug_coarse_features", ctypes.c_int,
                     [_vp, _u64, _u64, _vp, ctypes.c_uint32, _vp, _vp, _u64, _vp, ctypes.c_int, _vp])
+_scored_sel = _sig("hdg_debug_scored_select", ctypes.c_int,
+                   [_vp, _u64, _u64, _vp, ctypes.c_uint32, _vp, _vp, _u64, ctypes.c_uint32, _vp, ctypes.c_uint32,
+                    _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp])
 _fit_coeff = _sig("hdg_fit_mem_coeff", ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)])
 _autotune = _sig("hdg_autotune", ctypes.c_int,
                  [_vp, _vp, _u64, _u64, _vp, _u64, _vp, _vp, ctypes.c_uint32, _cp, _u64, _vp])
@@ -188,6 +191,24 @@ def coarse_features(queries, lists, n_list, scores, stream=None):
     _check(_coarse_feat(_ptr(queries), nq, d, _ptr(lists), lists.shape[1], _ptr(n_list), _ptr(scores),
                         scores.shape[1], _ptr(feat), queries.device.index or 0, _stream_ptr(stream)))
     return feat
+
+
+def scored_select(queries, lists, n_list, scores, k, budget=None, features=False, wave=True, stream=None):
+    """The query path's per-query top-k of scored lists (device tensors as in coarse_features; budget [nq] int32
+    or None): returns (chosen [nq][k] list positions, n_chosen [nq], feat [nq][32] or None), device tensors."""
+    import torch
+    queries, lists, n_list, scores = (t.contiguous() for t in (queries, lists, n_list, scores))
+    nq, d = queries.shape
+    dev = queries.device
+    chosen = torch.full((nq, k), -1, dtype=torch.int32, device=dev)
+    n_chosen = torch.zeros(nq, dtype=torch.int32, device=dev)
+    feat = torch.zeros((nq, 32), dtype=torch.float32, device=dev) if features else None
+    budget = budget.contiguous() if budget is not None else None
+    _check(_scored_sel(_ptr(queries), nq, d, _ptr(lists), lists.shape[1], _ptr(n_list), _ptr(scores),
+                       scores.shape[1], k, _ptr(budget) if budget is not None else None, k,
+                       _ptr(chosen), _ptr(n_chosen), _ptr(feat) if features else None, 1 if wave else 0,
+                       dev.index or 0, _stream_ptr(stream)))
+    return chosen, n_chosen, feat
 
 
 def validate_config_text(conf_text: str, filter_text: str) -> str:

@@ -404,6 +404,28 @@ hdg_status hdg_debug_coarse_features(const float* queries, uint64_t nq, uint64_t
   });
 }
 
+hdg_status hdg_debug_scored_select(const float* queries, uint64_t nq, uint64_t dim, const uint32_t* list,
+                                   uint32_t cap, const uint32_t* n_list, const float* scores, uint64_t ld,
+                                   uint32_t k, const uint32_t* budget, uint32_t out_ld, uint32_t* chosen,
+                                   uint32_t* n_chosen, float* feat, int wave, int device, void* stream) {
+  return guarded([&] {
+    if (nq && (!queries || !list || !n_list || !scores || !chosen || !n_chosen)) hdg::fail(HDG_EINVAL, "null argument");
+    if (dim == 0 || k == 0 || out_ld < k || cap == 0 || ld < cap) hdg::fail(HDG_EINVAL, "bad sizes");
+    if (nq == 0) return;
+    HDG_HIP(hipSetDevice(device));
+    uint64_t* scratch = nullptr;
+    HDG_HIP(hipMalloc((void**)&scratch, nq * cap * sizeof(uint64_t)));
+    const hipStream_t s = (hipStream_t)stream;
+    hdg::launch_scored_select(queries, 0, nq, dim, list, cap, n_list, scores, ld, k, budget, out_ld, chosen, n_chosen,
+                              feat, scratch, wave != 0, s, nullptr, nullptr, 0, nullptr, nullptr, false, false);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(s);
+    HDG_HIP(hipFree(scratch));
+    HDG_HIP(e1);
+    HDG_HIP(e2);
+  });
+}
+
 hdg_status hdg_debug_kmeans_balanced(const float* rows, uint64_t n, uint32_t w, uint32_t k, uint32_t n_iters,
                                      float* centers, uint32_t* labels, int device, void* stream) {
   return guarded([&] {

@@ -217,9 +217,55 @@ __global__ __launch_bounds__(512) void coarse_gemm_kernel(const uint16_t* __rest
 
 // |q|^2 of a d-dim query by one wave: lane partials over t = lane, lane + 64, ... in order, then a butterfly
 // over the 64 lanes; every kernel that needs the value calls this, so they agree bit for bit.
-__device__ __forceinline__ float wave_sq_norm(const float* q, uint64_t d, int lane) {
+// The empty asm takes the value as an operand, so the compiler keeps the load that produces it unconditional
+// and ahead of the statement: a batch of loads followed by hold() on each is in flight together, where loads
+// guarded by per-element conditions would each wait for the one before.
+__device__ __forceinline__ void hold(float& x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void hold(uint32_t& x) { asm volatile("" : "+v"(x)); }
+
+// wave_sq_norm in two halves for rows of at most 512 dims, so that a kernel can issue the row's loads with its
+// other first loads and finish the sum later: lane partials over t = lane, lane + 64, ... in order, then a
+// butterfly over the 64 lanes.
+__device__ __forceinline__ void sq_norm_load(const float* q, uint64_t d, int lane, float (&x)[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint64_t t = (uint64_t)lane + 64u * i;
+    x[i] = q[t < d ? t : d - 1];
+  }
+}
+
+__device__ __forceinline__ float sq_norm_finish(float (&x)[8], uint64_t d, int lane) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) hold(x[i]);
   float p = 0.0f;
-  for (uint64_t t = (uint64_t)lane; t < d; t += 64) p += q[t] * q[t];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if ((uint64_t)lane + 64u * i < d) p += x[i] * x[i];
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) p += __shfl_xor(p, m, 64);
+  return __shfl(p, 0, 64);
+}
+
+__device__ __forceinline__ float wave_sq_norm(const float* q, uint64_t d, int lane) {
+  if (d <= 512) {
+    float x[8];
+    sq_norm_load(q, d, lane, x);
+    return sq_norm_finish(x, d, lane);
+  }
+  float p = 0.0f;
+  for (uint64_t t0 = (uint64_t)lane; t0 < d; t0 += 512) {
+    float x[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const uint64_t t = t0 + 64u * i;
+      x[i] = q[t < d ? t : d - 1];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) hold(x[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (t0 + 64u * i < d) p += x[i] * x[i];
+  }
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) p += __shfl_xor(p, m, 64);
   return __shfl(p, 0, 64);
@@ -379,16 +425,198 @@ __global__ __launch_bounds__(256) void scored_select_kernel(const float* __restr
 // than kWaveSelMax items are left to the workgroup kernel.
 constexpr uint32_t kWaveSelMax = 512;
 
+// Sorts the c (<= 64 * R2) keys in lbuf and writes the first m indexes; feat_key as in wave_select.
+template <int R2>
+__device__ __forceinline__ void wave_sort_out(const uint64_t* lbuf, uint32_t c, uint32_t m, int lane, uint32_t* out,
+                                              uint32_t avail, uint64_t& feat_key) {
+  uint64_t key[R2];
+#pragma unroll
+  for (int r = 0; r < R2; ++r) {
+    const uint32_t e = (uint32_t)lane * R2 + r;
+    key[r] = e < c ? lbuf[e] : ~0ull;
+  }
+  dev::wave_bitonic<R2>(key, lane);
+#pragma unroll
+  for (int r = 0; r < R2; ++r) {
+    const uint32_t e = (uint32_t)lane * R2 + r;
+    if (e < m) out[e] = (uint32_t)key[r];
+  }
+  if (avail > 0) {
+    const uint32_t j = (uint32_t)lane < avail ? (uint32_t)lane : avail - 1;
+    feat_key = dev::wave_key_at<R2>(key, j & 63u);
+  }
+}
+
+// The m_need smallest of a wave's keys (key = ord(score) << 32 | element, ~0 = pad; keys are distinct) without
+// sorting all of them: a radix descent over the 32-bit score order, one byte a step (a 256-bin histogram of the
+// keys inside the current range in the wave's LDS, a scan of it across the lanes), narrows the score range
+// holding the m_need-th smallest key until the keys below the end of that range number at most `target`; those
+// keys (a superset of the m_need smallest: every key below the range is smaller, every key above it larger) are
+// compacted into lbuf in element order.  Returns their count, or 0 when more than `target` keys tie at one
+// score (the caller then sorts everything).
+template <int R>
+__device__ __forceinline__ uint32_t wave_cut_smallest(const uint64_t (&key)[R], uint32_t n, uint32_t m_need,
+                                                      uint32_t target, int lane, uint64_t* lbuf, uint32_t* hist) {
+  uint32_t below = 0, inb = n;  // valid keys below the range, in the range
+  uint64_t prefix = 0;          // range = the keys whose score order >> shift equals prefix
+  int shift = 32;
+  if (n > target) {
+    // the bits above the highest one in which the scores differ are common: start below them, so that the
+    // first histogram spreads over its bins instead of piling every key into one
+    uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (key[r] != ~0ull) {
+        const uint32_t o = (uint32_t)(key[r] >> 32);
+        mn = o < mn ? o : mn;
+        mx = o > mx ? o : mx;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint32_t a = (uint32_t)__shfl_xor((int)mn, o, 64), b = (uint32_t)__shfl_xor((int)mx, o, 64);
+      mn = a < mn ? a : mn;
+      mx = b > mx ? b : mx;
+    }
+    shift = 32 - __clz((int)(mn ^ mx));  // 0 when every score is equal
+    prefix = (uint64_t)mx >> shift;
+  }
+  while (below + inb > target) {
+    if (shift == 0) return 0;
+    const int wd = shift < 8 ? shift : 8;
+    shift -= wd;
+    reinterpret_cast<uint4*>(hist)[lane] = make_uint4(0u, 0u, 0u, 0u);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t o = key[r] >> 32;
+      if (key[r] != ~0ull && (o >> (shift + wd)) == prefix)
+        atomicAdd(&hist[(uint32_t)(o >> shift) & ((1u << wd) - 1u)], 1u);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint4 h = reinterpret_cast<const uint4*>(hist)[lane];  // bins 4 lane .. 4 lane + 3
+    const uint32_t s0 = h.x, s1 = s0 + h.y, s2 = s1 + h.z, s3 = s2 + h.w;
+    uint32_t incl = s3;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    const uint32_t ex = incl - s3;
+    const uint32_t need = m_need - below;  // rank (from 1) of the m_need-th smallest inside the range
+    // the one lane whose bins take the cumulative count from below `need` to at least `need`
+    const bool mine = ex < need && incl >= need;
+    const uint64_t bal = __ballot(mine);
+    const int src = __ffsll((unsigned long long)bal) - 1;
+    uint32_t lo = ex, hi = ex + s0, dg = 0;
+    if (ex + s0 < need) {
+      lo = ex + s0; hi = ex + s1; dg = 1;
+      if (ex + s1 < need) {
+        lo = ex + s1; hi = ex + s2; dg = 2;
+        if (ex + s2 < need) {
+          lo = ex + s2; hi = ex + s3; dg = 3;
+        }
+      }
+    }
+    const uint32_t bin = (uint32_t)__shfl((int)((uint32_t)lane * 4u + dg), src, 64);
+    const uint32_t blo = (uint32_t)__shfl((int)lo, src, 64), bhi = (uint32_t)__shfl((int)hi, src, 64);
+    below += blo;
+    inb = bhi - blo;
+    prefix = (prefix << wd) | bin;
+    __builtin_amdgcn_wave_barrier();  // every lane has read the histogram before the next step clears it
+  }
+  // compaction of the keys with score order < (prefix + 1) << shift, element order kept
+  bool keep[R];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    keep[r] = key[r] != ~0ull && ((key[r] >> 32) >> shift) <= prefix;
+    cnt += keep[r] ? 1u : 0u;
+  }
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  uint32_t off = incl - cnt;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (keep[r]) lbuf[off++] = key[r];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  return below + inb;
+}
+
 template <int R>
 __device__ void wave_select(const uint32_t* lst, const float* row, const uint32_t* nzb_q, const uint32_t* nzp_q,
                             bool by_pos, uint32_t n, uint32_t m, int lane, uint32_t* out, uint32_t avail,
-                            uint64_t& feat_key) {
+                            uint64_t& feat_key, uint64_t* lbuf, uint32_t* hist) {
+  // every load unconditional (elements past n read element n - 1), so the wave has all of them in flight
+  // at once instead of one guarded load chain per element
   uint64_t key[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const uint32_t e = (uint32_t)lane * R + r;
-    const float v = e < n ? (by_pos ? row[e] : item_score(row, lst[e], nzb_q, nzp_q)) : 0.0f;
-    key[r] = e < n ? (((uint64_t)ord(v) << 32) | e) : ~0ull;
+  for (int r = 0; r < R; ++r) key[r] = ~0ull;
+  if (n > 0) {
+    float v[R];
+    if (by_pos) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint32_t e = (uint32_t)lane * R + r;
+        v[r] = row[e < n ? e : n - 1];
+      }
+    } else {
+      // item_score's loads in three batches: list items, their bit words and word prefixes, the scores
+      uint32_t id[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint32_t e = (uint32_t)lane * R + r;
+        id[r] = lst[e < n ? e : n - 1];
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) hold(id[r]);
+      if (nzb_q) {
+        uint32_t wd[R], pf[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          wd[r] = nzb_q[id[r] >> 5];
+          pf[r] = nzp_q[id[r] >> 5];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          hold(wd[r]);
+          hold(pf[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = row[pf[r] + __popc(wd[r] & ((1u << (id[r] & 31u)) - 1u))];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = row[id[r]];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) hold(v[r]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t e = (uint32_t)lane * R + r;
+      if (e < n) key[r] = ((uint64_t)ord(v[r]) << 32) | e;
+    }
+  }
+  if constexpr (R >= 4) {
+    // fewer wanted than held: cut to a smaller sort
+    const uint32_t m_need = m > avail ? m : avail;
+    const uint32_t target = m_need <= 64u ? 64u : (m_need <= 128u ? 128u : 256u);
+    if (m_need > 0 && target < 64u * R) {
+      const uint32_t c = wave_cut_smallest<R>(key, n, m_need, target, lane, lbuf, hist);
+      if (c > 0) {
+        if (c <= 64u) wave_sort_out<1>(lbuf, c, m, lane, out, avail, feat_key);
+        else if (c <= 128u) wave_sort_out<2>(lbuf, c, m, lane, out, avail, feat_key);
+        else wave_sort_out<4>(lbuf, c, m, lane, out, avail, feat_key);
+        return;
+      }
+    }
   }
   dev::wave_bitonic<R>(key, lane);
 #pragma unroll
@@ -436,10 +664,14 @@ __global__ __launch_bounds__(256) void scored_select_wave_kernel(
   uint32_t* out = chosen + qi * out_ld;
   const uint32_t avail = feat_dist ? (n < 32 ? n : 32) : 0;
   uint64_t fk = 0;
-  if (n <= 64) wave_select<1>(lst, row, nzb_q, nzp_q, by_pos, n, m, lane, out, avail, fk);
-  else if (n <= 128) wave_select<2>(lst, row, nzb_q, nzp_q, by_pos, n, m, lane, out, avail, fk);
-  else if (n <= 256) wave_select<4>(lst, row, nzb_q, nzp_q, by_pos, n, m, lane, out, avail, fk);
-  else wave_select<8>(lst, row, nzb_q, nzp_q, by_pos, n, m, lane, out, avail, fk);
+  __shared__ uint64_t lbuf_all[4][256];
+  __shared__ __align__(16) uint32_t hist_all[4][256];
+  uint64_t* lbuf = lbuf_all[threadIdx.x >> 6];
+  uint32_t* hist = hist_all[threadIdx.x >> 6];
+  if (n <= 64) wave_select<1>(lst, row, nzb_q, nzp_q, by_pos, n, m, lane, out, avail, fk, lbuf, hist);
+  else if (n <= 128) wave_select<2>(lst, row, nzb_q, nzp_q, by_pos, n, m, lane, out, avail, fk, lbuf, hist);
+  else if (n <= 256) wave_select<4>(lst, row, nzb_q, nzp_q, by_pos, n, m, lane, out, avail, fk, lbuf, hist);
+  else wave_select<8>(lst, row, nzb_q, nzp_q, by_pos, n, m, lane, out, avail, fk, lbuf, hist);
   if (lane == 0) n_chosen[qi] = m;
   if (feat_dist) {
     const float qq = wave_sq_norm(queries + (q0 + qi) * d, d, lane);
@@ -455,29 +687,35 @@ __global__ __launch_bounds__(256) void scored_select_wave_kernel(
   }
 }
 
+// The c (<= 64 * R) keys in buf sorted; returns the lane-th smallest (~0 past c).
+template <int R>
+__device__ __forceinline__ uint32_t wave_sorted_at_lane(const uint32_t* buf, uint32_t c, int lane) {
+  uint32_t key[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)lane * R + r;
+    key[r] = e < c ? buf[e] : 0xFFFFFFFFu;
+  }
+  dev::wave_bitonic32<R>(key, lane);
+  // rank j lies in lane j / R, register j % R
+  const uint32_t src = (uint32_t)lane / R, rr = (uint32_t)lane % R;
+  uint32_t v = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t y = (uint32_t)__shfl((int)key[r], (int)src, 64);
+    v = (uint32_t)r == rr ? y : v;
+  }
+  return v;
+}
+
 // Budget-model features on the path that probes every non-zero cluster: per query, the 32 smallest coarse
 // scores among its non-zero clusters, ascending, plus |q|^2 (lane j < 32 writes the min(j, avail - 1)-th,
 // avail = min(n, 32)) -- the values scored_select_wave_kernel writes as its features, without its full sort of
-// 64-bit (score, position) keys.  One wave per query, lists of at most 512 items (8 keys per lane, lane-major):
-// each lane sorts its 8 keys; T = the 8th smallest of the lanes' 4th smallest keys, so at least 32 keys are
-// <= T and the 32 smallest are among them; those keys (normally at most 64) are compacted and sorted.  When more
-// than 64 keys are <= T, all 512 are sorted.  Keys are order-preserving u32 of the f32 scores, so equal scores
-// give equal features whatever their order.
-__device__ __forceinline__ void cas_up(uint32_t& a, uint32_t& b) {
-  const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-  a = lo;
-  b = hi;
-}
-
-__device__ __forceinline__ void sort8_up(uint32_t (&k)[8]) {  // Batcher's odd-even merge network, 19 steps
-  cas_up(k[0], k[1]); cas_up(k[2], k[3]); cas_up(k[4], k[5]); cas_up(k[6], k[7]);
-  cas_up(k[0], k[2]); cas_up(k[1], k[3]); cas_up(k[4], k[6]); cas_up(k[5], k[7]);
-  cas_up(k[1], k[2]); cas_up(k[5], k[6]);
-  cas_up(k[0], k[4]); cas_up(k[1], k[5]); cas_up(k[2], k[6]); cas_up(k[3], k[7]);
-  cas_up(k[2], k[4]); cas_up(k[3], k[5]);
-  cas_up(k[1], k[2]); cas_up(k[3], k[4]); cas_up(k[5], k[6]);
-}
-
+// 64-bit (score, position) keys.  One wave per query, lists of at most 512 items, element e = r * 64 + lane (the
+// list and score loads coalesced, every lane holding an item once n >= 64): T = the 32nd smallest of the 64
+// lane minima, so at least 32 keys are <= T and the 32 smallest are among them; those keys (typically ~n / 10)
+// are compacted into LDS and sorted.  Keys are order-preserving u32 of the f32 scores, so equal scores give
+// equal features whatever their order.
 __global__ __launch_bounds__(256) void coarse_features_kernel(const float* __restrict__ queries, uint64_t q0,
                                                               uint64_t qn, uint64_t d,
                                                               const uint32_t* __restrict__ list, uint32_t cap,
@@ -486,30 +724,47 @@ __global__ __launch_bounds__(256) void coarse_features_kernel(const float* __res
                                                               float* __restrict__ feat,
                                                               const uint32_t* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
-  __shared__ uint32_t cbuf_all[4][64];
+  __shared__ uint32_t cbuf_all[4][512];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint64_t qi = (uint64_t)blockIdx.x * 4 + wv;
   if (qi >= qn) return;
   uint32_t* cbuf = cbuf_all[wv];
-  const uint32_t n = n_list_q[qi] < cap ? n_list_q[qi] : cap;  // cap <= 512
+  // every first load issued before any wait: the query row (for |q|^2), the list length and the list row
+  // (read up to cap, as the length is not known yet), then the scores of the listed items
+  const float* qrow = queries + (q0 + qi) * d;
+  float qx[8];
+  if (d <= 512) sq_norm_load(qrow, d, lane, qx);
+  const uint32_t nraw = n_list_q[qi];
   const uint32_t* lst = list + qi * cap;
+  uint32_t key[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const uint32_t e = (uint32_t)(r * 64 + lane);
+    key[r] = lst[e < cap ? e : cap - 1];  // cap <= 512
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) hold(key[r]);
+  const uint32_t n = nraw < cap ? nraw : cap;
   const float* row = S + qi * ld;
+  float x[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) x[r] = row[(uint32_t)(r * 64 + lane) < n ? key[r] : 0u];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) hold(x[r]);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) key[r] = (uint32_t)(r * 64 + lane) < n ? ord(x[r]) : 0xFFFFFFFFu;
   uint32_t v;  // lane j: the j-th smallest key (j < min(n, 64))
   if (n <= 64) {
-    uint32_t k1[1] = {(uint32_t)lane < n ? ord(row[lst[lane]]) : 0xFFFFFFFFu};
+    uint32_t k1[1] = {key[0]};
     dev::wave_bitonic32<1>(k1, lane);
     v = k1[0];
   } else {
-    uint32_t key[8];
+    uint32_t mn = key[0];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const uint32_t e = (uint32_t)lane * 8 + r;
-      key[r] = e < n ? ord(row[lst[e]]) : 0xFFFFFFFFu;
-    }
-    sort8_up(key);
-    uint32_t t[1] = {key[3]};
+    for (int r = 1; r < 8; ++r) mn = key[r] < mn ? key[r] : mn;
+    uint32_t t[1] = {mn};
     dev::wave_bitonic32<1>(t, lane);
-    const uint32_t T = (uint32_t)__shfl((int)t[0], 7, 64);
+    const uint32_t T = (uint32_t)__shfl((int)t[0], 31, 64);
     uint32_t c = 0;
 #pragma unroll
     for (int r = 0; r < 8; ++r) c += key[r] <= T ? 1u : 0u;
@@ -520,33 +775,21 @@ __global__ __launch_bounds__(256) void coarse_features_kernel(const float* __res
       if (lane >= o) incl += y;
     }
     const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
-    if (total <= 64) {
-      const uint32_t off = incl - c;
+    uint32_t off = incl - c;
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
-        if ((uint32_t)r < c) cbuf[off + r] = key[r];  // the lane's keys are ascending: its first c are <= T
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      uint32_t k1[1] = {(uint32_t)lane < total ? cbuf[lane] : 0xFFFFFFFFu};
-      dev::wave_bitonic32<1>(k1, lane);
-      v = k1[0];
-    } else {
-      dev::wave_bitonic32<8>(key, lane);
-      // rank j lies in lane j / 8, register j % 8
-      const uint32_t src = (uint32_t)lane >> 3, rr = (uint32_t)lane & 7u;
-      uint32_t mine = 0;
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const uint32_t x = (uint32_t)__shfl((int)key[r], (int)src, 64);
-        mine = (uint32_t)r == rr ? x : mine;
-      }
-      v = mine;
-    }
+    for (int r = 0; r < 8; ++r)
+      if (key[r] <= T) cbuf[off++] = key[r];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (total <= 64) v = wave_sorted_at_lane<1>(cbuf, total, lane);
+    else if (total <= 128) v = wave_sorted_at_lane<2>(cbuf, total, lane);
+    else if (total <= 256) v = wave_sorted_at_lane<4>(cbuf, total, lane);
+    else v = wave_sorted_at_lane<8>(cbuf, total, lane);
   }
   const uint32_t avail = n < (uint32_t)kFeatDists ? n : (uint32_t)kFeatDists;
   const uint32_t j = avail == 0 ? 0u : ((uint32_t)lane < avail ? (uint32_t)lane : avail - 1);
   const uint32_t o = (uint32_t)__shfl((int)v, (int)j, 64);
-  const float qq = wave_sq_norm(queries + (q0 + qi) * d, d, lane);
+  const float qq = d <= 512 ? sq_norm_finish(qx, d, lane) : wave_sq_norm(qrow, d, lane);
   if (lane < kFeatDists) {
     const uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
     feat[qi * kFeatDists + lane] = avail > 0 ? __uint_as_float(u) + qq : 0.0f;

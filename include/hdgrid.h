@@ -176,6 +176,18 @@ hdg_status hdg_debug_coarse_features(const float* queries, uint64_t nq, uint64_t
                                      uint32_t cap, const uint32_t* n_list, const float* scores, uint64_t ld,
                                      float* feat, int device, void* stream);
 
+/* Per-query top-k of scored lists (device arrays), as the query path picks clusters and sub-clusters from
+ * their BF16 scores (replaces select_sub_clusters' sort of Super-Grid-IVF_PQ.cu:885-896 on those scores):
+ * query q's items are list[q][0 .. n_list[q]) (rows cap apart) scored scores[q][item] (rows ld apart); k_q =
+ * min(k, budget[q]) when budget (nullable) gives a non-zero value; chosen [nq][out_ld] gets the list positions
+ * of the min(n, k_q) smallest scores, ascending, ties by position, n_chosen [nq] their count; feat (nullable,
+ * [nq][32]) the 32 smallest + |queries[q]|^2.  wave = 1 runs the wave-per-query kernel (lists <= 512), 0 the
+ * workgroup kernel.  Synchronous.  For checking the two selection kernels against each other and a sort. */
+hdg_status hdg_debug_scored_select(const float* queries, uint64_t nq, uint64_t dim, const uint32_t* list,
+                                   uint32_t cap, const uint32_t* n_list, const float* scores, uint64_t ld,
+                                   uint32_t k, const uint32_t* budget, uint32_t out_ld, uint32_t* chosen,
+                                   uint32_t* n_chosen, float* feat, int wave, int device, void* stream);
+
 /* Balanced hierarchical k-means of n rows (device array rows [n][w]) into k clusters, as the index build
  * groups its clusters*sub_clusters fine centers into `clusters` groups (replaces the RAFT
  * kmeans_balanced::fit_predict call at Super-Grid-IVF_PQ.cu:466-476; n_iters = BALANCE_ITERS there).

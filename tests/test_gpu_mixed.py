@@ -207,3 +207,55 @@ def test_budget_fit_gain(mixed_index):
     with tempfile.TemporaryDirectory() as td:
         idx.budget_save(td + "/m.bin")
         idx.budget_load(td + "/m.bin")
+
+
+def _ord(x):
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32)
+    return np.where(u & 0x80000000, ~u, u | 0x80000000).astype(np.uint64)
+
+
+@pytest.mark.parametrize("kind", ["distinct", "ties", "one_value", "signed_zero"])
+def test_scored_select_kernels_against_sort(kind):
+    """Both selection kernels against a sort of (score order, list position) keys: lists of 0 .. 512 items
+    around every register-count boundary, k from 1 to 300 with per-query budgets (0 = none), heavy ties that
+    defeat the wave kernel's radix cut (full sort instead), -0.0 next to +0.0, and the 32-smallest features
+    equal between the kernels."""
+    rng = np.random.default_rng(["distinct", "ties", "one_value", "signed_zero"].index(kind) + 11)
+    ns = [0, 1, 2, 63, 64, 65, 127, 128, 129, 200, 255, 256, 257, 300, 400, 470, 511, 512]
+    ns = np.array(ns + list(rng.integers(0, 513, 43)), np.int64)
+    nq, cap, ld, d = len(ns), 512, 600, 16
+    S = {"distinct": rng.standard_normal((nq, ld)) * 100,
+         "ties": rng.integers(-2, 3, (nq, ld)).astype(np.float64),
+         "one_value": np.full((nq, ld), 3.5),
+         "signed_zero": np.where(rng.random((nq, ld)) < 0.5, -0.0, 0.0) + (rng.random((nq, ld)) < 0.3)}[kind]
+    S = S.astype(np.float32)
+    lists = np.zeros((nq, cap), np.uint32)
+    for q in range(nq):
+        lists[q, :ns[q]] = rng.permutation(ld)[:ns[q]]
+    queries = np.zeros((nq, d), np.float32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+    for k in (1, 16, 32, 100, 128, 129, 200, 256, 300):
+        budget = rng.integers(0, k + 40, nq).astype(np.int32)
+        budget[rng.random(nq) < 0.3] = 0
+        res = {}
+        for wave in (True, False):
+            c, nc, f = H.scored_select(t(queries), t(lists.view(np.int32)), t(ns.astype(np.int32)), t(S), k,
+                                       budget=t(budget), features=True, wave=wave)
+            torch.cuda.synchronize()
+            res[wave] = (c.cpu().numpy(), nc.cpu().numpy(), f.cpu().numpy())
+        for q in range(nq):
+            n = int(ns[q])
+            kq = k if budget[q] == 0 else min(k, int(budget[q]))
+            m = min(n, kq)
+            vals = S[q, lists[q, :n]]
+            order = np.lexsort((np.arange(n), _ord(vals)))
+            for wave in (True, False):
+                c, nc, f = res[wave]
+                assert nc[q] == m, (kind, k, q, wave)
+                np.testing.assert_array_equal(c[q, :m], order[:m], err_msg=f"{kind} k={k} q={q} n={n} wave={wave}")
+                if n:
+                    a = min(n, 32)
+                    exp = vals[order[np.minimum(np.arange(32), a - 1)]]
+                    assert np.array_equal(f[q].view(np.uint32), (exp + np.float32(0)).view(np.uint32)) or \
+                        np.array_equal(f[q], exp)
+        assert np.array_equal(res[True][2].view(np.uint32), res[False][2].view(np.uint32))
