@@ -14,6 +14,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -1189,7 +1190,7 @@ void Index::build(const float* dataset, uint64_t n, uint64_t d, const float* lab
   stage_done("cells, Morton codes, z-pool sort and gathers");
   build_bucket_grid(st_, s);
   stage_done("bucket grid and occupancy sums");
-  st_.labels_integral = std::all_of(h_lab.begin(), h_lab.end(), [](float v) { return v == std::floor(v); });
+  prepare_label_words(st_, h_lab, s);
   upload(st_.pq_centers, pq_centers, s);
   upload(st_.centers, coarse, s);
   prepare_mixed_centers(st_, coarse, sub_centers, s);
@@ -1218,6 +1219,64 @@ __global__ void block_codes_kernel(const uint8_t* __restrict__ codes, uint64_t n
   if (M & 15) reinterpret_cast<uint2*>(blk + n16 * 1024)[p & 63] = reinterpret_cast<const uint2*>(row)[2 * n16];
 }
 }  // namespace
+
+namespace {
+// One thread per point: its label dims as offsets from the column minima, packed into one word.
+__global__ void label_words_kernel(const float* __restrict__ lab, uint64_t n, int l, LabelWordLayout lay,
+                                   uint64_t* __restrict__ out) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint64_t w = 0;
+  for (int j = 0; j < l; ++j) w |= ((uint64_t)lab[p * l + j] - lay.minv[j]) << lay.off[j];
+  out[p] = w;
+}
+}  // namespace
+
+void prepare_label_words(IndexState& st, const std::vector<float>& h_lab, hipStream_t s) {
+  const int l = st.l;
+  float mn[kMaxDims], mx[kMaxDims];
+  for (int j = 0; j < l; ++j) {
+    mn[j] = FLT_MAX;
+    mx[j] = -FLT_MAX;
+  }
+  bool integral = true;
+  const uint64_t n = h_lab.size() / (l > 0 ? l : 1);
+  for (uint64_t p = 0; p < n; ++p) {
+    const float* r = h_lab.data() + p * l;
+    for (int j = 0; j < l; ++j) {
+      const float v = r[j];
+      integral = integral && v == std::floor(v);
+      mn[j] = v < mn[j] ? v : mn[j];
+      mx[j] = v > mx[j] ? v : mx[j];
+    }
+  }
+  st.labels_integral = integral;
+  st.lab_words = false;
+  st.lw = LabelWordLayout{};
+  if (!integral || l < 1 || l > kMaxDims || n == 0) return;
+  uint32_t off = 0;
+  for (int j = 0; j < l; ++j) {
+    if (!(mn[j] >= 0.0f) || !(mx[j] <= 9007199254740992.0f)) return;  // 2^53: exact in double
+    const uint64_t lo = (uint64_t)mn[j], range = (uint64_t)mx[j] - lo;
+    if (range >= (1ull << 31)) return;
+    uint32_t bits = 1;
+    while (bits < 31 && (range >> bits) != 0) ++bits;
+    if (off + bits + 1 > 64) return;
+    st.lw.off[j] = off;
+    st.lw.mask[j] = (uint32_t)((1ull << bits) - 1);
+    st.lw.minv[j] = lo;
+    st.lw.guard |= 1ull << (off + bits);
+    off += bits + 1;
+  }
+  st.labw.reserve(st.N * sizeof(uint64_t));
+  hipLaunchKernelGGL(label_words_kernel, dim3((unsigned)((st.N + 255) / 256)), dim3(256), 0, s, st.lab.as<float>(),
+                     st.N, l, st.lw, st.labw.as<uint64_t>());
+  HDG_HIP(hipGetLastError());
+  st.lw_dev.reserve(sizeof(LabelWordLayout));
+  HDG_HIP(hipMemcpyAsync(st.lw_dev.data(), &st.lw, sizeof(LabelWordLayout), hipMemcpyHostToDevice, s));
+  HDG_HIP(hipStreamSynchronize(s));  // the host copy of the layout may change before the stream runs
+  st.lab_words = true;
+}
 
 void prepare_blocked_codes(IndexState& st, hipStream_t s) {
   st.codes_wb = (st.M % 8) == 0 ? 16u : 0u;

@@ -102,7 +102,7 @@ FieldRef lookup(Config& c, const std::string& key) {
       "LABEL_MASK", "IS_SPLIT", "SEED", "KMEANS_ITERS", "KMEANS_TRAIN", "PQ_TRAIN", "BALANCE_ITERS",
       "PARITY_BATCH", "QUERY_CHUNK", "PRECISION", "PROBE_CLUSTERS", "BUDGET_MODEL", "SPARSE_COUNT",
       "SPARSE_MAX_TESTS", "ADC_PER", "ADC_WGS", "GRAPH", "SELECT_WAVE", "SCORE_BLOCK", "ADC_SPLIT",
-      "INBOX_CAP", "BUDGET_LABELS"};
+      "INBOX_CAP", "BUDGET_LABELS", "LABEL_WORDS"};
   uint64_t* const fields[] = {
       &c.data_width, &c.index_width, &c.topk, &c.break_down, &c.enable_multi_gpu, &c.mem_bound,
       &c.is_calc_mem_predictor_coeff, &c.lowest_query_batch, &c.filter_dim, &c.pq_dim,
@@ -110,7 +110,7 @@ FieldRef lookup(Config& c, const std::string& key) {
       &c.n_split, &c.label_mask, &c.is_split, &c.seed, &c.kmeans_iters, &c.kmeans_train,
       &c.pq_train, &c.balance_iters, &c.parity_batch, &c.query_chunk, &c.precision, &c.probe_clusters,
       &c.budget_model, &c.sparse_count, &c.sparse_max_tests, &c.adc_per, &c.adc_wgs, &c.graph, &c.select_wave, &c.score_block, &c.adc_split,
-      &c.inbox_cap, &c.budget_labels};
+      &c.inbox_cap, &c.budget_labels, &c.label_words};
   FieldRef r;
   if (key == "PATH") {
     r.s = &c.path;

@@ -62,6 +62,8 @@ struct Config {
                                        // under PRECISION 1 a walked query whose list fits refines from it
   uint64_t budget_labels = 0;          // BUDGET_LABELS: path whose traces label the budget model's training
                                        // queries: 0 = plain (PRECISION 0), 1 = mixed (PRECISION 1)
+  uint64_t label_words = 1;            // LABEL_WORDS: 1 = label post-filters read the packed label words when the
+                                       // index has them (integer labels whose ranges fit; same results)
 };
 
 // Parse with config_utils.h:33-88 semantics: `KEY = value` per line, key case-insensitive and

@@ -144,8 +144,9 @@ __global__ __launch_bounds__(512) void coarse_gemm_kernel(const uint16_t* __rest
     // step ks has landed once at most the next step's two loads of this lane are outstanding; the barrier
     // makes every lane's pieces visible and ends every wave's reads of step ks - 1, whose buffer step ks + 2
     // now refills.  Wait, barrier and fragment reads are written out (the compiler's fence and its alias
-    // check against the LDS loads in flight would wait for every outstanding load instead); the reads wait
-    // for their data in the same block.
+    // check against the LDS loads in flight would wait for every outstanding load instead).  The read block
+    // waits for its first four fragments only; the MFMAs that need just those run while the last two B
+    // fragments land, and the second wait takes those two as operands, so no use of them precedes it.
     if (ks + 1 < KS) asm volatile("s_waitcnt vmcnt(2)\n\ts_barrier" ::: "memory");
     else asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
     if (ks + 2 < KS) issue(ks + 2);
@@ -158,14 +159,19 @@ __global__ __launch_bounds__(512) void coarse_gemm_kernel(const uint16_t* __rest
         "ds_read_b128 %3, %7 offset:1024\n\t"
         "ds_read_b128 %4, %7 offset:2048\n\t"
         "ds_read_b128 %5, %7 offset:3072\n\t"
-        "s_waitcnt lgkmcnt(0)"
+        "s_waitcnt lgkmcnt(2)"
         : "=&v"(a[0]), "=&v"(a[1]), "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3])
         : "v"(bufb + offA), "v"(bufb + offB)
         : "memory");
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    asm volatile("s_waitcnt lgkmcnt(0)" : "+v"(b[2]), "+v"(b[3]) : : "memory");
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 2; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
   }
   __syncthreads();  // every wave is done with the ring before the epilogue reuses it
   if (nzb) {

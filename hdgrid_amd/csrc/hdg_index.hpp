@@ -14,6 +14,17 @@
 
 namespace hdg {
 
+// Label words: a point's label dims packed into one u64, dim j as (label - minv[j]) in the bits
+// [off[j], off[j] + popcount(mask[j])), each field followed by a zero guard bit (set in `guard`) so that all
+// fields can be compared with the query's bounds by two 64-bit subtractions.  Exact for non-negative integer
+// labels (see prepare_label_words).
+struct LabelWordLayout {
+  uint32_t off[kMaxDims] = {0};
+  uint32_t mask[kMaxDims] = {0};
+  uint64_t minv[kMaxDims] = {0};
+  uint64_t guard = 0;
+};
+
 // Device layout.  Every per-point array is in z-pool order: sub-clusters in cluster-major order
 // (Super-Grid-IVF_PQ.cu:486-504), points inside a sub-cluster ascending by (Morton code, id)
 // (:604-617).  Reading a Morton range therefore reads contiguous memory for codes, labels,
@@ -79,6 +90,12 @@ struct IndexState {
   // every label value is an integer: then a point passing the exact label predicate lies in the query's
   // cell box, so the in-box points of the chosen sub-clusters are a superset of the keep-all survivors
   bool labels_integral = false;
+  // the labels again as one word per point (z-pool order) when they are integers whose columns' ranges fit
+  // (lab_words): the label post-filters read 8 bytes per point instead of 4 l
+  DevBuf labw;        // u64 [N]
+  bool lab_words = false;
+  LabelWordLayout lw;
+  DevBuf lw_dev;      // the layout again in device memory (the kernels read it once per query)
   bool built = false;
 };
 
@@ -104,6 +121,12 @@ void kmeans_balanced(const float* X, uint64_t n, uint32_t w, uint32_t k, uint32_
                      std::vector<float>& centers, std::vector<uint32_t>& labels, hipStream_t s);
 // Fine cluster counts of the mesoclusters (sizes meso_sizes, n rows in all) for k clusters.
 std::vector<uint32_t> arrange_fine_clusters(uint32_t k, const std::vector<uint64_t>& meso_sizes, uint64_t n);
+
+// Sets labels_integral from h_lab (the labels in any row order, host) and, when every label is a non-negative
+// integer of at most 2^53, every column's range max - min is below 2^31 and the columns' bit widths (at least
+// one each) plus one guard bit per column sum to at most 64, packs st.lab (z-pool order, device) into st.labw
+// and sets lab_words.
+void prepare_label_words(IndexState& st, const std::vector<float>& h_lab, hipStream_t s);
 
 // Fills codes_blk / codes_wb from the row-layout codes (device).
 void prepare_blocked_codes(IndexState& st, hipStream_t s);

@@ -324,8 +324,8 @@ void Index::load(const std::string& dir, hipStream_t s) {
   upload(S.cells, H.cells, s);
   build_bucket_grid(S, s);
   upload(S.vec, H.vec, s);
-  S.labels_integral = std::all_of(H.lab.begin(), H.lab.end(), [](float v) { return v == std::floor(v); });
   upload(S.lab, H.lab, s);
+  prepare_label_words(S, H.lab, s);
   S.built = true;
   st_ = std::move(S);
 }

@@ -1812,6 +1812,63 @@ __device__ __forceinline__ uint32_t col_to_pos(const uint32_t* rs, const uint32_
   return rs[lo] + (col - rp[lo]);
 }
 
+// Label words (IndexState::labw): the label predicate on integer labels as bounds on the word's fields.
+// For an integer v = minv + f and f32 bounds a, b: a <= v <= b exactly when ceil(a) - minv <= f <=
+// floor(b) - minv; a NaN bound fails no comparison, so it does not bound.  label_box turns a query's 2 l
+// bounds into field values LO (lower bounds) and SPAN (upper - lower), and all fields are tested at once with
+// the zero guard bit above each field (G): X = (w | G) - LO keeps field j's guard set exactly when f_j >= lo_j
+// (no borrow crosses a guard), its field then holding f_j - lo_j; Z = (SPAN | G) - (X & ~G) keeps it set
+// exactly when f_j - lo_j <= span_j.  A dim no value can pass sets `none`.
+struct LabelWords {
+  const uint64_t* w;            // nullptr: read the f32 label rows
+  const LabelWordLayout* lay;   // device copy of the layout (read once per query)
+  uint64_t guard;               // lay->guard
+};
+
+struct LabelBox {
+  uint64_t lo, span;
+  uint32_t none;
+};
+
+__device__ inline LabelBox label_box(const LabelWords& P, int l, const float* rg) {
+  uint64_t lo_w = 0, span_w = 0;
+  bool empty = false;
+#pragma unroll
+  for (int j = 0; j < kMaxDims; ++j) {
+    if (j >= l) continue;
+    const double m = (double)P.lay->minv[j];
+    const double a = rg[2 * j], b = rg[2 * j + 1];
+    double lo = 0.0, hi = (double)P.lay->mask[j];
+    if (a == a) lo = fmax(lo, ceil(a) - m);
+    if (b == b) hi = fmin(hi, floor(b) - m);
+    if (lo > hi) {
+      empty = true;
+    } else {
+      lo_w |= (uint64_t)(uint32_t)lo << P.lay->off[j];
+      span_w |= (uint64_t)(uint32_t)(hi - lo) << P.lay->off[j];
+    }
+  }
+  // every caller's query is uniform over the wave: the bounds live in scalar registers
+  // (readfirstlane returns a signed int: each half goes through uint32_t before it is widened)
+  auto uniform_u64 = [](uint64_t v) {
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    return ((uint64_t)hi << 32) | (uint64_t)lo;
+  };
+  LabelBox B;
+  B.lo = uniform_u64(lo_w);
+  B.span = uniform_u64(span_w);
+  B.none = (uint32_t)__builtin_amdgcn_readfirstlane(empty ? 1 : 0);
+  return B;
+}
+
+__device__ __forceinline__ bool label_word_pass(const LabelWords& P, const LabelBox& B, uint64_t w) {
+  const uint64_t G = P.guard;
+  const uint64_t x = (w | G) - B.lo;
+  const uint64_t z = (B.span | G) - (x & ~G);
+  return !B.none && (x & z & G) == G;
+}
+
 // The exact label predicate lo <= label <= hi in every dim (filter.cu:30-64).  Up to four dims are loaded
 // together (one memory round trip); more dims are tested one by one.
 __device__ __forceinline__ bool label_pass(const float* __restrict__ lab, int l, uint32_t pos,
@@ -1849,9 +1906,9 @@ __global__ __launch_bounds__(256) void keep_all_kernel(
     uint64_t qn, int l, int integral, uint32_t sub_lists, const uint32_t* __restrict__ mode,
     const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zlo_cnt,
     const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ rstart,
-    const uint32_t* __restrict__ rpre, const float* __restrict__ lab, const float* __restrict__ ranges,
-    const uint64_t* __restrict__ loff, uint32_t* __restrict__ L, uint32_t* __restrict__ nL,
-    const uint32_t* __restrict__ abort_flag) {
+    const uint32_t* __restrict__ rpre, const float* __restrict__ lab, LabelWords lw,
+    const float* __restrict__ ranges, const uint64_t* __restrict__ loff, uint32_t* __restrict__ L,
+    uint32_t* __restrict__ nL, const uint32_t* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
   __shared__ uint32_t tmp[8];
   extern __shared__ uint32_t rsp[];  // rs[sub_lists], rp[sub_lists + 1]
@@ -1879,6 +1936,7 @@ __global__ __launch_bounds__(256) void keep_all_kernel(
   __shared__ float rg[2 * kMaxDims];
   if (threadIdx.x < 2 * (uint32_t)l) rg[threadIdx.x] = ranges[qi * 2 * l + threadIdx.x];
   __syncthreads();
+  const LabelBox box = lw.w ? label_box(lw, l, rg) : LabelBox{};
   // each thread owns kKeepPer consecutive columns, so all their label loads are in flight together and
   // one scan per kWG * kKeepPer columns keeps the column order
   constexpr int kKeepPer = 8;
@@ -1904,7 +1962,14 @@ __global__ __launch_bounds__(256) void keep_all_kernel(
 #pragma unroll
       for (int i = 0; i < kKeepPer; ++i)
         if (c0 + i < Cq) flags |= 1u << i;
-      if (l == 4) {
+      if (lw.w) {
+        uint64_t wv[kKeepPer];
+#pragma unroll
+        for (int i = 0; i < kKeepPer; ++i) wv[i] = (flags >> i) & 1u ? lw.w[pos[i]] : 0ull;
+#pragma unroll
+        for (int i = 0; i < kKeepPer; ++i)
+          if (!label_word_pass(lw, box, wv[i])) flags &= ~(1u << i);
+      } else if (l == 4) {
 #pragma unroll
         for (int i = 0; i < kKeepPer; i += 4) {
           float4 lv[4];
@@ -1953,7 +2018,7 @@ __global__ __launch_bounds__(256) void inbox_keep_kernel(
     const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zlo_cnt,
     const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ chosen_sid, uint32_t ibcap,
     const uint32_t* __restrict__ ib_pos, const uint32_t* __restrict__ ib_sub, const uint32_t* __restrict__ seg_off,
-    uint32_t pk_shift, const float* __restrict__ lab, const float* __restrict__ ranges,
+    uint32_t pk_shift, const float* __restrict__ lab, LabelWords lw, const float* __restrict__ ranges,
     const uint64_t* __restrict__ loff, uint32_t* __restrict__ L, uint32_t* __restrict__ nL,
     const uint32_t* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
@@ -1972,6 +2037,7 @@ __global__ __launch_bounds__(256) void inbox_keep_kernel(
   __syncthreads();
   float rg[2 * kMaxDims];
   for (int i = 0; i < 2 * l; ++i) rg[i] = ranges[qi * 2 * l + i];
+  const LabelBox box = lw.w ? label_box(lw, l, rg) : LabelBox{};
   const uint32_t n = n_ib[qi];
   const uint32_t md = mode[qi];
   uint32_t* out = L + loff[qi];
@@ -1984,7 +2050,8 @@ __global__ __launch_bounds__(256) void inbox_keep_kernel(
       const uint32_t sid = ib_sub ? ib_sub[qi * ibcap + i] : v >> pk_shift;
       if ((bits[sid >> 5] >> (sid & 31)) & 1u) {
         pos = ib_sub ? v : seg_off[sid] + (v & ((1u << pk_shift) - 1u));
-        flag = ((l == 1 && md != 3) || label_pass(lab, l, pos, rg)) ? 1u : 0u;
+        flag = ((l == 1 && md != 3) ||
+                (lw.w ? label_word_pass(lw, box, lw.w[pos]) : label_pass(lab, l, pos, rg))) ? 1u : 0u;
       }
     }
     uint32_t total;
@@ -2010,7 +2077,7 @@ __global__ __launch_bounds__(64 * kKeepWaves) void inbox_keep_wave_kernel(
     const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zlo_cnt,
     const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ chosen_sid, uint32_t ibcap,
     const uint32_t* __restrict__ ib_pos, const uint32_t* __restrict__ ib_sub, const uint32_t* __restrict__ seg_off,
-    uint32_t pk_shift, const float* __restrict__ lab, const float* __restrict__ ranges,
+    uint32_t pk_shift, const float* __restrict__ lab, LabelWords lw, const float* __restrict__ ranges,
     const uint64_t* __restrict__ loff, uint32_t* __restrict__ L, uint32_t* __restrict__ nL,
     const uint32_t* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
@@ -2036,6 +2103,7 @@ __global__ __launch_bounds__(64 * kKeepWaves) void inbox_keep_wave_kernel(
   wave_lds_sync();
   float rg[2 * kMaxDims];
   for (int i = 0; i < 2 * l; ++i) rg[i] = ranges[qi * 2 * l + i];
+  const LabelBox box = lw.w ? label_box(lw, l, rg) : LabelBox{};
   const uint32_t n = n_ib[qi];
   const uint32_t md = mode[qi];
   const bool filt = !(l == 1 && md != 3);
@@ -2063,7 +2131,13 @@ __global__ __launch_bounds__(64 * kKeepWaves) void inbox_keep_wave_kernel(
         if (flag[u]) pos[u] = seg_lds[sid[u]] + (pos[u] & ((1u << pk_shift) - 1u));
     }
     if (filt) {
-      if (l == 4) {
+      if (lw.w) {
+        uint64_t wv[kKeepBatches];
+#pragma unroll
+        for (int u = 0; u < kKeepBatches; ++u) wv[u] = flag[u] ? lw.w[pos[u]] : 0ull;
+#pragma unroll
+        for (int u = 0; u < kKeepBatches; ++u) flag[u] = flag[u] && label_word_pass(lw, box, wv[u]);
+      } else if (l == 4) {
         float4 lv[kKeepBatches];
 #pragma unroll
         for (int u = 0; u < kKeepBatches; ++u)
@@ -2498,10 +2572,12 @@ static_assert(kSelTile <= kWG * 64, "a tile's items per thread fit the 64-bit ma
 // scr[] for the final ranking.  A first pass marks both kinds in per-thread bit masks, one workgroup scan and
 // one reservation on the global counters cnt[0] (survivors) / cnt[1] (bin keys) place them, a second pass
 // writes them.  rg: the query's 2 l label bounds.  Called by the whole workgroup.
+template <bool WORDS>
 __device__ inline void sel_split_range(uint32_t a, uint32_t e, const uint32_t* __restrict__ d, uint32_t mn,
                                        uint32_t sft, uint32_t bstar, uint32_t nr, const uint32_t* rs,
-                                       const uint32_t* rp, int l, const float* __restrict__ lab, const float* rg,
-                                       uint32_t* __restrict__ out, uint64_t* __restrict__ scr, uint32_t* cnt) {
+                                       const uint32_t* rp, int l, const float* __restrict__ lab,
+                                       const LabelWords& lw, const float* rg, uint32_t* __restrict__ out,
+                                       uint64_t* __restrict__ scr, uint32_t* cnt) {
   __shared__ uint32_t tmp[kWG / 64 + 1];
   __shared__ uint32_t res[2];
   constexpr uint32_t kStep = kWG * kSplitUnroll;
@@ -2515,6 +2591,7 @@ __device__ inline void sel_split_range(uint32_t a, uint32_t e, const uint32_t* _
     }
   }
   uint32_t j = j0;
+  const LabelBox box = WORDS && l > 1 ? label_box(lw, l, rg) : LabelBox{};
   uint64_t flo = 0, fin = 0;  // bit step * kSplitUnroll + u: listed survivor / key in the bin
   uint32_t step = 0;
   for (uint32_t base = a; base < e; base += kStep, ++step) {
@@ -2541,7 +2618,16 @@ __device__ inline void sel_split_range(uint32_t a, uint32_t e, const uint32_t* _
       }
     }
     // the label filter, every item's loads in flight together
-    if (l == 4) {
+    if (WORDS) {
+      if (l > 1) {
+      uint64_t wv[kSplitUnroll];
+#pragma unroll
+      for (int u = 0; u < kSplitUnroll; ++u) wv[u] = (lo >> u) & 1u ? lw.w[pos[u]] : 0ull;
+#pragma unroll
+      for (int u = 0; u < kSplitUnroll; ++u)
+        if (!label_word_pass(lw, box, wv[u])) lo &= ~(1u << u);
+      }
+    } else if (l == 4) {
       float4 lv[kSplitUnroll];
 #pragma unroll
       for (int u = 0; u < kSplitUnroll; ++u)
@@ -2612,9 +2698,9 @@ __global__ __launch_bounds__(256) void sel_split_kernel(
     uint64_t qn, int l, uint32_t sub_lists, const uint64_t* __restrict__ aoff, const uint32_t* __restrict__ sslot,
     const uint32_t* __restrict__ smm, const uint64_t* __restrict__ kprime, const uint32_t* __restrict__ adist,
     const uint32_t* __restrict__ ghist, const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ rstart,
-    const uint32_t* __restrict__ rpre, const float* __restrict__ lab, const float* __restrict__ ranges,
-    const uint64_t* __restrict__ loff, uint32_t* __restrict__ L, uint64_t* __restrict__ scratch,
-    uint32_t* __restrict__ scnt, const uint32_t* __restrict__ abort_flag) {
+    const uint32_t* __restrict__ rpre, const float* __restrict__ lab, LabelWords lw,
+    const float* __restrict__ ranges, const uint64_t* __restrict__ loff, uint32_t* __restrict__ L,
+    uint64_t* __restrict__ scratch, uint32_t* __restrict__ scnt, const uint32_t* __restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
   __shared__ __align__(16) uint32_t hist[kSelBins];
   __shared__ uint32_t rs_l[kSelLdsRanges], rp_l[kSelLdsRanges + 1];
@@ -2630,9 +2716,14 @@ __global__ __launch_bounds__(256) void sel_split_kernel(
     dev::wg_hist_find(hist, (uint32_t)kprime[qi], sh);
     if (threadIdx.x == 0) scnt[4 * qi + 2] = (uint32_t)sh[1];  // the same value from every tile of the query
     const bool table_lds = nr <= kSelLdsRanges;
-    sel_split_range(a, e, adist + aoff[qi], mn, sft, (uint32_t)sh[0], nr, table_lds ? rs_l : rstart + qi * sub_lists,
-                    table_lds ? rp_l : rpre + qi * (sub_lists + 1), l, lab, rg, L + loff[qi], scratch + aoff[qi],
-                    scnt + 4 * qi);
+    if (lw.w)
+      sel_split_range<true>(a, e, adist + aoff[qi], mn, sft, (uint32_t)sh[0], nr,
+                            table_lds ? rs_l : rstart + qi * sub_lists, table_lds ? rp_l : rpre + qi * (sub_lists + 1),
+                            l, lab, lw, rg, L + loff[qi], scratch + aoff[qi], scnt + 4 * qi);
+    else
+      sel_split_range<false>(a, e, adist + aoff[qi], mn, sft, (uint32_t)sh[0], nr,
+                             table_lds ? rs_l : rstart + qi * sub_lists, table_lds ? rp_l : rpre + qi * (sub_lists + 1),
+                             l, lab, lw, rg, L + loff[qi], scratch + aoff[qi], scnt + 4 * qi);
     __syncthreads();
   });
 }
@@ -2654,8 +2745,8 @@ __global__ __launch_bounds__(256) void adc_select_kernel(
     uint64_t qn, int l, uint32_t sub_lists, const uint32_t* __restrict__ mode,
     const uint64_t* __restrict__ kprime, const uint64_t* __restrict__ cand,
     const uint32_t* __restrict__ n_chosen_s, const uint32_t* __restrict__ rstart,
-    const uint32_t* __restrict__ rpre, const float* __restrict__ lab, const float* __restrict__ ranges,
-    const uint64_t* __restrict__ aoff, uint64_t* __restrict__ akeys_sel_scratch,
+    const uint32_t* __restrict__ rpre, const float* __restrict__ lab, LabelWords lw,
+    const float* __restrict__ ranges, const uint64_t* __restrict__ aoff, uint64_t* __restrict__ akeys_sel_scratch,
     const uint32_t* __restrict__ adist, uint64_t* __restrict__ akeys2, const uint64_t* __restrict__ loff,
     uint32_t* __restrict__ L, uint32_t* __restrict__ nL, const uint32_t* __restrict__ sslot,
     const uint32_t* __restrict__ smm, const uint32_t* __restrict__ scnt, const uint32_t* __restrict__ abort_flag) {
@@ -2693,6 +2784,7 @@ __global__ __launch_bounds__(256) void adc_select_kernel(
   }
   float rg[2 * kMaxDims];
   for (int i = 0; i < 2 * l; ++i) rg[i] = ranges[qi * 2 * l + i];
+  const LabelBox box = lw.w && l > 1 ? label_box(lw, l, rg) : LabelBox{};
   if (threadIdx.x == 0) cnt_sh = 0;
   // the range table in LDS (the histogram's space, free after the selection) for the column lookups
   const bool table_lds = 2 * nr + 1 <= (uint32_t)kSelBins;
@@ -2713,7 +2805,7 @@ __global__ __launch_bounds__(256) void adc_select_kernel(
     const uint32_t pos = col_to_pos(rs_c, rp_c, nr, dev::key_tie(sel[e]));
     if (l == 1) {
       out[e] = pos;
-    } else if (label_pass(lab, l, pos, rg)) {
+    } else if (lw.w ? label_word_pass(lw, box, lw.w[pos]) : label_pass(lab, l, pos, rg)) {
       out[atomicAdd(&cnt_sh, 1u)] = pos;
     }
   }
@@ -3316,7 +3408,8 @@ uint64_t Index::config_digest() const {
   const uint64_t v[] = {cfg_.n_list, cfg_.sub_lists, cfg_.topk, cfg_.exps0, cfg_.precision, cfg_.probe_clusters,
                         cfg_.budget_model, cfg_.sparse_count, cfg_.sparse_max_tests, cfg_.adc_per, cfg_.adc_wgs,
                         cfg_.parity_batch, cfg_.query_chunk, cfg_.label_mask, cfg_.select_wave, cfg_.score_block,
-                        cfg_.adc_split, cfg_.inbox_cap, cfg_.is_split, cfg_.n_split, graph_epoch_};
+                        cfg_.adc_split, cfg_.inbox_cap, cfg_.is_split, cfg_.n_split, cfg_.label_words,
+                        graph_epoch_};
   uint64_t h = 1469598103934665603ull;
   for (uint64_t x : v) {
     h ^= x;
@@ -3746,24 +3839,28 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   // split mode keeps the keep-all candidates its pieces give (no in-box replacement)
   const int integral = n_split ? -1 : (S.labels_integral ? 1 : 0);
   const uint32_t* ib_sub_arg = S.entries_packed ? nullptr : w.ib_sub.as<uint32_t>();
+  LabelWords lw;
+  lw.w = S.lab_words && cfg_.label_words ? S.labw.as<uint64_t>() : nullptr;
+  lw.lay = S.lab_words ? S.lw_dev.as<LabelWordLayout>() : nullptr;
+  lw.guard = S.lw.guard;
   if (sparse && cfg_.select_wave)
     hipLaunchKernelGGL(inbox_keep_wave_kernel, dim3((unsigned)cdiv(qn, kKeepWaves)), dim3(64 * kKeepWaves),
                        S.entries_packed ? CS * 4 : 0, s, qn, l, integral,
                        (uint32_t)CS, sub_lists, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
                        w.zlo_cnt.as<uint64_t>(), w.n_chosen_s.as<uint32_t>(), csid, ibcap,
                        w.ib_pos.as<uint32_t>(), ib_sub_arg, S.seg_off.as<uint32_t>(), S.pk_shift, S.lab.as<float>(),
-                       w.ranges.as<float>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
+                       lw, w.ranges.as<float>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
   else if (sparse)
     hipLaunchKernelGGL(inbox_keep_kernel, dim3((unsigned)qn), dim3(256), 0, s, l, integral, (uint32_t)CS, sub_lists,
                        w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(), w.zlo_cnt.as<uint64_t>(),
                        w.n_chosen_s.as<uint32_t>(), csid, ibcap, w.ib_pos.as<uint32_t>(), ib_sub_arg,
-                       S.seg_off.as<uint32_t>(), S.pk_shift, S.lab.as<float>(), w.ranges.as<float>(),
+                       S.seg_off.as<uint32_t>(), S.pk_shift, S.lab.as<float>(), lw, w.ranges.as<float>(),
                        w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
   if (!all_walked)
   hipLaunchKernelGGL(keep_all_kernel, dim3((unsigned)qn), dim3(256), (2 * R + 1) * 4, s, qn, l, integral,
                      R, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
                      w.zlo_cnt.as<uint64_t>(), nr_arg, w.rstart.as<uint32_t>(),
-                     w.rpre.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
+                     w.rpre.as<uint32_t>(), S.lab.as<float>(), lw, w.ranges.as<float>(), w.lsize.as<uint64_t>(),
                      w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
   HDG_HIP(hipGetLastError());
   mark();
@@ -3835,12 +3932,12 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     hipLaunchKernelGGL(sel_split_kernel, dim3(sel_grid), dim3(256), 0, s, qn, l, R, w.asize.as<uint64_t>(),
                        w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(), w.kprime.as<uint64_t>(), w.adist.as<uint32_t>(),
                        w.ghist.as<uint32_t>(), nr_arg, w.rstart.as<uint32_t>(),
-                       w.rpre.as<uint32_t>(), S.lab.as<float>(), w.ranges.as<float>(), w.lsize.as<uint64_t>(),
+                       w.rpre.as<uint32_t>(), S.lab.as<float>(), lw, w.ranges.as<float>(), w.lsize.as<uint64_t>(),
                        w.L.as<uint32_t>(), w.akeys2.as<uint64_t>(), w.scnt.as<uint32_t>(), abort_p);
     HDG_HIP(hipGetLastError());
     hipLaunchKernelGGL(adc_select_kernel, dim3((unsigned)qn), dim3(256), 0, s, qn, l, R,
                        w.mode.as<uint32_t>(), w.kprime.as<uint64_t>(), w.cand.as<uint64_t>(),
-                       nr_arg, w.rstart.as<uint32_t>(), w.rpre.as<uint32_t>(), S.lab.as<float>(),
+                       nr_arg, w.rstart.as<uint32_t>(), w.rpre.as<uint32_t>(), S.lab.as<float>(), lw,
                        w.ranges.as<float>(), w.asize.as<uint64_t>(), w.Lkeys.as<uint64_t>(), w.adist.as<uint32_t>(),
                        w.akeys2.as<uint64_t>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(),
                        w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(), w.scnt.as<uint32_t>(), abort_p);

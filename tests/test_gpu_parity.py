@@ -272,6 +272,61 @@ def test_recall_vs_bruteforce(l3):
     assert r > 0.5
 
 
+def _edge_bound_queries(w, seed=7):
+    """Explicit ranges edited per (query, dim): fractional bounds (as drawn), integer bounds, a lower bound below
+    the label range, an upper bound above it, a point window on an existing label value, and a window between two
+    integers that no integer label can fall in."""
+    ql = w.qlabels.copy()
+    rng = np.random.default_rng(seed)
+    kinds = rng.integers(0, 6, size=(ql.shape[0], 2))
+    for q in range(ql.shape[0]):
+        for j in range(2):
+            lo, hi = float(ql[q, 2 * j]), float(ql[q, 2 * j + 1])
+            k = kinds[q, j]
+            if k == 1:
+                lo, hi = np.floor(lo), np.floor(hi)
+            elif k == 2:
+                lo = -5e6
+            elif k == 3:
+                hi = 1e12
+            elif k == 4:
+                lo = hi = float(w.labels[rng.integers(0, w.labels.shape[0]), j])
+            elif k == 5:
+                lo, hi = np.floor(lo) + 0.25, np.floor(lo) + 0.75
+            ql[q, 2 * j], ql[q, 2 * j + 1] = np.float32(lo), np.float32(hi)
+    return ql
+
+
+def test_label_words_match_f32_rows(l2_explicit):
+    """The label post-filters on packed label words (LABEL_WORDS 1, the default for integer labels) against the
+    f32 label rows (LABEL_WORDS 0) and the oracle, on bounds at, between and beyond the label values: the keep-all
+    list and the tiled and one-workgroup ADC post-filters (plain), and the in-box keep (PRECISION 1)."""
+    b = l2_explicit
+    w0, ql0 = b.w, b.ql
+    ql = _edge_bound_queries(w0)
+    b.w = datasets.Workload(w0.name, w0.conf, w0.filter_text, w0.data, w0.labels, w0.queries, ql, w0.filter_arrays)
+    b.ql = torch.from_numpy(ql).to(DEV)
+    try:
+        for exps0, split in ((30, 32768), (2, 32768), (2, 256)):
+            res = b.run(EXPS0=exps0, ADC_SPLIT=split, LABEL_WORDS=1)
+            assert_parity(res, f"label words exps0={exps0} ADC_SPLIT={split}")
+            ref = b.run(LABEL_WORDS=0)
+            assert np.array_equal(res[1], ref[1]) and np.array_equal(res[0].view(np.uint32), ref[0].view(np.uint32))
+        b.idx.set_param("PRECISION", 1)
+        out = {}
+        for lw in (1, 0):
+            b.idx.set_param("LABEL_WORDS", lw)
+            dist, ids = b.idx.query(b.q, b.ql)
+            torch.cuda.synchronize()
+            out[lw] = (dist.cpu().numpy(), ids.cpu().numpy())
+        assert np.array_equal(out[1][1], out[0][1]) and np.array_equal(out[1][0].view(np.uint32),
+                                                                         out[0][0].view(np.uint32))
+    finally:
+        for k, v in (("PRECISION", 0), ("LABEL_WORDS", 1), ("EXPS0", 30), ("ADC_SPLIT", 32768)):
+            b.idx.set_param(k, v)
+        b.w, b.ql = w0, ql0
+
+
 # ---- Morton split mode (IS_SPLIT = 1) ---------------------------------------------------------------------
 @pytest.mark.parametrize("n_split", [1, 4])
 def test_split_mode_l3(l3, n_split):
