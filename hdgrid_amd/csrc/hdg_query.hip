@@ -2072,6 +2072,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 constexpr int kKeepWaves = 8;  // queries (waves) per workgroup: they share the staged seg_off
 
+template <bool PACKED, bool WORDS>
 __global__ __launch_bounds__(64 * kKeepWaves) void inbox_keep_wave_kernel(
     uint64_t qn, int l, int integral, uint32_t CS, uint32_t sub_lists, const uint32_t* __restrict__ mode,
     const uint32_t* __restrict__ n_ib, const uint64_t* __restrict__ zlo, const uint64_t* __restrict__ zlo_cnt,
@@ -2083,7 +2084,7 @@ __global__ __launch_bounds__(64 * kKeepWaves) void inbox_keep_wave_kernel(
   if (abort_flag && *abort_flag) return;
   __shared__ uint32_t bits_all[kKeepWaves][kSparseMaxCS / 32];
   extern __shared__ uint32_t seg_lds[];  // packed entries: seg_off[0 .. CS) for the position of a kept entry
-  if (!ib_sub) {
+  if (PACKED) {
     for (uint32_t i = threadIdx.x; i < CS; i += blockDim.x) seg_lds[i] = seg_off[i];
     __syncthreads();
   }
@@ -2103,38 +2104,53 @@ __global__ __launch_bounds__(64 * kKeepWaves) void inbox_keep_wave_kernel(
   wave_lds_sync();
   float rg[2 * kMaxDims];
   for (int i = 0; i < 2 * l; ++i) rg[i] = ranges[qi * 2 * l + i];
-  const LabelBox box = lw.w ? label_box(lw, l, rg) : LabelBox{};
+  const LabelBox box = WORDS ? label_box(lw, l, rg) : LabelBox{};
   const uint32_t n = n_ib[qi];
   const uint32_t md = mode[qi];
   const bool filt = !(l == 1 && md != 3);
   uint32_t* out = L + loff[qi];
   uint32_t base = 0;
   // kKeepBatches batches of 64 list entries per pass: their list loads, then the label loads of the entries in
-  // chosen sub-clusters, are in flight together; the kept entries are compacted batch by batch in list order
+  // chosen sub-clusters, are in flight together; the kept entries are compacted batch by batch in list order.
+  // The next pass's list entries are loaded before this pass's label loads, so a pass waits on one round trip.
+  // Every load is issued unconditionally (indices clamped into the list / entry 0), so a batch's loads are in
+  // flight together.
   constexpr int kKeepBatches = 8;
+  uint32_t nxt_p[kKeepBatches], nxt_s[kKeepBatches];
+  auto load_list = [&](uint32_t b0) {
+#pragma unroll
+    for (int u = 0; u < kKeepBatches; ++u) {
+      const uint32_t i = b0 + 64u * u + (uint32_t)lane;
+      const uint64_t at = qi * ibcap + (i < n ? i : n - 1);
+      nxt_p[u] = ib_pos[at];
+      if (!PACKED) nxt_s[u] = ib_sub[at];
+    }
+  };
+  if (n > 0) load_list(0);
   for (uint32_t b0 = 0; b0 < n; b0 += 64u * kKeepBatches) {
     uint32_t sid[kKeepBatches], pos[kKeepBatches];
 #pragma unroll
     for (int u = 0; u < kKeepBatches; ++u) {
       const uint32_t i = b0 + 64u * u + (uint32_t)lane;
-      pos[u] = i < n ? ib_pos[qi * ibcap + i] : 0u;
-      sid[u] = i >= n ? kNone : (ib_sub ? ib_sub[qi * ibcap + i] : pos[u] >> pk_shift);
+      pos[u] = nxt_p[u];
+      sid[u] = i >= n ? kNone : (PACKED ? pos[u] >> pk_shift : nxt_s[u]);
     }
+    if (b0 + 64u * kKeepBatches < n) load_list(b0 + 64u * kKeepBatches);
     bool flag[kKeepBatches];
 #pragma unroll
     for (int u = 0; u < kKeepBatches; ++u)
       flag[u] = sid[u] != kNone && ((bits[sid[u] >> 5] >> (sid[u] & 31)) & 1u);
-    if (!ib_sub) {
+    if (PACKED) {
       // packed entries: the position of the kept ones from their sub-cluster's z-pool offset
 #pragma unroll
       for (int u = 0; u < kKeepBatches; ++u)
         if (flag[u]) pos[u] = seg_lds[sid[u]] + (pos[u] & ((1u << pk_shift) - 1u));
     }
     if (filt) {
-      if (lw.w) {
+      if (WORDS) {
         uint64_t wv[kKeepBatches];
 #pragma unroll
-        for (int u = 0; u < kKeepBatches; ++u) wv[u] = flag[u] ? lw.w[pos[u]] : 0ull;
+        for (int u = 0; u < kKeepBatches; ++u) wv[u] = lw.w[flag[u] ? pos[u] : 0u];
 #pragma unroll
         for (int u = 0; u < kKeepBatches; ++u) flag[u] = flag[u] && label_word_pass(lw, box, wv[u]);
       } else if (l == 4) {
@@ -3843,13 +3859,16 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
   lw.w = S.lab_words && cfg_.label_words ? S.labw.as<uint64_t>() : nullptr;
   lw.lay = S.lab_words ? S.lw_dev.as<LabelWordLayout>() : nullptr;
   lw.guard = S.lw.guard;
-  if (sparse && cfg_.select_wave)
-    hipLaunchKernelGGL(inbox_keep_wave_kernel, dim3((unsigned)cdiv(qn, kKeepWaves)), dim3(64 * kKeepWaves),
+  if (sparse && cfg_.select_wave) {
+    auto kern = S.entries_packed ? (lw.w ? inbox_keep_wave_kernel<true, true> : inbox_keep_wave_kernel<true, false>)
+                                 : (lw.w ? inbox_keep_wave_kernel<false, true> : inbox_keep_wave_kernel<false, false>);
+    hipLaunchKernelGGL(kern, dim3((unsigned)cdiv(qn, kKeepWaves)), dim3(64 * kKeepWaves),
                        S.entries_packed ? CS * 4 : 0, s, qn, l, integral,
                        (uint32_t)CS, sub_lists, w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(),
                        w.zlo_cnt.as<uint64_t>(), w.n_chosen_s.as<uint32_t>(), csid, ibcap,
                        w.ib_pos.as<uint32_t>(), ib_sub_arg, S.seg_off.as<uint32_t>(), S.pk_shift, S.lab.as<float>(),
                        lw, w.ranges.as<float>(), w.lsize.as<uint64_t>(), w.L.as<uint32_t>(), w.nL.as<uint32_t>(), abort_p);
+  }
   else if (sparse)
     hipLaunchKernelGGL(inbox_keep_kernel, dim3((unsigned)qn), dim3(256), 0, s, l, integral, (uint32_t)CS, sub_lists,
                        w.mode.as<uint32_t>(), w.n_ib.as<uint32_t>(), w.zlo.as<uint64_t>(), w.zlo_cnt.as<uint64_t>(),
