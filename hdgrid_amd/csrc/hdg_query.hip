@@ -2610,13 +2610,22 @@ __device__ inline void sel_split_range(uint32_t a, uint32_t e, const uint32_t* _
   const LabelBox box = WORDS && l > 1 ? label_box(lw, l, rg) : LabelBox{};
   uint64_t flo = 0, fin = 0;  // bit step * kSplitUnroll + u: listed survivor / key in the bin
   uint32_t step = 0;
-  for (uint32_t base = a; base < e; base += kStep, ++step) {
-    uint32_t v[kSplitUnroll], pos[kSplitUnroll];
+  // the next step's distances load while this step's label loads are in flight (indices clamped into [a, e),
+  // so every load is issued unconditionally)
+  uint32_t vn[kSplitUnroll];
+  auto load_d = [&](uint32_t base) {
 #pragma unroll
     for (int u = 0; u < kSplitUnroll; ++u) {
       const uint32_t i = base + (uint32_t)u * kWG + threadIdx.x;
-      v[u] = i < e ? d[i] : 0u;
+      vn[u] = d[i < e ? i : e - 1];
     }
+  };
+  if (a < e) load_d(a);
+  for (uint32_t base = a; base < e; base += kStep, ++step) {
+    uint32_t v[kSplitUnroll], pos[kSplitUnroll];
+#pragma unroll
+    for (int u = 0; u < kSplitUnroll; ++u) v[u] = vn[u];
+    if (base + kStep < e) load_d(base + kStep);
     uint32_t lo = 0;  // items of this step below the bin
 #pragma unroll
     for (int u = 0; u < kSplitUnroll; ++u) {
@@ -2638,7 +2647,7 @@ __device__ inline void sel_split_range(uint32_t a, uint32_t e, const uint32_t* _
       if (l > 1) {
       uint64_t wv[kSplitUnroll];
 #pragma unroll
-      for (int u = 0; u < kSplitUnroll; ++u) wv[u] = (lo >> u) & 1u ? lw.w[pos[u]] : 0ull;
+      for (int u = 0; u < kSplitUnroll; ++u) wv[u] = lw.w[pos[u]];  // pos 0 where not below the bin
 #pragma unroll
       for (int u = 0; u < kSplitUnroll; ++u)
         if (!label_word_pass(lw, box, wv[u])) lo &= ~(1u << u);
@@ -2710,6 +2719,7 @@ __device__ inline void sel_stage_query(uint64_t qi, uint32_t nr, uint32_t sub_li
 
 // Split pass of the many-workgroup selection over tiles.  Per query, scnt[4 qi + 0] counts the listed
 // survivors, [1] the keys in the k'-th key's bin and [2] holds the keys below that bin.
+template <bool WORDS>
 __global__ __launch_bounds__(256) void sel_split_kernel(
     uint64_t qn, int l, uint32_t sub_lists, const uint64_t* __restrict__ aoff, const uint32_t* __restrict__ sslot,
     const uint32_t* __restrict__ smm, const uint64_t* __restrict__ kprime, const uint32_t* __restrict__ adist,
@@ -2732,14 +2742,9 @@ __global__ __launch_bounds__(256) void sel_split_kernel(
     dev::wg_hist_find(hist, (uint32_t)kprime[qi], sh);
     if (threadIdx.x == 0) scnt[4 * qi + 2] = (uint32_t)sh[1];  // the same value from every tile of the query
     const bool table_lds = nr <= kSelLdsRanges;
-    if (lw.w)
-      sel_split_range<true>(a, e, adist + aoff[qi], mn, sft, (uint32_t)sh[0], nr,
-                            table_lds ? rs_l : rstart + qi * sub_lists, table_lds ? rp_l : rpre + qi * (sub_lists + 1),
-                            l, lab, lw, rg, L + loff[qi], scratch + aoff[qi], scnt + 4 * qi);
-    else
-      sel_split_range<false>(a, e, adist + aoff[qi], mn, sft, (uint32_t)sh[0], nr,
-                             table_lds ? rs_l : rstart + qi * sub_lists, table_lds ? rp_l : rpre + qi * (sub_lists + 1),
-                             l, lab, lw, rg, L + loff[qi], scratch + aoff[qi], scnt + 4 * qi);
+    sel_split_range<WORDS>(a, e, adist + aoff[qi], mn, sft, (uint32_t)sh[0], nr,
+                           table_lds ? rs_l : rstart + qi * sub_lists, table_lds ? rp_l : rpre + qi * (sub_lists + 1),
+                           l, lab, lw, rg, L + loff[qi], scratch + aoff[qi], scnt + 4 * qi);
     __syncthreads();
   });
 }
@@ -3948,7 +3953,8 @@ void Index::query_chunk(const float* queries, uint64_t q0, uint64_t qn, const fl
     hipLaunchKernelGGL(sel_hist_kernel, dim3(sel_grid), dim3(256), 0, s, qn, w.asize.as<uint64_t>(),
                        w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(), w.adist.as<uint32_t>(), w.ghist.as<uint32_t>(),
                        abort_p);
-    hipLaunchKernelGGL(sel_split_kernel, dim3(sel_grid), dim3(256), 0, s, qn, l, R, w.asize.as<uint64_t>(),
+    hipLaunchKernelGGL(lw.w ? sel_split_kernel<true> : sel_split_kernel<false>, dim3(sel_grid), dim3(256), 0, s, qn, l, R,
+                       w.asize.as<uint64_t>(),
                        w.sslot.as<uint32_t>(), w.smm.as<uint32_t>(), w.kprime.as<uint64_t>(), w.adist.as<uint32_t>(),
                        w.ghist.as<uint32_t>(), nr_arg, w.rstart.as<uint32_t>(),
                        w.rpre.as<uint32_t>(), S.lab.as<float>(), lw, w.ranges.as<float>(), w.lsize.as<uint64_t>(),
