@@ -1234,6 +1234,12 @@ __global__ void label_words_kernel(const float* __restrict__ lab, uint64_t n, in
 
 void prepare_label_words(IndexState& st, const std::vector<float>& h_lab, hipStream_t s) {
   const int l = st.l;
+  st.lab_words = false;
+  st.lw = LabelWordLayout{};
+  if (l < 1 || l > kMaxDims) {  // the grid shape refuses such l before this; no words then
+    st.labels_integral = std::all_of(h_lab.begin(), h_lab.end(), [](float v) { return v == std::floor(v); });
+    return;
+  }
   float mn[kMaxDims], mx[kMaxDims];
   for (int j = 0; j < l; ++j) {
     mn[j] = FLT_MAX;
@@ -1251,9 +1257,9 @@ void prepare_label_words(IndexState& st, const std::vector<float>& h_lab, hipStr
     }
   }
   st.labels_integral = integral;
-  st.lab_words = false;
-  st.lw = LabelWordLayout{};
-  if (!integral || l < 1 || l > kMaxDims || n == 0) return;
+  if (!integral || n == 0) return;
+  // the layout is built aside and taken only when every column fits
+  LabelWordLayout lay{};
   uint32_t off = 0;
   for (int j = 0; j < l; ++j) {
     if (!(mn[j] >= 0.0f) || !(mx[j] <= 9007199254740992.0f)) return;  // 2^53: exact in double
@@ -1262,12 +1268,13 @@ void prepare_label_words(IndexState& st, const std::vector<float>& h_lab, hipStr
     uint32_t bits = 1;
     while (bits < 31 && (range >> bits) != 0) ++bits;
     if (off + bits + 1 > 64) return;
-    st.lw.off[j] = off;
-    st.lw.mask[j] = (uint32_t)((1ull << bits) - 1);
-    st.lw.minv[j] = lo;
-    st.lw.guard |= 1ull << (off + bits);
+    lay.off[j] = off;
+    lay.mask[j] = (uint32_t)((1ull << bits) - 1);
+    lay.minv[j] = lo;
+    lay.guard |= 1ull << (off + bits);
     off += bits + 1;
   }
+  st.lw = lay;
   st.labw.reserve(st.N * sizeof(uint64_t));
   hipLaunchKernelGGL(label_words_kernel, dim3((unsigned)((st.N + 255) / 256)), dim3(256), 0, s, st.lab.as<float>(),
                      st.N, l, st.lw, st.labw.as<uint64_t>());
